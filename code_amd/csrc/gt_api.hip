@@ -1,0 +1,729 @@
+// gt_api.hip -- the C-ABI of include/getm_transport.h.
+//
+// A context holds one domain: its masks on the device (the t_adv_grid sets
+// of src/2d/advection.F90:263-318), the module array zz of
+// src/3d/internal_pressure.F90, and scratch fields that stand in for the
+// reference's automatic arrays (fi, hi, adv of do_advection_3d, the column
+// work arrays of adv_split_w, dR/dZ/P of the SMcW scheme).  Scratch is
+// allocated on first use and kept, so a time loop allocates nothing.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/getm_transport.h"
+#include "gt_kernels.h"
+
+using gt::Dims;
+
+static thread_local char g_err[512] = "";
+
+static int set_err(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return set_err(GT_EHIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_),     \
+                     __FILE__, __LINE__);                                         \
+  } while (0)
+
+struct GridMasks {
+  const uint8_t *uflux, *vflux, *uupd, *vupd, *fin;
+  const int32_t* az;
+};
+
+struct gt_ctx {
+  int imin, imax, jmin, jmax, kmax;
+  Dims d;
+  long N3;
+  double dx, dy, ard1;
+  hipStream_t st = nullptr;
+  uint8_t* dmask = nullptr;
+  int32_t* dint = nullptr;
+  GridMasks H{}, U{}, V{};
+  const int32_t *az = nullptr, *au = nullptr, *av = nullptr, *ax = nullptr;
+  double *fi = nullptr, *hi = nullptr, *adv = nullptr, *tmp = nullptr;
+  double *scr7 = nullptr, *zz = nullptr, *P = nullptr, *slice3 = nullptr;
+  double *bpart = nullptr, *bout = nullptr;
+  unsigned long long* nwarn = nullptr;
+  int adv_ver_iterations = 1;
+  int do_mirror_2d = 1;
+};
+
+extern "C" const char* gt_last_error(void) { return g_err; }
+extern "C" int gt_version(void) { return 1; }
+
+static int ensure(double*& p, long n) {
+  if (p) return GT_OK;
+  HIPCHK(hipMalloc(&p, sizeof(double) * (size_t)n));
+  return GT_OK;
+}
+#define ENSURE(p, n)                 \
+  do {                               \
+    int r_ = ensure((p), (n));       \
+    if (r_ != GT_OK) return r_;      \
+  } while (0)
+
+// ---- context ---------------------------------------------------------------
+// domain.F90:294-346 on the host
+static void masks_from_az(long NX, long NY, const int32_t* az, int32_t* au, int32_t* av,
+                          int32_t* ax) {
+  for (long n = 0; n < NX * NY; n++) au[n] = av[n] = ax[n] = 0;
+  for (long j = 0; j < NY; j++)
+    for (long i = 0; i < NX - 1; i++) {
+      int a = az[i + NX * j], b = az[i + 1 + NX * j];
+      int v = 0;
+      if (a == 1 && b == 1) v = 1;
+      if ((a == 1 && b == 2) || (a == 2 && b == 1)) v = 2;
+      if (a == 2 && b == 2) v = 3;
+      au[i + NX * j] = v;
+    }
+  for (long j = 0; j < NY - 1; j++)
+    for (long i = 0; i < NX; i++) {
+      int a = az[i + NX * j], b = az[i + NX * (j + 1)];
+      int v = 0;
+      if (a == 1 && b == 1) v = 1;
+      if ((a == 1 && b == 2) || (a == 2 && b == 1)) v = 2;
+      if (a == 2 && b == 2) v = 3;
+      av[i + NX * j] = v;
+    }
+  for (long j = 0; j < NY - 1; j++)
+    for (long i = 0; i < NX - 1; i++)
+      if (az[i + NX * j] >= 1 && az[i + NX * (j + 1)] >= 1 && az[i + 1 + NX * j] >= 1 &&
+          az[i + 1 + NX * (j + 1)] >= 1)
+        ax[i + NX * j] = 1;
+}
+
+extern "C" int gt_create(gt_ctx** out, int imin, int imax, int jmin, int jmax, int kmax,
+                         double dx, double dy, const int32_t* az, const int32_t* au,
+                         const int32_t* av, const int32_t* ax) {
+  if (!out || !az) return set_err(GT_EARG, "gt_create: NULL argument");
+  *out = nullptr;
+  if (imax < imin || jmax < jmin || kmax < 1 || !(dx > 0) || !(dy > 0))
+    return set_err(GT_EARG, "gt_create: bad extents imin=%d imax=%d jmin=%d jmax=%d kmax=%d",
+                   imin, imax, jmin, jmax, kmax);
+  gt_ctx* c = new gt_ctx();
+  c->imin = imin; c->imax = imax; c->jmin = jmin; c->jmax = jmax; c->kmax = kmax;
+  c->d.NX = imax - imin + 1 + 2 * GT_HALO;
+  c->d.NY = jmax - jmin + 1 + 2 * GT_HALO;
+  c->d.kmax = kmax;
+  c->d.NXY = (long)c->d.NX * c->d.NY;
+  c->N3 = c->d.NXY * (kmax + 1);
+  c->dx = dx; c->dy = dy;
+  c->ard1 = 1.0 / (dx * dy);  // domain.F90:596
+  const long NX = c->d.NX, NY = c->d.NY, NXY = c->d.NXY;
+
+  std::vector<int32_t> hint(4 * NXY);
+  int32_t *haz = hint.data(), *hau = haz + NXY, *hav = hau + NXY, *hax = hav + NXY;
+  memcpy(haz, az, sizeof(int32_t) * NXY);
+  std::vector<int32_t> dau(NXY), dav(NXY), dax(NXY);
+  masks_from_az(NX, NY, haz, dau.data(), dav.data(), dax.data());
+  memcpy(hau, au ? au : dau.data(), sizeof(int32_t) * NXY);
+  memcpy(hav, av ? av : dav.data(), sizeof(int32_t) * NXY);
+  memcpy(hax, ax ? ax : dax.data(), sizeof(int32_t) * NXY);
+
+  // init_advection, advection.F90:263-318
+  std::vector<uint8_t> m(8 * NXY, 0);
+  uint8_t *updH = &m[0], *uflux = &m[NXY], *vflux = &m[2 * NXY], *xflux = &m[3 * NXY],
+          *uupdU = &m[4 * NXY], *vupdV = &m[5 * NXY], *ufluxU = &m[6 * NXY],
+          *vfluxV = &m[7 * NXY];
+  for (long n = 0; n < NXY; n++) {
+    updH[n] = haz[n] == 1;
+    uflux[n] = hau[n] == 1 || hau[n] == 2;
+    vflux[n] = hav[n] == 1 || hav[n] == 2;
+    xflux[n] = hax[n] == 1;
+    uupdU[n] = hau[n] == 1;
+    vupdV[n] = hav[n] == 1;
+  }
+  for (long j = 0; j < NY; j++) uflux[NX - 1 + NX * j] = xflux[NX - 1 + NX * j] = 0;
+  for (long i = 0; i < NX; i++) vflux[i + NX * (NY - 1)] = xflux[i + NX * (NY - 1)] = 0;
+  for (long j = 0; j < NY; j++)
+    for (long i = 0; i < NX - 1; i++) ufluxU[i + NX * j] = updH[i + 1 + NX * j];
+  for (long j = 0; j < NY - 1; j++)
+    for (long i = 0; i < NX; i++) vfluxV[i + NX * j] = updH[i + NX * (j + 1)];
+
+  hipError_t e = hipMalloc(&c->dmask, m.size());
+  if (e == hipSuccess) e = hipMalloc(&c->dint, sizeof(int32_t) * 4 * NXY);
+  if (e == hipSuccess) e = hipMemcpy(c->dmask, m.data(), m.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(c->dint, hint.data(), sizeof(int32_t) * 4 * NXY, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&c->nwarn, sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMemset(c->nwarn, 0, sizeof(unsigned long long));
+  if (e == hipSuccess) e = hipMalloc(&c->zz, sizeof(double) * c->N3);
+  if (e == hipSuccess) e = hipMemset(c->zz, 0, sizeof(double) * c->N3);
+  if (e != hipSuccess) {
+    gt_destroy(c);
+    return set_err(GT_EHIP, "gt_create: %s", hipGetErrorString(e));
+  }
+  const uint8_t* dm = c->dmask;
+  c->az = c->dint; c->au = c->dint + NXY; c->av = c->dint + 2 * NXY; c->ax = c->dint + 3 * NXY;
+  c->H = {dm + NXY, dm + 2 * NXY, dm, dm, dm, c->az};
+  c->U = {dm + 6 * NXY, dm + 3 * NXY, dm + 4 * NXY, dm + NXY, dm + NXY, c->au};
+  c->V = {dm + 3 * NXY, dm + 7 * NXY, dm + 2 * NXY, dm + 5 * NXY, dm + 2 * NXY, c->av};
+  *out = c;
+  return GT_OK;
+}
+
+extern "C" int gt_destroy(gt_ctx* c) {
+  if (!c) return GT_OK;
+  void* ps[] = {c->dmask, c->dint, c->fi, c->hi, c->adv, c->tmp, c->scr7,
+                c->zz, c->P, c->slice3, c->bpart, c->bout, c->nwarn};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  delete c;
+  return GT_OK;
+}
+
+extern "C" int gt_set_stream(gt_ctx* c, void* s) {
+  if (!c) return set_err(GT_EARG, "gt_set_stream: NULL context");
+  c->st = (hipStream_t)s;
+  return GT_OK;
+}
+
+extern "C" int gt_synchronize(gt_ctx* c) {
+  if (!c) return set_err(GT_EARG, "gt_synchronize: NULL context");
+  HIPCHK(hipStreamSynchronize(c->st));
+  return GT_OK;
+}
+
+extern "C" int gt_set_adv_ver_iterations(gt_ctx* c, int iters) {
+  if (!c) return set_err(GT_EARG, "gt_set_adv_ver_iterations: NULL context");
+  c->adv_ver_iterations = iters;
+  return GT_OK;
+}
+
+extern "C" long long gt_warnings(gt_ctx* c) {
+  if (!c) return -1;
+  unsigned long long h = 0;
+  if (hipMemcpyAsync(&h, c->nwarn, sizeof(h), hipMemcpyDeviceToHost, c->st) != hipSuccess)
+    return -1;
+  if (hipStreamSynchronize(c->st) != hipSuccess) return -1;
+  if (hipMemsetAsync(c->nwarn, 0, sizeof(h), c->st) != hipSuccess) return -1;
+  return (long long)h;
+}
+
+// ---- helpers ---------------------------------------------------------------
+static const GridMasks* grid_for(gt_ctx* c, int tag) {
+  switch (tag) {
+    case GT_H_TAG: case GT_D_TAG: case GT_W_TAG: return &c->H;
+    case GT_U_TAG: return &c->U;
+    case GT_V_TAG: return &c->V;
+    default: return nullptr;
+  }
+}
+static bool split_scheme(int s) {
+  return s == GT_UPSTREAM || s == GT_P2 || s == GT_SUPERBEE || s == GT_MUSCL ||
+         s == GT_P2_PDM;
+}
+static bool twodh_scheme(int s) {
+  return s == GT_UPSTREAM_2DH || s == GT_J7 || s == GT_FCT || s == GT_P2_2DH;
+}
+static int copy_d2d(gt_ctx* c, double* dst, const double* src, long n) {
+  if (dst == src || n <= 0) return GT_OK;
+  HIPCHK(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToDevice, c->st));
+  return GT_OK;
+}
+#define TRY(x)                 \
+  do {                         \
+    int r_ = (x);              \
+    if (r_ != GT_OK) return r_; \
+  } while (0)
+#define LCHK(x)                                                                         \
+  do {                                                                                  \
+    hipError_t e_ = (x);                                                                \
+    if (e_ != hipSuccess) return set_err(GT_EHIP, "launch failed: %s", hipGetErrorString(e_)); \
+  } while (0)
+
+// A field advanced by a sequence of sub-steps that read neighbours of the
+// current field: every sub-step writes the other buffer, and the result is
+// copied back to the caller's array at the end if needed.  `nlev` levels
+// starting at level `k0` are sub-stepped.
+struct PingPong {
+  double* user;
+  double* cur;
+  double* oth;
+  void swap() { double* t = cur; cur = oth; oth = t; }
+};
+
+// horizontal sub-steps over `ns` slices starting at level k0
+static int step_u(gt_ctx* c, const GridMasks* g, int k0, int ns, const double* f,
+                  const double* fis, double* fid, double* Di, double* adv, const double* U,
+                  const double* DU, double dt, double sf, int scheme, double AH) {
+  long o = c->d.NXY * k0;
+  LCHK(gt::launch_split_u(c->st, c->d, ns, f + o, fis + o, fid + o, Di + o, adv + o, U + o,
+                          DU + o, g->uflux, g->uupd, dt, sf, scheme, AH, c->dx, c->dy,
+                          c->ard1));
+  return GT_OK;
+}
+static int step_v(gt_ctx* c, const GridMasks* g, int k0, int ns, const double* f,
+                  const double* fis, double* fid, double* Di, double* adv, const double* V,
+                  const double* DV, double dt, double sf, int scheme, double AH) {
+  long o = c->d.NXY * k0;
+  LCHK(gt::launch_split_v(c->st, c->d, ns, f + o, fis + o, fid + o, Di + o, adv + o, V + o,
+                          DV + o, g->vflux, g->vupd, dt, sf, scheme, AH, c->dx, c->dy,
+                          c->ard1));
+  return GT_OK;
+}
+static int step_up2dh(gt_ctx* c, const GridMasks* g, int k0, int ns, const double* f,
+                      const double* fis, double* fid, double* Di, double* adv, const double* U,
+                      const double* V, const double* Dn, double dt, double AH) {
+  long o = c->d.NXY * k0;
+  LCHK(gt::launch_upstream_2dh(c->st, c->d, ns, f + o, fis + o, fid + o, Di + o, adv + o,
+                               U + o, V + o, Dn + o, g->az, dt, AH, c->dx, c->dy, c->ard1));
+  return GT_OK;
+}
+static int step_w(gt_ctx* c, const GridMasks* g, const double* f, const double* fis,
+                  double* fid, double* hi, double* adv, const double* ww, double dt, double sf,
+                  int scheme, int tag, int itersmax) {
+  if (scheme == GT_NOADV) return copy_d2d(c, fid, fis, c->N3);
+  int kshift = tag == GT_W_TAG ? 1 : 0;
+  if (itersmax > 1) {
+    ENSURE(c->scr7, 7 * c->N3);
+    LCHK(gt::launch_split_w_iter(c->st, c->d, f, fis, fid, hi, adv, ww, g->az, c->scr7,
+                                 c->nwarn, dt, sf, scheme, kshift, itersmax));
+  } else {
+    LCHK(gt::launch_split_w(c->st, c->d, f, fis, fid, hi, adv, ww, g->az, dt, sf, scheme,
+                            kshift));
+  }
+  return GT_OK;
+}
+
+// do_advection (advection.F90:362-702) on ns slices from level k0 of 3-D
+// arrays (ns=1, k0=0 for the 2-D entry point).  p_Di/p_adv are already
+// initialised by the caller.
+static int adv2d_slices(gt_ctx* c, const GridMasks* g, int k0, int ns, double dt, double* f,
+                        const double* U, const double* V, const double* DU, const double* DV,
+                        const double* Do, const double* Dn, int split, int scheme, double AH,
+                        double* pDi, double* padv, double* fi_buf, double* tmp_buf,
+                        long nfield) {
+  if (scheme == GT_NOADV) return GT_OK;
+  const long o = c->d.NXY * k0, n = c->d.NXY * ns;
+  if (split == GT_NOSPLIT) {
+    if (split_scheme(scheme)) {
+      TRY(copy_d2d(c, fi_buf + o, f + o, n));
+      TRY(step_u(c, g, k0, ns, f, fi_buf, fi_buf, pDi, padv, U, DU, dt, 1.0, scheme, AH));
+      TRY(step_v(c, g, k0, ns, f, fi_buf, fi_buf, pDi, padv, V, DV, dt, 1.0, scheme, AH));
+      LCHK(gt::launch_finalise(c->st, c->d, k0, ns, f, Do, padv, pDi, g->fin, dt));
+      return GT_OK;
+    }
+    if (scheme == GT_UPSTREAM_2DH) {
+      TRY(step_up2dh(c, g, k0, ns, f, f, tmp_buf, pDi, padv, U, V, Dn, dt, AH));
+      return copy_d2d(c, f + o, tmp_buf + o, n);
+    }
+  } else if (split == GT_FULLSPLIT || split == GT_HALFSPLIT) {
+    (void)nfield;
+    PingPong pp{f, f, tmp_buf};
+    double sfu = split == GT_HALFSPLIT ? 0.5 : 1.0;
+    TRY(step_u(c, g, k0, ns, pp.cur, pp.cur, pp.oth, pDi, padv, U, DU, dt, sfu, scheme, AH));
+    pp.swap();
+    TRY(step_v(c, g, k0, ns, pp.cur, pp.cur, pp.oth, pDi, padv, V, DV, dt, 1.0, scheme, AH));
+    pp.swap();
+    if (split == GT_HALFSPLIT) {
+      TRY(step_u(c, g, k0, ns, pp.cur, pp.cur, pp.oth, pDi, padv, U, DU, dt, 0.5, scheme, AH));
+      pp.swap();
+    }
+    if (pp.cur != f) return copy_d2d(c, f + o, pp.cur + o, n);
+    return GT_OK;
+  }
+  return set_err(GT_EINVAL_SPLIT, "do_advection: split is invalid");
+}
+
+static int check_2d(int split, int scheme) {
+  if (scheme == GT_NOADV) return GT_OK;
+  if (scheme < 0 || scheme > 9) return set_err(GT_EINVAL_SCHEME, "do_advection: scheme is invalid");
+  switch (split) {
+    case GT_NOSPLIT:
+      if (scheme == GT_J7 || scheme == GT_FCT || scheme == GT_P2_2DH)
+        return set_err(GT_ENOTIMPL, "do_advection: scheme %d is not part of this build", scheme);
+      return GT_OK;
+    case GT_FULLSPLIT:
+    case GT_HALFSPLIT:
+      if (twodh_scheme(scheme))
+        return set_err(GT_ESCHEME_SPLIT, "do_advection: scheme not valid for split");
+      return GT_OK;
+    default:
+      return set_err(GT_EINVAL_SPLIT, "do_advection: split is invalid");
+  }
+}
+
+// ---- advection entry points --------------------------------------------------
+__global__ void k_reconstruct_batch(int scheme, long n, const double* cfl, const double* fuu,
+                                    const double* fu, const double* fd, double* out) {
+  long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q < n) out[q] = gt::reconstruct<double>(scheme, cfl[q], fuu[q], fu[q], fd[q]);
+}
+
+extern "C" int gt_adv_interfacial_reconstruction(gt_ctx* c, int scheme, long n,
+                                                 const double* cfl, const double* fuu,
+                                                 const double* fu, const double* fd,
+                                                 double* out) {
+  if (!c || n < 0) return set_err(GT_EARG, "gt_adv_interfacial_reconstruction: bad argument");
+  if (n == 0) return GT_OK;
+  hipLaunchKernelGGL(k_reconstruct_batch, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st,
+                     scheme, n, cfl, fuu, fu, fd, out);
+  LCHK(hipGetLastError());
+  return GT_OK;
+}
+
+extern "C" int gt_adv_split_u(gt_ctx* c, double dt, const double* f, double* fi, double* Di,
+                              double* adv, const double* U, const double* DU, double splitfac,
+                              int scheme, double AH, int tag) {
+  if (!c) return set_err(GT_EARG, "gt_adv_split_u: NULL context");
+  const GridMasks* g = grid_for(c, tag);
+  if (!g) return set_err(GT_EINVAL_TAG, "adv_split_u: tag is invalid");
+  if (!split_scheme(scheme)) return set_err(GT_EINVAL_SCHEME, "adv_split_u: scheme is invalid");
+  if (f == fi) {
+    ENSURE(c->slice3, 3 * c->d.NXY);
+    TRY(step_u(c, g, 0, 1, f, f, c->slice3, Di, adv, U, DU, dt, splitfac, scheme, AH));
+    return copy_d2d(c, fi, c->slice3, c->d.NXY);
+  }
+  return step_u(c, g, 0, 1, f, fi, fi, Di, adv, U, DU, dt, splitfac, scheme, AH);
+}
+
+extern "C" int gt_adv_split_v(gt_ctx* c, double dt, const double* f, double* fi, double* Di,
+                              double* adv, const double* V, const double* DV, double splitfac,
+                              int scheme, double AH, int tag) {
+  if (!c) return set_err(GT_EARG, "gt_adv_split_v: NULL context");
+  const GridMasks* g = grid_for(c, tag);
+  if (!g) return set_err(GT_EINVAL_TAG, "adv_split_v: tag is invalid");
+  if (!split_scheme(scheme)) return set_err(GT_EINVAL_SCHEME, "adv_split_v: scheme is invalid");
+  if (f == fi) {
+    ENSURE(c->slice3, 3 * c->d.NXY);
+    TRY(step_v(c, g, 0, 1, f, f, c->slice3, Di, adv, V, DV, dt, splitfac, scheme, AH));
+    return copy_d2d(c, fi, c->slice3, c->d.NXY);
+  }
+  return step_v(c, g, 0, 1, f, fi, fi, Di, adv, V, DV, dt, splitfac, scheme, AH);
+}
+
+extern "C" int gt_adv_split_w(gt_ctx* c, double dt, const double* f, double* fi, double* hi,
+                              double* adv, const double* ww, double splitfac, int scheme,
+                              int tag, int itersmax) {
+  if (!c) return set_err(GT_EARG, "gt_adv_split_w: NULL context");
+  if (!grid_for(c, tag)) return set_err(GT_EINVAL_TAG, "adv_split_w: tag is invalid");
+  if (scheme != GT_NOADV && !split_scheme(scheme))
+    return set_err(GT_EINVAL_SCHEME, "adv_split_w: scheme is invalid");
+  if (scheme == GT_NOADV) return GT_OK;
+  // adv_split_w receives adv_grid%az; for the 3-D tags that is the H grid
+  const GridMasks* g = grid_for(c, tag);
+  if (f == fi) {
+    ENSURE(c->tmp, c->N3);
+    TRY(step_w(c, g, f, f, c->tmp, hi, adv, ww, dt, splitfac, scheme, tag, itersmax));
+    return copy_d2d(c, fi, c->tmp, c->N3);
+  }
+  return step_w(c, g, f, fi, fi, hi, adv, ww, dt, splitfac, scheme, tag, itersmax);
+}
+
+extern "C" int gt_adv_upstream_2dh(gt_ctx* c, double dt, const double* f, double* fi,
+                                   double* Di, double* adv, const double* U, const double* V,
+                                   const double* Dn, const double* DU, const double* DV,
+                                   double AH, int tag) {
+  (void)DU; (void)DV;
+  if (!c) return set_err(GT_EARG, "gt_adv_upstream_2dh: NULL context");
+  const GridMasks* g = grid_for(c, tag);
+  if (!g) return set_err(GT_EINVAL_TAG, "adv_upstream_2dh: tag is invalid");
+  if (f == fi) {
+    ENSURE(c->slice3, 3 * c->d.NXY);
+    TRY(step_up2dh(c, g, 0, 1, f, f, c->slice3, Di, adv, U, V, Dn, dt, AH));
+    return copy_d2d(c, fi, c->slice3, c->d.NXY);
+  }
+  return step_up2dh(c, g, 0, 1, f, fi, fi, Di, adv, U, V, Dn, dt, AH);
+}
+
+extern "C" int gt_do_advection(gt_ctx* c, double dt, double* f, const double* U,
+                               const double* V, const double* DU, const double* DV,
+                               const double* Do, const double* Dn, int split, int scheme,
+                               double AH, int tag, double* Dires, double* advres) {
+  if (!c) return set_err(GT_EARG, "gt_do_advection: NULL context");
+  if (tag == GT_W_TAG) return set_err(GT_EINVAL_TAG, "do_advection: tag is invalid");
+  const GridMasks* g = grid_for(c, tag);
+  if (!g) return set_err(GT_EINVAL_TAG, "do_advection: tag is invalid");
+  TRY(check_2d(split, scheme));
+  const long n = c->d.NXY;
+  ENSURE(c->slice3, 3 * n);
+  double* fi = c->slice3;
+  double* tmp = c->slice3 + n;
+  double* pDi = Dires;
+  double* padv = advres;
+  if (!pDi || !padv) {
+    ENSURE(c->hi, c->N3);
+    ENSURE(c->adv, c->N3);
+    if (!pDi) pDi = c->hi;
+    if (!padv) padv = c->adv;
+  }
+  TRY(copy_d2d(c, pDi, Do, n));
+  HIPCHK(hipMemsetAsync(padv, 0, sizeof(double) * n, c->st));
+  return adv2d_slices(c, g, 0, 1, dt, f, U, V, DU, DV, Do, Dn, split, scheme, AH, pDi, padv, fi,
+                      tmp, n);
+}
+
+extern "C" int gt_do_advection_3d(gt_ctx* c, double dt, double* f, const double* uu,
+                                  const double* vv, const double* ww, const double* hu,
+                                  const double* hv, const double* ho, const double* hn,
+                                  int split, int hscheme, int vscheme, double AH, int tag,
+                                  double* hires, double* advres) {
+  if (!c) return set_err(GT_EARG, "gt_do_advection_3d: NULL context");
+  int tag2d;
+  const GridMasks* g;
+  switch (tag) {
+    case GT_H_TAG: case GT_D_TAG: tag2d = GT_H_TAG; g = &c->H; break;
+    case GT_U_TAG: tag2d = GT_U_TAG; g = &c->U; break;
+    case GT_V_TAG: tag2d = GT_V_TAG; g = &c->V; break;
+    case GT_W_TAG: tag2d = GT_H_TAG; g = &c->H; break;
+    default: return set_err(GT_EINVAL_TAG, "do_advection_3d: tag is invalid");
+  }
+  (void)tag2d;
+  const bool hadv = hscheme != GT_NOADV, vadv = vscheme != GT_NOADV;
+  // the checks print_adv_settings_3d makes at start-up (advection_3d.F90:
+  // 600-652) plus the stops of do_advection_3d, made before any work
+  if (hscheme < 0 || hscheme > 9)
+    return set_err(GT_EINVAL_SCHEME, "do_advection_3d: hscheme is invalid");
+  if (vadv && !split_scheme(vscheme))
+    return set_err(GT_EINVAL_SCHEME, "do_advection_3d: vscheme %d is invalid", vscheme);
+  if (hadv && vadv) {
+    if (split < 0 || split > 3) return set_err(GT_EINVAL_SPLIT, "do_advection_3d: split is invalid");
+    if ((split == GT_FULLSPLIT || split == GT_HALFSPLIT || split == GT_HVSPLIT) &&
+        twodh_scheme(hscheme) && !(split == GT_HVSPLIT && hscheme == GT_UPSTREAM_2DH))
+      return set_err(split == GT_HVSPLIT ? GT_ENOTIMPL : GT_ESCHEME_SPLIT,
+                     "do_advection_3d: hscheme not valid for split");
+    if (split == GT_NOSPLIT && (hscheme == GT_J7 || hscheme == GT_FCT || hscheme == GT_P2_2DH))
+      return set_err(GT_ENOTIMPL, "do_advection_3d: hscheme %d is not part of this build",
+                     hscheme);
+  } else if (hadv) {
+    TRY(check_2d(split, hscheme));
+  }
+  const long N3 = c->N3, NXY = c->d.NXY;
+  const int kmax = c->kmax;
+  double* phi = hires;
+  double* padv = advres;
+  if (!phi) { ENSURE(c->hi, N3); phi = c->hi; }
+  if (!padv) { ENSURE(c->adv, N3); padv = c->adv; }
+  TRY(copy_d2d(c, phi, ho, N3));
+  HIPCHK(hipMemsetAsync(padv, 0, sizeof(double) * N3, c->st));
+  if (!hadv && !vadv) return GT_OK;
+  ENSURE(c->tmp, N3);
+  ENSURE(c->fi, N3);
+  double* tmp = c->tmp;
+  // level 0 is never advanced horizontally; keep it equal in both buffers
+  TRY(copy_d2d(c, tmp, f, NXY));
+  const int itv = c->adv_ver_iterations;
+
+  if (!hadv) {
+    if (kmax > 1) {
+      TRY(step_w(c, g, f, f, tmp, phi, padv, ww, dt, 1.0, vscheme, tag, itv));
+      TRY(copy_d2d(c, f, tmp, N3));
+    }
+    return GT_OK;
+  }
+  if (!vadv) {
+    return adv2d_slices(c, g, 1, kmax, dt, f, uu, vv, hu, hv, ho, hn, split, hscheme, AH, phi,
+                        padv, c->fi, tmp, N3);
+  }
+  switch (split) {
+    case GT_NOSPLIT: {
+      double* fi = c->fi;
+      TRY(copy_d2d(c, fi, f, N3));
+      if (split_scheme(hscheme)) {
+        TRY(step_u(c, g, 1, kmax, f, fi, fi, phi, padv, uu, hu, dt, 1.0, hscheme, AH));
+        TRY(step_v(c, g, 1, kmax, f, fi, fi, phi, padv, vv, hv, dt, 1.0, hscheme, AH));
+      } else {  // UPSTREAM_2DH
+        TRY(step_up2dh(c, g, 1, kmax, f, fi, fi, phi, padv, uu, vv, hn, dt, AH));
+      }
+      if (kmax > 1) TRY(step_w(c, g, f, fi, fi, phi, padv, ww, dt, 1.0, vscheme, tag, 1));
+      LCHK(gt::launch_finalise(c->st, c->d, 1, kmax, f, ho, padv, phi, g->fin, dt));
+      return GT_OK;
+    }
+    case GT_FULLSPLIT: {
+      PingPong pp{f, f, tmp};
+      TRY(step_u(c, g, 1, kmax, pp.cur, pp.cur, pp.oth, phi, padv, uu, hu, dt, 1.0, hscheme, AH));
+      pp.swap();
+      TRY(step_v(c, g, 1, kmax, pp.cur, pp.cur, pp.oth, phi, padv, vv, hv, dt, 1.0, hscheme, AH));
+      pp.swap();
+      if (kmax > 1) {
+        TRY(step_w(c, g, pp.cur, pp.cur, pp.oth, phi, padv, ww, dt, 1.0, vscheme, tag, itv));
+        pp.swap();
+      }
+      if (pp.cur != f) TRY(copy_d2d(c, f, pp.cur, N3));
+      return GT_OK;
+    }
+    case GT_HALFSPLIT: {
+      PingPong pp{f, f, tmp};
+      TRY(step_u(c, g, 1, kmax, pp.cur, pp.cur, pp.oth, phi, padv, uu, hu, dt, 0.5, hscheme, AH));
+      pp.swap();
+      TRY(step_v(c, g, 1, kmax, pp.cur, pp.cur, pp.oth, phi, padv, vv, hv, dt, 0.5, hscheme, AH));
+      pp.swap();
+      if (kmax > 1) {
+        TRY(step_w(c, g, pp.cur, pp.cur, pp.oth, phi, padv, ww, dt, 1.0, vscheme, tag, itv));
+        pp.swap();
+      }
+      TRY(step_v(c, g, 1, kmax, pp.cur, pp.cur, pp.oth, phi, padv, vv, hv, dt, 0.5, hscheme, AH));
+      pp.swap();
+      TRY(step_u(c, g, 1, kmax, pp.cur, pp.cur, pp.oth, phi, padv, uu, hu, dt, 0.5, hscheme, AH));
+      pp.swap();
+      if (pp.cur != f) TRY(copy_d2d(c, f, pp.cur, N3));
+      return GT_OK;
+    }
+    case GT_HVSPLIT: {
+      TRY(adv2d_slices(c, g, 1, kmax, dt, f, uu, vv, hu, hv, ho, hn, GT_NOSPLIT, hscheme, AH, phi,
+                       padv, c->fi, tmp, N3));
+      if (kmax > 1) {
+        TRY(step_w(c, g, f, f, tmp, phi, padv, ww, dt, 1.0, vscheme, tag, itv));
+        TRY(copy_d2d(c, f, tmp, N3));
+      }
+      return GT_OK;
+    }
+  }
+  return set_err(GT_EINVAL_SPLIT, "do_advection_3d: split is invalid");
+}
+
+// ---- internal pressure -------------------------------------------------------
+extern "C" int gt_eos_linear(gt_ctx* c, const double* T, const double* S, double* rho,
+                             double* buoy, double T0, double S0, double dtr0, double dsr0) {
+  if (!c || !T || !S || !buoy) return set_err(GT_EARG, "gt_eos_linear: NULL argument");
+  LCHK(gt::launch_eos_linear(c->st, c->d, T, S, rho, buoy, T0, S0, dtr0, dsr0));
+  return GT_OK;
+}
+
+extern "C" int gt_ip_blumberg_mellor(gt_ctx* c, const double* H, const double* hn,
+                                     const double* hun, const double* hvn, const double* buoy,
+                                     double* zz, double* idpdx, double* idpdy) {
+  if (!c) return set_err(GT_EARG, "gt_ip_blumberg_mellor: NULL context");
+  if (c->kmax < 2) return set_err(GT_EARG, "ip_blumberg_mellor: kmax must be >= 2");
+  LCHK(gt::launch_ip_bm(c->st, c->d, c->az, c->au, c->av, H, hn, hun, hvn, buoy, zz, idpdx,
+                        idpdy, c->dx, c->dy));
+  return GT_OK;
+}
+
+extern "C" int gt_ip_shchepetkin_mcwilliams(gt_ctx* c, const double* H, const double* hn,
+                                            const double* hun, const double* hvn,
+                                            const double* buoy, const double* sseo, double* zz,
+                                            double* idpdx, double* idpdy) {
+  if (!c) return set_err(GT_EARG, "gt_ip_shchepetkin_mcwilliams: NULL context");
+  if (c->kmax < 2) return set_err(GT_EARG, "ip_shchepetkin_mcwilliams: kmax must be >= 2");
+  ENSURE(c->P, c->N3);
+  LCHK(gt::launch_ip_smcw(c->st, c->d, c->az, c->au, c->av, H, hn, hun, hvn, buoy, sseo, zz,
+                          c->P, idpdx, idpdy, c->dx, c->dy));
+  return GT_OK;
+}
+
+extern "C" int gt_do_internal_pressure(gt_ctx* c, int ip_method, const double* H,
+                                       const double* hn, const double* hun, const double* hvn,
+                                       const double* buoy, const double* sseo, double* idpdx,
+                                       double* idpdy) {
+  if (!c) return set_err(GT_EARG, "gt_do_internal_pressure: NULL context");
+  switch (ip_method) {
+    case GT_IP_BLUMBERG_MELLOR:
+      return gt_ip_blumberg_mellor(c, H, hn, hun, hvn, buoy, c->zz, idpdx, idpdy);
+    case GT_IP_SHCHEPETKIN_MCWILLIAMS:
+      return gt_ip_shchepetkin_mcwilliams(c, H, hn, hun, hvn, buoy, sseo, c->zz, idpdx, idpdy);
+    case 2: case 3: case 4: case 5: case 7:
+      return set_err(GT_ENOTIMPL, "do_internal_pressure: ip_method %d is not part of this build",
+                     ip_method);
+    default:
+      return set_err(GT_EINVAL_SCHEME, "Not valid ip_method specified");
+  }
+}
+
+// ---- ghost cells ---------------------------------------------------------------
+extern "C" int gt_update_3d_halo(gt_ctx* c, double* f1, const double* f2, int tag, int mirror) {
+  (void)tag;
+  if (!c) return set_err(GT_EARG, "gt_update_3d_halo: NULL context");
+  if (mirror != 1) return GT_OK;
+  LCHK(gt::launch_mirror_3d(c->st, c->d, f1, f2));
+  return GT_OK;
+}
+
+extern "C" int gt_update_2d_halo(gt_ctx* c, double* f1, const double* f2, int tag, int mirror) {
+  (void)tag;
+  if (!c) return set_err(GT_EARG, "gt_update_2d_halo: NULL context");
+  if (mirror >= 0) c->do_mirror_2d = mirror != 0;
+  if (!c->do_mirror_2d) return GT_OK;
+  LCHK(gt::launch_mirror_2d(c->st, c->d, f1, f2));
+  return GT_OK;
+}
+
+extern "C" int gt_wait_halo(gt_ctx* c, int tag) {
+  (void)tag;
+  return gt_synchronize(c);
+}
+
+extern "C" int gt_exchange_tiles(int px, int py, const int* ni, const int* nj, int nk,
+                                 double* const* tiles, void* stream) {
+  if (px < 1 || py < 1 || !ni || !nj || !tiles || nk < 1)
+    return set_err(GT_EARG, "gt_exchange_tiles: bad argument");
+  const int nt = px * py;
+  for (int ty = 0; ty < py; ty++)
+    for (int tx = 0; tx < px; tx++) {
+      int t = ty * px + tx;
+      if (ni[t] < GT_HALO || nj[t] < GT_HALO || !tiles[t])
+        return set_err(GT_EARG, "gt_exchange_tiles: tile %d smaller than the halo", t);
+      if (nj[t] != nj[ty * px] || ni[t] != ni[tx])
+        return set_err(GT_EARG, "gt_exchange_tiles: tiles of one row/column differ in size");
+    }
+  std::vector<gt::TileDesc> td(nt);
+  std::vector<long> off(nt + 1, 0);
+  for (int t = 0; t < nt; t++) {
+    td[t] = {tiles[t], ni[t], nj[t]};
+    long NX = ni[t] + 2 * GT_HALO;
+    off[t + 1] = off[t] + 2 * GT_HALO * NX + 2 * GT_HALO * (long)nj[t];
+  }
+  hipStream_t st = (hipStream_t)stream;
+  void* dbuf = nullptr;
+  size_t bytes = sizeof(gt::TileDesc) * nt + sizeof(long) * (nt + 1);
+  HIPCHK(hipMalloc(&dbuf, bytes));
+  gt::TileDesc* dtd = (gt::TileDesc*)dbuf;
+  long* doff = (long*)((char*)dbuf + sizeof(gt::TileDesc) * nt);
+  hipError_t e = hipMemcpyAsync(dtd, td.data(), sizeof(gt::TileDesc) * nt,
+                                hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(doff, off.data(), sizeof(long) * (nt + 1), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = gt::launch_tile_exchange(st, nt, px, dtd, doff, off[nt], nk);
+  hipError_t e2 = hipStreamSynchronize(st);
+  (void)hipFree(dbuf);
+  if (e != hipSuccess) return set_err(GT_EHIP, "gt_exchange_tiles: %s", hipGetErrorString(e));
+  if (e2 != hipSuccess) return set_err(GT_EHIP, "gt_exchange_tiles: %s", hipGetErrorString(e2));
+  return GT_OK;
+}
+
+// ---- budgets -----------------------------------------------------------------------
+extern "C" int gt_budget(gt_ctx* c, const double* h, const double* f, double* mass,
+                         double* volume) {
+  if (!c || !h) return set_err(GT_EARG, "gt_budget: NULL argument");
+  const int nblocks = 1024;
+  ENSURE(c->bpart, 4 * nblocks);
+  ENSURE(c->bout, 2);
+  LCHK(gt::launch_budget(c->st, c->d, c->az, h, f, c->bpart, nblocks, c->dx * c->dy, c->bout));
+  double hb[2];
+  HIPCHK(hipMemcpyAsync(hb, c->bout, sizeof(hb), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (mass) *mass = hb[0];
+  if (volume) *volume = hb[1];
+  return GT_OK;
+}
+
+// ---- memory helpers -----------------------------------------------------------------
+extern "C" int gt_malloc(void** p, size_t bytes) {
+  if (!p) return set_err(GT_EARG, "gt_malloc: NULL pointer");
+  HIPCHK(hipMalloc(p, bytes));
+  return GT_OK;
+}
+extern "C" int gt_free(void* p) {
+  if (p) HIPCHK(hipFree(p));
+  return GT_OK;
+}
+extern "C" int gt_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return GT_OK;
+}
+extern "C" int gt_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return GT_OK;
+}

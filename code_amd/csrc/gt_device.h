@@ -1,0 +1,74 @@
+// gt_device.h -- device-side helpers shared by the transport kernels.
+//
+// Field layout is the reference's: f(imin-2:imax+2, jmin-2:jmax+2, 0:kmax),
+// i fastest (include/cppdefs.h:146-159 of getm-model/code).  Kernels index
+// with ii = i-(imin-2) in [0,NX) and jj = j-(jmin-2) in [0,NY); the interior
+// is ii in [2, NX-3].  Every arithmetic expression keeps the reference's
+// left-to-right order; the library is built with -ffp-contract=off so the
+// FP64 path rounds exactly as the CPU build does.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GT_HALO 2
+
+namespace gt {
+
+enum Scheme { NOADV = 0, UPSTREAM = 1, UPSTREAM_2DH = 2, P2 = 3, SUPERBEE = 4,
+              MUSCL = 5, P2_PDM = 6, J7 = 7, FCT = 8, P2_2DH = 9 };
+enum Split { NOSPLIT = 0, FULLSPLIT = 1, HALFSPLIT = 2, HVSPLIT = 3 };
+enum Tag { H_TAG = 10, D_TAG = 20, U_TAG = 31, V_TAG = 32, W_TAG = 33 };
+
+struct Dims {
+  int NX, NY, kmax;
+  long NXY;
+};
+
+template <typename T>
+__device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
+template <typename T>
+__device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
+
+// adv_interfacial_reconstruction_p2, src/2d/advection.F90:854-879
+template <typename T>
+__device__ __forceinline__ T reconstruct_p2(T cfl, T fu, T deltafu, T deltaf) {
+  const T one6th = T(1) / T(6);
+  T x = one6th * (T(1) - T(2) * cfl);
+  return fu + T(0.5) * (T(1) - cfl) * ((T(0.5) + x) * deltaf + (T(0.5) - x) * deltafu);
+}
+
+// adv_interfacial_reconstruction, src/2d/advection.F90:785-845
+template <typename T>
+__device__ __forceinline__ T reconstruct(int scheme, T cfl, T fuu, T fu, T fd) {
+  const T one6th = T(1) / T(6);
+  T deltaf = fd - fu;
+  T deltafu = fu - fuu;
+  if (deltaf * deltafu > T(0)) {
+    T ratio = deltafu / deltaf;
+    T limiter = T(0);
+    switch (scheme) {
+      case P2_PDM: {
+        T x = one6th * (T(1) - T(2) * cfl);
+        limiter = (T(0.5) + x) + (T(0.5) - x) * ratio;
+        limiter = tmin(tmin(T(2) * ratio / (cfl + T(1.e-10)), limiter), T(2) / (T(1) - cfl));
+        break;
+      }
+      case SUPERBEE:
+        limiter = tmax(tmin(T(2) * ratio, T(1)), tmin(ratio, T(2)));
+        break;
+      case MUSCL:
+        limiter = tmin(tmin(T(2) * ratio, T(0.5) * (T(1) + ratio)), T(2));
+        break;
+      case P2:
+        return reconstruct_p2(cfl, fu, deltafu, deltaf);
+      default:  // UPSTREAM
+        return fu;
+    }
+    return fu + T(0.5) * limiter * (T(1) - cfl) * deltaf;
+  }
+  if (scheme == P2) return reconstruct_p2(cfl, fu, deltafu, deltaf);
+  return fu;
+}
+
+}  // namespace gt

@@ -1,0 +1,62 @@
+// gt_kernels.h -- host launchers for the transport kernels.
+#pragma once
+
+#include "gt_device.h"
+
+namespace gt {
+
+struct TileDesc {
+  double* f;
+  int ni, nj;
+};
+
+// every launcher enqueues on `st` and returns the launch status
+hipError_t launch_split_u(hipStream_t st, Dims d, int nslice, const double* f,
+                          const double* fi_src, double* fi_dst, double* Di, double* adv,
+                          const double* U, const double* DU, const uint8_t* mflux,
+                          const uint8_t* mupd, double dt, double splitfac, int scheme,
+                          double AH, double dx, double dy, double ard1);
+hipError_t launch_split_v(hipStream_t st, Dims d, int nslice, const double* f,
+                          const double* fi_src, double* fi_dst, double* Di, double* adv,
+                          const double* V, const double* DV, const uint8_t* mflux,
+                          const uint8_t* mupd, double dt, double splitfac, int scheme,
+                          double AH, double dx, double dy, double ard1);
+hipError_t launch_split_w(hipStream_t st, Dims d, const double* f, const double* fi_src,
+                          double* fi_dst, double* hi, double* adv, const double* ww,
+                          const int32_t* az, double dt, double splitfac, int scheme,
+                          int kshift);
+hipError_t launch_split_w_iter(hipStream_t st, Dims d, const double* f, const double* fi_src,
+                               double* fi_dst, double* hi, double* adv, const double* ww,
+                               const int32_t* az, double* scratch7, unsigned long long* nwarn,
+                               double dt, double splitfac, int scheme, int kshift,
+                               int itersmax);
+hipError_t launch_finalise(hipStream_t st, Dims d, int k0, int nk, double* f, const double* ho,
+                           const double* adv, const double* hi, const uint8_t* mfin,
+                           double dt);
+hipError_t launch_upstream_2dh(hipStream_t st, Dims d, int nslice, const double* f,
+                               const double* fi_src, double* fi_dst, double* Di, double* adv,
+                               const double* U, const double* V, const double* Dn,
+                               const int32_t* az, double dt, double AH, double dx, double dy,
+                               double ard1);
+hipError_t launch_eos_linear(hipStream_t st, Dims d, const double* T, const double* S,
+                             double* rho, double* buoy, double T0, double S0, double dtr0,
+                             double dsr0);
+hipError_t launch_ip_bm(hipStream_t st, Dims d, const int32_t* az, const int32_t* au,
+                        const int32_t* av, const double* H, const double* hn,
+                        const double* hun, const double* hvn, const double* buoy, double* zz,
+                        double* idpdx, double* idpdy, double dx, double dy);
+hipError_t launch_ip_smcw(hipStream_t st, Dims d, const int32_t* az, const int32_t* au,
+                          const int32_t* av, const double* H, const double* hn,
+                          const double* hun, const double* hvn, const double* buoy,
+                          const double* sseo, double* zz, double* P, double* idpdx,
+                          double* idpdy, double dx, double dy);
+hipError_t launch_mirror_3d(hipStream_t st, Dims d, double* f1, const double* f2);
+hipError_t launch_mirror_2d(hipStream_t st, Dims d, double* f1, const double* f2);
+hipError_t launch_tile_exchange(hipStream_t st, int ntiles, int px, const TileDesc* dtiles,
+                                const long* dring_off, long ring_total, int nk);
+// partial must hold 4*nblocks doubles; out two doubles (mass, volume)
+hipError_t launch_budget(hipStream_t st, Dims d, const int32_t* az, const double* h,
+                         const double* f, double* partial, int nblocks, double area,
+                         double* out);
+
+}  // namespace gt

@@ -1,0 +1,953 @@
+// gt_kernels.hip -- per-sub-step FP64 kernels of the transport step.
+//
+// One kernel per reference routine, each a faithful restatement of its loop
+// bounds and expression order (citations per kernel, getm-model/code).  The
+// reference updates fi in place while it reads neighbours of f; when f and
+// fi alias, the host passes a second buffer as fi_dst and swaps afterwards,
+// and every cell the reference does not touch is copied through.
+#include "gt_kernels.h"
+
+namespace gt {
+
+// ---------------------------------------------------------------------------
+// a2: adv_split_u, src/2d/adv_split_u.F90:205-286.  Grid: (ii, jj, slice).
+// The interface flux at i is a pure function of f, U, DU and the masks, so
+// each thread recomputes the two fluxes its update needs instead of storing
+// the uflux array.
+__device__ __forceinline__ double flux_u(const double* f, const double* U, const double* DU,
+                                         const uint8_t* mflux, long c, int scheme, double dti,
+                                         double dx, double AH) {
+  if (!mflux[c]) return 0.0;
+  double fu, fuu = 0.0, fd = 0.0, cfl = 0.0;
+  bool use_limiter = false;
+  double u = U[c];
+  if (u > 0.0) {
+    fu = f[c];
+    if (scheme != UPSTREAM) use_limiter = mflux[c - 1];
+    if (use_limiter) {
+      cfl = u / DU[c] * dti / dx;
+      fuu = f[c - 1];
+      fd = f[c + 1];
+    }
+  } else {
+    fu = f[c + 1];
+    if (scheme != UPSTREAM) use_limiter = mflux[c + 1];
+    if (use_limiter) {
+      cfl = -u / DU[c] * dti / dx;
+      fuu = f[c + 2];
+      fd = f[c];
+    }
+  }
+  if (use_limiter) fu = reconstruct<double>(scheme, cfl, fuu, fu, fd);
+  double fl = u * fu;
+  if (AH > 0.0) fl = fl - AH * DU[c] * (f[c + 1] - f[c]) / dx;
+  return fl;
+}
+
+__global__ void k_split_u(Dims d, int nslice, const double* __restrict__ f,
+                          const double* fi_src, double* fi_dst, double* Di, double* adv,
+                          const double* __restrict__ U, const double* __restrict__ DU,
+                          const uint8_t* __restrict__ mflux, const uint8_t* __restrict__ mupd,
+                          double dt, double splitfac, int scheme, double AH, double dx,
+                          double dy, double ard1) {
+  int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  int jj = blockIdx.y * blockDim.y + threadIdx.y;
+  int s = blockIdx.z;
+  if (ii >= d.NX || jj >= d.NY || s >= nslice) return;
+  long o = (long)s * d.NXY;
+  long c2 = ii + (long)d.NX * jj;
+  long c = o + c2;
+  const int isub = (scheme == UPSTREAM) ? 0 : 1;
+  bool upd = ii >= 1 + isub && ii <= d.NX - 2 - isub && mupd[c2];
+  if (upd) {
+    const double dti = splitfac * dt;
+    double fc = flux_u(f + o, U + o, DU + o, mflux, c2, scheme, dti, dx, AH);
+    double fw = flux_u(f + o, U + o, DU + o, mflux, c2 - 1, scheme, dti, dx, AH);
+    double Dio = Di[c];
+    double Dn = Dio - dti * (U[c] * dy - U[c - 1] * dy) * ard1;
+    Di[c] = Dn;
+    double advn = splitfac * (fc * dy - fw * dy) * ard1;
+    fi_dst[c] = (Dio * fi_src[c] - dt * advn) / Dn;
+    adv[c] = adv[c] + advn;
+  } else if (fi_dst != fi_src) {
+    fi_dst[c] = fi_src[c];
+  }
+}
+
+// a3: adv_split_v, src/2d/adv_split_v.F90:63-136
+__device__ __forceinline__ double flux_v(const double* f, const double* V, const double* DV,
+                                         const uint8_t* mflux, long c, long NX, int scheme,
+                                         double dti, double dy, double AH) {
+  if (!mflux[c]) return 0.0;
+  double fu, fuu = 0.0, fd = 0.0, cfl = 0.0;
+  bool use_limiter = false;
+  double v = V[c];
+  if (v > 0.0) {
+    fu = f[c];
+    if (scheme != UPSTREAM) use_limiter = mflux[c - NX];
+    if (use_limiter) {
+      cfl = v / DV[c] * dti / dy;
+      fuu = f[c - NX];
+      fd = f[c + NX];
+    }
+  } else {
+    fu = f[c + NX];
+    if (scheme != UPSTREAM) use_limiter = mflux[c + NX];
+    if (use_limiter) {
+      cfl = -v / DV[c] * dti / dy;
+      fuu = f[c + 2 * NX];
+      fd = f[c];
+    }
+  }
+  if (use_limiter) fu = reconstruct<double>(scheme, cfl, fuu, fu, fd);
+  double fl = v * fu;
+  if (AH > 0.0) fl = fl - AH * DV[c] * (f[c + NX] - f[c]) / dy;
+  return fl;
+}
+
+__global__ void k_split_v(Dims d, int nslice, const double* __restrict__ f,
+                          const double* fi_src, double* fi_dst, double* Di, double* adv,
+                          const double* __restrict__ V, const double* __restrict__ DV,
+                          const uint8_t* __restrict__ mflux, const uint8_t* __restrict__ mupd,
+                          double dt, double splitfac, int scheme, double AH, double dx,
+                          double dy, double ard1) {
+  int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  int jj = blockIdx.y * blockDim.y + threadIdx.y;
+  int s = blockIdx.z;
+  if (ii >= d.NX || jj >= d.NY || s >= nslice) return;
+  long o = (long)s * d.NXY;
+  long c2 = ii + (long)d.NX * jj;
+  long c = o + c2;
+  const int jsub = (scheme == UPSTREAM) ? 0 : 1;
+  bool upd = jj >= 1 + jsub && jj <= d.NY - 2 - jsub && mupd[c2];
+  if (upd) {
+    const double dti = splitfac * dt;
+    const long NX = d.NX;
+    double fc = flux_v(f + o, V + o, DV + o, mflux, c2, NX, scheme, dti, dy, AH);
+    double fs = flux_v(f + o, V + o, DV + o, mflux, c2 - NX, NX, scheme, dti, dy, AH);
+    double Dio = Di[c];
+    double Dn = Dio - dti * (V[c] * dx - V[c - NX] * dx) * ard1;
+    Di[c] = Dn;
+    double advn = splitfac * (fc * dx - fs * dx) * ard1;
+    fi_dst[c] = (Dio * fi_src[c] - dt * advn) / Dn;
+    adv[c] = adv[c] + advn;
+  } else if (fi_dst != fi_src) {
+    fi_dst[c] = fi_src[c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// a4: adv_split_w, src/3d/adv_split_w.F90:85-353, itersmax <= 1.
+// One thread per water column; the flux at level k only needs the original
+// f at k-1..k+2 and the original hi at k, k+1, and level k is updated right
+// after its top flux is known, so one upward sweep does the whole column.
+// f is never written (fi_dst != f is guaranteed by the host).
+__global__ void k_split_w(Dims d, const double* __restrict__ f, const double* fi_src,
+                          double* fi_dst, double* hi, double* adv,
+                          const double* __restrict__ ww, const int32_t* __restrict__ az,
+                          double dt, double splitfac, int scheme, int kshift) {
+  int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  int jj = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ii >= d.NX || jj >= d.NY) return;
+  const long NXY = d.NXY;
+  const int kmax = d.kmax;
+  long c0 = ii + (long)d.NX * jj;
+  bool col = ii <= d.NX - 2 && jj <= d.NY - 2 && az[c0] == 1;
+  const bool copy = fi_dst != fi_src;
+  if (!col) {
+    if (copy)
+      for (int k = 0; k <= kmax; k++) fi_dst[c0 + NXY * k] = fi_src[c0 + NXY * k];
+    return;
+  }
+  if (copy) {
+    fi_dst[c0] = fi_src[c0];
+    if (kshift) fi_dst[c0 + NXY * kmax] = fi_src[c0 + NXY * kmax];
+  }
+  const double dti = splitfac * dt;
+#define AT(a, k) (a)[c0 + NXY * (long)(k)]
+  auto wflux_at = [&](int k) -> double {
+    double w = AT(ww, k);
+    double fu, fuu = 0.0, fd = 0.0;
+    bool use_limiter = false;
+    if (w > 0.0) {
+      fu = AT(f, k);
+      if (scheme != UPSTREAM) use_limiter = k > 1 - kshift;
+      if (use_limiter) {
+        fuu = AT(f, k - 1);
+        fd = AT(f, k + 1);
+      }
+    } else {
+      fu = AT(f, k + 1);
+      if (scheme != UPSTREAM) use_limiter = k < kmax - 1;
+      if (use_limiter) {
+        fuu = AT(f, k + 2);
+        fd = AT(f, k);
+      }
+    }
+    if (use_limiter) {
+      double cfl = fabs(w) * dti / (0.5 * (AT(hi, k) + AT(hi, k + 1)));
+      fu = reconstruct<double>(scheme, cfl * 1.0, fuu, fu, fd);
+    }
+    return w * fu;
+  };
+  double wprev = kshift ? wflux_at(0) : 0.0;
+  double wwprev = AT(ww, 0);
+  for (int k = 1; k <= kmax - kshift; k++) {
+    double wk = (k <= kmax - 1) ? wflux_at(k) : 0.0;
+    double wwk = AT(ww, k);
+    double hio = AT(hi, k);
+    double hn = hio - dti * (wwk - wwprev);
+    AT(hi, k) = hn;
+    double advn = splitfac * (wk - wprev);
+    AT(fi_dst, k) = (hio * AT(fi_src, k) - dt * advn) / hn;
+    AT(adv, k) = AT(adv, k) + advn;
+    wprev = wk;
+    wwprev = wwk;
+  }
+#undef AT
+}
+
+// a4 with sub-cycling (itersmax > 1): the reference's column algorithm
+// literally, with its column work arrays in a scratch field laid out
+// [slot][k][column] so that neighbouring threads stay coalesced.
+// Scratch slots: 0 fo, 1 fiaux, 2 hiaux, 3 advaux, 4 cfl0, 5 cfls, 6 wflux.
+__global__ void k_split_w_iter(Dims d, const double* __restrict__ f, const double* fi_src,
+                               double* fi_dst, double* hi, double* adv,
+                               const double* __restrict__ ww, const int32_t* __restrict__ az,
+                               double* scratch, unsigned long long* nwarn, double dt,
+                               double splitfac, int scheme, int kshift, int itersmax) {
+  int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  int jj = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ii >= d.NX || jj >= d.NY) return;
+  const long NXY = d.NXY;
+  const int kmax = d.kmax;
+  const long N3 = NXY * (kmax + 1);
+  long c0 = ii + (long)d.NX * jj;
+  bool col = ii <= d.NX - 2 && jj <= d.NY - 2 && az[c0] == 1;
+  const bool copy = fi_dst != fi_src;
+  if (!col) {
+    if (copy)
+      for (int k = 0; k <= kmax; k++) fi_dst[c0 + NXY * k] = fi_src[c0 + NXY * k];
+    return;
+  }
+#define AT(a, k) (a)[c0 + NXY * (long)(k)]
+#define SC(slot, k) scratch[(slot) * N3 + c0 + NXY * (long)(k)]
+  const double dti = splitfac * dt;
+  for (int k = 1 - kshift; k <= kmax - 1; k++)
+    SC(4, k) = fabs(AT(ww, k)) * dti / (0.5 * (AT(hi, k) + AT(hi, k + 1)));
+  int iters = 1;
+  double itersm1 = 1.0, dtik = dti, splitfack = splitfac;
+  for (int k = 0; k <= kmax; k++) SC(0, k) = AT(f, k);
+  SC(6, 0) = 0.0;
+  SC(6, kmax) = 0.0;
+  int fa = 0;  // faux slot: 0 = fo, 1 = fiaux
+  int it = 1;
+  // every restart strictly raises iters (bounded by itersmax), so the loop
+  // runs at most itersmax*(itersmax+1)/2 sweeps
+  while (it <= iters) {
+    if (it == 1) {
+      for (int k = 0; k <= kmax; k++) {
+        SC(1, k) = AT(fi_src, k);
+        SC(2, k) = AT(hi, k);
+        SC(3, k) = AT(adv, k);
+      }
+      fa = 0;
+    }
+    if (it == 1) {
+      for (int k = 0; k <= kmax; k++) SC(5, k) = SC(4, k);
+    } else if (scheme != UPSTREAM || iters < itersmax) {
+      for (int k = 1 - kshift; k <= kmax - 1; k++)
+        SC(5, k) = fabs(AT(ww, k)) * dti / (0.5 * (SC(2, k) + SC(2, k + 1)));
+    }
+    if (iters < itersmax) {
+      int mx = INT_MIN;
+      for (int k = 1 - kshift; k <= kmax - 1; k++) {
+        double cc = ceil(SC(5, k));
+        int ci = cc > 2147483647.0 ? 2147483647 : (int)cc;
+        mx = ci > mx ? ci : mx;
+      }
+      int iters_new = mx > 1 ? mx : 1;
+      if (iters_new > iters) {
+        if (iters_new > itersmax) {
+          atomicAdd(nwarn, 1ull);
+          iters = itersmax;
+        } else {
+          iters = iters_new;
+        }
+        itersm1 = 1.0 / iters;
+        dtik = dti * itersm1;
+        splitfack = splitfac * itersm1;
+        if (it > 1) {
+          it = 1;
+          continue;
+        }
+      }
+    }
+    // fluxes from faux (slot 6 holds wflux(0:kmax), ends preset to 0),
+    // then the update of the aux column in place
+    for (int k = 1 - kshift; k <= kmax - 1; k++) {
+      double w = AT(ww, k);
+      double fu, fuu = 0.0, fd = 0.0;
+      bool use_limiter = false;
+      if (w > 0.0) {
+        fu = SC(fa, k);
+        if (scheme != UPSTREAM) use_limiter = k > 1 - kshift;
+        if (use_limiter) {
+          fuu = SC(fa, k - 1);
+          fd = SC(fa, k + 1);
+        }
+      } else {
+        fu = SC(fa, k + 1);
+        if (scheme != UPSTREAM) use_limiter = k < kmax - 1;
+        if (use_limiter) {
+          fuu = SC(fa, k + 2);
+          fd = SC(fa, k);
+        }
+      }
+      if (use_limiter) fu = reconstruct<double>(scheme, SC(5, k) * itersm1, fuu, fu, fd);
+      SC(6, k) = w * fu;
+    }
+    for (int k = 1; k <= kmax - kshift; k++) {
+      double hio = SC(2, k);
+      SC(2, k) = hio - dtik * (AT(ww, k) - AT(ww, k - 1));
+      double advn = splitfack * (SC(6, k) - SC(6, k - 1));
+      SC(1, k) = (hio * SC(1, k) - dt * advn) / SC(2, k);
+      SC(3, k) = SC(3, k) + advn;
+    }
+    fa = 1;
+    it++;
+  }
+  if (copy) fi_dst[c0] = fi_src[c0];
+  for (int k = 1; k <= kmax - kshift; k++) {
+    AT(fi_dst, k) = SC(1, k);
+    AT(hi, k) = SC(2, k);
+    AT(adv, k) = SC(3, k);
+  }
+  if (kshift) AT(fi_dst, kmax) = AT(fi_src, kmax);
+#undef SC
+#undef AT
+}
+
+// NOSPLIT finalisation, advection_3d.F90:416-423 (and advection.F90:450-458
+// per slice): f = (ho*f - dt*adv)/hi on mask_finalise, k = 1..kmax.
+__global__ void k_finalise(Dims d, int k0, int nk, double* f, const double* __restrict__ ho,
+                           const double* __restrict__ adv, const double* __restrict__ hi,
+                           const uint8_t* __restrict__ mfin, double dt) {
+  int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  int jj = blockIdx.y * blockDim.y + threadIdx.y;
+  int s = blockIdx.z;
+  if (ii >= d.NX || jj >= d.NY || s >= nk) return;
+  long c2 = ii + (long)d.NX * jj;
+  if (!mfin[c2]) return;
+  long c = c2 + d.NXY * (long)(k0 + s);
+  f[c] = (ho[c] * f[c] - dt * adv[c]) / hi[c];
+}
+
+// ---------------------------------------------------------------------------
+// a7: adv_upstream_2dh, src/2d/adv_upstream_2dh.F90:70-160.  Interior only.
+__device__ __forceinline__ double uflux_2dh(const double* f, const double* U, const double* Dn,
+                                            const int32_t* az, long c, long c2, double AH,
+                                            double dx) {
+  double u = U[c];
+  double fl = u > 0.0 ? u * f[c] : u * f[c + 1];
+  if (AH > 0.0 && az[c2] > 0 && az[c2 + 1] > 0)
+    fl = fl - AH * (f[c + 1] - f[c]) / dx * 0.5 * (Dn[c + 1] + Dn[c]);
+  return fl;
+}
+__device__ __forceinline__ double vflux_2dh(const double* f, const double* V, const double* Dn,
+                                            const int32_t* az, long c, long c2, long NX,
+                                            double AH, double dy) {
+  double v = V[c];
+  double fl = v > 0.0 ? v * f[c] : v * f[c + NX];
+  if (AH > 0.0 && az[c2] > 0 && az[c2 + NX] > 0)
+    fl = fl - AH * (f[c + NX] - f[c]) / dy * 0.5 * (Dn[c + NX] + Dn[c]);
+  return fl;
+}
+
+__global__ void k_upstream_2dh(Dims d, int nslice, const double* __restrict__ f,
+                               const double* fi_src, double* fi_dst, double* Di, double* adv,
+                               const double* __restrict__ U, const double* __restrict__ V,
+                               const double* __restrict__ Dn, const int32_t* __restrict__ az,
+                               double dt, double AH, double dx, double dy, double ard1) {
+  int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  int jj = blockIdx.y * blockDim.y + threadIdx.y;
+  int s = blockIdx.z;
+  if (ii >= d.NX || jj >= d.NY || s >= nslice) return;
+  const long NX = d.NX;
+  long o = (long)s * d.NXY;
+  long c2 = ii + NX * jj;
+  long c = o + c2;
+  bool upd = ii >= 2 && ii <= d.NX - 3 && jj >= 2 && jj <= d.NY - 3 && az[c2] == 1;
+  if (!upd) {
+    if (fi_dst != fi_src) fi_dst[c] = fi_src[c];
+    return;
+  }
+  const double* fs = f + o;
+  double mn = 100000 * 1.0, mx = -100000 * 1.0;
+  for (int q = -1; q <= 1; q++)
+    if (az[c2 + q] == 1) {
+      double v = fs[c2 + q];
+      if (v > mx) mx = v;
+      if (v < mn) mn = v;
+    }
+  for (int q = -1; q <= 1; q += 2)
+    if (az[c2 + q * NX] == 1) {
+      double v = fs[c2 + q * NX];
+      if (v > mx) mx = v;
+      if (v < mn) mn = v;
+    }
+  double ue = uflux_2dh(fs, U + o, Dn + o, az, c2, c2, AH, dx);
+  double uw = uflux_2dh(fs, U + o, Dn + o, az, c2 - 1, c2 - 1, AH, dx);
+  double vn = vflux_2dh(fs, V + o, Dn + o, az, c2, c2, NX, AH, dy);
+  double vs = vflux_2dh(fs, V + o, Dn + o, az, c2 - NX, c2 - NX, NX, AH, dy);
+  double Dio = Di[c];
+  double Dnew = Dio - dt * (U[c] * dy - U[c - 1] * dy + V[c] * dx - V[c - NX] * dx) * ard1;
+  Di[c] = Dnew;
+  double advn = (ue * dy - uw * dy + vn * dx - vs * dx) * ard1;
+  double fv = (Dio * fi_src[c] - dt * advn) / Dnew;
+  adv[c] = adv[c] + advn;
+  if (fv > mx) fv = mx;
+  if (fv < mn) fv = mn;
+  fi_dst[c] = fv;
+}
+
+// ---------------------------------------------------------------------------
+// a12: linear EOS and buoyancy, src/3d/eqstate.F90:127-149,252-261; all
+// cells (land sentinels included), k = 1..kmax.
+__global__ void k_eos_linear(long n, long off, const double* __restrict__ T,
+                             const double* __restrict__ S, double* rho, double* buoy, double T0,
+                             double S0, double dtr0, double dsr0) {
+  long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  long c = off + q;
+  const double g = (double)9.81f, rho_0 = (double)1025.f;
+  const double x = -g / rho_0;
+  double r = rho_0 + dtr0 * (T[c] - T0) + dsr0 * (S[c] - S0);
+  if (rho) rho[c] = r;
+  buoy[c] = x * (r - rho_0);
+}
+
+// ---------------------------------------------------------------------------
+// a9: ip_blumberg_mellor, src/3d/ip_blumberg_mellor.F90:67-150.
+// zz: interface heights over [imin..imax+1]x[jmin..jmax+1] where az>=1.
+__global__ void k_bm_zz(Dims d, const int32_t* __restrict__ az, const double* __restrict__ H,
+                        const double* __restrict__ hn, double* zz) {
+  int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  int jj = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ii >= d.NX || jj >= d.NY) return;
+  long c0 = ii + (long)d.NX * jj;
+  zz[c0] = 0.0;
+  if (ii < 2 || ii > d.NX - 2 || jj < 2 || jj > d.NY - 2 || az[c0] < 1) return;
+  double z = -H[c0] + hn[c0 + d.NXY];
+  zz[c0 + d.NXY] = z;
+  for (int k = 2; k <= d.kmax; k++) {
+    z = z + hn[c0 + d.NXY * k];
+    zz[c0 + d.NXY * k] = z;
+  }
+}
+
+__global__ void k_bm_idpd(Dims d, const int32_t* __restrict__ au, const int32_t* __restrict__ av,
+                          const double* __restrict__ buoy, const double* __restrict__ zz,
+                          const double* __restrict__ hun, const double* __restrict__ hvn,
+                          double* idpdx, double* idpdy, double dxm1, double dym1) {
+  int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  int jj = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ii >= d.NX || jj >= d.NY) return;
+  const long NXY = d.NXY;
+  const int kmax = d.kmax;
+  long c0 = ii + (long)d.NX * jj;
+  idpdx[c0] = 0.0;
+  idpdy[c0] = 0.0;
+  if (ii < 2 || ii > d.NX - 3 || jj < 2 || jj > d.NY - 3) return;
+#define AT(a, q, k) (a)[(q) + NXY * (long)(k)]
+  if (au[c0] >= 1) {
+    const long e = c0 + 1;
+    double grdl = (AT(buoy, e, kmax) - AT(buoy, c0, kmax)) * dxm1;
+    double buoyl = 0.5 * (AT(buoy, e, kmax) + AT(buoy, c0, kmax));
+    double hk = AT(hun, c0, kmax);
+    double prgr = grdl * 0.5 * hk;
+    AT(idpdx, c0, kmax) = hk * prgr;
+    for (int k = kmax - 1; k >= 1; k--) {
+      double grdu = grdl;
+      grdl = (AT(buoy, e, k) - AT(buoy, c0, k)) * dxm1;
+      double buoyu = buoyl;
+      buoyl = 0.5 * (AT(buoy, e, k) + AT(buoy, c0, k));
+      double dxz = (AT(zz, e, k) - AT(zz, c0, k)) * dxm1;
+      double hkk = AT(hun, c0, k);
+      prgr = prgr + 0.5 * (grdu + grdl) * 0.5 * (hkk + hk) - dxz * (buoyu - buoyl);
+      AT(idpdx, c0, k) = hkk * prgr;
+      hk = hkk;
+    }
+  }
+  if (av[c0] >= 1) {
+    const long n = c0 + d.NX;
+    double grdl = (AT(buoy, n, kmax) - AT(buoy, c0, kmax)) * dym1;
+    double buoyl = 0.5 * (AT(buoy, n, kmax) + AT(buoy, c0, kmax));
+    double hk = AT(hvn, c0, kmax);
+    double prgr = grdl * 0.5 * hk;
+    AT(idpdy, c0, kmax) = hk * prgr;
+    for (int k = kmax - 1; k >= 1; k--) {
+      double grdu = grdl;
+      grdl = (AT(buoy, n, k) - AT(buoy, c0, k)) * dym1;
+      double buoyu = buoyl;
+      buoyl = 0.5 * (AT(buoy, n, k) + AT(buoy, c0, k));
+      double dyz = (AT(zz, n, k) - AT(zz, c0, k)) * dym1;
+      double hkk = AT(hvn, c0, k);
+      prgr = prgr + 0.5 * (grdu + grdl) * 0.5 * (hkk + hk) - dyz * (buoyu - buoyl);
+      AT(idpdy, c0, k) = hkk * prgr;
+      hk = hkk;
+    }
+  }
+#undef AT
+}
+
+// ---------------------------------------------------------------------------
+// a10: ip_shchepetkin_mcwilliams, src/3d/ip_shchepetkin_mcwilliams.F90:71-288
+// The reference's eps, OneFifth and 1./12. are single-precision literals.
+#define SM_EPS ((double)1.e-10f)
+#define SM_ONEFIFTH ((double)0.2f)
+#define SM_C12 ((double)(1.0f / 12.0f))
+
+// zz: layer centres over the full halo range where az>=1 (:90-99)
+__global__ void k_sm_zz(Dims d, const int32_t* __restrict__ az, const double* __restrict__ H,
+                        const double* __restrict__ hn, double* zz) {
+  int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  int jj = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ii >= d.NX || jj >= d.NY) return;
+  long c0 = ii + (long)d.NX * jj;
+  zz[c0] = 0.0;
+  if (az[c0] < 1) return;
+  double z = -H[c0] + 0.5 * hn[c0 + d.NXY];
+  zz[c0 + d.NXY] = z;
+  for (int k = 2; k <= d.kmax; k++) {
+    z = z + 0.5 * (hn[c0 + d.NXY * (k - 1)] + hn[c0 + d.NXY * k]);
+    zz[c0 + d.NXY * k] = z;
+  }
+}
+
+// vertical harmonic means and the pressure integral P (:108-146), one
+// thread per column over [imin..imax+1]x[jmin..jmax+1] where az>=1
+__device__ __forceinline__ double sm_hmR(double a, double b) {
+  double cff = 2.0 * a * b;
+  return cff > SM_EPS ? cff / (a + b) : 0.0;
+}
+__global__ void k_sm_P(Dims d, const int32_t* __restrict__ az, const double* __restrict__ buoy,
+                       const double* __restrict__ zz, const double* __restrict__ hn,
+                       double* P) {
+  int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  int jj = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ii >= d.NX || jj >= d.NY) return;
+  long c0 = ii + (long)d.NX * jj;
+  if (ii < 2 || ii > d.NX - 2 || jj < 2 || jj > d.NY - 2 || az[c0] < 1) return;
+  const long NXY = d.NXY;
+  const int kmax = d.kmax;
+#define AT(a, k) (a)[c0 + NXY * (long)(k)]
+  // raw differences: rawR(k) = b(k+1)-b(k) for k in 1..kmax-1,
+  // rawR(kmax) = rawR(kmax-1), rawR(0) = rawR(1)
+  auto rawR = [&](int k) -> double {
+    int kk = k < 1 ? 1 : (k > kmax - 1 ? kmax - 1 : k);
+    return AT(buoy, kk + 1) - AT(buoy, kk);
+  };
+  auto rawZ = [&](int k) -> double {
+    int kk = k < 1 ? 1 : (k > kmax - 1 ? kmax - 1 : k);
+    return AT(zz, kk + 1) - AT(zz, kk);
+  };
+  auto hmZ = [&](double a, double b) -> double { return 2.0 * a * b / (a + b); };
+  double cff1 = 1.0 / (AT(zz, kmax) - AT(zz, kmax - 1));
+  double cff2 = 0.5 * (AT(buoy, kmax) - AT(buoy, kmax - 1)) * 0.5 * AT(hn, kmax) * cff1;
+  double p = (AT(buoy, kmax) + cff2) * 0.5 * AT(hn, kmax);
+  AT(P, kmax) = p;
+  double rk1 = rawR(kmax), zk1 = rawZ(kmax);      // raw at k+1
+  double rk = rawR(kmax - 1), zk = rawZ(kmax - 1);  // raw at k
+  double hR1 = sm_hmR(rk1, rk), hZ1 = hmZ(zk1, zk);  // harmonic at kmax
+  for (int k = kmax - 1; k >= 1; k--) {
+    double rkm = rawR(k - 1), zkm = rawZ(k - 1);
+    double hR = sm_hmR(rk, rkm), hZ = hmZ(zk, zkm);
+    double b1 = AT(buoy, k + 1), b0 = AT(buoy, k);
+    double z1 = AT(zz, k + 1), z0 = AT(zz, k);
+    p = p + 0.5 * ((b1 + b0) * (z1 - z0) -
+                   SM_ONEFIFTH * ((hR1 - hR) * (z1 - z0 - SM_C12 * (hZ1 + hZ)) -
+                                  (hZ1 - hZ) * (b1 - b0 - SM_C12 * (hR1 + hR))));
+    AT(P, k) = p;
+    hR1 = hR;
+    hZ1 = hZ;
+    rk = rkm;
+    zk = zkm;
+  }
+#undef AT
+}
+
+// horizontal harmonic mean with the eps test (:171-188)
+__device__ __forceinline__ double sm_hmx(double a, double b) {
+  double cff = 2.0 * a * b;
+  if (cff > SM_EPS) {
+    double cff1 = 1.0 / (a + b);
+    return cff * cff1;
+  }
+  return 0.0;
+}
+
+// idpdx (:153-211) and idpdy (:214-287), one thread per cell
+__global__ void k_sm_idpd(Dims d, const int32_t* __restrict__ au, const int32_t* __restrict__ av,
+                          const double* __restrict__ buoy, const double* __restrict__ zz,
+                          const double* __restrict__ P, const double* __restrict__ hun,
+                          const double* __restrict__ hvn, const double* __restrict__ sseo,
+                          double* idpdx, double* idpdy, double dxm1, double dym1) {
+  int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  int jj = blockIdx.y * blockDim.y + threadIdx.y;
+  int k = blockIdx.z;
+  if (ii >= d.NX || jj >= d.NY || k > d.kmax) return;
+  const long NXY = d.NXY;
+  const long NX = d.NX;
+  long c0 = ii + NX * jj;
+  if (k == 0) {
+    idpdx[c0] = 0.0;
+    idpdy[c0] = 0.0;
+    return;
+  }
+  if (ii < 2 || ii > d.NX - 3 || jj < 2 || jj > d.NY - 3) return;
+  const long o = NXY * (long)k, okm = NXY * (long)d.kmax;
+  const double* b = buoy + o;
+  const double* z = zz + o;
+  if (au[c0] >= 1) {
+    long c = c0;
+    double zrm = z[c] - z[c - 1], zr0 = z[c + 1] - z[c], zrp = z[c + 2] - z[c + 1];
+    double brm = b[c] - b[c - 1], br0 = b[c + 1] - b[c], brp = b[c + 2] - b[c + 1];
+    double dZx0 = sm_hmx(zrm, zr0), dZx1 = sm_hmx(zr0, zrp);
+    double dRx0 = sm_hmx(brm, br0), dRx1 = sm_hmx(br0, brp);
+    double AJ = P[o + c + 1] - P[o + c];
+    double FC = 0.5 * ((b[c + 1] + b[c]) * (z[c + 1] - z[c]) -
+                       SM_ONEFIFTH * ((dRx1 - dRx0) * (z[c + 1] - z[c] - SM_C12 * (dZx1 + dZx0)) -
+                                      (dZx1 - dZx0) * (b[c + 1] - b[c] - SM_C12 * (dRx1 + dRx0))));
+    AJ = AJ + FC;
+    double h = hun[o + c];
+    idpdx[o + c] = AJ * dxm1 * h - h * dxm1 * (sseo[c + 1] - sseo[c]) * 0.5 *
+                                       (buoy[okm + c + 1] + buoy[okm + c]);
+  }
+  if (av[c0] >= 1) {
+    long c = c0;
+    double zrm = z[c] - z[c - NX], zr0 = z[c + NX] - z[c], zrp = z[c + 2 * NX] - z[c + NX];
+    double brm = b[c] - b[c - NX], br0 = b[c + NX] - b[c], brp = b[c + 2 * NX] - b[c + NX];
+    double dZx0 = sm_hmx(zrm, zr0), dZx1 = sm_hmx(zr0, zrp);
+    double dRx0 = sm_hmx(brm, br0), dRx1 = sm_hmx(br0, brp);
+    double AJ = P[o + c + NX] - P[o + c];
+    double FC = 0.5 * ((b[c + NX] + b[c]) * (z[c + NX] - z[c]) -
+                       SM_ONEFIFTH * ((dRx1 - dRx0) * (z[c + NX] - z[c] - SM_C12 * (dZx1 + dZx0)) -
+                                      (dZx1 - dZx0) * (b[c + NX] - b[c] - SM_C12 * (dRx1 + dRx0))));
+    AJ = AJ + FC;
+    double h = hvn[o + c];
+    idpdy[o + c] = AJ * dym1 * h - h * dym1 * (sseo[c + NX] - sseo[c]) * 0.5 *
+                                       (buoy[okm + c + NX] + buoy[okm + c]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// a13: serial ghost fill, src/futils/halo_zones.F90:141-187 (mirror=.true.):
+// the four 1-wide edge assignments applied in order, each reading its whole
+// right-hand side first.  aliased == (f1 == f2) decides what the corners see.
+__global__ void k_mirror_3d(Dims d, int nk, double* f1, const double* f2, int aliased) {
+  int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  int jj = blockIdx.y * blockDim.y + threadIdx.y;
+  int k = blockIdx.z;
+  if (ii >= d.NX || jj >= d.NY || k >= nk) return;
+  const int il = 2, ih = d.NX - 3, jl = 2, jh = d.NY - 3;
+  const long NX = d.NX;
+  long o = d.NXY * (long)k;
+  // value of f2 after the first two (column) statements, as seen by the row
+  // statements when f1 and f2 are one array
+  auto col_val = [&](int i, int j) -> double {
+    if (aliased) {
+      if (i == il - 1) return f2[o + il + NX * j];
+      if (i == ih + 1) return f2[o + ih + NX * j];
+    }
+    return f2[o + i + NX * j];
+  };
+  double v;
+  if (jj == jl - 1) v = col_val(ii, jl);
+  else if (jj == jh + 1) v = col_val(ii, jh);
+  else if (ii == il - 1) v = f2[o + il + NX * jj];
+  else if (ii == ih + 1) v = f2[o + ih + NX * jj];
+  else return;
+  f1[o + ii + NX * jj] = v;
+}
+
+// serial update_2d_halo with mirror (halo_zones.F90:102-120): edges then the
+// four corners from the diagonal interior cell
+__global__ void k_mirror_2d(Dims d, double* f1, const double* f2, int aliased) {
+  int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  int jj = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ii >= d.NX || jj >= d.NY) return;
+  const int il = 2, ih = d.NX - 3, jl = 2, jh = d.NY - 3;
+  const long NX = d.NX;
+  double v;
+  bool ci = ii == il - 1 || ii == ih + 1, cj = jj == jl - 1 || jj == jh + 1;
+  if (ci && cj) {
+    v = f2[(ii == il - 1 ? il : ih) + NX * (jj == jl - 1 ? jl : jh)];
+  } else if (cj) {
+    int j = jj == jl - 1 ? jl : jh;
+    if (aliased && (ii == il - 1 || ii == ih + 1)) v = f2[(ii == il - 1 ? il : ih) + NX * j];
+    else v = f2[ii + NX * j];
+  } else if (ci) {
+    v = f2[(ii == il - 1 ? il : ih) + NX * jj];
+  } else {
+    return;
+  }
+  f1[ii + NX * jj] = v;
+}
+
+// TWOD exchange between the tiles of one decomposed domain inside one HBM
+// (the message pattern of halo_mpi.F90:1095-1154): every ghost cell that lies
+// in a neighbour tile's interior takes that value.  One launch covers all
+// tiles: blockIdx.z = k, and a flat thread index walks the ghost ring of
+// every tile (ring offsets in ring_off[]).
+__global__ void k_tile_exchange(int ntiles, int px, const TileDesc* __restrict__ tiles,
+                                const long* __restrict__ ring_off, long ring_total, int nk) {
+  long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  int k = blockIdx.y;
+  if (q >= ring_total || k >= nk) return;
+  // find the tile (few tiles: linear scan)
+  int t = 0;
+  while (t + 1 < ntiles && ring_off[t + 1] <= q) t++;
+  const TileDesc& T = tiles[t];
+  long r = q - ring_off[t];
+  long NX = T.ni + 2 * GT_HALO, NY = T.nj + 2 * GT_HALO;
+  // ring enumeration: rows jj in [0,2) and [nj+2, nj+4) full width, then
+  // columns ii in [0,2) and [ni+2, ni+4) for the interior rows
+  long ii, jj;
+  long rows = 2 * GT_HALO * NX;
+  if (r < rows) {
+    long rr = r / NX;
+    ii = r % NX;
+    jj = rr < GT_HALO ? rr : T.nj + GT_HALO + (rr - GT_HALO);
+  } else {
+    long rc = r - rows;
+    long cc = rc % (2 * GT_HALO);
+    jj = GT_HALO + rc / (2 * GT_HALO);
+    ii = cc < GT_HALO ? cc : T.ni + GT_HALO + (cc - GT_HALO);
+  }
+  int ox = ii < GT_HALO ? -1 : (ii >= T.ni + GT_HALO ? 1 : 0);
+  int oy = jj < GT_HALO ? -1 : (jj >= T.nj + GT_HALO ? 1 : 0);
+  int tx = t % px, ty = t / px;
+  int sx = tx + ox, sy = ty + oy;
+  int py = ntiles / px;
+  if (sx < 0 || sx >= px || sy < 0 || sy >= py) return;
+  const TileDesc& S = tiles[sy * px + sx];
+  long SNX = S.ni + 2 * GT_HALO, SNY = S.nj + 2 * GT_HALO;
+  long si = ox < 0 ? ii + S.ni : (ox > 0 ? ii - T.ni : ii);
+  long sj = oy < 0 ? jj + S.nj : (oy > 0 ? jj - T.nj : jj);
+  T.f[ii + NX * (jj + NY * (long)k)] = S.f[si + SNX * (sj + SNY * (long)k)];
+}
+
+// ---------------------------------------------------------------------------
+// a15: budgets.  Error-free accumulation: each thread keeps a double-double
+// (hi, lo) sum of h*f (exact products via fma) and of h; waves and blocks
+// combine double-doubles; one partial per block, then one block finishes.
+__device__ __forceinline__ void dd_add(double& sh, double& sl, double a) {
+  double s = sh + a;
+  double bv = s - sh;
+  double e = (sh - (s - bv)) + (a - bv);
+  e += sl;
+  double r = s + e;
+  sl = e - (r - s);
+  sh = r;
+}
+__device__ __forceinline__ void dd_add_dd(double& sh, double& sl, double ah, double al) {
+  dd_add(sh, sl, ah);
+  dd_add(sh, sl, al);
+}
+
+__global__ void k_budget_partial(Dims d, const int32_t* __restrict__ az,
+                                 const double* __restrict__ h, const double* __restrict__ f,
+                                 double* partial) {
+  const long nint = (long)(d.NX - 4) * (d.NY - 4) * d.kmax;
+  double mh = 0, ml = 0, vh = 0, vl = 0;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < nint;
+       q += (long)gridDim.x * blockDim.x) {
+    long ni = d.NX - 4, nj = d.NY - 4;
+    long i = q % ni, r = q / ni;
+    long j = r % nj, k = 1 + r / nj;
+    long c2 = (i + 2) + (long)d.NX * (j + 2);
+    if (az[c2] != 1) continue;
+    long c = c2 + d.NXY * k;
+    double hv = h[c];
+    double fv = f ? f[c] : 1.0;
+    double p = hv * fv;
+    double pe = fma(hv, fv, -p);
+    dd_add(mh, ml, p);
+    dd_add(mh, ml, pe);
+    dd_add(vh, vl, hv);
+  }
+  __shared__ double sm[4][256];
+  int t = threadIdx.x;
+  sm[0][t] = mh; sm[1][t] = ml; sm[2][t] = vh; sm[3][t] = vl;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      double a = sm[0][t], b = sm[1][t], c = sm[2][t], e = sm[3][t];
+      dd_add_dd(a, b, sm[0][t + s], sm[1][t + s]);
+      dd_add_dd(c, e, sm[2][t + s], sm[3][t + s]);
+      sm[0][t] = a; sm[1][t] = b; sm[2][t] = c; sm[3][t] = e;
+    }
+    __syncthreads();
+  }
+  if (t == 0)
+    for (int q = 0; q < 4; q++) partial[4 * blockIdx.x + q] = sm[q][0];
+}
+
+__global__ void k_budget_final(int nblocks, const double* __restrict__ partial, double area,
+                               double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double mh = 0, ml = 0, vh = 0, vl = 0;
+  for (int b = 0; b < nblocks; b++) {
+    dd_add_dd(mh, ml, partial[4 * b], partial[4 * b + 1]);
+    dd_add_dd(vh, vl, partial[4 * b + 2], partial[4 * b + 3]);
+  }
+  double p = mh * area, pe = fma(mh, area, -p);
+  pe += ml * area;
+  out[0] = p + pe;
+  p = vh * area;
+  pe = fma(vh, area, -p);
+  pe += vl * area;
+  out[1] = p + pe;
+}
+
+}  // namespace gt
+
+// ---------------------------------------------------------------------------
+// host launchers
+namespace gt {
+
+static inline dim3 grid2(const Dims& d, int nz, dim3 b) {
+  return dim3((d.NX + b.x - 1) / b.x, (d.NY + b.y - 1) / b.y, nz);
+}
+static const dim3 kBlk2(64, 4, 1);
+
+hipError_t launch_split_u(hipStream_t st, Dims d, int nslice, const double* f,
+                          const double* fi_src, double* fi_dst, double* Di, double* adv,
+                          const double* U, const double* DU, const uint8_t* mflux,
+                          const uint8_t* mupd, double dt, double splitfac, int scheme,
+                          double AH, double dx, double dy, double ard1) {
+  hipLaunchKernelGGL(k_split_u, grid2(d, nslice, kBlk2), kBlk2, 0, st, d, nslice, f, fi_src,
+                     fi_dst, Di, adv, U, DU, mflux, mupd, dt, splitfac, scheme, AH, dx, dy,
+                     ard1);
+  return hipGetLastError();
+}
+
+hipError_t launch_split_v(hipStream_t st, Dims d, int nslice, const double* f,
+                          const double* fi_src, double* fi_dst, double* Di, double* adv,
+                          const double* V, const double* DV, const uint8_t* mflux,
+                          const uint8_t* mupd, double dt, double splitfac, int scheme,
+                          double AH, double dx, double dy, double ard1) {
+  hipLaunchKernelGGL(k_split_v, grid2(d, nslice, kBlk2), kBlk2, 0, st, d, nslice, f, fi_src,
+                     fi_dst, Di, adv, V, DV, mflux, mupd, dt, splitfac, scheme, AH, dx, dy,
+                     ard1);
+  return hipGetLastError();
+}
+
+hipError_t launch_split_w(hipStream_t st, Dims d, const double* f, const double* fi_src,
+                          double* fi_dst, double* hi, double* adv, const double* ww,
+                          const int32_t* az, double dt, double splitfac, int scheme,
+                          int kshift) {
+  hipLaunchKernelGGL(k_split_w, grid2(d, 1, kBlk2), kBlk2, 0, st, d, f, fi_src, fi_dst, hi,
+                     adv, ww, az, dt, splitfac, scheme, kshift);
+  return hipGetLastError();
+}
+
+hipError_t launch_split_w_iter(hipStream_t st, Dims d, const double* f, const double* fi_src,
+                               double* fi_dst, double* hi, double* adv, const double* ww,
+                               const int32_t* az, double* scratch7, unsigned long long* nwarn,
+                               double dt, double splitfac, int scheme, int kshift,
+                               int itersmax) {
+  hipLaunchKernelGGL(k_split_w_iter, grid2(d, 1, kBlk2), kBlk2, 0, st, d, f, fi_src, fi_dst,
+                     hi, adv, ww, az, scratch7, nwarn, dt, splitfac, scheme, kshift, itersmax);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalise(hipStream_t st, Dims d, int k0, int nk, double* f, const double* ho,
+                           const double* adv, const double* hi, const uint8_t* mfin,
+                           double dt) {
+  hipLaunchKernelGGL(k_finalise, grid2(d, nk, kBlk2), kBlk2, 0, st, d, k0, nk, f, ho, adv, hi,
+                     mfin, dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_upstream_2dh(hipStream_t st, Dims d, int nslice, const double* f,
+                               const double* fi_src, double* fi_dst, double* Di, double* adv,
+                               const double* U, const double* V, const double* Dn,
+                               const int32_t* az, double dt, double AH, double dx, double dy,
+                               double ard1) {
+  hipLaunchKernelGGL(k_upstream_2dh, grid2(d, nslice, kBlk2), kBlk2, 0, st, d, nslice, f,
+                     fi_src, fi_dst, Di, adv, U, V, Dn, az, dt, AH, dx, dy, ard1);
+  return hipGetLastError();
+}
+
+hipError_t launch_eos_linear(hipStream_t st, Dims d, const double* T, const double* S,
+                             double* rho, double* buoy, double T0, double S0, double dtr0,
+                             double dsr0) {
+  long n = d.NXY * (long)d.kmax;
+  hipLaunchKernelGGL(k_eos_linear, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n,
+                     d.NXY, T, S, rho, buoy, T0, S0, dtr0, dsr0);
+  return hipGetLastError();
+}
+
+hipError_t launch_ip_bm(hipStream_t st, Dims d, const int32_t* az, const int32_t* au,
+                        const int32_t* av, const double* H, const double* hn,
+                        const double* hun, const double* hvn, const double* buoy, double* zz,
+                        double* idpdx, double* idpdy, double dx, double dy) {
+  const double dxm1 = 1.0 / dx, dym1 = 1.0 / dy;
+  hipLaunchKernelGGL(k_bm_zz, grid2(d, 1, kBlk2), kBlk2, 0, st, d, az, H, hn, zz);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bm_idpd, grid2(d, 1, kBlk2), kBlk2, 0, st, d, au, av, buoy, zz, hun, hvn,
+                     idpdx, idpdy, dxm1, dym1);
+  return hipGetLastError();
+}
+
+hipError_t launch_ip_smcw(hipStream_t st, Dims d, const int32_t* az, const int32_t* au,
+                          const int32_t* av, const double* H, const double* hn,
+                          const double* hun, const double* hvn, const double* buoy,
+                          const double* sseo, double* zz, double* P, double* idpdx,
+                          double* idpdy, double dx, double dy) {
+  const double dxm1 = 1.0 / dx, dym1 = 1.0 / dy;
+  hipLaunchKernelGGL(k_sm_zz, grid2(d, 1, kBlk2), kBlk2, 0, st, d, az, H, hn, zz);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_sm_P, grid2(d, 1, kBlk2), kBlk2, 0, st, d, az, buoy, zz, hn, P);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_sm_idpd, grid2(d, d.kmax + 1, kBlk2), kBlk2, 0, st, d, au, av, buoy, zz,
+                     P, hun, hvn, sseo, idpdx, idpdy, dxm1, dym1);
+  return hipGetLastError();
+}
+
+hipError_t launch_mirror_3d(hipStream_t st, Dims d, double* f1, const double* f2) {
+  hipLaunchKernelGGL(k_mirror_3d, grid2(d, d.kmax + 1, kBlk2), kBlk2, 0, st, d, d.kmax + 1, f1,
+                     f2, (int)(f1 == f2));
+  return hipGetLastError();
+}
+
+hipError_t launch_mirror_2d(hipStream_t st, Dims d, double* f1, const double* f2) {
+  hipLaunchKernelGGL(k_mirror_2d, grid2(d, 1, kBlk2), kBlk2, 0, st, d, f1, f2,
+                     (int)(f1 == f2));
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_exchange(hipStream_t st, int ntiles, int px, const TileDesc* dtiles,
+                                const long* dring_off, long ring_total, int nk) {
+  if (ring_total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tile_exchange, dim3((unsigned)((ring_total + 255) / 256), nk), dim3(256),
+                     0, st, ntiles, px, dtiles, dring_off, ring_total, nk);
+  return hipGetLastError();
+}
+
+hipError_t launch_budget(hipStream_t st, Dims d, const int32_t* az, const double* h,
+                         const double* f, double* partial, int nblocks, double area,
+                         double* out) {
+  hipLaunchKernelGGL(k_budget_partial, dim3(nblocks), dim3(256), 0, st, d, az, h, f, partial);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_budget_final, dim3(1), dim3(64), 0, st, nblocks, partial, area, out);
+  return hipGetLastError();
+}
+
+}  // namespace gt

@@ -1,0 +1,262 @@
+"""Python binding of the C-ABI in include/getm_transport.h.
+
+This is the host-side mirror used by tests and the benchmark: a thin ctypes
+layer over ``libgetm_transport.so`` whose methods keep the argument lists of
+the reference routines (getm-model/code) they replace.  Device memory and
+streams come from PyTorch; every call goes to the HIP library, and loading
+fails loudly if the library has not been built.
+
+Fields are torch tensors of shape (kmax+1, NY, NX) (3-D) or (NY, NX) (one
+slice), C-contiguous, so memory order equals the reference's Fortran order
+f(imin-2:imax+2, jmin-2:jmax+2, 0:kmax).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from . import build as _build
+
+# ids (advection.F90:55-73, advection_3d.F90:42, halo_zones.F90:24-28)
+NOADV, UPSTREAM, UPSTREAM_2DH, P2, SUPERBEE, MUSCL, P2_PDM, J7, FCT, P2_2DH = range(10)
+NOSPLIT, FULLSPLIT, HALFSPLIT, HVSPLIT = range(4)
+H_TAG, D_TAG, U_TAG, V_TAG, W_TAG = 10, 20, 31, 32, 33
+BLUMBERG_MELLOR, SHCHEPETKIN_MCWILLIAMS = 1, 6
+
+GT_ERRORS = {1: "EINVAL_TAG", 2: "EINVAL_SPLIT", 3: "EINVAL_SCHEME",
+             4: "ESCHEME_SPLIT", 5: "ENOMEM", 6: "EHIP", 7: "EARG",
+             8: "ENOTIMPL"}
+
+# every symbol include/getm_transport.h declares
+EXPORTS = [
+    "gt_last_error", "gt_version", "gt_create", "gt_destroy", "gt_set_stream",
+    "gt_synchronize", "gt_set_adv_ver_iterations", "gt_warnings",
+    "gt_adv_interfacial_reconstruction", "gt_adv_split_u", "gt_adv_split_v",
+    "gt_adv_split_w", "gt_adv_upstream_2dh", "gt_do_advection",
+    "gt_do_advection_3d", "gt_eos_linear", "gt_ip_blumberg_mellor",
+    "gt_ip_shchepetkin_mcwilliams", "gt_do_internal_pressure",
+    "gt_update_3d_halo", "gt_update_2d_halo", "gt_wait_halo",
+    "gt_exchange_tiles", "gt_budget", "gt_malloc", "gt_free",
+    "gt_memcpy_h2d", "gt_memcpy_d2h",
+]
+
+
+class GtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{GT_ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def library_path() -> Path:
+    return _build.BUILD / _build.LIB_NAME
+
+
+def load_library():
+    """Load libgetm_transport.so (built in-tree by code_amd/build.py)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = library_path()
+    if not p.exists():
+        raise ImportError(f"{p} is missing: build it with `python code_amd/build.py` "
+                          "(the transport step has no CPU fallback)")
+    lib = C.CDLL(str(p))
+    vp, d, i, l = C.c_void_p, C.c_double, C.c_int, C.c_long
+    sig = {
+        "gt_last_error": (C.c_char_p, []),
+        "gt_version": (i, []),
+        "gt_create": (i, [C.POINTER(vp), i, i, i, i, i, d, d, vp, vp, vp, vp]),
+        "gt_destroy": (i, [vp]),
+        "gt_set_stream": (i, [vp, vp]),
+        "gt_synchronize": (i, [vp]),
+        "gt_set_adv_ver_iterations": (i, [vp, i]),
+        "gt_warnings": (C.c_longlong, [vp]),
+        "gt_adv_interfacial_reconstruction": (i, [vp, i, l, vp, vp, vp, vp, vp]),
+        "gt_adv_split_u": (i, [vp, d, vp, vp, vp, vp, vp, vp, d, i, d, i]),
+        "gt_adv_split_v": (i, [vp, d, vp, vp, vp, vp, vp, vp, d, i, d, i]),
+        "gt_adv_split_w": (i, [vp, d, vp, vp, vp, vp, vp, d, i, i, i]),
+        "gt_adv_upstream_2dh": (i, [vp, d, vp, vp, vp, vp, vp, vp, vp, vp, vp, d, i]),
+        "gt_do_advection": (i, [vp, d, vp, vp, vp, vp, vp, vp, vp, i, i, d, i, vp, vp]),
+        "gt_do_advection_3d": (i, [vp, d, vp, vp, vp, vp, vp, vp, vp, vp, i, i, i, d, i,
+                                   vp, vp]),
+        "gt_eos_linear": (i, [vp, vp, vp, vp, vp, d, d, d, d]),
+        "gt_ip_blumberg_mellor": (i, [vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "gt_ip_shchepetkin_mcwilliams": (i, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "gt_do_internal_pressure": (i, [vp, i, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "gt_update_3d_halo": (i, [vp, vp, vp, i, i]),
+        "gt_update_2d_halo": (i, [vp, vp, vp, i, i]),
+        "gt_wait_halo": (i, [vp, i]),
+        "gt_exchange_tiles": (i, [i, i, vp, vp, i, vp, vp]),
+        "gt_budget": (i, [vp, vp, vp, C.POINTER(d), C.POINTER(d)]),
+        "gt_malloc": (i, [C.POINTER(vp), C.c_size_t]),
+        "gt_free": (i, [vp]),
+        "gt_memcpy_h2d": (i, [vp, vp, C.c_size_t]),
+        "gt_memcpy_d2h": (i, [vp, vp, C.c_size_t]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(t):
+    """device pointer of a tensor (or None)"""
+    if t is None:
+        return None
+    return C.c_void_p(t.data_ptr())
+
+
+def check(rc):
+    if rc != 0:
+        raise GtError(rc, load_library().gt_last_error().decode())
+
+
+class Context:
+    """One domain on the device (gt_create).
+
+    ``az``, ``au``, ``av``, ``ax`` are (NY, NX) int32 numpy arrays; ``au``,
+    ``av``, ``ax`` may be None (derived from az as domain.F90 does).
+    """
+
+    def __init__(self, imax, jmax, kmax, dx, dy, az, au=None, av=None, ax=None,
+                 imin=1, jmin=1, stream=None):
+        lib = load_library()
+        self.lib = lib
+        self.imin, self.imax, self.jmin, self.jmax, self.kmax = imin, imax, jmin, jmax, kmax
+        self.NX = imax - imin + 5
+        self.NY = jmax - jmin + 5
+        self.dx, self.dy = float(dx), float(dy)
+
+        def arr(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.int32)
+            assert a.shape == (self.NY, self.NX), (a.shape, (self.NY, self.NX))
+            return a
+        self._masks = [arr(m) for m in (az, au, av, ax)]
+        ptrs = [None if m is None else m.ctypes.data_as(C.c_void_p) for m in self._masks]
+        h = C.c_void_p()
+        check(lib.gt_create(C.byref(h), imin, imax, jmin, jmax, kmax, self.dx, self.dy,
+                            *ptrs))
+        self.h = h
+        if stream is not None:
+            self.set_stream(stream)
+
+    @classmethod
+    def from_domain(cls, dom, **kw):
+        return cls(dom.imax, dom.jmax, dom.kmax, dom.dx, dom.dy, dom.az, dom.au, dom.av,
+                   dom.ax, imin=dom.imin, jmin=dom.jmin, **kw)
+
+    def close(self):
+        if getattr(self, "h", None) is not None:
+            self.lib.gt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream):
+        """stream: a torch.cuda.Stream (or raw handle int)"""
+        handle = getattr(stream, "cuda_stream", stream)
+        check(self.lib.gt_set_stream(self.h, C.c_void_p(handle)))
+
+    def synchronize(self):
+        check(self.lib.gt_synchronize(self.h))
+
+    def set_adv_ver_iterations(self, n):
+        check(self.lib.gt_set_adv_ver_iterations(self.h, int(n)))
+
+    def warnings(self):
+        return int(self.lib.gt_warnings(self.h))
+
+    # ---- advection --------------------------------------------------------
+    def adv_interfacial_reconstruction(self, scheme, cfl, fuu, fu, fd, out):
+        check(self.lib.gt_adv_interfacial_reconstruction(
+            self.h, scheme, cfl.numel(), _ptr(cfl), _ptr(fuu), _ptr(fu), _ptr(fd), _ptr(out)))
+
+    def adv_split_u(self, dt, f, fi, Di, adv, U, DU, splitfac, scheme, AH, tag=H_TAG):
+        check(self.lib.gt_adv_split_u(self.h, dt, _ptr(f), _ptr(fi), _ptr(Di), _ptr(adv),
+                                      _ptr(U), _ptr(DU), splitfac, scheme, AH, tag))
+
+    def adv_split_v(self, dt, f, fi, Di, adv, V, DV, splitfac, scheme, AH, tag=H_TAG):
+        check(self.lib.gt_adv_split_v(self.h, dt, _ptr(f), _ptr(fi), _ptr(Di), _ptr(adv),
+                                      _ptr(V), _ptr(DV), splitfac, scheme, AH, tag))
+
+    def adv_split_w(self, dt, f, fi, hi, adv, ww, splitfac, scheme, tag, itersmax):
+        check(self.lib.gt_adv_split_w(self.h, dt, _ptr(f), _ptr(fi), _ptr(hi), _ptr(adv),
+                                      _ptr(ww), splitfac, scheme, tag, itersmax))
+
+    def adv_upstream_2dh(self, dt, f, fi, Di, adv, U, V, Dn, DU, DV, AH, tag=H_TAG):
+        check(self.lib.gt_adv_upstream_2dh(self.h, dt, _ptr(f), _ptr(fi), _ptr(Di),
+                                           _ptr(adv), _ptr(U), _ptr(V), _ptr(Dn), _ptr(DU),
+                                           _ptr(DV), AH, tag))
+
+    def do_advection(self, dt, f, U, V, DU, DV, Do, Dn, split, scheme, AH, tag,
+                     Dires=None, advres=None):
+        check(self.lib.gt_do_advection(self.h, dt, _ptr(f), _ptr(U), _ptr(V), _ptr(DU),
+                                       _ptr(DV), _ptr(Do), _ptr(Dn), split, scheme, AH, tag,
+                                       _ptr(Dires), _ptr(advres)))
+
+    def do_advection_3d(self, dt, f, uu, vv, ww, hu, hv, ho, hn, split, hscheme, vscheme,
+                        AH, tag, hires=None, advres=None):
+        check(self.lib.gt_do_advection_3d(self.h, dt, _ptr(f), _ptr(uu), _ptr(vv), _ptr(ww),
+                                          _ptr(hu), _ptr(hv), _ptr(ho), _ptr(hn), split,
+                                          hscheme, vscheme, AH, tag, _ptr(hires),
+                                          _ptr(advres)))
+
+    # ---- internal pressure --------------------------------------------------
+    def eos_linear(self, T, S, rho, buoy, T0, S0, dtr0, dsr0):
+        check(self.lib.gt_eos_linear(self.h, _ptr(T), _ptr(S), _ptr(rho), _ptr(buoy),
+                                     T0, S0, dtr0, dsr0))
+
+    def ip_blumberg_mellor(self, H, hn, hun, hvn, buoy, zz, idpdx, idpdy):
+        check(self.lib.gt_ip_blumberg_mellor(self.h, _ptr(H), _ptr(hn), _ptr(hun), _ptr(hvn),
+                                             _ptr(buoy), _ptr(zz), _ptr(idpdx), _ptr(idpdy)))
+
+    def ip_shchepetkin_mcwilliams(self, H, hn, hun, hvn, buoy, sseo, zz, idpdx, idpdy):
+        check(self.lib.gt_ip_shchepetkin_mcwilliams(
+            self.h, _ptr(H), _ptr(hn), _ptr(hun), _ptr(hvn), _ptr(buoy), _ptr(sseo), _ptr(zz),
+            _ptr(idpdx), _ptr(idpdy)))
+
+    def do_internal_pressure(self, ip_method, H, hn, hun, hvn, buoy, sseo, idpdx, idpdy):
+        check(self.lib.gt_do_internal_pressure(
+            self.h, ip_method, _ptr(H), _ptr(hn), _ptr(hun), _ptr(hvn), _ptr(buoy), _ptr(sseo),
+            _ptr(idpdx), _ptr(idpdy)))
+
+    # ---- ghost cells -----------------------------------------------------------
+    def update_3d_halo(self, f1, f2, tag=H_TAG, mirror=-1):
+        check(self.lib.gt_update_3d_halo(self.h, _ptr(f1), _ptr(f2), tag, mirror))
+
+    def update_2d_halo(self, f1, f2, tag=H_TAG, mirror=-1):
+        check(self.lib.gt_update_2d_halo(self.h, _ptr(f1), _ptr(f2), tag, mirror))
+
+    def wait_halo(self, tag=H_TAG):
+        check(self.lib.gt_wait_halo(self.h, tag))
+
+    # ---- budgets ---------------------------------------------------------------
+    def budget(self, h, f=None):
+        m, v = C.c_double(), C.c_double()
+        check(self.lib.gt_budget(self.h, _ptr(h), _ptr(f), C.byref(m), C.byref(v)))
+        return m.value, v.value
+
+
+def exchange_tiles(px, py, ni, nj, nk, tiles, stream=None):
+    """gt_exchange_tiles on a list of device tensors"""
+    lib = load_library()
+    n = px * py
+    ni_a = (C.c_int * n)(*ni)
+    nj_a = (C.c_int * n)(*nj)
+    tp = (C.c_void_p * n)(*[t.data_ptr() for t in tiles])
+    handle = getattr(stream, "cuda_stream", stream)
+    check(lib.gt_exchange_tiles(px, py, ni_a, nj_a, nk, tp,
+                                C.c_void_p(handle) if handle else None))

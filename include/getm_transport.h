@@ -1,0 +1,186 @@
+/*
+ * getm_transport.h -- C-ABI of the MI355X transport step (libgetm_transport.so).
+ *
+ * Drop-in boundary for GETM's 3-D baroclinic transport step
+ * (getm-model/code).  Each entry point replaces one Fortran routine and keeps
+ * its argument list and meaning; the reference routine is cited beside it.
+ * A Fortran caller binds these through ISO_C_BINDING (the module in
+ * code_amd/fortran/getm_transport_gpu.F90 keeps the reference's Fortran
+ * signatures on top of them).
+ *
+ * Conventions
+ *  - Fields use the reference layout f(imin-2:imax+2, jmin-2:jmax+2, 0:kmax),
+ *    column-major, i fastest, FP64 (include/cppdefs.h:116-159).  A "slice"
+ *    argument is one E2DFIELD level f(:,:,k).
+ *  - Field pointers are DEVICE pointers (hipMalloc'd or gt_malloc'd); masks
+ *    are passed once, as host arrays, to gt_create.
+ *  - Every entry point returns 0 on success or a GT_E* code; the message of
+ *    the last failure on the calling thread is gt_last_error().  The
+ *    reference stops on the same conditions (advection_3d.F90:276,403,541,
+ *    545,564; advection.F90:436,534,...).
+ *  - Work is enqueued on the context's stream (gt_set_stream; default: the
+ *    null stream).  Calls return once enqueued; gt_synchronize waits.
+ *    gt_budget and gt_warnings synchronise because they return host values.
+ *  - One context per domain; a context is not re-entrant (the reference's
+ *    module state is not either) but distinct contexts are independent.
+ */
+#ifndef GETM_TRANSPORT_H
+#define GETM_TRANSPORT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ids: advection.F90:55-73, advection_3d.F90:42, halo_zones.F90:24-28 */
+enum { GT_NOADV = 0, GT_UPSTREAM = 1, GT_UPSTREAM_2DH = 2, GT_P2 = 3,
+       GT_SUPERBEE = 4, GT_MUSCL = 5, GT_P2_PDM = 6, GT_J7 = 7, GT_FCT = 8,
+       GT_P2_2DH = 9 };
+enum { GT_NOSPLIT = 0, GT_FULLSPLIT = 1, GT_HALFSPLIT = 2, GT_HVSPLIT = 3 };
+enum { GT_H_TAG = 10, GT_D_TAG = 20, GT_U_TAG = 31, GT_V_TAG = 32,
+       GT_W_TAG = 33 };
+/* internal_pressure.F90:94-100 */
+enum { GT_IP_BLUMBERG_MELLOR = 1, GT_IP_SHCHEPETKIN_MCWILLIAMS = 6 };
+
+enum {
+  GT_OK = 0,
+  GT_EINVAL_TAG = 1,      /* 'do_advection_3d: tag is invalid' */
+  GT_EINVAL_SPLIT = 2,    /* 'split is invalid' */
+  GT_EINVAL_SCHEME = 3,   /* 'hscheme is invalid' / 'scheme is invalid' */
+  GT_ESCHEME_SPLIT = 4,   /* 'hscheme not valid for split' */
+  GT_ENOMEM = 5,
+  GT_EHIP = 6,            /* a HIP runtime error (message has the detail) */
+  GT_EARG = 7,            /* bad size, NULL pointer or context */
+  GT_ENOTIMPL = 8         /* scheme outside this build (J7, FCT, P2_2DH) */
+};
+
+typedef struct gt_ctx gt_ctx;
+
+const char *gt_last_error(void);
+int gt_version(void);
+
+/* ---- context: init_domain masks + init_advection + init_internal_pressure
+ * Replaces the set-up of src/2d/advection.F90:204-354 (t_adv_grid masks),
+ * src/3d/internal_pressure.F90:117-176 (zz allocation) on one domain.
+ * az is required; au/av/ax may be NULL and are then derived from az as
+ * src/domain/domain.F90:294-346 does.  Host arrays of E2DFIELD size. */
+int gt_create(gt_ctx **ctx, int imin, int imax, int jmin, int jmax, int kmax,
+              double dx, double dy, const int32_t *az, const int32_t *au,
+              const int32_t *av, const int32_t *ax);
+int gt_destroy(gt_ctx *ctx);
+int gt_set_stream(gt_ctx *ctx, void *hip_stream);
+int gt_synchronize(gt_ctx *ctx);
+/* module variable adv_ver_iterations, advection_3d.F90:41 (default 1) */
+int gt_set_adv_ver_iterations(gt_ctx *ctx, int iters);
+/* columns that asked for more than itersmax vertical iterations since the
+ * last call (the reference prints 'too many iterations needed',
+ * adv_split_w.F90:219-231); synchronises */
+long long gt_warnings(gt_ctx *ctx);
+
+/* ---- advection --------------------------------------------------------- */
+/* adv_interfacial_reconstruction, advection.F90:785-879, evaluated on n
+ * device values (a check entry; the kernels inline the same function) */
+int gt_adv_interfacial_reconstruction(gt_ctx *ctx, int scheme, long n,
+                                      const double *cfl, const double *fuu,
+                                      const double *fu, const double *fd,
+                                      double *out);
+/* adv_split_u, src/2d/adv_split_u.F90:8-295 (one slice; the mask pair of
+ * the reference is chosen by tag: H/D/W -> adv_gridH, U -> adv_gridU,
+ * V -> adv_gridV).  f and fi may be the same array. */
+int gt_adv_split_u(gt_ctx *ctx, double dt, const double *f, double *fi,
+                   double *Di, double *adv, const double *U, const double *DU,
+                   double splitfac, int scheme, double AH, int tag);
+/* adv_split_v, src/2d/adv_split_v.F90:8-145 */
+int gt_adv_split_v(gt_ctx *ctx, double dt, const double *f, double *fi,
+                   double *Di, double *adv, const double *V, const double *DV,
+                   double splitfac, int scheme, double AH, int tag);
+/* adv_split_w, src/3d/adv_split_w.F90:7-360 (full I3DFIELD arrays) */
+int gt_adv_split_w(gt_ctx *ctx, double dt, const double *f, double *fi,
+                   double *hi, double *adv, const double *ww,
+                   double splitfac, int scheme, int tag, int itersmax);
+/* adv_upstream_2dh, src/2d/adv_upstream_2dh.F90:8-169 (one slice) */
+int gt_adv_upstream_2dh(gt_ctx *ctx, double dt, const double *f, double *fi,
+                        double *Di, double *adv, const double *U,
+                        const double *V, const double *Dn, const double *DU,
+                        const double *DV, double AH, int tag);
+/* do_advection, src/2d/advection.F90:362-702 (one slice).  Dires and advres
+ * are the optional outputs (NULL = absent). */
+int gt_do_advection(gt_ctx *ctx, double dt, double *f, const double *U,
+                    const double *V, const double *DU, const double *DV,
+                    const double *Do, const double *Dn, int split, int scheme,
+                    double AH, int tag, double *Dires, double *advres);
+/* do_advection_3d, src/3d/advection_3d.F90:120-598.  hires and advres are
+ * the optional outputs (NULL = absent). */
+int gt_do_advection_3d(gt_ctx *ctx, double dt, double *f, const double *uu,
+                       const double *vv, const double *ww, const double *hu,
+                       const double *hv, const double *ho, const double *hn,
+                       int split, int hscheme, int vscheme, double AH, int tag,
+                       double *hires, double *advres);
+
+/* ---- internal pressure ------------------------------------------------- */
+/* do_eqstate with eqstate_method=1 (linear), src/3d/eqstate.F90:127-149 and
+ * the buoyancy of :252-261.  rho may be NULL.  The reference defaults are
+ * T0=10, S0=33.75, dtr0=-0.17, dsr0=0.78 as FP32 literals. */
+int gt_eos_linear(gt_ctx *ctx, const double *T, const double *S, double *rho,
+                  double *buoy, double T0, double S0, double dtr0,
+                  double dsr0);
+/* ip_blumberg_mellor, src/3d/ip_blumberg_mellor.F90:8-152.  The module
+ * variables it reads are arguments here; zz is the module array
+ * (internal_pressure.F90:66-72), written where the reference writes it. */
+int gt_ip_blumberg_mellor(gt_ctx *ctx, const double *H, const double *hn,
+                          const double *hun, const double *hvn,
+                          const double *buoy, double *zz, double *idpdx,
+                          double *idpdy);
+/* ip_shchepetkin_mcwilliams, src/3d/ip_shchepetkin_mcwilliams.F90:8-291 */
+int gt_ip_shchepetkin_mcwilliams(gt_ctx *ctx, const double *H,
+                                 const double *hn, const double *hun,
+                                 const double *hvn, const double *buoy,
+                                 const double *sseo, double *zz,
+                                 double *idpdx, double *idpdy);
+/* do_internal_pressure, src/3d/internal_pressure.F90:181-274, with the
+ * context's own zz (zero at creation, as init_internal_pressure leaves it).
+ * sseo is only read by method 6. */
+int gt_do_internal_pressure(gt_ctx *ctx, int ip_method, const double *H,
+                            const double *hn, const double *hun,
+                            const double *hvn, const double *buoy,
+                            const double *sseo, double *idpdx, double *idpdy);
+
+/* ---- ghost cells --------------------------------------------------------- */
+/* update_3d_halo, src/futils/halo_zones.F90:141-187, serial semantics:
+ * mirror = 1 copies the 1-wide edges, 0 or -1 (absent) does nothing. */
+int gt_update_3d_halo(gt_ctx *ctx, double *f1, const double *f2, int tag,
+                      int mirror);
+/* update_2d_halo, halo_zones.F90:81-132; mirror = -1 means "absent" and
+ * uses the flag the previous call left (the reference's SAVE'd do_mirror) */
+int gt_update_2d_halo(gt_ctx *ctx, double *f1, const double *f2, int tag,
+                      int mirror);
+/* wait_halo, halo_zones.F90:196-217: waits for the context's stream */
+int gt_wait_halo(gt_ctx *ctx, int tag);
+/* The exchange of halo_mpi.F90:1095-1154 (TWOD, HALO-wide strips and
+ * corners, levels 0..nk-1) between the px*py tiles of one decomposed domain
+ * held in one device memory.  tiles[t] (device pointer, t = ty*px+tx) has
+ * interior ni[t] x nj[t]; tiles in one row share nj, in one column share ni.
+ * Edges without a neighbour are left unchanged (MPI_PROC_NULL).  Synchronous
+ * on the given stream (NULL = null stream). */
+int gt_exchange_tiles(int px, int py, const int *ni, const int *nj, int nk,
+                      double *const *tiles, void *hip_stream);
+
+/* ---- budgets (no reference counterpart) ---------------------------------
+ * mass = sum over interior water cells (az==1), k=1..kmax of h*f*dx*dy and
+ * volume = sum of h*dx*dy, accumulated error-free in double-double.  f may be
+ * NULL (volume only; mass then equals volume).  Synchronises. */
+int gt_budget(gt_ctx *ctx, const double *h, const double *f, double *mass,
+              double *volume);
+
+/* ---- device memory helpers for callers without a HIP binding ---------- */
+int gt_malloc(void **ptr, size_t bytes);
+int gt_free(void *ptr);
+int gt_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int gt_memcpy_d2h(void *dst, const void *src, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,366 @@
+"""FP64 HIP path against the oracle, through the C-ABI.
+
+Tolerance: the north-star bound for the plain FP64 path is 1e-12 relative per
+cell; the kernels keep the reference's operation order and are built without
+FMA contraction, so they are expected (and reported) to agree bit for bit.
+Each test compares full arrays, halos included.
+"""
+import numpy as np
+import pytest
+
+from code_amd import inputs
+from code_amd import transport as gt
+from oracle import pyoracle as po
+from tests.util import rel_err, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+SCHEMES = [gt.UPSTREAM, gt.P2, gt.SUPERBEE, gt.MUSCL, gt.P2_PDM]
+
+
+def random_domain(imax, jmax, kmax, seed, kind):
+    """kind: 'walls' (closed basin), 'open' (water up to and into the halo,
+    so the serial halo bounds of the reference are exercised), 'islands'."""
+    NX, NY = imax + 4, jmax + 4
+    if kind == "walls":
+        return inputs.make_domain(imax, jmax, kmax, seed, walls=3)
+    az = np.ones((NY, NX), np.int32)
+    if kind == "islands":
+        u = inputs.uniform((NY, NX), seed + 17)
+        az[u < 0.15] = 0
+    au, av, ax = inputs.masks_from_az(az)
+    H = inputs.uniform((NY, NX), seed + 5, 10.0, 40.0)
+    HU = H.copy()
+    HV = H.copy()
+    HU[:, :-1] = 0.5 * (H[:, :-1] + H[:, 1:])
+    HV[:-1, :] = 0.5 * (H[:-1, :] + H[1:, :])
+    return inputs.Domain(imax, jmax, kmax, 500.0, 400.0, H, HU, HV, az, au, av, ax)
+
+
+def random_fields(dom, seed, cmax=0.9, wmax=None):
+    """random tracer, layer heights and transports with |CFL| < cmax"""
+    sh = dom.shape3
+    f = inputs.uniform(sh, seed + 1)
+    # a few plateaus so the limiter's deltaf*deltafu<=0 branches are hit
+    f[inputs.uniform(sh, seed + 2) < 0.1] = 0.5
+    h = inputs.uniform(sh, seed + 3, 0.5, 2.0)
+    hu = inputs.uniform(sh, seed + 4, 0.5, 2.0)
+    hv = inputs.uniform(sh, seed + 5, 0.5, 2.0)
+    dt = 30.0
+    uu = inputs.uniform(sh, seed + 6, -1, 1) * hu * cmax * dom.dx / dt
+    vv = inputs.uniform(sh, seed + 7, -1, 1) * hv * cmax * dom.dy / dt
+    wmax = cmax * 0.5 / dt if wmax is None else wmax
+    ww = inputs.uniform(sh, seed + 8, -1, 1) * wmax
+    ww[0] = 0.0
+    ww[-1] = 0.0
+    ho = h.copy()
+    hn = inputs.uniform(sh, seed + 9, 0.5, 2.0)
+    return dict(f=f, h=h, hu=hu, hv=hv, uu=uu, vv=vv, ww=ww, ho=ho, hn=hn, dt=dt)
+
+
+@pytest.fixture(scope="module", params=["walls", "open", "islands"])
+def case(request, device):
+    dom = random_domain(23, 17, 6, 11, request.param)
+    ctx = gt.Context.from_domain(dom)
+    orc = po.Oracle.from_domain(dom)
+    flds = random_fields(dom, 100)
+    return dom, ctx, orc, flds, device
+
+
+def test_reconstruction_bit_exact(device):
+    import torch
+    dom = random_domain(8, 8, 2, 1, "walls")
+    ctx = gt.Context.from_domain(dom)
+    n = 20000
+    cfl = inputs.uniform((n,), 1, 0.0, 1.0)
+    cfl[:50] = [0.0, 1.0, 0.5, 1e-12, 0.999999] * 10
+    fuu = inputs.uniform((n,), 2, -1, 1)
+    fu = inputs.uniform((n,), 3, -1, 1)
+    fd = inputs.uniform((n,), 4, -1, 1)
+    fd[:100] = fu[:100]        # deltaf = 0
+    fuu[100:200] = fu[100:200]  # deltafu = 0
+    for scheme in SCHEMES:
+        out = torch.empty(n, dtype=torch.float64, device=device)
+        ctx.adv_interfacial_reconstruction(scheme, *(to_dev(a, device) for a in (cfl, fuu, fu, fd)),
+                                           out)
+        ref = np.array([po.reconstruct(scheme, *v) for v in zip(cfl, fuu, fu, fd)])
+        got = to_host(out)
+        assert np.array_equal(np.isfinite(got), np.isfinite(ref))
+        fin = np.isfinite(ref)
+        assert rel_err(got[fin], ref[fin]) <= TOL, scheme
+        assert np.array_equal(got[fin], ref[fin]), f"scheme {scheme} not bit-exact"
+
+
+@pytest.mark.parametrize("direction", ["u", "v"])
+def test_split_uv(case, direction):
+    dom, ctx, orc, F, dev = case
+    k = 3
+    for scheme in SCHEMES:
+        for AH in (-1.0, 7.5):
+            for tag in (gt.H_TAG, gt.U_TAG, gt.V_TAG):
+                for alias in (True, False):
+                    f = F["f"][k].copy()
+                    fi = f if alias else F["f"][k - 1].copy()
+                    Di = F["h"][k].copy()
+                    adv = F["hn"][k].copy()
+                    vel = F["uu"][k] if direction == "u" else F["vv"][k]
+                    DV = F["hu"][k] if direction == "u" else F["hv"][k]
+                    tf, tfi, tDi, tadv = (to_dev(a, dev) for a in (f, fi, Di, adv))
+                    if alias:
+                        tfi = tf
+                    tvel, tDV = to_dev(vel, dev), to_dev(DV, dev)
+                    fn = ctx.adv_split_u if direction == "u" else ctx.adv_split_v
+                    fn(F["dt"], tf, tfi, tDi, tadv, tvel, tDV, 0.5, scheme, AH, tag)
+                    on = orc.adv_split_u if direction == "u" else orc.adv_split_v
+                    on(F["dt"], f, fi, Di, adv, vel, DV, 0.5, scheme, AH, tag)
+                    for got, ref, nm in ((tfi, fi, "fi"), (tDi, Di, "Di"), (tadv, adv, "adv")):
+                        e = rel_err(to_host(got), ref)
+                        assert e <= TOL, (direction, scheme, AH, tag, alias, nm, e)
+
+
+@pytest.mark.parametrize("itersmax", [1, 4])
+def test_split_w(case, itersmax):
+    dom, ctx, orc, F, dev = case
+    dt = F["dt"]
+    # vertical CFL up to ~2.5 so that sub-cycling and its restarts happen
+    ww = F["ww"] * (5.0 if itersmax > 1 else 1.0)
+    for scheme in SCHEMES:
+        for tag in (gt.H_TAG, gt.W_TAG):
+            for alias in (True, False):
+                f = F["f"].copy()
+                fi = f if alias else F["hn"].copy()
+                hi = F["h"].copy()
+                adv = F["uu"].copy()
+                tf = to_dev(f, dev)
+                tfi = tf if alias else to_dev(fi, dev)
+                thi, tadv = to_dev(hi, dev), to_dev(adv, dev)
+                ctx.warnings()
+                ctx.adv_split_w(dt, tf, tfi, thi, tadv, to_dev(ww, dev), 1.0, scheme, tag,
+                                itersmax)
+                nw_gpu = ctx.warnings()
+                nw = orc.adv_split_w(dt, f, fi, hi, adv, ww, 1.0, scheme, tag, itersmax)
+                assert nw_gpu == nw
+                for got, ref, nm in ((tfi, fi, "fi"), (thi, hi, "hi"), (tadv, adv, "adv")):
+                    e = rel_err(to_host(got), ref)
+                    assert e <= TOL, (itersmax, scheme, tag, alias, nm, e)
+
+
+def test_upstream_2dh(case):
+    dom, ctx, orc, F, dev = case
+    k = 2
+    for AH in (-1.0, 4.0):
+        for alias in (True, False):
+            f = F["f"][k].copy()
+            fi = f if alias else F["f"][k + 1].copy()
+            Di, adv = F["h"][k].copy(), F["ww"][k].copy()
+            args = [F["uu"][k], F["vv"][k], F["hn"][k], F["hu"][k], F["hv"][k]]
+            tf = to_dev(f, dev)
+            tfi = tf if alias else to_dev(fi, dev)
+            tDi, tadv = to_dev(Di, dev), to_dev(adv, dev)
+            ctx.adv_upstream_2dh(F["dt"], tf, tfi, tDi, tadv, *(to_dev(a, dev) for a in args), AH)
+            orc.adv_upstream_2dh(F["dt"], f, fi, Di, adv, *args, AH)
+            for got, ref in ((tfi, fi), (tDi, Di), (tadv, adv)):
+                assert rel_err(to_host(got), ref) <= TOL
+
+
+def test_do_advection_2d(case):
+    dom, ctx, orc, F, dev = case
+    k = 4
+    for split in (gt.NOSPLIT, gt.FULLSPLIT, gt.HALFSPLIT, gt.HVSPLIT):
+        for scheme in [gt.NOADV, gt.UPSTREAM_2DH] + SCHEMES + [gt.J7]:
+            for outs in (True, False):
+                f = F["f"][k].copy()
+                args = [F["uu"][k], F["vv"][k], F["hu"][k], F["hv"][k], F["ho"][k], F["hn"][k]]
+                Dires = np.zeros_like(f) if outs else None
+                advres = np.zeros_like(f) if outs else None
+                rc = orc.do_advection(F["dt"], f, *args, split, scheme, 2.0, gt.H_TAG, Dires,
+                                      advres)
+                tf = to_dev(F["f"][k], dev)
+                tD = to_dev(np.zeros_like(f), dev) if outs else None
+                tA = to_dev(np.zeros_like(f), dev) if outs else None
+                try:
+                    ctx.do_advection(F["dt"], tf, *(to_dev(a, dev) for a in args), split, scheme,
+                                     2.0, gt.H_TAG, tD, tA)
+                    code = 0
+                except gt.GtError as e:
+                    code = e.code
+                if rc != 0 or code != 0:
+                    assert code != 0 and rc != 0, (split, scheme, rc, code)
+                    continue
+                assert rel_err(to_host(tf), f) <= TOL, (split, scheme)
+                if outs:
+                    assert rel_err(to_host(tD), Dires) <= TOL
+                    assert rel_err(to_host(tA), advres) <= TOL
+
+
+HS = [gt.NOADV, gt.UPSTREAM, gt.UPSTREAM_2DH, gt.P2, gt.SUPERBEE, gt.MUSCL, gt.P2_PDM]
+VS = [gt.NOADV, gt.UPSTREAM, gt.P2, gt.SUPERBEE, gt.MUSCL, gt.P2_PDM]
+
+
+@pytest.mark.parametrize("split", [gt.NOSPLIT, gt.FULLSPLIT, gt.HALFSPLIT, gt.HVSPLIT])
+def test_do_advection_3d_all_schemes(case, split):
+    dom, ctx, orc, F, dev = case
+    names = ["uu", "vv", "ww", "hu", "hv", "ho", "hn"]
+    dargs = [to_dev(F[n], dev) for n in names]
+    for itv in (1, 3):
+        ctx.set_adv_ver_iterations(itv)
+        for hs in HS:
+            for vs in VS:
+                for tag in (gt.H_TAG, gt.W_TAG, gt.U_TAG):
+                    f = F["f"].copy()
+                    hires = np.zeros_like(f)
+                    advres = np.zeros_like(f)
+                    rc, _ = orc.do_advection_3d(F["dt"], f, *(F[n] for n in names), split, hs, vs,
+                                                1.5, tag, itv, hires, advres)
+                    tf = to_dev(F["f"], dev)
+                    th = to_dev(np.zeros_like(f), dev)
+                    ta = to_dev(np.zeros_like(f), dev)
+                    try:
+                        ctx.do_advection_3d(F["dt"], tf, *dargs, split, hs, vs, 1.5, tag, th, ta)
+                        code = 0
+                    except gt.GtError as e:
+                        code = e.code
+                    if rc != 0 or code != 0:
+                        assert rc != 0 and code != 0, (split, hs, vs, tag, rc, code)
+                        continue
+                    for got, ref, nm in ((tf, f, "f"), (th, hires, "hires"), (ta, advres, "advres")):
+                        e = rel_err(to_host(got), ref)
+                        assert e <= TOL, (split, hs, vs, tag, itv, nm, e)
+    ctx.set_adv_ver_iterations(1)
+
+
+def test_config1_trajectory_bit_exact(device):
+    """100x100x20 sigma, HALFSPLIT Superbee, 100 steps."""
+    tr = inputs.config1()
+    d = tr.dom
+    ctx = gt.Context.from_domain(d)
+    orc = po.Oracle.from_domain(d)
+    f = tr.tracers["c"].copy()
+    m0, v0 = orc.budget(tr.hn, f)
+    names = [tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho, tr.hn]
+    dn = [to_dev(a, device) for a in names]
+    tf = to_dev(f, device)
+    for _ in range(100):
+        orc.do_advection_3d(tr.dt, f, *names, gt.HALFSPLIT, gt.SUPERBEE, gt.SUPERBEE, -1.0,
+                            gt.H_TAG)
+        ctx.do_advection_3d(tr.dt, tf, *dn, gt.HALFSPLIT, gt.SUPERBEE, gt.SUPERBEE, -1.0, gt.H_TAG)
+    got = to_host(tf)
+    assert rel_err(got, f) <= TOL
+    assert np.array_equal(got, f), "config-1 trajectory not bit-exact"
+    m1, _ = ctx.budget(to_dev(tr.hn, device), tf)
+    assert abs(m1 - m0) <= 1e-13 * abs(m0)
+
+
+def ip_fields(seed=2, imax=30, jmax=26, kmax=9):
+    tr = inputs.config2(seed=seed, imax=imax, jmax=jmax, kmax=kmax)
+    d = tr.dom
+    T, S = tr.tracers["T"], tr.tracers["S"]
+    sseo = inputs.uniform((d.NY, d.NX), seed + 40, -0.2, 0.2)
+    return tr, d, T, S, sseo
+
+
+def test_eos_and_internal_pressure(device):
+    import torch
+    tr, d, T, S, sseo = ip_fields()
+    ctx = gt.Context.from_domain(d)
+    orc = po.Oracle.from_domain(d)
+    consts = (inputs.EOS_T0, inputs.EOS_S0, inputs.EOS_DTR0, inputs.EOS_DSR0)
+    rho = np.zeros(d.shape3)
+    buoy = np.zeros(d.shape3)
+    orc.eos_linear(T, S, rho, buoy, *consts)
+    trho = torch.zeros(d.shape3, dtype=torch.float64, device=device)
+    tbuoy = torch.zeros_like(trho)
+    ctx.eos_linear(to_dev(T, device), to_dev(S, device), trho, tbuoy, *consts)
+    assert rel_err(to_host(trho), rho) <= TOL
+    assert rel_err(to_host(tbuoy), buoy) <= TOL
+    H = d.H
+    for method in ("bm", "sm"):
+        zz = np.zeros(d.shape3)
+        idx = np.zeros(d.shape3)
+        idy = np.zeros(d.shape3)
+        tz, tx, ty = (torch.zeros_like(trho) for _ in range(3))
+        dargs = [to_dev(a, device) for a in (H, tr.hn, tr.hun, tr.hvn, buoy)]
+        if method == "bm":
+            orc.ip_blumberg_mellor(H, tr.hn, tr.hun, tr.hvn, buoy, zz, idx, idy)
+            ctx.ip_blumberg_mellor(*dargs, tz, tx, ty)
+        else:
+            orc.ip_shchepetkin_mcwilliams(H, tr.hn, tr.hun, tr.hvn, buoy, sseo, zz, idx, idy)
+            ctx.ip_shchepetkin_mcwilliams(*dargs, to_dev(sseo, device), tz, tx, ty)
+        assert np.abs(idx).max() > 0 and np.abs(idy).max() > 0
+        for got, ref, nm in ((tz, zz, "zz"), (tx, idx, "idpdx"), (ty, idy, "idpdy")):
+            e = rel_err(to_host(got), ref)
+            assert e <= TOL, (method, nm, e)
+
+
+def test_halo_fill(case):
+    import torch
+    dom, ctx, orc, F, dev = case
+    for alias in (True, False):
+        f1 = F["f"].copy()
+        f2 = f1 if alias else F["h"].copy()
+        t1 = to_dev(f1, dev)
+        t2 = t1 if alias else to_dev(f2, dev)
+        orc.update_3d_halo(f1, f2, 1)
+        ctx.update_3d_halo(t1, t2, gt.H_TAG, 1)
+        assert np.array_equal(to_host(t1), f1)
+    # 2-D: the SAVE'd mirror flag carries over to calls without `mirror`
+    for alias in (True, False):
+        state = 1
+        for mirror in (1, -1, 0, -1, 1, -1):
+            f1 = F["f"][2].copy()
+            f2 = f1 if alias else F["h"][2].copy()
+            t1 = to_dev(f1, dev)
+            t2 = t1 if alias else to_dev(f2, dev)
+            state = orc.update_2d_halo(f1, f2, mirror, state)
+            ctx.update_2d_halo(t1, t2, gt.H_TAG, mirror)
+            assert np.array_equal(to_host(t1), f1), (mirror, alias)
+
+
+def test_tile_exchange(device):
+    px, py, nk = 3, 2, 4
+    ni, nj = [], []
+    for ty in range(py):
+        for tx in range(px):
+            ni.append([7, 5, 9][tx])
+            nj.append([6, 8][ty])
+    tiles = [inputs.uniform((nk, nj[t] + 4, ni[t] + 4), 300 + t) for t in range(px * py)]
+    ref = [t.copy() for t in tiles]
+    po.exchange_tiles(px, py, ni, nj, nk, ref)
+    dt = [to_dev(t, device) for t in tiles]
+    gt.exchange_tiles(px, py, ni, nj, nk, dt)
+    for t in range(px * py):
+        assert np.array_equal(to_host(dt[t]), ref[t]), t
+
+
+def test_budget(case):
+    dom, ctx, orc, F, dev = case
+    m, v = orc.budget(F["h"], F["f"])
+    gm, gv = ctx.budget(to_dev(F["h"], dev), to_dev(F["f"], dev))
+    assert abs(gm - m) <= 1e-15 * abs(m)
+    assert abs(gv - v) <= 1e-15 * abs(v)
+
+
+def test_config2_full_size_properties(device):
+    """configs[1] sizes (400x400x40 GVC, P2-PDM): one step conserves tracer
+    mass to round-off, keeps a constant field constant, and ends the volume
+    trajectory on hn."""
+    import torch
+    tr = inputs.config2()
+    d = tr.dom
+    ctx = gt.Context.from_domain(d)
+    dn = [to_dev(a, device) for a in (tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho, tr.hn)]
+    tS = to_dev(tr.tracers["S"], device)
+    th = torch.zeros_like(tS)
+    ta = torch.zeros_like(tS)
+    thn = dn[-1]
+    m0, v0 = ctx.budget(thn, tS)
+    ctx.do_advection_3d(tr.dt, tS, *dn, gt.HALFSPLIT, gt.P2_PDM, gt.P2_PDM, -1.0, gt.H_TAG, th, ta)
+    m1, v1 = ctx.budget(th, tS)
+    assert abs(m1 - m0) <= 1e-12 * abs(m0), (m0, m1)
+    wet = d.az == 1
+    hires = to_host(th)
+    assert np.abs(hires[1:, wet] - tr.hn[1:, wet]).max() <= 1e-9
+    c = torch.full_like(tS, 2.5)
+    ctx.do_advection_3d(tr.dt, c, *dn, gt.HALFSPLIT, gt.P2_PDM, gt.P2_PDM, -1.0, gt.H_TAG)
+    cc = to_host(c)
+    assert np.abs(cc[1:, wet] - 2.5).max() <= 1e-12
