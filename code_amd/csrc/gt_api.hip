@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/getm_transport.h"
+#include "gt_batched.h"
 #include "gt_kernels.h"
 
 using gt::Dims;
@@ -57,6 +58,9 @@ struct gt_ctx {
   unsigned long long* nwarn = nullptr;
   int adv_ver_iterations = 1;
   int do_mirror_2d = 1;
+  int fused = 1;        // use the one-pass HALFSPLIT kernel where it applies
+  int tile = 0;         // fused tile shape (0: 32x8, 1: 64x4, 2: 16x8)
+  int group = 3;        // tracers per workgroup of the FP32 batched kernel
 };
 
 extern "C" const char* gt_last_error(void) { return g_err; }
@@ -355,6 +359,26 @@ static int check_2d(int split, int scheme) {
   }
 }
 
+// descriptor of one domain for the fused kernel
+template <typename TF>
+static void fill_domain(gt_ctx* c, const GridMasks* g, gt::FusedDomain<TF>* D, const TF* fin,
+                        TF* fout, long tstride, const double* uu, const double* vv,
+                        const double* ww, const double* hu, const double* hv, const double* ho,
+                        double* hires, double* advres, long tile_begin) {
+  int TX, TY;
+  gt::fused_tile_shape(c->tile, &TX, &TY);
+  D->d = c->d;
+  D->ntx = (c->d.NX + TX - 1) / TX;
+  D->nty = (c->d.NY + TY - 1) / TY;
+  D->tile_begin = tile_begin;
+  D->fin = fin; D->fout = fout; D->tstride = tstride;
+  D->uu = uu; D->vv = vv; D->ww = ww; D->hu = hu; D->hv = hv; D->ho = ho;
+  D->m_uflux = g->uflux; D->m_vflux = g->vflux; D->m_uupd = g->uupd; D->m_vupd = g->vupd;
+  D->az = g->az;
+  D->hires = hires; D->advres = advres;
+  D->dx = c->dx; D->dy = c->dy; D->ard1 = c->ard1;
+}
+
 // ---- advection entry points --------------------------------------------------
 __global__ void k_reconstruct_batch(int scheme, long n, const double* cfl, const double* fuu,
                                     const double* fu, const double* fd, double* out) {
@@ -555,6 +579,17 @@ extern "C" int gt_do_advection_3d(gt_ctx* c, double dt, double* f, const double*
       return GT_OK;
     }
     case GT_HALFSPLIT: {
+      if (c->fused && itv <= 1 && kmax > 1) {
+        gt::FusedArgs<double> A{};
+        fill_domain(c, g, &A.doms[0], f, tmp, 0, uu, vv, ww, hu, hv, ho, hires, advres, 0);
+        A.ndom = 1;
+        A.ntiles = A.doms[0].ntx * (long)A.doms[0].nty;
+        A.ntracers = 1;
+        A.dt = dt; A.AH = AH;
+        A.hscheme = hscheme; A.vscheme = vscheme; A.kshift = tag == GT_W_TAG ? 1 : 0;
+        LCHK(gt::launch_fused_f64(c->st, A, c->tile, advres != nullptr));
+        return copy_d2d(c, f, tmp, N3);
+      }
       PingPong pp{f, f, tmp};
       TRY(step_u(c, g, 1, kmax, pp.cur, pp.cur, pp.oth, phi, padv, uu, hu, dt, 0.5, hscheme, AH));
       pp.swap();
@@ -693,20 +728,100 @@ extern "C" int gt_exchange_tiles(int px, int py, const int* ni, const int* nj, i
   return GT_OK;
 }
 
+// ---- batched multi-tracer step (extension) ----------------------------------
+extern "C" int gt_set_tuning(gt_ctx* c, int fused, int tile, int group) {
+  if (!c) return set_err(GT_EARG, "gt_set_tuning: NULL context");
+  if (tile < 0 || tile > 2) return set_err(GT_EARG, "gt_set_tuning: tile must be 0, 1 or 2");
+  if (group != 1 && group != 2 && group != 3 && group != 6)
+    return set_err(GT_EARG, "gt_set_tuning: group must be 1, 2, 3 or 6");
+  c->fused = fused != 0;
+  c->tile = tile;
+  c->group = group;
+  return GT_OK;
+}
+
+template <typename TF>
+static int batch_impl(int ndom, const gt_batch_domain* doms, int ntracers, double dt,
+                      int hscheme, int vscheme, double AH, int tag, void* stream) {
+  if (ndom < 1 || !doms || ntracers < 1)
+    return set_err(GT_EARG, "gt_adv3d_batch: bad argument");
+  if (!split_scheme(hscheme) || !split_scheme(vscheme))
+    return set_err(GT_EINVAL_SCHEME, "gt_adv3d_batch: schemes must be UPSTREAM..P2_PDM");
+  const GridMasks* gsel = nullptr;
+  for (int i = 0; i < ndom; i++) {
+    gt_ctx* c = doms[i].ctx;
+    if (!c || !doms[i].f_in || !doms[i].f_out || doms[i].f_in == doms[i].f_out)
+      return set_err(GT_EARG, "gt_adv3d_batch: domain %d needs a context and distinct "
+                              "f_in/f_out", i);
+    if (c->kmax < 2) return set_err(GT_EARG, "gt_adv3d_batch: kmax must be >= 2");
+    gsel = grid_for(c, tag);
+    if (!gsel) return set_err(GT_EINVAL_TAG, "gt_adv3d_batch: tag is invalid");
+  }
+  gt_ctx* c0 = doms[0].ctx;
+  hipStream_t st = stream ? (hipStream_t)stream : c0->st;
+  for (int b = 0; b < ndom; b += gt::kMaxFusedDomains) {
+    gt::FusedArgs<TF> A{};
+    long tiles = 0;
+    int nd = ndom - b < gt::kMaxFusedDomains ? ndom - b : gt::kMaxFusedDomains;
+    for (int i = 0; i < nd; i++) {
+      const gt_batch_domain& bd = doms[b + i];
+      gt_ctx* c = bd.ctx;
+      c->tile = c0->tile;
+      fill_domain<TF>(c, grid_for(c, tag), &A.doms[i], (const TF*)bd.f_in, (TF*)bd.f_out,
+                      c->N3, bd.uu, bd.vv, bd.ww, bd.hu, bd.hv, bd.ho, bd.hires, nullptr, tiles);
+      tiles += A.doms[i].ntx * (long)A.doms[i].nty;
+    }
+    A.ndom = nd;
+    A.ntiles = tiles;
+    A.ntracers = ntracers;
+    A.dt = dt; A.AH = AH;
+    A.hscheme = hscheme; A.vscheme = vscheme; A.kshift = tag == GT_W_TAG ? 1 : 0;
+    if constexpr (sizeof(TF) == 4) {
+      LCHK(gt::launch_fused_f32(st, A, c0->tile, c0->group));
+    } else {
+      LCHK(gt::launch_fused_f64(st, A, c0->tile, false));
+    }
+  }
+  return GT_OK;
+}
+
+extern "C" int gt_adv3d_batch_f32(int ndom, const gt_batch_domain* doms, int ntracers,
+                                  double dt, int hscheme, int vscheme, double AH, int tag,
+                                  void* stream) {
+  return batch_impl<float>(ndom, doms, ntracers, dt, hscheme, vscheme, AH, tag, stream);
+}
+
+extern "C" int gt_adv3d_batch_f64(int ndom, const gt_batch_domain* doms, int ntracers,
+                                  double dt, int hscheme, int vscheme, double AH, int tag,
+                                  void* stream) {
+  return batch_impl<double>(ndom, doms, ntracers, dt, hscheme, vscheme, AH, tag, stream);
+}
+
 // ---- budgets -----------------------------------------------------------------------
-extern "C" int gt_budget(gt_ctx* c, const double* h, const double* f, double* mass,
-                         double* volume) {
+template <typename TF>
+static int budget_impl(gt_ctx* c, const double* h, const TF* f, double* mass, double* volume) {
   if (!c || !h) return set_err(GT_EARG, "gt_budget: NULL argument");
   const int nblocks = 1024;
   ENSURE(c->bpart, 4 * nblocks);
   ENSURE(c->bout, 2);
-  LCHK(gt::launch_budget(c->st, c->d, c->az, h, f, c->bpart, nblocks, c->dx * c->dy, c->bout));
+  LCHK(gt::launch_budget<TF>(c->st, c->d, c->az, h, f, c->bpart, nblocks, c->dx * c->dy,
+                             c->bout));
   double hb[2];
   HIPCHK(hipMemcpyAsync(hb, c->bout, sizeof(hb), hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   if (mass) *mass = hb[0];
   if (volume) *volume = hb[1];
   return GT_OK;
+}
+
+extern "C" int gt_budget(gt_ctx* c, const double* h, const double* f, double* mass,
+                         double* volume) {
+  return budget_impl<double>(c, h, f, mass, volume);
+}
+
+extern "C" int gt_budget_f32(gt_ctx* c, const double* h, const float* f, double* mass,
+                             double* volume) {
+  return budget_impl<float>(c, h, f, mass, volume);
 }
 
 // ---- memory helpers -----------------------------------------------------------------

@@ -55,8 +55,8 @@ hipError_t launch_mirror_2d(hipStream_t st, Dims d, double* f1, const double* f2
 hipError_t launch_tile_exchange(hipStream_t st, int ntiles, int px, const TileDesc* dtiles,
                                 const long* dring_off, long ring_total, int nk);
 // partial must hold 4*nblocks doubles; out two doubles (mass, volume)
+template <typename TF>
 hipError_t launch_budget(hipStream_t st, Dims d, const int32_t* az, const double* h,
-                         const double* f, double* partial, int nblocks, double area,
-                         double* out);
+                         const TF* f, double* partial, int nblocks, double area, double* out);
 
 }  // namespace gt

@@ -756,8 +756,9 @@ __device__ __forceinline__ void dd_add_dd(double& sh, double& sl, double ah, dou
   dd_add(sh, sl, al);
 }
 
+template <typename TF>
 __global__ void k_budget_partial(Dims d, const int32_t* __restrict__ az,
-                                 const double* __restrict__ h, const double* __restrict__ f,
+                                 const double* __restrict__ h, const TF* __restrict__ f,
                                  double* partial) {
   const long nint = (long)(d.NX - 4) * (d.NY - 4) * d.kmax;
   double mh = 0, ml = 0, vh = 0, vl = 0;
@@ -770,7 +771,7 @@ __global__ void k_budget_partial(Dims d, const int32_t* __restrict__ az,
     if (az[c2] != 1) continue;
     long c = c2 + d.NXY * k;
     double hv = h[c];
-    double fv = f ? f[c] : 1.0;
+    double fv = f ? (double)f[c] : 1.0;
     double p = hv * fv;
     double pe = fma(hv, fv, -p);
     dd_add(mh, ml, p);
@@ -940,14 +941,21 @@ hipError_t launch_tile_exchange(hipStream_t st, int ntiles, int px, const TileDe
   return hipGetLastError();
 }
 
+template <typename TF>
 hipError_t launch_budget(hipStream_t st, Dims d, const int32_t* az, const double* h,
-                         const double* f, double* partial, int nblocks, double area,
-                         double* out) {
-  hipLaunchKernelGGL(k_budget_partial, dim3(nblocks), dim3(256), 0, st, d, az, h, f, partial);
+                         const TF* f, double* partial, int nblocks, double area, double* out) {
+  hipLaunchKernelGGL(k_budget_partial<TF>, dim3(nblocks), dim3(256), 0, st, d, az, h, f, partial);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_budget_final, dim3(1), dim3(64), 0, st, nblocks, partial, area, out);
   return hipGetLastError();
 }
 
+}  // namespace gt
+
+namespace gt {
+template hipError_t launch_budget<double>(hipStream_t, Dims, const int32_t*, const double*,
+                                          const double*, double*, int, double, double*);
+template hipError_t launch_budget<float>(hipStream_t, Dims, const int32_t*, const double*,
+                                         const float*, double*, int, double, double*);
 }  // namespace gt

@@ -1,0 +1,279 @@
+! getm_transport_gpu.F90 -- Fortran binding of libgetm_transport.so.
+!
+! Keeps the Fortran call signatures of the reference routines
+! (getm-model/code) so that the existing driver can call the MI355X kernels:
+!   do_advection_3d    src/3d/advection_3d.F90:120-598
+!   do_advection       src/2d/advection.F90:362-702
+!   adv_split_u/v/w    src/2d/adv_split_u.F90, adv_split_v.F90, src/3d/adv_split_w.F90
+!   adv_upstream_2dh   src/2d/adv_upstream_2dh.F90
+!   update_3d_halo     src/futils/halo_zones.F90:141-187
+! plus do_internal_pressure_gpu / do_eqstate_linear_gpu, which take as
+! arguments the module variables the argument-less reference routines
+! (internal_pressure.F90, eqstate.F90) read.
+!
+! Host arrays are copied to device mirrors owned by this module, the step
+! runs on the GPU, and the outputs are copied back.  Callers that keep their
+! fields on the device use the *_dev entry points with type(c_ptr) handles
+! from gt_malloc.  Errors stop with the reference's message.
+#define HALO 2
+   module getm_transport_gpu
+   use, intrinsic :: iso_c_binding
+   implicit none
+   private
+
+   public gt_init, gt_finalize
+   public do_advection_3d, do_advection_3d_dev
+   public do_internal_pressure_gpu, do_eqstate_linear_gpu, update_3d_halo
+   public gt_malloc_f, gt_free_f, gt_to_device, gt_to_host
+
+   integer, public, parameter :: NOSPLIT=0, FULLSPLIT=1, HALFSPLIT=2, HVSPLIT=3
+   integer, public, parameter :: NOADV=0, UPSTREAM=1, UPSTREAM_2DH=2, P2=3
+   integer, public, parameter :: SUPERBEE=4, MUSCL=5, P2_PDM=6
+   integer, public, parameter :: H_TAG=10, D_TAG=20, U_TAG=31, V_TAG=32, W_TAG=33
+
+   type(c_ptr), save :: ctx = c_null_ptr
+   integer, save :: il, ih, jl, jh, km
+   integer(c_size_t), save :: n3 = 0, n2 = 0
+   ! device mirrors: f uu vv ww hu hv ho hn hires advres
+   type(c_ptr), save :: dbuf(10) = c_null_ptr
+
+   interface
+      function c_gt_create(ctx, imin, imax, jmin, jmax, kmax, dx, dy, az, au, av, ax) &
+               bind(C, name='gt_create') result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), intent(out) :: ctx
+         integer(c_int), value :: imin, imax, jmin, jmax, kmax
+         real(c_double), value :: dx, dy
+         integer(c_int), intent(in) :: az(*), au(*), av(*), ax(*)
+         integer(c_int) :: rc
+      end function
+      function c_gt_destroy(ctx) bind(C, name='gt_destroy') result(rc)
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int) :: rc
+      end function
+      function c_gt_synchronize(ctx) bind(C, name='gt_synchronize') result(rc)
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx
+         integer(c_int) :: rc
+      end function
+      function c_gt_do_advection_3d(ctx, dt, f, uu, vv, ww, hu, hv, ho, hn, split, &
+               hscheme, vscheme, AH, tag, hires, advres) &
+               bind(C, name='gt_do_advection_3d') result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx, f, uu, vv, ww, hu, hv, ho, hn, hires, advres
+         real(c_double), value :: dt, AH
+         integer(c_int), value :: split, hscheme, vscheme, tag
+         integer(c_int) :: rc
+      end function
+      function c_gt_do_internal_pressure(ctx, method, H, hn, hun, hvn, buoy, sseo, &
+               idpdx, idpdy) bind(C, name='gt_do_internal_pressure') result(rc)
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx, H, hn, hun, hvn, buoy, sseo, idpdx, idpdy
+         integer(c_int), value :: method
+         integer(c_int) :: rc
+      end function
+      function c_gt_eos_linear(ctx, T, S, rho, buoy, T0, S0, dtr0, dsr0) &
+               bind(C, name='gt_eos_linear') result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx, T, S, rho, buoy
+         real(c_double), value :: T0, S0, dtr0, dsr0
+         integer(c_int) :: rc
+      end function
+      function c_gt_update_3d_halo(ctx, f1, f2, tag, mirror) &
+               bind(C, name='gt_update_3d_halo') result(rc)
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx, f1, f2
+         integer(c_int), value :: tag, mirror
+         integer(c_int) :: rc
+      end function
+      function c_gt_malloc(p, bytes) bind(C, name='gt_malloc') result(rc)
+         import :: c_ptr, c_int, c_size_t
+         type(c_ptr), intent(out) :: p
+         integer(c_size_t), value :: bytes
+         integer(c_int) :: rc
+      end function
+      function c_gt_free(p) bind(C, name='gt_free') result(rc)
+         import :: c_ptr, c_int
+         type(c_ptr), value :: p
+         integer(c_int) :: rc
+      end function
+      function c_gt_memcpy_h2d(dst, src, bytes) bind(C, name='gt_memcpy_h2d') result(rc)
+         import :: c_ptr, c_int, c_size_t
+         type(c_ptr), value :: dst, src
+         integer(c_size_t), value :: bytes
+         integer(c_int) :: rc
+      end function
+      function c_gt_memcpy_d2h(dst, src, bytes) bind(C, name='gt_memcpy_d2h') result(rc)
+         import :: c_ptr, c_int, c_size_t
+         type(c_ptr), value :: dst, src
+         integer(c_size_t), value :: bytes
+         integer(c_int) :: rc
+      end function
+      function c_gt_last_error() bind(C, name='gt_last_error') result(p)
+         import :: c_ptr
+         type(c_ptr) :: p
+      end function
+   end interface
+
+   contains
+
+   subroutine check(rc, where)
+   integer(c_int), intent(in) :: rc
+   character(len=*), intent(in) :: where
+   character(kind=c_char), pointer :: msg(:)
+   character(len=512) :: text
+   integer :: n
+   if (rc == 0) return
+   call c_f_pointer(c_gt_last_error(), msg, [512])
+   text = ' '
+   do n = 1, 512
+      if (msg(n) == c_null_char) exit
+      text(n:n) = msg(n)
+   end do
+   write(0,*) 'FATAL ERROR: ', where, ': ', trim(text)
+   stop 1
+   end subroutine check
+
+   ! masks of init_domain (src/domain/domain.F90:249-346); au/av/ax as there
+   subroutine gt_init(imin, imax, jmin, jmax, kmax, dx, dy, az, au, av, ax)
+   integer, intent(in) :: imin, imax, jmin, jmax, kmax
+   real(c_double), intent(in) :: dx, dy
+   integer(c_int), intent(in) :: az(imin-HALO:imax+HALO, jmin-HALO:jmax+HALO)
+   integer(c_int), intent(in) :: au(imin-HALO:imax+HALO, jmin-HALO:jmax+HALO)
+   integer(c_int), intent(in) :: av(imin-HALO:imax+HALO, jmin-HALO:jmax+HALO)
+   integer(c_int), intent(in) :: ax(imin-HALO:imax+HALO, jmin-HALO:jmax+HALO)
+   integer :: q
+   il = imin; ih = imax; jl = jmin; jh = jmax; km = kmax
+   call check(c_gt_create(ctx, imin, imax, jmin, jmax, kmax, dx, dy, az, au, av, ax), &
+              'gt_init')
+   n2 = int(imax - imin + 1 + 2*HALO, c_size_t) * int(jmax - jmin + 1 + 2*HALO, c_size_t)
+   n3 = n2 * int(kmax + 1, c_size_t)
+   do q = 1, size(dbuf)
+      call check(c_gt_malloc(dbuf(q), 8_c_size_t * n3), 'gt_init')
+   end do
+   end subroutine gt_init
+
+   subroutine gt_finalize()
+   integer :: q
+   do q = 1, size(dbuf)
+      if (c_associated(dbuf(q))) call check(c_gt_free(dbuf(q)), 'gt_finalize')
+      dbuf(q) = c_null_ptr
+   end do
+   if (c_associated(ctx)) call check(c_gt_destroy(ctx), 'gt_finalize')
+   ctx = c_null_ptr
+   end subroutine gt_finalize
+
+   subroutine gt_malloc_f(p, nbytes)
+   type(c_ptr), intent(out) :: p
+   integer(c_size_t), intent(in) :: nbytes
+   call check(c_gt_malloc(p, nbytes), 'gt_malloc')
+   end subroutine gt_malloc_f
+
+   subroutine gt_free_f(p)
+   type(c_ptr), intent(in) :: p
+   call check(c_gt_free(p), 'gt_free')
+   end subroutine gt_free_f
+
+   subroutine gt_to_device(dst, src)
+   type(c_ptr), intent(in) :: dst
+   real(c_double), intent(in), target :: src(*)
+   call check(c_gt_memcpy_h2d(dst, c_loc(src), 8_c_size_t * n3), 'gt_to_device')
+   end subroutine gt_to_device
+
+   subroutine gt_to_host(dst, src)
+   real(c_double), intent(inout), target :: dst(*)
+   type(c_ptr), intent(in) :: src
+   call check(c_gt_memcpy_d2h(c_loc(dst), src, 8_c_size_t * n3), 'gt_to_host')
+   end subroutine gt_to_host
+
+   ! do_advection_3d with the reference's argument list on host arrays
+   subroutine do_advection_3d(dt, f, uu, vv, ww, hu, hv, ho, hn, split, hscheme, vscheme, &
+                              AH, tag, hires, advres)
+   real(c_double), intent(in) :: dt, AH
+   real(c_double), intent(in), target :: uu(il-HALO:ih+HALO, jl-HALO:jh+HALO, 0:km)
+   real(c_double), intent(in), target :: vv(il-HALO:ih+HALO, jl-HALO:jh+HALO, 0:km)
+   real(c_double), intent(in), target :: ww(il-HALO:ih+HALO, jl-HALO:jh+HALO, 0:km)
+   real(c_double), intent(in), target :: hu(il-HALO:ih+HALO, jl-HALO:jh+HALO, 0:km)
+   real(c_double), intent(in), target :: hv(il-HALO:ih+HALO, jl-HALO:jh+HALO, 0:km)
+   real(c_double), intent(in), target :: ho(il-HALO:ih+HALO, jl-HALO:jh+HALO, 0:km)
+   real(c_double), intent(in), target :: hn(il-HALO:ih+HALO, jl-HALO:jh+HALO, 0:km)
+   integer, intent(in) :: split, hscheme, vscheme, tag
+   real(c_double), intent(inout), target :: f(il-HALO:ih+HALO, jl-HALO:jh+HALO, 0:km)
+   real(c_double), intent(out), target, optional :: &
+        hires(il-HALO:ih+HALO, jl-HALO:jh+HALO, 0:km), &
+        advres(il-HALO:ih+HALO, jl-HALO:jh+HALO, 0:km)
+   type(c_ptr) :: ph, pa
+   call gt_to_device(dbuf(1), f)
+   call gt_to_device(dbuf(2), uu)
+   call gt_to_device(dbuf(3), vv)
+   call gt_to_device(dbuf(4), ww)
+   call gt_to_device(dbuf(5), hu)
+   call gt_to_device(dbuf(6), hv)
+   call gt_to_device(dbuf(7), ho)
+   call gt_to_device(dbuf(8), hn)
+   ph = c_null_ptr
+   pa = c_null_ptr
+   if (present(hires)) ph = dbuf(9)
+   if (present(advres)) pa = dbuf(10)
+   call do_advection_3d_dev(dt, dbuf(1), dbuf(2), dbuf(3), dbuf(4), dbuf(5), dbuf(6), &
+                            dbuf(7), dbuf(8), split, hscheme, vscheme, AH, tag, ph, pa)
+   call gt_to_host(f, dbuf(1))
+   if (present(hires)) call gt_to_host(hires, dbuf(9))
+   if (present(advres)) call gt_to_host(advres, dbuf(10))
+   end subroutine do_advection_3d
+
+   ! the same on device-resident fields (c_null_ptr for absent outputs)
+   subroutine do_advection_3d_dev(dt, f, uu, vv, ww, hu, hv, ho, hn, split, hscheme, &
+                                  vscheme, AH, tag, hires, advres)
+   real(c_double), intent(in) :: dt, AH
+   type(c_ptr), intent(in) :: f, uu, vv, ww, hu, hv, ho, hn, hires, advres
+   integer, intent(in) :: split, hscheme, vscheme, tag
+   call check(c_gt_do_advection_3d(ctx, dt, f, uu, vv, ww, hu, hv, ho, hn, split, &
+              hscheme, vscheme, AH, tag, hires, advres), 'do_advection_3d')
+   call check(c_gt_synchronize(ctx), 'do_advection_3d')
+   end subroutine do_advection_3d_dev
+
+   ! do_eqstate (eqstate_method=1) on device-resident T, S -> buoy
+   subroutine do_eqstate_linear_gpu(T, S, buoy, T0, S0, dtr0, dsr0)
+   type(c_ptr), intent(in) :: T, S, buoy
+   real(c_double), intent(in) :: T0, S0, dtr0, dsr0
+   call check(c_gt_eos_linear(ctx, T, S, c_null_ptr, buoy, T0, S0, dtr0, dsr0), &
+              'do_eqstate')
+   end subroutine do_eqstate_linear_gpu
+
+   ! do_internal_pressure (internal_pressure.F90:181-274), device-resident
+   subroutine do_internal_pressure_gpu(ip_method, H, hn, hun, hvn, buoy, sseo, idpdx, idpdy)
+   integer, intent(in) :: ip_method
+   type(c_ptr), intent(in) :: H, hn, hun, hvn, buoy, sseo, idpdx, idpdy
+   call check(c_gt_do_internal_pressure(ctx, ip_method, H, hn, hun, hvn, buoy, sseo, &
+              idpdx, idpdy), 'do_internal_pressure()')
+   call check(c_gt_synchronize(ctx), 'do_internal_pressure()')
+   end subroutine do_internal_pressure_gpu
+
+   ! update_3d_halo (halo_zones.F90:141-187) on host arrays, serial semantics
+   subroutine update_3d_halo(f1, f2, mask, imin, jmin, imax, jmax, kmax, tag, mirror)
+   integer, intent(in) :: imin, jmin, imax, jmax, kmax, tag
+   integer, intent(in) :: mask(-HALO+1:, -HALO+1:)
+   logical, optional, intent(in) :: mirror
+   real(c_double), intent(inout), target :: f1(imin-HALO:imax+HALO, jmin-HALO:jmax+HALO, 0:kmax)
+   real(c_double), intent(inout), target :: f2(imin-HALO:imax+HALO, jmin-HALO:jmax+HALO, 0:kmax)
+   integer(c_int) :: m
+   if (.false.) print *, mask(-HALO+1, -HALO+1)
+   m = -1
+   if (present(mirror)) then
+      m = 0
+      if (mirror) m = 1
+   end if
+   if (m /= 1) return
+   call gt_to_device(dbuf(1), f1)
+   if (c_associated(c_loc(f1), c_loc(f2))) then
+      call check(c_gt_update_3d_halo(ctx, dbuf(1), dbuf(1), tag, m), 'update_3d_halo')
+   else
+      call gt_to_device(dbuf(2), f2)
+      call check(c_gt_update_3d_halo(ctx, dbuf(1), dbuf(2), tag, m), 'update_3d_halo')
+   end if
+   call gt_to_host(f1, dbuf(1))
+   end subroutine update_3d_halo
+
+   end module getm_transport_gpu

@@ -39,8 +39,17 @@ EXPORTS = [
     "gt_ip_shchepetkin_mcwilliams", "gt_do_internal_pressure",
     "gt_update_3d_halo", "gt_update_2d_halo", "gt_wait_halo",
     "gt_exchange_tiles", "gt_budget", "gt_malloc", "gt_free",
-    "gt_memcpy_h2d", "gt_memcpy_d2h",
+    "gt_memcpy_h2d", "gt_memcpy_d2h", "gt_adv3d_batch_f32", "gt_adv3d_batch_f64",
+    "gt_set_tuning", "gt_budget_f32",
 ]
+
+
+class BatchDomain(C.Structure):
+    """gt_batch_domain of include/getm_transport.h"""
+    _fields_ = [("ctx", C.c_void_p), ("f_in", C.c_void_p), ("f_out", C.c_void_p),
+                ("uu", C.c_void_p), ("vv", C.c_void_p), ("ww", C.c_void_p),
+                ("hu", C.c_void_p), ("hv", C.c_void_p), ("ho", C.c_void_p),
+                ("hires", C.c_void_p)]
 
 
 class GtError(RuntimeError):
@@ -97,6 +106,10 @@ def load_library():
         "gt_free": (i, [vp]),
         "gt_memcpy_h2d": (i, [vp, vp, C.c_size_t]),
         "gt_memcpy_d2h": (i, [vp, vp, C.c_size_t]),
+        "gt_adv3d_batch_f32": (i, [i, C.POINTER(BatchDomain), i, d, i, i, d, i, vp]),
+        "gt_adv3d_batch_f64": (i, [i, C.POINTER(BatchDomain), i, d, i, i, d, i, vp]),
+        "gt_set_tuning": (i, [vp, i, i, i]),
+        "gt_budget_f32": (i, [vp, vp, vp, C.POINTER(d), C.POINTER(d)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -245,9 +258,16 @@ class Context:
 
     # ---- budgets ---------------------------------------------------------------
     def budget(self, h, f=None):
+        """(mass, volume); f may be an FP64 or FP32 field"""
+        import torch
         m, v = C.c_double(), C.c_double()
-        check(self.lib.gt_budget(self.h, _ptr(h), _ptr(f), C.byref(m), C.byref(v)))
+        fn = self.lib.gt_budget_f32 if (f is not None and f.dtype == torch.float32) \
+            else self.lib.gt_budget
+        check(fn(self.h, _ptr(h), _ptr(f), C.byref(m), C.byref(v)))
         return m.value, v.value
+
+    def set_tuning(self, fused=1, tile=0, group=3):
+        check(self.lib.gt_set_tuning(self.h, int(fused), int(tile), int(group)))
 
 
 def exchange_tiles(px, py, ni, nj, nk, tiles, stream=None):
@@ -260,3 +280,22 @@ def exchange_tiles(px, py, ni, nj, nk, tiles, stream=None):
     handle = getattr(stream, "cuda_stream", stream)
     check(lib.gt_exchange_tiles(px, py, ni_a, nj_a, nk, tp,
                                 C.c_void_p(handle) if handle else None))
+
+
+def adv3d_batch(domains, ntracers, dt, hscheme, vscheme, AH=-1.0, tag=H_TAG, stream=None):
+    """gt_adv3d_batch_f32/_f64.  domains: list of dicts with keys ctx, f_in,
+    f_out (tensors of shape (ntracers, kmax+1, NY, NX)), uu, vv, ww, hu, hv,
+    ho and optional hires.  The dtype of f_in picks the FP32 or FP64 kernel."""
+    import torch
+    lib = load_library()
+    n = len(domains)
+    arr = (BatchDomain * n)()
+    for q, d in enumerate(domains):
+        arr[q] = BatchDomain(d["ctx"].h.value, d["f_in"].data_ptr(), d["f_out"].data_ptr(),
+                             *(d[k].data_ptr() for k in ("uu", "vv", "ww", "hu", "hv", "ho")),
+                             d["hires"].data_ptr() if d.get("hires") is not None else None)
+    f32 = domains[0]["f_in"].dtype == torch.float32
+    fn = lib.gt_adv3d_batch_f32 if f32 else lib.gt_adv3d_batch_f64
+    handle = getattr(stream, "cuda_stream", stream)
+    check(fn(n, arr, ntracers, dt, hscheme, vscheme, AH, tag,
+             C.c_void_p(handle) if handle else None))

@@ -167,12 +167,42 @@ int gt_wait_halo(gt_ctx *ctx, int tag);
 int gt_exchange_tiles(int px, int py, const int *ni, const int *nj, int nk,
                       double *const *tiles, void *hip_stream);
 
+/* ---- batched multi-tracer step (no reference counterpart) ---------------
+ * The HALFSPLIT step of do_advection_3d (advection_3d.F90:439-547) for
+ * ntracers tracers that share one velocity field, over ndom domains (e.g.
+ * uneven subdomains), in one fused launch per 8 domains.  Tracer t of a
+ * domain is the I3DFIELD at f_in + t*(kmax+1)*NY*NX; the result goes to
+ * f_out (which must not overlap f_in).  The layer heights stay FP64; hires
+ * (optional) receives the end of the layer-height trajectory.
+ * _f32: tracer and flux arithmetic in FP32, the update (Dio*f-dt*advn)/Di in
+ * FP64.  _f64: the reference arithmetic, bit-identical to do_advection_3d.
+ * Vertical sub-cycling is not applied (adv_ver_iterations = 1). */
+typedef struct {
+  gt_ctx *ctx;
+  const void *f_in;
+  void *f_out;
+  const double *uu, *vv, *ww, *hu, *hv, *ho;
+  double *hires;
+} gt_batch_domain;
+int gt_adv3d_batch_f32(int ndom, const gt_batch_domain *doms, int ntracers,
+                       double dt, int hscheme, int vscheme, double AH, int tag,
+                       void *hip_stream);
+int gt_adv3d_batch_f64(int ndom, const gt_batch_domain *doms, int ntracers,
+                       double dt, int hscheme, int vscheme, double AH, int tag,
+                       void *hip_stream);
+/* kernel choice: fused (1) or per-sub-step (0) HALFSPLIT, fused tile shape
+ * (0: 32x8, 1: 64x4, 2: 16x8) and tracers per workgroup (1, 2, 3, 6) */
+int gt_set_tuning(gt_ctx *ctx, int fused, int tile, int group);
+
 /* ---- budgets (no reference counterpart) ---------------------------------
  * mass = sum over interior water cells (az==1), k=1..kmax of h*f*dx*dy and
  * volume = sum of h*dx*dy, accumulated error-free in double-double.  f may be
  * NULL (volume only; mass then equals volume).  Synchronises. */
 int gt_budget(gt_ctx *ctx, const double *h, const double *f, double *mass,
               double *volume);
+/* the same with an FP32 tracer (each h*f product formed exactly in FP64) */
+int gt_budget_f32(gt_ctx *ctx, const double *h, const float *f, double *mass,
+                  double *volume);
 
 /* ---- device memory helpers for callers without a HIP binding ---------- */
 int gt_malloc(void **ptr, size_t bytes);

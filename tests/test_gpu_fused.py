@@ -1,0 +1,136 @@
+"""The one-pass HALFSPLIT kernel and the batched multi-tracer step.
+
+* FP64 (fused) must equal the per-sub-step kernels and the oracle bit for bit
+  (bound written: 1e-12 relative per cell), for every scheme pair, tag, tile
+  shape, with and without the optional outputs.
+* FP32 batched: max and RMS error against the FP64 oracle, normalised by the
+  field's range, within 1e-5 after several steps; the FP64 mass budget of
+  each tracer is conserved to the FP32 rounding of the stored tracer.
+"""
+import numpy as np
+import pytest
+import torch
+
+from code_amd import inputs
+from code_amd import transport as gt
+from oracle import pyoracle as po
+from tests.test_gpu_parity import random_domain, random_fields
+from tests.util import rel_err, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+SCHEMES = [gt.UPSTREAM, gt.P2, gt.SUPERBEE, gt.MUSCL, gt.P2_PDM]
+NAMES = ["uu", "vv", "ww", "hu", "hv", "ho", "hn"]
+
+
+@pytest.mark.parametrize("kind", ["walls", "open", "islands"])
+def test_fused_f64_bit_exact(device, kind):
+    dom = random_domain(37, 29, 7, 5, kind)
+    F = random_fields(dom, 200)
+    ctx = gt.Context.from_domain(dom)
+    orc = po.Oracle.from_domain(dom)
+    dargs = [to_dev(F[n], device) for n in NAMES]
+    for tile in (0, 1, 2):
+        for hs in SCHEMES:
+            for vs in SCHEMES:
+                for tag in (gt.H_TAG, gt.W_TAG, gt.V_TAG):
+                    for AH in (-1.0, 3.0):
+                        outs = (hs + vs + tile) % 2 == 0
+                        f = F["f"].copy()
+                        hires = np.zeros_like(f) if outs else None
+                        advres = np.zeros_like(f) if outs else None
+                        rc, _ = orc.do_advection_3d(F["dt"], f, *(F[n] for n in NAMES),
+                                                    gt.HALFSPLIT, hs, vs, AH, tag, 1, hires,
+                                                    advres)
+                        assert rc == 0
+                        ctx.set_tuning(fused=1, tile=tile)
+                        tf = to_dev(F["f"], device)
+                        th = to_dev(np.zeros_like(f), device) if outs else None
+                        ta = to_dev(np.zeros_like(f), device) if outs else None
+                        ctx.do_advection_3d(F["dt"], tf, *dargs, gt.HALFSPLIT, hs, vs, AH, tag,
+                                            th, ta)
+                        got = to_host(tf)
+                        e = rel_err(got, f)
+                        assert e <= TOL, (tile, hs, vs, tag, AH, e)
+                        assert np.array_equal(got, f), (tile, hs, vs, tag, AH, "not bit-exact")
+                        if outs:
+                            assert np.array_equal(to_host(th), hires), (tile, hs, vs, tag)
+                            assert rel_err(to_host(ta), advres) <= TOL, (tile, hs, vs, tag)
+    ctx.set_tuning(fused=1, tile=0)
+
+
+def test_batch_f64_two_tracers_bit_exact(device):
+    tr = inputs.config2(seed=4, imax=45, jmax=38, kmax=9)
+    d = tr.dom
+    ctx = gt.Context.from_domain(d)
+    orc = po.Oracle.from_domain(d)
+    T, S = tr.tracers["T"].copy(), tr.tracers["S"].copy()
+    host = [tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho, tr.hn]
+    dv = dict(zip(NAMES, [to_dev(a, device) for a in host]))
+    fin = to_dev(np.stack([T, S]), device)
+    fout = torch.empty_like(fin)
+    for _ in range(3):
+        for f in (T, S):
+            orc.do_advection_3d(tr.dt, f, *host, gt.HALFSPLIT, gt.P2_PDM, gt.P2_PDM, -1.0,
+                                gt.H_TAG)
+        gt.adv3d_batch([dict(ctx=ctx, f_in=fin, f_out=fout, **dv)], 2, tr.dt, gt.P2_PDM,
+                       gt.P2_PDM)
+        fin, fout = fout, fin
+    got = to_host(fin)
+    assert np.array_equal(got[0], T) and np.array_equal(got[1], S)
+
+
+def normwise(a, b):
+    scale = np.max(np.abs(b))
+    d = np.abs(a - b)
+    return float(d.max() / scale), float(np.sqrt(np.mean(d ** 2)) / scale)
+
+
+def test_batch_f32_multidomain(device):
+    """T, S and passive tracers on uneven subdomains in one launch, FP32
+    against the FP64 oracle over several P2-PDM steps."""
+    sizes = [(24, 20), (41, 33), (30, 52)]
+    subs = inputs.config3(seed=9, kmax=8, sizes=sizes, npassive=4)
+    names = list(subs[0].tracers)
+    nt = len(names)
+    doms, refs, oracles, tins, touts, ctxs = [], [], [], [], [], []
+    for tr in subs:
+        d = tr.dom
+        ctx = gt.Context.from_domain(d)
+        ctxs.append(ctx)
+        oracles.append(po.Oracle.from_domain(d))
+        stack = np.stack([tr.tracers[n] for n in names])
+        refs.append(stack.copy())
+        tin = to_dev(stack.astype(np.float32), device)
+        tout = torch.empty_like(tin)
+        tins.append(tin)
+        touts.append(tout)
+        host = [tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho]
+        doms.append(dict(ctx=ctx, **dict(zip(NAMES[:6], [to_dev(a, device) for a in host]))))
+    m0 = [[ctxs[s].budget(to_dev(subs[s].hn, device), tins[s][q]) for q in range(nt)]
+          for s in range(len(subs))]
+    nsteps = 6
+    for group in (1, 3):
+        ctxs[0].set_tuning(fused=1, tile=0, group=group)
+    for step in range(nsteps):
+        for s, tr in enumerate(subs):
+            host = [tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho, tr.hn]
+            for q in range(nt):
+                f = refs[s][q]
+                oracles[s].do_advection_3d(tr.dt, f, *host, gt.HALFSPLIT, gt.P2_PDM, gt.P2_PDM,
+                                           -1.0, gt.H_TAG)
+        for s in range(len(subs)):
+            doms[s]["f_in"], doms[s]["f_out"] = tins[s], touts[s]
+        gt.adv3d_batch(doms, nt, subs[0].dt, gt.P2_PDM, gt.P2_PDM)
+        tins, touts = touts, tins
+    for s, tr in enumerate(subs):
+        wet = tr.dom.az == 1
+        got = to_host(tins[s]).astype(np.float64)
+        for q, n in enumerate(names):
+            a = got[q][1:, wet]
+            b = refs[s][q][1:, wet]
+            emax, erms = normwise(a, b)
+            assert emax <= 1e-5, (s, n, emax, erms)
+            m1 = ctxs[s].budget(to_dev(tr.hn, device), tins[s][q])[0]
+            mass0 = m0[s][q][0]
+            assert abs(m1 - mass0) <= 1e-6 * abs(mass0), (s, n, mass0, m1)
