@@ -1,10 +1,15 @@
 """Benchmark of the MI355X transport step.
 
-Workload (one "step"): the FP64 baroclinic transport step of a
-400x400x40 GVC domain -- linear EOS, Blumberg-Mellor and
+Headline workload (one "step", FP64): the baroclinic transport step of a
+400x400x40 GVC basin -- linear EOS, Blumberg-Mellor and
 Shchepetkin-McWilliams internal pressure, and P2-PDM HALFSPLIT advection of
 temperature and salinity -- with every input resident in HBM.
-metric: tracer cell-updates/s (imax*jmax*kmax*ntracers per step / time).
+metric: tracer cell-updates/s = imax*jmax*kmax*2 per step / time.
+
+Second section (batched FP32): T, S and 16 passive tracers on 8 uneven
+subdomains (64x64 ... 512x384, 30 layers) in one fused launch per step,
+FP32 tracers with FP64 layer heights; its error against the FP64 path after
+the timed steps, and its mass-budget drift.
 
 Prints one JSON line.  Run: python bench.py [--steps K] [--warmup W]
 """
@@ -21,6 +26,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
+METRIC = "tracer cell-updates/s + % HBM peak; batched FP32 vs plain FP64 at <=1e-5 err"
 
 
 def parse():
@@ -34,7 +40,34 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="bound on the CPU-baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-batched", action="store_true")
+    p.add_argument("--batched-steps", type=int, default=20)
+    p.add_argument("--tile", type=int, default=-1, help="fused tile shape (-1: tuned)")
+    p.add_argument("--group", type=int, default=-1, help="tracers per workgroup (-1: tuned)")
     return p.parse_args()
+
+
+class EventTimer:
+    """HIP events around launches on one stream; mean duration in seconds"""
+
+    def __init__(self, torch, stream):
+        self.torch, self.st, self.pairs = torch, stream, []
+
+    def __enter__(self):
+        e = self.torch.cuda.Event(enable_timing=True)
+        e.record(self.st)
+        self.pairs.append([e, None])
+        return self
+
+    def __exit__(self, *exc):
+        e = self.torch.cuda.Event(enable_timing=True)
+        e.record(self.st)
+        self.pairs[-1][1] = e
+
+    def mean(self):
+        self.st.synchronize()
+        ts = [a.elapsed_time(b) * 1e-3 for a, b in self.pairs]
+        return sum(ts) / len(ts)
 
 
 def main():
@@ -58,92 +91,63 @@ def main():
     d = tr.dom
     st = torch.cuda.Stream(device=dev)
     ctx = gt.Context.from_domain(d, stream=st)
-    D = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+    if a.tile >= 0:
+        ctx.set_tuning(fused=1, tile=a.tile, group=max(a.group, 1) if a.group > 0 else 3)
+    D = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)  # noqa: E731
     uu, vv, ww, hun, hvn, ho, hn = (D(x) for x in (tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho,
                                                    tr.hn))
     H = D(d.H)
     sseo = torch.zeros((d.NY, d.NX), dtype=torch.float64, device=dev)
-    T = D(tr.tracers["T"])
-    S = D(tr.tracers["S"])
-    buoy = torch.zeros_like(T)
-    idpdx = torch.zeros_like(T)
-    idpdy = torch.zeros_like(T)
-    idpdx6 = torch.zeros_like(T)
-    idpdy6 = torch.zeros_like(T)
-    zz = torch.zeros_like(T)
-    zz6 = torch.zeros_like(T)
-    hires = torch.zeros_like(T)
+    TS = D(np.stack([tr.tracers["T"], tr.tracers["S"]]))
+    TS2 = torch.empty_like(TS)
+    shape = TS.shape[1:]
+    buoy = torch.zeros(shape, dtype=torch.float64, device=dev)
+    idpdx, idpdy, idpdx6, idpdy6, zz, zz6 = (torch.zeros_like(buoy) for _ in range(6))
+    hires = torch.zeros_like(buoy)
     consts = (inputs.EOS_T0, inputs.EOS_S0, inputs.EOS_DTR0, inputs.EOS_DSR0)
     dt = tr.dt
+    bufs = [TS, TS2]
+    t_adv = EventTimer(torch, st)
+    t_ip = EventTimer(torch, st)
+
+    def step(timed):
+        cur, nxt = bufs
+        with (t_ip if timed else _Null()):
+            ctx.eos_linear(cur[0], cur[1], None, buoy, *consts)
+            ctx.ip_blumberg_mellor(H, hn, hun, hvn, buoy, zz, idpdx, idpdy)
+            ctx.ip_shchepetkin_mcwilliams(H, hn, hun, hvn, buoy, sseo, zz6, idpdx6, idpdy6)
+        with (t_adv if timed else _Null()):
+            gt.adv3d_batch([dict(ctx=ctx, f_in=cur, f_out=nxt, uu=uu, vv=vv, ww=ww, hu=hun,
+                                 hv=hvn, ho=ho, hires=hires)], 2, dt, gt.P2_PDM, gt.P2_PDM,
+                           stream=st)
+        bufs.reverse()
+
     torch.cuda.synchronize()
-
-    def eos():
-        ctx.eos_linear(T, S, None, buoy, *consts)
-
-    def ip_bm():
-        ctx.ip_blumberg_mellor(H, hn, hun, hvn, buoy, zz, idpdx, idpdy)
-
-    def ip_sm():
-        ctx.ip_shchepetkin_mcwilliams(H, hn, hun, hvn, buoy, sseo, zz6, idpdx6, idpdy6)
-
-    def adv(f, hr=None):
-        ctx.do_advection_3d(dt, f, uu, vv, ww, hun, hvn, ho, hn, gt.HALFSPLIT, gt.P2_PDM,
-                            gt.P2_PDM, -1.0, gt.H_TAG, hr, None)
-
-    def step():
-        eos()
-        ip_bm()
-        ip_sm()
-        adv(T, hires)
-        adv(S)
-
     with torch.cuda.stream(st):
         for _ in range(a.warmup):
-            step()
+            step(False)
         st.synchronize()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(a.steps):
-            step()
+            step(True)
         st.synchronize()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-
-        # component durations with HIP events on the context's stream
-        def timed(fn, n):
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(st)
-            for _ in range(n):
-                fn()
-            e1.record(st)
-            e1.synchronize()
-            return e0.elapsed_time(e1) * 1e-3 / n
-
-        nrep = max(5, a.steps // 2)
-        t_adv = timed(lambda: adv(S), nrep)
-        t_bm = timed(ip_bm, nrep)
-        t_sm = timed(ip_sm, nrep)
-        t_eos = timed(eos, nrep)
-
     ms = (t1 - t0) / a.steps * 1e3
     ncell = d.imax * d.jmax * d.kmax
     ntr = 2
     value = ncell * ntr / (ms * 1e-3)
-    # algorithmic bytes of one tracer's advection step: f in/out + 7 FP64
-    # inputs once (uu, vv, ww, hun, hvn, ho, hn)
-    b_adv = ncell * (2 * 8 + 7 * 8)
-    roof = {
-        "bound": "hbm",
-        "kernel": "do_advection_3d (one tracer, all sub-steps)",
-        "achieved": b_adv / t_adv / 1e9,
-        "peak": HBM_PEAK / 1e9,
-        "unit": "GB/s",
-        "frac": b_adv / t_adv / HBM_PEAK,
-        "traffic": None,
-    }
+    adv_s = t_adv.mean()
+    # algorithmic bytes of the fused advection launch (T and S): each tracer
+    # read and written once, uu vv ww hu hv ho read once, FP64
+    b_adv = ncell * (ntr * 2 * 8 + 6 * 8)
+    roof = {"bound": "hbm", "kernel": "k_fused_halfsplit<double> (T+S, one launch)",
+            "achieved": b_adv / adv_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": b_adv / adv_s / HBM_PEAK, "traffic": None,
+            "bytes_per_launch": b_adv, "launch_us": adv_s * 1e6}
     out = {
-        "metric": "tracer cell-updates/s (FP64 transport step, T+S)",
+        "metric": METRIC,
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -154,26 +158,129 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded GVC basin, streamfunction flow)",
-        "config": {"workload": f"{d.imax}x{d.jmax}x{d.kmax} GVC, EOS+BM+SMcW+2-tracer "
-                               "P2-PDM HALFSPLIT advection", "imax": d.imax, "jmax": d.jmax,
+        "data": "synthetic (seeded GVC basin, streamfunction flow, T/S profiles)",
+        "config": {"workload": f"{d.imax}x{d.jmax}x{d.kmax} GVC basin: linear EOS + "
+                               "Blumberg-Mellor + Shchepetkin-McWilliams + P2-PDM HALFSPLIT "
+                               "advection of T and S", "imax": d.imax, "jmax": d.jmax,
                    "kmax": d.kmax, "tracers": ntr, "parallelism": "single GPU"},
-        "components_ms": {"advection_one_tracer": t_adv * 1e3, "ip_blumberg_mellor": t_bm * 1e3,
-                          "ip_shchepetkin_mcwilliams": t_sm * 1e3, "eos_linear": t_eos * 1e3},
+        "fp64_advection_cell_updates_per_s": ncell * ntr / adv_s,
+        "components_ms": {"advection_T_S": adv_s * 1e3, "eos_and_pressure": t_ip.mean() * 1e3},
         "roofline": roof,
     }
+    del TS2, idpdx6, idpdy6
+    if not a.no_batched:
+        out["batched_fp32"] = batched(torch, np, gt, inputs, dev, a, ncell * ntr / adv_s)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(tr, a.cpu_seconds)
     print(json.dumps(out))
 
 
+class _Null:
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        return False
+
+
+def batched(torch, np, gt, inputs, dev, a, fp64_rate):
+    """18 tracers on 8 uneven subdomains, one fused FP32 launch per step"""
+    subs = inputs.config3(npassive=16)
+    names = list(subs[0].tracers)
+    nt = len(names)
+    st = torch.cuda.Stream(device=dev)
+    doms, doms64, ctxs, fin32, fin64 = [], [], [], [], []
+    ncell = 0
+    for tr in subs:
+        d = tr.dom
+        ncell += d.imax * d.jmax * d.kmax
+        ctx = gt.Context.from_domain(d, stream=st)
+        if a.tile >= 0:
+            ctx.set_tuning(1, a.tile, a.group if a.group > 0 else 3)
+        ctxs.append(ctx)
+        stack = np.stack([tr.tracers[n] for n in names])
+        f64 = torch.from_numpy(stack).to(dev)
+        f32 = f64.to(torch.float32)
+        fin64.append([f64, torch.empty_like(f64)])
+        fin32.append([f32, torch.empty_like(f32)])
+        vel = {k: torch.from_numpy(np.ascontiguousarray(getattr(tr, k2))).to(dev)
+               for k, k2 in (("uu", "uu"), ("vv", "vv"), ("ww", "ww"), ("hu", "hun"),
+                             ("hv", "hvn"), ("ho", "ho"))}
+        vel["hn"] = torch.from_numpy(tr.hn).to(dev)
+        doms.append(dict(ctx=ctx, **vel))
+        tr.tracers.clear()
+    dt = subs[0].dt
+    hs = vs = gt.P2_PDM
+    # FP64 budgets of the initial state (for the drift check)
+    m0 = [[ctxs[s].budget(doms[s]["hn"], fin64[s][0][q])[0] for q in range(nt)]
+          for s in range(len(subs))]
+
+    def launch(bufs, dtype_doms):
+        gt.adv3d_batch([dict(ctx=dd["ctx"], f_in=bufs[s][0], f_out=bufs[s][1], uu=dd["uu"],
+                             vv=dd["vv"], ww=dd["ww"], hu=dd["hu"], hv=dd["hv"], ho=dd["ho"])
+                        for s, dd in enumerate(dtype_doms)], nt, dt, hs, vs, stream=st)
+        for bb in bufs:
+            bb.reverse()
+
+    timer = EventTimer(torch, st)
+    nsteps = a.batched_steps
+    with torch.cuda.stream(st):
+        for _ in range(2):   # warm-up: two steps forward and back is not neutral,
+            launch(fin32, doms)  # so the FP64 reference takes the same two steps
+            launch(fin64, doms)
+        st.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            with timer:
+                launch(fin32, doms)
+        st.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(nsteps):
+            launch(fin64, doms)
+        st.synchronize()
+    kern = timer.mean()
+    wall = (t1 - t0) / nsteps
+    emax, erms, drift = 0.0, 0.0, 0.0
+    for s in range(len(subs)):
+        wet = torch.from_numpy(subs[s].dom.az == 1).to(dev)
+        g = fin32[s][0].to(torch.float64)
+        r = fin64[s][0]
+        for q in range(nt):
+            gq = g[q, 1:][:, wet]
+            rq = r[q, 1:][:, wet]
+            scale = float(rq.abs().max())
+            dq = (gq - rq).abs()
+            emax = max(emax, float(dq.max()) / scale)
+            erms = max(erms, float(torch.sqrt(torch.mean(dq * dq))) / scale)
+            m1 = ctxs[s].budget(doms[s]["hn"], fin32[s][0][q])[0]
+            drift = max(drift, abs(m1 - m0[s][q]) / abs(m0[s][q]))
+    # algorithmic bytes: each FP32 tracer read+written, 6 FP64 fields read once
+    bytes_step = ncell * (nt * 2 * 4 + 6 * 8)
+    rate = ncell * nt / kern
+    return {
+        "workload": "T, S + 16 passive tracers on 8 subdomains (64x64 ... 512x384) x 30 "
+                    "layers, P2-PDM HALFSPLIT, FP32 tracers / FP64 layer heights, one launch",
+        "cells": ncell, "tracers": nt,
+        "value": rate, "unit": "cell-updates/s", "ms_per_step": kern * 1e3,
+        "wall_ms_per_step": wall * 1e3,
+        "roofline": {"bound": "hbm", "kernel": "k_fused_halfsplit<float>",
+                     "achieved": bytes_step / kern / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": bytes_step / kern / HBM_PEAK,
+                     "bytes_per_launch": bytes_step},
+        "vs_fp64_advection": rate / fp64_rate,
+        "err_vs_fp64": {"max": emax, "rms": erms, "steps": nsteps + 2,
+                        "norm": "|f32-f64| / max|f64| over wet cells, worst tracer"},
+        "mass_drift_max_rel": drift,
+    }
+
+
 def cpu_baseline(tr, seconds):
-    """The oracle (C/OpenMP restatement) timed on this host's cores on whole
-    steps of the same workload, until `seconds` have elapsed."""
+    """The oracle (C/OpenMP restatement of the reference) timed on this host's
+    cores on whole steps of the same workload, until `seconds` have elapsed."""
     import numpy as np
 
-    from oracle import pyoracle as po
     from code_amd import inputs
+    from oracle import pyoracle as po
     ncores = len(os.sched_getaffinity(0))
     po.set_threads(ncores)
     d = tr.dom
@@ -206,7 +313,8 @@ def cpu_baseline(tr, seconds):
     t = (time.perf_counter() - t0) / n
     ncell = d.imax * d.jmax * d.kmax
     return {"value": ncell * 2 / t, "unit": "cell-updates/s", "cores": ncores, "kind": "port",
-            "sample": f"{n} whole steps of the same workload (s/step {t:.3f})"}
+            "sample": f"{n} whole steps of the same 400x400x40 workload, s/step {t:.4f}, "
+                      f"OpenMP threads {ncores}"}
 
 
 if __name__ == "__main__":

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/getm_transport.h"
+#include "gt_adjoint.h"
 #include "gt_batched.h"
 #include "gt_kernels.h"
 
@@ -55,6 +56,7 @@ struct gt_ctx {
   double *fi = nullptr, *hi = nullptr, *adv = nullptr, *tmp = nullptr;
   double *scr7 = nullptr, *zz = nullptr, *P = nullptr, *slice3 = nullptr;
   double *bpart = nullptr, *bout = nullptr;
+  std::vector<double*> adj;  // reverse-mode work fields (allocated on first use)
   unsigned long long* nwarn = nullptr;
   int adv_ver_iterations = 1;
   int do_mirror_2d = 1;
@@ -182,6 +184,8 @@ extern "C" int gt_destroy(gt_ctx* c) {
   void* ps[] = {c->dmask, c->dint, c->fi, c->hi, c->adv, c->tmp, c->scr7,
                 c->zz, c->P, c->slice3, c->bpart, c->bout, c->nwarn};
   for (void* p : ps)
+    if (p) (void)hipFree(p);
+  for (double* p : c->adj)
     if (p) (void)hipFree(p);
   delete c;
   return GT_OK;
@@ -795,6 +799,159 @@ extern "C" int gt_adv3d_batch_f64(int ndom, const gt_batch_domain* doms, int ntr
                                   double dt, int hscheme, int vscheme, double AH, int tag,
                                   void* stream) {
   return batch_impl<double>(ndom, doms, ntracers, dt, hscheme, vscheme, AH, tag, stream);
+}
+
+// ---- reverse mode (extension) ------------------------------------------------
+// work fields: D0..D5 (layer heights through the five sub-steps), S0..S4
+// (tracer states before each sub-step), two adjoint buffers, then the
+// checkpoints and the states of one checkpoint segment
+static int adj_fields(gt_ctx* c, int n) {
+  while ((int)c->adj.size() < n) {
+    double* p = nullptr;
+    HIPCHK(hipMalloc(&p, sizeof(double) * (size_t)c->N3));
+    c->adj.push_back(p);
+  }
+  return GT_OK;
+}
+
+// one HALFSPLIT step from s0, keeping every intermediate state and layer
+// height (the forward of advection_3d.F90:439-547 with the sub-step kernels)
+static int forward_capture(gt_ctx* c, const GridMasks* g, double dt, const double* s0,
+                           double* out, const double* uu, const double* vv, const double* ww,
+                           const double* hu, const double* hv, const double* ho, int hs, int vs,
+                           double AH, int kshift, double** D, double** S) {
+  const long N3 = c->N3, NXY = c->d.NXY;
+  const int kmax = c->kmax;
+  ENSURE(c->adv, N3);
+  double* adv = c->adv;
+  TRY(copy_d2d(c, D[0], ho, N3));
+  TRY(copy_d2d(c, D[1], D[0], N3));
+  TRY(copy_d2d(c, S[1], s0, NXY));  // level 0 is never advanced horizontally
+  TRY(step_u(c, g, 1, kmax, s0, s0, S[1], D[1], adv, uu, hu, dt, 0.5, hs, AH));
+  TRY(copy_d2d(c, D[2], D[1], N3));
+  TRY(copy_d2d(c, S[2], s0, NXY));
+  TRY(step_v(c, g, 1, kmax, S[1], S[1], S[2], D[2], adv, vv, hv, dt, 0.5, hs, AH));
+  TRY(copy_d2d(c, D[3], D[2], N3));
+  LCHK(gt::launch_split_w(c->st, c->d, S[2], S[2], S[3], D[3], adv, ww, g->az, dt, 1.0, vs,
+                          kshift));
+  TRY(copy_d2d(c, D[4], D[3], N3));
+  TRY(copy_d2d(c, S[4], S[3], NXY));
+  TRY(step_v(c, g, 1, kmax, S[3], S[3], S[4], D[4], adv, vv, hv, dt, 0.5, hs, AH));
+  TRY(copy_d2d(c, D[5], D[4], N3));
+  if (out) {
+    TRY(copy_d2d(c, out, S[4], NXY));
+    TRY(step_u(c, g, 1, kmax, S[4], S[4], out, D[5], adv, uu, hu, dt, 0.5, hs, AH));
+  } else {
+    // only D5 is wanted
+    ENSURE(c->tmp, N3);
+    TRY(step_u(c, g, 1, kmax, S[4], S[4], c->tmp, D[5], adv, uu, hu, dt, 0.5, hs, AH));
+  }
+  return GT_OK;
+}
+
+// transpose of one HALFSPLIT step: lam (adjoint of the step's output) is
+// replaced by the adjoint of its input state s0
+static int reverse_step(gt_ctx* c, const GridMasks* g, double dt, const double* s0,
+                        double** lam, const double* uu, const double* vv, const double* ww,
+                        const double* hu, const double* hv, const double* ho, int hs, int vs,
+                        double AH, int kshift, double** D, double** S) {
+  const long NXY = c->d.NXY;
+  const int kmax = c->kmax;
+  TRY(forward_capture(c, g, dt, s0, nullptr, uu, vv, ww, hu, hv, ho, hs, vs, AH, kshift, D, S));
+  double* a = lam[0];
+  double* b = lam[1];
+  auto uv = [&](int dir, const double* f, const double* Dio, const double* Dn) -> int {
+    const double* vel = dir == 0 ? uu : vv;
+    const double* Dv = dir == 0 ? hu : hv;
+    const uint8_t* mf = dir == 0 ? g->uflux : g->vflux;
+    const uint8_t* mu = dir == 0 ? g->uupd : g->vupd;
+    double lflux = dir == 0 ? c->dx : c->dy, lface = dir == 0 ? c->dy : c->dx;
+    TRY(copy_d2d(c, b, a, NXY));
+    LCHK(gt::launch_split_uv_adj(c->st, c->d, 1, kmax, dir, f, a, b, Dio, Dn, vel, Dv, mf, mu,
+                                 dt, 0.5, hs, AH, lflux, lface, c->ard1));
+    double* t = a; a = b; b = t;
+    return GT_OK;
+  };
+  TRY(uv(0, S[4], D[4], D[5]));
+  TRY(uv(1, S[3], D[3], D[4]));
+  LCHK(gt::launch_split_w_adj(c->st, c->d, S[2], a, b, D[2], D[3], ww, g->az, dt, 1.0, vs,
+                              kshift));
+  { double* t = a; a = b; b = t; }
+  TRY(uv(1, S[1], D[1], D[2]));
+  TRY(uv(0, S[0], D[0], D[1]));
+  lam[0] = a;
+  lam[1] = b;
+  return GT_OK;
+}
+
+extern "C" int gt_adv3d_adjoint(gt_ctx* c, int nsteps, int ckpt_every, double dt,
+                                const double* f0, double* f_final, const double* lam_final,
+                                double* lam0, const double* uu, const double* vv,
+                                const double* ww, const double* hu, const double* hv,
+                                const double* ho, int hscheme, int vscheme, double AH,
+                                int tag) {
+  if (!c || !f0 || !lam_final || !lam0 || nsteps < 1 || ckpt_every < 1)
+    return set_err(GT_EARG, "gt_adv3d_adjoint: bad argument");
+  const GridMasks* g = grid_for(c, tag);
+  if (!g) return set_err(GT_EINVAL_TAG, "gt_adv3d_adjoint: tag is invalid");
+  if (!split_scheme(hscheme) || !split_scheme(vscheme))
+    return set_err(GT_EINVAL_SCHEME, "gt_adv3d_adjoint: schemes must be UPSTREAM..P2_PDM");
+  if (c->kmax < 2) return set_err(GT_EARG, "gt_adv3d_adjoint: kmax must be >= 2");
+  const int kshift = tag == GT_W_TAG ? 1 : 0;
+  const int nck = (nsteps + ckpt_every - 1) / ckpt_every;
+  const int seg = ckpt_every;
+  // D[6] S[5] lam[2] work[1] ck[nck] seg[seg]
+  const int nf = 6 + 5 + 2 + 1 + nck + seg;
+  TRY(adj_fields(c, nf));
+  double** D = &c->adj[0];
+  double** S = &c->adj[6];
+  double* lam[2] = {c->adj[11], c->adj[12]};
+  double* work = c->adj[13];
+  double** ck = &c->adj[14];
+  double** sg = &c->adj[14 + nck];
+  const long N3 = c->N3;
+  // forward sweep, keeping a checkpoint every ckpt_every steps
+  TRY(copy_d2d(c, work, f0, N3));
+  for (int t = 0; t < nsteps; t++) {
+    if (t % ckpt_every == 0) TRY(copy_d2d(c, ck[t / ckpt_every], work, N3));
+    TRY(forward_capture(c, g, dt, work, sg[0], uu, vv, ww, hu, hv, ho, hscheme, vscheme, AH,
+                        kshift, D, S));
+    TRY(copy_d2d(c, work, sg[0], N3));
+  }
+  if (f_final) TRY(copy_d2d(c, f_final, work, N3));
+  // reverse sweep, segment by segment from the last checkpoint
+  TRY(copy_d2d(c, lam[0], lam_final, N3));
+  for (int s = nck - 1; s >= 0; s--) {
+    int t0 = s * ckpt_every, t1 = t0 + ckpt_every < nsteps ? t0 + ckpt_every : nsteps;
+    TRY(copy_d2d(c, sg[0], ck[s], N3));
+    for (int t = t0; t < t1 - 1; t++)
+      TRY(forward_capture(c, g, dt, sg[t - t0], sg[t - t0 + 1], uu, vv, ww, hu, hv, ho, hscheme,
+                          vscheme, AH, kshift, D, S));
+    for (int t = t1 - 1; t >= t0; t--) {
+      TRY(copy_d2d(c, S[0], sg[t - t0], N3));
+      TRY(reverse_step(c, g, dt, S[0], lam, uu, vv, ww, hu, hv, ho, hscheme, vscheme, AH, kshift,
+                       D, S));
+    }
+  }
+  return copy_d2d(c, lam0, lam[0], N3);
+}
+
+extern "C" int gt_ip_blumberg_mellor_adjoint(gt_ctx* c, const double* hun, const double* hvn,
+                                             const double* zz, const double* idpdx_bar,
+                                             const double* idpdy_bar, double* buoy_bar) {
+  if (!c || !hun || !hvn || !zz || !idpdx_bar || !idpdy_bar || !buoy_bar)
+    return set_err(GT_EARG, "gt_ip_blumberg_mellor_adjoint: NULL argument");
+  if (c->kmax > 64) return set_err(GT_EARG, "gt_ip_blumberg_mellor_adjoint: kmax > 64");
+  LCHK(gt::launch_bm_adj(c->st, c->d, c->au, c->av, zz, hun, hvn, idpdx_bar, idpdy_bar,
+                         buoy_bar, c->dx, c->dy));
+  return GT_OK;
+}
+
+extern "C" int gt_eos_linear_adjoint(gt_ctx* c, const double* buoy_bar, double* T_bar,
+                                     double* S_bar, double dtr0, double dsr0) {
+  if (!c || !buoy_bar) return set_err(GT_EARG, "gt_eos_linear_adjoint: NULL argument");
+  LCHK(gt::launch_eos_linear_adj(c->st, c->d, buoy_bar, T_bar, S_bar, dtr0, dsr0));
+  return GT_OK;
 }
 
 // ---- budgets -----------------------------------------------------------------------

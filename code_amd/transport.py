@@ -40,7 +40,8 @@ EXPORTS = [
     "gt_update_3d_halo", "gt_update_2d_halo", "gt_wait_halo",
     "gt_exchange_tiles", "gt_budget", "gt_malloc", "gt_free",
     "gt_memcpy_h2d", "gt_memcpy_d2h", "gt_adv3d_batch_f32", "gt_adv3d_batch_f64",
-    "gt_set_tuning", "gt_budget_f32",
+    "gt_set_tuning", "gt_budget_f32", "gt_adv3d_adjoint",
+    "gt_ip_blumberg_mellor_adjoint", "gt_eos_linear_adjoint",
 ]
 
 
@@ -110,6 +111,10 @@ def load_library():
         "gt_adv3d_batch_f64": (i, [i, C.POINTER(BatchDomain), i, d, i, i, d, i, vp]),
         "gt_set_tuning": (i, [vp, i, i, i]),
         "gt_budget_f32": (i, [vp, vp, vp, C.POINTER(d), C.POINTER(d)]),
+        "gt_adv3d_adjoint": (i, [vp, i, i, d, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i, i,
+                                 d, i]),
+        "gt_ip_blumberg_mellor_adjoint": (i, [vp, vp, vp, vp, vp, vp, vp]),
+        "gt_eos_linear_adjoint": (i, [vp, vp, vp, vp, d, d]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -265,6 +270,23 @@ class Context:
             else self.lib.gt_budget
         check(fn(self.h, _ptr(h), _ptr(f), C.byref(m), C.byref(v)))
         return m.value, v.value
+
+    # ---- reverse mode -------------------------------------------------------------
+    def adv3d_adjoint(self, nsteps, ckpt_every, dt, f0, lam_final, lam0, uu, vv, ww, hu, hv, ho,
+                      hscheme, vscheme, AH=-1.0, tag=H_TAG, f_final=None):
+        check(self.lib.gt_adv3d_adjoint(self.h, nsteps, ckpt_every, dt, _ptr(f0), _ptr(f_final),
+                                        _ptr(lam_final), _ptr(lam0), _ptr(uu), _ptr(vv),
+                                        _ptr(ww), _ptr(hu), _ptr(hv), _ptr(ho), hscheme,
+                                        vscheme, AH, tag))
+
+    def ip_blumberg_mellor_adjoint(self, hun, hvn, zz, idpdx_bar, idpdy_bar, buoy_bar):
+        check(self.lib.gt_ip_blumberg_mellor_adjoint(self.h, _ptr(hun), _ptr(hvn), _ptr(zz),
+                                                     _ptr(idpdx_bar), _ptr(idpdy_bar),
+                                                     _ptr(buoy_bar)))
+
+    def eos_linear_adjoint(self, buoy_bar, T_bar, S_bar, dtr0, dsr0):
+        check(self.lib.gt_eos_linear_adjoint(self.h, _ptr(buoy_bar), _ptr(T_bar), _ptr(S_bar),
+                                             dtr0, dsr0))
 
     def set_tuning(self, fused=1, tile=0, group=3):
         check(self.lib.gt_set_tuning(self.h, int(fused), int(tile), int(group)))

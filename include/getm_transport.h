@@ -194,6 +194,31 @@ int gt_adv3d_batch_f64(int ndom, const gt_batch_domain *doms, int ntracers,
  * (0: 32x8, 1: 64x4, 2: 16x8) and tracers per workgroup (1, 2, 3, 6) */
 int gt_set_tuning(gt_ctx *ctx, int fused, int tile, int group);
 
+/* ---- reverse mode (no reference counterpart) -----------------------------
+ * Adjoint of nsteps HALFSPLIT steps of do_advection_3d (advection_3d.F90:
+ * 439-547, adv_ver_iterations = 1) with respect to the tracer, velocities
+ * and layer heights held fixed.  Runs the forward from f0 (result in f_final,
+ * optional), keeping a checkpoint every ckpt_every steps, then sweeps back:
+ * lam0 = (d f_final / d f0)^T lam_final.  The limiter enters through the
+ * derivative of the branch the forward took. */
+int gt_adv3d_adjoint(gt_ctx *ctx, int nsteps, int ckpt_every, double dt,
+                     const double *f0, double *f_final, const double *lam_final,
+                     double *lam0, const double *uu, const double *vv,
+                     const double *ww, const double *hu, const double *hv,
+                     const double *ho, int hscheme, int vscheme, double AH,
+                     int tag);
+/* transpose of ip_blumberg_mellor (ip_blumberg_mellor.F90:99-149) with
+ * respect to buoy, zz/hun/hvn from the forward call: buoy_bar is overwritten
+ * (kmax <= 64) */
+int gt_ip_blumberg_mellor_adjoint(gt_ctx *ctx, const double *hun,
+                                  const double *hvn, const double *zz,
+                                  const double *idpdx_bar,
+                                  const double *idpdy_bar, double *buoy_bar);
+/* transpose of the linear EOS + buoyancy (eqstate.F90:127-149,252-261):
+ * T_bar and S_bar (either may be NULL) are overwritten on levels 1..kmax */
+int gt_eos_linear_adjoint(gt_ctx *ctx, const double *buoy_bar, double *T_bar,
+                          double *S_bar, double dtr0, double dsr0);
+
 /* ---- budgets (no reference counterpart) ---------------------------------
  * mass = sum over interior water cells (az==1), k=1..kmax of h*f*dx*dy and
  * volume = sum of h*dx*dy, accumulated error-free in double-double.  f may be

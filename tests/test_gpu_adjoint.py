@@ -1,0 +1,124 @@
+"""Reverse mode against central finite differences and the dot-product
+identity (FP64).  Bound: relative difference <= 1e-6 against central FD with
+eps = 1e-6, at seeded random wet cells."""
+import numpy as np
+import pytest
+import torch
+
+from code_amd import inputs
+from code_amd import transport as gt
+from tests.adjoint_harness import AdvectSteps, PressureFromT
+from tests.test_gpu_parity import random_domain, random_fields
+from tests.util import to_dev
+
+pytestmark = pytest.mark.gpu
+SCHEMES = [gt.UPSTREAM, gt.P2, gt.SUPERBEE, gt.MUSCL, gt.P2_PDM]
+
+
+def setup(device, imax=26, jmax=22, kmax=6, seed=3):
+    tr = inputs.config1(seed=seed, imax=imax, jmax=jmax, kmax=kmax)
+    d = tr.dom
+    ctx = gt.Context.from_domain(d)
+    fields = tuple(to_dev(a, device) for a in (tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho, tr.hn))
+    return tr, d, ctx, fields
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_dot_product_one_step(device, scheme):
+    """<lam, J dx> (central differences) == <J^T lam, dx>"""
+    dom = random_domain(21, 17, 5, 8, "islands")
+    F = random_fields(dom, 400)
+    ctx = gt.Context.from_domain(dom)
+    fields = tuple(to_dev(F[n], device) for n in ("uu", "vv", "ww", "hu", "hv", "ho", "hn"))
+    f0 = to_dev(F["f"], device)
+    rng = np.random.default_rng(scheme)
+    dx = torch.from_numpy(rng.standard_normal(dom.shape3)).to(device)
+    lam = torch.from_numpy(rng.standard_normal(dom.shape3)).to(device)
+    eps = 1e-7
+    fp = AdvectSteps.apply(f0 + eps * dx, ctx, fields, 1, 1, scheme, scheme, F["dt"])
+    fm = AdvectSteps.apply(f0 - eps * dx, ctx, fields, 1, 1, scheme, scheme, F["dt"])
+    lhs = float(torch.sum(lam * (fp - fm) / (2 * eps)))
+    f0r = f0.clone().requires_grad_(True)
+    out = AdvectSteps.apply(f0r, ctx, fields, 1, 1, scheme, scheme, F["dt"])
+    (g,) = torch.autograd.grad(out, f0r, lam)
+    rhs = float(torch.sum(g * dx))
+    assert abs(lhs - rhs) <= 1e-6 * abs(rhs), (scheme, lhs, rhs)
+
+
+def test_gradcheck_tiny_linear_schemes(device):
+    """torch.autograd.gradcheck on a tiny basin (full Jacobian by FD)"""
+    tr, d, ctx, fields = setup(device, imax=9, jmax=8, kmax=3, seed=5)
+    f0 = to_dev(tr.tracers["c"] + 0.1 * inputs.uniform(d.shape3, 3), device).requires_grad_(True)
+    for scheme in (gt.UPSTREAM, gt.P2):
+        fn = lambda f: AdvectSteps.apply(f, ctx, fields, 2, 1, scheme, scheme, tr.dt)  # noqa
+        assert torch.autograd.gradcheck(fn, (f0,), eps=1e-6, atol=1e-8, rtol=1e-6)
+
+
+def box_cost(f, box):
+    j0, j1, i0, i1 = box
+    return torch.sum(f[1:, j0:j1, i0:i1] ** 2)
+
+
+def test_multistep_checkpointed_gradient_vs_fd(device):
+    """J = sum over a target box of c^2 after 12 Superbee steps, checkpoint
+    every 4; dJ/dc0 at 16 random wet cells against central FD (eps=1e-6)"""
+    tr, d, ctx, fields = setup(device, imax=40, jmax=36, kmax=6, seed=4)
+    nsteps, ck = 12, 4
+    c0 = to_dev(tr.tracers["c"], device)
+    box = (18, 26, 22, 30)
+    c0r = c0.clone().requires_grad_(True)
+    out = AdvectSteps.apply(c0r, ctx, fields, nsteps, ck, gt.SUPERBEE, gt.SUPERBEE, tr.dt)
+    (g,) = torch.autograd.grad(box_cost(out, box), c0r)
+    rng = np.random.default_rng(11)
+    wet = np.argwhere(d.az == 1)
+    picks = wet[rng.choice(len(wet), 16, replace=False)]
+    eps = 1e-6
+    gh = g.cpu().numpy()
+    for (j, i) in picks:
+        k = int(rng.integers(1, d.kmax + 1))
+        e = torch.zeros_like(c0)
+        e[k, j, i] = eps
+        jp = box_cost(AdvectSteps.apply(c0 + e, ctx, fields, nsteps, ck, gt.SUPERBEE,
+                                        gt.SUPERBEE, tr.dt), box)
+        jm = box_cost(AdvectSteps.apply(c0 - e, ctx, fields, nsteps, ck, gt.SUPERBEE,
+                                        gt.SUPERBEE, tr.dt), box)
+        fd = float(jp - jm) / (2 * eps)
+        ad = gh[k, j, i]
+        scale = max(abs(fd), 1e-3 * float(np.abs(gh).max()))
+        assert abs(ad - fd) <= 1e-6 * scale + 1e-12, (k, j, i, ad, fd)
+
+
+def test_internal_pressure_adjoint_vs_fd(device):
+    """J = sum_box w * idpdx (Blumberg-Mellor of the linear-EOS buoyancy):
+    dJ/dT against central FD"""
+    tr = inputs.config2(seed=6, imax=30, jmax=28, kmax=7)
+    d = tr.dom
+    ctx = gt.Context.from_domain(d)
+    T = to_dev(tr.tracers["T"], device)
+    S = to_dev(tr.tracers["S"], device)
+    H, hn, hun, hvn = (to_dev(a, device) for a in (d.H, tr.hn, tr.hun, tr.hvn))
+    w = to_dev(inputs.uniform(d.shape3, 9, -1, 1), device)
+    wy = to_dev(inputs.uniform(d.shape3, 10, -1, 1), device)
+
+    def J(Tt):
+        ix, iy = PressureFromT.apply(Tt, S, ctx, H, hn, hun, hvn)
+        return torch.sum(w[1:, 8:20, 8:20] * ix[1:, 8:20, 8:20]) + \
+            torch.sum(wy[1:, 8:20, 8:20] * iy[1:, 8:20, 8:20])
+
+    Tr = T.clone().requires_grad_(True)
+    (g,) = torch.autograd.grad(J(Tr), Tr)
+    gh = g.cpu().numpy()
+    rng = np.random.default_rng(2)
+    eps = 1e-6
+    n = 0
+    for _ in range(40):
+        j, i = int(rng.integers(6, 22)), int(rng.integers(6, 22))
+        k = int(rng.integers(1, d.kmax + 1))
+        if d.az[j, i] != 1:
+            continue
+        e = torch.zeros_like(T)
+        e[k, j, i] = eps
+        fd = float(J(T + e) - J(T - e)) / (2 * eps)
+        assert abs(gh[k, j, i] - fd) <= 1e-6 * max(abs(fd), 1e-9), (k, j, i, gh[k, j, i], fd)
+        n += 1
+    assert n >= 10
