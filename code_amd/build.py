@@ -22,7 +22,8 @@ ORACLE_BUILD = ORACLE / "_build"
 LIB_NAME = "libgetm_transport.so"
 ORACLE_LIB = "libgetm_oracle.so"
 
-HIP_SOURCES = ["gt_kernels.hip", "gt_api.hip", "gt_batched.hip", "gt_adjoint.hip"]
+HIP_SOURCES = ["gt_kernels.hip", "gt_api.hip", "gt_batched_f64.hip", "gt_batched_f32_g1.hip",
+               "gt_batched_f32_g3.hip", "gt_batched_f32_g6.hip", "gt_adjoint.hip"]
 
 # -ffp-contract=off: no fused multiply-add, so FP64 results round exactly as
 # the CPU build's.  Packed FP32 instructions are disabled on purpose.
@@ -56,7 +57,7 @@ def build_hip(verbose=False) -> Path:
         _run([hipcc(), *HIPCC_FLAGS, "-c", str(s), "-o", str(o)])
         return o
 
-    with ThreadPoolExecutor(max_workers=min(4, len(srcs))) as ex:
+    with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         list(ex.map(comp, zip(srcs, objs)))
     out = BUILD / LIB_NAME
     tmp = BUILD / (LIB_NAME + ".tmp")

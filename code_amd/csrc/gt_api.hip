@@ -583,7 +583,7 @@ extern "C" int gt_do_advection_3d(gt_ctx* c, double dt, double* f, const double*
       return GT_OK;
     }
     case GT_HALFSPLIT: {
-      if (c->fused && itv <= 1 && kmax > 1) {
+      if (c->fused && itv <= 1 && kmax > 1 && gt::fused_supported(hscheme, vscheme)) {
         gt::FusedArgs<double> A{};
         fill_domain(c, g, &A.doms[0], f, tmp, 0, uu, vv, ww, hu, hv, ho, hires, advres, 0);
         A.ndom = 1;
@@ -736,8 +736,8 @@ extern "C" int gt_exchange_tiles(int px, int py, const int* ni, const int* nj, i
 extern "C" int gt_set_tuning(gt_ctx* c, int fused, int tile, int group) {
   if (!c) return set_err(GT_EARG, "gt_set_tuning: NULL context");
   if (tile < 0 || tile > 2) return set_err(GT_EARG, "gt_set_tuning: tile must be 0, 1 or 2");
-  if (group != 1 && group != 2 && group != 3 && group != 6)
-    return set_err(GT_EARG, "gt_set_tuning: group must be 1, 2, 3 or 6");
+  if (group != 1 && group != 3 && group != 6)
+    return set_err(GT_EARG, "gt_set_tuning: group must be 1, 3 or 6");
   c->fused = fused != 0;
   c->tile = tile;
   c->group = group;
@@ -751,6 +751,8 @@ static int batch_impl(int ndom, const gt_batch_domain* doms, int ntracers, doubl
     return set_err(GT_EARG, "gt_adv3d_batch: bad argument");
   if (!split_scheme(hscheme) || !split_scheme(vscheme))
     return set_err(GT_EINVAL_SCHEME, "gt_adv3d_batch: schemes must be UPSTREAM..P2_PDM");
+  if (!gt::fused_supported(hscheme, vscheme))
+    return set_err(GT_ENOTIMPL, "gt_adv3d_batch: hscheme and vscheme must be equal");
   const GridMasks* gsel = nullptr;
   for (int i = 0; i < ndom; i++) {
     gt_ctx* c = doms[i].ctx;

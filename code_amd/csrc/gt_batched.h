@@ -39,8 +39,10 @@ struct FusedArgs {
 
 // tile shapes: 0 -> 32x8, 1 -> 64x4, 2 -> 16x8
 void fused_tile_shape(int tile, int* TX, int* TY);
+// the fused kernel is instantiated for hscheme == vscheme in UPSTREAM..P2_PDM
+bool fused_supported(int hscheme, int vscheme);
 hipError_t launch_fused_f64(hipStream_t st, const FusedArgs<double>& a, int tile, bool adv);
-// group: tracers per workgroup (1, 2, 3 or 6)
+// group: tracers per workgroup (1, 3 or 6)
 hipError_t launch_fused_f32(hipStream_t st, const FusedArgs<float>& a, int tile, int group);
 
 }  // namespace gt

@@ -797,12 +797,27 @@ __global__ void k_budget_partial(Dims d, const int32_t* __restrict__ az,
 
 __global__ void k_budget_final(int nblocks, const double* __restrict__ partial, double area,
                                double* out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  // one block: a strided double-double sum per thread, then a tree
+  __shared__ double sm[4][256];
+  const int t = threadIdx.x;
   double mh = 0, ml = 0, vh = 0, vl = 0;
-  for (int b = 0; b < nblocks; b++) {
+  for (int b = t; b < nblocks; b += blockDim.x) {
     dd_add_dd(mh, ml, partial[4 * b], partial[4 * b + 1]);
     dd_add_dd(vh, vl, partial[4 * b + 2], partial[4 * b + 3]);
   }
+  sm[0][t] = mh; sm[1][t] = ml; sm[2][t] = vh; sm[3][t] = vl;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      double a = sm[0][t], b = sm[1][t], c = sm[2][t], e = sm[3][t];
+      dd_add_dd(a, b, sm[0][t + s], sm[1][t + s]);
+      dd_add_dd(c, e, sm[2][t + s], sm[3][t + s]);
+      sm[0][t] = a; sm[1][t] = b; sm[2][t] = c; sm[3][t] = e;
+    }
+    __syncthreads();
+  }
+  if (t != 0) return;
+  mh = sm[0][0]; ml = sm[1][0]; vh = sm[2][0]; vl = sm[3][0];
   double p = mh * area, pe = fma(mh, area, -p);
   pe += ml * area;
   out[0] = p + pe;
@@ -947,7 +962,7 @@ hipError_t launch_budget(hipStream_t st, Dims d, const int32_t* az, const double
   hipLaunchKernelGGL(k_budget_partial<TF>, dim3(nblocks), dim3(256), 0, st, d, az, h, f, partial);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_budget_final, dim3(1), dim3(64), 0, st, nblocks, partial, area, out);
+  hipLaunchKernelGGL(k_budget_final, dim3(1), dim3(256), 0, st, nblocks, partial, area, out);
   return hipGetLastError();
 }
 

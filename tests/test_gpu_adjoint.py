@@ -1,6 +1,13 @@
 """Reverse mode against central finite differences and the dot-product
 identity (FP64).  Bound: relative difference <= 1e-6 against central FD with
-eps = 1e-6, at seeded random wet cells."""
+eps = 1e-6, at seeded random wet cells.
+
+The limiters are piecewise linear: at a kink (r at a breakpoint, or
+deltaf*deltafu = 0) the central difference averages two one-sided slopes
+while the adjoint takes the branch the forward evaluation took.  A cell is
+therefore compared only where the forward and backward one-sided slopes
+agree (no kink within eps); the tests require most sampled cells to qualify
+and report how many did not."""
 import numpy as np
 import pytest
 import torch
@@ -27,7 +34,7 @@ def setup(device, imax=26, jmax=22, kmax=6, seed=3):
 def test_dot_product_one_step(device, scheme):
     """<lam, J dx> (central differences) == <J^T lam, dx>"""
     dom = random_domain(21, 17, 5, 8, "islands")
-    F = random_fields(dom, 400)
+    F = random_fields(dom, 400, plateaus=False)
     ctx = gt.Context.from_domain(dom)
     fields = tuple(to_dev(F[n], device) for n in ("uu", "vv", "ww", "hu", "hv", "ho", "hn"))
     f0 = to_dev(F["f"], device)
@@ -46,9 +53,13 @@ def test_dot_product_one_step(device, scheme):
 
 
 def test_gradcheck_tiny_linear_schemes(device):
-    """torch.autograd.gradcheck on a tiny basin (full Jacobian by FD)"""
-    tr, d, ctx, fields = setup(device, imax=9, jmax=8, kmax=3, seed=5)
-    f0 = to_dev(tr.tracers["c"] + 0.1 * inputs.uniform(d.shape3, 3), device).requires_grad_(True)
+    """torch.autograd.gradcheck on a tiny basin (full Jacobian by FD) for the
+    schemes without kinks"""
+    tr = inputs.config_advection(10, 9, 3, 5, walls=1)
+    d = tr.dom
+    ctx = gt.Context.from_domain(d)
+    fields = tuple(to_dev(a, device) for a in (tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho, tr.hn))
+    f0 = to_dev(inputs.uniform(d.shape3, 3), device).requires_grad_(True)
     for scheme in (gt.UPSTREAM, gt.P2):
         fn = lambda f: AdvectSteps.apply(f, ctx, fields, 2, 1, scheme, scheme, tr.dt)  # noqa
         assert torch.autograd.gradcheck(fn, (f0,), eps=1e-6, atol=1e-8, rtol=1e-6)
@@ -61,7 +72,11 @@ def box_cost(f, box):
 
 def test_multistep_checkpointed_gradient_vs_fd(device):
     """J = sum over a target box of c^2 after 12 Superbee steps, checkpoint
-    every 4; dJ/dc0 at 16 random wet cells against central FD (eps=1e-6)"""
+    every 4; dJ/dc0 at 64 random wet cells against central FD (eps=1e-6).
+    FD is taken of J's linearisation at the final state,
+    Jlin(c0) = sum lam * c_final(c0) with lam = 2 c_final in the box, which
+    has the same gradient at c0 and no curvature, so the one-sided slopes
+    differ only where a limiter kink lies within eps."""
     tr, d, ctx, fields = setup(device, imax=40, jmax=36, kmax=6, seed=4)
     nsteps, ck = 12, 4
     c0 = to_dev(tr.tracers["c"], device)
@@ -69,23 +84,52 @@ def test_multistep_checkpointed_gradient_vs_fd(device):
     c0r = c0.clone().requires_grad_(True)
     out = AdvectSteps.apply(c0r, ctx, fields, nsteps, ck, gt.SUPERBEE, gt.SUPERBEE, tr.dt)
     (g,) = torch.autograd.grad(box_cost(out, box), c0r)
+    lam = torch.zeros_like(c0)
+    j0, j1, i0, i1 = box
+    lam[1:, j0:j1, i0:i1] = 2.0 * out.detach()[1:, j0:j1, i0:i1]
+    run = lambda c: float(torch.sum(lam * AdvectSteps.apply(  # noqa: E731
+        c, ctx, fields, nsteps, ck, gt.SUPERBEE, gt.SUPERBEE, tr.dt)))
     rng = np.random.default_rng(11)
     wet = np.argwhere(d.az == 1)
-    picks = wet[rng.choice(len(wet), 16, replace=False)]
+    picks = wet[rng.choice(len(wet), 64, replace=False)]
     eps = 1e-6
     gh = g.cpu().numpy()
+    J0 = run(c0)
+    scale = float(np.abs(gh).max())
+    ok = kinks = 0
     for (j, i) in picks:
         k = int(rng.integers(1, d.kmax + 1))
         e = torch.zeros_like(c0)
         e[k, j, i] = eps
-        jp = box_cost(AdvectSteps.apply(c0 + e, ctx, fields, nsteps, ck, gt.SUPERBEE,
-                                        gt.SUPERBEE, tr.dt), box)
-        jm = box_cost(AdvectSteps.apply(c0 - e, ctx, fields, nsteps, ck, gt.SUPERBEE,
-                                        gt.SUPERBEE, tr.dt), box)
-        fd = float(jp - jm) / (2 * eps)
+        jp, jm = run(c0 + e), run(c0 - e)
+        fwd, bwd = (jp - J0) / eps, (J0 - jm) / eps
+        if abs(fwd - bwd) > 1e-6 * scale:
+            kinks += 1
+            continue
+        fd = (jp - jm) / (2 * eps)
         ad = gh[k, j, i]
-        scale = max(abs(fd), 1e-3 * float(np.abs(gh).max()))
-        assert abs(ad - fd) <= 1e-6 * scale + 1e-12, (k, j, i, ad, fd)
+        assert abs(ad - fd) <= 1e-6 * scale, (k, j, i, ad, fd)
+        ok += 1
+    print(f"adjoint vs FD: {ok} cells agree to 1e-6, {kinks} cells within eps of a kink")
+    assert ok >= 40
+
+
+def test_dot_product_smooth_superbee(device):
+    """dot-product identity on the smooth configs[0]-type blob"""
+    tr, d, ctx, fields = setup(device, imax=40, jmax=36, kmax=6, seed=4)
+    f0 = to_dev(tr.tracers["c"], device)
+    rng = np.random.default_rng(5)
+    dx = torch.from_numpy(rng.standard_normal(d.shape3)).to(device)
+    lam = torch.from_numpy(rng.standard_normal(d.shape3)).to(device)
+    eps = 1e-8
+    fp = AdvectSteps.apply(f0 + eps * dx, ctx, fields, 3, 2, gt.SUPERBEE, gt.SUPERBEE, tr.dt)
+    fm = AdvectSteps.apply(f0 - eps * dx, ctx, fields, 3, 2, gt.SUPERBEE, gt.SUPERBEE, tr.dt)
+    lhs = float(torch.sum(lam * (fp - fm) / (2 * eps)))
+    f0r = f0.clone().requires_grad_(True)
+    out = AdvectSteps.apply(f0r, ctx, fields, 3, 2, gt.SUPERBEE, gt.SUPERBEE, tr.dt)
+    (g,) = torch.autograd.grad(out, f0r, lam)
+    rhs = float(torch.sum(g * dx))
+    assert abs(lhs - rhs) <= 1e-5 * abs(rhs), (lhs, rhs)
 
 
 def test_internal_pressure_adjoint_vs_fd(device):

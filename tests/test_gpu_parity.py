@@ -37,12 +37,13 @@ def random_domain(imax, jmax, kmax, seed, kind):
     return inputs.Domain(imax, jmax, kmax, 500.0, 400.0, H, HU, HV, az, au, av, ax)
 
 
-def random_fields(dom, seed, cmax=0.9, wmax=None):
+def random_fields(dom, seed, cmax=0.9, wmax=None, plateaus=True):
     """random tracer, layer heights and transports with |CFL| < cmax"""
     sh = dom.shape3
     f = inputs.uniform(sh, seed + 1)
-    # a few plateaus so the limiter's deltaf*deltafu<=0 branches are hit
-    f[inputs.uniform(sh, seed + 2) < 0.1] = 0.5
+    if plateaus:
+        # a few plateaus so the limiter's deltaf*deltafu<=0 branches are hit
+        f[inputs.uniform(sh, seed + 2) < 0.1] = 0.5
     h = inputs.uniform(sh, seed + 3, 0.5, 2.0)
     hu = inputs.uniform(sh, seed + 4, 0.5, 2.0)
     hv = inputs.uniform(sh, seed + 5, 0.5, 2.0)
