@@ -74,7 +74,7 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the transport "
-              "step's multi-GPU decomposition is described in DESIGN.md and not run here.",
+              "step is measured on one GPU's subdomain; several GPUs are not run here.",
               file=sys.stderr)
         sys.exit(2)
     import numpy as np
@@ -87,7 +87,7 @@ def main():
     from code_amd import transport as gt
 
     dev = torch.device("cuda:0")
-    tr = inputs.config2(imax=a.imax, jmax=a.jmax, kmax=a.kmax)
+    tr = inputs.ts_case(imax=a.imax, jmax=a.jmax, kmax=a.kmax)
     d = tr.dom
     st = torch.cuda.Stream(device=dev)
     ctx = gt.Context.from_domain(d, stream=st)
@@ -185,7 +185,7 @@ class _Null:
 
 def batched(torch, np, gt, inputs, dev, a, fp64_rate):
     """18 tracers on 8 uneven subdomains, one fused FP32 launch per step"""
-    subs = inputs.config3(npassive=16)
+    subs = inputs.multitracer_case(npassive=16)
     names = list(subs[0].tracers)
     nt = len(names)
     st = torch.cuda.Stream(device=dev)

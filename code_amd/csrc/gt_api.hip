@@ -803,6 +803,69 @@ extern "C" int gt_adv3d_batch_f64(int ndom, const gt_batch_domain* doms, int ntr
   return batch_impl<double>(ndom, doms, ntracers, dt, hscheme, vscheme, AH, tag, stream);
 }
 
+// ---- autotuner (extension) ------------------------------------------------------
+// Times the batched step for every tile shape x tracers-per-workgroup pair
+// (reps launches each, after one warm-up) with HIP events on the stream and
+// keeps the fastest on every context of the batch.  Outputs are written to
+// f_out (f_in is not modified), so the sweep can run on live buffers.
+extern "C" int gt_autotune_batch(int ndom, const gt_batch_domain* doms, int ntracers, double dt,
+                                 int hscheme, int vscheme, double AH, int tag, int f32, int reps,
+                                 void* stream, int* best_tile, int* best_group, float* best_ms,
+                                 float* table) {
+  if (ndom < 1 || !doms || !doms[0].ctx || reps < 1)
+    return set_err(GT_EARG, "gt_autotune_batch: bad argument");
+  hipStream_t st = stream ? (hipStream_t)stream : doms[0].ctx->st;
+  const int tiles[3] = {0, 1, 2};
+  const int groups[3] = {1, 3, 6};
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  float best = 1e30f;
+  int bt = 0, bg = 3;
+  int n = 0;
+  for (int ti = 0; ti < 3; ti++)
+    for (int gi = 0; gi < (f32 ? 3 : 1); gi++) {
+      for (int i = 0; i < ndom; i++) {
+        doms[i].ctx->tile = tiles[ti];
+        doms[i].ctx->group = groups[gi];
+      }
+      auto run = [&]() -> int {
+        return f32 ? gt_adv3d_batch_f32(ndom, doms, ntracers, dt, hscheme, vscheme, AH, tag, st)
+                   : gt_adv3d_batch_f64(ndom, doms, ntracers, dt, hscheme, vscheme, AH, tag, st);
+      };
+      int rc = run();
+      if (rc != GT_OK) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return rc;
+      }
+      HIPCHK(hipEventRecord(e0, st));
+      for (int r = 0; r < reps; r++) TRY(run());
+      HIPCHK(hipEventRecord(e1, st));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      ms /= reps;
+      if (table) table[n] = ms;
+      n++;
+      if (ms < best) {
+        best = ms;
+        bt = tiles[ti];
+        bg = groups[gi];
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (int i = 0; i < ndom; i++) {
+    doms[i].ctx->tile = bt;
+    doms[i].ctx->group = bg;
+  }
+  if (best_tile) *best_tile = bt;
+  if (best_group) *best_group = bg;
+  if (best_ms) *best_ms = best;
+  return GT_OK;
+}
+
 // ---- reverse mode (extension) ------------------------------------------------
 // work fields: D0..D5 (layer heights through the five sub-steps), S0..S4
 // (tracer states before each sub-step), two adjoint buffers, then the

@@ -285,7 +285,7 @@ def config_advection(imax, jmax, kmax, seed, *, hmin=10.0, hmax=40.0,
     return Transport(dom, dt, uu, vv, ww, hun, hvn, ho, hn, {})
 
 
-def config1(seed=1, imax=100, jmax=100, kmax=20) -> Transport:
+def blob_case(seed=1, imax=100, jmax=100, kmax=20) -> Transport:
     """Plain FP64 split advection of one Gaussian tracer on sigma layers with
     solid-body-like rotation (max 0.5 m/s), dx=dy=500 m, dt=30 s."""
     tr = config_advection(imax, jmax, kmax, seed, hmin=10.0, hmax=40.0,
@@ -325,7 +325,7 @@ def temperature_salinity(tr: Transport, seed):
     return T, S
 
 
-def config2(seed=2, imax=400, jmax=400, kmax=40) -> Transport:
+def ts_case(seed=2, imax=400, jmax=400, kmax=40) -> Transport:
     """FP64 transport step on GVC layers: T (5-20 C, linear in depth plus
     noise) and S (5-30 psu, estuarine gradient), P2-PDM advection,
     divergence-free streamfunction flow with max 1 m/s."""
@@ -349,13 +349,13 @@ def passive_blobs(dom: Domain, n, seed):
     return out
 
 
-C3_SUBDOMAINS = [(64, 64), (96, 96), (128, 128), (192, 128), (256, 192),
+MULTITRACER_SUBDOMAINS = [(64, 64), (96, 96), (128, 128), (192, 128), (256, 192),
                  (320, 256), (384, 320), (512, 384)]
 
 
-def config3(seed=3, kmax=30, sizes=None, npassive=16):
+def multitracer_case(seed=3, kmax=30, sizes=None, npassive=16):
     """T, S and ``npassive`` passive tracers on uneven subdomains."""
-    sizes = C3_SUBDOMAINS if sizes is None else sizes
+    sizes = MULTITRACER_SUBDOMAINS if sizes is None else sizes
     subs = []
     for s, (ni, nj) in enumerate(sizes):
         tr = config_advection(ni, nj, kmax, seed * 100 + s, hmin=10.0,

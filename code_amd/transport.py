@@ -41,7 +41,7 @@ EXPORTS = [
     "gt_exchange_tiles", "gt_budget", "gt_malloc", "gt_free",
     "gt_memcpy_h2d", "gt_memcpy_d2h", "gt_adv3d_batch_f32", "gt_adv3d_batch_f64",
     "gt_set_tuning", "gt_budget_f32", "gt_adv3d_adjoint",
-    "gt_ip_blumberg_mellor_adjoint", "gt_eos_linear_adjoint",
+    "gt_ip_blumberg_mellor_adjoint", "gt_eos_linear_adjoint", "gt_autotune_batch",
 ]
 
 
@@ -115,6 +115,9 @@ def load_library():
                                  d, i]),
         "gt_ip_blumberg_mellor_adjoint": (i, [vp, vp, vp, vp, vp, vp, vp]),
         "gt_eos_linear_adjoint": (i, [vp, vp, vp, vp, d, d]),
+        "gt_autotune_batch": (i, [i, C.POINTER(BatchDomain), i, d, i, i, d, i, i, i, vp,
+                                  C.POINTER(i), C.POINTER(i), C.POINTER(C.c_float),
+                                  C.POINTER(C.c_float)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -304,6 +307,36 @@ def exchange_tiles(px, py, ni, nj, nk, tiles, stream=None):
                                 C.c_void_p(handle) if handle else None))
 
 
+def _batch_array(domains):
+    n = len(domains)
+    arr = (BatchDomain * n)()
+    for q, d in enumerate(domains):
+        arr[q] = BatchDomain(d["ctx"].h.value, d["f_in"].data_ptr(), d["f_out"].data_ptr(),
+                             *(d[k].data_ptr() for k in ("uu", "vv", "ww", "hu", "hv", "ho")),
+                             d["hires"].data_ptr() if d.get("hires") is not None else None)
+    return arr
+
+
+TILES = {0: "32x8", 1: "64x4", 2: "16x8"}
+
+
+def autotune_batch(domains, ntracers, dt, hscheme, vscheme, AH=-1.0, tag=H_TAG, reps=3,
+                   stream=None):
+    """gt_autotune_batch: returns (tile, group, best_ms, {(tile, group): ms})"""
+    import torch
+    lib = load_library()
+    arr = _batch_array(domains)
+    f32 = domains[0]["f_in"].dtype == torch.float32
+    bt, bg, bm = C.c_int(), C.c_int(), C.c_float()
+    table = (C.c_float * 9)()
+    handle = getattr(stream, "cuda_stream", stream)
+    check(lib.gt_autotune_batch(len(domains), arr, ntracers, dt, hscheme, vscheme, AH, tag,
+                                int(f32), reps, C.c_void_p(handle) if handle else None,
+                                C.byref(bt), C.byref(bg), C.byref(bm), table))
+    pairs = [(t, g) for t in (0, 1, 2) for g in ((1, 3, 6) if f32 else (1,))]
+    return bt.value, bg.value, bm.value, {p: table[i] for i, p in enumerate(pairs)}
+
+
 def adv3d_batch(domains, ntracers, dt, hscheme, vscheme, AH=-1.0, tag=H_TAG, stream=None):
     """gt_adv3d_batch_f32/_f64.  domains: list of dicts with keys ctx, f_in,
     f_out (tensors of shape (ntracers, kmax+1, NY, NX)), uu, vv, ww, hu, hv,
@@ -311,11 +344,7 @@ def adv3d_batch(domains, ntracers, dt, hscheme, vscheme, AH=-1.0, tag=H_TAG, str
     import torch
     lib = load_library()
     n = len(domains)
-    arr = (BatchDomain * n)()
-    for q, d in enumerate(domains):
-        arr[q] = BatchDomain(d["ctx"].h.value, d["f_in"].data_ptr(), d["f_out"].data_ptr(),
-                             *(d[k].data_ptr() for k in ("uu", "vv", "ww", "hu", "hv", "ho")),
-                             d["hires"].data_ptr() if d.get("hires") is not None else None)
+    arr = _batch_array(domains)
     f32 = domains[0]["f_in"].dtype == torch.float32
     fn = lib.gt_adv3d_batch_f32 if f32 else lib.gt_adv3d_batch_f64
     handle = getattr(stream, "cuda_stream", stream)

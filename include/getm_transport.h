@@ -193,6 +193,15 @@ int gt_adv3d_batch_f64(int ndom, const gt_batch_domain *doms, int ntracers,
 /* kernel choice: fused (1) or per-sub-step (0) HALFSPLIT, fused tile shape
  * (0: 32x8, 1: 64x4, 2: 16x8) and tracers per workgroup (1, 2, 3, 6) */
 int gt_set_tuning(gt_ctx *ctx, int fused, int tile, int group);
+/* autotuner: times gt_adv3d_batch_f32 (f32=1) or _f64 over tile shapes
+ * {32x8, 64x4, 16x8} x tracers per workgroup {1, 3, 6} (FP64: tiles only),
+ * reps timed launches each, keeps the fastest on every context.  table
+ * (optional) receives the mean ms of each pair in sweep order.  f_out is
+ * overwritten, f_in is not touched. */
+int gt_autotune_batch(int ndom, const gt_batch_domain *doms, int ntracers,
+                      double dt, int hscheme, int vscheme, double AH, int tag,
+                      int f32, int reps, void *hip_stream, int *best_tile,
+                      int *best_group, float *best_ms, float *table);
 
 /* ---- reverse mode (no reference counterpart) -----------------------------
  * Adjoint of nsteps HALFSPLIT steps of do_advection_3d (advection_3d.F90:

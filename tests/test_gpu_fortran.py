@@ -19,7 +19,7 @@ def test_fortran_driver(tmp_path, device, split, hs, vs):
     exe = build.BUILD / "fortran" / "adv_driver"
     if not exe.exists():
         build.build_fortran()
-    tr = inputs.config1(imax=34, jmax=30, kmax=7)
+    tr = inputs.blob_case(imax=34, jmax=30, kmax=7)
     d = tr.dom
     nsteps = 4
     f = tr.tracers["c"].copy()

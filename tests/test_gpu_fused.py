@@ -60,7 +60,7 @@ def test_fused_f64_bit_exact(device, kind):
 
 
 def test_batch_f64_two_tracers_bit_exact(device):
-    tr = inputs.config2(seed=4, imax=45, jmax=38, kmax=9)
+    tr = inputs.ts_case(seed=4, imax=45, jmax=38, kmax=9)
     d = tr.dom
     ctx = gt.Context.from_domain(d)
     orc = po.Oracle.from_domain(d)
@@ -90,7 +90,7 @@ def test_batch_f32_multidomain(device):
     """T, S and passive tracers on uneven subdomains in one launch, FP32
     against the FP64 oracle over several P2-PDM steps."""
     sizes = [(24, 20), (41, 33), (30, 52)]
-    subs = inputs.config3(seed=9, kmax=8, sizes=sizes, npassive=4)
+    subs = inputs.multitracer_case(seed=9, kmax=8, sizes=sizes, npassive=4)
     names = list(subs[0].tracers)
     nt = len(names)
     doms, refs, oracles, tins, touts, ctxs = [], [], [], [], [], []

@@ -231,8 +231,9 @@ def test_do_advection_3d_all_schemes(case, split):
 
 
 def test_config1_trajectory_bit_exact(device):
-    """100x100x20 sigma, HALFSPLIT Superbee, 100 steps."""
-    tr = inputs.config1()
+    """The Gaussian-blob case at full size: 100x100x20 sigma, HALFSPLIT
+    Superbee, 100 steps."""
+    tr = inputs.blob_case()
     d = tr.dom
     ctx = gt.Context.from_domain(d)
     orc = po.Oracle.from_domain(d)
@@ -253,7 +254,7 @@ def test_config1_trajectory_bit_exact(device):
 
 
 def ip_fields(seed=2, imax=30, jmax=26, kmax=9):
-    tr = inputs.config2(seed=seed, imax=imax, jmax=jmax, kmax=kmax)
+    tr = inputs.ts_case(seed=seed, imax=imax, jmax=jmax, kmax=kmax)
     d = tr.dom
     T, S = tr.tracers["T"], tr.tracers["S"]
     sseo = inputs.uniform((d.NY, d.NX), seed + 40, -0.2, 0.2)
@@ -342,11 +343,11 @@ def test_budget(case):
 
 
 def test_config2_full_size_properties(device):
-    """configs[1] sizes (400x400x40 GVC, P2-PDM): one step conserves tracer
+    """The T/S case at full size (400x400x40 GVC, P2-PDM): one step conserves tracer
     mass to round-off, keeps a constant field constant, and ends the volume
     trajectory on hn."""
     import torch
-    tr = inputs.config2()
+    tr = inputs.ts_case()
     d = tr.dom
     ctx = gt.Context.from_domain(d)
     dn = [to_dev(a, device) for a in (tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho, tr.hn)]

@@ -294,7 +294,7 @@ def test_upstream_2dh_bounds_and_textbook():
 
 # ---------------------------------------------------------- composition --
 def test_halfsplit_is_the_five_substeps_and_conserves():
-    tr = inputs.config1(imax=40, jmax=36, kmax=6)
+    tr = inputs.blob_case(imax=40, jmax=36, kmax=6)
     d = tr.dom
     orc = po.Oracle.from_domain(d)
     f = tr.tracers["c"].copy()
