@@ -41,6 +41,7 @@ static int set_err(int code, const char* fmt, ...) {
 struct GridMasks {
   const uint8_t *uflux, *vflux, *uupd, *vupd, *fin;
   const int32_t* az;
+  const uint8_t* flags;  // the five masks packed as gt::MB_* bits
 };
 
 struct gt_ctx {
@@ -60,13 +61,28 @@ struct gt_ctx {
   unsigned long long* nwarn = nullptr;
   int adv_ver_iterations = 1;
   int do_mirror_2d = 1;
-  int fused = 1;        // use the one-pass HALFSPLIT kernel where it applies
-  int tile = 0;         // fused tile shape (0: 32x8, 1: 64x4, 2: 16x8)
-  int group = 3;        // tracers per workgroup of the FP32 batched kernel
+  int fused = 1;        // use the HALFSPLIT strip kernels where they apply
+  int tile = 0;         // rows per strip (0: 16, 1: 32, 2: 8)
+  int group = 3;        // tracers per wavefront of the FP32 batched kernels
+  void* ws = nullptr;   // work fields of the strip kernels (grown on demand)
+  size_t ws_bytes = 0;
 };
 
 extern "C" const char* gt_last_error(void) { return g_err; }
 extern "C" int gt_version(void) { return 1; }
+
+static int ensure_ws(gt_ctx* c, size_t bytes) {
+  if (c->ws_bytes >= bytes) return GT_OK;
+  if (c->ws) {
+    HIPCHK(hipDeviceSynchronize());  // the old fields may still be in use on any stream
+    HIPCHK(hipFree(c->ws));
+    c->ws = nullptr;
+    c->ws_bytes = 0;
+  }
+  HIPCHK(hipMalloc(&c->ws, bytes));
+  c->ws_bytes = bytes;
+  return GT_OK;
+}
 
 static int ensure(double*& p, long n) {
   if (p) return GT_OK;
@@ -138,7 +154,7 @@ extern "C" int gt_create(gt_ctx** out, int imin, int imax, int jmin, int jmax, i
   memcpy(hax, ax ? ax : dax.data(), sizeof(int32_t) * NXY);
 
   // init_advection, advection.F90:263-318
-  std::vector<uint8_t> m(8 * NXY, 0);
+  std::vector<uint8_t> m(11 * NXY, 0);
   uint8_t *updH = &m[0], *uflux = &m[NXY], *vflux = &m[2 * NXY], *xflux = &m[3 * NXY],
           *uupdU = &m[4 * NXY], *vupdV = &m[5 * NXY], *ufluxU = &m[6 * NXY],
           *vfluxV = &m[7 * NXY];
@@ -156,6 +172,19 @@ extern "C" int gt_create(gt_ctx** out, int imin, int imax, int jmin, int jmax, i
     for (long i = 0; i < NX - 1; i++) ufluxU[i + NX * j] = updH[i + 1 + NX * j];
   for (long j = 0; j < NY - 1; j++)
     for (long i = 0; i < NX; i++) vfluxV[i + NX * j] = updH[i + NX * (j + 1)];
+  // the same masks packed one byte per cell for the strip kernels
+  uint8_t *flH = &m[8 * NXY], *flU = &m[9 * NXY], *flV = &m[10 * NXY];
+  for (long n = 0; n < NXY; n++) {
+    flH[n] = (uint8_t)(uflux[n] * gt::MB_UFLUX | vflux[n] * gt::MB_VFLUX |
+                       updH[n] * gt::MB_UUPD | updH[n] * gt::MB_VUPD |
+                       (haz[n] == 1) * gt::MB_WCOL);
+    flU[n] = (uint8_t)(ufluxU[n] * gt::MB_UFLUX | xflux[n] * gt::MB_VFLUX |
+                       uupdU[n] * gt::MB_UUPD | uflux[n] * gt::MB_VUPD |
+                       (hau[n] == 1) * gt::MB_WCOL);
+    flV[n] = (uint8_t)(xflux[n] * gt::MB_UFLUX | vfluxV[n] * gt::MB_VFLUX |
+                       vflux[n] * gt::MB_UUPD | vupdV[n] * gt::MB_VUPD |
+                       (hav[n] == 1) * gt::MB_WCOL);
+  }
 
   hipError_t e = hipMalloc(&c->dmask, m.size());
   if (e == hipSuccess) e = hipMalloc(&c->dint, sizeof(int32_t) * 4 * NXY);
@@ -172,9 +201,10 @@ extern "C" int gt_create(gt_ctx** out, int imin, int imax, int jmin, int jmax, i
   }
   const uint8_t* dm = c->dmask;
   c->az = c->dint; c->au = c->dint + NXY; c->av = c->dint + 2 * NXY; c->ax = c->dint + 3 * NXY;
-  c->H = {dm + NXY, dm + 2 * NXY, dm, dm, dm, c->az};
-  c->U = {dm + 6 * NXY, dm + 3 * NXY, dm + 4 * NXY, dm + NXY, dm + NXY, c->au};
-  c->V = {dm + 3 * NXY, dm + 7 * NXY, dm + 2 * NXY, dm + 5 * NXY, dm + 2 * NXY, c->av};
+  c->H = {dm + NXY, dm + 2 * NXY, dm, dm, dm, c->az, dm + 8 * NXY};
+  c->U = {dm + 6 * NXY, dm + 3 * NXY, dm + 4 * NXY, dm + NXY, dm + NXY, c->au, dm + 9 * NXY};
+  c->V = {dm + 3 * NXY, dm + 7 * NXY, dm + 2 * NXY, dm + 5 * NXY, dm + 2 * NXY, c->av,
+          dm + 10 * NXY};
   *out = c;
   return GT_OK;
 }
@@ -182,7 +212,7 @@ extern "C" int gt_create(gt_ctx** out, int imin, int imax, int jmin, int jmax, i
 extern "C" int gt_destroy(gt_ctx* c) {
   if (!c) return GT_OK;
   void* ps[] = {c->dmask, c->dint, c->fi, c->hi, c->adv, c->tmp, c->scr7,
-                c->zz, c->P, c->slice3, c->bpart, c->bout, c->nwarn};
+                c->zz, c->P, c->slice3, c->bpart, c->bout, c->nwarn, c->ws};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   for (double* p : c->adj)
@@ -363,24 +393,29 @@ static int check_2d(int split, int scheme) {
   }
 }
 
-// descriptor of one domain for the fused kernel
+// descriptor of one domain for the strip kernels; ws holds the work fields
+// f2 (ntracers fields of TF), d2 and, with advres, a2
 template <typename TF>
 static void fill_domain(gt_ctx* c, const GridMasks* g, gt::FusedDomain<TF>* D, const TF* fin,
-                        TF* fout, long tstride, const double* uu, const double* vv,
+                        TF* fout, long tstride, int ntracers, const double* uu, const double* vv,
                         const double* ww, const double* hu, const double* hv, const double* ho,
-                        double* hires, double* advres, long tile_begin) {
-  int TX, TY;
-  gt::fused_tile_shape(c->tile, &TX, &TY);
+                        double* hires, double* advres) {
   D->d = c->d;
-  D->ntx = (c->d.NX + TX - 1) / TX;
-  D->nty = (c->d.NY + TY - 1) / TY;
-  D->tile_begin = tile_begin;
   D->fin = fin; D->fout = fout; D->tstride = tstride;
+  char* w = (char*)c->ws;
+  D->d2 = (double*)w;
+  D->a2 = advres ? (double*)w + c->N3 : nullptr;
+  D->f2 = (TF*)((double*)w + (advres ? 2 : 1) * c->N3);
+  (void)ntracers;
   D->uu = uu; D->vv = vv; D->ww = ww; D->hu = hu; D->hv = hv; D->ho = ho;
-  D->m_uflux = g->uflux; D->m_vflux = g->vflux; D->m_uupd = g->uupd; D->m_vupd = g->vupd;
-  D->az = g->az;
+  D->flags = g->flags;
   D->hires = hires; D->advres = advres;
   D->dx = c->dx; D->dy = c->dy; D->ard1 = c->ard1;
+}
+
+// bytes of work fields a domain needs for fill_domain
+static size_t strip_ws_bytes(const gt_ctx* c, size_t tf_bytes, int ntracers, bool adv) {
+  return sizeof(double) * (size_t)c->N3 * (adv ? 2 : 1) + tf_bytes * (size_t)c->N3 * ntracers;
 }
 
 // ---- advection entry points --------------------------------------------------
@@ -584,15 +619,19 @@ extern "C" int gt_do_advection_3d(gt_ctx* c, double dt, double* f, const double*
     }
     case GT_HALFSPLIT: {
       if (c->fused && itv <= 1 && kmax > 1 && gt::fused_supported(hscheme, vscheme)) {
+        // f is read at level 0 only by the second kernel, which writes the
+        // same values there, so the result goes to f directly
+        TRY(ensure_ws(c, strip_ws_bytes(c, sizeof(double), 1, advres != nullptr)));
         gt::FusedArgs<double> A{};
-        fill_domain(c, g, &A.doms[0], f, tmp, 0, uu, vv, ww, hu, hv, ho, hires, advres, 0);
+        fill_domain<double>(c, g, &A.doms[0], f, f, 0, 1, uu, vv, ww, hu, hv, ho, hires, advres);
         A.ndom = 1;
-        A.ntiles = A.doms[0].ntx * (long)A.doms[0].nty;
         A.ntracers = 1;
+        A.rows = gt::strip_rows(c->tile);
         A.dt = dt; A.AH = AH;
         A.hscheme = hscheme; A.vscheme = vscheme; A.kshift = tag == GT_W_TAG ? 1 : 0;
-        LCHK(gt::launch_fused_f64(c->st, A, c->tile, advres != nullptr));
-        return copy_d2d(c, f, tmp, N3);
+        gt::strip_plan(A, 1);
+        LCHK(gt::launch_fused_f64(c->st, A, 1, advres != nullptr));
+        return GT_OK;
       }
       PingPong pp{f, f, tmp};
       TRY(step_u(c, g, 1, kmax, pp.cur, pp.cur, pp.oth, phi, padv, uu, hu, dt, 0.5, hscheme, AH));
@@ -735,7 +774,8 @@ extern "C" int gt_exchange_tiles(int px, int py, const int* ni, const int* nj, i
 // ---- batched multi-tracer step (extension) ----------------------------------
 extern "C" int gt_set_tuning(gt_ctx* c, int fused, int tile, int group) {
   if (!c) return set_err(GT_EARG, "gt_set_tuning: NULL context");
-  if (tile < 0 || tile > 2) return set_err(GT_EARG, "gt_set_tuning: tile must be 0, 1 or 2");
+  if (tile < 0 || tile > 2)
+    return set_err(GT_EARG, "gt_set_tuning: tile (rows per strip 16, 32, 8) must be 0, 1 or 2");
   if (group != 1 && group != 3 && group != 6)
     return set_err(GT_EARG, "gt_set_tuning: group must be 1, 3 or 6");
   c->fused = fused != 0;
@@ -765,27 +805,32 @@ static int batch_impl(int ndom, const gt_batch_domain* doms, int ntracers, doubl
   }
   gt_ctx* c0 = doms[0].ctx;
   hipStream_t st = stream ? (hipStream_t)stream : c0->st;
+  // FP64: two tracers per wavefront (T and S) when there are two or more
+  const int group = sizeof(TF) == 4 ? c0->group : (ntracers >= 2 ? 2 : 1);
+  for (int i = 0; i < ndom; i++) {
+    gt_ctx* c = doms[i].ctx;
+    TRY(ensure_ws(c, strip_ws_bytes(c, sizeof(TF), ntracers, false)));
+  }
   for (int b = 0; b < ndom; b += gt::kMaxFusedDomains) {
     gt::FusedArgs<TF> A{};
-    long tiles = 0;
     int nd = ndom - b < gt::kMaxFusedDomains ? ndom - b : gt::kMaxFusedDomains;
     for (int i = 0; i < nd; i++) {
       const gt_batch_domain& bd = doms[b + i];
       gt_ctx* c = bd.ctx;
-      c->tile = c0->tile;
       fill_domain<TF>(c, grid_for(c, tag), &A.doms[i], (const TF*)bd.f_in, (TF*)bd.f_out,
-                      c->N3, bd.uu, bd.vv, bd.ww, bd.hu, bd.hv, bd.ho, bd.hires, nullptr, tiles);
-      tiles += A.doms[i].ntx * (long)A.doms[i].nty;
+                      c->N3, ntracers, bd.uu, bd.vv, bd.ww, bd.hu, bd.hv, bd.ho, bd.hires,
+                      nullptr);
     }
     A.ndom = nd;
-    A.ntiles = tiles;
     A.ntracers = ntracers;
+    A.rows = gt::strip_rows(c0->tile);
     A.dt = dt; A.AH = AH;
     A.hscheme = hscheme; A.vscheme = vscheme; A.kshift = tag == GT_W_TAG ? 1 : 0;
+    gt::strip_plan(A, group);
     if constexpr (sizeof(TF) == 4) {
-      LCHK(gt::launch_fused_f32(st, A, c0->tile, c0->group));
+      LCHK(gt::launch_fused_f32(st, A, group));
     } else {
-      LCHK(gt::launch_fused_f64(st, A, c0->tile, false));
+      LCHK(gt::launch_fused_f64(st, A, group, false));
     }
   }
   return GT_OK;
@@ -804,7 +849,7 @@ extern "C" int gt_adv3d_batch_f64(int ndom, const gt_batch_domain* doms, int ntr
 }
 
 // ---- autotuner (extension) ------------------------------------------------------
-// Times the batched step for every tile shape x tracers-per-workgroup pair
+// Times the batched step for every rows-per-strip x tracers-per-wavefront pair
 // (reps launches each, after one warm-up) with HIP events on the stream and
 // keeps the fastest on every context of the batch.  Outputs are written to
 // f_out (f_in is not modified), so the sweep can run on live buffers.

@@ -1,25 +1,37 @@
-// gt_batched.h -- the fused HALFSPLIT transport kernel (gt_batched.hip).
+// gt_batched.h -- the HALFSPLIT transport step as two strip kernels
+// (gt_strip_impl.h), used by the single-tracer FP64 path and the batched
+// multi-tracer FP32/FP64 entry points.
 #pragma once
 
 #include "gt_device.h"
 
 namespace gt {
 
-// one domain of a fused launch.  A launch covers the tile tables of one or
-// more domains (the uneven subdomains of a batched run) back to back.
+// columns a strip writes: a wavefront covers 64 columns, the outer two on each
+// side are the ghost columns of the two consecutive i-direction sub-steps
+constexpr int kStripOut = 60;
+
+// packed per-cell mask bits (one byte per cell, per mask set H/U/V)
+enum : uint8_t { MB_UFLUX = 1, MB_VFLUX = 2, MB_UUPD = 4, MB_VUPD = 8, MB_WCOL = 16 };
+
+// one domain of a launch.  A launch covers the work items of one or more
+// domains (the uneven subdomains of a batched run) back to back.  A work item
+// is one strip (60 columns x `rows` rows) at one level for one tracer group.
 template <typename TF>
 struct FusedDomain {
   Dims d;
-  int ntx, nty;       // tiles along i and j
-  long tile_begin;    // first launch-wide tile index of this domain
-  const TF* fin;   // tracer t of this domain starts at fin + t*tstride
-  TF* fout;        // output, same layout; must not overlap fin
+  int nsx, nsy;       // strips along i and j
+  long item_begin;    // first launch-wide work item of this domain
+  const TF* fin;      // tracer t of this domain starts at fin + t*tstride
+  TF* fout;           // output, same layout (may equal fin)
   long tstride;
+  TF* f2;             // work: tracers after u/2 + v/2, same layout as fin
+  double* d2;         // work: layer heights after u/2 + v/2 (one field)
+  double* a2;         // work (ADV): advective term accumulated over u/2 + v/2
   const double *uu, *vv, *ww, *hu, *hv, *ho;
-  const uint8_t *m_uflux, *m_vflux, *m_uupd, *m_vupd;
-  const int32_t* az;
-  double* hires;   // optional: end of the layer-height trajectory
-  double* advres;  // optional (FP64, one tracer): accumulated advective term
+  const uint8_t* flags;  // MB_* bits of the mask set of the grid
+  double* hires;      // optional: end of the layer-height trajectory
+  double* advres;     // optional (FP64, one tracer): accumulated advective term
   double dx, dy, ard1;
 };
 
@@ -30,19 +42,24 @@ template <typename TF>
 struct FusedArgs {
   FusedDomain<TF> doms[kMaxFusedDomains];
   int ndom;
-  long ntiles;     // tiles over all domains
-  int ngroups;     // tracer groups (set by the launcher)
+  long nitems;     // work items over all domains
+  int ngroups;     // tracer groups
   int ntracers;
+  int rows;        // rows per strip
   double dt, AH;
   int hscheme, vscheme, kshift;
 };
 
-// tile shapes: 0 -> 32x8, 1 -> 64x4, 2 -> 16x8
-void fused_tile_shape(int tile, int* TX, int* TY);
-// the fused kernel is instantiated for hscheme == vscheme in UPSTREAM..P2_PDM
+// rows per strip for a tuning index: 0 -> 16, 1 -> 32, 2 -> 8
+int strip_rows(int tile);
+// the strip kernels are instantiated for hscheme == vscheme in UPSTREAM..P2_PDM
 bool fused_supported(int hscheme, int vscheme);
-hipError_t launch_fused_f64(hipStream_t st, const FusedArgs<double>& a, int tile, bool adv);
-// group: tracers per workgroup (1, 3 or 6)
-hipError_t launch_fused_f32(hipStream_t st, const FusedArgs<float>& a, int tile, int group);
+// group: tracers per wavefront.  FP64: 1 or 2 (adv needs 1); FP32: 1, 3 or 6.
+// The caller fills everything but ngroups and the item ranges, which
+// strip_plan sets.
+template <typename TF>
+void strip_plan(FusedArgs<TF>& a, int group);
+hipError_t launch_fused_f64(hipStream_t st, const FusedArgs<double>& a, int group, bool adv);
+hipError_t launch_fused_f32(hipStream_t st, const FusedArgs<float>& a, int group);
 
 }  // namespace gt

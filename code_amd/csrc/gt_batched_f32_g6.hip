@@ -1,15 +1,11 @@
-// gt_batched_f32_g6.hip -- FP32 instantiations of the fused HALFSPLIT kernel
-// (gt_fused_impl.h) with 6 tracer(s) per workgroup.
-#include "gt_fused_impl.h"
+// gt_batched_f32_g6.hip -- FP32 instantiations of the HALFSPLIT strip
+// kernels (gt_strip_impl.h) with 6 tracer(s) per wavefront.
+#include "gt_strip_impl.h"
 
 namespace gt {
 
-hipError_t launch_fused_f32_g6(hipStream_t st, const FusedArgs<float>& a, int tile) {
-  switch (tile) {
-    case 1: return dispatch_scheme<float, 6, false, false, 64, 4, 512>(st, a);
-    case 2: return dispatch_scheme<float, 6, false, false, 16, 8, 256>(st, a);
-    default: return dispatch_scheme<float, 6, false, false, 32, 8, 512>(st, a);
-  }
+hipError_t launch_fused_f32_g6(hipStream_t st, const FusedArgs<float>& a) {
+  return dispatch_scheme<float, 6, false, false>(st, a);
 }
 
 }  // namespace gt

@@ -317,12 +317,14 @@ def _batch_array(domains):
     return arr
 
 
-TILES = {0: "32x8", 1: "64x4", 2: "16x8"}
+# tuning index -> rows per strip of the HALFSPLIT strip kernels
+ROWS = {0: 16, 1: 32, 2: 8}
 
 
 def autotune_batch(domains, ntracers, dt, hscheme, vscheme, AH=-1.0, tag=H_TAG, reps=3,
                    stream=None):
-    """gt_autotune_batch: returns (tile, group, best_ms, {(tile, group): ms})"""
+    """gt_autotune_batch: returns (tile, group, best_ms, {(tile, group): ms}); tile
+    indexes ROWS (rows per strip), group is the number of tracers per wavefront"""
     import torch
     lib = load_library()
     arr = _batch_array(domains)

@@ -1,8 +1,9 @@
-"""The one-pass HALFSPLIT kernel and the batched multi-tracer step.
+"""The HALFSPLIT strip kernels and the batched multi-tracer step.
 
-* FP64 (fused) must equal the per-sub-step kernels and the oracle bit for bit
-  (bound written: 1e-12 relative per cell), for every scheme pair, tag, tile
-  shape, with and without the optional outputs.
+* FP64 (strip kernels) must equal the per-sub-step kernels and the oracle bit
+  for bit (bound written: 1e-12 relative per cell), for every scheme pair,
+  tag, strip height, with and without the optional outputs, on domains
+  narrower and wider than one strip.
 * FP32 batched: max and RMS error against the FP64 oracle, normalised by the
   field's range, within 1e-5 after several steps; the FP64 mass budget of
   each tracer is conserved to the FP32 rounding of the stored tracer.
@@ -59,8 +60,40 @@ def test_fused_f64_bit_exact(device, kind):
     ctx.set_tuning(fused=1, tile=0)
 
 
+def test_fused_f64_bit_exact_wide(device):
+    """several strips across i and j (NX = 154: strips at i = 0, 60, 120)"""
+    dom = random_domain(150, 40, 6, 7, "islands")
+    F = random_fields(dom, 200)
+    ctx = gt.Context.from_domain(dom)
+    orc = po.Oracle.from_domain(dom)
+    dargs = [to_dev(F[n], device) for n in NAMES]
+    for tile in (0, 1, 2):
+        for sch in SCHEMES:
+            for tag in (gt.H_TAG, gt.W_TAG):
+                for AH in (-1.0, 3.0):
+                    outs = (sch + tile) % 2 == 0
+                    f = F["f"].copy()
+                    hires = np.zeros_like(f) if outs else None
+                    advres = np.zeros_like(f) if outs else None
+                    rc, _ = orc.do_advection_3d(F["dt"], f, *(F[n] for n in NAMES),
+                                                gt.HALFSPLIT, sch, sch, AH, tag, 1, hires, advres)
+                    assert rc == 0
+                    ctx.set_tuning(fused=1, tile=tile)
+                    tf = to_dev(F["f"], device)
+                    th = to_dev(np.zeros_like(f), device) if outs else None
+                    ta = to_dev(np.zeros_like(f), device) if outs else None
+                    ctx.do_advection_3d(F["dt"], tf, *dargs, gt.HALFSPLIT, sch, sch, AH, tag,
+                                        th, ta)
+                    got = to_host(tf)
+                    assert np.array_equal(got, f), (tile, sch, tag, AH, rel_err(got, f))
+                    if outs:
+                        assert np.array_equal(to_host(th), hires), (tile, sch, tag)
+                        assert rel_err(to_host(ta), advres) <= TOL, (tile, sch, tag)
+    ctx.set_tuning(fused=1, tile=0)
+
+
 def test_batch_f64_two_tracers_bit_exact(device):
-    tr = inputs.ts_case(seed=4, imax=45, jmax=38, kmax=9)
+    tr = inputs.ts_case(seed=4, imax=125, jmax=38, kmax=9)
     d = tr.dom
     ctx = gt.Context.from_domain(d)
     orc = po.Oracle.from_domain(d)
@@ -86,10 +119,11 @@ def normwise(a, b):
     return float(d.max() / scale), float(np.sqrt(np.mean(d ** 2)) / scale)
 
 
-def test_batch_f32_multidomain(device):
+@pytest.mark.parametrize("tile,group", [(0, 1), (1, 3), (2, 6)])
+def test_batch_f32_multidomain(device, tile, group):
     """T, S and passive tracers on uneven subdomains in one launch, FP32
     against the FP64 oracle over several P2-PDM steps."""
-    sizes = [(24, 20), (41, 33), (30, 52)]
+    sizes = [(24, 20), (71, 33), (30, 52)]
     subs = inputs.multitracer_case(seed=9, kmax=8, sizes=sizes, npassive=4)
     names = list(subs[0].tracers)
     nt = len(names)
@@ -110,8 +144,7 @@ def test_batch_f32_multidomain(device):
     m0 = [[ctxs[s].budget(to_dev(subs[s].hn, device), tins[s][q]) for q in range(nt)]
           for s in range(len(subs))]
     nsteps = 6
-    for group in (1, 3):
-        ctxs[0].set_tuning(fused=1, tile=0, group=group)
+    ctxs[0].set_tuning(fused=1, tile=tile, group=group)
     for step in range(nsteps):
         for s, tr in enumerate(subs):
             host = [tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho, tr.hn]
