@@ -7,9 +7,16 @@ temperature and salinity -- with every input resident in HBM.
 metric: tracer cell-updates/s = imax*jmax*kmax*2 per step / time.
 
 Second section (batched FP32): T, S and 16 passive tracers on 8 uneven
-subdomains (64x64 ... 512x384, 30 layers) in one fused launch per step,
+subdomains (64x64 ... 512x384, 30 layers) in one launch pair per step,
 FP32 tracers with FP64 layer heights; its error against the FP64 path after
 the timed steps, and its mass-budget drift.
+
+Advection runs as two kernels per step (k_strip_uv: u/2 + v/2, k_strip_wvu:
+w + v/2 + u/2).  Their roofline is quoted for the pair, with the algorithmic
+bytes of the whole step Ncell*(2*Ntr*s + 7*8): each tracer read and written
+once, uu vv ww hu hv ho read and hires written once in FP64.  Strip height
+and tracers per wavefront are autotuned (untimed) unless --tile/--group fix
+them.
 
 Prints one JSON line.  Run: python bench.py [--steps K] [--warmup W]
 """
@@ -42,8 +49,12 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-batched", action="store_true")
     p.add_argument("--batched-steps", type=int, default=20)
-    p.add_argument("--tile", type=int, default=-1, help="fused tile shape (-1: tuned)")
-    p.add_argument("--group", type=int, default=-1, help="tracers per workgroup (-1: tuned)")
+    p.add_argument("--tile", type=int, default=-1,
+                   help="rows per strip index 0:16 1:32 2:8 (-1: autotuned)")
+    p.add_argument("--copy-mb", type=int, default=0,
+                   help="also run a plain device copy of this many MiB (counter calibration)")
+    p.add_argument("--group", type=int, default=-1,
+                   help="FP32 tracers per wavefront 1, 3 or 6 (-1: autotuned)")
     return p.parse_args()
 
 
@@ -87,6 +98,15 @@ def main():
     from code_amd import transport as gt
 
     dev = torch.device("cuda:0")
+    if a.copy_mb > 0:
+        # a plain copy of known size (read N bytes, write N bytes) whose
+        # hardware byte counters calibrate those of the transport kernels
+        src = torch.ones(a.copy_mb * 2 ** 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     tr = inputs.ts_case(imax=a.imax, jmax=a.jmax, kmax=a.kmax)
     d = tr.dom
     st = torch.cuda.Stream(device=dev)
@@ -109,6 +129,14 @@ def main():
     bufs = [TS, TS2]
     t_adv = EventTimer(torch, st)
     t_ip = EventTimer(torch, st)
+
+    tuned = None
+    if a.tile < 0:
+        bt, _, bms, table = gt.autotune_batch(
+            [dict(ctx=ctx, f_in=TS, f_out=TS2, uu=uu, vv=vv, ww=ww, hu=hun, hv=hvn, ho=ho,
+                  hires=hires)], 2, dt, gt.P2_PDM, gt.P2_PDM, reps=5, stream=st)
+        tuned = {"rows": gt.ROWS[bt], "ms": bms,
+                 "table_ms": {str(gt.ROWS[t]): v for (t, _), v in table.items()}}
 
     def step(timed):
         cur, nxt = bufs
@@ -139,13 +167,19 @@ def main():
     ntr = 2
     value = ncell * ntr / (ms * 1e-3)
     adv_s = t_adv.mean()
-    # algorithmic bytes of the fused advection launch (T and S): each tracer
-    # read and written once, uu vv ww hu hv ho read once, FP64
-    b_adv = ncell * (ntr * 2 * 8 + 6 * 8)
-    roof = {"bound": "hbm", "kernel": "k_fused_halfsplit<double> (T+S, one launch)",
+    ip_s = t_ip.mean()
+    # algorithmic bytes of one advection step (T and S, FP64): each tracer
+    # read and written once, uu vv ww hu hv ho read and hires written once
+    b_adv = ncell * (ntr * 2 * 8 + 7 * 8)
+    roof = {"bound": "hbm", "kernel": "k_strip_uv + k_strip_wvu <double, P2-PDM, 2 tracers> "
+                                      "(one advection step, two launches)",
             "achieved": b_adv / adv_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": b_adv / adv_s / HBM_PEAK, "traffic": None,
-            "bytes_per_launch": b_adv, "launch_us": adv_s * 1e6}
+            "bytes_per_launch": b_adv, "launch_us": adv_s * 1e6,
+            "bytes_per_cell_tracer": b_adv / (ncell * ntr)}
+    # EOS (read T S, write buoy) + two internal-pressure methods (read buoy hn
+    # hun hvn, write idpdx idpdy): 24 + 2*48 B/cell
+    b_ip = ncell * (24 + 2 * 48)
     out = {
         "metric": METRIC,
         "value": value,
@@ -164,7 +198,9 @@ def main():
                                "advection of T and S", "imax": d.imax, "jmax": d.jmax,
                    "kmax": d.kmax, "tracers": ntr, "parallelism": "single GPU"},
         "fp64_advection_cell_updates_per_s": ncell * ntr / adv_s,
-        "components_ms": {"advection_T_S": adv_s * 1e3, "eos_and_pressure": t_ip.mean() * 1e3},
+        "components_ms": {"advection_T_S": adv_s * 1e3, "eos_and_pressure": ip_s * 1e3},
+        "eos_and_pressure_GBps": b_ip / ip_s / 1e9,
+        "tuning": tuned,
         "roofline": roof,
     }
     del TS2, idpdx6, idpdy6
@@ -207,6 +243,7 @@ def batched(torch, np, gt, inputs, dev, a, fp64_rate):
                for k, k2 in (("uu", "uu"), ("vv", "vv"), ("ww", "ww"), ("hu", "hun"),
                              ("hv", "hvn"), ("ho", "ho"))}
         vel["hn"] = torch.from_numpy(tr.hn).to(dev)
+        vel["hires"] = torch.zeros_like(vel["hn"])
         doms.append(dict(ctx=ctx, **vel))
         tr.tracers.clear()
     dt = subs[0].dt
@@ -217,11 +254,20 @@ def batched(torch, np, gt, inputs, dev, a, fp64_rate):
 
     def launch(bufs, dtype_doms):
         gt.adv3d_batch([dict(ctx=dd["ctx"], f_in=bufs[s][0], f_out=bufs[s][1], uu=dd["uu"],
-                             vv=dd["vv"], ww=dd["ww"], hu=dd["hu"], hv=dd["hv"], ho=dd["ho"])
+                             vv=dd["vv"], ww=dd["ww"], hu=dd["hu"], hv=dd["hv"], ho=dd["ho"],
+                             hires=dd["hires"])
                         for s, dd in enumerate(dtype_doms)], nt, dt, hs, vs, stream=st)
         for bb in bufs:
             bb.reverse()
 
+    tuned = None
+    if a.tile < 0 or a.group < 0:
+        bt, bg, bms, table = gt.autotune_batch(
+            [dict(ctx=dd["ctx"], f_in=fin32[s][0], f_out=fin32[s][1], uu=dd["uu"], vv=dd["vv"],
+                  ww=dd["ww"], hu=dd["hu"], hv=dd["hv"], ho=dd["ho"], hires=dd["hires"])
+             for s, dd in enumerate(doms)], nt, dt, hs, vs, reps=3, stream=st)
+        tuned = {"rows": gt.ROWS[bt], "tracers_per_wave": bg, "ms": bms,
+                 "table_ms": {f"{gt.ROWS[t]}x{g}": v for (t, g), v in table.items()}}
     timer = EventTimer(torch, st)
     nsteps = a.batched_steps
     with torch.cuda.stream(st):
@@ -254,16 +300,19 @@ def batched(torch, np, gt, inputs, dev, a, fp64_rate):
             erms = max(erms, float(torch.sqrt(torch.mean(dq * dq))) / scale)
             m1 = ctxs[s].budget(doms[s]["hn"], fin32[s][0][q])[0]
             drift = max(drift, abs(m1 - m0[s][q]) / abs(m0[s][q]))
-    # algorithmic bytes: each FP32 tracer read+written, 6 FP64 fields read once
-    bytes_step = ncell * (nt * 2 * 4 + 6 * 8)
+    # algorithmic bytes: each FP32 tracer read and written, 6 FP64 fields read
+    # and hires written once
+    bytes_step = ncell * (nt * 2 * 4 + 7 * 8)
     rate = ncell * nt / kern
     return {
         "workload": "T, S + 16 passive tracers on 8 subdomains (64x64 ... 512x384) x 30 "
-                    "layers, P2-PDM HALFSPLIT, FP32 tracers / FP64 layer heights, one launch",
+                    "layers, P2-PDM HALFSPLIT, FP32 tracers / FP64 layer heights, one launch "
+                    "pair per step",
         "cells": ncell, "tracers": nt,
         "value": rate, "unit": "cell-updates/s", "ms_per_step": kern * 1e3,
         "wall_ms_per_step": wall * 1e3,
-        "roofline": {"bound": "hbm", "kernel": "k_fused_halfsplit<float>",
+        "tuning": tuned,
+        "roofline": {"bound": "hbm", "kernel": "k_strip_uv + k_strip_wvu <float, P2-PDM>",
                      "achieved": bytes_step / kern / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": bytes_step / kern / HBM_PEAK,
                      "bytes_per_launch": bytes_step},

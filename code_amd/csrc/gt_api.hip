@@ -413,6 +413,11 @@ static void fill_domain(gt_ctx* c, const GridMasks* g, gt::FusedDomain<TF>* D, c
   D->dx = c->dx; D->dy = c->dy; D->ard1 = c->ard1;
 }
 
+// the strip kernels address one field of one tracer with 32-bit byte offsets
+static bool strip_fits(const gt_ctx* c) {
+  return (unsigned long long)c->N3 * sizeof(double) < (1ull << 32);
+}
+
 // bytes of work fields a domain needs for fill_domain
 static size_t strip_ws_bytes(const gt_ctx* c, size_t tf_bytes, int ntracers, bool adv) {
   return sizeof(double) * (size_t)c->N3 * (adv ? 2 : 1) + tf_bytes * (size_t)c->N3 * ntracers;
@@ -618,7 +623,8 @@ extern "C" int gt_do_advection_3d(gt_ctx* c, double dt, double* f, const double*
       return GT_OK;
     }
     case GT_HALFSPLIT: {
-      if (c->fused && itv <= 1 && kmax > 1 && gt::fused_supported(hscheme, vscheme)) {
+      if (c->fused && itv <= 1 && kmax > 1 && gt::fused_supported(hscheme, vscheme) &&
+          strip_fits(c)) {
         // f is read at level 0 only by the second kernel, which writes the
         // same values there, so the result goes to f directly
         TRY(ensure_ws(c, strip_ws_bytes(c, sizeof(double), 1, advres != nullptr)));
@@ -800,6 +806,9 @@ static int batch_impl(int ndom, const gt_batch_domain* doms, int ntracers, doubl
       return set_err(GT_EARG, "gt_adv3d_batch: domain %d needs a context and distinct "
                               "f_in/f_out", i);
     if (c->kmax < 2) return set_err(GT_EARG, "gt_adv3d_batch: kmax must be >= 2");
+    if (!strip_fits(c))
+      return set_err(GT_ENOTIMPL, "gt_adv3d_batch: domain %d has %ld cells; one field must "
+                                  "stay below 4 GiB in FP64 (split the domain)", i, c->N3);
     gsel = grid_for(c, tag);
     if (!gsel) return set_err(GT_EINVAL_TAG, "gt_adv3d_batch: tag is invalid");
   }

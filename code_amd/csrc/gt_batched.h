@@ -13,6 +13,8 @@ constexpr int kStripOut = 60;
 
 // packed per-cell mask bits (one byte per cell, per mask set H/U/V)
 enum : uint8_t { MB_UFLUX = 1, MB_VFLUX = 2, MB_UUPD = 4, MB_VUPD = 8, MB_WCOL = 16 };
+// in kernel registers only: the lane's column is one the strip writes
+enum : int { MB_XOUT = 32 };
 
 // one domain of a launch.  A launch covers the work items of one or more
 // domains (the uneven subdomains of a batched run) back to back.  A work item
@@ -50,7 +52,8 @@ struct FusedArgs {
   int hscheme, vscheme, kshift;
 };
 
-// rows per strip for a tuning index: 0 -> 16, 1 -> 32, 2 -> 8
+// rows per strip for a tuning index: 0 -> 16, 1 -> 32, 2 -> 8 (a multiple of
+// 4: the row march is unrolled four times)
 int strip_rows(int tile);
 // the strip kernels are instantiated for hscheme == vscheme in UPSTREAM..P2_PDM
 bool fused_supported(int hscheme, int vscheme);

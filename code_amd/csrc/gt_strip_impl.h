@@ -29,13 +29,16 @@
 //   fast (TF=float): tracer and flux arithmetic in FP32 with a division-free
 //     form of each limiter (phi(r)*deltaf as a min/max of products, the sign
 //     of deltaf choosing min or max) and reciprocals for the face constants;
-//     the cell update (Dio*f - dt*advn)/Di is done in FP64 with 1/Di shared by
-//     the tracers.
+//     the layer heights stay FP64 and the cell update (Dio*f - dt*advn)/Di is
+//     f + [f*(Dio-Di)/Di - dF*(dt*c/Di)] with the two factors formed in FP64
+//     once per cell and shared by the tracers.
 //
 // Which cells each sub-step updates follows the reference exactly (the isub /
 // jsub bounds of adv_split_u/v.F90 and the column range of adv_split_w.F90),
 // so cells the reference leaves alone pass through unchanged.
 #pragma once
+
+#include <type_traits>
 
 #include "gt_batched.h"
 
@@ -43,10 +46,10 @@ namespace gt {
 
 // ---- cross-lane moves (DPP wave shifts; call with every lane active) -------
 __device__ __forceinline__ int from_lower_lane(int v) {  // lane l gets lane l-1 (0 in lane 0)
-  return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, false);
+  return __builtin_amdgcn_mov_dpp(v, 0x138, 0xF, 0xF, true);
 }
 __device__ __forceinline__ int from_upper_lane(int v) {  // lane l gets lane l+1 (0 in lane 63)
-  return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, false);
+  return __builtin_amdgcn_mov_dpp(v, 0x130, 0xF, 0xF, true);
 }
 __device__ __forceinline__ float from_lower_lane(float v) {
   return __builtin_bit_cast(float, from_lower_lane(__builtin_bit_cast(int, v)));
@@ -92,13 +95,22 @@ struct FaceC<false> {
   bool right;
 };
 
+// the sweep's constants: dti = splitfac*dt, the face length dlen, and for
+// the fast path dti/dlen and AH/dlen (0 without diffusion)
+struct FaceK {
+  double dti, dlen, cscale, kscale;
+};
+__device__ __forceinline__ FaceK face_k(double dti, double dlen, double AH) {
+  return {dti, dlen, dti / dlen, AH > 0.0 ? AH / dlen : 0.0};
+}
+
 // face constants from the transport (vel), layer height (Dv) and masks;
 // lim_l/lim_r: the flux mask of the upwind neighbour allows the limiter
 // (adv_split_u.F90:219-265, adv_split_v.F90:77-119)
 template <int SCH>
 __device__ __forceinline__ void face_setup(FaceC<true>& fc, bool active, bool lim_l, bool lim_r,
-                                           double vel, double Dv, double dti, double dlen,
-                                           double /*AH*/) {
+                                           double vel, double Dv, const FaceK& k) {
+  const double dti = k.dti, dlen = k.dlen;
   fc.vel = vel;
   fc.Dv = Dv;
   fc.cfl = 0.0;
@@ -117,18 +129,18 @@ __device__ __forceinline__ void face_setup(FaceC<true>& fc, bool active, bool li
 
 template <int SCH>
 __device__ __forceinline__ void face_setup(FaceC<false>& fc, bool active, bool lim_l, bool lim_r,
-                                           double vel, double Dv, double dti, double dlen,
-                                           double AH) {
+                                           double vel, double Dv, const FaceK& k) {
   const bool right = !(vel > 0.0);
   const bool lim = active && SCH != UPSTREAM && (right ? lim_r : lim_l);
-  const float c = lim ? (float)(__builtin_fabs(vel) * (dti / dlen) * rcp_nr(Dv)) : 0.0f;
+  const float c0 = (float)(__builtin_fabs(vel) * k.cscale * rcp_nr(Dv));
+  const float c = lim ? c0 : 0.0f;
   fc.right = right;
   fc.vel = active ? (float)vel : 0.0f;
   fc.g = lim ? 0.5f * (1.0f - c) : 0.0f;
   fc.A = 2.0f * __builtin_amdgcn_rcpf(c + 1.e-10f);
   fc.al = 0.5f + (1.0f / 6.0f) * (1.0f - 2.0f * c);
   fc.E = 2.0f * __builtin_amdgcn_rcpf(1.0f - c);
-  fc.kA = (active && AH > 0.0) ? (float)(AH * Dv / dlen) : 0.0f;
+  fc.kA = active ? (float)(k.kscale * Dv) : 0.0f;
 }
 
 // the limiter, division-free FP32 form.  With deltaf = fd-fu,
@@ -137,34 +149,42 @@ __device__ __forceinline__ void face_setup(FaceC<false>& fc, bool active, bool l
 // the min (deltaf > 0) or max (deltaf < 0) of the branch terms times deltaf.
 // P2-PDM: A = 2/(c+1e-10), al = 1/2 + (1-2c)/6, E = 2/(1-c).
 // (adv_interfacial_reconstruction, src/2d/advection.F90:785-879)
+// Both branch terms are formed and one is selected (no divergent branch);
+// fused multiply-adds are written out where they shorten the chain.
 template <int SCH>
 __device__ __forceinline__ float rec_fast(float fuu, float fu, float fd, float g, float A,
                                           float al, float E) {
   const float df = fd - fu, du = fu - fuu;
   if constexpr (SCH == P2) {
-    return fu + g * (al * df + (1.0f - al) * du);
+    // al*df + (1-al)*du = du + al*(df-du)
+    return __builtin_fmaf(g, __builtin_fmaf(al, df - du, du), fu);
   } else if constexpr (SCH == UPSTREAM) {
     return fu;
   } else {
-    float t;
+    float lo, hi;
     if constexpr (SCH == P2_PDM) {
-      const float t1 = A * du, t2 = du + al * (df - du), t3 = E * df;
-      t = df > 0.0f ? fminf(fminf(t1, t2), t3) : fmaxf(fmaxf(t1, t2), t3);
+      const float t1 = A * du, t2 = __builtin_fmaf(al, df - du, du), t3 = E * df;
+      lo = __builtin_fminf(__builtin_fminf(t1, t2), t3);
+      hi = __builtin_fmaxf(__builtin_fmaxf(t1, t2), t3);
     } else if constexpr (SCH == SUPERBEE) {
-      t = df > 0.0f ? fmaxf(fminf(2.0f * du, df), fminf(du, 2.0f * df))
-                    : fminf(fmaxf(2.0f * du, df), fmaxf(du, 2.0f * df));
+      const float a2 = 2.0f * du, b2 = 2.0f * df;
+      lo = __builtin_fmaxf(__builtin_fminf(a2, df), __builtin_fminf(du, b2));
+      hi = __builtin_fminf(__builtin_fmaxf(a2, df), __builtin_fmaxf(du, b2));
     } else {  // MUSCL
-      const float h = 0.5f * (df + du);
-      t = df > 0.0f ? fminf(fminf(2.0f * du, h), 2.0f * df)
-                    : fmaxf(fmaxf(2.0f * du, h), 2.0f * df);
+      const float a2 = 2.0f * du, b2 = 2.0f * df, h = 0.5f * (df + du);
+      lo = __builtin_fminf(__builtin_fminf(a2, h), b2);
+      hi = __builtin_fmaxf(__builtin_fmaxf(a2, h), b2);
     }
-    return df * du > 0.0f ? fu + g * t : fu;
+    const float t = df > 0.0f ? lo : hi;
+    const float r = __builtin_fmaf(g, t, fu);
+    return df * du > 0.0f ? r : fu;
   }
 }
 
 // flux through the face between cells X and X+1 of a sweep, from the cells
 // X-1, X, X+1, X+2 (fm1, f0, fp1, fp2)
-template <typename TF, int SCH>
+// DIFF: horizontal diffusion on (AH > 0), fixed at compile time
+template <typename TF, int SCH, bool DIFF>
 __device__ __forceinline__ TF face_flux(const FaceC<true>& fc, TF fm1, TF f0, TF fp1, TF fp2,
                                         double AH, double dlen) {
   if (fc.code == F_NONE) return TF(0);
@@ -177,41 +197,46 @@ __device__ __forceinline__ TF face_flux(const FaceC<true>& fc, TF fm1, TF f0, TF
     if (fc.code == F_RIGHT_LIM) fu = reconstruct<double>(SCH, fc.cfl, fp2, fp1, f0);
   }
   TF fl = fc.vel * fu;
-  if (AH > 0.0) fl = fl - AH * fc.Dv * (fp1 - f0) / dlen;
+  if constexpr (DIFF) fl = fl - AH * fc.Dv * (fp1 - f0) / dlen;
   return fl;
 }
 
-template <typename TF, int SCH>
+template <typename TF, int SCH, bool DIFF>
 __device__ __forceinline__ TF face_flux(const FaceC<false>& fc, TF fm1, TF f0, TF fp1, TF fp2,
-                                        double AH, double /*dlen*/) {
+                                        double /*AH*/, double /*dlen*/) {
   const TF fuu = fc.right ? fp2 : fm1;
   const TF fu0 = fc.right ? fp1 : f0;
   const TF fd = fc.right ? f0 : fp1;
   const TF fu = rec_fast<SCH>(fuu, fu0, fd, fc.g, fc.A, fc.al, fc.E);
   TF fl = fc.vel * fu;
-  if (AH > 0.0) fl = fl - fc.kA * (fp1 - f0);
+  if constexpr (DIFF) fl = __builtin_fmaf(-fc.kA, fp1 - f0, fl);
   return fl;
 }
 
-// the tracer update of one cell of a horizontal sub-step.  EXACT: the
-// reference's (Dio*fo - dt*advn)/Dn, advn = splitfac*(Fc*dlen - Fm*dlen)*ard1
-// (adv_split_u.F90:267-286).  fast: the same in FP64 with the shared
-// reciprocal rDn (dtc = dt*splitfac*dlen*ard1).
-template <typename TF, bool EXACT>
-__device__ __forceinline__ TF cell_update(double Dio, double Dn, double rDn, TF fo, TF Fc, TF Fm,
-                                          double dtc, double dlen, double ard1, double dt,
-                                          double* advn_out) {
-  if constexpr (EXACT) {
-    (void)rDn; (void)dtc;
-    const double advn = 0.5 * (Fc * dlen - Fm * dlen) * ard1;
-    *advn_out = advn;
-    return (Dio * fo - dt * advn) / Dn;
-  } else {
-    (void)Dn; (void)dlen; (void)ard1; (void)dt;
-    const float dF = Fc - Fm;
-    *advn_out = 0.0;
-    return (TF)((Dio * (double)fo - dtc * (double)dF) * rDn);
-  }
+// the tracer update of one cell of a horizontal sub-step: the reference's
+// (Dio*fo - dt*advn)/Dn, advn = splitfac*(Fc*dlen - Fm*dlen)*ard1
+// (adv_split_u.F90:267-286), in the reference's order (EXACT).
+__device__ __forceinline__ double cell_update_exact(double Dio, double Dn, double fo, double Fc,
+                                                    double Fm, double dlen, double ard1,
+                                                    double dt, double* advn_out) {
+  const double advn = 0.5 * (Fc * dlen - Fm * dlen) * ard1;
+  *advn_out = advn;
+  return (Dio * fo - dt * advn) / Dn;
+}
+
+// fast: the same as fo + [fo*(Dio-Dn)/Dn - (Fc-Fm)*(dt*splitfac*dlen*ard1/Dn)]:
+// the two factors are formed once per cell in FP64 from the layer heights and
+// rounded to FP32 (CellK); the bracket is a small correction, so its FP32
+// rounding is far below the final rounding of fo + correction.
+struct CellK {
+  float am1, b;
+};
+__device__ __forceinline__ CellK cell_factors(double Dio, double Dn, double dtc) {
+  const double r = rcp_nr(Dn);
+  return {(float)((Dio - Dn) * r), (float)(dtc * r)};
+}
+__device__ __forceinline__ float cell_update_fast(const CellK& k, float fo, float Fc, float Fm) {
+  return fo + __builtin_fmaf(fo, k.am1, -(Fc - Fm) * k.b);
 }
 
 // ---- vertical faces (adv_split_w.F90:249-277) --------------------------------
@@ -268,6 +293,22 @@ __device__ __forceinline__ TF wflux(const WFace<EXACT>& wf, int k, int kshift, i
   return TF(wf.w) * fu;
 }
 
+// fast form of wflux: every operand formed, the limiter switched off by g = 0
+// (lim_up / lim_dn: whether it applies going up / down at this face)
+template <int SCH>
+__device__ __forceinline__ float wflux_fast(const WFace<false>& wf, bool lim_up, bool lim_dn,
+                                            float fkm1, float fk, float fk1, float fk2) {
+  const bool up = wf.w > 0.0f;
+  const float fu = up ? fk : fk1, fuu = up ? fkm1 : fk2, fd = up ? fk1 : fk;
+  const bool lim = up ? lim_up : lim_dn;
+  return wf.w * rec_fast<SCH>(fuu, fu, fd, lim ? wf.g : 0.0f, wf.A, wf.al, wf.E);
+}
+
+template <typename T>
+__device__ __forceinline__ T sel(bool c, T a, T b) {
+  return c ? a : b;
+}
+
 // ---- work items ------------------------------------------------------------------
 struct StripItem {
   int dom, lev, sx, sy, grp;
@@ -298,15 +339,50 @@ __device__ __forceinline__ bool strip_item(const FusedArgs<TF>& A, StripItem& it
   return true;
 }
 
-template <typename T, int N>
-__device__ __forceinline__ void push(T (&w)[N], T v) {
-#pragma unroll
-  for (int i = 0; i + 1 < N; i++) w[i] = w[i + 1];
-  w[N - 1] = v;
+template <int R>
+using Slot = std::integral_constant<int, R>;
+
+// Per-lane column data, kept in vector registers: xc = x (0 outside the
+// array, so an address is always valid), xin = 0xff inside the array (else
+// 0, masks the flags), xm = the MB_* bits the column's i-bounds allow
+// (adv_split_u.F90 isub bounds; the w column stops at NX-2) plus MB_XOUT on
+// the 60 columns a strip writes.
+struct ColumnBits {
+  int xc, xin, xm;
+};
+__device__ __forceinline__ ColumnBits column_bits(int x, int lane, int NX, int isub) {
+  const bool in = x >= 0 && x < NX;
+  int xm = MB_VFLUX | MB_VUPD;
+  if (x >= isub && x <= NX - 2 - isub) xm |= MB_UFLUX;
+  if (x >= 1 + isub && x <= NX - 2 - isub) xm |= MB_UUPD;
+  if (x <= NX - 2) xm |= MB_WCOL;
+  if (lane >= 2 && lane < 2 + kStripOut) xm |= MB_XOUT;
+  return {in ? x : 0, in ? 0xff : 0, in ? xm : 0};
 }
 
+// load / store at a wave-uniform base plus a 32-bit byte offset: the base
+// stays in scalar registers and the address costs one vector register (the
+// saddr form of global_load/store).  A field of one tracer must stay below
+// 4 GiB, which the launch plan checks.
+template <typename T>
+__device__ __forceinline__ T ldo(const T* base, uint32_t off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+template <typename T>
+__device__ __forceinline__ void sto(T* base, uint32_t off, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off) = v;
+}
+
+// The row march is unrolled four times: the four-row windows rotate by
+// renaming (logical row q of step s lives in slot (s+q)&3), and the inputs of
+// a row are loaded one step ahead into one of two row buffers.  Loads read
+// valid addresses everywhere (cell 0 outside the array, the group's last
+// tracer past its end, level K for a level outside 0..kmax); only the flags
+// are masked, since every other value of a cell outside the array, or of a
+// missing level, reaches discarded results only.
+
 // ---- kernel 1: u/2 then v/2 at level L ------------------------------------------------
-template <typename TF, int SCH, int G, bool EXACT, bool ADV>
+template <typename TF, int SCH, int G, bool EXACT, bool ADV, bool DIFF>
 __global__ __launch_bounds__(256) void k_strip_uv(FusedArgs<TF> A) {
   StripItem it;
   if (!strip_item(A, it)) return;
@@ -318,44 +394,55 @@ __global__ __launch_bounds__(256) void k_strip_uv(FusedArgs<TF> A) {
   const int y0 = it.sy * A.rows;
   const int t0 = it.grp * G;
   const int ntr = min(G, A.ntracers - t0);
-  const bool xin = x >= 0 && x < NX;
-  const bool xout = xin && lane >= 2 && lane < 2 + kStripOut;
   constexpr int isub = SCH == UPSTREAM ? 0 : 1;
-  const bool uface_x = x >= isub && x <= NX - 2 - isub;
-  const bool uupd_x = x >= 1 + isub && x <= NX - 2 - isub;
+  const ColumnBits cb = column_bits(x, lane, NX, isub);
+  const bool xout = (cb.xm & MB_XOUT) != 0;
   const double dt = A.dt, dth = 0.5 * A.dt, dx = a.dx, dy = a.dy, ard1 = a.ard1, AH = A.AH;
   const double dtcu = dt * (0.5 * dy * ard1), dtcv = dt * (0.5 * dx * ard1);
+  const FaceK ku = face_k(dth, dx, AH), kv = face_k(dth, dy, AH);
   const long oL = NXY * (long)it.lev;
   const long ts = a.tstride;
-  const TF* fin = a.fin + (long)t0 * ts + oL;
-  TF* f2 = a.f2 + (long)t0 * ts + oL;
+  const TF* fin_g[G];
+  TF* f2_g[G];
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    const long tg = (long)(t0 + min(g, ntr - 1)) * ts + oL;
+    fin_g[g] = a.fin + tg;
+    f2_g[g] = a.f2 + tg;
+  }
   const double *ho = a.ho + oL, *uu = a.uu + oL, *hu = a.hu + oL, *vv = a.vv + oL,
                *hv = a.hv + oL;
   double* d2 = a.d2 + oL;
   double* a2 = ADV ? a.a2 + oL : nullptr;
+  const uint8_t* flags = a.flags;
   const int nsteps = A.rows + 4;
+  auto cell = [&](int y) -> uint32_t {
+    return (y >= 0 && y < NY) ? (uint32_t)(cb.xc + NX * y) : (uint32_t)cb.xc;
+  };
 
+  // inputs of the step of row y: the u/2 row y and the v faces of rows y-2
+  // and y-3
   struct Row {
     int fl;
-    double ho, uu, hu, vv, hv;
+    double ho, uu, hu, vv, vvm, hv;
     TF f[G];
   };
   auto load_row = [&](int y, Row& r) {
-    const bool in = xin && y >= 0 && y < NY;
-    const long c = in ? (long)x + (long)NX * y : 0;
-    r.fl = in ? (int)a.flags[c] : 0;
-    r.ho = in ? ho[c] : 1.0;
-    r.uu = in ? uu[c] : 0.0;
-    r.hu = in ? hu[c] : 1.0;
-    r.vv = in ? vv[c] : 0.0;
-    r.hv = in ? hv[c] : 1.0;
+    const uint32_t c = cell(y), c2 = cell(y - 2), c3 = cell(y - 3);
+    const uint32_t b = c * (uint32_t)sizeof(TF), b8 = c * 8u;
+    r.fl = flags[c];
+    r.ho = ldo(ho, b8);
+    r.uu = ldo(uu, b8);
+    r.hu = ldo(hu, b8);
+    r.vv = ldo(vv, c2 * 8u);
+    r.vvm = ldo(vv, c3 * 8u);
+    r.hv = ldo(hv, c2 * 8u);
 #pragma unroll
-    for (int g = 0; g < G; g++) r.f[g] = (in && g < ntr) ? fin[(long)g * ts + c] : TF(0);
+    for (int g = 0; g < G; g++) r.f[g] = ldo(fin_g[g], b);
   };
 
-  // windows over the last four rows: index 3 = row y, 0 = row y-3
   int flw[4] = {0, 0, 0, 0};
-  double vvw[4] = {0, 0, 0, 0}, hvw[4] = {1, 1, 1, 1}, d1w[4] = {1, 1, 1, 1};
+  double d1w[4] = {1, 1, 1, 1};
   double a1w[ADV ? 4 : 1][G];
   TF f1w[G][4];
   TF fvp[G];  // v flux through the face below the row being updated
@@ -372,85 +459,113 @@ __global__ __launch_bounds__(256) void k_strip_uv(FusedArgs<TF> A) {
       for (int g = 0; g < G; g++) a1w[q][g] = 0.0;
   }
 
-  Row nxt;
-  load_row(y0 - 2, nxt);
-  for (int s = 0; s < nsteps; s++) {
+  auto step = [&](auto slot, Row& buf, int s) {
+    constexpr int R = decltype(slot)::value;
+    constexpr int w0 = R & 3, w1 = (R + 1) & 3, w2 = (R + 2) & 3, w3 = (R + 3) & 3;
     const int y = y0 - 2 + s;
-    const Row cur = nxt;
-    if (s + 1 < nsteps) load_row(y + 1, nxt);
+    const int fl = (y >= 0 && y < NY) ? (buf.fl & cb.xin) : 0;  // raw masks of the cell
+    const int fo = (fl | MB_XOUT) & cb.xm;                      // ... with the column bounds
 
     // ---- u/2 on row y: shared part
-    const int flm = from_lower_lane(cur.fl), flp = from_upper_lane(cur.fl);
+    const int flm = from_lower_lane(fl), flp = from_upper_lane(fl);
     FaceC<EXACT> fu;
-    face_setup<SCH>(fu, (cur.fl & MB_UFLUX) && uface_x, flm & MB_UFLUX, flp & MB_UFLUX, cur.uu,
-                    cur.hu, dth, dx, AH);
-    const double uum = from_lower_lane(cur.uu);
-    const bool updu = (cur.fl & MB_UUPD) && uupd_x;
-    const double D0 = cur.ho;
+    face_setup<SCH>(fu, fo & MB_UFLUX, flm & MB_UFLUX, flp & MB_UFLUX, buf.uu, buf.hu, ku);
+    const double uum = from_lower_lane(buf.uu);
+    const bool updu = fo & MB_UUPD;
+    const double D0 = buf.ho;
     double D1 = D0;
-    if (updu) D1 = D0 - dth * (cur.uu * dy - uum * dy) * ard1;
-    const double rD1 = EXACT ? 0.0 : rcp_nr(D1);
-    push(flw, cur.fl);
-    push(vvw, cur.vv);
-    push(hvw, cur.hv);
-    push(d1w, D1);
+    D1 = sel(updu, D0 - dth * (buf.uu * dy - uum * dy) * ard1, D0);
+    CellK k1{};
+    if constexpr (!EXACT) k1 = cell_factors(D0, D1, dtcu);
+    flw[w3] = fo;
+    d1w[w3] = D1;
 
     // ---- v/2 on row yv = y-2: shared part (faces from row y0-1 on)
     const int yv = y - 2;
     const bool vface = s >= 3, vrow = s >= 4;
     FaceC<EXACT> fv;
     bool updv = false;
-    double D2 = d1w[1], rD2 = 0.0;
+    double D2 = d1w[w1];
+    CellK k2{};
     if (vface) {
-      face_setup<SCH>(fv, (flw[1] & MB_VFLUX) && yv >= isub && yv <= NY - 2 - isub,
-                      flw[0] & MB_VFLUX, flw[2] & MB_VFLUX, vvw[1], hvw[1], dth, dy, AH);
-      updv = (flw[1] & MB_VUPD) && yv >= 1 + isub && yv <= NY - 2 - isub;
-      if (updv) D2 = d1w[1] - dth * (vvw[1] * dx - vvw[0] * dx) * ard1;
-      if constexpr (!EXACT) rD2 = rcp_nr(D2);
+      face_setup<SCH>(fv, (flw[w1] & MB_VFLUX) && yv >= isub && yv <= NY - 2 - isub,
+                      flw[w0] & MB_VFLUX, flw[w2] & MB_VFLUX, buf.vv, buf.hv, kv);
+      updv = (flw[w1] & MB_VUPD) && yv >= 1 + isub && yv <= NY - 2 - isub;
+      D2 = sel(updv, d1w[w1] - dth * (buf.vv * dx - buf.vvm * dx) * ard1, D2);
+      if constexpr (!EXACT) k2 = cell_factors(d1w[w1], D2, dtcv);
     }
-    const bool st_ok = vrow && xout && yv < NY;
-    const long cv = (long)x + (long)NX * yv;
+    const bool st_ok = vrow && (flw[w1] & MB_XOUT) && yv < NY;
+    const uint32_t cv = (uint32_t)(x + NX * yv);
 
 #pragma unroll
     for (int g = 0; g < G; g++) {
       // u/2 on row y (adv_split_u.F90:219-286)
-      const TF f0 = cur.f[g];
-      const TF fm1 = from_lower_lane(f0), fp1 = from_upper_lane(f0);
+      const TF fc = buf.f[g];
+      const TF fm1 = from_lower_lane(fc), fp1 = from_upper_lane(fc);
       const TF fp2 = from_upper_lane(fp1);
-      const TF Fu = face_flux<TF, SCH>(fu, fm1, f0, fp1, fp2, AH, dx);
+      const TF Fu = face_flux<TF, SCH, DIFF>(fu, fm1, fc, fp1, fp2, AH, dx);
       const TF Fum = from_lower_lane(Fu);
       double advu = 0.0;
-      TF F1 = f0;
-      if (updu) F1 = cell_update<TF, EXACT>(D0, D1, rD1, f0, Fu, Fum, dtcu, dy, ard1, dt, &advu);
-      push(f1w[g], F1);
-      if constexpr (ADV) {
-#pragma unroll
-        for (int q = 0; q < 3; q++) a1w[q][g] = a1w[q + 1][g];
-        a1w[3][g] = 0.0 + advu;
+      TF F1 = fc;
+      if constexpr (EXACT) {
+        if (updu) F1 = cell_update_exact(D0, D1, fc, Fu, Fum, dy, ard1, dt, &advu);
+      } else {
+        const TF u = cell_update_fast(k1, fc, Fu, Fum);
+        F1 = updu ? u : fc;
       }
+      f1w[g][w3] = F1;
+      if constexpr (ADV) a1w[w3][g] = 0.0 + advu;
       // v/2 on row yv (adv_split_v.F90:77-131)
       if (vface) {
-        const TF Fv = face_flux<TF, SCH>(fv, f1w[g][0], f1w[g][1], f1w[g][2], f1w[g][3], AH, dy);
+        const TF Fv =
+            face_flux<TF, SCH, DIFF>(fv, f1w[g][w0], f1w[g][w1], f1w[g][w2], f1w[g][w3], AH, dy);
         if (vrow) {
           double advv = 0.0;
-          TF F2 = f1w[g][1];
-          if (updv)
-            F2 = cell_update<TF, EXACT>(d1w[1], D2, rD2, f1w[g][1], Fv, fvp[g], dtcv, dx, ard1,
-                                        dt, &advv);
-          if (st_ok && g < ntr) f2[(long)g * ts + cv] = F2;
+          TF F2 = f1w[g][w1];
+          if constexpr (EXACT) {
+            if (updv) F2 = cell_update_exact(d1w[w1], D2, F2, Fv, fvp[g], dx, ard1, dt, &advv);
+          } else {
+            const TF u = cell_update_fast(k2, F2, Fv, fvp[g]);
+            F2 = updv ? u : F2;
+          }
+          if (st_ok && g < ntr) sto(f2_g[g], cv * (uint32_t)sizeof(TF), F2);
           if constexpr (ADV) {
-            if (st_ok && g == 0) a2[cv] = a1w[1][0] + advv;
+            if (st_ok && g == 0) sto(a2, cv * 8u, a1w[w1][0] + advv);
           }
         }
         fvp[g] = Fv;
       }
     }
-    if (st_ok && it.grp == 0) d2[cv] = D2;
+    if (st_ok && it.grp == 0) sto(d2, cv * 8u, D2);
+    load_row(y + 2, buf);
+  };
+
+  Row ra, rb;
+  load_row(y0 - 2, ra);
+  load_row(y0 - 1, rb);
+  for (int s = 0; s < nsteps; s += 4) {
+    step(Slot<0>{}, ra, s);
+    step(Slot<1>{}, rb, s + 1);
+    step(Slot<2>{}, ra, s + 2);
+    step(Slot<3>{}, rb, s + 3);
+  }
+
+  // level 0 of the work fields (the second kernel reads levels K-2..K+2 of
+  // f2 and K-1..K+1 of d2 from one base): the input tracers and ho
+  if (it.lev == 1 && xout) {
+    for (int yv = y0; yv < y0 + A.rows && yv < NY; yv++) {
+      const uint32_t c = (uint32_t)(x + NX * yv);
+      for (int g = 0; g < ntr; g++) {
+        const long tg = (long)(t0 + g) * ts;
+        a.f2[tg + c] = a.fin[tg + c];
+      }
+      if (it.grp == 0) a.d2[c] = a.ho[c];
+    }
   }
 }
 
 // ---- kernel 2: w, then v/2 and u/2 at level K -------------------------------------------
-template <typename TF, int SCH, int G, bool EXACT, bool ADV>
+template <typename TF, int SCH, int G, bool EXACT, bool ADV, bool DIFF>
 __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
   StripItem it;
   if (!strip_item(A, it)) return;
@@ -464,67 +579,84 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
   const int y0 = it.sy * A.rows;
   const int t0 = it.grp * G;
   const int ntr = min(G, A.ntracers - t0);
-  const bool xin = x >= 0 && x < NX;
-  const bool xout = xin && lane >= 2 && lane < 2 + kStripOut;
   constexpr int isub = SCH == UPSTREAM ? 0 : 1;
-  const bool uface_x = x >= isub && x <= NX - 2 - isub;
-  const bool uupd_x = x >= 1 + isub && x <= NX - 2 - isub;
-  const bool wcol_x = x <= NX - 2;
+  const ColumnBits cb = column_bits(x, lane, NX, isub);
+  const bool xout = (cb.xm & MB_XOUT) != 0;
   const double dt = A.dt, dth = 0.5 * A.dt, dx = a.dx, dy = a.dy, ard1 = a.ard1, AH = A.AH;
   const double dtcu = dt * (0.5 * dy * ard1), dtcv = dt * (0.5 * dx * ard1);
+  const FaceK ku = face_k(dth, dx, AH), kv = face_k(dth, dy, AH);
   const long ts = a.tstride;
   const long oK = NXY * (long)K;
   // the w sub-step's levels: the flux at K+1/2 exists below kmax, the one at
   // K-1/2 above level 1 (or at level 1 with kshift); level K is updated up
   // to kmax-kshift (adv_split_w.F90:249-300)
   const bool wk_on = K <= kmax - 1, wp_on = K > 1 || kshift, kdo = K <= kmax - kshift;
-  // sources of levels K-2..K+2 of the tracers after u/2 + v/2 (level 0 is
-  // the input), and of the layer heights K-1..K+1
-  const TF* fsrc[5];
+  // where the limiter applies at the faces K-1/2 and K+1/2, going up / down
+  const bool limp_up = SCH != UPSTREAM && K - 1 > 1 - kshift, limp_dn = SCH != UPSTREAM && K - 1 < kmax - 1;
+  const bool limk_up = SCH != UPSTREAM && K > 1 - kshift, limk_dn = SCH != UPSTREAM && K < kmax - 1;
+  // tracers after u/2 + v/2: one base per tracer (level 0), levels K-2..K+2
+  // as byte offsets; layer heights K-1..K+1 (level 0 of both holds the input)
+  const TF* f2_g[G];
+  TF* fo_g[G];
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    const long tg = (long)(t0 + min(g, ntr - 1)) * ts;
+    f2_g[g] = a.f2 + tg;
+    fo_g[g] = a.fout + tg + oK;
+  }
+  uint32_t lvo[5];
 #pragma unroll
   for (int m = 0; m < 5; m++) {
     const int l = K - 2 + m;
-    fsrc[m] = (l < 0 || l > kmax) ? nullptr
-              : (l == 0 ? a.fin + (long)t0 * ts : a.f2 + (long)t0 * ts + NXY * (long)l);
+    lvo[m] = (uint32_t)(((l >= 0 && l <= kmax) ? l : K) * NXY * (long)sizeof(TF));
   }
-  const double* dKm1 = K == 1 ? a.ho : a.d2 + NXY * (long)(K - 1);
-  const double* dK = a.d2 + oK;
-  const double* dKp1 = K < kmax ? a.d2 + oK + NXY : nullptr;
-  const double *wwK = a.ww + oK, *wwKm1 = a.ww + oK - NXY;
+  const bool has_kp1 = K < kmax;
+  // d2 at K-1, K, K+1 and ww at K-1, K from one base each (level K-1) and
+  // byte offsets
+  const double *dKm1 = a.d2 + oK - NXY, *wwKm1 = a.ww + oK - NXY;
+  const uint32_t lstep = (uint32_t)(NXY * 8), lup = has_kp1 ? 2u * lstep : lstep;
   const double *vv = a.vv + oK, *hv = a.hv + oK, *uu = a.uu + oK, *hu = a.hu + oK;
   const double* a2 = ADV ? a.a2 + oK : nullptr;
-  TF* fout = a.fout + (long)t0 * ts + oK;
+  double* hires = a.hires ? a.hires + oK : nullptr;
+  const uint8_t* flags = a.flags;
   const int nsteps = A.rows + 4;
+  auto cell = [&](int y) -> uint32_t {
+    return (y >= 0 && y < NY) ? (uint32_t)(cb.xc + NX * y) : (uint32_t)cb.xc;
+  };
 
+  // inputs of the step of row y: the w row y, and the v and u faces of row
+  // y-2 (v also y-3)
   struct Row {
     int fl;
-    double wwK, wwKm1, dKm1, dK, dKp1, vv, hv, uu, hu, a2;
+    double wwK, wwKm1, dKm1, dK, dKp1, vv, vvm, hv, uu, hu, a2;
     TF f[G][5];
   };
   auto load_row = [&](int y, Row& r) {
-    const bool in = xin && y >= 0 && y < NY;
-    const long c = in ? (long)x + (long)NX * y : 0;
-    r.fl = in ? (int)a.flags[c] : 0;
-    r.wwK = in ? wwK[c] : 0.0;
-    r.wwKm1 = in ? wwKm1[c] : 0.0;
-    r.dKm1 = in ? dKm1[c] : 1.0;
-    r.dK = in ? dK[c] : 1.0;
-    r.dKp1 = (in && dKp1) ? dKp1[c] : 1.0;
-    r.vv = in ? vv[c] : 0.0;
-    r.hv = in ? hv[c] : 1.0;
-    r.uu = in ? uu[c] : 0.0;
-    r.hu = in ? hu[c] : 1.0;
-    r.a2 = (ADV && in) ? a2[c] : 0.0;
+    const uint32_t c = cell(y), c2 = cell(y - 2), c3 = cell(y - 3);
+    const uint32_t b = c * (uint32_t)sizeof(TF), b8 = c * 8u, b82 = c2 * 8u;
+    r.fl = flags[c];
+    r.wwK = ldo(wwKm1, b8 + lstep);
+    r.wwKm1 = ldo(wwKm1, b8);
+    r.dKm1 = ldo(dKm1, b8);
+    r.dK = ldo(dKm1, b8 + lstep);
+    r.dKp1 = ldo(dKm1, b8 + lup);
+    r.vv = ldo(vv, b82);
+    r.vvm = ldo(vv, c3 * 8u);
+    r.hv = ldo(hv, b82);
+    r.uu = ldo(uu, b82);
+    r.hu = ldo(hu, b82);
+    if constexpr (ADV) r.a2 = ldo(a2, b8);
+    uint32_t lb[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) lb[m] = b + lvo[m];
 #pragma unroll
     for (int g = 0; g < G; g++)
 #pragma unroll
-      for (int m = 0; m < 5; m++)
-        r.f[g][m] = (in && fsrc[m] && g < ntr) ? fsrc[m][(long)g * ts + c] : TF(0);
+      for (int m = 0; m < 5; m++) r.f[g][m] = ldo(f2_g[g], lb[m]);
   };
 
-  int flw[4] = {0, 0, 0, 0};
-  double vvw[4] = {0, 0, 0, 0}, hvw[4] = {1, 1, 1, 1}, d3w[4] = {1, 1, 1, 1};
-  double uuw[4] = {0, 0, 0, 0}, huw[4] = {1, 1, 1, 1};
+  int flw[4] = {0, 0, 0, 0}, fow[4] = {0, 0, 0, 0};
+  double d3w[4] = {1, 1, 1, 1};
   double a3w[ADV ? 4 : 1][G];
   TF f3w[G][4];
   TF fvp[G];
@@ -541,104 +673,116 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
       for (int g = 0; g < G; g++) a3w[q][g] = 0.0;
   }
 
-  Row nxt;
-  load_row(y0 - 2, nxt);
-  for (int s = 0; s < nsteps; s++) {
+  auto step = [&](auto slot, Row& buf, int s) {
+    constexpr int R = decltype(slot)::value;
+    constexpr int w0 = R & 3, w1 = (R + 1) & 3, w2 = (R + 2) & 3, w3 = (R + 3) & 3;
     const int y = y0 - 2 + s;
-    const Row cur = nxt;
-    if (s + 1 < nsteps) load_row(y + 1, nxt);
+    const int fl = (y >= 0 && y < NY) ? (buf.fl & cb.xin) : 0;  // raw masks of the cell
+    const int fo = (fl | MB_XOUT) & cb.xm;                      // ... with the column bounds
 
     // ---- w on row y: shared part
-    const bool wcol = (cur.fl & MB_WCOL) && wcol_x && y <= NY - 2;
+    const bool wcol = (fo & MB_WCOL) && y <= NY - 2;
     const bool wupd = wcol && kdo;
-    const double D2K = cur.dK;
+    const double D2K = buf.dK;
     double D3 = D2K;
-    if (wupd) D3 = D2K - dt * (cur.wwK - cur.wwKm1);
-    const double rD3 = EXACT ? 0.0 : rcp_nr(D3);
+    D3 = sel(wupd, D2K - dt * (buf.wwK - buf.wwKm1), D2K);
+    CellK k3{};
+    if constexpr (!EXACT) k3 = cell_factors(D2K, D3, dt);
     WFace<EXACT> wfk, wfp;
-    wface_setup<SCH>(wfk, cur.wwK, dt, D2K, cur.dKp1);
-    wface_setup<SCH>(wfp, cur.wwKm1, dt, cur.dKm1, D2K);
-    push(flw, cur.fl);
-    push(vvw, cur.vv);
-    push(hvw, cur.hv);
-    push(uuw, cur.uu);
-    push(huw, cur.hu);
-    push(d3w, D3);
+    wface_setup<SCH>(wfk, buf.wwK, dt, D2K, buf.dKp1);
+    wface_setup<SCH>(wfp, buf.wwKm1, dt, buf.dKm1, D2K);
+    flw[w3] = fl;
+    fow[w3] = fo;
+    d3w[w3] = D3;
 
     // ---- v/2 and u/2 on row yv = y-2: shared part
     const int yv = y - 2;
     const bool vface = s >= 3, vrow = s >= 4;
     FaceC<EXACT> fv, fu;
     bool updv = false, updu = false;
-    double D4 = d3w[1], rD4 = 0.0, D5 = d3w[1], rD5 = 0.0;
+    double D4 = d3w[w1], D5 = d3w[w1];
+    CellK k4{}, k5{};
     if (vface) {
-      face_setup<SCH>(fv, (flw[1] & MB_VFLUX) && yv >= isub && yv <= NY - 2 - isub,
-                      flw[0] & MB_VFLUX, flw[2] & MB_VFLUX, vvw[1], hvw[1], dth, dy, AH);
-      updv = (flw[1] & MB_VUPD) && yv >= 1 + isub && yv <= NY - 2 - isub;
-      if (updv) D4 = d3w[1] - dth * (vvw[1] * dx - vvw[0] * dx) * ard1;
-      if constexpr (!EXACT) rD4 = rcp_nr(D4);
+      face_setup<SCH>(fv, (flw[w1] & MB_VFLUX) && yv >= isub && yv <= NY - 2 - isub,
+                      flw[w0] & MB_VFLUX, flw[w2] & MB_VFLUX, buf.vv, buf.hv, kv);
+      updv = (flw[w1] & MB_VUPD) && yv >= 1 + isub && yv <= NY - 2 - isub;
+      D4 = sel(updv, d3w[w1] - dth * (buf.vv * dx - buf.vvm * dx) * ard1, D4);
+      if constexpr (!EXACT) k4 = cell_factors(d3w[w1], D4, dtcv);
     }
     if (vrow) {
-      const int fl = flw[1];
-      const int flm = from_lower_lane(fl), flp = from_upper_lane(fl);
-      face_setup<SCH>(fu, (fl & MB_UFLUX) && uface_x, flm & MB_UFLUX, flp & MB_UFLUX, uuw[1],
-                      huw[1], dth, dx, AH);
-      const double uum = from_lower_lane(uuw[1]);
-      updu = (fl & MB_UUPD) && uupd_x;
+      const int flv = flw[w1], fov = fow[w1];
+      const int flm = from_lower_lane(flv), flp = from_upper_lane(flv);
+      face_setup<SCH>(fu, fov & MB_UFLUX, flm & MB_UFLUX, flp & MB_UFLUX, buf.uu, buf.hu, ku);
+      const double uum = from_lower_lane(buf.uu);
+      updu = fov & MB_UUPD;
       D5 = D4;
-      if (updu) D5 = D4 - dth * (uuw[1] * dy - uum * dy) * ard1;
-      if constexpr (!EXACT) rD5 = rcp_nr(D5);
+      D5 = sel(updu, D4 - dth * (buf.uu * dy - uum * dy) * ard1, D4);
+      if constexpr (!EXACT) k5 = cell_factors(D4, D5, dtcu);
     }
-    const bool st_ok = vrow && xout && yv < NY;
-    const long cv = (long)x + (long)NX * yv;
+    const bool st_ok = vrow && (fow[w1] & MB_XOUT) && yv < NY;
+    const uint32_t cv = (uint32_t)(x + NX * yv);
 
 #pragma unroll
     for (int g = 0; g < G; g++) {
-      // w on row y (adv_split_w.F90:249-300)
-      const TF fKm2 = cur.f[g][0], fKm1 = cur.f[g][1], fK = cur.f[g][2];
-      const TF fKp1 = cur.f[g][3], fKp2 = cur.f[g][4];
+      // w on row y (adv_split_w.F90:249-300); the level below 1 is zero in
+      // the argument, level kmax+1 is never read by the limiter
+      const TF fKm2 = buf.f[g][0], fKm1 = buf.f[g][1], fK = buf.f[g][2];
+      const TF fKp1 = buf.f[g][3], fKp2 = buf.f[g][4];
       TF wprev = TF(0), wk = TF(0);
-      if (wcol && wp_on)
-        wprev = wflux<TF, SCH, EXACT>(wfp, K - 1, kshift, kmax, K > 1 ? fKm2 : TF(0), fKm1, fK,
-                                      fKp1);
-      if (wcol && wk_on) wk = wflux<TF, SCH, EXACT>(wfk, K, kshift, kmax, fKm1, fK, fKp1, fKp2);
+      if constexpr (EXACT) {
+        if (wcol && wp_on)
+          wprev = wflux<TF, SCH, EXACT>(wfp, K - 1, kshift, kmax, K > 1 ? fKm2 : TF(0), fKm1, fK,
+                                        fKp1);
+        if (wcol && wk_on)
+          wk = wflux<TF, SCH, EXACT>(wfk, K, kshift, kmax, fKm1, fK, fKp1, fKp2);
+      } else {
+        const TF p = wflux_fast<SCH>(wfp, limp_up, limp_dn, K > 1 ? fKm2 : TF(0), fKm1, fK, fKp1);
+        const TF q = wflux_fast<SCH>(wfk, limk_up, limk_dn, fKm1, fK, fKp1, fKp2);
+        wprev = (wcol && wp_on) ? p : TF(0);
+        wk = (wcol && wk_on) ? q : TF(0);
+      }
       TF F3 = fK;
       double advw = 0.0;
-      if (wupd) {
-        if constexpr (EXACT) {
+      if constexpr (EXACT) {
+        if (wupd) {
           advw = 1.0 * (wk - wprev);
           F3 = (D2K * fK - dt * advw) / D3;
-        } else {
-          F3 = (TF)((D2K * (double)fK - dt * (double)(wk - wprev)) * rD3);
         }
+      } else {
+        const TF u = cell_update_fast(k3, fK, wk, wprev);
+        F3 = wupd ? u : fK;
       }
-      push(f3w[g], F3);
-      if constexpr (ADV) {
-#pragma unroll
-        for (int q = 0; q < 3; q++) a3w[q][g] = a3w[q + 1][g];
-        a3w[3][g] = cur.a2 + advw;
-      }
+      f3w[g][w3] = F3;
+      if constexpr (ADV) a3w[w3][g] = buf.a2 + advw;
       if (vface) {
         // v/2 on row yv (adv_split_v.F90:77-131)
-        const TF Fv = face_flux<TF, SCH>(fv, f3w[g][0], f3w[g][1], f3w[g][2], f3w[g][3], AH, dy);
+        const TF Fv =
+            face_flux<TF, SCH, DIFF>(fv, f3w[g][w0], f3w[g][w1], f3w[g][w2], f3w[g][w3], AH, dy);
         if (vrow) {
           double advv = 0.0, advu = 0.0;
-          TF F4 = f3w[g][1];
-          if (updv)
-            F4 = cell_update<TF, EXACT>(d3w[1], D4, rD4, f3w[g][1], Fv, fvp[g], dtcv, dx, ard1,
-                                        dt, &advv);
+          TF F4 = f3w[g][w1];
+          if constexpr (EXACT) {
+            if (updv) F4 = cell_update_exact(d3w[w1], D4, F4, Fv, fvp[g], dx, ard1, dt, &advv);
+          } else {
+            const TF u = cell_update_fast(k4, F4, Fv, fvp[g]);
+            F4 = updv ? u : F4;
+          }
           // u/2 on row yv (adv_split_u.F90:219-286)
           const TF fm1 = from_lower_lane(F4), fp1 = from_upper_lane(F4);
           const TF fp2 = from_upper_lane(fp1);
-          const TF Fu = face_flux<TF, SCH>(fu, fm1, F4, fp1, fp2, AH, dx);
+          const TF Fu = face_flux<TF, SCH, DIFF>(fu, fm1, F4, fp1, fp2, AH, dx);
           const TF Fum = from_lower_lane(Fu);
           TF F5 = F4;
-          if (updu)
-            F5 = cell_update<TF, EXACT>(D4, D5, rD5, F4, Fu, Fum, dtcu, dy, ard1, dt, &advu);
-          if (st_ok && g < ntr) fout[(long)g * ts + cv] = F5;
+          if constexpr (EXACT) {
+            if (updu) F5 = cell_update_exact(D4, D5, F4, Fu, Fum, dy, ard1, dt, &advu);
+          } else {
+            const TF u = cell_update_fast(k5, F4, Fu, Fum);
+            F5 = updu ? u : F4;
+          }
+          if (st_ok && g < ntr) sto(fo_g[g], cv * (uint32_t)sizeof(TF), F5);
           if constexpr (ADV) {
             if (st_ok && g == 0 && a.advres) {
-              const double A4 = a3w[1][0] + advv;
+              const double A4 = a3w[w1][0] + advv;
               a.advres[oK + cv] = A4 + advu;
             }
           }
@@ -646,15 +790,28 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
         fvp[g] = Fv;
       }
     }
-    if (st_ok && it.grp == 0 && a.hires) a.hires[oK + cv] = D5;
+    if (st_ok && it.grp == 0 && hires) sto(hires, cv * 8u, D5);
+    load_row(y + 2, buf);
+  };
+
+  Row ra, rb;
+  load_row(y0 - 2, ra);
+  load_row(y0 - 1, rb);
+  for (int s = 0; s < nsteps; s += 4) {
+    step(Slot<0>{}, ra, s);
+    step(Slot<1>{}, rb, s + 1);
+    step(Slot<2>{}, ra, s + 2);
+    step(Slot<3>{}, rb, s + 3);
   }
 
   // level 0 passes through (the step updates levels 1..kmax)
-  if (K == 1) {
+  if (K == 1 && xout) {
     for (int yv = y0; yv < y0 + A.rows && yv < NY; yv++) {
-      if (!xout) continue;
-      const long c = (long)x + (long)NX * yv;
-      for (int g = 0; g < ntr; g++) a.fout[(long)(t0 + g) * ts + c] = a.fin[(long)(t0 + g) * ts + c];
+      const uint32_t c = (uint32_t)(x + NX * yv);
+      for (int g = 0; g < ntr; g++) {
+        const long tg = (long)(t0 + g) * ts;
+        a.fout[tg + c] = a.fin[tg + c];
+      }
       if (it.grp == 0 && a.hires) a.hires[c] = a.ho[c];
       if constexpr (ADV) {
         if (a.advres) a.advres[c] = 0.0;
@@ -677,17 +834,23 @@ void strip_plan(FusedArgs<TF>& a, int group) {
   a.nitems = items;
 }
 
-template <typename TF, int SCH, int G, bool EXACT, bool ADV>
-hipError_t launch_one(hipStream_t st, const FusedArgs<TF>& A) {
+template <typename TF, int SCH, int G, bool EXACT, bool ADV, bool DIFF>
+hipError_t launch_two(hipStream_t st, const FusedArgs<TF>& A) {
   const long nblk = (A.nitems + 3) / 4;
   const long grid = ((nblk + 7) / 8) * 8;
-  hipLaunchKernelGGL((k_strip_uv<TF, SCH, G, EXACT, ADV>), dim3((unsigned)grid), dim3(256), 0,
-                     st, A);
+  hipLaunchKernelGGL((k_strip_uv<TF, SCH, G, EXACT, ADV, DIFF>), dim3((unsigned)grid),
+                     dim3(256), 0, st, A);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_strip_wvu<TF, SCH, G, EXACT, ADV>), dim3((unsigned)grid), dim3(256), 0,
-                     st, A);
+  hipLaunchKernelGGL((k_strip_wvu<TF, SCH, G, EXACT, ADV, DIFF>), dim3((unsigned)grid),
+                     dim3(256), 0, st, A);
   return hipGetLastError();
+}
+
+template <typename TF, int SCH, int G, bool EXACT, bool ADV>
+hipError_t launch_one(hipStream_t st, const FusedArgs<TF>& A) {
+  return A.AH > 0.0 ? launch_two<TF, SCH, G, EXACT, ADV, true>(st, A)
+                    : launch_two<TF, SCH, G, EXACT, ADV, false>(st, A);
 }
 
 // one scheme at compile time (hscheme == vscheme)
