@@ -15,8 +15,8 @@ Advection runs as two kernels per step (k_strip_uv: u/2 + v/2, k_strip_wvu:
 w + v/2 + u/2).  Their roofline is quoted for the pair, with the algorithmic
 bytes of the whole step Ncell*(2*Ntr*s + 7*8): each tracer read and written
 once, uu vv ww hu hv ho read and hires written once in FP64.  Strip height
-and tracers per wavefront are autotuned (untimed) unless --tile/--group fix
-them.
+and tracers per wavefront are autotuned (untimed) unless --rows64 (headline)
+and --rows32/--group (batched) fix them.
 
 Prints one JSON line.  Run: python bench.py [--steps K] [--warmup W]
 """
@@ -49,8 +49,10 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-batched", action="store_true")
     p.add_argument("--batched-steps", type=int, default=20)
-    p.add_argument("--tile", type=int, default=-1,
-                   help="rows per strip index 0:16 1:32 2:8 (-1: autotuned)")
+    p.add_argument("--rows64", type=int, default=-1,
+                   help="FP64 headline: rows per strip 8, 16 or 32 (-1: autotuned)")
+    p.add_argument("--rows32", type=int, default=-1,
+                   help="FP32 batched: rows per strip 8, 16 or 32 (-1: autotuned)")
     p.add_argument("--copy-mb", type=int, default=0,
                    help="also run a plain device copy of this many MiB (counter calibration)")
     p.add_argument("--group", type=int, default=-1,
@@ -111,8 +113,8 @@ def main():
     d = tr.dom
     st = torch.cuda.Stream(device=dev)
     ctx = gt.Context.from_domain(d, stream=st)
-    if a.tile >= 0:
-        ctx.set_tuning(fused=1, tile=a.tile, group=max(a.group, 1) if a.group > 0 else 3)
+    if a.rows64 > 0:
+        ctx.set_tuning(fused=1, tile=gt.tile_for_rows(a.rows64))
     D = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)  # noqa: E731
     uu, vv, ww, hun, hvn, ho, hn = (D(x) for x in (tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho,
                                                    tr.hn))
@@ -131,7 +133,7 @@ def main():
     t_ip = EventTimer(torch, st)
 
     tuned = None
-    if a.tile < 0:
+    if a.rows64 < 0:
         bt, _, bms, table = gt.autotune_batch(
             [dict(ctx=ctx, f_in=TS, f_out=TS2, uu=uu, vv=vv, ww=ww, hu=hun, hv=hvn, ho=ho,
                   hires=hires)], 2, dt, gt.P2_PDM, gt.P2_PDM, reps=5, stream=st)
@@ -231,8 +233,8 @@ def batched(torch, np, gt, inputs, dev, a, fp64_rate):
         d = tr.dom
         ncell += d.imax * d.jmax * d.kmax
         ctx = gt.Context.from_domain(d, stream=st)
-        if a.tile >= 0:
-            ctx.set_tuning(1, a.tile, a.group if a.group > 0 else 3)
+        if a.rows32 > 0:
+            ctx.set_tuning(1, gt.tile_for_rows(a.rows32), a.group if a.group > 0 else 3)
         ctxs.append(ctx)
         stack = np.stack([tr.tracers[n] for n in names])
         f64 = torch.from_numpy(stack).to(dev)
@@ -261,7 +263,7 @@ def batched(torch, np, gt, inputs, dev, a, fp64_rate):
             bb.reverse()
 
     tuned = None
-    if a.tile < 0 or a.group < 0:
+    if a.rows32 < 0 or a.group < 0:
         bt, bg, bms, table = gt.autotune_batch(
             [dict(ctx=dd["ctx"], f_in=fin32[s][0], f_out=fin32[s][1], uu=dd["uu"], vv=dd["vv"],
                   ww=dd["ww"], hu=dd["hu"], hv=dd["hv"], ho=dd["ho"], hires=dd["hires"])

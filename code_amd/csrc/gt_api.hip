@@ -782,8 +782,8 @@ extern "C" int gt_set_tuning(gt_ctx* c, int fused, int tile, int group) {
   if (!c) return set_err(GT_EARG, "gt_set_tuning: NULL context");
   if (tile < 0 || tile > 2)
     return set_err(GT_EARG, "gt_set_tuning: tile (rows per strip 16, 32, 8) must be 0, 1 or 2");
-  if (group != 1 && group != 3 && group != 6)
-    return set_err(GT_EARG, "gt_set_tuning: group must be 1, 3 or 6");
+  if (group != 1 && group != 3 && group != 6 && group != 9)
+    return set_err(GT_EARG, "gt_set_tuning: group must be 1, 3, 6 or 9");
   c->fused = fused != 0;
   c->tile = tile;
   c->group = group;
@@ -870,7 +870,7 @@ extern "C" int gt_autotune_batch(int ndom, const gt_batch_domain* doms, int ntra
     return set_err(GT_EARG, "gt_autotune_batch: bad argument");
   hipStream_t st = stream ? (hipStream_t)stream : doms[0].ctx->st;
   const int tiles[3] = {0, 1, 2};
-  const int groups[3] = {1, 3, 6};
+  const int groups[4] = {1, 3, 6, 9};
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -878,7 +878,7 @@ extern "C" int gt_autotune_batch(int ndom, const gt_batch_domain* doms, int ntra
   int bt = 0, bg = 3;
   int n = 0;
   for (int ti = 0; ti < 3; ti++)
-    for (int gi = 0; gi < (f32 ? 3 : 1); gi++) {
+    for (int gi = 0; gi < (f32 ? 4 : 1); gi++) {
       for (int i = 0; i < ndom; i++) {
         doms[i].ctx->tile = tiles[ti];
         doms[i].ctx->group = groups[gi];

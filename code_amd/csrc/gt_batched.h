@@ -57,7 +57,7 @@ struct FusedArgs {
 int strip_rows(int tile);
 // the strip kernels are instantiated for hscheme == vscheme in UPSTREAM..P2_PDM
 bool fused_supported(int hscheme, int vscheme);
-// group: tracers per wavefront.  FP64: 1 or 2 (adv needs 1); FP32: 1, 3 or 6.
+// group: tracers per wavefront.  FP64: 1 or 2 (adv needs 1); FP32: 1, 3, 6 or 9.
 // The caller fills everything but ngroups and the item ranges, which
 // strip_plan sets.
 template <typename TF>

@@ -33,11 +33,13 @@ namespace gt {
 hipError_t launch_fused_f32_g1(hipStream_t st, const FusedArgs<float>& a);
 hipError_t launch_fused_f32_g3(hipStream_t st, const FusedArgs<float>& a);
 hipError_t launch_fused_f32_g6(hipStream_t st, const FusedArgs<float>& a);
+hipError_t launch_fused_f32_g9(hipStream_t st, const FusedArgs<float>& a);
 
 hipError_t launch_fused_f32(hipStream_t st, const FusedArgs<float>& a, int group) {
   switch (group) {
     case 1: return launch_fused_f32_g1(st, a);
     case 6: return launch_fused_f32_g6(st, a);
+    case 9: return launch_fused_f32_g9(st, a);
     default: return launch_fused_f32_g3(st, a);
   }
 }

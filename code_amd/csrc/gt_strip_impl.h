@@ -316,8 +316,11 @@ struct StripItem {
 
 // Blocks of 4 wavefronts, one work item per wavefront.  Consecutive blocks
 // are dealt round-robin to the 8 XCDs, so block b is given the work of
-// block (b%8)*per + b/8: each XCD takes a contiguous run of items, in which
-// the levels of one strip follow each other and meet in that XCD's L2.
+// block (b%8)*per + b/8: each XCD takes a contiguous run of items.  Items are
+// numbered tracer group fastest, then level, then strip, so the groups of one
+// strip-level (which read the same layer heights, transports and masks) and
+// then the neighbouring levels (which read the same tracer levels in the w
+// sub-step) run side by side and meet in that XCD's L2.
 template <typename TF>
 __device__ __forceinline__ bool strip_item(const FusedArgs<TF>& A, StripItem& it) {
   const long nblk = (A.nitems + 3) / 4;
@@ -330,12 +333,12 @@ __device__ __forceinline__ bool strip_item(const FusedArgs<TF>& A, StripItem& it
   const FusedDomain<TF>& a = A.doms[di];
   long loc = item - a.item_begin;
   it.dom = di;
+  it.grp = (int)(loc % A.ngroups);
+  loc /= A.ngroups;
   it.lev = 1 + (int)(loc % a.d.kmax);
   loc /= a.d.kmax;
   it.sx = (int)(loc % a.nsx);
-  loc /= a.nsx;
-  it.sy = (int)(loc % a.nsy);
-  it.grp = (int)(loc / a.nsy);
+  it.sy = (int)(loc / a.nsx);
   return true;
 }
 

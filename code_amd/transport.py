@@ -321,6 +321,14 @@ def _batch_array(domains):
 ROWS = {0: 16, 1: 32, 2: 8}
 
 
+def tile_for_rows(rows):
+    """the tuning index of a strip height (8, 16 or 32 rows)"""
+    for t, r in ROWS.items():
+        if r == rows:
+            return t
+    raise ValueError(f"rows per strip must be one of {sorted(ROWS.values())}, not {rows}")
+
+
 def autotune_batch(domains, ntracers, dt, hscheme, vscheme, AH=-1.0, tag=H_TAG, reps=3,
                    stream=None):
     """gt_autotune_batch: returns (tile, group, best_ms, {(tile, group): ms}); tile
@@ -330,12 +338,12 @@ def autotune_batch(domains, ntracers, dt, hscheme, vscheme, AH=-1.0, tag=H_TAG, 
     arr = _batch_array(domains)
     f32 = domains[0]["f_in"].dtype == torch.float32
     bt, bg, bm = C.c_int(), C.c_int(), C.c_float()
-    table = (C.c_float * 9)()
+    table = (C.c_float * 12)()
     handle = getattr(stream, "cuda_stream", stream)
     check(lib.gt_autotune_batch(len(domains), arr, ntracers, dt, hscheme, vscheme, AH, tag,
                                 int(f32), reps, C.c_void_p(handle) if handle else None,
                                 C.byref(bt), C.byref(bg), C.byref(bm), table))
-    pairs = [(t, g) for t in (0, 1, 2) for g in ((1, 3, 6) if f32 else (1,))]
+    pairs = [(t, g) for t in (0, 1, 2) for g in ((1, 3, 6, 9) if f32 else (1,))]
     return bt.value, bg.value, bm.value, {p: table[i] for i, p in enumerate(pairs)}
 
 

@@ -190,13 +190,15 @@ int gt_adv3d_batch_f32(int ndom, const gt_batch_domain *doms, int ntracers,
 int gt_adv3d_batch_f64(int ndom, const gt_batch_domain *doms, int ntracers,
                        double dt, int hscheme, int vscheme, double AH, int tag,
                        void *hip_stream);
-/* kernel choice: fused (1) or per-sub-step (0) HALFSPLIT, fused tile shape
- * (0: 32x8, 1: 64x4, 2: 16x8) and tracers per workgroup (1, 2, 3, 6) */
+/* kernel choice: the two HALFSPLIT strip kernels (fused=1) or the
+ * per-sub-step kernels (0), rows per strip (tile 0: 16, 1: 32, 2: 8) and FP32
+ * tracers per wavefront (group 1, 3, 6 or 9; FP64 uses 2 for two or more
+ * tracers) */
 int gt_set_tuning(gt_ctx *ctx, int fused, int tile, int group);
-/* autotuner: times gt_adv3d_batch_f32 (f32=1) or _f64 over tile shapes
- * {32x8, 64x4, 16x8} x tracers per workgroup {1, 3, 6} (FP64: tiles only),
- * reps timed launches each, keeps the fastest on every context.  table
- * (optional) receives the mean ms of each pair in sweep order.  f_out is
+/* autotuner: times gt_adv3d_batch_f32 (f32=1) or _f64 over rows per strip
+ * {16, 32, 8} x tracers per wavefront {1, 3, 6, 9} (FP64: rows only), reps
+ * timed launches each, keeps the fastest on every context.  table (optional,
+ * 12 floats) receives the mean ms of each pair in sweep order.  f_out is
  * overwritten, f_in is not touched. */
 int gt_autotune_batch(int ndom, const gt_batch_domain *doms, int ntracers,
                       double dt, int hscheme, int vscheme, double AH, int tag,
