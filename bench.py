@@ -33,6 +33,36 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "strip_kernels_hbm_traffic.txt")
+
+
+def measured_traffic(section, rows, group=None):
+    """HBM bytes of one advection step (both strip kernels) from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary, scaled by what the same
+    counters read for a plain copy of known size; None when the summary does
+    not cover this strip height / tracer group."""
+    try:
+        lines = open(TRAFFIC_FILE).read().splitlines()
+    except OSError:
+        return None
+    recs = {}
+    for ln in lines:
+        if ln.startswith("#") or not ln.strip():
+            continue
+        head, *kv = ln.split()
+        recs.setdefault(head, []).append(dict(x.split("=") for x in kv))
+    if "calibration" not in recs or section not in recs:
+        return None
+    cal = recs["calibration"][0]
+    cf = float(cal["copy_read_bytes"]) / (float(cal["fetch_kib"]) * 1024)
+    cw = float(cal["copy_write_bytes"]) / (float(cal["write_kib"]) * 1024)
+    for r in recs[section]:
+        if int(r["rows"]) != rows or (group is not None and int(r["group"]) != group):
+            continue
+        fetch = float(r["uv_fetch_kib"]) + float(r["wvu_fetch_kib"])
+        write = float(r["uv_write_kib"]) + float(r["wvu_write_kib"])
+        return fetch * 1024 * cf + write * 1024 * cw
+    return None
 METRIC = "tracer cell-updates/s + % HBM peak; batched FP32 vs plain FP64 at <=1e-5 err"
 
 
@@ -133,12 +163,14 @@ def main():
     t_ip = EventTimer(torch, st)
 
     tuned = None
+    rows64 = a.rows64
     if a.rows64 < 0:
         bt, _, bms, table = gt.autotune_batch(
             [dict(ctx=ctx, f_in=TS, f_out=TS2, uu=uu, vv=vv, ww=ww, hu=hun, hv=hvn, ho=ho,
                   hires=hires)], 2, dt, gt.P2_PDM, gt.P2_PDM, reps=5, stream=st)
         tuned = {"rows": gt.ROWS[bt], "ms": bms,
                  "table_ms": {str(gt.ROWS[t]): v for (t, _), v in table.items()}}
+        rows64 = gt.ROWS[bt]
 
     def step(timed):
         cur, nxt = bufs
@@ -176,7 +208,7 @@ def main():
     roof = {"bound": "hbm", "kernel": "k_strip_uv + k_strip_wvu <double, P2-PDM, 2 tracers> "
                                       "(one advection step, two launches)",
             "achieved": b_adv / adv_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": b_adv / adv_s / HBM_PEAK, "traffic": None,
+            "frac": b_adv / adv_s / HBM_PEAK, "traffic": measured_traffic("headline", rows64),
             "bytes_per_launch": b_adv, "launch_us": adv_s * 1e6,
             "bytes_per_cell_tracer": b_adv / (ncell * ntr)}
     # EOS (read T S, write buoy) + two internal-pressure methods (read buoy hn
@@ -263,6 +295,7 @@ def batched(torch, np, gt, inputs, dev, a, fp64_rate):
             bb.reverse()
 
     tuned = None
+    rows32, group = a.rows32, a.group
     if a.rows32 < 0 or a.group < 0:
         bt, bg, bms, table = gt.autotune_batch(
             [dict(ctx=dd["ctx"], f_in=fin32[s][0], f_out=fin32[s][1], uu=dd["uu"], vv=dd["vv"],
@@ -270,6 +303,7 @@ def batched(torch, np, gt, inputs, dev, a, fp64_rate):
              for s, dd in enumerate(doms)], nt, dt, hs, vs, reps=3, stream=st)
         tuned = {"rows": gt.ROWS[bt], "tracers_per_wave": bg, "ms": bms,
                  "table_ms": {f"{gt.ROWS[t]}x{g}": v for (t, g), v in table.items()}}
+        rows32, group = gt.ROWS[bt], bg
     timer = EventTimer(torch, st)
     nsteps = a.batched_steps
     with torch.cuda.stream(st):
@@ -317,6 +351,7 @@ def batched(torch, np, gt, inputs, dev, a, fp64_rate):
         "roofline": {"bound": "hbm", "kernel": "k_strip_uv + k_strip_wvu <float, P2-PDM>",
                      "achieved": bytes_step / kern / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": bytes_step / kern / HBM_PEAK,
+                     "traffic": measured_traffic("batched", rows32, group),
                      "bytes_per_launch": bytes_step},
         "vs_fp64_advection": rate / fp64_rate,
         "err_vs_fp64": {"max": emax, "rms": erms, "steps": nsteps + 2,
