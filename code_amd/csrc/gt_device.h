@@ -25,6 +25,32 @@ struct Dims {
   long NXY;
 };
 
+// ---- cross-lane moves (DPP wave shifts; call with every lane active) -------
+__device__ __forceinline__ int from_lower_lane(int v) {  // lane l gets lane l-1 (0 in lane 0)
+  return __builtin_amdgcn_mov_dpp(v, 0x138, 0xF, 0xF, true);
+}
+__device__ __forceinline__ int from_upper_lane(int v) {  // lane l gets lane l+1 (0 in lane 63)
+  return __builtin_amdgcn_mov_dpp(v, 0x130, 0xF, 0xF, true);
+}
+__device__ __forceinline__ float from_lower_lane(float v) {
+  return __builtin_bit_cast(float, from_lower_lane(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ float from_upper_lane(float v) {
+  return __builtin_bit_cast(float, from_upper_lane(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double from_lower_lane(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)from_lower_lane((int)(unsigned)b);
+  const unsigned hi = (unsigned)from_lower_lane((int)(unsigned)(b >> 32));
+  return __builtin_bit_cast(double, (unsigned long long)lo | ((unsigned long long)hi << 32));
+}
+__device__ __forceinline__ double from_upper_lane(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)from_upper_lane((int)(unsigned)b);
+  const unsigned hi = (unsigned)from_upper_lane((int)(unsigned)(b >> 32));
+  return __builtin_bit_cast(double, (unsigned long long)lo | ((unsigned long long)hi << 32));
+}
+
 template <typename T>
 __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
 template <typename T>

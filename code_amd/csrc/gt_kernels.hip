@@ -440,6 +440,7 @@ __global__ void k_bm_zz(Dims d, const int32_t* __restrict__ az, const double* __
   if (ii < 2 || ii > d.NX - 2 || jj < 2 || jj > d.NY - 2 || az[c0] < 1) return;
   double z = -H[c0] + hn[c0 + d.NXY];
   zz[c0 + d.NXY] = z;
+#pragma unroll 4
   for (int k = 2; k <= d.kmax; k++) {
     z = z + hn[c0 + d.NXY * k];
     zz[c0 + d.NXY * k] = z;
@@ -508,39 +509,39 @@ __global__ void k_bm_idpd(Dims d, const int32_t* __restrict__ au, const int32_t*
 #define SM_ONEFIFTH ((double)0.2f)
 #define SM_C12 ((double)(1.0f / 12.0f))
 
-// zz: layer centres over the full halo range where az>=1 (:90-99)
-__global__ void k_sm_zz(Dims d, const int32_t* __restrict__ az, const double* __restrict__ H,
-                        const double* __restrict__ hn, double* zz) {
-  int ii = blockIdx.x * blockDim.x + threadIdx.x;
-  int jj = blockIdx.y * blockDim.y + threadIdx.y;
-  if (ii >= d.NX || jj >= d.NY) return;
-  long c0 = ii + (long)d.NX * jj;
-  zz[c0] = 0.0;
-  if (az[c0] < 1) return;
-  double z = -H[c0] + 0.5 * hn[c0 + d.NXY];
-  zz[c0 + d.NXY] = z;
-  for (int k = 2; k <= d.kmax; k++) {
-    z = z + 0.5 * (hn[c0 + d.NXY * (k - 1)] + hn[c0 + d.NXY * k]);
-    zz[c0 + d.NXY * k] = z;
-  }
-}
-
-// vertical harmonic means and the pressure integral P (:108-146), one
-// thread per column over [imin..imax+1]x[jmin..jmax+1] where az>=1
+// One thread per column: the layer centres zz over the full halo range where
+// az>=1 (:90-99), then, on [imin..imax+1]x[jmin..jmax+1], the vertical
+// harmonic means and the pressure integral P (:108-146), reading back the
+// column's zz just written (one launch instead of two, zz and hn stay in
+// cache between the two loops).
 __device__ __forceinline__ double sm_hmR(double a, double b) {
   double cff = 2.0 * a * b;
   return cff > SM_EPS ? cff / (a + b) : 0.0;
 }
-__global__ void k_sm_P(Dims d, const int32_t* __restrict__ az, const double* __restrict__ buoy,
-                       const double* __restrict__ zz, const double* __restrict__ hn,
-                       double* P) {
+__global__ void k_sm_column(Dims d, const int32_t* __restrict__ az, const double* __restrict__ H,
+                            const double* __restrict__ hn, const double* __restrict__ buoy,
+                            double* zz, double* P) {
   int ii = blockIdx.x * blockDim.x + threadIdx.x;
   int jj = blockIdx.y * blockDim.y + threadIdx.y;
   if (ii >= d.NX || jj >= d.NY) return;
-  long c0 = ii + (long)d.NX * jj;
-  if (ii < 2 || ii > d.NX - 2 || jj < 2 || jj > d.NY - 2 || az[c0] < 1) return;
   const long NXY = d.NXY;
   const int kmax = d.kmax;
+  long c0 = ii + (long)d.NX * jj;
+  zz[c0] = 0.0;
+  if (az[c0] < 1) return;
+  {
+    double z = -H[c0] + 0.5 * hn[c0 + NXY];
+    zz[c0 + NXY] = z;
+    double hprev = hn[c0 + NXY];
+#pragma unroll 4
+    for (int k = 2; k <= kmax; k++) {
+      const double hk = hn[c0 + NXY * k];
+      z = z + 0.5 * (hprev + hk);
+      zz[c0 + NXY * k] = z;
+      hprev = hk;
+    }
+  }
+  if (ii < 2 || ii > d.NX - 2 || jj < 2 || jj > d.NY - 2) return;
 #define AT(a, k) (a)[c0 + NXY * (long)(k)]
   // raw differences: rawR(k) = b(k+1)-b(k) for k in 1..kmax-1,
   // rawR(kmax) = rawR(kmax-1), rawR(0) = rawR(1)
@@ -587,57 +588,105 @@ __device__ __forceinline__ double sm_hmx(double a, double b) {
   return 0.0;
 }
 
-// idpdx (:153-211) and idpdy (:214-287), one thread per cell
-__global__ void k_sm_idpd(Dims d, const int32_t* __restrict__ au, const int32_t* __restrict__ av,
-                          const double* __restrict__ buoy, const double* __restrict__ zz,
-                          const double* __restrict__ P, const double* __restrict__ hun,
-                          const double* __restrict__ hvn, const double* __restrict__ sseo,
-                          double* idpdx, double* idpdy, double dxm1, double dym1) {
-  int ii = blockIdx.x * blockDim.x + threadIdx.x;
-  int jj = blockIdx.y * blockDim.y + threadIdx.y;
-  int k = blockIdx.z;
-  if (ii >= d.NX || jj >= d.NY || k > d.kmax) return;
+// idpdx (:153-211) and idpdy (:214-287).  One wavefront per (strip of 60
+// columns x 16 rows, level): lane l owns column x = 60*sx - 1 + l, the
+// i-neighbours x-1..x+2 come from adjacent lanes (DPP), the j-neighbours
+// y-1..y+2 from a four-row window of the march, so every value is loaded
+// once.  Level 0 is zeroed; the arithmetic is the per-cell formula's.
+constexpr int kIpRows = 16;
+__global__ __launch_bounds__(256) void k_sm_idpd_strip(
+    Dims d, int nsx, int nsy, const int32_t* __restrict__ au, const int32_t* __restrict__ av,
+    const double* __restrict__ buoy, const double* __restrict__ zz, const double* __restrict__ P,
+    const double* __restrict__ hun, const double* __restrict__ hvn,
+    const double* __restrict__ sseo, double* idpdx, double* idpdy, double dxm1, double dym1) {
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const long item = (long)blockIdx.x * 4 + wave;
+  const long nitems = (long)nsx * nsy * (d.kmax + 1);
+  if (item >= nitems) return;
+  const int k = (int)(item % (d.kmax + 1));
+  const int sx = (int)((item / (d.kmax + 1)) % nsx);
+  const int sy = (int)(item / ((long)(d.kmax + 1) * nsx));
+  const int lane = threadIdx.x & 63;
+  const int NX = d.NX, NY = d.NY;
   const long NXY = d.NXY;
-  const long NX = d.NX;
-  long c0 = ii + NX * jj;
+  const int x = 60 * sx - 1 + lane;
+  const int y0 = sy * kIpRows;
+  const bool xout = lane >= 1 && lane <= 60 && x < NX && x >= 0;
+  const int xc = x < 0 ? 0 : (x >= NX ? NX - 1 : x);
   if (k == 0) {
-    idpdx[c0] = 0.0;
-    idpdy[c0] = 0.0;
+    for (int y = y0; y < y0 + kIpRows && y < NY; y++) {
+      if (!xout) continue;
+      const long c = (long)x + (long)NX * y;
+      idpdx[c] = 0.0;
+      idpdy[c] = 0.0;
+    }
     return;
   }
-  if (ii < 2 || ii > d.NX - 3 || jj < 2 || jj > d.NY - 3) return;
   const long o = NXY * (long)k, okm = NXY * (long)d.kmax;
-  const double* b = buoy + o;
-  const double* z = zz + o;
-  if (au[c0] >= 1) {
-    long c = c0;
-    double zrm = z[c] - z[c - 1], zr0 = z[c + 1] - z[c], zrp = z[c + 2] - z[c + 1];
-    double brm = b[c] - b[c - 1], br0 = b[c + 1] - b[c], brp = b[c + 2] - b[c + 1];
-    double dZx0 = sm_hmx(zrm, zr0), dZx1 = sm_hmx(zr0, zrp);
-    double dRx0 = sm_hmx(brm, br0), dRx1 = sm_hmx(br0, brp);
-    double AJ = P[o + c + 1] - P[o + c];
-    double FC = 0.5 * ((b[c + 1] + b[c]) * (z[c + 1] - z[c]) -
-                       SM_ONEFIFTH * ((dRx1 - dRx0) * (z[c + 1] - z[c] - SM_C12 * (dZx1 + dZx0)) -
-                                      (dZx1 - dZx0) * (b[c + 1] - b[c] - SM_C12 * (dRx1 + dRx0))));
-    AJ = AJ + FC;
-    double h = hun[o + c];
-    idpdx[o + c] = AJ * dxm1 * h - h * dxm1 * (sseo[c + 1] - sseo[c]) * 0.5 *
-                                       (buoy[okm + c + 1] + buoy[okm + c]);
-  }
-  if (av[c0] >= 1) {
-    long c = c0;
-    double zrm = z[c] - z[c - NX], zr0 = z[c + NX] - z[c], zrp = z[c + 2 * NX] - z[c + NX];
-    double brm = b[c] - b[c - NX], br0 = b[c + NX] - b[c], brp = b[c + 2 * NX] - b[c + NX];
-    double dZx0 = sm_hmx(zrm, zr0), dZx1 = sm_hmx(zr0, zrp);
-    double dRx0 = sm_hmx(brm, br0), dRx1 = sm_hmx(br0, brp);
-    double AJ = P[o + c + NX] - P[o + c];
-    double FC = 0.5 * ((b[c + NX] + b[c]) * (z[c + NX] - z[c]) -
-                       SM_ONEFIFTH * ((dRx1 - dRx0) * (z[c + NX] - z[c] - SM_C12 * (dZx1 + dZx0)) -
-                                      (dZx1 - dZx0) * (b[c + NX] - b[c] - SM_C12 * (dRx1 + dRx0))));
-    AJ = AJ + FC;
-    double h = hvn[o + c];
-    idpdy[o + c] = AJ * dym1 * h - h * dym1 * (sseo[c + NX] - sseo[c]) * 0.5 *
-                                       (buoy[okm + c + NX] + buoy[okm + c]);
+  const bool xint = x >= 2 && x <= NX - 3;
+  auto cell = [&](int y) -> long {
+    const int yc = y < 0 ? 0 : (y >= NY ? NY - 1 : y);
+    return (long)xc + (long)NX * yc;
+  };
+  // window of rows y-1, y, y+1, y+2 (index 0..3) for b and z; rows y, y+1 of
+  // P, sseo and buoy(kmax) are indices 1, 2
+  double bw[4], zw[4], pw[4], sw[4], kw[4];
+  auto load = [&](int y, int q) {
+    const long c = cell(y);
+    bw[q] = buoy[o + c];
+    zw[q] = zz[o + c];
+    pw[q] = P[o + c];
+    sw[q] = sseo[c];
+    kw[q] = buoy[okm + c];
+  };
+  load(y0 - 1, 0);
+  load(y0, 1);
+  load(y0 + 1, 2);
+  for (int y = y0; y < y0 + kIpRows; y++) {
+    load(y + 2, 3);
+    const bool ok = xout && xint && y >= 2 && y <= NY - 3;
+    const long c = (long)xc + (long)NX * (y < NY ? y : NY - 1);
+    // x: cells x-1..x+2 of row y
+    const double b0 = bw[1], z0 = zw[1];
+    const double bm = from_lower_lane(b0), zm = from_lower_lane(z0);
+    const double b1 = from_upper_lane(b0), z1 = from_upper_lane(z0);
+    const double b2 = from_upper_lane(b1), z2 = from_upper_lane(z1);
+    const double p1 = from_upper_lane(pw[1]), s1 = from_upper_lane(sw[1]);
+    const double k1 = from_upper_lane(kw[1]);
+    if (ok && au[c] >= 1) {
+      double zrm = z0 - zm, zr0 = z1 - z0, zrp = z2 - z1;
+      double brm = b0 - bm, br0 = b1 - b0, brp = b2 - b1;
+      double dZx0 = sm_hmx(zrm, zr0), dZx1 = sm_hmx(zr0, zrp);
+      double dRx0 = sm_hmx(brm, br0), dRx1 = sm_hmx(br0, brp);
+      double AJ = p1 - pw[1];
+      double FC = 0.5 * ((b1 + b0) * (z1 - z0) -
+                         SM_ONEFIFTH * ((dRx1 - dRx0) * (z1 - z0 - SM_C12 * (dZx1 + dZx0)) -
+                                        (dZx1 - dZx0) * (b1 - b0 - SM_C12 * (dRx1 + dRx0))));
+      AJ = AJ + FC;
+      double h = hun[o + c];
+      idpdx[o + c] = AJ * dxm1 * h - h * dxm1 * (s1 - sw[1]) * 0.5 * (k1 + kw[1]);
+    }
+    if (ok && av[c] >= 1) {
+      double zrm = zw[1] - zw[0], zr0 = zw[2] - zw[1], zrp = zw[3] - zw[2];
+      double brm = bw[1] - bw[0], br0 = bw[2] - bw[1], brp = bw[3] - bw[2];
+      double dZx0 = sm_hmx(zrm, zr0), dZx1 = sm_hmx(zr0, zrp);
+      double dRx0 = sm_hmx(brm, br0), dRx1 = sm_hmx(br0, brp);
+      double AJ = pw[2] - pw[1];
+      double FC = 0.5 * ((bw[2] + bw[1]) * (zw[2] - zw[1]) -
+                         SM_ONEFIFTH * ((dRx1 - dRx0) * (zw[2] - zw[1] - SM_C12 * (dZx1 + dZx0)) -
+                                        (dZx1 - dZx0) * (bw[2] - bw[1] - SM_C12 * (dRx1 + dRx0))));
+      AJ = AJ + FC;
+      double h = hvn[o + c];
+      idpdy[o + c] = AJ * dym1 * h - h * dym1 * (sw[2] - sw[1]) * 0.5 * (kw[2] + kw[1]);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      bw[q] = bw[q + 1];
+      zw[q] = zw[q + 1];
+      pw[q] = pw[q + 1];
+      sw[q] = sw[q + 1];
+      kw[q] = kw[q + 1];
+    }
   }
 }
 
@@ -925,14 +974,13 @@ hipError_t launch_ip_smcw(hipStream_t st, Dims d, const int32_t* az, const int32
                           const double* sseo, double* zz, double* P, double* idpdx,
                           double* idpdy, double dx, double dy) {
   const double dxm1 = 1.0 / dx, dym1 = 1.0 / dy;
-  hipLaunchKernelGGL(k_sm_zz, grid2(d, 1, kBlk2), kBlk2, 0, st, d, az, H, hn, zz);
+  hipLaunchKernelGGL(k_sm_column, grid2(d, 1, kBlk2), kBlk2, 0, st, d, az, H, hn, buoy, zz, P);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_sm_P, grid2(d, 1, kBlk2), kBlk2, 0, st, d, az, buoy, zz, hn, P);
-  e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_sm_idpd, grid2(d, d.kmax + 1, kBlk2), kBlk2, 0, st, d, au, av, buoy, zz,
-                     P, hun, hvn, sseo, idpdx, idpdy, dxm1, dym1);
+  const int nsx = (d.NX + 59) / 60, nsy = (d.NY + kIpRows - 1) / kIpRows;
+  const long nitems = (long)nsx * nsy * (d.kmax + 1);
+  hipLaunchKernelGGL(k_sm_idpd_strip, dim3((unsigned)((nitems + 3) / 4)), dim3(256), 0, st, d,
+                     nsx, nsy, au, av, buoy, zz, P, hun, hvn, sseo, idpdx, idpdy, dxm1, dym1);
   return hipGetLastError();
 }
 

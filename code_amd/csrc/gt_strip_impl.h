@@ -44,32 +44,6 @@
 
 namespace gt {
 
-// ---- cross-lane moves (DPP wave shifts; call with every lane active) -------
-__device__ __forceinline__ int from_lower_lane(int v) {  // lane l gets lane l-1 (0 in lane 0)
-  return __builtin_amdgcn_mov_dpp(v, 0x138, 0xF, 0xF, true);
-}
-__device__ __forceinline__ int from_upper_lane(int v) {  // lane l gets lane l+1 (0 in lane 63)
-  return __builtin_amdgcn_mov_dpp(v, 0x130, 0xF, 0xF, true);
-}
-__device__ __forceinline__ float from_lower_lane(float v) {
-  return __builtin_bit_cast(float, from_lower_lane(__builtin_bit_cast(int, v)));
-}
-__device__ __forceinline__ float from_upper_lane(float v) {
-  return __builtin_bit_cast(float, from_upper_lane(__builtin_bit_cast(int, v)));
-}
-__device__ __forceinline__ double from_lower_lane(double v) {
-  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-  const unsigned lo = (unsigned)from_lower_lane((int)(unsigned)b);
-  const unsigned hi = (unsigned)from_lower_lane((int)(unsigned)(b >> 32));
-  return __builtin_bit_cast(double, (unsigned long long)lo | ((unsigned long long)hi << 32));
-}
-__device__ __forceinline__ double from_upper_lane(double v) {
-  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-  const unsigned lo = (unsigned)from_upper_lane((int)(unsigned)b);
-  const unsigned hi = (unsigned)from_upper_lane((int)(unsigned)(b >> 32));
-  return __builtin_bit_cast(double, (unsigned long long)lo | ((unsigned long long)hi << 32));
-}
-
 // 1/x to about one FP64 ulp: hardware estimate and one Newton step (fast path)
 __device__ __forceinline__ double rcp_nr(double x) {
   const double r = __builtin_amdgcn_rcp(x);
@@ -85,7 +59,7 @@ struct FaceC;
 // EXACT: the values the reference's flux expression reads
 template <>
 struct FaceC<true> {
-  double vel, cfl, Dv;
+  double vel, cfl, Dv, E;  // E = 2/(1-cfl), P2-PDM's third bound, formed once
   uint8_t code;
 };
 // fast: vel = 0 for an inactive face, g = 0 where the limiter does not apply
@@ -110,20 +84,42 @@ __device__ __forceinline__ FaceK face_k(double dti, double dlen, double AH) {
 template <int SCH>
 __device__ __forceinline__ void face_setup(FaceC<true>& fc, bool active, bool lim_l, bool lim_r,
                                            double vel, double Dv, const FaceK& k) {
+  // one division chain for both directions: the reference's vel/Dv*dti/dlen
+  // going right, -vel/Dv*dti/dlen going left, with the operand selected
+  // first (a wavefront holding both directions runs the chain once)
   const double dti = k.dti, dlen = k.dlen;
   fc.vel = vel;
   fc.Dv = Dv;
   fc.cfl = 0.0;
-  if (!active) {
-    fc.code = F_NONE;
-  } else if (vel > 0.0) {
-    const bool lim = SCH != UPSTREAM && lim_l;
-    fc.code = lim ? F_LEFT_LIM : F_LEFT;
-    if (lim) fc.cfl = vel / Dv * dti / dlen;
+  const bool left = vel > 0.0;
+  const bool lim = active && SCH != UPSTREAM && (left ? lim_l : lim_r);
+  fc.code = !active ? F_NONE : (left ? (lim ? F_LEFT_LIM : F_LEFT) : (lim ? F_RIGHT_LIM : F_RIGHT));
+  if (lim) fc.cfl = (left ? vel : -vel) / Dv * dti / dlen;
+  fc.E = 0.0;
+  if (SCH == P2_PDM && lim) fc.E = 2.0 / (1.0 - fc.cfl);
+}
+
+// adv_interfacial_reconstruction (src/2d/advection.F90:806-845) with P2-PDM's
+// 2/(1-cfl) taken from the face (E) instead of formed per tracer; every other
+// operation is reconstruct()'s, in its order, so the result is identical
+template <int SCH>
+__device__ __forceinline__ double reconstruct_face(double cfl, double E, double fuu, double fu,
+                                                   double fd) {
+  if constexpr (SCH != P2_PDM) {
+    (void)E;
+    return reconstruct<double>(SCH, cfl, fuu, fu, fd);
   } else {
-    const bool lim = SCH != UPSTREAM && lim_r;
-    fc.code = lim ? F_RIGHT_LIM : F_RIGHT;
-    if (lim) fc.cfl = -vel / Dv * dti / dlen;
+    const double one6th = 1.0 / 6.0;
+    const double deltaf = fd - fu;
+    const double deltafu = fu - fuu;
+    if (deltaf * deltafu > 0.0) {
+      const double ratio = deltafu / deltaf;
+      const double x = one6th * (1.0 - 2.0 * cfl);
+      double limiter = (0.5 + x) + (0.5 - x) * ratio;
+      limiter = tmin(tmin(2.0 * ratio / (cfl + 1.e-10), limiter), E);
+      return fu + 0.5 * limiter * (1.0 - cfl) * deltaf;
+    }
+    return fu;
   }
 }
 
@@ -188,14 +184,13 @@ template <typename TF, int SCH, bool DIFF>
 __device__ __forceinline__ TF face_flux(const FaceC<true>& fc, TF fm1, TF f0, TF fp1, TF fp2,
                                         double AH, double dlen) {
   if (fc.code == F_NONE) return TF(0);
-  TF fu;
-  if (fc.code <= F_LEFT_LIM) {
-    fu = f0;
-    if (fc.code == F_LEFT_LIM) fu = reconstruct<double>(SCH, fc.cfl, fm1, f0, fp1);
-  } else {
-    fu = fp1;
-    if (fc.code == F_RIGHT_LIM) fu = reconstruct<double>(SCH, fc.cfl, fp2, fp1, f0);
-  }
+  // upwind operands selected first, then one reconstruction for both
+  // directions (left: fm1, f0, fp1; right: fp2, fp1, f0)
+  const bool left = fc.code <= F_LEFT_LIM;
+  const TF fuu = left ? fm1 : fp2, fd = left ? fp1 : f0;
+  TF fu = left ? f0 : fp1;
+  if (fc.code == F_LEFT_LIM || fc.code == F_RIGHT_LIM)
+    fu = reconstruct_face<SCH>(fc.cfl, fc.E, fuu, fu, fd);
   TF fl = fc.vel * fu;
   if constexpr (DIFF) fl = fl - AH * fc.Dv * (fp1 - f0) / dlen;
   return fl;
@@ -244,7 +239,7 @@ template <bool EXACT>
 struct WFace;
 template <>
 struct WFace<true> {
-  double w, cfl;
+  double w, cfl, E;
 };
 template <>
 struct WFace<false> {
@@ -257,6 +252,7 @@ __device__ __forceinline__ void wface_setup(WFace<true>& wf, double w, double dt
                                             double Dhi) {
   wf.w = w;
   wf.cfl = __builtin_fabs(w) * dt / (0.5 * (Dlo + Dhi));
+  wf.E = SCH == P2_PDM ? 2.0 / (1.0 - wf.cfl * 1.0) : 0.0;
 }
 template <int SCH>
 __device__ __forceinline__ void wface_setup(WFace<false>& wf, double w, double dt, double Dlo,
@@ -286,7 +282,7 @@ __device__ __forceinline__ TF wflux(const WFace<EXACT>& wf, int k, int kshift, i
     if (lim) { fuu = fk2; fd = fk; }
   }
   if constexpr (EXACT) {
-    if (lim) fu = reconstruct<double>(SCH, wf.cfl * 1.0, fuu, fu, fd);
+    if (lim) fu = reconstruct_face<SCH>(wf.cfl * 1.0, wf.E, fuu, fu, fd);
   } else {
     fu = rec_fast<SCH>(fuu, fu, fd, lim ? wf.g : 0.0f, wf.A, wf.al, wf.E);
   }

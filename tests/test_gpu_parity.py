@@ -261,9 +261,12 @@ def ip_fields(seed=2, imax=30, jmax=26, kmax=9):
     return tr, d, T, S, sseo
 
 
-def test_eos_and_internal_pressure(device):
+@pytest.mark.parametrize("size", [(30, 26, 9), (130, 70, 7)])
+def test_eos_and_internal_pressure(device, size):
+    """one strip and several strips (60 columns x 16 rows) of the SMcW
+    kernel; bit-exact against the oracle"""
     import torch
-    tr, d, T, S, sseo = ip_fields()
+    tr, d, T, S, sseo = ip_fields(imax=size[0], jmax=size[1], kmax=size[2])
     ctx = gt.Context.from_domain(d)
     orc = po.Oracle.from_domain(d)
     consts = (inputs.EOS_T0, inputs.EOS_S0, inputs.EOS_DTR0, inputs.EOS_DSR0)
@@ -292,6 +295,7 @@ def test_eos_and_internal_pressure(device):
         for got, ref, nm in ((tz, zz, "zz"), (tx, idx, "idpdx"), (ty, idy, "idpdy")):
             e = rel_err(to_host(got), ref)
             assert e <= TOL, (method, nm, e)
+            assert np.array_equal(to_host(got), ref), (method, nm, "not bit-exact")
 
 
 def test_halo_fill(case):
