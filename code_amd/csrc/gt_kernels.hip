@@ -447,6 +447,9 @@ __global__ void k_bm_zz(Dims d, const int32_t* __restrict__ az, const double* __
   }
 }
 
+// idpdx and idpdy (:99-149) in one downward loop per column (the column's
+// own buoy and zz are loaded once for both directions); each recurrence is
+// the reference's, in its order.
 __global__ void k_bm_idpd(Dims d, const int32_t* __restrict__ au, const int32_t* __restrict__ av,
                           const double* __restrict__ buoy, const double* __restrict__ zz,
                           const double* __restrict__ hun, const double* __restrict__ hvn,
@@ -460,43 +463,44 @@ __global__ void k_bm_idpd(Dims d, const int32_t* __restrict__ au, const int32_t*
   idpdx[c0] = 0.0;
   idpdy[c0] = 0.0;
   if (ii < 2 || ii > d.NX - 3 || jj < 2 || jj > d.NY - 3) return;
+  const bool ux = au[c0] >= 1, vy = av[c0] >= 1;
+  if (!ux && !vy) return;
 #define AT(a, q, k) (a)[(q) + NXY * (long)(k)]
-  if (au[c0] >= 1) {
-    const long e = c0 + 1;
-    double grdl = (AT(buoy, e, kmax) - AT(buoy, c0, kmax)) * dxm1;
-    double buoyl = 0.5 * (AT(buoy, e, kmax) + AT(buoy, c0, kmax));
-    double hk = AT(hun, c0, kmax);
-    double prgr = grdl * 0.5 * hk;
-    AT(idpdx, c0, kmax) = hk * prgr;
-    for (int k = kmax - 1; k >= 1; k--) {
-      double grdu = grdl;
-      grdl = (AT(buoy, e, k) - AT(buoy, c0, k)) * dxm1;
-      double buoyu = buoyl;
-      buoyl = 0.5 * (AT(buoy, e, k) + AT(buoy, c0, k));
-      double dxz = (AT(zz, e, k) - AT(zz, c0, k)) * dxm1;
+  const long e = c0 + 1, n = c0 + d.NX;
+  double b0 = AT(buoy, c0, kmax), be = AT(buoy, e, kmax), bn = AT(buoy, n, kmax);
+  double grdlx = (be - b0) * dxm1, buoylx = 0.5 * (be + b0), hkx = AT(hun, c0, kmax);
+  double prgrx = grdlx * 0.5 * hkx;
+  double grdly = (bn - b0) * dym1, buoyly = 0.5 * (bn + b0), hky = AT(hvn, c0, kmax);
+  double prgry = grdly * 0.5 * hky;
+  if (ux) AT(idpdx, c0, kmax) = hkx * prgrx;
+  if (vy) AT(idpdy, c0, kmax) = hky * prgry;
+#pragma unroll 4
+  for (int k = kmax - 1; k >= 1; k--) {
+    b0 = AT(buoy, c0, k);
+    be = AT(buoy, e, k);
+    bn = AT(buoy, n, k);
+    const double z0 = AT(zz, c0, k), ze = AT(zz, e, k), zn = AT(zz, n, k);
+    {
+      double grdu = grdlx;
+      grdlx = (be - b0) * dxm1;
+      double buoyu = buoylx;
+      buoylx = 0.5 * (be + b0);
+      double dxz = (ze - z0) * dxm1;
       double hkk = AT(hun, c0, k);
-      prgr = prgr + 0.5 * (grdu + grdl) * 0.5 * (hkk + hk) - dxz * (buoyu - buoyl);
-      AT(idpdx, c0, k) = hkk * prgr;
-      hk = hkk;
+      prgrx = prgrx + 0.5 * (grdu + grdlx) * 0.5 * (hkk + hkx) - dxz * (buoyu - buoylx);
+      if (ux) AT(idpdx, c0, k) = hkk * prgrx;
+      hkx = hkk;
     }
-  }
-  if (av[c0] >= 1) {
-    const long n = c0 + d.NX;
-    double grdl = (AT(buoy, n, kmax) - AT(buoy, c0, kmax)) * dym1;
-    double buoyl = 0.5 * (AT(buoy, n, kmax) + AT(buoy, c0, kmax));
-    double hk = AT(hvn, c0, kmax);
-    double prgr = grdl * 0.5 * hk;
-    AT(idpdy, c0, kmax) = hk * prgr;
-    for (int k = kmax - 1; k >= 1; k--) {
-      double grdu = grdl;
-      grdl = (AT(buoy, n, k) - AT(buoy, c0, k)) * dym1;
-      double buoyu = buoyl;
-      buoyl = 0.5 * (AT(buoy, n, k) + AT(buoy, c0, k));
-      double dyz = (AT(zz, n, k) - AT(zz, c0, k)) * dym1;
+    {
+      double grdu = grdly;
+      grdly = (bn - b0) * dym1;
+      double buoyu = buoyly;
+      buoyly = 0.5 * (bn + b0);
+      double dyz = (zn - z0) * dym1;
       double hkk = AT(hvn, c0, k);
-      prgr = prgr + 0.5 * (grdu + grdl) * 0.5 * (hkk + hk) - dyz * (buoyu - buoyl);
-      AT(idpdy, c0, k) = hkk * prgr;
-      hk = hkk;
+      prgry = prgry + 0.5 * (grdu + grdly) * 0.5 * (hkk + hky) - dyz * (buoyu - buoyly);
+      if (vy) AT(idpdy, c0, k) = hkk * prgry;
+      hky = hkk;
     }
   }
 #undef AT
@@ -561,6 +565,7 @@ __global__ void k_sm_column(Dims d, const int32_t* __restrict__ az, const double
   double rk1 = rawR(kmax), zk1 = rawZ(kmax);      // raw at k+1
   double rk = rawR(kmax - 1), zk = rawZ(kmax - 1);  // raw at k
   double hR1 = sm_hmR(rk1, rk), hZ1 = hmZ(zk1, zk);  // harmonic at kmax
+#pragma unroll 4
   for (int k = kmax - 1; k >= 1; k--) {
     double rkm = rawR(k - 1), zkm = rawZ(k - 1);
     double hR = sm_hmR(rk, rkm), hZ = hmZ(zk, zkm);
