@@ -163,14 +163,14 @@ def main():
     t_ip = EventTimer(torch, st)
 
     tuned = None
-    rows64 = a.rows64
+    rows64, g64 = a.rows64, 2
     if a.rows64 < 0:
-        bt, _, bms, table = gt.autotune_batch(
+        bt, bg, bms, table = gt.autotune_batch(
             [dict(ctx=ctx, f_in=TS, f_out=TS2, uu=uu, vv=vv, ww=ww, hu=hun, hv=hvn, ho=ho,
                   hires=hires)], 2, dt, gt.P2_PDM, gt.P2_PDM, reps=5, stream=st)
-        tuned = {"rows": gt.ROWS[bt], "ms": bms,
-                 "table_ms": {str(gt.ROWS[t]): v for (t, _), v in table.items()}}
-        rows64 = gt.ROWS[bt]
+        tuned = {"rows": gt.ROWS[bt], "tracers_per_wave": bg, "ms": bms,
+                 "table_ms": {f"{gt.ROWS[t]}x{g}": v for (t, g), v in table.items()}}
+        rows64, g64 = gt.ROWS[bt], bg
 
     def step(timed):
         cur, nxt = bufs
@@ -205,10 +205,10 @@ def main():
     # algorithmic bytes of one advection step (T and S, FP64): each tracer
     # read and written once, uu vv ww hu hv ho read and hires written once
     b_adv = ncell * (ntr * 2 * 8 + 7 * 8)
-    roof = {"bound": "hbm", "kernel": "k_strip_uv + k_strip_wvu <double, P2-PDM, 2 tracers> "
-                                      "(one advection step, two launches)",
+    roof = {"bound": "hbm", "kernel": f"k_strip_uv + k_strip_wvu <double, P2-PDM, {g64} "
+                                      "tracer(s) per wave> (one advection step, two launches)",
             "achieved": b_adv / adv_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": b_adv / adv_s / HBM_PEAK, "traffic": measured_traffic("headline", rows64),
+            "frac": b_adv / adv_s / HBM_PEAK, "traffic": measured_traffic("headline", rows64, g64),
             "bytes_per_launch": b_adv, "launch_us": adv_s * 1e6,
             "bytes_per_cell_tracer": b_adv / (ncell * ntr)}
     # EOS (read T S, write buoy) + two internal-pressure methods (read buoy hn

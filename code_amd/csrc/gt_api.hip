@@ -782,8 +782,8 @@ extern "C" int gt_set_tuning(gt_ctx* c, int fused, int tile, int group) {
   if (!c) return set_err(GT_EARG, "gt_set_tuning: NULL context");
   if (tile < 0 || tile > 2)
     return set_err(GT_EARG, "gt_set_tuning: tile (rows per strip 16, 32, 8) must be 0, 1 or 2");
-  if (group != 1 && group != 3 && group != 6 && group != 9)
-    return set_err(GT_EARG, "gt_set_tuning: group must be 1, 3, 6 or 9");
+  if (group != 1 && group != 2 && group != 3 && group != 6 && group != 9)
+    return set_err(GT_EARG, "gt_set_tuning: group must be 1, 2, 3, 6 or 9");
   c->fused = fused != 0;
   c->tile = tile;
   c->group = group;
@@ -814,8 +814,9 @@ static int batch_impl(int ndom, const gt_batch_domain* doms, int ntracers, doubl
   }
   gt_ctx* c0 = doms[0].ctx;
   hipStream_t st = stream ? (hipStream_t)stream : c0->st;
-  // FP64: two tracers per wavefront (T and S) when there are two or more
-  const int group = sizeof(TF) == 4 ? c0->group : (ntracers >= 2 ? 2 : 1);
+  // FP64: two tracers per wavefront (T and S) when there are two or more,
+  // unless the tuning asks for one
+  const int group = sizeof(TF) == 4 ? c0->group : (ntracers >= 2 && c0->group != 1 ? 2 : 1);
   for (int i = 0; i < ndom; i++) {
     gt_ctx* c = doms[i].ctx;
     TRY(ensure_ws(c, strip_ws_bytes(c, sizeof(TF), ntracers, false)));
@@ -871,6 +872,7 @@ extern "C" int gt_autotune_batch(int ndom, const gt_batch_domain* doms, int ntra
   hipStream_t st = stream ? (hipStream_t)stream : doms[0].ctx->st;
   const int tiles[3] = {0, 1, 2};
   const int groups[4] = {1, 3, 6, 9};
+  const int groups64[2] = {1, 2};
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
@@ -878,10 +880,11 @@ extern "C" int gt_autotune_batch(int ndom, const gt_batch_domain* doms, int ntra
   int bt = 0, bg = 3;
   int n = 0;
   for (int ti = 0; ti < 3; ti++)
-    for (int gi = 0; gi < (f32 ? 4 : 1); gi++) {
+    for (int gi = 0; gi < (f32 ? 4 : 2); gi++) {
+      const int grp = f32 ? groups[gi] : groups64[gi];
       for (int i = 0; i < ndom; i++) {
         doms[i].ctx->tile = tiles[ti];
-        doms[i].ctx->group = groups[gi];
+        doms[i].ctx->group = grp;
       }
       auto run = [&]() -> int {
         return f32 ? gt_adv3d_batch_f32(ndom, doms, ntracers, dt, hscheme, vscheme, AH, tag, st)
@@ -905,7 +908,7 @@ extern "C" int gt_autotune_batch(int ndom, const gt_batch_domain* doms, int ntra
       if (ms < best) {
         best = ms;
         bt = tiles[ti];
-        bg = groups[gi];
+        bg = grp;
       }
     }
   (void)hipEventDestroy(e0);

@@ -343,7 +343,7 @@ def autotune_batch(domains, ntracers, dt, hscheme, vscheme, AH=-1.0, tag=H_TAG, 
     check(lib.gt_autotune_batch(len(domains), arr, ntracers, dt, hscheme, vscheme, AH, tag,
                                 int(f32), reps, C.c_void_p(handle) if handle else None,
                                 C.byref(bt), C.byref(bg), C.byref(bm), table))
-    pairs = [(t, g) for t in (0, 1, 2) for g in ((1, 3, 6, 9) if f32 else (1,))]
+    pairs = [(t, g) for t in (0, 1, 2) for g in ((1, 3, 6, 9) if f32 else (1, 2))]
     return bt.value, bg.value, bm.value, {p: table[i] for i, p in enumerate(pairs)}
 
 
