@@ -78,6 +78,7 @@ def parse():
                    help="bound on the CPU-baseline sample")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-batched", action="store_true")
+    p.add_argument("--no-adjoint", action="store_true")
     p.add_argument("--batched-steps", type=int, default=20)
     p.add_argument("--rows64", type=int, default=-1,
                    help="FP64 headline: rows per strip 8, 16 or 32 (-1: autotuned)")
@@ -240,6 +241,8 @@ def main():
     del TS2, idpdx6, idpdy6
     if not a.no_batched:
         out["batched_fp32"] = batched(torch, np, gt, inputs, dev, a, ncell * ntr / adv_s)
+    if not a.no_adjoint:
+        out["adjoint"] = adjoint_section(torch, np, gt, inputs, dev)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(tr, a.cpu_seconds)
     print(json.dumps(out))
@@ -358,6 +361,73 @@ def batched(torch, np, gt, inputs, dev, a, fp64_rate):
                         "norm": "|f32-f64| / max|f64| over wet cells, worst tracer"},
         "mass_drift_max_rel": drift,
     }
+
+
+def adjoint_section(torch, np, gt, inputs, dev, nsteps=48, ckpt=8):
+    """Reverse mode at 256x256x30 (sigma, Superbee HALFSPLIT): the gradient of
+    J = sum_{32x32 box, all k} c^2 (final c) + sum_box w*idpdx(T_final)
+    (Blumberg-Mellor of the linear-EOS buoyancy) with respect to the initial c
+    and T, by 48 forward steps with a checkpoint every 8 and the reverse sweep;
+    velocities and layer heights are prescribed."""
+    tr = inputs.adjoint_case()
+    d = tr.dom
+    st = torch.cuda.Stream(device=dev)
+    ctx = gt.Context.from_domain(d, stream=st)
+    D = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)  # noqa: E731
+    uu, vv, ww, hu, hv, ho, hn = (D(x) for x in (tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho,
+                                                 tr.hn))
+    H = D(d.H)
+    c0, T0, S0 = D(tr.tracers["c"]), D(tr.tracers["T"]), D(tr.tracers["S"])
+    consts = (inputs.EOS_T0, inputs.EOS_S0, inputs.EOS_DTR0, inputs.EOS_DSR0)
+    j0, i0 = d.NY // 2 - 16, d.NX // 2 - 16
+    w = torch.zeros_like(c0)
+    w[1:, j0:j0 + 32, i0:i0 + 32] = D(inputs.uniform((d.kmax, 32, 32), 17, -1, 1))
+    cF, TF, lam_c, g_c, g_T = (torch.empty_like(c0) for _ in range(5))
+    buoy, zz, ix, iy, bbar, Tbar, zero = (torch.zeros_like(c0) for _ in range(7))
+
+    def forward(f0, f):
+        f.copy_(f0)
+        for _ in range(nsteps):
+            ctx.do_advection_3d(tr.dt, f, uu, vv, ww, hu, hv, ho, hn, gt.HALFSPLIT, gt.SUPERBEE,
+                                gt.SUPERBEE, -1.0, gt.H_TAG)
+
+    def gradient():
+        # c: lam = 2 c_final in the box
+        forward(c0, cF)
+        lam_c.zero_()
+        lam_c[1:, j0:j0 + 32, i0:i0 + 32] = 2.0 * cF[1:, j0:j0 + 32, i0:i0 + 32]
+        ctx.adv3d_adjoint(nsteps, ckpt, tr.dt, c0, lam_c, g_c, uu, vv, ww, hu, hv, ho,
+                          gt.SUPERBEE, gt.SUPERBEE)
+        # T: forward to T_final, IP of its buoyancy, then EOS^T BM^T w and the
+        # advection adjoint
+        forward(T0, TF)
+        ctx.eos_linear(TF, S0, None, buoy, *consts)
+        ctx.ip_blumberg_mellor(H, hn, hu, hv, buoy, zz, ix, iy)
+        bbar.zero_()
+        Tbar.zero_()
+        ctx.ip_blumberg_mellor_adjoint(hu, hv, zz, w, zero, bbar)
+        ctx.eos_linear_adjoint(bbar, Tbar, None, inputs.EOS_DTR0, inputs.EOS_DSR0)
+        ctx.adv3d_adjoint(nsteps, ckpt, tr.dt, T0, Tbar, g_T, uu, vv, ww, hu, hv, ho,
+                          gt.SUPERBEE, gt.SUPERBEE)
+
+    with torch.cuda.stream(st):
+        gradient()
+        st.synchronize()
+        t0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            gradient()
+        st.synchronize()
+        t = (time.perf_counter() - t0) / reps
+    ncell = d.imax * d.jmax * d.kmax
+    return {"workload": f"{d.imax}x{d.jmax}x{d.kmax} sigma, Superbee HALFSPLIT, {nsteps} steps, "
+                        f"checkpoint every {ckpt}; dJ/dc0 and dJ/dT0 (J: box c^2 + BM term)",
+            "ms_per_gradient": t * 1e3,
+            "gradient_cell_steps_per_s": ncell * nsteps * 2 / t,
+            "note": "per tracer: 48 forward steps for the cost, then the adjoint call (48 "
+                    "forward steps to the checkpoints, 48 recomputed with the sub-step "
+                    "states, 48 reverse steps)",
+            "grad_norm_c": float(g_c.norm()), "grad_norm_T": float(g_T.norm())}
 
 
 def cpu_baseline(tr, seconds):

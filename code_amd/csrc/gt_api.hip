@@ -1032,13 +1032,22 @@ extern "C" int gt_adv3d_adjoint(gt_ctx* c, int nsteps, int ckpt_every, double dt
   double** ck = &c->adj[14];
   double** sg = &c->adj[14 + nck];
   const long N3 = c->N3;
-  // forward sweep, keeping a checkpoint every ckpt_every steps
+  // forward sweep, keeping a checkpoint every ckpt_every steps; the plain
+  // step (the strip kernels, bit-identical to the captured sub-steps) where
+  // it applies, since nothing but the checkpoints is kept here
   TRY(copy_d2d(c, work, f0, N3));
+  const bool plain = c->fused && c->adv_ver_iterations == 1 &&
+                     gt::fused_supported(hscheme, vscheme) && strip_fits(c);
   for (int t = 0; t < nsteps; t++) {
     if (t % ckpt_every == 0) TRY(copy_d2d(c, ck[t / ckpt_every], work, N3));
-    TRY(forward_capture(c, g, dt, work, sg[0], uu, vv, ww, hu, hv, ho, hscheme, vscheme, AH,
-                        kshift, D, S));
-    TRY(copy_d2d(c, work, sg[0], N3));
+    if (plain) {
+      TRY(gt_do_advection_3d(c, dt, work, uu, vv, ww, hu, hv, ho, ho, GT_HALFSPLIT, hscheme,
+                             vscheme, AH, tag, nullptr, nullptr));
+    } else {
+      TRY(forward_capture(c, g, dt, work, sg[0], uu, vv, ww, hu, hv, ho, hscheme, vscheme, AH,
+                          kshift, D, S));
+      TRY(copy_d2d(c, work, sg[0], N3));
+    }
   }
   if (f_final) TRY(copy_d2d(c, f_final, work, N3));
   // reverse sweep, segment by segment from the last checkpoint

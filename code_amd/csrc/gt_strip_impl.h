@@ -84,42 +84,55 @@ __device__ __forceinline__ FaceK face_k(double dti, double dlen, double AH) {
 template <int SCH>
 __device__ __forceinline__ void face_setup(FaceC<true>& fc, bool active, bool lim_l, bool lim_r,
                                            double vel, double Dv, const FaceK& k) {
-  // one division chain for both directions: the reference's vel/Dv*dti/dlen
-  // going right, -vel/Dv*dti/dlen going left, with the operand selected
-  // first (a wavefront holding both directions runs the chain once)
+  // The exact path is branch-free like the fast one: every operand is formed
+  // and the result selected, so the division chains of neighbouring faces and
+  // tracers share one basic block and interleave; a lane whose value is not
+  // selected may compute inf or NaN, which is never used.  One division
+  // chain serves both directions: the reference's vel/Dv*dti/dlen going
+  // right, -vel/Dv*dti/dlen going left, with the operand selected first.
   const double dti = k.dti, dlen = k.dlen;
   fc.vel = vel;
   fc.Dv = Dv;
-  fc.cfl = 0.0;
   const bool left = vel > 0.0;
   const bool lim = active && SCH != UPSTREAM && (left ? lim_l : lim_r);
   fc.code = !active ? F_NONE : (left ? (lim ? F_LEFT_LIM : F_LEFT) : (lim ? F_RIGHT_LIM : F_RIGHT));
-  if (lim) fc.cfl = (left ? vel : -vel) / Dv * dti / dlen;
-  fc.E = 0.0;
-  if (SCH == P2_PDM && lim) fc.E = 2.0 / (1.0 - fc.cfl);
+  const double c = (left ? vel : -vel) / Dv * dti / dlen;
+  fc.cfl = lim ? c : 0.0;
+  fc.E = SCH == P2_PDM ? 2.0 / (1.0 - fc.cfl) : 0.0;
 }
 
-// adv_interfacial_reconstruction (src/2d/advection.F90:806-845) with P2-PDM's
-// 2/(1-cfl) taken from the face (E) instead of formed per tracer; every other
-// operation is reconstruct()'s, in its order, so the result is identical
+// adv_interfacial_reconstruction (src/2d/advection.F90:806-845), branch-free:
+// every operation of reconstruct() in its order, the result selected by
+// deltaf*deltafu > 0, and P2-PDM's 2/(1-cfl) taken from the face (E) instead
+// of formed per tracer; the selected value is identical
 template <int SCH>
-__device__ __forceinline__ double reconstruct_face(double cfl, double E, double fuu, double fu,
-                                                   double fd) {
-  if constexpr (SCH != P2_PDM) {
-    (void)E;
-    return reconstruct<double>(SCH, cfl, fuu, fu, fd);
-  } else {
-    const double one6th = 1.0 / 6.0;
-    const double deltaf = fd - fu;
-    const double deltafu = fu - fuu;
-    if (deltaf * deltafu > 0.0) {
-      const double ratio = deltafu / deltaf;
-      const double x = one6th * (1.0 - 2.0 * cfl);
-      double limiter = (0.5 + x) + (0.5 - x) * ratio;
-      limiter = tmin(tmin(2.0 * ratio / (cfl + 1.e-10), limiter), E);
-      return fu + 0.5 * limiter * (1.0 - cfl) * deltaf;
-    }
+__device__ __forceinline__ double reconstruct_bf(double cfl, double E, double fuu, double fu,
+                                                 double fd) {
+  const double deltaf = fd - fu;
+  const double deltafu = fu - fuu;
+  if constexpr (SCH == UPSTREAM) {
+    (void)E; (void)cfl;
     return fu;
+  } else if constexpr (SCH == P2) {
+    (void)E;
+    return reconstruct_p2<double>(cfl, fu, deltafu, deltaf);
+  } else {
+    const double ratio = deltafu / deltaf;
+    double limiter;
+    if constexpr (SCH == P2_PDM) {
+      const double one6th = 1.0 / 6.0;
+      const double x = one6th * (1.0 - 2.0 * cfl);
+      limiter = (0.5 + x) + (0.5 - x) * ratio;
+      limiter = tmin(tmin(2.0 * ratio / (cfl + 1.e-10), limiter), E);
+    } else if constexpr (SCH == SUPERBEE) {
+      (void)E;
+      limiter = tmax(tmin(2.0 * ratio, 1.0), tmin(ratio, 2.0));
+    } else {  // MUSCL
+      (void)E;
+      limiter = tmin(tmin(2.0 * ratio, 0.5 * (1.0 + ratio)), 2.0);
+    }
+    const double r = fu + 0.5 * limiter * (1.0 - cfl) * deltaf;
+    return deltaf * deltafu > 0.0 ? r : fu;
   }
 }
 
@@ -183,17 +196,17 @@ __device__ __forceinline__ float rec_fast(float fuu, float fu, float fd, float g
 template <typename TF, int SCH, bool DIFF>
 __device__ __forceinline__ TF face_flux(const FaceC<true>& fc, TF fm1, TF f0, TF fp1, TF fp2,
                                         double AH, double dlen) {
-  if (fc.code == F_NONE) return TF(0);
   // upwind operands selected first, then one reconstruction for both
-  // directions (left: fm1, f0, fp1; right: fp2, fp1, f0)
+  // directions (left: fm1, f0, fp1; right: fp2, fp1, f0), selected by the
+  // face code (branch-free, see face_setup)
   const bool left = fc.code <= F_LEFT_LIM;
   const TF fuu = left ? fm1 : fp2, fd = left ? fp1 : f0;
-  TF fu = left ? f0 : fp1;
-  if (fc.code == F_LEFT_LIM || fc.code == F_RIGHT_LIM)
-    fu = reconstruct_face<SCH>(fc.cfl, fc.E, fuu, fu, fd);
+  const TF fu0 = left ? f0 : fp1;
+  const TF rr = reconstruct_bf<SCH>(fc.cfl, fc.E, fuu, fu0, fd);
+  const TF fu = (fc.code == F_LEFT_LIM || fc.code == F_RIGHT_LIM) ? rr : fu0;
   TF fl = fc.vel * fu;
   if constexpr (DIFF) fl = fl - AH * fc.Dv * (fp1 - f0) / dlen;
-  return fl;
+  return fc.code == F_NONE ? TF(0) : fl;
 }
 
 template <typename TF, int SCH, bool DIFF>
@@ -210,13 +223,16 @@ __device__ __forceinline__ TF face_flux(const FaceC<false>& fc, TF fm1, TF f0, T
 
 // the tracer update of one cell of a horizontal sub-step: the reference's
 // (Dio*fo - dt*advn)/Dn, advn = splitfac*(Fc*dlen - Fm*dlen)*ard1
-// (adv_split_u.F90:267-286), in the reference's order (EXACT).
-__device__ __forceinline__ double cell_update_exact(double Dio, double Dn, double fo, double Fc,
-                                                    double Fm, double dlen, double ard1,
-                                                    double dt, double* advn_out) {
+// (adv_split_u.F90:267-286), in the reference's order (EXACT); computed on
+// every lane and selected by upd (advn_out is 0 where the cell is not
+// updated)
+__device__ __forceinline__ double cell_update_exact(bool upd, double Dio, double Dn, double fo,
+                                                    double Fc, double Fm, double dlen,
+                                                    double ard1, double dt, double* advn_out) {
   const double advn = 0.5 * (Fc * dlen - Fm * dlen) * ard1;
-  *advn_out = advn;
-  return (Dio * fo - dt * advn) / Dn;
+  *advn_out = upd ? advn : 0.0;
+  const double fn = (Dio * fo - dt * advn) / Dn;
+  return upd ? fn : fo;
 }
 
 // fast: the same as fo + [fo*(Dio-Dn)/Dn - (Fc-Fm)*(dt*splitfac*dlen*ard1/Dn)]:
@@ -282,7 +298,8 @@ __device__ __forceinline__ TF wflux(const WFace<EXACT>& wf, int k, int kshift, i
     if (lim) { fuu = fk2; fd = fk; }
   }
   if constexpr (EXACT) {
-    if (lim) fu = reconstruct_face<SCH>(wf.cfl * 1.0, wf.E, fuu, fu, fd);
+    const TF rr = reconstruct_bf<SCH>(wf.cfl * 1.0, wf.E, fuu, fu, fd);
+    fu = lim ? rr : fu;
   } else {
     fu = rec_fast<SCH>(fuu, fu, fd, lim ? wf.g : 0.0f, wf.A, wf.al, wf.E);
   }
@@ -507,7 +524,7 @@ __global__ __launch_bounds__(256) void k_strip_uv(FusedArgs<TF> A) {
       double advu = 0.0;
       TF F1 = fc;
       if constexpr (EXACT) {
-        if (updu) F1 = cell_update_exact(D0, D1, fc, Fu, Fum, dy, ard1, dt, &advu);
+        F1 = cell_update_exact(updu, D0, D1, fc, Fu, Fum, dy, ard1, dt, &advu);
       } else {
         const TF u = cell_update_fast(k1, fc, Fu, Fum);
         F1 = updu ? u : fc;
@@ -522,7 +539,7 @@ __global__ __launch_bounds__(256) void k_strip_uv(FusedArgs<TF> A) {
           double advv = 0.0;
           TF F2 = f1w[g][w1];
           if constexpr (EXACT) {
-            if (updv) F2 = cell_update_exact(d1w[w1], D2, F2, Fv, fvp[g], dx, ard1, dt, &advv);
+            F2 = cell_update_exact(updv, d1w[w1], D2, F2, Fv, fvp[g], dx, ard1, dt, &advv);
           } else {
             const TF u = cell_update_fast(k2, F2, Fv, fvp[g]);
             F2 = updv ? u : F2;
@@ -729,11 +746,11 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
       const TF fKp1 = buf.f[g][3], fKp2 = buf.f[g][4];
       TF wprev = TF(0), wk = TF(0);
       if constexpr (EXACT) {
-        if (wcol && wp_on)
-          wprev = wflux<TF, SCH, EXACT>(wfp, K - 1, kshift, kmax, K > 1 ? fKm2 : TF(0), fKm1, fK,
-                                        fKp1);
-        if (wcol && wk_on)
-          wk = wflux<TF, SCH, EXACT>(wfk, K, kshift, kmax, fKm1, fK, fKp1, fKp2);
+        const TF p =
+            wflux<TF, SCH, EXACT>(wfp, K - 1, kshift, kmax, K > 1 ? fKm2 : TF(0), fKm1, fK, fKp1);
+        const TF q = wflux<TF, SCH, EXACT>(wfk, K, kshift, kmax, fKm1, fK, fKp1, fKp2);
+        wprev = (wcol && wp_on) ? p : TF(0);
+        wk = (wcol && wk_on) ? q : TF(0);
       } else {
         const TF p = wflux_fast<SCH>(wfp, limp_up, limp_dn, K > 1 ? fKm2 : TF(0), fKm1, fK, fKp1);
         const TF q = wflux_fast<SCH>(wfk, limk_up, limk_dn, fKm1, fK, fKp1, fKp2);
@@ -743,10 +760,10 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
       TF F3 = fK;
       double advw = 0.0;
       if constexpr (EXACT) {
-        if (wupd) {
-          advw = 1.0 * (wk - wprev);
-          F3 = (D2K * fK - dt * advw) / D3;
-        }
+        const double aw = 1.0 * (wk - wprev);
+        const double f3 = (D2K * fK - dt * aw) / D3;
+        advw = wupd ? aw : 0.0;
+        F3 = wupd ? f3 : fK;
       } else {
         const TF u = cell_update_fast(k3, fK, wk, wprev);
         F3 = wupd ? u : fK;
@@ -761,7 +778,7 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
           double advv = 0.0, advu = 0.0;
           TF F4 = f3w[g][w1];
           if constexpr (EXACT) {
-            if (updv) F4 = cell_update_exact(d3w[w1], D4, F4, Fv, fvp[g], dx, ard1, dt, &advv);
+            F4 = cell_update_exact(updv, d3w[w1], D4, F4, Fv, fvp[g], dx, ard1, dt, &advv);
           } else {
             const TF u = cell_update_fast(k4, F4, Fv, fvp[g]);
             F4 = updv ? u : F4;
@@ -773,7 +790,7 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
           const TF Fum = from_lower_lane(Fu);
           TF F5 = F4;
           if constexpr (EXACT) {
-            if (updu) F5 = cell_update_exact(D4, D5, F4, Fu, Fum, dy, ard1, dt, &advu);
+            F5 = cell_update_exact(updu, D4, D5, F4, Fu, Fum, dy, ard1, dt, &advu);
           } else {
             const TF u = cell_update_fast(k5, F4, Fu, Fum);
             F5 = updu ? u : F4;

@@ -297,6 +297,17 @@ def blob_case(seed=1, imax=100, jmax=100, kmax=20) -> Transport:
     return tr
 
 
+def adjoint_case(seed=4, imax=256, jmax=256, kmax=30) -> Transport:
+    """Reverse-mode workload: sigma layers, the Gaussian-blob flow and tracer
+    of blob_case at 256x256x30, plus T and S for the internal-pressure term of
+    the cost."""
+    tr = blob_case(seed=seed, imax=imax, jmax=jmax, kmax=kmax)
+    T, S = temperature_salinity(tr, seed)
+    tr.tracers["T"] = T
+    tr.tracers["S"] = S
+    return tr
+
+
 def seeded_modes(seed, n=4, mmax=3):
     u = uniform((n, 3), seed * 104729 + 5)
     return [(1 + int(u[q, 0] * mmax), 1 + int(u[q, 1] * mmax),

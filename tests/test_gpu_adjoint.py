@@ -114,6 +114,56 @@ def test_multistep_checkpointed_gradient_vs_fd(device):
     assert ok >= 40
 
 
+def test_adjoint_case_full_size_vs_fd(device):
+    """The reverse-mode workload at full size (256x256x30 sigma, Superbee
+    HALFSPLIT, 48 steps, checkpoint every 8): dJ/dc0 for J = sum over a 32x32
+    box (all k) of c^2 at the final step, against central FD (eps 1e-6) of
+    J's linearisation at 24 seeded wet cells of the box's domain of
+    dependence, to 1e-6 of the largest gradient entry (kinks as in
+    test_multistep_checkpointed_gradient_vs_fd)."""
+    tr = inputs.adjoint_case()
+    d = tr.dom
+    ctx = gt.Context.from_domain(d)
+    fields = tuple(to_dev(a, device) for a in (tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho, tr.hn))
+    nsteps, ck = 48, 8
+    c0 = to_dev(tr.tracers["c"], device)
+    j0, i0 = d.NY // 2 - 16, d.NX // 2 + 24
+    box = (j0, j0 + 32, i0, i0 + 32)
+    c0r = c0.clone().requires_grad_(True)
+    out = AdvectSteps.apply(c0r, ctx, fields, nsteps, ck, gt.SUPERBEE, gt.SUPERBEE, tr.dt)
+    (g,) = torch.autograd.grad(box_cost(out, box), c0r)
+    lam = torch.zeros_like(c0)
+    lam[1:, j0:j0 + 32, i0:i0 + 32] = 2.0 * out.detach()[1:, j0:j0 + 32, i0:i0 + 32]
+    # J sums ~3e4 cells, so the states are differenced before the sum with
+    # lam (differencing two values of J would lose ~1e-16*|J|/eps)
+    state = lambda c: AdvectSteps.apply(  # noqa: E731
+        c, ctx, fields, nsteps, ck, gt.SUPERBEE, gt.SUPERBEE, tr.dt)
+    gh = g.cpu().numpy()
+    rng = np.random.default_rng(23)
+    s0 = state(c0)
+    scale = float(np.abs(gh).max())
+    assert scale > 0
+    eps = 1e-6
+    ok = kinks = 0
+    for _ in range(24):
+        j = int(rng.integers(j0 - 1, j0 + 33))
+        i = int(rng.integers(i0 - 1, i0 + 33))
+        k = int(rng.integers(1, d.kmax + 1))
+        e = torch.zeros_like(c0)
+        e[k, j, i] = eps
+        sp, sm = state(c0 + e), state(c0 - e)
+        fwd = float(torch.sum(lam * (sp - s0))) / eps
+        bwd = float(torch.sum(lam * (s0 - sm))) / eps
+        if abs(fwd - bwd) > 1e-6 * scale:
+            kinks += 1
+            continue
+        fd = float(torch.sum(lam * (sp - sm))) / (2 * eps)
+        assert abs(gh[k, j, i] - fd) <= 1e-6 * scale, (k, j, i, gh[k, j, i], fd)
+        ok += 1
+    print(f"full-size adjoint vs FD: {ok} cells agree, {kinks} within eps of a kink")
+    assert ok >= 16
+
+
 def test_dot_product_smooth_superbee(device):
     """dot-product identity on the smooth Gaussian-blob case (sigma layers,
     Superbee both ways), 3 steps with a checkpoint every 2.
