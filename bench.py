@@ -79,6 +79,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-batched", action="store_true")
     p.add_argument("--no-adjoint", action="store_true")
+    p.add_argument("--shard", action="store_true",
+                   help="also time one GPU's share (1000x375x50, 18 FP32 tracers) of the "
+                        "2000x1500x50 case")
     p.add_argument("--batched-steps", type=int, default=20)
     p.add_argument("--rows64", type=int, default=-1,
                    help="FP64 headline: rows per strip 8, 16 or 32 (-1: autotuned)")
@@ -241,6 +244,8 @@ def main():
     del TS2, idpdx6, idpdy6
     if not a.no_batched:
         out["batched_fp32"] = batched(torch, np, gt, inputs, dev, a, ncell * ntr / adv_s)
+    if a.shard:
+        out["shard"] = shard_section(torch, np, gt, inputs, dev, a.batched_steps)
     if not a.no_adjoint:
         out["adjoint"] = adjoint_section(torch, np, gt, inputs, dev)
     if not a.no_cpu:
@@ -428,6 +433,50 @@ def adjoint_section(torch, np, gt, inputs, dev, nsteps=48, ckpt=8):
                     "forward steps to the checkpoints, 48 recomputed with the sub-step "
                     "states, 48 reverse steps)",
             "grad_norm_c": float(g_c.norm()), "grad_norm_T": float(g_T.norm())}
+
+
+def shard_section(torch, np, gt, inputs, dev, steps):
+    """One GPU's share of the 2000x1500x50 domain split 2x4: a 1000x375x50 GVC
+    subdomain (ghost cells filled locally) with T, S and 16 passive tracers in
+    FP32, the batched step autotuned and timed."""
+    tr = inputs.config_advection(1000, 375, 50, 5, hmin=10.0, hmax=50.0, coords="gvc", umax=1.0,
+                                 modes=inputs.seeded_modes(5))
+    d = tr.dom
+    T, S = inputs.temperature_salinity(tr, 5)
+    st = torch.cuda.Stream(device=dev)
+    ctx = gt.Context.from_domain(d, stream=st)
+    fin = torch.empty((18,) + d.shape3, dtype=torch.float32, device=dev)
+    fin[0] = torch.from_numpy(T).to(dev)
+    fin[1] = torch.from_numpy(S).to(dev)
+    del T, S
+    for q, b in enumerate(inputs.passive_blobs(d, 16, 5)):
+        fin[2 + q] = torch.from_numpy(b).to(dev)
+    fout = torch.empty_like(fin)
+    D = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)  # noqa: E731
+    dd = dict(ctx=ctx, uu=D(tr.uu), vv=D(tr.vv), ww=D(tr.ww), hu=D(tr.hun), hv=D(tr.hvn),
+              ho=D(tr.ho), hires=torch.zeros(d.shape3, dtype=torch.float64, device=dev))
+    bt, bg, bms, _ = gt.autotune_batch([dict(dd, f_in=fin, f_out=fout)], 18, tr.dt, gt.P2_PDM,
+                                       gt.P2_PDM, reps=3, stream=st)
+    bufs = [fin, fout]
+    timer = EventTimer(torch, st)
+    with torch.cuda.stream(st):
+        for _ in range(2):
+            gt.adv3d_batch([dict(dd, f_in=bufs[0], f_out=bufs[1])], 18, tr.dt, gt.P2_PDM,
+                           gt.P2_PDM, stream=st)
+            bufs.reverse()
+        for _ in range(steps):
+            with timer:
+                gt.adv3d_batch([dict(dd, f_in=bufs[0], f_out=bufs[1])], 18, tr.dt, gt.P2_PDM,
+                               gt.P2_PDM, stream=st)
+            bufs.reverse()
+    t = timer.mean()
+    ncell = d.imax * d.jmax * d.kmax
+    b = ncell * (18 * 2 * 4 + 7 * 8)
+    return {"workload": "one GPU's share of 2000x1500x50 split 2x4: 1000x375x50 GVC, T, S + 16 "
+                        "passive tracers FP32, P2-PDM HALFSPLIT",
+            "rows": gt.ROWS[bt], "tracers_per_wave": bg, "ms_per_step": t * 1e3,
+            "cell_updates_per_s": ncell * 18 / t, "algorithmic_GBps": b / t / 1e9,
+            "frac_of_8TBps": b / t / HBM_PEAK}
 
 
 def cpu_baseline(tr, seconds):
