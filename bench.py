@@ -79,6 +79,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-batched", action="store_true")
     p.add_argument("--no-adjoint", action="store_true")
+    p.add_argument("--serial", action="store_true",
+                   help="run EOS + pressure and the advection on one stream")
     p.add_argument("--shard", action="store_true",
                    help="also time one GPU's share (1000x375x50, 18 FP32 tracers) of the "
                         "2000x1500x50 case")
@@ -147,6 +149,11 @@ def main():
     d = tr.dom
     st = torch.cuda.Stream(device=dev)
     ctx = gt.Context.from_domain(d, stream=st)
+    # the EOS + internal-pressure chain and the T/S advection read the same T
+    # and S and write disjoint fields, so they run on two streams (default
+    # priority); --serial puts both on one
+    st_ip = st if a.serial else torch.cuda.Stream(device=dev)
+    ctx_ip = ctx if a.serial else gt.Context.from_domain(d, stream=st_ip)
     if a.rows64 > 0:
         ctx.set_tuning(fused=1, tile=gt.tile_for_rows(a.rows64))
     D = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)  # noqa: E731
@@ -164,7 +171,7 @@ def main():
     dt = tr.dt
     bufs = [TS, TS2]
     t_adv = EventTimer(torch, st)
-    t_ip = EventTimer(torch, st)
+    t_ip = EventTimer(torch, st_ip)
 
     tuned = None
     rows64, g64 = a.rows64, 2
@@ -176,16 +183,30 @@ def main():
                  "table_ms": {f"{gt.ROWS[t]}x{g}": v for (t, g), v in table.items()}}
         rows64, g64 = gt.ROWS[bt], bg
 
+    ev = {}
+
     def step(timed):
         cur, nxt = bufs
-        with (t_ip if timed else _Null()):
-            ctx.eos_linear(cur[0], cur[1], None, buoy, *consts)
-            ctx.ip_blumberg_mellor(H, hn, hun, hvn, buoy, zz, idpdx, idpdy)
-            ctx.ip_shchepetkin_mcwilliams(H, hn, hun, hvn, buoy, sseo, zz6, idpdx6, idpdy6)
+        if not a.serial:
+            # IP reads cur, which the previous advection wrote; this
+            # advection overwrites nxt, which the previous EOS read
+            if "adv" in ev:
+                st_ip.wait_event(ev["adv"])
+            if "eos" in ev:
+                st.wait_event(ev["eos"])
+        with torch.cuda.stream(st_ip):
+            with (t_ip if timed else _Null()):
+                ctx_ip.eos_linear(cur[0], cur[1], None, buoy, *consts)
+                ev["eos"] = torch.cuda.Event()
+                ev["eos"].record(st_ip)
+                ctx_ip.ip_blumberg_mellor(H, hn, hun, hvn, buoy, zz, idpdx, idpdy)
+                ctx_ip.ip_shchepetkin_mcwilliams(H, hn, hun, hvn, buoy, sseo, zz6, idpdx6, idpdy6)
         with (t_adv if timed else _Null()):
             gt.adv3d_batch([dict(ctx=ctx, f_in=cur, f_out=nxt, uu=uu, vv=vv, ww=ww, hu=hun,
                                  hv=hvn, ho=ho, hires=hires)], 2, dt, gt.P2_PDM, gt.P2_PDM,
                            stream=st)
+        ev["adv"] = torch.cuda.Event()
+        ev["adv"].record(st)
         bufs.reverse()
 
     torch.cuda.synchronize()
@@ -193,11 +214,13 @@ def main():
         for _ in range(a.warmup):
             step(False)
         st.synchronize()
+        st_ip.synchronize()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(a.steps):
             step(True)
         st.synchronize()
+        st_ip.synchronize()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
     ms = (t1 - t0) / a.steps * 1e3
@@ -237,6 +260,7 @@ def main():
                    "kmax": d.kmax, "tracers": ntr, "parallelism": "single GPU"},
         "fp64_advection_cell_updates_per_s": ncell * ntr / adv_s,
         "components_ms": {"advection_T_S": adv_s * 1e3, "eos_and_pressure": ip_s * 1e3},
+        "streams": 1 if a.serial else 2,
         "eos_and_pressure_GBps": b_ip / ip_s / 1e9,
         "tuning": tuned,
         "roofline": roof,
