@@ -16,7 +16,7 @@ w + v/2 + u/2).  Their roofline is quoted for the pair, with the algorithmic
 bytes of the whole step Ncell*(2*Ntr*s + 7*8): each tracer read and written
 once, uu vv ww hu hv ho read and hires written once in FP64.  Strip height
 and tracers per wavefront are autotuned (untimed) unless --rows64 (headline)
-and --rows32/--group (batched) fix them.
+and --rows32/--group (batched) fix them (rows "R" for both kernels or "R1/R2").
 
 Prints one JSON line.  Run: python bench.py [--steps K] [--warmup W]
 """
@@ -34,6 +34,18 @@ if ROOT not in sys.path:
 
 HBM_PEAK = 8.0e12
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "strip_kernels_hbm_traffic.txt")
+
+
+def parse_rows(v):
+    """'R' or 'R1/R2' -> (rows of the first kernel, rows of the second); None if unset"""
+    if v in (None, "", "-1"):
+        return None
+    a, _, b = v.partition("/")
+    return int(a), int(b or a)
+
+
+def rows_str(pair):
+    return f"{pair[0]}/{pair[1]}"
 
 
 def measured_traffic(section, rows, group=None):
@@ -57,7 +69,8 @@ def measured_traffic(section, rows, group=None):
     cf = float(cal["copy_read_bytes"]) / (float(cal["fetch_kib"]) * 1024)
     cw = float(cal["copy_write_bytes"]) / (float(cal["write_kib"]) * 1024)
     for r in recs[section]:
-        if int(r["rows"]) != rows or (group is not None and int(r["group"]) != group):
+        if parse_rows(r["rows"]) != tuple(rows) or (group is not None and
+                                                    int(r["group"]) != group):
             continue
         fetch = float(r["uv_fetch_kib"]) + float(r["wvu_fetch_kib"])
         write = float(r["uv_write_kib"]) + float(r["wvu_write_kib"])
@@ -85,10 +98,11 @@ def parse():
                    help="also time one GPU's share (1000x375x50, 18 FP32 tracers) of the "
                         "2000x1500x50 case")
     p.add_argument("--batched-steps", type=int, default=20)
-    p.add_argument("--rows64", type=int, default=-1,
-                   help="FP64 headline: rows per strip 8, 16 or 32 (-1: autotuned)")
-    p.add_argument("--rows32", type=int, default=-1,
-                   help="FP32 batched: rows per strip 8, 16 or 32 (-1: autotuned)")
+    p.add_argument("--rows64", default=None,
+                   help="FP64 headline: rows per strip, 8, 16 or 32, as R or R1/R2 (default: "
+                        "autotuned)")
+    p.add_argument("--rows32", default=None,
+                   help="FP32 batched: rows per strip as for --rows64 (default: autotuned)")
     p.add_argument("--copy-mb", type=int, default=0,
                    help="also run a plain device copy of this many MiB (counter calibration)")
     p.add_argument("--group", type=int, default=-1,
@@ -154,8 +168,8 @@ def main():
     # priority); --serial puts both on one
     st_ip = st if a.serial else torch.cuda.Stream(device=dev)
     ctx_ip = ctx if a.serial else gt.Context.from_domain(d, stream=st_ip)
-    if a.rows64 > 0:
-        ctx.set_tuning(fused=1, tile=gt.tile_for_rows(a.rows64))
+    if parse_rows(a.rows64):
+        ctx.set_tuning(fused=1, tile=gt.tile_for_rows(*parse_rows(a.rows64)))
     D = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)  # noqa: E731
     uu, vv, ww, hun, hvn, ho, hn = (D(x) for x in (tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho,
                                                    tr.hn))
@@ -174,14 +188,15 @@ def main():
     t_ip = EventTimer(torch, st_ip)
 
     tuned = None
-    rows64, g64 = a.rows64, 2
-    if a.rows64 < 0:
+    rows64, g64 = parse_rows(a.rows64), 2
+    if rows64 is None:
         bt, bg, bms, table = gt.autotune_batch(
             [dict(ctx=ctx, f_in=TS, f_out=TS2, uu=uu, vv=vv, ww=ww, hu=hun, hv=hvn, ho=ho,
                   hires=hires)], 2, dt, gt.P2_PDM, gt.P2_PDM, reps=5, stream=st)
-        tuned = {"rows": gt.ROWS[bt], "tracers_per_wave": bg, "ms": bms,
-                 "table_ms": {f"{gt.ROWS[t]}x{g}": v for (t, g), v in table.items()}}
-        rows64, g64 = gt.ROWS[bt], bg
+        tuned = {"rows": rows_str(gt.strip_rows(bt)), "tracers_per_wave": bg, "ms": bms,
+                 "table_ms": {f"{rows_str(gt.strip_rows(t))}x{g}": v
+                              for (t, g), v in table.items()}}
+        rows64, g64 = gt.strip_rows(bt), bg
 
     ev = {}
 
@@ -297,8 +312,9 @@ def batched(torch, np, gt, inputs, dev, a, fp64_rate):
         d = tr.dom
         ncell += d.imax * d.jmax * d.kmax
         ctx = gt.Context.from_domain(d, stream=st)
-        if a.rows32 > 0:
-            ctx.set_tuning(1, gt.tile_for_rows(a.rows32), a.group if a.group > 0 else 3)
+        if parse_rows(a.rows32):
+            ctx.set_tuning(1, gt.tile_for_rows(*parse_rows(a.rows32)),
+                           a.group if a.group > 0 else 3)
         ctxs.append(ctx)
         stack = np.stack([tr.tracers[n] for n in names])
         f64 = torch.from_numpy(stack).to(dev)
@@ -327,15 +343,16 @@ def batched(torch, np, gt, inputs, dev, a, fp64_rate):
             bb.reverse()
 
     tuned = None
-    rows32, group = a.rows32, a.group
-    if a.rows32 < 0 or a.group < 0:
+    rows32, group = parse_rows(a.rows32), a.group
+    if rows32 is None or a.group < 0:
         bt, bg, bms, table = gt.autotune_batch(
             [dict(ctx=dd["ctx"], f_in=fin32[s][0], f_out=fin32[s][1], uu=dd["uu"], vv=dd["vv"],
                   ww=dd["ww"], hu=dd["hu"], hv=dd["hv"], ho=dd["ho"], hires=dd["hires"])
              for s, dd in enumerate(doms)], nt, dt, hs, vs, reps=3, stream=st)
-        tuned = {"rows": gt.ROWS[bt], "tracers_per_wave": bg, "ms": bms,
-                 "table_ms": {f"{gt.ROWS[t]}x{g}": v for (t, g), v in table.items()}}
-        rows32, group = gt.ROWS[bt], bg
+        tuned = {"rows": rows_str(gt.strip_rows(bt)), "tracers_per_wave": bg, "ms": bms,
+                 "table_ms": {f"{rows_str(gt.strip_rows(t))}x{g}": v
+                              for (t, g), v in table.items()}}
+        rows32, group = gt.strip_rows(bt), bg
     timer = EventTimer(torch, st)
     nsteps = a.batched_steps
     with torch.cuda.stream(st):
@@ -498,7 +515,7 @@ def shard_section(torch, np, gt, inputs, dev, steps):
     b = ncell * (18 * 2 * 4 + 7 * 8)
     return {"workload": "one GPU's share of 2000x1500x50 split 2x4: 1000x375x50 GVC, T, S + 16 "
                         "passive tracers FP32, P2-PDM HALFSPLIT",
-            "rows": gt.ROWS[bt], "tracers_per_wave": bg, "ms_per_step": t * 1e3,
+            "rows": rows_str(gt.strip_rows(bt)), "tracers_per_wave": bg, "ms_per_step": t * 1e3,
             "cell_updates_per_s": ncell * 18 / t, "algorithmic_GBps": b / t / 1e9,
             "frac_of_8TBps": b / t / HBM_PEAK}
 

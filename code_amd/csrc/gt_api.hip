@@ -62,7 +62,7 @@ struct gt_ctx {
   int adv_ver_iterations = 1;
   int do_mirror_2d = 1;
   int fused = 1;        // use the HALFSPLIT strip kernels where they apply
-  int tile = 0;         // rows per strip (0: 16, 1: 32, 2: 8)
+  int tile = 0;         // rows per strip of the two strip kernels (gt::strip_rows)
   int group = 3;        // tracers per wavefront of the FP32 batched kernels
   void* ws = nullptr;   // work fields of the strip kernels (grown on demand)
   size_t ws_bytes = 0;
@@ -632,7 +632,7 @@ extern "C" int gt_do_advection_3d(gt_ctx* c, double dt, double* f, const double*
         fill_domain<double>(c, g, &A.doms[0], f, f, 0, 1, uu, vv, ww, hu, hv, ho, hires, advres);
         A.ndom = 1;
         A.ntracers = 1;
-        A.rows = gt::strip_rows(c->tile);
+        gt::strip_rows(c->tile, A.rows);
         A.dt = dt; A.AH = AH;
         A.hscheme = hscheme; A.vscheme = vscheme; A.kshift = tag == GT_W_TAG ? 1 : 0;
         gt::strip_plan(A, 1);
@@ -780,8 +780,8 @@ extern "C" int gt_exchange_tiles(int px, int py, const int* ni, const int* nj, i
 // ---- batched multi-tracer step (extension) ----------------------------------
 extern "C" int gt_set_tuning(gt_ctx* c, int fused, int tile, int group) {
   if (!c) return set_err(GT_EARG, "gt_set_tuning: NULL context");
-  if (tile < 0 || tile > 2)
-    return set_err(GT_EARG, "gt_set_tuning: tile (rows per strip 16, 32, 8) must be 0, 1 or 2");
+  if (tile < 0 || tile > 11)
+    return set_err(GT_EARG, "gt_set_tuning: tile must be 0..11 (rows per strip, see the header)");
   if (group != 1 && group != 2 && group != 3 && group != 6 && group != 9)
     return set_err(GT_EARG, "gt_set_tuning: group must be 1, 2, 3, 6 or 9");
   c->fused = fused != 0;
@@ -833,7 +833,7 @@ static int batch_impl(int ndom, const gt_batch_domain* doms, int ntracers, doubl
     }
     A.ndom = nd;
     A.ntracers = ntracers;
-    A.rows = gt::strip_rows(c0->tile);
+    gt::strip_rows(c0->tile, A.rows);
     A.dt = dt; A.AH = AH;
     A.hscheme = hscheme; A.vscheme = vscheme; A.kshift = tag == GT_W_TAG ? 1 : 0;
     gt::strip_plan(A, group);
@@ -870,7 +870,8 @@ extern "C" int gt_autotune_batch(int ndom, const gt_batch_domain* doms, int ntra
   if (ndom < 1 || !doms || !doms[0].ctx || reps < 1)
     return set_err(GT_EARG, "gt_autotune_batch: bad argument");
   hipStream_t st = stream ? (hipStream_t)stream : doms[0].ctx->st;
-  const int tiles[3] = {0, 1, 2};
+  // strip heights of the two kernels: 9 pairs {16, 32, 8} x {16, 32, 8}
+  const int tiles[9] = {3, 4, 5, 6, 7, 8, 9, 10, 11};
   const int groups[4] = {1, 3, 6, 9};
   const int groups64[2] = {1, 2};
   hipEvent_t e0, e1;
@@ -879,7 +880,7 @@ extern "C" int gt_autotune_batch(int ndom, const gt_batch_domain* doms, int ntra
   float best = 1e30f;
   int bt = 0, bg = 3;
   int n = 0;
-  for (int ti = 0; ti < 3; ti++)
+  for (int ti = 0; ti < 9; ti++)
     for (int gi = 0; gi < (f32 ? 4 : 2); gi++) {
       const int grp = f32 ? groups[gi] : groups64[gi];
       for (int i = 0; i < ndom; i++) {

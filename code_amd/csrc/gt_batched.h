@@ -22,8 +22,9 @@ enum : int { MB_XOUT = 32 };
 template <typename TF>
 struct FusedDomain {
   Dims d;
-  int nsx, nsy;       // strips along i and j
-  long item_begin;    // first launch-wide work item of this domain
+  int nsx;            // strips along i
+  int nsy[2];         // strips along j, of the first and the second kernel
+  long item_begin[2]; // first launch-wide work item of this domain, per kernel
   const TF* fin;      // tracer t of this domain starts at fin + t*tstride
   TF* fout;           // output, same layout (may equal fin)
   long tstride;
@@ -44,17 +45,18 @@ template <typename TF>
 struct FusedArgs {
   FusedDomain<TF> doms[kMaxFusedDomains];
   int ndom;
-  long nitems;     // work items over all domains
+  long nitems[2];  // work items over all domains, per kernel
   int ngroups;     // tracer groups
   int ntracers;
-  int rows;        // rows per strip
+  int rows[2];     // rows per strip of the first (u/2 v/2) and second (w v/2 u/2) kernel
   double dt, AH;
   int hscheme, vscheme, kshift;
 };
 
-// rows per strip for a tuning index: 0 -> 16, 1 -> 32, 2 -> 8 (a multiple of
-// 4: the row march is unrolled four times)
-int strip_rows(int tile);
+// rows per strip for a tuning index: 0 -> 16, 1 -> 32, 2 -> 8 for both
+// kernels; 3..11 -> the first kernel {16, 32, 8}[(t-3)%3], the second
+// {16, 32, 8}[(t-3)/3] (multiples of 4: the row march is unrolled four times)
+void strip_rows(int tile, int rows[2]);
 // the strip kernels are instantiated for hscheme == vscheme in UPSTREAM..P2_PDM
 bool fused_supported(int hscheme, int vscheme);
 // group: tracers per wavefront.  FP64: 1 or 2 (adv needs 1); FP32: 1, 3, 6 or 9.

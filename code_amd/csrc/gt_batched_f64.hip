@@ -4,11 +4,13 @@
 
 namespace gt {
 
-int strip_rows(int tile) {
-  switch (tile) {
-    case 1: return 32;
-    case 2: return 8;
-    default: return 16;
+void strip_rows(int tile, int rows[2]) {
+  static const int r[3] = {16, 32, 8};
+  if (tile >= 3 && tile < 12) {
+    rows[0] = r[(tile - 3) % 3];
+    rows[1] = r[(tile - 3) / 3];
+  } else {
+    rows[0] = rows[1] = r[(tile >= 0 && tile < 3) ? tile : 0];
   }
 }
 

@@ -334,17 +334,17 @@ struct StripItem {
 // strip-level (which read the same layer heights, transports and masks) and
 // then the neighbouring levels (which read the same tracer levels in the w
 // sub-step) run side by side and meet in that XCD's L2.
-template <typename TF>
+template <int P, typename TF>
 __device__ __forceinline__ bool strip_item(const FusedArgs<TF>& A, StripItem& it) {
-  const long nblk = (A.nitems + 3) / 4;
+  const long nblk = (A.nitems[P] + 3) / 4;
   const long per = (nblk + 7) / 8;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const long item = ((long)(blockIdx.x % 8) * per + blockIdx.x / 8) * 4 + wave;
-  if (item >= A.nitems) return false;
+  if (item >= A.nitems[P]) return false;
   int di = 0;
-  while (di + 1 < A.ndom && A.doms[di + 1].item_begin <= item) di++;
+  while (di + 1 < A.ndom && A.doms[di + 1].item_begin[P] <= item) di++;
   const FusedDomain<TF>& a = A.doms[di];
-  long loc = item - a.item_begin;
+  long loc = item - a.item_begin[P];
   it.dom = di;
   it.grp = (int)(loc % A.ngroups);
   loc /= A.ngroups;
@@ -401,13 +401,13 @@ __device__ __forceinline__ void sto(T* base, uint32_t off, T v) {
 template <typename TF, int SCH, int G, bool EXACT, bool ADV, bool DIFF>
 __global__ __launch_bounds__(256) void k_strip_uv(FusedArgs<TF> A) {
   StripItem it;
-  if (!strip_item(A, it)) return;
+  if (!strip_item<0>(A, it)) return;
   const FusedDomain<TF>& a = A.doms[it.dom];
   const int lane = threadIdx.x & 63;
   const int NX = a.d.NX, NY = a.d.NY;
   const long NXY = a.d.NXY;
   const int x = it.sx * kStripOut - 2 + lane;
-  const int y0 = it.sy * A.rows;
+  const int y0 = it.sy * A.rows[0];
   const int t0 = it.grp * G;
   const int ntr = min(G, A.ntracers - t0);
   constexpr int isub = SCH == UPSTREAM ? 0 : 1;
@@ -431,7 +431,7 @@ __global__ __launch_bounds__(256) void k_strip_uv(FusedArgs<TF> A) {
   double* d2 = a.d2 + oL;
   double* a2 = ADV ? a.a2 + oL : nullptr;
   const uint8_t* flags = a.flags;
-  const int nsteps = A.rows + 4;
+  const int nsteps = A.rows[0] + 4;
   auto cell = [&](int y) -> uint32_t {
     return (y >= 0 && y < NY) ? (uint32_t)(cb.xc + NX * y) : (uint32_t)cb.xc;
   };
@@ -569,7 +569,7 @@ __global__ __launch_bounds__(256) void k_strip_uv(FusedArgs<TF> A) {
   // level 0 of the work fields (the second kernel reads levels K-2..K+2 of
   // f2 and K-1..K+1 of d2 from one base): the input tracers and ho
   if (it.lev == 1 && xout) {
-    for (int yv = y0; yv < y0 + A.rows && yv < NY; yv++) {
+    for (int yv = y0; yv < y0 + A.rows[0] && yv < NY; yv++) {
       const uint32_t c = (uint32_t)(x + NX * yv);
       for (int g = 0; g < ntr; g++) {
         const long tg = (long)(t0 + g) * ts;
@@ -584,7 +584,7 @@ __global__ __launch_bounds__(256) void k_strip_uv(FusedArgs<TF> A) {
 template <typename TF, int SCH, int G, bool EXACT, bool ADV, bool DIFF>
 __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
   StripItem it;
-  if (!strip_item(A, it)) return;
+  if (!strip_item<1>(A, it)) return;
   const FusedDomain<TF>& a = A.doms[it.dom];
   const int lane = threadIdx.x & 63;
   const int NX = a.d.NX, NY = a.d.NY, kmax = a.d.kmax;
@@ -592,7 +592,7 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
   const int K = it.lev;
   const int kshift = A.kshift;
   const int x = it.sx * kStripOut - 2 + lane;
-  const int y0 = it.sy * A.rows;
+  const int y0 = it.sy * A.rows[1];
   const int t0 = it.grp * G;
   const int ntr = min(G, A.ntracers - t0);
   constexpr int isub = SCH == UPSTREAM ? 0 : 1;
@@ -635,7 +635,7 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
   const double* a2 = ADV ? a.a2 + oK : nullptr;
   double* hires = a.hires ? a.hires + oK : nullptr;
   const uint8_t* flags = a.flags;
-  const int nsteps = A.rows + 4;
+  const int nsteps = A.rows[1] + 4;
   auto cell = [&](int y) -> uint32_t {
     return (y >= 0 && y < NY) ? (uint32_t)(cb.xc + NX * y) : (uint32_t)cb.xc;
   };
@@ -822,7 +822,7 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
 
   // level 0 passes through (the step updates levels 1..kmax)
   if (K == 1 && xout) {
-    for (int yv = y0; yv < y0 + A.rows && yv < NY; yv++) {
+    for (int yv = y0; yv < y0 + A.rows[1] && yv < NY; yv++) {
       const uint32_t c = (uint32_t)(x + NX * yv);
       for (int g = 0; g < ntr; g++) {
         const long tg = (long)(t0 + g) * ts;
@@ -839,27 +839,28 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
 template <typename TF>
 void strip_plan(FusedArgs<TF>& a, int group) {
   a.ngroups = (a.ntracers + group - 1) / group;
-  long items = 0;
-  for (int i = 0; i < a.ndom; i++) {
-    FusedDomain<TF>& d = a.doms[i];
-    d.nsx = (d.d.NX + kStripOut - 1) / kStripOut;
-    d.nsy = (d.d.NY + a.rows - 1) / a.rows;
-    d.item_begin = items;
-    items += (long)d.nsx * d.nsy * d.d.kmax * a.ngroups;
+  for (int p = 0; p < 2; p++) {
+    long items = 0;
+    for (int i = 0; i < a.ndom; i++) {
+      FusedDomain<TF>& d = a.doms[i];
+      d.nsx = (d.d.NX + kStripOut - 1) / kStripOut;
+      d.nsy[p] = (d.d.NY + a.rows[p] - 1) / a.rows[p];
+      d.item_begin[p] = items;
+      items += (long)d.nsx * d.nsy[p] * d.d.kmax * a.ngroups;
+    }
+    a.nitems[p] = items;
   }
-  a.nitems = items;
 }
 
 template <typename TF, int SCH, int G, bool EXACT, bool ADV, bool DIFF>
 hipError_t launch_two(hipStream_t st, const FusedArgs<TF>& A) {
-  const long nblk = (A.nitems + 3) / 4;
-  const long grid = ((nblk + 7) / 8) * 8;
-  hipLaunchKernelGGL((k_strip_uv<TF, SCH, G, EXACT, ADV, DIFF>), dim3((unsigned)grid),
-                     dim3(256), 0, st, A);
+  auto grid = [&](int p) { return (unsigned)((((A.nitems[p] + 3) / 4 + 7) / 8) * 8); };
+  hipLaunchKernelGGL((k_strip_uv<TF, SCH, G, EXACT, ADV, DIFF>), dim3(grid(0)), dim3(256), 0, st,
+                     A);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_strip_wvu<TF, SCH, G, EXACT, ADV, DIFF>), dim3((unsigned)grid),
-                     dim3(256), 0, st, A);
+  hipLaunchKernelGGL((k_strip_wvu<TF, SCH, G, EXACT, ADV, DIFF>), dim3(grid(1)), dim3(256), 0,
+                     st, A);
   return hipGetLastError();
 }
 

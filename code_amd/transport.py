@@ -317,33 +317,43 @@ def _batch_array(domains):
     return arr
 
 
-# tuning index -> rows per strip of the HALFSPLIT strip kernels
-ROWS = {0: 16, 1: 32, 2: 8}
+# strip heights of the HALFSPLIT strip kernels; a tuning index t < 3 gives
+# ROWS[t] rows to both kernels, 3 <= t < 12 gives ROWS[(t-3)%3] to the first
+# (u/2 v/2) and ROWS[(t-3)//3] to the second (w v/2 u/2)
+ROWS = (16, 32, 8)
 
 
-def tile_for_rows(rows):
-    """the tuning index of a strip height (8, 16 or 32 rows)"""
-    for t, r in ROWS.items():
-        if r == rows:
-            return t
-    raise ValueError(f"rows per strip must be one of {sorted(ROWS.values())}, not {rows}")
+def strip_rows(tile):
+    """(rows of the first kernel, rows of the second) for a tuning index"""
+    if 3 <= tile < 12:
+        return ROWS[(tile - 3) % 3], ROWS[(tile - 3) // 3]
+    r = ROWS[tile if 0 <= tile < 3 else 0]
+    return r, r
+
+
+def tile_for_rows(rows_uv, rows_wvu=None):
+    """the tuning index of a strip-height pair (8, 16 or 32 rows each)"""
+    rows_wvu = rows_uv if rows_wvu is None else rows_wvu
+    if rows_uv not in ROWS or rows_wvu not in ROWS:
+        raise ValueError(f"rows per strip must be one of {sorted(ROWS)}")
+    return 3 + ROWS.index(rows_uv) + 3 * ROWS.index(rows_wvu)
 
 
 def autotune_batch(domains, ntracers, dt, hscheme, vscheme, AH=-1.0, tag=H_TAG, reps=3,
                    stream=None):
     """gt_autotune_batch: returns (tile, group, best_ms, {(tile, group): ms}); tile
-    indexes ROWS (rows per strip), group is the number of tracers per wavefront"""
+    is a strip-height index (strip_rows), group the tracers per wavefront"""
     import torch
     lib = load_library()
     arr = _batch_array(domains)
     f32 = domains[0]["f_in"].dtype == torch.float32
     bt, bg, bm = C.c_int(), C.c_int(), C.c_float()
-    table = (C.c_float * 12)()
+    table = (C.c_float * 36)()
     handle = getattr(stream, "cuda_stream", stream)
     check(lib.gt_autotune_batch(len(domains), arr, ntracers, dt, hscheme, vscheme, AH, tag,
                                 int(f32), reps, C.c_void_p(handle) if handle else None,
                                 C.byref(bt), C.byref(bg), C.byref(bm), table))
-    pairs = [(t, g) for t in (0, 1, 2) for g in ((1, 3, 6, 9) if f32 else (1, 2))]
+    pairs = [(t, g) for t in range(3, 12) for g in ((1, 3, 6, 9) if f32 else (1, 2))]
     return bt.value, bg.value, bm.value, {p: table[i] for i, p in enumerate(pairs)}
 
 

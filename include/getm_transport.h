@@ -191,15 +191,17 @@ int gt_adv3d_batch_f64(int ndom, const gt_batch_domain *doms, int ntracers,
                        double dt, int hscheme, int vscheme, double AH, int tag,
                        void *hip_stream);
 /* kernel choice: the two HALFSPLIT strip kernels (fused=1) or the
- * per-sub-step kernels (0), rows per strip (tile 0: 16, 1: 32, 2: 8) and
- * tracers per wavefront (group: FP32 1, 3, 6 or 9; FP64 1, or 2 for any other
- * value when there are two or more tracers) */
+ * per-sub-step kernels (0); rows per strip: tile 0, 1, 2 give 16, 32, 8 rows
+ * to both kernels, tile 3..11 gives {16, 32, 8}[(tile-3)%3] rows to the first
+ * (u/2 v/2) and {16, 32, 8}[(tile-3)/3] to the second (w v/2 u/2); tracers
+ * per wavefront (group: FP32 1, 3, 6 or 9; FP64 1, or 2 for any other value
+ * when there are two or more tracers) */
 int gt_set_tuning(gt_ctx *ctx, int fused, int tile, int group);
-/* autotuner: times gt_adv3d_batch_f32 (f32=1) or _f64 over rows per strip
- * {16, 32, 8} x tracers per wavefront {1, 3, 6, 9} (FP64: {1, 2}), reps
- * timed launches each, keeps the fastest on every context.  table (optional,
- * 12 floats) receives the mean ms of each pair in sweep order.  f_out is
- * overwritten, f_in is not touched. */
+/* autotuner: times gt_adv3d_batch_f32 (f32=1) or _f64 over the nine strip
+ * height pairs (tile 3..11) x tracers per wavefront {1, 3, 6, 9} (FP64:
+ * {1, 2}), reps timed launches each, keeps the fastest on every context.
+ * table (optional, 36 floats) receives the mean ms of each pair in sweep
+ * order (tile outer).  f_out is overwritten, f_in is not touched. */
 int gt_autotune_batch(int ndom, const gt_batch_domain *doms, int ntracers,
                       double dt, int hscheme, int vscheme, double AH, int tag,
                       int f32, int reps, void *hip_stream, int *best_tile,
