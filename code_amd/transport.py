@@ -190,6 +190,7 @@ class Context:
         """stream: a torch.cuda.Stream (or raw handle int)"""
         handle = getattr(stream, "cuda_stream", stream)
         check(self.lib.gt_set_stream(self.h, C.c_void_p(handle)))
+        self.stream = stream
 
     def synchronize(self):
         check(self.lib.gt_synchronize(self.h))
