@@ -676,6 +676,44 @@ extern "C" int gt_eos_linear(gt_ctx* c, const double* T, const double* S, double
   return GT_OK;
 }
 
+extern "C" int gt_do_eqstate(gt_ctx* c, int method, int nonnegsalt, const double* T,
+                             const double* S, const double* hn, double* rho, double* buoy,
+                             double* alpha, double* beta, double T0, double S0, double dtr0,
+                             double dsr0) {
+  if (!c || !T || !S || !rho || !buoy) return set_err(GT_EARG, "gt_do_eqstate: NULL argument");
+  if (method < 1 || method > 3)
+    return set_err(GT_EARG, "init_eqstate(): not a valid eqstate_method");
+  if (!alpha != !beta) return set_err(GT_EARG, "gt_do_eqstate: give both alpha and beta or neither");
+  if (method != 1 && alpha && !hn) return set_err(GT_EARG, "gt_do_eqstate: hn is needed for alpha/beta");
+  LCHK(gt::launch_eqstate(c->st, c->d, method, nonnegsalt != 0, c->az, T, S, hn, rho, buoy,
+                          alpha, beta, T0, S0, dtr0, dsr0));
+  return GT_OK;
+}
+
+static int eos_points(gt_ctx* c, int fn, long n, const double* s, const double* th,
+                      const double* p, double* o1, double* o2, int nonneg, const char* who) {
+  if (!c || n < 0 || (n > 0 && (!s || !th || !o1 || (fn == 2 && !o2))))
+    return set_err(GT_EARG, "%s: bad argument", who);
+  LCHK(gt::launch_eos_points(c->st, fn, n, s, th, p, o1, o2, nonneg != 0));
+  return GT_OK;
+}
+
+extern "C" int gt_rho_from_theta_unesco80(gt_ctx* c, long n, const double* T, const double* S,
+                                          double* rho, int nonnegsalt) {
+  return eos_points(c, 0, n, S, T, nullptr, rho, nullptr, nonnegsalt,
+                    "gt_rho_from_theta_unesco80");
+}
+
+extern "C" int gt_rho_from_theta(gt_ctx* c, long n, const double* s, const double* th,
+                                 const double* p, double* dens0, double* densp) {
+  return eos_points(c, 1, n, s, th, p, dens0, densp, 0, "gt_rho_from_theta");
+}
+
+extern "C" int gt_eosall_from_theta(gt_ctx* c, long n, const double* s, const double* th,
+                                    const double* p, double* rho_s, double* rho_th) {
+  return eos_points(c, 2, n, s, th, p, rho_s, rho_th, 0, "gt_eosall_from_theta");
+}
+
 extern "C" int gt_ip_blumberg_mellor(gt_ctx* c, const double* H, const double* hn,
                                      const double* hun, const double* hvn, const double* buoy,
                                      double* zz, double* idpdx, double* idpdy) {

@@ -41,6 +41,16 @@ hipError_t launch_upstream_2dh(hipStream_t st, Dims d, int nslice, const double*
 hipError_t launch_eos_linear(hipStream_t st, Dims d, const double* T, const double* S,
                              double* rho, double* buoy, double T0, double S0, double dtr0,
                              double dsr0);
+// do_eqstate (eqstate.F90:94-271): method 1 linear (+ alpha/beta fill), 2
+// UNESCO 1980, 3 Jackett et al. (2006); alpha/beta may be NULL
+hipError_t launch_eqstate(hipStream_t st, Dims d, int method, bool nonnegsalt,
+                          const int32_t* az, const double* T, const double* S,
+                          const double* hn, double* rho, double* buoy, double* alpha,
+                          double* beta, double T0, double S0, double dtr0, double dsr0);
+// fn 0: rho_from_theta_unesco80(th=T, s=S) -> o1; 1: rho_from_theta -> o1 =
+// dens0, o2 = densp; 2: eosall_from_theta -> o1 = rho_s, o2 = rho_th
+hipError_t launch_eos_points(hipStream_t st, int fn, long n, const double* s, const double* th,
+                             const double* p, double* o1, double* o2, bool nonnegsalt);
 hipError_t launch_ip_bm(hipStream_t st, Dims d, const int32_t* az, const int32_t* au,
                         const int32_t* av, const double* H, const double* hn,
                         const double* hun, const double* hvn, const double* buoy, double* zz,

@@ -42,6 +42,7 @@ EXPORTS = [
     "gt_memcpy_h2d", "gt_memcpy_d2h", "gt_adv3d_batch_f32", "gt_adv3d_batch_f64",
     "gt_set_tuning", "gt_budget_f32", "gt_adv3d_adjoint",
     "gt_ip_blumberg_mellor_adjoint", "gt_eos_linear_adjoint", "gt_autotune_batch",
+    "gt_do_eqstate", "gt_rho_from_theta_unesco80", "gt_rho_from_theta", "gt_eosall_from_theta",
 ]
 
 
@@ -95,6 +96,10 @@ def load_library():
         "gt_do_advection_3d": (i, [vp, d, vp, vp, vp, vp, vp, vp, vp, vp, i, i, i, d, i,
                                    vp, vp]),
         "gt_eos_linear": (i, [vp, vp, vp, vp, vp, d, d, d, d]),
+        "gt_do_eqstate": (i, [vp, i, i, vp, vp, vp, vp, vp, vp, vp, d, d, d, d]),
+        "gt_rho_from_theta_unesco80": (i, [vp, C.c_long, vp, vp, vp, i]),
+        "gt_rho_from_theta": (i, [vp, C.c_long, vp, vp, vp, vp, vp]),
+        "gt_eosall_from_theta": (i, [vp, C.c_long, vp, vp, vp, vp, vp]),
         "gt_ip_blumberg_mellor": (i, [vp, vp, vp, vp, vp, vp, vp, vp, vp]),
         "gt_ip_shchepetkin_mcwilliams": (i, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
         "gt_do_internal_pressure": (i, [vp, i, vp, vp, vp, vp, vp, vp, vp, vp]),
@@ -240,6 +245,27 @@ class Context:
     def eos_linear(self, T, S, rho, buoy, T0, S0, dtr0, dsr0):
         check(self.lib.gt_eos_linear(self.h, _ptr(T), _ptr(S), _ptr(rho), _ptr(buoy),
                                      T0, S0, dtr0, dsr0))
+
+    def do_eqstate(self, method, T, S, hn, rho, buoy, alpha=None, beta=None, T0=10.0,
+                   S0=33.75, dtr0=-0.17, dsr0=0.78, nonnegsalt=False):
+        """do_eqstate (eqstate.F90:94-271); the defaults are the reference's
+        namelist defaults, FP32 literals (rounded here)."""
+        f32 = lambda v: float(np.float32(v))
+        check(self.lib.gt_do_eqstate(self.h, method, int(nonnegsalt), _ptr(T), _ptr(S),
+                                     _ptr(hn), _ptr(rho), _ptr(buoy), _ptr(alpha), _ptr(beta),
+                                     f32(T0), f32(S0), f32(dtr0), f32(dsr0)))
+
+    def rho_from_theta_unesco80(self, T, S, rho, nonnegsalt=False):
+        check(self.lib.gt_rho_from_theta_unesco80(self.h, T.numel(), _ptr(T), _ptr(S),
+                                                  _ptr(rho), int(nonnegsalt)))
+
+    def rho_from_theta(self, s, th, p, dens0, densp=None):
+        check(self.lib.gt_rho_from_theta(self.h, s.numel(), _ptr(s), _ptr(th), _ptr(p),
+                                         _ptr(dens0), _ptr(densp)))
+
+    def eosall_from_theta(self, s, th, p, rho_s, rho_th):
+        check(self.lib.gt_eosall_from_theta(self.h, s.numel(), _ptr(s), _ptr(th), _ptr(p),
+                                            _ptr(rho_s), _ptr(rho_th)))
 
     def ip_blumberg_mellor(self, H, hn, hun, hvn, buoy, zz, idpdx, idpdy):
         check(self.lib.gt_ip_blumberg_mellor(self.h, _ptr(H), _ptr(hn), _ptr(hun), _ptr(hvn),

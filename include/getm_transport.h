@@ -126,6 +126,32 @@ int gt_do_advection_3d(gt_ctx *ctx, double dt, double *f, const double *uu,
 int gt_eos_linear(gt_ctx *ctx, const double *T, const double *S, double *rho,
                   double *buoy, double T0, double S0, double dtr0,
                   double dsr0);
+/* do_eqstate, src/3d/eqstate.F90:94-271 (the routine is argument-less; its
+ * module variables are arguments here).  method = eqstate_method: 1 linear
+ * (rho on every cell, alpha = dtr0 and beta = dsr0 on the whole array), 2
+ * UNESCO 1980, 3 Jackett et al. (2006) (rho on water columns az > 0; land
+ * keeps its rho).  For 2 and 3, alpha/beta are d rho/d theta and d rho/d S at
+ * the layer interfaces k = 1..kmax-1 (zero at kmax) from the interface means
+ * of T and S at the pressure p = sum of hn above (kmin = 1, as sigma and
+ * general coordinates set it).  buoy = -g/rho_0 (rho - rho_0) on every cell.
+ * alpha and beta may both be NULL (the _OLD_BVF_ build); hn is read only for
+ * them.  nonnegsalt = 1 is the NONNEGSALT build (negative S taken as 0 in
+ * rho).  A bad method returns GT_EARG ('init_eqstate(): not a valid
+ * eqstate_method', eqstate.F90:75-77). */
+int gt_do_eqstate(gt_ctx *ctx, int method, int nonnegsalt, const double *T,
+                  const double *S, const double *hn, double *rho,
+                  double *buoy, double *alpha, double *beta, double T0,
+                  double S0, double dtr0, double dsr0);
+/* the module's point functions, elementwise over n device points:
+ * rho_from_theta_unesco80 (eqstate.F90:280-329), rho_from_theta (:338-423;
+ * p may be NULL for p = 0; densp may be NULL), eosall_from_theta (:432-562) */
+int gt_rho_from_theta_unesco80(gt_ctx *ctx, long n, const double *T,
+                               const double *S, double *rho, int nonnegsalt);
+int gt_rho_from_theta(gt_ctx *ctx, long n, const double *s, const double *th,
+                      const double *p, double *dens0, double *densp);
+int gt_eosall_from_theta(gt_ctx *ctx, long n, const double *s,
+                         const double *th, const double *p, double *rho_s,
+                         double *rho_th);
 /* ip_blumberg_mellor, src/3d/ip_blumberg_mellor.F90:8-152.  The module
  * variables it reads are arguments here; zz is the module array
  * (internal_pressure.F90:66-72), written where the reference writes it. */

@@ -813,6 +813,176 @@ void or_eos_linear(const or_dom *d, const double *T, const double *S,
             }
 }
 
+/* ---- full equation of state, eqstate.F90:94-562 --------------------------
+ * Literals written without a kind in the reference are default REAL (FP32)
+ * and enter the FP64 arithmetic rounded to FP32; the Jackett et al. (2006)
+ * coefficients are d0 literals (FP64).  Every expression keeps the
+ * reference's left-to-right order. */
+
+/* rho_from_theta_unesco80, eqstate.F90:280-329 (UNESCO 1980 one-atmosphere
+ * density; NONNEGSALT clamps negative salinity, :309-317) */
+double or_rho_unesco80(double T, double S, int nonnegsalt)
+{
+    const double t1 = T, t2 = t1 * t1, t3 = t1 * t2, t4 = t2 * t2,
+                 t5 = t1 * t4;
+    double s1 = S;
+    if (nonnegsalt && s1 < 0.0) s1 = 0.0;
+    const double s2 = s1 * s1, s3 = s1 * s2;
+    double x = 999.842594f + 6.793952e-02f * t1 - 9.09529e-03f * t2 +
+               1.001685e-04f * t3;
+    x = x - 1.120083e-06f * t4 + 6.536332e-09f * t5;
+    x = x + s1 * (0.824493f - 4.0899e-03f * t1 + 7.6438e-05f * t2 -
+                  8.2467e-07f * t3);
+    x = x + s1 * 5.3875e-09f * t4;
+    x = x + sqrt(s3) * (-5.72466e-03f + 1.0227e-04f * t1 - 1.6546e-06f * t2);
+    x = x + 4.8314e-04f * s2;
+    return x;
+}
+
+/* numerator and denominator of the Jackett et al. (2006) rational function
+ * at p = 0, eqstate.F90:381-400 (same terms at :478-498) */
+static void jackett_p0(double s, double th, double th2, double sqrts,
+                       double *anum, double *aden)
+{
+    *anum = 9.9984085444849347e+02 +
+            th * (7.3471625860981584e+00 +
+                  th * (-5.3211231792841769e-02 + th * 3.6492439109814549e-04)) +
+            s * (2.5880571023991390e+00 - th * 6.7168282786692355e-03 +
+                 s * 1.9203202055760151e-03);
+    *aden = 1.0000000000000000e+00 +
+            th * (7.2815210113327091e-03 +
+                  th * (-4.4787265461983921e-05 +
+                        th * (3.3851002965802430e-07 +
+                              th * 1.3651202389758572e-10))) +
+            s * (1.7632126669040377e-03 -
+                 th * (8.8066583251206474e-06 + th2 * 1.8832689434804897e-10) +
+                 sqrts * (5.7463776745432097e-06 + th2 * 1.4716275472242334e-09));
+}
+
+/* the pressure terms, eqstate.F90:407-417 (same at :526-536) */
+static void jackett_p(double s, double th2, double p, double pth, double *anum,
+                      double *aden)
+{
+    *anum = *anum + p * (1.1798263740430364e-02 + th2 * 9.8920219266399117e-08 +
+                         s * 4.6996642771754730e-06 -
+                         p * (2.5862187075154352e-08 + th2 * 3.2921414007960662e-12));
+    *aden = *aden + p * (6.7103246285651894e-06 -
+                         pth * (th2 * 2.4461698007024582e-17 +
+                                p * 9.1534417604289062e-18));
+}
+
+/* rho_from_theta, eqstate.F90:338-423 */
+void or_rho_from_theta(double s, double th, double p, double *dens0,
+                       double *densp)
+{
+    const double th2 = th * th, sqrts = sqrt(s);
+    double anum, aden;
+    jackett_p0(s, th, th2, sqrts, &anum, &aden);
+    *dens0 = anum / aden;
+    *densp = *dens0;
+    if (p != 0.0) {
+        jackett_p(s, th2, p, p * th, &anum, &aden);
+        *densp = anum / aden;
+    }
+}
+
+/* eosall_from_theta, eqstate.F90:432-562: d(rho)/ds and d(rho)/d(theta) */
+void or_eosall_from_theta(double s, double th, double p, double *rho_s,
+                          double *rho_th)
+{
+    const double th2 = th * th, sqrts = sqrt(s);
+    double anum, aden;
+    jackett_p0(s, th, th2, sqrts, &anum, &aden);
+    double anum_s = 2.5880571023991390e+00 - th * 6.7168282786692355e-03 +
+                    s * 3.8406404111520300e-03;
+    const double aden_s =
+        1.7632126669040377e-03 +
+        th * (-8.8066583251206470e-06 - th2 * 1.8832689434804897e-10) +
+        sqrts * (8.6195665118148150e-06 + th2 * 2.2074413208363504e-09);
+    double anum_th = 7.3471625860981580e+00 +
+                     th * (-1.0642246358568354e-01 + th * 1.0947731732944364e-03) -
+                     s * 6.7168282786692355e-03;
+    double aden_th =
+        7.2815210113327090e-03 +
+        th * (-8.9574530923967840e-05 +
+              th * (1.0155300889740728e-06 + th * 5.4604809559034290e-10)) +
+        s * (-8.8066583251206470e-06 - th2 * 5.6498068304414700e-10 +
+             th * sqrts * 2.9432550944484670e-09);
+    if (p != 0.0) {
+        const double pth = p * th;
+        jackett_p(s, th2, p, pth, &anum, &aden);
+        anum_s = anum_s + p * 4.6996642771754730e-06;
+        anum_th = anum_th + pth * (1.9784043853279823e-07 - p * 6.5842828015921320e-12);
+        aden_th = aden_th - p * p * (th2 * 7.3385094021073750e-17 +
+                                     p * 9.1534417604289060e-18);
+    }
+    const double rec_aden = 1.0 / aden;
+    const double rho = anum * rec_aden;
+    *rho_s = (anum_s - aden_s * rho) * rec_aden;
+    *rho_th = (anum_th - aden_th * rho) * rec_aden;
+}
+
+/* do_eqstate, eqstate.F90:94-271, with kmin = 1 (sigma and general
+ * coordinates, sigma_coordinates.F90:78, general_coordinates.F90:104).
+ * method 1: linear rho on every cell, alpha = dtr0 and beta = dsr0 on the
+ * whole array (:127-149); 2: UNESCO 1980 and 3: Jackett et al. (2006) on
+ * water columns (az > 0), land keeps its rho (:151-219); for 2 and 3 the
+ * expansion coefficients at the layer interfaces from the interface means of
+ * T and S and the pressure accumulated from the top (:168-190, :222-245).
+ * Then buoy on every cell from rho (:252-261).  alpha/beta may be NULL (the
+ * _OLD_BVF_ build does not compute them).  Returns 0, or 1 for a bad method
+ * (init_eqstate stops, :75-77). */
+int or_do_eqstate(const or_dom *d, const int32_t *az, int method,
+                  int nonnegsalt, const double *T, const double *S,
+                  const double *hn, double *rho, double *buoy, double *alpha,
+                  double *beta, double T0, double S0, double dtr0, double dsr0)
+{
+    DOMVARS(d);
+    if (method < 1 || method > 3) return 1;
+    const double g = (double)9.81f, rho_0 = (double)1025.f;
+    if (method == 1) {
+        for (long k = 1; k <= kmax; k++)
+            for (long c = 0; c < NXY; c++) {
+                const long q = c + NXY * k;
+                rho[q] = rho_0 + dtr0 * (T[q] - T0) + dsr0 * (S[q] - S0);
+            }
+        if (alpha)
+            for (long q = 0; q < NXY * (kmax + 1); q++) alpha[q] = dtr0;
+        if (beta)
+            for (long q = 0; q < NXY * (kmax + 1); q++) beta[q] = dsr0;
+    } else {
+#pragma omp parallel for OMP_NT schedule(static)
+        for (long c = 0; c < NXY; c++) {
+            if (az[c] <= 0) continue;
+            for (int k = 1; k <= kmax; k++) {
+                const long q = c + NXY * k;
+                if (method == 2) {
+                    rho[q] = or_rho_unesco80(T[q], S[q], nonnegsalt);
+                } else {
+                    double s1 = S[q], densp;
+                    if (nonnegsalt && s1 < 0.0) s1 = 0.0;
+                    or_rho_from_theta(s1, T[q], 0.0, &rho[q], &densp);
+                }
+            }
+            if (alpha && beta) {
+                alpha[c + NXY * kmax] = 0.0;
+                beta[c + NXY * kmax] = 0.0;
+                double p1 = 0.0;
+                for (int k = kmax - 1; k >= 1; k--) {
+                    const long q = c + NXY * k, qu = q + NXY;
+                    const double th = 0.5 * (T[qu] + T[q]);
+                    const double s1 = 0.5 * (S[qu] + S[q]);
+                    p1 = p1 + hn[qu];
+                    or_eosall_from_theta(s1, th, p1, &beta[q], &alpha[q]);
+                }
+            }
+        }
+    }
+    const double x = -g / rho_0;
+    for (long q = NXY; q < NXY * (kmax + 1); q++) buoy[q] = x * (rho[q] - rho_0);
+    return 0;
+}
+
 /* ---- a9: ip_blumberg_mellor.F90:67-150 ---------------------------------- */
 void or_ip_blumberg_mellor(const or_dom *d, const int32_t *az,
                            const int32_t *au, const int32_t *av,
@@ -1180,4 +1350,326 @@ void or_budget(const or_dom *d, const int32_t *az, const double *h,
     vl += v.lo * area;
     *mass = mh + ml;
     *volume = vh + vl;
+}
+
+/* ---- input producers of the transport step --------------------------------
+ * start_macro (elevations, depths and mean transports of the macro step),
+ * coordinates (layer heights on T, U and V points) and ww_momentum_3d (the
+ * grid-related vertical velocity), which m3d.F90:446-548 runs before the
+ * tracer advection.  Cartesian metrics; kmin = 1 (sigma and general
+ * coordinates). */
+
+static inline double dmax2(double a, double b) { return a > b ? a : b; }
+static inline double dmin2(double a, double b) { return a < b ? a : b; }
+
+/* start_macro, start_macro.F90:54-86 */
+void or_start_macro(const or_dom *d, int M, double min_depth, const double *H,
+                    const double *HU, const double *HV, const double *z,
+                    double *sseo, double *ssen, double *ssuo, double *ssun,
+                    double *ssvo, double *ssvn, double *Dn, double *Dun,
+                    double *Dvn, double *Uint, double *Vint)
+{
+    DOMVARS(d);
+    for (long c = 0; c < NXY; c++) {
+        sseo[c] = ssen[c];
+        ssen[c] = z[c];
+        Dn[c] = ssen[c] + H[c];
+    }
+    for (long jj = 0; jj < NY; jj++)
+        for (long ii = 0; ii < NX - 1; ii++) {
+            const long c = ii + NX * jj;
+            ssuo[c] = ssun[c];
+            ssun[c] = 0.25 * (sseo[c] + sseo[c + 1] + ssen[c] + ssen[c + 1]);
+            ssun[c] = dmax2(ssun[c], -HU[c] + min_depth);
+            Dun[c] = ssun[c] + HU[c];
+        }
+    for (long jj = 0; jj < NY - 1; jj++)
+        for (long ii = 0; ii < NX; ii++) {
+            const long c = ii + NX * jj;
+            ssvo[c] = ssvn[c];
+            ssvn[c] = 0.25 * (sseo[c] + sseo[c + NX] + ssen[c] + ssen[c + NX]);
+            ssvn[c] = dmax2(ssvn[c], -HV[c] + min_depth);
+            Dvn[c] = ssvn[c] + HV[c];
+        }
+    const double split = 1.0 / (double)(float)M;
+    for (long c = 0; c < NXY; c++) {
+        Uint[c] = split * Uint[c];
+        Vint[c] = split * Vint[c];
+    }
+}
+
+/* the SAVE'd state of coordinates(), sigma_coordinates() and
+ * general_coordinates(): the first-call flag and the level arrays */
+typedef struct {
+    int vert_cord, gamma_surf, first, equiv_sigma, kmax;
+    double ddu, ddl, d_gamma, min_depth;
+    double *ga, *dga, *sig, *be, *gga;
+} or_coord;
+
+void *or_coord_new(const or_dom *d, int vert_cord, double ddu, double ddl,
+                   double d_gamma, int gamma_surf, double min_depth)
+{
+    DOMVARS(d);
+    or_coord *s = (or_coord *)calloc(1, sizeof(or_coord));
+    s->vert_cord = vert_cord;
+    s->ddu = ddu;
+    s->ddl = ddl;
+    s->d_gamma = d_gamma;
+    s->gamma_surf = gamma_surf;
+    s->min_depth = min_depth;
+    s->first = 1;
+    s->kmax = kmax;
+    s->ga = (double *)calloc(4 * (size_t)(kmax + 1), sizeof(double));
+    s->dga = s->ga + (kmax + 1);
+    s->sig = s->dga + (kmax + 1);
+    s->be = s->sig + (kmax + 1);
+    if (vert_cord == 3)
+        s->gga = (double *)calloc((size_t)(NXY * (kmax + 1)), sizeof(double));
+    return s;
+}
+
+void or_coord_delete(void *p)
+{
+    or_coord *s = (or_coord *)p;
+    if (!s) return;
+    free(s->ga);
+    free(s->gga);
+    free(s);
+}
+
+/* the surface/bottom zooming of Garapon, sigma_coordinates.F90:72-76 and
+ * general_coordinates.F90:72-75 */
+static double zoom_level(double ddu, double ddl, int k, int kmax)
+{
+    double g = tanh((ddl + ddu) * k / (double)(float)kmax - ddl) + tanh(ddl);
+    return g / (tanh(ddl) + tanh(ddu)) - 1.0;
+}
+
+/* coordinates(), coordinates.F90:66-107, for vert_cord 1 (sigma,
+ * sigma_coordinates.F90:53-147), 2 (z-levels: nothing but zc) and 3
+ * (general, general_coordinates.F90:56-210).  zc may be NULL.  Returns 0,
+ * 1 when the general coordinates are not monotone (the reference stops,
+ * :92-99), 2 for another vert_cord. */
+int or_coordinates(const or_dom *d, void *state, double dt, double cord_relax,
+                   double maxdepth, const double *H, const double *HU,
+                   const double *HV, const double *sseo, const double *ssen,
+                   const double *ssuo, const double *ssun, const double *ssvo,
+                   const double *ssvn, double *ho, double *hn, double *huo,
+                   double *hun, double *hvo, double *hvn, double *zc)
+{
+    DOMVARS(d);
+    or_coord *s = (or_coord *)state;
+    double *ga = s->ga, *dga = s->dga, *sig = s->sig, *be = s->be,
+           *gga = s->gga;
+#define G3(c, k) gga[(c) + NXY * (long)(k)]
+#define F3(a, c, k) a[(c) + NXY * (long)(k)]
+    if (s->vert_cord == 1) {
+        if (s->first) {
+            if (s->ddu <= 0.0 && s->ddl <= 0.0) {
+                s->equiv_sigma = 1;
+                ga[0] = -1.0;
+                for (int k = 1; k <= kmax; k++) ga[k] = ga[k - 1] + 1.0 / kmax;
+                ga[kmax] = 0.0;
+            } else {
+                if (s->ddu < 0.0) s->ddu = 0.0;
+                if (s->ddl < 0.0) s->ddl = 0.0;
+                ga[0] = -1.0;
+                dga[0] = 0.0;
+                for (int k = 1; k <= kmax; k++) {
+                    ga[k] = zoom_level(s->ddu, s->ddl, k, kmax);
+                    dga[k] = ga[k] - ga[k - 1];
+                }
+            }
+            s->first = 0;
+        }
+        if (s->equiv_sigma) {
+            const double kmaxm1 = 1.0 / (double)(float)kmax;
+            for (long c = 0; c < NXY; c++)
+                for (int k = 0; k <= kmax; k++) {
+                    F3(ho, c, k) = (sseo[c] + H[c]) * kmaxm1;
+                    F3(hn, c, k) = (ssen[c] + H[c]) * kmaxm1;
+                }
+            for (long jj = 0; jj < NY; jj++)
+                for (long ii = 0; ii < NX - 1; ii++) {
+                    const long c = ii + NX * jj;
+                    for (int k = 0; k <= kmax; k++) {
+                        F3(huo, c, k) = (ssuo[c] + HU[c]) * kmaxm1;
+                        F3(hun, c, k) = (ssun[c] + HU[c]) * kmaxm1;
+                    }
+                }
+            for (long jj = 0; jj < NY - 1; jj++)
+                for (long ii = 0; ii < NX; ii++) {
+                    const long c = ii + NX * jj;
+                    for (int k = 0; k <= kmax; k++) {
+                        F3(hvo, c, k) = (ssvo[c] + HV[c]) * kmaxm1;
+                        F3(hvn, c, k) = (ssvn[c] + HV[c]) * kmaxm1;
+                    }
+                }
+        } else {
+            for (int k = 1; k <= kmax; k++) {
+                for (long c = 0; c < NXY; c++) {
+                    F3(ho, c, k) = (sseo[c] + H[c]) * dga[k];
+                    F3(hn, c, k) = (ssen[c] + H[c]) * dga[k];
+                }
+                for (long jj = 0; jj < NY; jj++)
+                    for (long ii = 0; ii < NX - 1; ii++) {
+                        const long c = ii + NX * jj;
+                        F3(huo, c, k) = (ssuo[c] + HU[c]) * dga[k];
+                        F3(hun, c, k) = (ssun[c] + HU[c]) * dga[k];
+                    }
+                for (long jj = 0; jj < NY - 1; jj++)
+                    for (long ii = 0; ii < NX; ii++) {
+                        const long c = ii + NX * jj;
+                        F3(hvo, c, k) = (ssvo[c] + HV[c]) * dga[k];
+                        F3(hvn, c, k) = (ssvn[c] + HV[c]) * dga[k];
+                    }
+            }
+        }
+    } else if (s->vert_cord == 3) {
+        if (s->first) {
+            for (int k = 0; k <= kmax; k++) ga[k] = k;
+            be[0] = -1.0;
+            sig[0] = -1.0;
+            if (s->ddu < 0.0) s->ddu = 0.0;
+            if (s->ddl < 0.0) s->ddl = 0.0;
+            for (int k = 1; k <= kmax; k++) {
+                be[k] = zoom_level(s->ddu, s->ddl, k, kmax);
+                /* k/float(kmax) is a default-REAL (FP32) quotient */
+                sig[k] = (double)((float)k / (float)kmax) - 1.0;
+            }
+            const int kk = s->gamma_surf ? kmax : 1;
+            for (long c = 0; c < NXY; c++) {
+                const double HH = dmax2(sseo[c] + H[c], s->min_depth);
+                const double alpha =
+                    dmin2(((be[kk] - be[kk - 1]) -
+                           s->d_gamma / HH * (sig[kk] - sig[kk - 1])) /
+                              ((be[kk] - be[kk - 1]) - (sig[kk] - sig[kk - 1])),
+                          1.0);
+                G3(c, 0) = -1.0;
+                for (int k = 1; k <= kmax; k++) {
+                    G3(c, k) = alpha * sig[k] + (1.0 - alpha) * be[k];
+                    if (G3(c, k) < G3(c, k - 1)) return 1;
+                }
+            }
+            for (int k = 1; k <= kmax; k++)
+                for (long c = 0; c < NXY; c++) {
+                    const double HH = dmax2(sseo[c] + H[c], s->min_depth);
+                    F3(hn, c, k) = HH * (G3(c, k) - G3(c, k - 1));
+                }
+            for (int k = 1; k <= kmax; k++)
+                for (long jj = 0; jj < NY; jj++)
+                    for (long ii = 0; ii < NX - 1; ii++) {
+                        const long c = ii + NX * jj;
+                        const double HH = dmax2(ssuo[c] + HU[c], s->min_depth);
+                        F3(huo, c, k) = HH * 0.5 *
+                                        (G3(c, k) - G3(c, k - 1) + G3(c + 1, k) -
+                                         G3(c + 1, k - 1));
+                        F3(hun, c, k) = F3(huo, c, k);
+                    }
+            for (int k = 1; k <= kmax; k++)
+                for (long jj = 0; jj < NY - 1; jj++)
+                    for (long ii = 0; ii < NX; ii++) {
+                        const long c = ii + NX * jj;
+                        const double HH = dmax2(ssvo[c] + HV[c], s->min_depth);
+                        F3(hvo, c, k) = HH * 0.5 *
+                                        (G3(c, k) - G3(c, k - 1) + G3(c + NX, k) -
+                                         G3(c + NX, k - 1));
+                        F3(hvn, c, k) = F3(hvo, c, k);
+                    }
+            s->first = 0;
+        }
+        /* relaxation towards the target layer heights, :146-210 */
+        for (long c = 0; c < NXY; c++) {
+            const double r = cord_relax / dt * H[c] / maxdepth;
+            const double HH = ssen[c] + H[c];
+            if (HH < s->d_gamma) {
+                for (int k = 1; k <= kmax; k++) {
+                    F3(ho, c, k) = F3(hn, c, k);
+                    F3(hn, c, k) = HH / kmax;
+                }
+            } else {
+                double zz = -H[c];
+                for (int k = 1; k <= kmax - 1; k++) {
+                    F3(ho, c, k) = F3(hn, c, k);
+                    F3(hn, c, k) = (F3(ho, c, k) * r + HH * (G3(c, k) - G3(c, k - 1))) /
+                                   (r + 1.0);
+                    zz = zz + F3(hn, c, k);
+                }
+                F3(ho, c, kmax) = F3(hn, c, kmax);
+                F3(hn, c, kmax) = ssen[c] - zz;
+            }
+        }
+        for (long jj = 0; jj < NY; jj++)
+            for (long ii = 0; ii < NX - 1; ii++) {
+                const long c = ii + NX * jj;
+                const double r = cord_relax / dt * HU[c] / maxdepth;
+                double zz = -HU[c];
+                const double HH = ssun[c] + HU[c];
+                for (int k = 1; k <= kmax - 1; k++) {
+                    F3(huo, c, k) = F3(hun, c, k);
+                    F3(hun, c, k) = (F3(huo, c, k) * r +
+                                     HH * 0.5 *
+                                         (G3(c, k) - G3(c, k - 1) + G3(c + 1, k) -
+                                          G3(c + 1, k - 1))) /
+                                    (r + 1.0);
+                    zz = zz + F3(hun, c, k);
+                }
+                F3(huo, c, kmax) = F3(hun, c, kmax);
+                F3(hun, c, kmax) = ssun[c] - zz;
+            }
+        for (long jj = 0; jj < NY - 1; jj++)
+            for (long ii = 0; ii < NX; ii++) {
+                const long c = ii + NX * jj;
+                const double r = cord_relax / dt * HV[c] / maxdepth;
+                double zz = -HV[c];
+                const double HH = ssvn[c] + HV[c];
+                for (int k = 1; k <= kmax - 1; k++) {
+                    F3(hvo, c, k) = F3(hvn, c, k);
+                    F3(hvn, c, k) = (F3(hvo, c, k) * r +
+                                     HH * 0.5 *
+                                         (G3(c, k) - G3(c, k - 1) + G3(c + NX, k) -
+                                          G3(c + NX, k - 1))) /
+                                    (r + 1.0);
+                    zz = zz + F3(hvn, c, k);
+                }
+                F3(hvo, c, kmax) = F3(hvn, c, kmax);
+                F3(hvn, c, kmax) = ssvn[c] - zz;
+            }
+    } else if (s->vert_cord != 2) {
+        return 2;
+    }
+    s->first = 0;
+    if (zc) {
+        for (long c = 0; c < NXY; c++) {
+            F3(zc, c, 0) = -H[c];
+            F3(zc, c, 1) = -H[c] + 0.5 * F3(hn, c, 1);
+            for (int k = 2; k <= kmax; k++)
+                F3(zc, c, k) = F3(zc, c, k - 1) + 0.5 * (F3(hn, c, k - 1) + F3(hn, c, k));
+        }
+    }
+#undef G3
+#undef F3
+    return 0;
+}
+
+/* ww_momentum_3d, ww_momentum_3d.F90:66-119 (Cartesian, kmin = 1): the
+ * grid-related vertical velocity from the layer continuity equation, upward
+ * from ww(0) (kept), k = 1..kmax-1 on interior water columns.  The serial
+ * 3-D halo update that follows is a no-op. */
+void or_ww_momentum_3d(const or_dom *d, const int32_t *az, double dt,
+                       const double *uu, const double *vv, const double *ho,
+                       const double *hn, double *ww)
+{
+    DOMVARS(d);
+    const double dtm1 = 1.0 / dt, dx = d->dx, dy = d->dy, ard1 = d->ard1;
+    for (int k = 1; k <= kmax - 1; k++)
+        for (int j = jl; j <= jh; j++)
+            for (int i = il; i <= ih; i++) {
+                if (az[IJ(i, j)] != 1) continue;
+                const long c = IJK(i, j, k);
+                ww[c] = ww[c - NXY] - (hn[c] - ho[c]) * dtm1 -
+                        (uu[c] * dy - uu[IJK(i - 1, j, k)] * dy + vv[c] * dx -
+                         vv[IJK(i, j - 1, k)] * dx) *
+                            ard1;
+            }
 }

@@ -108,6 +108,20 @@ int or_do_advection_3d(const or_dom *d, const or_advgrids *g, double dt,
 void or_eos_linear(const or_dom *d, const double *T, const double *S,
                    double *rho, double *buoy, double T0, double S0,
                    double dtr0, double dsr0);
+/* full EOS, eqstate.F90:280-562: UNESCO 1980 one-atmosphere density,
+ * Jackett et al. (2006) density at 0 and p dbar, and its derivatives wrt
+ * salinity and potential temperature */
+double or_rho_unesco80(double T, double S, int nonnegsalt);
+void or_rho_from_theta(double s, double th, double p, double *dens0,
+                       double *densp);
+void or_eosall_from_theta(double s, double th, double p, double *rho_s,
+                          double *rho_th);
+/* do_eqstate, eqstate.F90:94-271 (method 1 linear, 2 UNESCO, 3 Jackett);
+ * kmin = 1; alpha/beta may be NULL; returns 1 for a bad method */
+int or_do_eqstate(const or_dom *d, const int32_t *az, int method,
+                  int nonnegsalt, const double *T, const double *S,
+                  const double *hn, double *rho, double *buoy, double *alpha,
+                  double *beta, double T0, double S0, double dtr0, double dsr0);
 /* a9: ip_blumberg_mellor.F90:8-152; zz is the module array, in/out */
 void or_ip_blumberg_mellor(const or_dom *d, const int32_t *az,
                            const int32_t *au, const int32_t *av,
@@ -143,6 +157,32 @@ void or_decomp_1d(int n, int np, int m, int *s, int *e);
  * h*f*area (mass) and h*area (volume), error-free (double-double). */
 void or_budget(const or_dom *d, const int32_t *az, const double *h,
                const double *f, double *mass, double *volume);
+
+/* input producers (m3d.F90:446-548), Cartesian, kmin = 1 */
+/* start_macro, start_macro.F90:54-86 */
+void or_start_macro(const or_dom *d, int M, double min_depth, const double *H,
+                    const double *HU, const double *HV, const double *z,
+                    double *sseo, double *ssen, double *ssuo, double *ssun,
+                    double *ssvo, double *ssvn, double *Dn, double *Dun,
+                    double *Dvn, double *Uint, double *Vint);
+/* the SAVE'd state of coordinates() (vert_cord 1 sigma, 2 z-level, 3
+ * general; the namelist values ddu, ddl, d_gamma, gamma_surf, min_depth) */
+void *or_coord_new(const or_dom *d, int vert_cord, double ddu, double ddl,
+                   double d_gamma, int gamma_surf, double min_depth);
+void or_coord_delete(void *state);
+/* coordinates, coordinates.F90:66-107 with sigma_coordinates.F90:53-147 and
+ * general_coordinates.F90:56-210; zc may be NULL; returns 0, 1 (general
+ * coordinates not monotone) or 2 (vert_cord not 1, 2 or 3) */
+int or_coordinates(const or_dom *d, void *state, double dt, double cord_relax,
+                   double maxdepth, const double *H, const double *HU,
+                   const double *HV, const double *sseo, const double *ssen,
+                   const double *ssuo, const double *ssun, const double *ssvo,
+                   const double *ssvn, double *ho, double *hn, double *huo,
+                   double *hun, double *hvo, double *hvn, double *zc);
+/* ww_momentum_3d, ww_momentum_3d.F90:66-119 */
+void or_ww_momentum_3d(const or_dom *d, const int32_t *az, double dt,
+                       const double *uu, const double *vv, const double *ho,
+                       const double *hn, double *ww);
 
 /* OpenMP thread count used by the restatement (0 = runtime default) */
 void or_set_threads(int n);

@@ -63,6 +63,10 @@ def lib():
             "or_do_advection_3d": (i, [D, G, d, vp, vp, vp, vp, vp, vp, vp, vp, i, i, i, d, i,
                                        i, vp, vp, C.POINTER(C.c_long)]),
             "or_eos_linear": (None, [D, vp, vp, vp, vp, d, d, d, d]),
+            "or_rho_unesco80": (d, [d, d, i]),
+            "or_rho_from_theta": (None, [d, d, d, C.POINTER(d), C.POINTER(d)]),
+            "or_eosall_from_theta": (None, [d, d, d, C.POINTER(d), C.POINTER(d)]),
+            "or_do_eqstate": (i, [D, vp, i, i, vp, vp, vp, vp, vp, vp, vp, d, d, d, d]),
             "or_ip_blumberg_mellor": (None, [D, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
             "or_ip_shchepetkin_mcwilliams": (None, [D, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                                     vp, vp]),
@@ -71,6 +75,11 @@ def lib():
             "or_exchange_tiles": (None, [i, i, vp, vp, i, vp]),
             "or_decomp_1d": (None, [i, i, i, C.POINTER(i), C.POINTER(i)]),
             "or_budget": (None, [D, vp, vp, vp, C.POINTER(d), C.POINTER(d)]),
+            "or_start_macro": (None, [D, i, d] + [vp] * 15),
+            "or_coord_new": (vp, [D, i, d, d, d, i, d]),
+            "or_coord_delete": (None, [vp]),
+            "or_coordinates": (i, [D, vp, d, d, d] + [vp] * 16),
+            "or_ww_momentum_3d": (None, [D, vp, d, vp, vp, vp, vp, vp]),
             "or_set_threads": (None, [i]),
             "or_get_threads": (i, []),
         }
@@ -91,6 +100,24 @@ def P(a):
 
 def reconstruct(scheme, cfl, fuu, fu, fd):
     return lib().or_adv_interfacial_reconstruction(scheme, cfl, fuu, fu, fd)
+
+
+def rho_unesco80(T, S, nonnegsalt=False):
+    return lib().or_rho_unesco80(T, S, int(nonnegsalt))
+
+
+def rho_from_theta(s, th, p):
+    """(dens0, densp) of eqstate.F90:338-423"""
+    a, b = C.c_double(), C.c_double()
+    lib().or_rho_from_theta(s, th, p, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def eosall_from_theta(s, th, p):
+    """(rho_s, rho_th) of eqstate.F90:432-562"""
+    a, b = C.c_double(), C.c_double()
+    lib().or_eosall_from_theta(s, th, p, C.byref(a), C.byref(b))
+    return a.value, b.value
 
 
 def set_threads(n):
@@ -135,6 +162,8 @@ class Oracle:
     def __del__(self):
         try:
             lib().or_free_advection(C.byref(self.g))
+            if getattr(self, "_coord", None):
+                lib().or_coord_delete(self._coord)
         except Exception:
             pass
 
@@ -179,6 +208,41 @@ class Oracle:
 
     def eos_linear(self, T, S, rho, buoy, T0, S0, dtr0, dsr0):
         lib().or_eos_linear(C.byref(self.dom), P(T), P(S), P(rho), P(buoy), T0, S0, dtr0, dsr0)
+
+    def do_eqstate(self, method, T, S, hn, rho, buoy, alpha=None, beta=None, T0=10.0,
+                   S0=33.75, dtr0=-0.17, dsr0=0.78, nonnegsalt=False):
+        """do_eqstate (eqstate.F90:94-271); the defaults are the reference's
+        namelist defaults (FP32 literals, rounded here)."""
+        f32 = lambda v: float(np.float32(v))
+        return lib().or_do_eqstate(C.byref(self.dom), P(self.az), method, int(nonnegsalt),
+                                   P(T), P(S), P(hn), P(rho), P(buoy), P(alpha), P(beta),
+                                   f32(T0), f32(S0), f32(dtr0), f32(dsr0))
+
+    # ---- input producers (m3d.F90:446-548) ----
+    def start_macro(self, M, min_depth, H, HU, HV, z, sseo, ssen, ssuo, ssun, ssvo, ssvn,
+                    Dn, Dun, Dvn, Uint, Vint):
+        lib().or_start_macro(C.byref(self.dom), M, min_depth,
+                             *(P(a) for a in (H, HU, HV, z, sseo, ssen, ssuo, ssun, ssvo, ssvn,
+                                              Dn, Dun, Dvn, Uint, Vint)))
+
+    def coordinates(self, dt, cord_relax, maxdepth, H, HU, HV, sseo, ssen, ssuo, ssun, ssvo,
+                    ssvn, ho, hn, huo, hun, hvo, hvn, zc=None):
+        """coordinates(); the first call initialises (set_coordinates
+        chooses the kind and its namelist values)"""
+        return lib().or_coordinates(C.byref(self.dom), self._coord, dt, cord_relax, maxdepth,
+                                    *(P(a) for a in (H, HU, HV, sseo, ssen, ssuo, ssun, ssvo,
+                                                     ssvn, ho, hn, huo, hun, hvo, hvn, zc)))
+
+    def set_coordinates(self, vert_cord, ddu=0.0, ddl=0.0, d_gamma=20.0, gamma_surf=True,
+                        min_depth=0.5):
+        if getattr(self, "_coord", None):
+            lib().or_coord_delete(self._coord)
+        self._coord = lib().or_coord_new(C.byref(self.dom), vert_cord, ddu, ddl, d_gamma,
+                                         int(gamma_surf), min_depth)
+
+    def ww_momentum_3d(self, dt, uu, vv, ho, hn, ww):
+        lib().or_ww_momentum_3d(C.byref(self.dom), P(self.az), dt, P(uu), P(vv), P(ho), P(hn),
+                                P(ww))
 
     def ip_blumberg_mellor(self, H, hn, hun, hvn, buoy, zz, idpdx, idpdy):
         lib().or_ip_blumberg_mellor(C.byref(self.dom), P(self.az), P(self.au), P(self.av),
