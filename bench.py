@@ -92,6 +92,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-batched", action="store_true")
     p.add_argument("--no-adjoint", action="store_true")
+    p.add_argument("--no-ext", action="store_true",
+                   help="skip the full-EOS and input-producer timings")
     p.add_argument("--serial", action="store_true",
                    help="run EOS + pressure and the advection on one stream")
     p.add_argument("--shard", action="store_true",
@@ -300,6 +302,9 @@ def main():
         out["batched_fp32"] = batched(torch, np, gt, inputs, dev, a, ncell * ntr / adv_s)
     if a.shard:
         out["shard"] = shard_section(torch, np, gt, inputs, dev, a.batched_steps)
+    if not a.no_ext:
+        out["extensions"] = extensions_section(torch, np, gt, inputs, dev, tr, ctx, st,
+                                               a.steps)
     if not a.no_adjoint:
         out["adjoint"] = adjoint_section(torch, np, gt, inputs, dev)
     if not a.no_cpu:
@@ -421,6 +426,71 @@ def batched(torch, np, gt, inputs, dev, a, fp64_rate):
         "err_vs_fp64": {"max": emax, "rms": erms, "steps": nsteps + 2,
                         "norm": "|f32-f64| / max|f64| over wet cells, worst tracer"},
         "mass_drift_max_rel": drift,
+    }
+
+
+def extensions_section(torch, np, gt, inputs, dev, tr, ctx, st, steps):
+    """The callers either side of the step on the headline domain, each alone
+    on one stream with HIP events: the full equation of state (Jackett et al.
+    2006 with the interface expansion coefficients) and the input producers
+    (start_macro, general coordinates, ww_momentum_3d).  Algorithmic bytes
+    per cell over the whole (halo-included) arrays: EOS read T S hn, write
+    rho buoy alpha beta = 56 B; coordinates read hn hun hvn gga, write ho hn
+    huo hun hvo hvn zc = 88 B; ww read uu vv ho hn, write ww = 40 B."""
+    d = tr.dom
+    n3 = d.NX * d.NY * d.kmax
+    D = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)  # noqa: E731
+    T, S, hn = D(tr.tracers["T"]), D(tr.tracers["S"]), D(tr.hn)
+    rho, buoy, alpha, beta = (torch.zeros_like(T) for _ in range(4))
+    t_eos = EventTimer(torch, st)
+    H, HU, HV = D(d.H), D(d.HU), D(d.HV)
+    sh2 = (d.NY, d.NX)
+    e2 = {n: torch.zeros(sh2, dtype=torch.float64, device=dev)
+          for n in ("sseo", "ssen", "ssuo", "ssun", "ssvo", "ssvn", "Dn", "Dun", "Dvn", "Uint",
+                    "Vint")}
+    h3 = {n: torch.zeros_like(T) for n in ("ho", "hn", "huo", "hun", "hvo", "hvn", "zc")}
+    z = D(0.2 * inputs.uniform(sh2, 9, -1, 1) * (d.az > 0))
+    uu, vv, ww = D(tr.uu), D(tr.vv), torch.zeros_like(T)
+    ctx.set_coordinates(3, ddu=1.0, ddl=1.0, d_gamma=20.0, gamma_surf=True, min_depth=0.5)
+    t_sm, t_co, t_ww = (EventTimer(torch, st) for _ in range(3))
+    names2 = ("sseo", "ssen", "ssuo", "ssun", "ssvo", "ssvn", "Dn", "Dun", "Dvn", "Uint", "Vint")
+    names_e = ("sseo", "ssen", "ssuo", "ssun", "ssvo", "ssvn")
+    names_h = ("ho", "hn", "huo", "hun", "hvo", "hvn", "zc")
+
+    def producers(timed):
+        for tm, fn in ((t_sm, lambda: ctx.start_macro(30, 0.5, H, HU, HV, z,
+                                                       *(e2[n] for n in names2))),
+                       (t_co, lambda: ctx.coordinates(tr.dt, 3600.0, 50.0, H, HU, HV,
+                                                      *(e2[n] for n in names_e),
+                                                      *(h3[n] for n in names_h))),
+                       (t_ww, lambda: ctx.ww_momentum_3d(tr.dt, uu, vv, h3["ho"], h3["hn"],
+                                                         ww))):
+            with (tm if timed else _Null()):
+                fn()
+
+    with torch.cuda.stream(st):
+        for _ in range(3):
+            ctx.do_eqstate(3, T, S, hn, rho, buoy, alpha, beta)
+            producers(False)  # the first coordinates call initialises
+        st.synchronize()
+        for _ in range(steps):
+            with t_eos:
+                ctx.do_eqstate(3, T, S, hn, rho, buoy, alpha, beta)
+            producers(True)
+        st.synchronize()
+    e_s, c_s, w_s, m_s = t_eos.mean(), t_co.mean(), t_ww.mean(), t_sm.mean()
+    gb = lambda b, s: b / s / 1e9  # noqa: E731
+    return {
+        "workload": f"{d.imax}x{d.jmax}x{d.kmax} (arrays {d.NX}x{d.NY}x{d.kmax + 1})",
+        "eqstate_jackett": {"kernel": "k_eqstate_column<3> (rho, buoy, alpha, beta)",
+                            "ms": e_s * 1e3, "bytes_per_cell": 56, "GBps": gb(56 * n3, e_s),
+                            "frac": 56 * n3 / e_s / HBM_PEAK},
+        "coordinates_general": {"kernel": "k_gvc_step (ho hn huo hun hvo hvn zc)",
+                                "ms": c_s * 1e3, "bytes_per_cell": 88,
+                                "GBps": gb(88 * n3, c_s), "frac": 88 * n3 / c_s / HBM_PEAK},
+        "ww_momentum_3d": {"kernel": "k_ww_momentum", "ms": w_s * 1e3, "bytes_per_cell": 40,
+                           "GBps": gb(40 * n3, w_s), "frac": 40 * n3 / w_s / HBM_PEAK},
+        "start_macro_ms": m_s * 1e3,
     }
 
 

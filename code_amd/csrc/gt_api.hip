@@ -8,6 +8,7 @@
 // allocated on first use and kept, so a time loop allocates nothing.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -66,6 +67,12 @@ struct gt_ctx {
   int group = 3;        // tracers per wavefront of the FP32 batched kernels
   void* ws = nullptr;   // work fields of the strip kernels (grown on demand)
   size_t ws_bytes = 0;
+  // coordinates(): namelist values (domain module) and the SAVE'd state
+  int vert_cord = 0, gamma_surf = 1, coord_first = 1, equiv_sigma = 0;
+  double ddu = 0.0, ddl = 0.0, d_gamma = 20.0, min_depth = 0.5;
+  double* dlev = nullptr;  // device: dga (sigma) or sig | be (general), kmax+1 each
+  double* gga = nullptr;   // device: gamma levels of the general coordinates, 3-D
+  int* dbad = nullptr;     // device: set when the gamma levels fold over
 };
 
 extern "C" const char* gt_last_error(void) { return g_err; }
@@ -211,8 +218,9 @@ extern "C" int gt_create(gt_ctx** out, int imin, int imax, int jmin, int jmax, i
 
 extern "C" int gt_destroy(gt_ctx* c) {
   if (!c) return GT_OK;
-  void* ps[] = {c->dmask, c->dint, c->fi, c->hi, c->adv, c->tmp, c->scr7,
-                c->zz, c->P, c->slice3, c->bpart, c->bout, c->nwarn, c->ws};
+  void* ps[] = {c->dmask, c->dint, c->fi, c->hi, c->adv, c->tmp, c->scr7, c->zz,
+                c->P, c->slice3, c->bpart, c->bout, c->nwarn, c->ws, c->dlev, c->gga,
+                c->dbad};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   for (double* p : c->adj)
@@ -666,6 +674,140 @@ extern "C" int gt_do_advection_3d(gt_ctx* c, double dt, double* f, const double*
     }
   }
   return set_err(GT_EINVAL_SPLIT, "do_advection_3d: split is invalid");
+}
+
+// ---- input producers (m3d.F90:446-548) ---------------------------------------
+extern "C" int gt_start_macro(gt_ctx* c, int M, double min_depth, const double* H,
+                              const double* HU, const double* HV, const double* z, double* sseo,
+                              double* ssen, double* ssuo, double* ssun, double* ssvo,
+                              double* ssvn, double* Dn, double* Dun, double* Dvn, double* Uint,
+                              double* Vint) {
+  const void* ps[] = {H, HU, HV, z, sseo, ssen, ssuo, ssun, ssvo, ssvn, Dn, Dun, Dvn, Uint, Vint};
+  if (!c) return set_err(GT_EARG, "gt_start_macro: NULL context");
+  for (const void* p : ps)
+    if (!p) return set_err(GT_EARG, "gt_start_macro: NULL field");
+  if (M < 1) return set_err(GT_EARG, "gt_start_macro: M must be >= 1");
+  const double split = 1.0 / (double)(float)M;  // _ONE_/float(M), start_macro.F90:84
+  LCHK(gt::launch_start_macro(c->st, c->d, split, min_depth, H, HU, HV, z, sseo, ssen, ssuo,
+                              ssun, ssvo, ssvn, Dn, Dun, Dvn, Uint, Vint));
+  return GT_OK;
+}
+
+extern "C" int gt_set_coordinates(gt_ctx* c, int vert_cord, double ddu, double ddl,
+                                  double d_gamma, int gamma_surf, double min_depth) {
+  if (!c) return set_err(GT_EARG, "gt_set_coordinates: NULL context");
+  if (vert_cord == 4)
+    return set_err(GT_ENOTIMPL, "coordinates(): hybrid_coordinates not coded yet");
+  if (vert_cord < 1 || vert_cord > 3)
+    return set_err(GT_ENOTIMPL, "gt_set_coordinates: vert_cord %d is not in this build",
+                   vert_cord);
+  c->vert_cord = vert_cord;
+  c->ddu = ddu;
+  c->ddl = ddl;
+  c->d_gamma = d_gamma;
+  c->gamma_surf = gamma_surf != 0;
+  c->min_depth = min_depth;
+  c->coord_first = 1;
+  c->equiv_sigma = 0;
+  return GT_OK;
+}
+
+// the surface/bottom zoom, sigma_coordinates.F90:72-76, general_coordinates.F90:72-75
+static double zoom_level(double ddu, double ddl, int k, int kmax) {
+  double g = tanh((ddl + ddu) * k / (double)(float)kmax - ddl) + tanh(ddl);
+  return g / (tanh(ddl) + tanh(ddu)) - 1.0;
+}
+
+// the first call of sigma_coordinates / general_coordinates: the level
+// arrays on the host, as the reference computes them, then on the device
+static int coord_levels(gt_ctx* c) {
+  const int kmax = c->kmax, K1 = kmax + 1;
+  std::vector<double> lev(2 * (size_t)K1, 0.0);
+  if (c->vert_cord == 1) {
+    if (c->ddu <= 0.0 && c->ddl <= 0.0) {
+      c->equiv_sigma = 1;
+      return GT_OK;
+    }
+    if (c->ddu < 0.0) c->ddu = 0.0;
+    if (c->ddl < 0.0) c->ddl = 0.0;
+    double gprev = -1.0;  // ga(0)
+    lev[0] = 0.0;         // dga(0)
+    for (int k = 1; k <= kmax; k++) {
+      const double g = zoom_level(c->ddu, c->ddl, k, kmax);
+      lev[k] = g - gprev;
+      gprev = g;
+    }
+  } else {
+    double* sig = lev.data();
+    double* be = lev.data() + K1;
+    be[0] = -1.0;
+    sig[0] = -1.0;
+    if (c->ddu < 0.0) c->ddu = 0.0;
+    if (c->ddl < 0.0) c->ddl = 0.0;
+    for (int k = 1; k <= kmax; k++) {
+      be[k] = zoom_level(c->ddu, c->ddl, k, kmax);
+      sig[k] = (double)((float)k / (float)kmax) - 1.0;  // k/float(kmax): an FP32 quotient
+    }
+  }
+  ENSURE(c->dlev, 2 * (long)K1);
+  HIPCHK(hipMemcpy(c->dlev, lev.data(), sizeof(double) * lev.size(), hipMemcpyHostToDevice));
+  return GT_OK;
+}
+
+extern "C" int gt_coordinates(gt_ctx* c, double dt, double cord_relax, double maxdepth,
+                              const double* H, const double* HU, const double* HV,
+                              const double* sseo, const double* ssen, const double* ssuo,
+                              const double* ssun, const double* ssvo, const double* ssvn,
+                              double* ho, double* hn, double* huo, double* hun, double* hvo,
+                              double* hvn, double* zc) {
+  if (!c) return set_err(GT_EARG, "gt_coordinates: NULL context");
+  if (!c->vert_cord) return set_err(GT_EARG, "gt_coordinates: call gt_set_coordinates first");
+  if (!H || !hn) return set_err(GT_EARG, "gt_coordinates: NULL field");
+  if (c->vert_cord == 2) {  // z-levels: nothing but zc, coordinates.F90:71,90,103-107
+    c->coord_first = 0;
+    if (zc) LCHK(gt::launch_zc(c->st, c->d, H, hn, zc));
+    return GT_OK;
+  }
+  const void* ps[] = {HU, HV, sseo, ssen, ssuo, ssun, ssvo, ssvn, ho, huo, hun, hvo, hvn};
+  for (const void* p : ps)
+    if (!p) return set_err(GT_EARG, "gt_coordinates: NULL field");
+  if (c->vert_cord == 3 && (dt == 0.0 || maxdepth == 0.0))
+    return set_err(GT_EARG, "gt_coordinates: dt and maxdepth must be nonzero");
+  if (c->coord_first) TRY(coord_levels(c));
+  if (c->vert_cord == 1) {
+    c->coord_first = 0;
+    LCHK(gt::launch_sigma(c->st, c->d, c->equiv_sigma != 0, 1.0 / (double)(float)c->kmax,
+                          c->dlev, H, HU, HV, sseo, ssen, ssuo, ssun, ssvo, ssvn, ho, hn, huo,
+                          hun, hvo, hvn, zc));
+    return GT_OK;
+  }
+  if (c->coord_first) {
+    ENSURE(c->gga, c->N3);
+    if (!c->dbad) HIPCHK(hipMalloc(&c->dbad, sizeof(int)));
+    HIPCHK(hipMemsetAsync(c->dbad, 0, sizeof(int), c->st));
+    const int kk = c->gamma_surf ? c->kmax : 1;
+    LCHK(gt::launch_gvc_init(c->st, c->d, kk, c->d_gamma, c->min_depth, c->dlev, H, HU, HV, sseo,
+                             ssuo, ssvo, c->gga, hn, huo, hun, hvo, hvn, c->dbad));
+    int bad = 0;
+    HIPCHK(hipMemcpyAsync(&bad, c->dbad, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (bad)
+      return set_err(GT_EARG, "coordinates: general vertical coordinates not monotone "
+                              "(general_coordinates.F90:92-98)");
+    c->coord_first = 0;
+  }
+  LCHK(gt::launch_gvc_step(c->st, c->d, cord_relax / dt, maxdepth, c->d_gamma, c->gga, H, HU, HV,
+                           ssen, ssun, ssvn, ho, hn, huo, hun, hvo, hvn, zc));
+  return GT_OK;
+}
+
+extern "C" int gt_ww_momentum_3d(gt_ctx* c, double dt, const double* uu, const double* vv,
+                                 const double* ho, const double* hn, double* ww) {
+  if (!c || !uu || !vv || !ho || !hn || !ww)
+    return set_err(GT_EARG, "gt_ww_momentum_3d: NULL argument");
+  LCHK(gt::launch_ww_momentum(c->st, c->d, dt, c->dx, c->dy, c->ard1, c->az, uu, vv, ho, hn,
+                              ww));
+  return GT_OK;
 }
 
 // ---- internal pressure -------------------------------------------------------

@@ -51,6 +51,31 @@ hipError_t launch_eqstate(hipStream_t st, Dims d, int method, bool nonnegsalt,
 // dens0, o2 = densp; 2: eosall_from_theta -> o1 = rho_s, o2 = rho_th
 hipError_t launch_eos_points(hipStream_t st, int fn, long n, const double* s, const double* th,
                              const double* p, double* o1, double* o2, bool nonnegsalt);
+// input producers (gt_inputs.hip)
+hipError_t launch_start_macro(hipStream_t st, Dims d, double split, double min_depth,
+                              const double* H, const double* HU, const double* HV,
+                              const double* z, double* sseo, double* ssen, double* ssuo,
+                              double* ssun, double* ssvo, double* ssvn, double* Dn, double* Dun,
+                              double* Dvn, double* Uint, double* Vint);
+hipError_t launch_sigma(hipStream_t st, Dims d, bool equiv, double kmaxm1, const double* dga,
+                        const double* H, const double* HU, const double* HV, const double* sseo,
+                        const double* ssen, const double* ssuo, const double* ssun,
+                        const double* ssvo, const double* ssvn, double* ho, double* hn,
+                        double* huo, double* hun, double* hvo, double* hvn, double* zc);
+hipError_t launch_gvc_init(hipStream_t st, Dims d, int kk, double d_gamma, double min_depth,
+                           const double* lev, const double* H, const double* HU,
+                           const double* HV, const double* sseo, const double* ssuo,
+                           const double* ssvo, double* gga, double* hn, double* huo,
+                           double* hun, double* hvo, double* hvn, int* bad);
+hipError_t launch_gvc_step(hipStream_t st, Dims d, double relax_dt, double maxdepth,
+                           double d_gamma, const double* gga, const double* H, const double* HU,
+                           const double* HV, const double* ssen, const double* ssun,
+                           const double* ssvn, double* ho, double* hn, double* huo, double* hun,
+                           double* hvo, double* hvn, double* zc);
+hipError_t launch_zc(hipStream_t st, Dims d, const double* H, const double* hn, double* zc);
+hipError_t launch_ww_momentum(hipStream_t st, Dims d, double dt, double dx, double dy,
+                              double ard1, const int32_t* az, const double* uu, const double* vv,
+                              const double* ho, const double* hn, double* ww);
 hipError_t launch_ip_bm(hipStream_t st, Dims d, const int32_t* az, const int32_t* au,
                         const int32_t* av, const double* H, const double* hn,
                         const double* hun, const double* hvn, const double* buoy, double* zz,

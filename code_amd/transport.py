@@ -43,6 +43,7 @@ EXPORTS = [
     "gt_set_tuning", "gt_budget_f32", "gt_adv3d_adjoint",
     "gt_ip_blumberg_mellor_adjoint", "gt_eos_linear_adjoint", "gt_autotune_batch",
     "gt_do_eqstate", "gt_rho_from_theta_unesco80", "gt_rho_from_theta", "gt_eosall_from_theta",
+    "gt_start_macro", "gt_set_coordinates", "gt_coordinates", "gt_ww_momentum_3d",
 ]
 
 
@@ -97,6 +98,10 @@ def load_library():
                                    vp, vp]),
         "gt_eos_linear": (i, [vp, vp, vp, vp, vp, d, d, d, d]),
         "gt_do_eqstate": (i, [vp, i, i, vp, vp, vp, vp, vp, vp, vp, d, d, d, d]),
+        "gt_start_macro": (i, [vp, i, d] + [vp] * 15),
+        "gt_set_coordinates": (i, [vp, i, d, d, d, i, d]),
+        "gt_coordinates": (i, [vp, d, d, d] + [vp] * 16),
+        "gt_ww_momentum_3d": (i, [vp, d, vp, vp, vp, vp, vp]),
         "gt_rho_from_theta_unesco80": (i, [vp, C.c_long, vp, vp, vp, i]),
         "gt_rho_from_theta": (i, [vp, C.c_long, vp, vp, vp, vp, vp]),
         "gt_eosall_from_theta": (i, [vp, C.c_long, vp, vp, vp, vp, vp]),
@@ -254,6 +259,29 @@ class Context:
         check(self.lib.gt_do_eqstate(self.h, method, int(nonnegsalt), _ptr(T), _ptr(S),
                                      _ptr(hn), _ptr(rho), _ptr(buoy), _ptr(alpha), _ptr(beta),
                                      f32(T0), f32(S0), f32(dtr0), f32(dsr0)))
+
+    # ---- input producers (m3d.F90:446-548) ----------------------------------
+    def start_macro(self, M, min_depth, H, HU, HV, z, sseo, ssen, ssuo, ssun, ssvo, ssvn, Dn,
+                    Dun, Dvn, Uint, Vint):
+        check(self.lib.gt_start_macro(self.h, int(M), min_depth,
+                                      *(_ptr(a) for a in (H, HU, HV, z, sseo, ssen, ssuo, ssun,
+                                                          ssvo, ssvn, Dn, Dun, Dvn, Uint, Vint))))
+
+    def set_coordinates(self, vert_cord, ddu=0.0, ddl=0.0, d_gamma=20.0, gamma_surf=True,
+                        min_depth=0.5):
+        check(self.lib.gt_set_coordinates(self.h, int(vert_cord), ddu, ddl, d_gamma,
+                                          int(gamma_surf), min_depth))
+
+    def coordinates(self, dt, cord_relax, maxdepth, H, HU, HV, sseo, ssen, ssuo, ssun, ssvo,
+                    ssvn, ho, hn, huo, hun, hvo, hvn, zc=None):
+        check(self.lib.gt_coordinates(self.h, dt, cord_relax, maxdepth,
+                                      *(_ptr(a) for a in (H, HU, HV, sseo, ssen, ssuo, ssun,
+                                                          ssvo, ssvn, ho, hn, huo, hun, hvo, hvn,
+                                                          zc))))
+
+    def ww_momentum_3d(self, dt, uu, vv, ho, hn, ww):
+        check(self.lib.gt_ww_momentum_3d(self.h, dt, _ptr(uu), _ptr(vv), _ptr(ho), _ptr(hn),
+                                         _ptr(ww)))
 
     def rho_from_theta_unesco80(self, T, S, rho, nonnegsalt=False):
         check(self.lib.gt_rho_from_theta_unesco80(self.h, T.numel(), _ptr(T), _ptr(S),

@@ -119,6 +119,46 @@ int gt_do_advection_3d(gt_ctx *ctx, double dt, double *f, const double *uu,
                        int split, int hscheme, int vscheme, double AH, int tag,
                        double *hires, double *advres);
 
+/* ---- input producers: what integrate_3d runs before the tracers
+ * (m3d.F90:446-548).  Cartesian metrics (dx, dy of the context); kmin = 1.
+ * 2-D fields are E2DFIELD, 3-D fields I3DFIELD, device pointers. */
+/* start_macro, src/3d/start_macro.F90:54-86: sseo = ssen, ssen = z,
+ * Dn = ssen + H; the U/V-point elevations are the means of the old and new
+ * elevations clipped at -HU + min_depth (-HV + min_depth), ssuo/ssvo the
+ * previous ones, Dun/Dvn the depths; Uint and Vint are divided by M. */
+int gt_start_macro(gt_ctx *ctx, int M, double min_depth, const double *H,
+                   const double *HU, const double *HV, const double *z,
+                   double *sseo, double *ssen, double *ssuo, double *ssun,
+                   double *ssvo, double *ssvn, double *Dn, double *Dun,
+                   double *Dvn, double *Uint, double *Vint);
+/* the domain-namelist values coordinates() reads (domain.F90: vert_cord,
+ * ddu, ddl, d_gamma, gamma_surf, min_depth).  vert_cord 1 (sigma), 2
+ * (z-levels: zc only, as the reference, which codes nothing else for them)
+ * or 3 (general); 4 returns GT_ENOTIMPL with the reference's 'hybrid
+ * coordinates not coded yet', 5 (adaptive) is outside this build.  Resets
+ * the first-call state. */
+int gt_set_coordinates(gt_ctx *ctx, int vert_cord, double ddu, double ddl,
+                       double d_gamma, int gamma_surf, double min_depth);
+/* coordinates, src/3d/coordinates.F90:66-107 with
+ * sigma_coordinates.F90:53-147 and general_coordinates.F90:56-210.  The
+ * first call initialises (level arrays; for general coordinates the gamma
+ * levels of every column and the initial hn, huo = hun, hvo = hvn), every
+ * call updates ho, hn, huo, hun, hvo, hvn; zc (may be NULL) gets the layer
+ * centres.  Folded gamma levels return GT_EARG (the reference stops,
+ * general_coordinates.F90:92-98); the first call synchronises. */
+int gt_coordinates(gt_ctx *ctx, double dt, double cord_relax, double maxdepth,
+                   const double *H, const double *HU, const double *HV,
+                   const double *sseo, const double *ssen, const double *ssuo,
+                   const double *ssun, const double *ssvo, const double *ssvn,
+                   double *ho, double *hn, double *huo, double *hun,
+                   double *hvo, double *hvn, double *zc);
+/* ww_momentum_3d, src/3d/ww_momentum_3d.F90:66-119: ww(k) for k =
+ * 1..kmax-1 on interior water columns, upward from ww(0) (read); the serial
+ * 3-D halo update after it is a no-op */
+int gt_ww_momentum_3d(gt_ctx *ctx, double dt, const double *uu,
+                      const double *vv, const double *ho, const double *hn,
+                      double *ww);
+
 /* ---- internal pressure ------------------------------------------------- */
 /* do_eqstate with eqstate_method=1 (linear), src/3d/eqstate.F90:127-149 and
  * the buoyancy of :252-261.  rho may be NULL.  The reference defaults are
