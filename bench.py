@@ -433,10 +433,13 @@ def extensions_section(torch, np, gt, inputs, dev, tr, ctx, st, steps):
     """The callers either side of the step on the headline domain, each alone
     on one stream with HIP events: the full equation of state (Jackett et al.
     2006 with the interface expansion coefficients) and the input producers
-    (start_macro, general coordinates, ww_momentum_3d).  Algorithmic bytes
-    per cell over the whole (halo-included) arrays: EOS read T S hn, write
-    rho buoy alpha beta = 56 B; coordinates read hn hun hvn gga, write ho hn
-    huo hun hvo hvn zc = 88 B; ww read uu vv ho hn, write ww = 40 B."""
+    (start_macro, general coordinates, ww_momentum_3d), and the vertical
+    diffusion after the advection (salinity; temperature with short-wave).
+    Algorithmic bytes per cell over the whole (halo-included) arrays: EOS
+    read T S hn, write rho buoy alpha beta = 56 B; coordinates read hn hun
+    hvn gga, write ho hn huo hun hvo hvn zc = 88 B; ww read uu vv ho hn,
+    write ww = 40 B; salinity diffusion read S hn nuh, write S = 32 B;
+    temperature + rad = 40 B."""
     d = tr.dom
     n3 = d.NX * d.NY * d.kmax
     D = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)  # noqa: E731
@@ -468,17 +471,37 @@ def extensions_section(torch, np, gt, inputs, dev, tr, ctx, st, steps):
             with (tm if timed else _Null()):
                 fn()
 
+    nuh = D(inputs.uniform(T.shape, 3, 1e-5, 5e-2))
+    sh2t = (d.NY, d.NX)
+    swr = D(inputs.uniform(sh2t, 5, 0.0, 600.0))
+    shf = D(inputs.uniform(sh2t, 6, -200.0, 100.0))
+    Aa, g1, g2 = (D(np.full(sh2t, v)) for v in (0.58, 0.35, 23.0))  # Jerlov type I
+    dry = D(np.ones(sh2t))
+    rad = torch.zeros_like(T)
+    Sd, Td = S.clone(), T.clone()
+    t_sd, t_td = EventTimer(torch, st), EventTimer(torch, st)
+
+    def diffusion(timed):
+        with (t_sd if timed else _Null()):
+            ctx.salinity_diffusion(tr.dt, 0.5, hn, nuh, Sd)
+        with (t_td if timed else _Null()):
+            ctx.temperature_diffusion(tr.dt, 0.5, hn, nuh, swr, shf, Aa, g1, g2, dry, Sd, rad,
+                                      Td)
+
     with torch.cuda.stream(st):
         for _ in range(3):
             ctx.do_eqstate(3, T, S, hn, rho, buoy, alpha, beta)
             producers(False)  # the first coordinates call initialises
+            diffusion(False)
         st.synchronize()
         for _ in range(steps):
             with t_eos:
                 ctx.do_eqstate(3, T, S, hn, rho, buoy, alpha, beta)
             producers(True)
+            diffusion(True)
         st.synchronize()
     e_s, c_s, w_s, m_s = t_eos.mean(), t_co.mean(), t_ww.mean(), t_sm.mean()
+    sd_s, td_s = t_sd.mean(), t_td.mean()
     gb = lambda b, s: b / s / 1e9  # noqa: E731
     return {
         "workload": f"{d.imax}x{d.jmax}x{d.kmax} (arrays {d.NX}x{d.NY}x{d.kmax + 1})",
@@ -491,6 +514,13 @@ def extensions_section(torch, np, gt, inputs, dev, tr, ctx, st, steps):
         "ww_momentum_3d": {"kernel": "k_ww_momentum", "ms": w_s * 1e3, "bytes_per_cell": 40,
                            "GBps": gb(40 * n3, w_s), "frac": 40 * n3 / w_s / HBM_PEAK},
         "start_macro_ms": m_s * 1e3,
+        "salinity_diffusion": {"kernel": "k_vertical_diffusion<false>", "ms": sd_s * 1e3,
+                               "bytes_per_cell": 32, "GBps": gb(32 * n3, sd_s),
+                               "frac": 32 * n3 / sd_s / HBM_PEAK},
+        "temperature_diffusion": {"kernel": "k_vertical_diffusion<true> (with short-wave)",
+                                  "ms": td_s * 1e3, "bytes_per_cell": 40,
+                                  "GBps": gb(40 * n3, td_s),
+                                  "frac": 40 * n3 / td_s / HBM_PEAK},
     }
 
 

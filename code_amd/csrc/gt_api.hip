@@ -73,6 +73,9 @@ struct gt_ctx {
   double* dlev = nullptr;  // device: dga (sigma) or sig | be (general), kmax+1 each
   double* gga = nullptr;   // device: gamma levels of the general coordinates, 3-D
   int* dbad = nullptr;     // device: set when the gamma levels fold over
+  double* vdw = nullptr;   // vertical diffusion work: ru, qu, reflected profile (3 fields)
+  double* tri = nullptr;   // batched tridiagonal work (grown on demand)
+  long tri_n = 0;
 };
 
 extern "C" const char* gt_last_error(void) { return g_err; }
@@ -220,7 +223,7 @@ extern "C" int gt_destroy(gt_ctx* c) {
   if (!c) return GT_OK;
   void* ps[] = {c->dmask, c->dint, c->fi, c->hi, c->adv, c->tmp, c->scr7, c->zz,
                 c->P, c->slice3, c->bpart, c->bout, c->nwarn, c->ws, c->dlev, c->gga,
-                c->dbad};
+                c->dbad, c->vdw, c->tri};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   for (double* p : c->adj)
@@ -807,6 +810,58 @@ extern "C" int gt_ww_momentum_3d(gt_ctx* c, double dt, const double* uu, const d
     return set_err(GT_EARG, "gt_ww_momentum_3d: NULL argument");
   LCHK(gt::launch_ww_momentum(c->st, c->d, dt, c->dx, c->dy, c->ard1, c->az, uu, vv, ho, hn,
                               ww));
+  return GT_OK;
+}
+
+// ---- the step after advection: vertical diffusion --------------------------
+extern "C" int gt_salinity_diffusion(gt_ctx* c, double dt, double cnpar, const double* hn,
+                                     const double* nuh, double* S) {
+  if (!c || !hn || !nuh || !S) return set_err(GT_EARG, "gt_salinity_diffusion: NULL argument");
+  if (c->kmax <= 1) return GT_OK;
+  ENSURE(c->vdw, 3 * c->N3);
+  LCHK(gt::launch_vertical_diffusion(c->st, c->d, false, dt, cnpar, c->az, hn, nuh, S, c->vdw,
+                                     nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                     nullptr, -1.0, 0.0, nullptr));
+  return GT_OK;
+}
+
+extern "C" int gt_temperature_diffusion(gt_ctx* c, double dt, double cnpar, const double* hn,
+                                        const double* nuh, const double* swr, const double* shf,
+                                        const double* A, const double* g1, const double* g2,
+                                        const double* dry_z, const double* S,
+                                        double swr_bot_refl_frac, double swr_min_bot_frac,
+                                        double* rad, double* T) {
+  const void* ps[] = {hn, nuh, swr, shf, A, g1, g2, dry_z, S, rad, T};
+  if (!c) return set_err(GT_EARG, "gt_temperature_diffusion: NULL context");
+  for (const void* p : ps)
+    if (!p) return set_err(GT_EARG, "gt_temperature_diffusion: NULL argument");
+  ENSURE(c->vdw, 3 * c->N3);
+  LCHK(gt::launch_vertical_diffusion(c->st, c->d, true, dt, cnpar, c->az, hn, nuh, T, c->vdw,
+                                     swr, shf, A, g1, g2, dry_z, S, swr_bot_refl_frac,
+                                     swr_min_bot_frac, rad));
+  return GT_OK;
+}
+
+extern "C" int gt_tridiagonal_batch(gt_ctx* c, long n, int fi, int lt, const double* au,
+                                    const double* bu, const double* cu, const double* du,
+                                    double* value) {
+  if (!c || n < 0 || fi < 0 || lt <= fi)
+    return set_err(GT_EARG, "gt_tridiagonal_batch: need n >= 0 and 0 <= fi < lt");
+  if (n == 0) return GT_OK;
+  if (!au || !bu || !cu || !du || !value)
+    return set_err(GT_EARG, "gt_tridiagonal_batch: NULL argument");
+  const long need = 2 * n * (long)(lt + 1);
+  if (c->tri_n < need) {
+    if (c->tri) {
+      HIPCHK(hipDeviceSynchronize());  // the old work may still be in use
+      HIPCHK(hipFree(c->tri));
+      c->tri = nullptr;
+      c->tri_n = 0;
+    }
+    HIPCHK(hipMalloc(&c->tri, sizeof(double) * (size_t)need));
+    c->tri_n = need;
+  }
+  LCHK(gt::launch_tridiagonal(c->st, n, fi, lt, au, bu, cu, du, value, c->tri));
   return GT_OK;
 }
 

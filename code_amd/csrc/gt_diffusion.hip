@@ -1,0 +1,234 @@
+// gt_diffusion.hip -- the step after advection: the implicit vertical
+// diffusion of do_temperature and do_salinity (temperature.F90:463-551,
+// salinity.F90:346-393) and the batched getm_tridiagonal
+// (src/futils/tridiagonal.F90).
+//
+// One thread per water column.  The elimination runs down from the surface
+// (the reference's lt = kmax to fi = 1), forming each row of the
+// Crank-Nicolson matrix on the fly from a three-level window of the tracer
+// and the layer heights, and keeps the eliminated coefficients ru, qu in a
+// level-major work field (coalesced across the wavefront; the wavefront reads
+// them back at once, so they stay in L2); the substitution runs back up and
+// writes the tracer.  Expression order is the reference's (built with
+// -ffp-contract=off): bit-identical to the CPU build, except the short-wave
+// exponentials, which go through the device exp (within an ulp of libm's).
+#include "gt_kernels.h"
+
+namespace gt {
+
+namespace {
+
+constexpr int kBx = 64, kBy = 4;
+
+struct DiffArgs {
+  Dims d;
+  double dt, cnpar, avm;
+  const int32_t* az;
+  const double* hn;
+  const double* nuh;
+  double* f;
+  double* ru;  // work: level-major, one value per cell
+  double* qu;
+  // temperature only
+  const double *swr, *shf, *A, *g1, *g2, *dry_z, *S;
+  double* rad;
+  double* rad1d;  // work: the reflected profile (when reflection is on)
+  double refl_frac, min_bot_frac, rho_0_cpi;
+};
+
+template <bool TEMP>
+__global__ void __launch_bounds__(kBx* kBy) k_vertical_diffusion(DiffArgs a) {
+  const int ii = blockIdx.x * kBx + threadIdx.x + 2;
+  const int jj = blockIdx.y * kBy + threadIdx.y + 2;
+  const Dims d = a.d;
+  if (ii > d.NX - 3 || jj > d.NY - 3) return;
+  const long c = ii + (long)d.NX * jj;
+  const int azc = a.az[c];
+  const long NXY = d.NXY;
+  const int kmax = d.kmax;
+#define L(p, k) (p)[c + NXY * (long)(k)]
+  // where the short-wave profile is read from (rad, or the reflected copy)
+  const double* radsrc = a.rad;
+  double shf_dz = 0.0, dzdt = 0.0;
+  if (TEMP) {
+    if (azc < 1) return;
+    // solar radiation, temperature.F90:466-480
+    const double swr = a.swr[c], Ac = a.A[c], g1 = a.g1[c], g2 = a.g2[c];
+    L(a.rad, kmax) = swr;
+    double zz = 0.0;
+    double rad0 = swr;
+    for (int k = kmax - 1; k >= 0; k--) {
+      zz = zz + L(a.hn, k + 1);
+      rad0 = swr * (Ac * exp(-zz / g1) + (1 - Ac) * exp(-zz / g2));
+      L(a.rad, k) = rad0;
+    }
+    if (azc != 1) return;
+    // the primitive sea-ice model, :490-493
+    double shf_loc = a.shf[c];
+    if (L(a.f, kmax) <= (double)-0.0575f * L(a.S, kmax)) shf_loc = shf_loc < 0.0 ? 0.0 : shf_loc;
+    // bottom reflection, :486-500
+    if (a.refl_frac > 0.0 && rad0 / swr > a.min_bot_frac) {
+      const double swr_refl = rad0 * a.refl_frac;
+      L(a.rad1d, 0) = rad0 - swr_refl;
+      double z2 = 0.0;
+      for (int k = 1; k <= kmax; k++) {
+        z2 = z2 + L(a.hn, k);
+        L(a.rad1d, k) = L(a.rad, k) - swr_refl * (Ac * exp(-z2 / g1) + (1 - Ac) * exp(-z2 / g2));
+      }
+      radsrc = a.rad1d;
+    }
+    shf_dz = shf_loc * a.rho_0_cpi;
+    dzdt = a.dry_z[c] * a.dt;
+    if (kmax <= 1) return;
+  } else {
+    if (azc != 1 || kmax <= 1) return;
+  }
+  const double dt = a.dt, cnpar = a.cnpar, avm = a.avm, rc = a.rho_0_cpi;
+  // the surface row, k = kmax (auxo/auxn of interface kmax-1)
+  double hk = L(a.hn, kmax), hkm = L(a.hn, kmax - 1);
+  double fk = L(a.f, kmax), fkm = L(a.f, kmax - 1);
+  double nk = L(a.nuh, kmax - 1);
+  double auxo_m = 2.0 * (1 - cnpar) * dt * (nk + avm) / (hk + hkm);
+  double auxn_m = 2.0 * cnpar * dt * (nk + avm) / (hk + hkm);
+  double rk = 0.0, rkm = 0.0;
+  if (TEMP) {
+    rk = L(radsrc, kmax) * rc;
+    rkm = L(radsrc, kmax - 1) * rc;
+  }
+  double a4 = fk * (hk - auxo_m) + fkm * auxo_m;
+  if (TEMP) a4 = a4 + dzdt * (rk + shf_dz - rkm);
+  double a1 = -auxn_m, a2 = hk + auxn_m;
+  double ru = a1 / a2, qu = a4 / a2;
+  L(a.ru, kmax) = ru;
+  L(a.qu, kmax) = qu;
+  // down the column: row k uses interfaces k (carried) and k-1
+  for (int k = kmax - 1; k >= 1; k--) {
+    const double auxo_k = auxo_m, auxn_k = auxn_m;
+    const double fkp = fk;
+    fk = fkm;
+    hk = hkm;
+    if (TEMP) rk = rkm;
+    if (k >= 2) {
+      hkm = L(a.hn, k - 1);
+      fkm = L(a.f, k - 1);
+      nk = L(a.nuh, k - 1);
+      auxo_m = 2.0 * (1 - cnpar) * dt * (nk + avm) / (hk + hkm);
+      auxn_m = 2.0 * cnpar * dt * (nk + avm) / (hk + hkm);
+      if (TEMP) rkm = L(radsrc, k - 1) * rc;
+      const double a3 = -auxn_k;
+      a1 = -auxn_m;
+      a2 = hk + auxn_k + auxn_m;
+      a4 = fkp * auxo_k + fk * (hk - auxo_k - auxo_m) + fkm * auxo_m;
+      if (TEMP) a4 = a4 + dzdt * (rk - rkm);
+      const double den = a2 - a3 * ru;
+      const double ru_n = a1 / den;
+      qu = (a4 - a3 * qu) / (a2 - a3 * ru);
+      ru = ru_n;
+      (void)den;
+      L(a.ru, k) = ru;
+      L(a.qu, k) = qu;
+    } else {
+      // the bottom row, k = fi = 1
+      if (TEMP) rkm = L(radsrc, 0) * rc;
+      const double a3 = -auxn_k;
+      a2 = hk + auxn_k;
+      a4 = fkp * auxo_k + fk * (hk - auxo_k);
+      if (TEMP) a4 = a4 + dzdt * (rk - rkm);
+      qu = (a4 - a3 * qu) / (a2 - a3 * ru);
+    }
+  }
+  // back up: value(1) = qu(1), value(k) = qu(k) - ru(k) value(k-1)
+  double v = qu;
+  L(a.f, 1) = v;
+  for (int k = 2; k <= kmax; k++) {
+    v = L(a.qu, k) - L(a.ru, k) * v;
+    L(a.f, k) = v;
+  }
+#undef L
+}
+
+// getm_tridiagonal over n systems, level-major: element k of system s at
+// [k*n + s]
+__global__ void k_tridiagonal(long n, int fi, int lt, const double* __restrict__ au,
+                              const double* __restrict__ bu, const double* __restrict__ cu,
+                              const double* __restrict__ du, double* __restrict__ value,
+                              double* __restrict__ ru, double* __restrict__ qu) {
+  const long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (s >= n) return;
+#define E(p, k) (p)[(long)(k) * n + s]
+  double r = E(au, lt) / E(bu, lt);
+  double q = E(du, lt) / E(bu, lt);
+  E(ru, lt) = r;
+  E(qu, lt) = q;
+  for (int i = lt - 1; i >= fi + 1; i--) {
+    const double c = E(cu, i), b = E(bu, i);
+    const double rn = E(au, i) / (b - c * r);
+    q = (E(du, i) - c * q) / (b - c * r);
+    r = rn;
+    E(ru, i) = r;
+    E(qu, i) = q;
+  }
+  q = (E(du, fi) - E(cu, fi) * q) / (E(bu, fi) - E(cu, fi) * r);
+  double v = q;
+  E(value, fi) = v;
+  for (int i = fi + 1; i <= lt; i++) {
+    v = E(qu, i) - E(ru, i) * v;
+    E(value, i) = v;
+  }
+#undef E
+}
+
+}  // namespace
+
+hipError_t launch_vertical_diffusion(hipStream_t st, Dims d, bool temperature, double dt,
+                                     double cnpar, const int32_t* az, const double* hn,
+                                     const double* nuh, double* f, double* work,
+                                     const double* swr, const double* shf, const double* A,
+                                     const double* g1, const double* g2, const double* dry_z,
+                                     const double* S, double refl_frac, double min_bot_frac,
+                                     double* rad) {
+  if (d.NX < 5 || d.NY < 5) return hipSuccess;
+  DiffArgs a{};
+  a.d = d;
+  a.dt = dt;
+  a.cnpar = cnpar;
+  a.az = az;
+  a.hn = hn;
+  a.nuh = nuh;
+  a.f = f;
+  const long N3 = d.NXY * (long)(d.kmax + 1);
+  a.ru = work;
+  a.qu = work + N3;
+  a.rad1d = work + 2 * N3;
+  a.swr = swr;
+  a.shf = shf;
+  a.A = A;
+  a.g1 = g1;
+  a.g2 = g2;
+  a.dry_z = dry_z;
+  a.S = S;
+  a.rad = rad;
+  a.refl_frac = refl_frac;
+  a.min_bot_frac = min_bot_frac;
+  // parameters.F90:22-27 (FP32 literals)
+  a.rho_0_cpi = 1.0 / ((double)1025.f * (double)3985.f);
+  a.avm = temperature ? (double)1.4e-7f : (double)1.1e-9f;
+  const dim3 g((unsigned)((d.NX - 4 + kBx - 1) / kBx), (unsigned)((d.NY - 4 + kBy - 1) / kBy));
+  if (temperature)
+    hipLaunchKernelGGL(k_vertical_diffusion<true>, g, dim3(kBx, kBy), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_vertical_diffusion<false>, g, dim3(kBx, kBy), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_tridiagonal(hipStream_t st, long n, int fi, int lt, const double* au,
+                              const double* bu, const double* cu, const double* du,
+                              double* value, double* work) {
+  if (n <= 0) return hipSuccess;
+  const long nk = n * (long)(lt + 1);
+  hipLaunchKernelGGL(k_tridiagonal, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, fi,
+                     lt, au, bu, cu, du, value, work, work + nk);
+  return hipGetLastError();
+}
+
+}  // namespace gt

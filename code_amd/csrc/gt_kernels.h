@@ -76,6 +76,18 @@ hipError_t launch_zc(hipStream_t st, Dims d, const double* H, const double* hn, 
 hipError_t launch_ww_momentum(hipStream_t st, Dims d, double dt, double dx, double dy,
                               double ard1, const int32_t* az, const double* uu, const double* vv,
                               const double* ho, const double* hn, double* ww);
+// vertical diffusion (gt_diffusion.hip); work holds 3 fields of N3
+hipError_t launch_vertical_diffusion(hipStream_t st, Dims d, bool temperature, double dt,
+                                     double cnpar, const int32_t* az, const double* hn,
+                                     const double* nuh, double* f, double* work,
+                                     const double* swr, const double* shf, const double* A,
+                                     const double* g1, const double* g2, const double* dry_z,
+                                     const double* S, double refl_frac, double min_bot_frac,
+                                     double* rad);
+// n systems, level-major; work holds 2*n*(lt+1)
+hipError_t launch_tridiagonal(hipStream_t st, long n, int fi, int lt, const double* au,
+                              const double* bu, const double* cu, const double* du,
+                              double* value, double* work);
 hipError_t launch_ip_bm(hipStream_t st, Dims d, const int32_t* az, const int32_t* au,
                         const int32_t* av, const double* H, const double* hn,
                         const double* hun, const double* hvn, const double* buoy, double* zz,

@@ -7,9 +7,15 @@
 !   adv_split_u/v/w    src/2d/adv_split_u.F90, adv_split_v.F90, src/3d/adv_split_w.F90
 !   adv_upstream_2dh   src/2d/adv_upstream_2dh.F90
 !   update_3d_halo     src/futils/halo_zones.F90:141-187
-! plus do_internal_pressure_gpu / do_eqstate_linear_gpu, which take as
-! arguments the module variables the argument-less reference routines
-! (internal_pressure.F90, eqstate.F90) read.
+! plus do_internal_pressure_gpu / do_eqstate_linear_gpu / do_eqstate_gpu,
+! which take as arguments the module variables the argument-less reference
+! routines (internal_pressure.F90, eqstate.F90) read, and the device-resident
+! forms of the routines either side of the step:
+!   start_macro_gpu        src/3d/start_macro.F90:54-86
+!   coordinates_gpu        src/3d/coordinates.F90:66-107 (sigma, general)
+!   ww_momentum_3d_gpu     src/3d/ww_momentum_3d.F90:66-119
+!   salinity_diffusion_gpu, temperature_diffusion_gpu
+!                          src/3d/salinity.F90:346-393, temperature.F90:463-551
 !
 ! Host arrays are copied to device mirrors owned by this module, the step
 ! runs on the GPU, and the outputs are copied back.  Callers that keep their
@@ -25,6 +31,8 @@
    public do_advection_3d, do_advection_3d_dev
    public do_internal_pressure_gpu, do_eqstate_linear_gpu, update_3d_halo
    public gt_malloc_f, gt_free_f, gt_to_device, gt_to_host
+   public do_eqstate_gpu, start_macro_gpu, set_coordinates_gpu, coordinates_gpu
+   public ww_momentum_3d_gpu, salinity_diffusion_gpu, temperature_diffusion_gpu
 
    integer, public, parameter :: NOSPLIT=0, FULLSPLIT=1, HALFSPLIT=2, HVSPLIT=3
    integer, public, parameter :: NOADV=0, UPSTREAM=1, UPSTREAM_2DH=2, P2=3
@@ -113,6 +121,62 @@
       function c_gt_last_error() bind(C, name='gt_last_error') result(p)
          import :: c_ptr
          type(c_ptr) :: p
+      end function
+      function c_gt_do_eqstate(ctx, method, nonneg, T, S, hn, rho, buoy, alpha, beta, &
+               T0, S0, dtr0, dsr0) bind(C, name='gt_do_eqstate') result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx, T, S, hn, rho, buoy, alpha, beta
+         integer(c_int), value :: method, nonneg
+         real(c_double), value :: T0, S0, dtr0, dsr0
+         integer(c_int) :: rc
+      end function
+      function c_gt_start_macro(ctx, M, min_depth, H, HU, HV, z, sseo, ssen, ssuo, ssun, &
+               ssvo, ssvn, Dn, Dun, Dvn, Uint, Vint) bind(C, name='gt_start_macro') result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx, H, HU, HV, z, sseo, ssen, ssuo, ssun, ssvo, ssvn
+         type(c_ptr), value :: Dn, Dun, Dvn, Uint, Vint
+         integer(c_int), value :: M
+         real(c_double), value :: min_depth
+         integer(c_int) :: rc
+      end function
+      function c_gt_set_coordinates(ctx, vert_cord, ddu, ddl, d_gamma, gamma_surf, min_depth) &
+               bind(C, name='gt_set_coordinates') result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: vert_cord, gamma_surf
+         real(c_double), value :: ddu, ddl, d_gamma, min_depth
+         integer(c_int) :: rc
+      end function
+      function c_gt_coordinates(ctx, dt, cord_relax, maxdepth, H, HU, HV, sseo, ssen, ssuo, &
+               ssun, ssvo, ssvn, ho, hn, huo, hun, hvo, hvn, zc) &
+               bind(C, name='gt_coordinates') result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx, H, HU, HV, sseo, ssen, ssuo, ssun, ssvo, ssvn
+         type(c_ptr), value :: ho, hn, huo, hun, hvo, hvn, zc
+         real(c_double), value :: dt, cord_relax, maxdepth
+         integer(c_int) :: rc
+      end function
+      function c_gt_ww_momentum_3d(ctx, dt, uu, vv, ho, hn, ww) &
+               bind(C, name='gt_ww_momentum_3d') result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx, uu, vv, ho, hn, ww
+         real(c_double), value :: dt
+         integer(c_int) :: rc
+      end function
+      function c_gt_salinity_diffusion(ctx, dt, cnpar, hn, nuh, S) &
+               bind(C, name='gt_salinity_diffusion') result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx, hn, nuh, S
+         real(c_double), value :: dt, cnpar
+         integer(c_int) :: rc
+      end function
+      function c_gt_temperature_diffusion(ctx, dt, cnpar, hn, nuh, swr, shf, A, g1, g2, &
+               dry_z, S, refl, minfrac, rad, T) &
+               bind(C, name='gt_temperature_diffusion') result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx, hn, nuh, swr, shf, A, g1, g2, dry_z, S, rad, T
+         real(c_double), value :: dt, cnpar, refl, minfrac
+         integer(c_int) :: rc
       end function
    end interface
 
@@ -250,6 +314,77 @@
               idpdx, idpdy), 'do_internal_pressure()')
    call check(c_gt_synchronize(ctx), 'do_internal_pressure()')
    end subroutine do_internal_pressure_gpu
+
+   ! do_eqstate (eqstate.F90:94-271), eqstate_method 1, 2 or 3, device-resident;
+   ! alpha/beta may be c_null_ptr (the _OLD_BVF_ build)
+   subroutine do_eqstate_gpu(method, T, S, hn, rho, buoy, alpha, beta, T0, S0, dtr0, dsr0)
+   integer, intent(in) :: method
+   type(c_ptr), intent(in) :: T, S, hn, rho, buoy, alpha, beta
+   real(c_double), intent(in) :: T0, S0, dtr0, dsr0
+   integer(c_int) :: nonneg
+   nonneg = 0
+#ifdef NONNEGSALT
+   nonneg = 1
+#endif
+   call check(c_gt_do_eqstate(ctx, method, nonneg, T, S, hn, rho, buoy, alpha, beta, &
+              T0, S0, dtr0, dsr0), 'do_eqstate()')
+   end subroutine do_eqstate_gpu
+
+   ! start_macro (start_macro.F90:54-86), device-resident 2-D fields
+   subroutine start_macro_gpu(M, min_depth, H, HU, HV, z, sseo, ssen, ssuo, ssun, ssvo, &
+                              ssvn, Dn, Dun, Dvn, Uint, Vint)
+   integer, intent(in) :: M
+   real(c_double), intent(in) :: min_depth
+   type(c_ptr), intent(in) :: H, HU, HV, z, sseo, ssen, ssuo, ssun, ssvo, ssvn
+   type(c_ptr), intent(in) :: Dn, Dun, Dvn, Uint, Vint
+   call check(c_gt_start_macro(ctx, M, min_depth, H, HU, HV, z, sseo, ssen, ssuo, ssun, &
+              ssvo, ssvn, Dn, Dun, Dvn, Uint, Vint), 'start_macro()')
+   end subroutine start_macro_gpu
+
+   ! the domain-namelist values coordinates() uses (vert_cord 1, 2 or 3)
+   subroutine set_coordinates_gpu(vert_cord, ddu, ddl, d_gamma, gamma_surf, min_depth)
+   integer, intent(in) :: vert_cord
+   real(c_double), intent(in) :: ddu, ddl, d_gamma, min_depth
+   logical, intent(in) :: gamma_surf
+   integer(c_int) :: gs
+   gs = 0
+   if (gamma_surf) gs = 1
+   call check(c_gt_set_coordinates(ctx, vert_cord, ddu, ddl, d_gamma, gs, min_depth), &
+              'coordinates()')
+   end subroutine set_coordinates_gpu
+
+   ! coordinates (coordinates.F90:66-107), device-resident; zc may be c_null_ptr
+   subroutine coordinates_gpu(dt, cord_relax, maxdepth, H, HU, HV, sseo, ssen, ssuo, ssun, &
+                              ssvo, ssvn, ho, hn, huo, hun, hvo, hvn, zc)
+   real(c_double), intent(in) :: dt, cord_relax, maxdepth
+   type(c_ptr), intent(in) :: H, HU, HV, sseo, ssen, ssuo, ssun, ssvo, ssvn
+   type(c_ptr), intent(in) :: ho, hn, huo, hun, hvo, hvn, zc
+   call check(c_gt_coordinates(ctx, dt, cord_relax, maxdepth, H, HU, HV, sseo, ssen, ssuo, &
+              ssun, ssvo, ssvn, ho, hn, huo, hun, hvo, hvn, zc), 'coordinates')
+   end subroutine coordinates_gpu
+
+   ! ww_momentum_3d (ww_momentum_3d.F90:66-119), device-resident
+   subroutine ww_momentum_3d_gpu(dt, uu, vv, ho, hn, ww)
+   real(c_double), intent(in) :: dt
+   type(c_ptr), intent(in) :: uu, vv, ho, hn, ww
+   call check(c_gt_ww_momentum_3d(ctx, dt, uu, vv, ho, hn, ww), 'ww_momentum_3d()')
+   end subroutine ww_momentum_3d_gpu
+
+   ! the vertical diffusion of do_salinity (salinity.F90:346-393)
+   subroutine salinity_diffusion_gpu(dt, cnpar, hn, nuh, S)
+   real(c_double), intent(in) :: dt, cnpar
+   type(c_ptr), intent(in) :: hn, nuh, S
+   call check(c_gt_salinity_diffusion(ctx, dt, cnpar, hn, nuh, S), 'do_salinity()')
+   end subroutine salinity_diffusion_gpu
+
+   ! the short-wave and vertical diffusion of do_temperature (temperature.F90:463-551)
+   subroutine temperature_diffusion_gpu(dt, cnpar, hn, nuh, swr, shf, A, g1, g2, dry_z, S, &
+                                        swr_bot_refl_frac, swr_min_bot_frac, rad, T)
+   real(c_double), intent(in) :: dt, cnpar, swr_bot_refl_frac, swr_min_bot_frac
+   type(c_ptr), intent(in) :: hn, nuh, swr, shf, A, g1, g2, dry_z, S, rad, T
+   call check(c_gt_temperature_diffusion(ctx, dt, cnpar, hn, nuh, swr, shf, A, g1, g2, &
+              dry_z, S, swr_bot_refl_frac, swr_min_bot_frac, rad, T), 'do_temperature()')
+   end subroutine temperature_diffusion_gpu
 
    ! update_3d_halo (halo_zones.F90:141-187) on host arrays, serial semantics
    subroutine update_3d_halo(f1, f2, mask, imin, jmin, imax, jmax, kmax, tag, mirror)

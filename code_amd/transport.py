@@ -44,6 +44,7 @@ EXPORTS = [
     "gt_ip_blumberg_mellor_adjoint", "gt_eos_linear_adjoint", "gt_autotune_batch",
     "gt_do_eqstate", "gt_rho_from_theta_unesco80", "gt_rho_from_theta", "gt_eosall_from_theta",
     "gt_start_macro", "gt_set_coordinates", "gt_coordinates", "gt_ww_momentum_3d",
+    "gt_salinity_diffusion", "gt_temperature_diffusion", "gt_tridiagonal_batch",
 ]
 
 
@@ -102,6 +103,9 @@ def load_library():
         "gt_set_coordinates": (i, [vp, i, d, d, d, i, d]),
         "gt_coordinates": (i, [vp, d, d, d] + [vp] * 16),
         "gt_ww_momentum_3d": (i, [vp, d, vp, vp, vp, vp, vp]),
+        "gt_salinity_diffusion": (i, [vp, d, d, vp, vp, vp]),
+        "gt_temperature_diffusion": (i, [vp, d, d] + [vp] * 9 + [d, d, vp, vp]),
+        "gt_tridiagonal_batch": (i, [vp, C.c_long, i, i, vp, vp, vp, vp, vp]),
         "gt_rho_from_theta_unesco80": (i, [vp, C.c_long, vp, vp, vp, i]),
         "gt_rho_from_theta": (i, [vp, C.c_long, vp, vp, vp, vp, vp]),
         "gt_eosall_from_theta": (i, [vp, C.c_long, vp, vp, vp, vp, vp]),
@@ -282,6 +286,22 @@ class Context:
     def ww_momentum_3d(self, dt, uu, vv, ho, hn, ww):
         check(self.lib.gt_ww_momentum_3d(self.h, dt, _ptr(uu), _ptr(vv), _ptr(ho), _ptr(hn),
                                          _ptr(ww)))
+
+    # ---- vertical diffusion (temperature.F90:463-551, salinity.F90:346-393) ----
+    def salinity_diffusion(self, dt, cnpar, hn, nuh, S):
+        check(self.lib.gt_salinity_diffusion(self.h, dt, cnpar, _ptr(hn), _ptr(nuh), _ptr(S)))
+
+    def temperature_diffusion(self, dt, cnpar, hn, nuh, swr, shf, A, g1, g2, dry_z, S, rad, T,
+                              swr_bot_refl_frac=-1.0, swr_min_bot_frac=0.01):
+        f32 = lambda v: float(np.float32(v))
+        check(self.lib.gt_temperature_diffusion(
+            self.h, dt, cnpar, *(_ptr(x) for x in (hn, nuh, swr, shf, A, g1, g2, dry_z, S)),
+            swr_bot_refl_frac, f32(swr_min_bot_frac), _ptr(rad), _ptr(T)))
+
+    def tridiagonal_batch(self, fi, lt, au, bu, cu, du, value):
+        """systems along the last axis: arrays of shape (lt+1, n)"""
+        check(self.lib.gt_tridiagonal_batch(self.h, au.shape[-1], fi, lt, _ptr(au), _ptr(bu),
+                                            _ptr(cu), _ptr(du), _ptr(value)))
 
     def rho_from_theta_unesco80(self, T, S, rho, nonnegsalt=False):
         check(self.lib.gt_rho_from_theta_unesco80(self.h, T.numel(), _ptr(T), _ptr(S),

@@ -159,6 +159,36 @@ int gt_ww_momentum_3d(gt_ctx *ctx, double dt, const double *uu,
                       const double *vv, const double *ho, const double *hn,
                       double *ww);
 
+/* ---- the step after advection: implicit vertical diffusion ---------------
+ * The part of do_salinity / do_temperature that follows their
+ * do_advection_3d call, on interior water columns (az == 1), kmax > 1:
+ * Crank-Nicolson (cnpar) diffusion with nuh + the molecular diffusivity
+ * (parameters.F90:26-27: avmolt = 1.4e-7, avmols = 1.1e-9, FP32 literals),
+ * solved per column with getm_tridiagonal. */
+/* salinity.F90:346-393 */
+int gt_salinity_diffusion(gt_ctx *ctx, double dt, double cnpar,
+                          const double *hn, const double *nuh, double *S);
+/* temperature.F90:463-551: rad = swr (A exp(-z/g1) + (1-A) exp(-z/g2)) on
+ * columns with az >= 1 (written, k = 0..kmax), the sea-ice rule (no cooling
+ * flux where T(kmax) <= -0.0575 S(kmax)), the bottom reflection
+ * (swr_bot_refl_frac > 0 and rad(0)/rad(kmax) > swr_min_bot_frac), and the
+ * diffusion with dry_z dt (rad divergence + shf at the surface)/(rho_0 cp) as
+ * source.  swr, shf, A, g1, g2, dry_z are 2-D; S is read at kmax. */
+int gt_temperature_diffusion(gt_ctx *ctx, double dt, double cnpar,
+                             const double *hn, const double *nuh,
+                             const double *swr, const double *shf,
+                             const double *A, const double *g1,
+                             const double *g2, const double *dry_z,
+                             const double *S, double swr_bot_refl_frac,
+                             double swr_min_bot_frac, double *rad, double *T);
+/* getm_tridiagonal, src/futils/tridiagonal.F90:8-51, over n independent
+ * systems, rows fi..lt (0 <= fi < lt): au sub-, bu main, cu super-diagonal,
+ * du right-hand side, value the solution; element k of system s at
+ * [k*n + s] (level-major, so the systems run across a wavefront) */
+int gt_tridiagonal_batch(gt_ctx *ctx, long n, int fi, int lt,
+                         const double *au, const double *bu, const double *cu,
+                         const double *du, double *value);
+
 /* ---- internal pressure ------------------------------------------------- */
 /* do_eqstate with eqstate_method=1 (linear), src/3d/eqstate.F90:127-149 and
  * the buoyancy of :252-261.  rho may be NULL.  The reference defaults are

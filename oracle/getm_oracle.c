@@ -1673,3 +1673,159 @@ void or_ww_momentum_3d(const or_dom *d, const int32_t *az, double dt,
                             ard1;
             }
 }
+
+/* ---- the step after advection: vertical diffusion ------------------------ */
+
+/* getm_tridiagonal, src/futils/tridiagonal.F90:8-51: rows fi..lt of a
+ * tridiagonal system, au sub-, bu main, cu super-diagonal, du right-hand
+ * side; elimination from lt down to fi, then substitution upwards.  ru/qu
+ * are work arrays of kmax+1. */
+void or_tridiagonal(int kmax, int fi, int lt, const double *au, const double *bu,
+                    const double *cu, const double *du, double *value)
+{
+    double *ru = (double *)malloc(2 * (size_t)(kmax + 1) * sizeof(double));
+    double *qu = ru + (kmax + 1);
+    ru[lt] = au[lt] / bu[lt];
+    qu[lt] = du[lt] / bu[lt];
+    for (int i = lt - 1; i >= fi + 1; i--) {
+        ru[i] = au[i] / (bu[i] - cu[i] * ru[i + 1]);
+        qu[i] = (du[i] - cu[i] * qu[i + 1]) / (bu[i] - cu[i] * ru[i + 1]);
+    }
+    qu[fi] = (du[fi] - cu[fi] * qu[fi + 1]) / (bu[fi] - cu[fi] * ru[fi + 1]);
+    value[fi] = qu[fi];
+    for (int i = fi + 1; i <= lt; i++) value[i] = qu[i] - ru[i] * value[i - 1];
+    free(ru);
+}
+
+/* the implicit vertical diffusion of one column (temperature.F90:505-547,
+ * salinity.F90:351-387); rad1d = NULL for salinity (no source terms) */
+static void diffuse_column(int kmax, double dt, double cnpar, double avm,
+                           const double *hn, const double *nuh, long c,
+                           long NXY, double *f, const double *rad1d,
+                           double shf_dz, double dry_z, double *w)
+{
+#define C3(a, k) a[c + NXY * (long)(k)]
+    double *auxo = w, *auxn = w + (kmax + 1), *a1 = w + 2 * (kmax + 1),
+           *a2 = w + 3 * (kmax + 1), *a3 = w + 4 * (kmax + 1),
+           *a4 = w + 5 * (kmax + 1), *res = w + 6 * (kmax + 1);
+    for (int k = 1; k <= kmax - 1; k++) {
+        auxo[k] = 2.0 * (1 - cnpar) * dt * (C3(nuh, k) + avm) /
+                  (C3(hn, k + 1) + C3(hn, k));
+        auxn[k] = 2.0 * cnpar * dt * (C3(nuh, k) + avm) /
+                  (C3(hn, k + 1) + C3(hn, k));
+    }
+    int k = kmax;
+    a1[k] = -auxn[k - 1];
+    a2[k] = C3(hn, k) + auxn[k - 1];
+    a4[k] = C3(f, k) * (C3(hn, k) - auxo[k - 1]) + C3(f, k - 1) * auxo[k - 1];
+    if (rad1d) a4[k] = a4[k] + dry_z * dt * (rad1d[k] + shf_dz - rad1d[k - 1]);
+    for (k = 2; k <= kmax - 1; k++) {
+        a3[k] = -auxn[k];
+        a1[k] = -auxn[k - 1];
+        a2[k] = C3(hn, k) + auxn[k] + auxn[k - 1];
+        a4[k] = C3(f, k + 1) * auxo[k] +
+                C3(f, k) * (C3(hn, k) - auxo[k] - auxo[k - 1]) +
+                C3(f, k - 1) * auxo[k - 1];
+        if (rad1d) a4[k] = a4[k] + dry_z * dt * (rad1d[k] - rad1d[k - 1]);
+    }
+    k = 1;
+    a3[k] = -auxn[k];
+    a2[k] = C3(hn, k) + auxn[k];
+    a4[k] = C3(f, k + 1) * auxo[k] + C3(f, k) * (C3(hn, k) - auxo[k]);
+    if (rad1d) a4[k] = a4[k] + dry_z * dt * (rad1d[k] - rad1d[k - 1]);
+    or_tridiagonal(kmax, 1, kmax, a1, a2, a3, a4, res);
+    for (k = 1; k <= kmax; k++) C3(f, k) = res[k];
+#undef C3
+}
+
+/* the vertical diffusion of do_salinity, salinity.F90:346-393 (after the
+ * advection): Crank-Nicolson (cnpar) with nuh + avmols on interior water
+ * columns, kmax > 1 */
+void or_salinity_diffusion(const or_dom *d, const int32_t *az, double dt,
+                           double cnpar, const double *hn, const double *nuh,
+                           double *S)
+{
+    DOMVARS(d);
+    const double avmols = (double)1.1e-9f; /* parameters.F90:27 */
+    if (kmax <= 1) return;
+#pragma omp parallel OMP_NT
+    {
+        double *w = (double *)malloc(7 * (size_t)(kmax + 1) * sizeof(double));
+#pragma omp for schedule(static)
+        for (int j = jl; j <= jh; j++)
+            for (int i = il; i <= ih; i++)
+                if (az[IJ(i, j)] == 1)
+                    diffuse_column(kmax, dt, cnpar, avmols, hn, nuh, IJ(i, j), NXY, S,
+                                   NULL, 0.0, 0.0, w);
+        free(w);
+    }
+}
+
+/* the solar radiation and vertical diffusion of do_temperature,
+ * temperature.F90:463-551 (after the advection): the short-wave profile
+ * rad = swr (A exp(-z/g1) + (1-A) exp(-z/g2)) on columns with az >= 1, then
+ * on water columns (az == 1) the primitive sea-ice clip of the surface heat
+ * flux, the optional bottom reflection, and the implicit diffusion with the
+ * radiation divergence and the surface flux as sources.  2-D inputs: swr,
+ * shf, A, g1, g2, dry_z; S is read at kmax only.  rad is written. */
+void or_temperature_diffusion(const or_dom *d, const int32_t *az, double dt,
+                              double cnpar, const double *hn, const double *nuh,
+                              const double *swr, const double *shf,
+                              const double *A, const double *g1, const double *g2,
+                              const double *dry_z, const double *S,
+                              double swr_bot_refl_frac, double swr_min_bot_frac,
+                              double *rad, double *T)
+{
+    DOMVARS(d);
+    const double avmolt = (double)1.4e-7f; /* parameters.F90:26 */
+    const double rho_0 = (double)1025.f, cp = (double)3985.f;
+    const double rho_0_cpi = 1.0 / (rho_0 * cp);
+#define C3(a, c, k) a[(c) + NXY * (long)(k)]
+#pragma omp parallel for OMP_NT schedule(static)
+    for (int j = jl; j <= jh; j++)
+        for (int i = il; i <= ih; i++) {
+            const long c = IJ(i, j);
+            if (az[c] < 1) continue;
+            const double swr_loc = swr[c];
+            C3(rad, c, kmax) = swr_loc;
+            double zz = 0.0;
+            for (int k = kmax - 1; k >= 0; k--) {
+                zz = zz + C3(hn, c, k + 1);
+                C3(rad, c, k) = swr_loc * (A[c] * exp(-zz / g1[c]) +
+                                           (1 - A[c]) * exp(-zz / g2[c]));
+            }
+        }
+#pragma omp parallel OMP_NT
+    {
+        double *w = (double *)malloc(8 * (size_t)(kmax + 1) * sizeof(double));
+        double *rad1d = w + 7 * (kmax + 1);
+#pragma omp for schedule(static)
+        for (int j = jl; j <= jh; j++)
+            for (int i = il; i <= ih; i++) {
+                const long c = IJ(i, j);
+                if (az[c] != 1) continue;
+                double shf_loc = shf[c];
+                if (C3(T, c, kmax) <= -0.0575f * C3(S, c, kmax))
+                    shf_loc = shf_loc < 0.0 ? 0.0 : shf_loc; /* max(0, shf) */
+                for (int k = 0; k <= kmax; k++) rad1d[k] = C3(rad, c, k);
+                if (swr_bot_refl_frac > 0.0 &&
+                    rad1d[0] / rad1d[kmax] > swr_min_bot_frac) {
+                    const double swr_refl = rad1d[0] * swr_bot_refl_frac;
+                    rad1d[0] = rad1d[0] - swr_refl;
+                    double zz = 0.0;
+                    for (int k = 1; k <= kmax; k++) {
+                        zz = zz + C3(hn, c, k);
+                        rad1d[k] = C3(rad, c, k) -
+                                   swr_refl * (A[c] * exp(-zz / g1[c]) +
+                                               (1 - A[c]) * exp(-zz / g2[c]));
+                    }
+                }
+                for (int k = 0; k <= kmax; k++) rad1d[k] = rad1d[k] * rho_0_cpi;
+                if (kmax > 1)
+                    diffuse_column(kmax, dt, cnpar, avmolt, hn, nuh, c, NXY, T, rad1d,
+                                   shf_loc * rho_0_cpi, dry_z[c], w);
+            }
+        free(w);
+    }
+#undef C3
+}

@@ -184,6 +184,24 @@ void or_ww_momentum_3d(const or_dom *d, const int32_t *az, double dt,
                        const double *uu, const double *vv, const double *ho,
                        const double *hn, double *ww);
 
+/* the step after advection: implicit vertical diffusion */
+/* getm_tridiagonal, src/futils/tridiagonal.F90:8-51 */
+void or_tridiagonal(int kmax, int fi, int lt, const double *au, const double *bu,
+                    const double *cu, const double *du, double *value);
+/* do_salinity's vertical diffusion, salinity.F90:346-393 */
+void or_salinity_diffusion(const or_dom *d, const int32_t *az, double dt,
+                           double cnpar, const double *hn, const double *nuh,
+                           double *S);
+/* do_temperature's solar radiation and vertical diffusion,
+ * temperature.F90:463-551; rad is written */
+void or_temperature_diffusion(const or_dom *d, const int32_t *az, double dt,
+                              double cnpar, const double *hn, const double *nuh,
+                              const double *swr, const double *shf,
+                              const double *A, const double *g1, const double *g2,
+                              const double *dry_z, const double *S,
+                              double swr_bot_refl_frac, double swr_min_bot_frac,
+                              double *rad, double *T);
+
 /* OpenMP thread count used by the restatement (0 = runtime default) */
 void or_set_threads(int n);
 int or_get_threads(void);

@@ -80,6 +80,9 @@ def lib():
             "or_coord_delete": (None, [vp]),
             "or_coordinates": (i, [D, vp, d, d, d] + [vp] * 16),
             "or_ww_momentum_3d": (None, [D, vp, d, vp, vp, vp, vp, vp]),
+            "or_tridiagonal": (None, [i, i, i, vp, vp, vp, vp, vp]),
+            "or_salinity_diffusion": (None, [D, vp, d, d, vp, vp, vp]),
+            "or_temperature_diffusion": (None, [D, vp, d, d] + [vp] * 9 + [d, d, vp, vp]),
             "or_set_threads": (None, [i]),
             "or_get_threads": (i, []),
         }
@@ -118,6 +121,14 @@ def eosall_from_theta(s, th, p):
     a, b = C.c_double(), C.c_double()
     lib().or_eosall_from_theta(s, th, p, C.byref(a), C.byref(b))
     return a.value, b.value
+
+
+def tridiagonal(kmax, fi, lt, au, bu, cu, du):
+    """getm_tridiagonal (tridiagonal.F90:8-51) on arrays of kmax+1"""
+    value = np.zeros(kmax + 1)
+    arr = [np.ascontiguousarray(a, np.float64) for a in (au, bu, cu, du)]
+    lib().or_tridiagonal(kmax, fi, lt, *(P(a) for a in arr), P(value))
+    return value
 
 
 def set_threads(n):
@@ -243,6 +254,18 @@ class Oracle:
     def ww_momentum_3d(self, dt, uu, vv, ho, hn, ww):
         lib().or_ww_momentum_3d(C.byref(self.dom), P(self.az), dt, P(uu), P(vv), P(ho), P(hn),
                                 P(ww))
+
+    # ---- vertical diffusion (temperature.F90:463-551, salinity.F90:346-393) ----
+    def salinity_diffusion(self, dt, cnpar, hn, nuh, S):
+        lib().or_salinity_diffusion(C.byref(self.dom), P(self.az), dt, cnpar, P(hn), P(nuh),
+                                    P(S))
+
+    def temperature_diffusion(self, dt, cnpar, hn, nuh, swr, shf, A, g1, g2, dry_z, S, rad, T,
+                              swr_bot_refl_frac=-1.0, swr_min_bot_frac=0.01):
+        f32 = lambda v: float(np.float32(v))
+        lib().or_temperature_diffusion(C.byref(self.dom), P(self.az), dt, cnpar, P(hn), P(nuh),
+                                       P(swr), P(shf), P(A), P(g1), P(g2), P(dry_z), P(S),
+                                       swr_bot_refl_frac, f32(swr_min_bot_frac), P(rad), P(T))
 
     def ip_blumberg_mellor(self, H, hn, hun, hvn, buoy, zz, idpdx, idpdy):
         lib().or_ip_blumberg_mellor(C.byref(self.dom), P(self.az), P(self.au), P(self.av),
