@@ -488,20 +488,34 @@ def extensions_section(torch, np, gt, inputs, dev, tr, ctx, st, steps):
             ctx.temperature_diffusion(tr.dt, 0.5, hn, nuh, swr, shf, Aa, g1, g2, dry, Sd, rad,
                                       Td)
 
+    # momentum advection as a caller of the path: P2-PDM HALFSPLIT on the
+    # U and V control volumes (two transport steps and their averaging)
+    hun, hvn = D(tr.hun), D(tr.hvn)
+    uuEx, vvEx = torch.zeros_like(T), torch.zeros_like(T)
+    t_uv = EventTimer(torch, st)
+
+    def momentum(timed):
+        with (t_uv if timed else _Null()):
+            ctx.uv_advect_3d(tr.dt, uu, vv, ww_in, hn, hn, hun, hvn,
+                             gt.HALFSPLIT, gt.P2_PDM, gt.P2_PDM, uuEx, vvEx)
+
+    ww_in = D(tr.ww)
     with torch.cuda.stream(st):
         for _ in range(3):
             ctx.do_eqstate(3, T, S, hn, rho, buoy, alpha, beta)
             producers(False)  # the first coordinates call initialises
             diffusion(False)
+            momentum(False)
         st.synchronize()
         for _ in range(steps):
             with t_eos:
                 ctx.do_eqstate(3, T, S, hn, rho, buoy, alpha, beta)
             producers(True)
             diffusion(True)
+            momentum(True)
         st.synchronize()
     e_s, c_s, w_s, m_s = t_eos.mean(), t_co.mean(), t_ww.mean(), t_sm.mean()
-    sd_s, td_s = t_sd.mean(), t_td.mean()
+    sd_s, td_s, uv_s = t_sd.mean(), t_td.mean(), t_uv.mean()
     gb = lambda b, s: b / s / 1e9  # noqa: E731
     return {
         "workload": f"{d.imax}x{d.jmax}x{d.kmax} (arrays {d.NX}x{d.NY}x{d.kmax + 1})",
@@ -517,6 +531,7 @@ def extensions_section(torch, np, gt, inputs, dev, tr, ctx, st, steps):
         "salinity_diffusion": {"kernel": "k_vertical_diffusion<false>", "ms": sd_s * 1e3,
                                "bytes_per_cell": 32, "GBps": gb(32 * n3, sd_s),
                                "frac": 32 * n3 / sd_s / HBM_PEAK},
+        "uv_advect_3d_ms": uv_s * 1e3,
         "temperature_diffusion": {"kernel": "k_vertical_diffusion<true> (with short-wave)",
                                   "ms": td_s * 1e3, "bytes_per_cell": 40,
                                   "GBps": gb(40 * n3, td_s),

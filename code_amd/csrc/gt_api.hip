@@ -76,9 +76,13 @@ struct gt_ctx {
   double* vdw = nullptr;   // vertical diffusion work: ru, qu, reflected profile (3 fields)
   double* tri = nullptr;   // batched tridiagonal work (grown on demand)
   long tri_n = 0;
+  double* mwork = nullptr; // uv_advect_3d work: fadv uuadv vvadv wwadv huadv hvadv
 };
 
 extern "C" const char* gt_last_error(void) { return g_err; }
+
+// for the host-only translation units (gt_restart.cpp)
+int gt_internal_set_error(int code, const char* msg) { return set_err(code, "%s", msg); }
 extern "C" int gt_version(void) { return 1; }
 
 static int ensure_ws(gt_ctx* c, size_t bytes) {
@@ -223,7 +227,7 @@ extern "C" int gt_destroy(gt_ctx* c) {
   if (!c) return GT_OK;
   void* ps[] = {c->dmask, c->dint, c->fi, c->hi, c->adv, c->tmp, c->scr7, c->zz,
                 c->P, c->slice3, c->bpart, c->bout, c->nwarn, c->ws, c->dlev, c->gga,
-                c->dbad, c->vdw, c->tri};
+                c->dbad, c->vdw, c->tri, c->mwork};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   for (double* p : c->adj)
@@ -810,6 +814,76 @@ extern "C" int gt_ww_momentum_3d(gt_ctx* c, double dt, const double* uu, const d
     return set_err(GT_EARG, "gt_ww_momentum_3d: NULL argument");
   LCHK(gt::launch_ww_momentum(c->st, c->d, dt, c->dx, c->dy, c->ard1, c->az, uu, vv, ho, hn,
                               ww));
+  return GT_OK;
+}
+
+// ---- restart fields (gt_restart.cpp does the file format) ---------------------
+static int restart_host_copies(gt_ctx* c, int n, const gt_restart_field* f,
+                               std::vector<std::vector<double>>& host,
+                               std::vector<gt_restart_field>& hf, bool download) {
+  if (!c || n < 0 || (n > 0 && !f)) return set_err(GT_EARG, "restart: bad argument");
+  host.resize((size_t)n);
+  hf.resize((size_t)n);
+  const long N2 = c->d.NXY;
+  HIPCHK(hipStreamSynchronize(c->st));
+  for (int v = 0; v < n; v++) {
+    if (!f[v].data || (f[v].ndim != 2 && f[v].ndim != 3))
+      return set_err(GT_EARG, "restart: field %s: bad data or ndim", f[v].name ? f[v].name : "?");
+    const long cnt = f[v].ndim == 3 ? c->N3 : N2;
+    host[v].assign((size_t)cnt, 0.0);
+    if (download)
+      HIPCHK(hipMemcpy(host[v].data(), f[v].data, sizeof(double) * (size_t)cnt,
+                       hipMemcpyDeviceToHost));
+    hf[v] = f[v];
+    hf[v].data = host[v].data();
+  }
+  return GT_OK;
+}
+
+extern "C" int gt_write_restart(gt_ctx* c, const char* path, int with_halos, int loop, int n,
+                                const gt_restart_field* f) {
+  std::vector<std::vector<double>> host;
+  std::vector<gt_restart_field> hf;
+  TRY(restart_host_copies(c, n, f, host, hf, true));
+  return gt_restart_write_host(path, c->imin, c->imax, c->jmin, c->jmax, c->kmax, with_halos,
+                               loop, n, hf.data());
+}
+
+extern "C" int gt_read_restart(gt_ctx* c, const char* path, int with_halos, int* loop, int n,
+                               const gt_restart_field* f) {
+  std::vector<std::vector<double>> host;
+  std::vector<gt_restart_field> hf;
+  // start from the device arrays, so the cells the file does not hold stay
+  TRY(restart_host_copies(c, n, f, host, hf, true));
+  TRY(gt_restart_read_host(path, c->imin, c->imax, c->jmin, c->jmax, c->kmax, with_halos, loop,
+                           n, hf.data()));
+  for (int v = 0; v < n; v++)
+    HIPCHK(hipMemcpy(f[v].data, host[v].data(), sizeof(double) * host[v].size(),
+                     hipMemcpyHostToDevice));
+  return GT_OK;
+}
+
+// ---- a caller of the path: momentum advection --------------------------------
+extern "C" int gt_uv_advect_3d(gt_ctx* c, double dt, const double* uu, const double* vv,
+                               const double* ww, const double* ho, const double* hn,
+                               const double* hun, const double* hvn, int split, int hscheme,
+                               int vscheme, double* uuEx, double* vvEx) {
+  const void* ps[] = {uu, vv, ww, ho, hn, hun, hvn, uuEx, vvEx};
+  if (!c) return set_err(GT_EARG, "gt_uv_advect_3d: NULL context");
+  for (const void* p : ps)
+    if (!p) return set_err(GT_EARG, "gt_uv_advect_3d: NULL argument");
+  if (hscheme == GT_J7) return set_err(GT_ENOTIMPL, "uv_advect_3d: J7 is not in this build");
+  if (hscheme == GT_NOADV && vscheme == GT_NOADV) return GT_OK;
+  ENSURE(c->mwork, 6 * c->N3);
+  double* w[6];
+  for (int q = 0; q < 6; q++) w[q] = c->mwork + q * c->N3;
+  // U control volumes (uv_advect_3d.F90:62-176), then V (:236-300)
+  LCHK(gt::launch_uv_prep(c->st, c->d, true, uu, vv, ww, ho, hn, hun, hvn, c->mwork));
+  TRY(gt_do_advection_3d(c, dt, w[0], w[1], w[2], w[3], w[4], w[5], hun, hun, split, hscheme,
+                         vscheme, 0.0, GT_U_TAG, nullptr, uuEx));
+  LCHK(gt::launch_uv_prep(c->st, c->d, false, uu, vv, ww, ho, hn, hun, hvn, c->mwork));
+  TRY(gt_do_advection_3d(c, dt, w[0], w[1], w[2], w[3], w[4], w[5], hvn, hvn, split, hscheme,
+                         vscheme, 0.0, GT_V_TAG, nullptr, vvEx));
   return GT_OK;
 }
 

@@ -88,6 +88,10 @@ hipError_t launch_vertical_diffusion(hipStream_t st, Dims d, bool temperature, d
 hipError_t launch_tridiagonal(hipStream_t st, long n, int fi, int lt, const double* au,
                               const double* bu, const double* cu, const double* du,
                               double* value, double* work);
+// uv_advect_3d averaging (gt_momentum.hip); work holds 6 fields of N3
+hipError_t launch_uv_prep(hipStream_t st, Dims d, bool upart, const double* uu, const double* vv,
+                          const double* ww, const double* ho, const double* hn,
+                          const double* hun, const double* hvn, double* work);
 hipError_t launch_ip_bm(hipStream_t st, Dims d, const int32_t* az, const int32_t* au,
                         const int32_t* av, const double* H, const double* hn,
                         const double* hun, const double* hvn, const double* buoy, double* zz,

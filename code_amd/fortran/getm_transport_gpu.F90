@@ -16,6 +16,7 @@
 !   ww_momentum_3d_gpu     src/3d/ww_momentum_3d.F90:66-119
 !   salinity_diffusion_gpu, temperature_diffusion_gpu
 !                          src/3d/salinity.F90:346-393, temperature.F90:463-551
+!   uv_advect_3d_gpu       src/3d/uv_advect_3d.F90:8-466 (not J7)
 !
 ! Host arrays are copied to device mirrors owned by this module, the step
 ! runs on the GPU, and the outputs are copied back.  Callers that keep their
@@ -33,6 +34,7 @@
    public gt_malloc_f, gt_free_f, gt_to_device, gt_to_host
    public do_eqstate_gpu, start_macro_gpu, set_coordinates_gpu, coordinates_gpu
    public ww_momentum_3d_gpu, salinity_diffusion_gpu, temperature_diffusion_gpu
+   public uv_advect_3d_gpu
 
    integer, public, parameter :: NOSPLIT=0, FULLSPLIT=1, HALFSPLIT=2, HVSPLIT=3
    integer, public, parameter :: NOADV=0, UPSTREAM=1, UPSTREAM_2DH=2, P2=3
@@ -161,6 +163,14 @@
          import :: c_ptr, c_int, c_double
          type(c_ptr), value :: ctx, uu, vv, ho, hn, ww
          real(c_double), value :: dt
+         integer(c_int) :: rc
+      end function
+      function c_gt_uv_advect_3d(ctx, dt, uu, vv, ww, ho, hn, hun, hvn, split, hscheme, &
+               vscheme, uuEx, vvEx) bind(C, name='gt_uv_advect_3d') result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx, uu, vv, ww, ho, hn, hun, hvn, uuEx, vvEx
+         real(c_double), value :: dt
+         integer(c_int), value :: split, hscheme, vscheme
          integer(c_int) :: rc
       end function
       function c_gt_salinity_diffusion(ctx, dt, cnpar, hn, nuh, S) &
@@ -369,6 +379,17 @@
    type(c_ptr), intent(in) :: uu, vv, ho, hn, ww
    call check(c_gt_ww_momentum_3d(ctx, dt, uu, vv, ho, hn, ww), 'ww_momentum_3d()')
    end subroutine ww_momentum_3d_gpu
+
+   ! uv_advect_3d (uv_advect_3d.F90:8-466), device-resident; the namelist
+   ! values vel3d_adv_split/hor/ver of m3d are arguments
+   subroutine uv_advect_3d_gpu(dt, uu, vv, ww, ho, hn, hun, hvn, split, hscheme, vscheme, &
+                               uuEx, vvEx)
+   real(c_double), intent(in) :: dt
+   type(c_ptr), intent(in) :: uu, vv, ww, ho, hn, hun, hvn, uuEx, vvEx
+   integer, intent(in) :: split, hscheme, vscheme
+   call check(c_gt_uv_advect_3d(ctx, dt, uu, vv, ww, ho, hn, hun, hvn, split, hscheme, &
+              vscheme, uuEx, vvEx), 'uv_advect_3d()')
+   end subroutine uv_advect_3d_gpu
 
    ! the vertical diffusion of do_salinity (salinity.F90:346-393)
    subroutine salinity_diffusion_gpu(dt, cnpar, hn, nuh, S)

@@ -45,6 +45,8 @@ EXPORTS = [
     "gt_do_eqstate", "gt_rho_from_theta_unesco80", "gt_rho_from_theta", "gt_eosall_from_theta",
     "gt_start_macro", "gt_set_coordinates", "gt_coordinates", "gt_ww_momentum_3d",
     "gt_salinity_diffusion", "gt_temperature_diffusion", "gt_tridiagonal_batch",
+    "gt_uv_advect_3d", "gt_restart_write_host", "gt_restart_read_host", "gt_write_restart",
+    "gt_read_restart",
 ]
 
 
@@ -54,6 +56,11 @@ class BatchDomain(C.Structure):
                 ("uu", C.c_void_p), ("vv", C.c_void_p), ("ww", C.c_void_p),
                 ("hu", C.c_void_p), ("hv", C.c_void_p), ("ho", C.c_void_p),
                 ("hires", C.c_void_p)]
+
+
+class RestartField(C.Structure):
+    """gt_restart_field of include/getm_transport.h"""
+    _fields_ = [("name", C.c_char_p), ("ndim", C.c_int), ("data", C.c_void_p)]
 
 
 class GtError(RuntimeError):
@@ -104,6 +111,11 @@ def load_library():
         "gt_coordinates": (i, [vp, d, d, d] + [vp] * 16),
         "gt_ww_momentum_3d": (i, [vp, d, vp, vp, vp, vp, vp]),
         "gt_salinity_diffusion": (i, [vp, d, d, vp, vp, vp]),
+        "gt_uv_advect_3d": (i, [vp, d] + [vp] * 7 + [i, i, i, vp, vp]),
+        "gt_restart_write_host": (i, [C.c_char_p, i, i, i, i, i, i, i, i, vp]),
+        "gt_restart_read_host": (i, [C.c_char_p, i, i, i, i, i, i, vp, i, vp]),
+        "gt_write_restart": (i, [vp, C.c_char_p, i, i, i, vp]),
+        "gt_read_restart": (i, [vp, C.c_char_p, i, vp, i, vp]),
         "gt_temperature_diffusion": (i, [vp, d, d] + [vp] * 9 + [d, d, vp, vp]),
         "gt_tridiagonal_batch": (i, [vp, C.c_long, i, i, vp, vp, vp, vp, vp]),
         "gt_rho_from_theta_unesco80": (i, [vp, C.c_long, vp, vp, vp, i]),
@@ -151,6 +163,33 @@ def _ptr(t):
 def check(rc):
     if rc != 0:
         raise GtError(rc, load_library().gt_last_error().decode())
+
+
+def _restart_fields(fields, ptr):
+    """dict name -> array (kmax+1, NY, NX) or (NY, NX) -> gt_restart_field[]"""
+    arr = (RestartField * max(1, len(fields)))()
+    keep = []
+    for q, (name, a) in enumerate(fields.items()):
+        nm = name.encode()
+        keep.append(nm)
+        arr[q] = RestartField(nm, 3 if a.ndim == 3 else 2, ptr(a))
+    return arr, keep
+
+
+def restart_write_host(path, imin, imax, jmin, jmax, kmax, fields, loop=0, with_halos=False):
+    """gt_restart_write_host on C-contiguous float64 numpy arrays"""
+    arr, keep = _restart_fields(fields, lambda a: a.ctypes.data)
+    check(load_library().gt_restart_write_host(str(path).encode(), imin, imax, jmin, jmax, kmax,
+                                               int(with_halos), int(loop), len(fields), arr))
+
+
+def restart_read_host(path, imin, imax, jmin, jmax, kmax, fields, with_halos=False):
+    """gt_restart_read_host into numpy arrays; returns loop"""
+    arr, keep = _restart_fields(fields, lambda a: a.ctypes.data)
+    loop = C.c_int()
+    check(load_library().gt_restart_read_host(str(path).encode(), imin, imax, jmin, jmax, kmax,
+                                              int(with_halos), C.byref(loop), len(fields), arr))
+    return loop.value
 
 
 class Context:
@@ -286,6 +325,25 @@ class Context:
     def ww_momentum_3d(self, dt, uu, vv, ho, hn, ww):
         check(self.lib.gt_ww_momentum_3d(self.h, dt, _ptr(uu), _ptr(vv), _ptr(ho), _ptr(hn),
                                          _ptr(ww)))
+
+    # ---- restart fields (the NetCDF hotstart file) ----------------------------
+    def write_restart(self, path, fields, loop=0, with_halos=False):
+        arr, keep = _restart_fields(fields, lambda t: t.data_ptr())
+        check(self.lib.gt_write_restart(self.h, str(path).encode(), int(with_halos), int(loop),
+                                        len(fields), arr))
+
+    def read_restart(self, path, fields, with_halos=False):
+        arr, keep = _restart_fields(fields, lambda t: t.data_ptr())
+        loop = C.c_int()
+        check(self.lib.gt_read_restart(self.h, str(path).encode(), int(with_halos),
+                                       C.byref(loop), len(fields), arr))
+        return loop.value
+
+    def uv_advect_3d(self, dt, uu, vv, ww, ho, hn, hun, hvn, split, hscheme, vscheme, uuEx,
+                     vvEx):
+        check(self.lib.gt_uv_advect_3d(self.h, dt, *(_ptr(a) for a in (uu, vv, ww, ho, hn, hun,
+                                                                       hvn)),
+                                       split, hscheme, vscheme, _ptr(uuEx), _ptr(vvEx)))
 
     # ---- vertical diffusion (temperature.F90:463-551, salinity.F90:346-393) ----
     def salinity_diffusion(self, dt, cnpar, hn, nuh, S):

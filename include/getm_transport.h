@@ -159,6 +159,18 @@ int gt_ww_momentum_3d(gt_ctx *ctx, double dt, const double *uu,
                       const double *vv, const double *ho, const double *hn,
                       double *ww);
 
+/* ---- a caller of the path: momentum advection ------------------------------
+ * uv_advect_3d, src/3d/uv_advect_3d.F90:8-466, for every scheme but J7
+ * (GT_ENOTIMPL) and without the numerical analyses: u = uu/hun on the U
+ * control volumes, with the transports, ww and the layer heights averaged
+ * onto them, advected by do_advection_3d (tag U_TAG, AH = 0), advection term
+ * into uuEx; the same on the V volumes into vvEx.  The context keeps six
+ * work fields (the reference's automatic arrays). */
+int gt_uv_advect_3d(gt_ctx *ctx, double dt, const double *uu, const double *vv,
+                    const double *ww, const double *ho, const double *hn,
+                    const double *hun, const double *hvn, int split,
+                    int hscheme, int vscheme, double *uuEx, double *vvEx);
+
 /* ---- the step after advection: implicit vertical diffusion ---------------
  * The part of do_salinity / do_temperature that follows their
  * do_advection_3d call, on interior water columns (az == 1), kmax > 1:
@@ -188,6 +200,34 @@ int gt_temperature_diffusion(gt_ctx *ctx, double dt, double cnpar,
 int gt_tridiagonal_batch(gt_ctx *ctx, long n, int fi, int lt,
                          const double *au, const double *bu, const double *cu,
                          const double *du, double *value);
+
+/* ---- restart fields: the NetCDF hotstart file ---------------------------
+ * create_restart_ncdf.F90 / write_restart_ncdf.F90 / read_restart_ncdf.F90
+ * for the fields a caller names (e.g. hn, T, S, uu, vv, ww, ssen, sseo):
+ * NetCDF classic format written and read natively, dimensions xax, yax,
+ * zax = kmax+1, the integer "loop", FP64 variables over (x, y) or
+ * (x, y, z) in the reference's Fortran order, the title attribute.
+ * with_halos = 0 stores imin..imax x jmin..jmax (the default build), 1 the
+ * whole array (_WRITE_HOT_HALOS_).  64-bit offsets when the file exceeds
+ * 2 GiB.  Reading checks the dimensions and fills the same cells (halos of
+ * the arrays are kept when with_halos = 0); loop may be NULL. */
+typedef struct {
+  const char *name; /* NetCDF variable name, as the reference's */
+  int ndim;         /* 2: E2DFIELD, 3: I3DFIELD */
+  double *data;     /* the whole array, halos included */
+} gt_restart_field;
+/* data are host arrays */
+int gt_restart_write_host(const char *path, int imin, int imax, int jmin,
+                          int jmax, int kmax, int with_halos, int loop, int n,
+                          const gt_restart_field *fields);
+int gt_restart_read_host(const char *path, int imin, int imax, int jmin,
+                         int jmax, int kmax, int with_halos, int *loop, int n,
+                         const gt_restart_field *fields);
+/* data are device arrays of the context's domain (synchronises) */
+int gt_write_restart(gt_ctx *ctx, const char *path, int with_halos, int loop,
+                     int n, const gt_restart_field *fields);
+int gt_read_restart(gt_ctx *ctx, const char *path, int with_halos, int *loop,
+                    int n, const gt_restart_field *fields);
 
 /* ---- internal pressure ------------------------------------------------- */
 /* do_eqstate with eqstate_method=1 (linear), src/3d/eqstate.F90:127-149 and

@@ -1829,3 +1829,61 @@ void or_temperature_diffusion(const or_dom *d, const int32_t *az, double dt,
     }
 #undef C3
 }
+
+/* ---- a caller of the path: momentum advection ------------------------------
+ * uv_advect_3d, src/3d/uv_advect_3d.F90:60-300 (all schemes but J7, no
+ * numerical analyses): the velocity of the U (V) control volumes is
+ * advected by do_advection_3d with the transports, vertical velocity and
+ * layer heights averaged onto those volumes and tag U_TAG (V_TAG); the
+ * advection terms are returned in uuEx (vvEx).  The reference's work arrays
+ * are automatic, so the cells its loops leave unset hold whatever the stack
+ * held; here they start at zero, and the V part sees what the U part left,
+ * as in the reference. */
+int or_uv_advect_3d(const or_dom *d, const or_advgrids *g, double dt,
+                    const double *uu, const double *vv, const double *ww,
+                    const double *ho, const double *hn, const double *hun,
+                    const double *hvn, int split, int hscheme, int vscheme,
+                    int adv_ver_iterations, double *uuEx, double *vvEx)
+{
+    DOMVARS(d);
+    if (hscheme == OR_J7) return OR_EINVAL_SCHEME;
+    if (hscheme == OR_NOADV && vscheme == OR_NOADV) return OR_OK;
+    const long N3 = NXY * (kmax + 1);
+    double *w = (double *)calloc(6 * (size_t)N3, sizeof(double));
+    if (!w) return OR_ENOMEM;
+    double *fadv = w, *uuadv = w + N3, *vvadv = w + 2 * N3, *wwadv = w + 3 * N3,
+           *huadv = w + 4 * N3, *hvadv = w + 5 * N3;
+    for (long c = 0; c < NXY; c++) wwadv[c] = 0.0;
+    for (int k = 1; k <= kmax; k++)
+        for (long jj = 0; jj < NY; jj++)
+            for (long ii = 0; ii < NX - 1; ii++) {
+                const long c = ii + NX * jj + NXY * k;
+                fadv[c] = uu[c] / hun[c];
+                uuadv[c] = 0.5 * (uu[c] + uu[c + 1]);
+                vvadv[c] = 0.5 * (vv[c] + vv[c + 1]);
+                wwadv[c] = 0.5 * (ww[c] + ww[c + 1]);
+                huadv[c] = 0.5 * (ho[c + 1] + hn[c + 1]);
+                hvadv[c] = 0.5 * (hvn[c] + hvn[c + 1]);
+            }
+    int rc = or_do_advection_3d(d, g, dt, fadv, uuadv, vvadv, wwadv, huadv, hvadv, hun, hun,
+                                split, hscheme, vscheme, 0.0, OR_U_TAG,
+                                adv_ver_iterations, NULL, uuEx, NULL);
+    if (rc == OR_OK) {
+        for (int k = 1; k <= kmax; k++)
+            for (long jj = 0; jj < NY - 1; jj++)
+                for (long ii = 0; ii < NX; ii++) {
+                    const long c = ii + NX * jj + NXY * k;
+                    fadv[c] = vv[c] / hvn[c];
+                    uuadv[c] = 0.5 * (uu[c] + uu[c + NX]);
+                    vvadv[c] = 0.5 * (vv[c] + vv[c + NX]);
+                    wwadv[c] = 0.5 * (ww[c] + ww[c + NX]);
+                    huadv[c] = 0.5 * (hun[c] + hun[c + NX]);
+                    hvadv[c] = 0.5 * (ho[c + NX] + hn[c + NX]);
+                }
+        rc = or_do_advection_3d(d, g, dt, fadv, uuadv, vvadv, wwadv, huadv, hvadv, hvn, hvn,
+                                split, hscheme, vscheme, 0.0, OR_V_TAG,
+                                adv_ver_iterations, NULL, vvEx, NULL);
+    }
+    free(w);
+    return rc;
+}

@@ -202,6 +202,14 @@ void or_temperature_diffusion(const or_dom *d, const int32_t *az, double dt,
                               double swr_bot_refl_frac, double swr_min_bot_frac,
                               double *rad, double *T);
 
+/* a caller of the path: uv_advect_3d, src/3d/uv_advect_3d.F90:60-300
+ * (J7 and the numerical analyses excluded) */
+int or_uv_advect_3d(const or_dom *d, const or_advgrids *g, double dt,
+                    const double *uu, const double *vv, const double *ww,
+                    const double *ho, const double *hn, const double *hun,
+                    const double *hvn, int split, int hscheme, int vscheme,
+                    int adv_ver_iterations, double *uuEx, double *vvEx);
+
 /* OpenMP thread count used by the restatement (0 = runtime default) */
 void or_set_threads(int n);
 int or_get_threads(void);

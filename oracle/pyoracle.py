@@ -83,6 +83,7 @@ def lib():
             "or_tridiagonal": (None, [i, i, i, vp, vp, vp, vp, vp]),
             "or_salinity_diffusion": (None, [D, vp, d, d, vp, vp, vp]),
             "or_temperature_diffusion": (None, [D, vp, d, d] + [vp] * 9 + [d, d, vp, vp]),
+            "or_uv_advect_3d": (i, [D, G, d] + [vp] * 7 + [i, i, i, i, vp, vp]),
             "or_set_threads": (None, [i]),
             "or_get_threads": (i, []),
         }
@@ -254,6 +255,12 @@ class Oracle:
     def ww_momentum_3d(self, dt, uu, vv, ho, hn, ww):
         lib().or_ww_momentum_3d(C.byref(self.dom), P(self.az), dt, P(uu), P(vv), P(ho), P(hn),
                                 P(ww))
+
+    def uv_advect_3d(self, dt, uu, vv, ww, ho, hn, hun, hvn, split, hscheme, vscheme, uuEx,
+                     vvEx, adv_ver_iterations=1):
+        return lib().or_uv_advect_3d(C.byref(self.dom), C.byref(self.g), dt,
+                                     *(P(a) for a in (uu, vv, ww, ho, hn, hun, hvn)), split,
+                                     hscheme, vscheme, adv_ver_iterations, P(uuEx), P(vvEx))
 
     # ---- vertical diffusion (temperature.F90:463-551, salinity.F90:346-393) ----
     def salinity_diffusion(self, dt, cnpar, hn, nuh, S):
