@@ -6,12 +6,12 @@
 // One thread per water column.  The elimination runs down from the surface
 // (the reference's lt = kmax to fi = 1), forming each row of the
 // Crank-Nicolson matrix on the fly from a three-level window of the tracer
-// and the layer heights, and keeps the eliminated coefficients ru, qu in a
-// level-major work field (coalesced across the wavefront; the wavefront reads
-// them back at once, so they stay in L2); the substitution runs back up and
-// writes the tracer.  Expression order is the reference's (built with
-// -ffp-contract=off): bit-identical to the CPU build, except the short-wave
-// exponentials, which go through the device exp (within an ulp of libm's).
+// and the layer heights, and keeps the eliminated coefficients ru, qu in
+// registers for columns of up to 48 levels (in a level-major work field
+// beyond that); the substitution runs back up and writes the tracer.
+// Expression order is the reference's (built with -ffp-contract=off):
+// bit-identical to the CPU build, except the short-wave exponentials, which
+// go through the device exp (within an ulp of libm's).
 #include "gt_kernels.h"
 
 namespace gt {
@@ -19,6 +19,7 @@ namespace gt {
 namespace {
 
 constexpr int kBx = 64, kBy = 4;
+constexpr int kChunk = 4;  // levels whose loads are issued ahead of their arithmetic
 
 struct DiffArgs {
   Dims d;
@@ -36,7 +37,9 @@ struct DiffArgs {
   double refl_frac, min_bot_frac, rho_0_cpi;
 };
 
-template <bool TEMP>
+// KM = 0: ru, qu in the level-major work field (any kmax); KM > 0: in
+// registers (kmax <= KM), every level's loads independent of the arithmetic
+template <bool TEMP, int KM>
 __global__ void __launch_bounds__(kBx* kBy) k_vertical_diffusion(DiffArgs a) {
   const int ii = blockIdx.x * kBx + threadIdx.x + 2;
   const int jj = blockIdx.y * kBy + threadIdx.y + 2;
@@ -84,6 +87,72 @@ __global__ void __launch_bounds__(kBx* kBy) k_vertical_diffusion(DiffArgs a) {
     if (azc != 1 || kmax <= 1) return;
   }
   const double dt = a.dt, cnpar = a.cnpar, avm = a.avm, rc = a.rho_0_cpi;
+  if constexpr (KM > 0) {
+    // rows k = kmax (surface) .. 1 (bottom); carried: f(k+1), f(k), hn(k),
+    // auxo/auxn of interface k, the short-wave at k
+    double ruv[KM + 1], quv[KM + 1];
+    double fkp = 0.0, fk = 0.0, hk = 0.0, auxo_k = 0.0, auxn_k = 0.0, rk = 0.0;
+    double ru = 0.0, qu = 0.0;
+#pragma unroll
+    for (int k = KM; k >= 1; k--) {
+      // a guard, not continue/break: the loop must unroll fully so ruv/quv
+      // stay in registers
+      if (k <= kmax) {
+      if (k == kmax) {
+        hk = L(a.hn, k);
+        fk = L(a.f, k);
+        if (TEMP) rk = L(radsrc, k) * rc;
+      }
+      double hkm = 0.0, fkm = 0.0, auxo_m = 0.0, auxn_m = 0.0, rkm = 0.0;
+      if (k >= 2) {
+        hkm = L(a.hn, k - 1);
+        fkm = L(a.f, k - 1);
+        const double nk = L(a.nuh, k - 1);
+        auxo_m = 2.0 * (1 - cnpar) * dt * (nk + avm) / (hk + hkm);
+        auxn_m = 2.0 * cnpar * dt * (nk + avm) / (hk + hkm);
+      }
+      if (TEMP) rkm = L(radsrc, k - 1) * rc;
+      if (k == kmax) {
+        const double a1 = -auxn_m, a2 = hk + auxn_m;
+        double a4 = fk * (hk - auxo_m) + fkm * auxo_m;
+        if (TEMP) a4 = a4 + dzdt * (rk + shf_dz - rkm);
+        ru = a1 / a2;
+        qu = a4 / a2;
+      } else if (k >= 2) {
+        const double a3 = -auxn_k, a1 = -auxn_m;
+        const double a2 = hk + auxn_k + auxn_m;
+        double a4 = fkp * auxo_k + fk * (hk - auxo_k - auxo_m) + fkm * auxo_m;
+        if (TEMP) a4 = a4 + dzdt * (rk - rkm);
+        const double ru_n = a1 / (a2 - a3 * ru);
+        qu = (a4 - a3 * qu) / (a2 - a3 * ru);
+        ru = ru_n;
+      } else {
+        const double a3 = -auxn_k, a2 = hk + auxn_k;
+        double a4 = fkp * auxo_k + fk * (hk - auxo_k);
+        if (TEMP) a4 = a4 + dzdt * (rk - rkm);
+        qu = (a4 - a3 * qu) / (a2 - a3 * ru);
+      }
+      ruv[k] = ru;
+      quv[k] = qu;
+      fkp = fk;
+      fk = fkm;
+      hk = hkm;
+      auxo_k = auxo_m;
+      auxn_k = auxn_m;
+      rk = rkm;
+      }
+    }
+    double v = quv[1];
+    L(a.f, 1) = v;
+#pragma unroll
+    for (int k = 2; k <= KM; k++) {
+      if (k <= kmax) {
+        v = quv[k] - ruv[k] * v;
+        L(a.f, k) = v;
+      }
+    }
+    return;
+  }
   // the surface row, k = kmax (auxo/auxn of interface kmax-1)
   double hk = L(a.hn, kmax), hkm = L(a.hn, kmax - 1);
   double fk = L(a.f, kmax), fkm = L(a.f, kmax - 1);
@@ -101,48 +170,74 @@ __global__ void __launch_bounds__(kBx* kBy) k_vertical_diffusion(DiffArgs a) {
   double ru = a1 / a2, qu = a4 / a2;
   L(a.ru, kmax) = ru;
   L(a.qu, kmax) = qu;
-  // down the column: row k uses interfaces k (carried) and k-1
-  for (int k = kmax - 1; k >= 1; k--) {
-    const double auxo_k = auxo_m, auxn_k = auxn_m;
-    const double fkp = fk;
-    fk = fkm;
-    hk = hkm;
-    if (TEMP) rk = rkm;
-    if (k >= 2) {
-      hkm = L(a.hn, k - 1);
-      fkm = L(a.f, k - 1);
-      nk = L(a.nuh, k - 1);
-      auxo_m = 2.0 * (1 - cnpar) * dt * (nk + avm) / (hk + hkm);
-      auxn_m = 2.0 * cnpar * dt * (nk + avm) / (hk + hkm);
-      if (TEMP) rkm = L(radsrc, k - 1) * rc;
-      const double a3 = -auxn_k;
-      a1 = -auxn_m;
-      a2 = hk + auxn_k + auxn_m;
-      a4 = fkp * auxo_k + fk * (hk - auxo_k - auxo_m) + fkm * auxo_m;
-      if (TEMP) a4 = a4 + dzdt * (rk - rkm);
-      const double den = a2 - a3 * ru;
-      const double ru_n = a1 / den;
-      qu = (a4 - a3 * qu) / (a2 - a3 * ru);
-      ru = ru_n;
-      (void)den;
-      L(a.ru, k) = ru;
-      L(a.qu, k) = qu;
-    } else {
-      // the bottom row, k = fi = 1
-      if (TEMP) rkm = L(radsrc, 0) * rc;
-      const double a3 = -auxn_k;
-      a2 = hk + auxn_k;
-      a4 = fkp * auxo_k + fk * (hk - auxo_k);
-      if (TEMP) a4 = a4 + dzdt * (rk - rkm);
-      qu = (a4 - a3 * qu) / (a2 - a3 * ru);
+  // down the column: row k uses interfaces k (carried) and k-1.  The
+  // level-(k-1) inputs of four rows are loaded before their arithmetic.
+  for (int k0 = kmax - 1; k0 >= 1; k0 -= kChunk) {
+    double hv[kChunk], fv[kChunk], nv[kChunk], rv[kChunk];
+#pragma unroll
+    for (int u = 0; u < kChunk; u++) {
+      const int k = k0 - u;
+      if (k >= 2) {
+        hv[u] = L(a.hn, k - 1);
+        fv[u] = L(a.f, k - 1);
+        nv[u] = L(a.nuh, k - 1);
+      }
+      if (TEMP && k >= 1) rv[u] = L(radsrc, k - 1);
+    }
+#pragma unroll
+    for (int u = 0; u < kChunk; u++) {
+      const int k = k0 - u;
+      if (k < 1) break;
+      const double auxo_k = auxo_m, auxn_k = auxn_m;
+      const double fkp = fk;
+      fk = fkm;
+      hk = hkm;
+      if (TEMP) rk = rkm;
+      if (k >= 2) {
+        hkm = hv[u];
+        fkm = fv[u];
+        nk = nv[u];
+        auxo_m = 2.0 * (1 - cnpar) * dt * (nk + avm) / (hk + hkm);
+        auxn_m = 2.0 * cnpar * dt * (nk + avm) / (hk + hkm);
+        if (TEMP) rkm = rv[u] * rc;
+        const double a3 = -auxn_k;
+        a1 = -auxn_m;
+        a2 = hk + auxn_k + auxn_m;
+        a4 = fkp * auxo_k + fk * (hk - auxo_k - auxo_m) + fkm * auxo_m;
+        if (TEMP) a4 = a4 + dzdt * (rk - rkm);
+        const double ru_n = a1 / (a2 - a3 * ru);
+        qu = (a4 - a3 * qu) / (a2 - a3 * ru);
+        ru = ru_n;
+        L(a.ru, k) = ru;
+        L(a.qu, k) = qu;
+      } else {
+        // the bottom row, k = fi = 1
+        if (TEMP) rkm = rv[u] * rc;
+        const double a3 = -auxn_k;
+        a2 = hk + auxn_k;
+        a4 = fkp * auxo_k + fk * (hk - auxo_k);
+        if (TEMP) a4 = a4 + dzdt * (rk - rkm);
+        qu = (a4 - a3 * qu) / (a2 - a3 * ru);
+      }
     }
   }
   // back up: value(1) = qu(1), value(k) = qu(k) - ru(k) value(k-1)
   double v = qu;
   L(a.f, 1) = v;
-  for (int k = 2; k <= kmax; k++) {
-    v = L(a.qu, k) - L(a.ru, k) * v;
-    L(a.f, k) = v;
+  for (int k0 = 2; k0 <= kmax; k0 += kChunk) {
+    double qv[kChunk], rw[kChunk];
+#pragma unroll
+    for (int u = 0; u < kChunk; u++)
+      if (k0 + u <= kmax) {
+        qv[u] = L(a.qu, k0 + u);
+        rw[u] = L(a.ru, k0 + u);
+      }
+#pragma unroll
+    for (int u = 0; u < kChunk; u++)
+      if (k0 + u <= kmax) {
+        v = qv[u] - rw[u] * v;
+        L(a.f, k0 + u) = v;
+      }
   }
 #undef L
 }
@@ -214,10 +309,23 @@ hipError_t launch_vertical_diffusion(hipStream_t st, Dims d, bool temperature, d
   a.rho_0_cpi = 1.0 / ((double)1025.f * (double)3985.f);
   a.avm = temperature ? (double)1.4e-7f : (double)1.1e-9f;
   const dim3 g((unsigned)((d.NX - 4 + kBx - 1) / kBx), (unsigned)((d.NY - 4 + kBy - 1) / kBy));
-  if (temperature)
-    hipLaunchKernelGGL(k_vertical_diffusion<true>, g, dim3(kBx, kBy), 0, st, a);
+  // the smallest register variant that holds the column, else the work field
+#define GT_VD(KM)                                                                  \
+  do {                                                                             \
+    if (temperature)                                                               \
+      hipLaunchKernelGGL((k_vertical_diffusion<true, KM>), g, dim3(kBx, kBy), 0, st, a); \
+    else                                                                           \
+      hipLaunchKernelGGL((k_vertical_diffusion<false, KM>), g, dim3(kBx, kBy), 0, st, a); \
+  } while (0)
+  if (d.kmax <= 16)
+    GT_VD(16);
+  else if (d.kmax <= 32)
+    GT_VD(32);
+  else if (d.kmax <= 48)
+    GT_VD(48);
   else
-    hipLaunchKernelGGL(k_vertical_diffusion<false>, g, dim3(kBx, kBy), 0, st, a);
+    GT_VD(0);
+#undef GT_VD
   return hipGetLastError();
 }
 

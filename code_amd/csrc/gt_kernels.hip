@@ -431,7 +431,7 @@ __global__ void k_eos_linear(long n, long off, const double* __restrict__ T,
 // a9: ip_blumberg_mellor, src/3d/ip_blumberg_mellor.F90:67-150.
 // zz: interface heights over [imin..imax+1]x[jmin..jmax+1] where az>=1.
 __global__ void k_bm_zz(Dims d, const int32_t* __restrict__ az, const double* __restrict__ H,
-                        const double* __restrict__ hn, double* zz) {
+                        const double* __restrict__ hn, double* __restrict__ zz) {
   int ii = blockIdx.x * blockDim.x + threadIdx.x;
   int jj = blockIdx.y * blockDim.y + threadIdx.y;
   if (ii >= d.NX || jj >= d.NY) return;
@@ -453,7 +453,8 @@ __global__ void k_bm_zz(Dims d, const int32_t* __restrict__ az, const double* __
 __global__ void k_bm_idpd(Dims d, const int32_t* __restrict__ au, const int32_t* __restrict__ av,
                           const double* __restrict__ buoy, const double* __restrict__ zz,
                           const double* __restrict__ hun, const double* __restrict__ hvn,
-                          double* idpdx, double* idpdy, double dxm1, double dym1) {
+                          double* __restrict__ idpdx, double* __restrict__ idpdy, double dxm1,
+                          double dym1) {
   int ii = blockIdx.x * blockDim.x + threadIdx.x;
   int jj = blockIdx.y * blockDim.y + threadIdx.y;
   if (ii >= d.NX || jj >= d.NY) return;
@@ -524,7 +525,7 @@ __device__ __forceinline__ double sm_hmR(double a, double b) {
 }
 __global__ void k_sm_column(Dims d, const int32_t* __restrict__ az, const double* __restrict__ H,
                             const double* __restrict__ hn, const double* __restrict__ buoy,
-                            double* zz, double* P) {
+                            double* __restrict__ zz, double* __restrict__ P) {
   int ii = blockIdx.x * blockDim.x + threadIdx.x;
   int jj = blockIdx.y * blockDim.y + threadIdx.y;
   if (ii >= d.NX || jj >= d.NY) return;
@@ -533,44 +534,50 @@ __global__ void k_sm_column(Dims d, const int32_t* __restrict__ az, const double
   long c0 = ii + (long)d.NX * jj;
   zz[c0] = 0.0;
   if (az[c0] < 1) return;
+#define AT(a, k) (a)[c0 + NXY * (long)(k)]
+  double zt, zm;  // zz at kmax and kmax-1, kept for the second loop
   {
-    double z = -H[c0] + 0.5 * hn[c0 + NXY];
-    zz[c0 + NXY] = z;
-    double hprev = hn[c0 + NXY];
+    double z = -H[c0] + 0.5 * AT(hn, 1);
+    AT(zz, 1) = z;
+    double hprev = AT(hn, 1);
+    zm = z;
 #pragma unroll 4
     for (int k = 2; k <= kmax; k++) {
-      const double hk = hn[c0 + NXY * k];
+      const double hk = AT(hn, k);
+      zm = z;
       z = z + 0.5 * (hprev + hk);
-      zz[c0 + NXY * k] = z;
+      AT(zz, k) = z;
       hprev = hk;
     }
+    zt = z;
   }
   if (ii < 2 || ii > d.NX - 2 || jj < 2 || jj > d.NY - 2) return;
-#define AT(a, k) (a)[c0 + NXY * (long)(k)]
-  // raw differences: rawR(k) = b(k+1)-b(k) for k in 1..kmax-1,
-  // rawR(kmax) = rawR(kmax-1), rawR(0) = rawR(1)
-  auto rawR = [&](int k) -> double {
-    int kk = k < 1 ? 1 : (k > kmax - 1 ? kmax - 1 : k);
-    return AT(buoy, kk + 1) - AT(buoy, kk);
-  };
-  auto rawZ = [&](int k) -> double {
-    int kk = k < 1 ? 1 : (k > kmax - 1 ? kmax - 1 : k);
-    return AT(zz, kk + 1) - AT(zz, kk);
-  };
+  // vertical differences dR(k) = b(k+1)-b(k), dZ likewise, for k in
+  // 1..kmax-1, extended by dR(kmax) = dR(kmax-1) and dR(0) = dR(1) (:108-121);
+  // the window b(k+1), b(k), zz(k+1), zz(k) moves down in registers
   auto hmZ = [&](double a, double b) -> double { return 2.0 * a * b / (a + b); };
-  double cff1 = 1.0 / (AT(zz, kmax) - AT(zz, kmax - 1));
-  double cff2 = 0.5 * (AT(buoy, kmax) - AT(buoy, kmax - 1)) * 0.5 * AT(hn, kmax) * cff1;
-  double p = (AT(buoy, kmax) + cff2) * 0.5 * AT(hn, kmax);
+  double b1 = AT(buoy, kmax), b0 = AT(buoy, kmax - 1);
+  double z1 = zt, z0 = zm;
+  const double ht = AT(hn, kmax);
+  double cff1 = 1.0 / (z1 - z0);
+  double cff2 = 0.5 * (b1 - b0) * 0.5 * ht * cff1;
+  double p = (b1 + cff2) * 0.5 * ht;
   AT(P, kmax) = p;
-  double rk1 = rawR(kmax), zk1 = rawZ(kmax);      // raw at k+1
-  double rk = rawR(kmax - 1), zk = rawZ(kmax - 1);  // raw at k
-  double hR1 = sm_hmR(rk1, rk), hZ1 = hmZ(zk1, zk);  // harmonic at kmax
+  double rk = b1 - b0, zk = z1 - z0;             // dR, dZ at kmax-1 (= at kmax)
+  double hR1 = sm_hmR(rk, rk), hZ1 = hmZ(zk, zk);  // harmonic means at kmax
 #pragma unroll 4
   for (int k = kmax - 1; k >= 1; k--) {
-    double rkm = rawR(k - 1), zkm = rawZ(k - 1);
+    double bm = 0.0, zmv = 0.0, rkm, zkm;
+    if (k >= 2) {
+      bm = AT(buoy, k - 1);
+      zmv = AT(zz, k - 1);
+      rkm = b0 - bm;
+      zkm = z0 - zmv;
+    } else {
+      rkm = b1 - b0;  // dR(0) = dR(1)
+      zkm = z1 - z0;
+    }
     double hR = sm_hmR(rk, rkm), hZ = hmZ(zk, zkm);
-    double b1 = AT(buoy, k + 1), b0 = AT(buoy, k);
-    double z1 = AT(zz, k + 1), z0 = AT(zz, k);
     p = p + 0.5 * ((b1 + b0) * (z1 - z0) -
                    SM_ONEFIFTH * ((hR1 - hR) * (z1 - z0 - SM_C12 * (hZ1 + hZ)) -
                                   (hZ1 - hZ) * (b1 - b0 - SM_C12 * (hR1 + hR))));
@@ -579,6 +586,10 @@ __global__ void k_sm_column(Dims d, const int32_t* __restrict__ az, const double
     hZ1 = hZ;
     rk = rkm;
     zk = zkm;
+    b1 = b0;
+    b0 = bm;
+    z1 = z0;
+    z0 = zmv;
   }
 #undef AT
 }
@@ -603,7 +614,8 @@ __global__ __launch_bounds__(256) void k_sm_idpd_strip(
     Dims d, int nsx, int nsy, const int32_t* __restrict__ au, const int32_t* __restrict__ av,
     const double* __restrict__ buoy, const double* __restrict__ zz, const double* __restrict__ P,
     const double* __restrict__ hun, const double* __restrict__ hvn,
-    const double* __restrict__ sseo, double* idpdx, double* idpdy, double dxm1, double dym1) {
+    const double* __restrict__ sseo, double* __restrict__ idpdx, double* __restrict__ idpdy,
+    double dxm1, double dym1) {
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const long item = (long)blockIdx.x * 4 + wave;
   const long nitems = (long)nsx * nsy * (d.kmax + 1);
