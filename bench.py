@@ -34,6 +34,9 @@ if ROOT not in sys.path:
 
 HBM_PEAK = 8.0e12
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "strip_kernels_hbm_traffic.txt")
+VALU_FILE = os.path.join(ROOT, "profiles", "strip_kernels_valu_issue.txt")
+# wave64 VALU instructions per second: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles
+VALU_PEAK = 256 * 4 * 2.4e9 / 4
 
 
 def parse_rows(v):
@@ -76,6 +79,33 @@ def measured_traffic(section, rows, group=None):
         write = float(r["uv_write_kib"]) + float(r["wvu_write_kib"])
         return fetch * 1024 * cf + write * 1024 * cw
     return None
+def valu_issue(section, rows, group, seconds):
+    """VALU issue rate of one advection step (both strip kernels): the
+    wavefront instructions the committed rocprofv3 SQ_INSTS_VALU summary
+    counts per dispatch, over this run's HIP-event duration of the pair;
+    None when the summary does not cover this strip height / tracer group."""
+    try:
+        lines = open(VALU_FILE).read().splitlines()
+    except OSError:
+        return None
+    insts = 0.0
+    found = set()
+    for ln in lines:
+        if ln.startswith("#") or not ln.strip():
+            continue
+        head, *kv = ln.split()
+        r = dict(x.split("=") for x in kv)
+        if head != section or parse_rows(r["rows"]) != tuple(rows) or int(r["group"]) != group:
+            continue
+        insts += float(r["valu_insts"])
+        found.add(r["kernel"])
+    if found != {"uv", "wvu"}:
+        return None
+    return {"insts_per_step": insts, "achieved": insts / seconds / 1e9,
+            "peak": VALU_PEAK / 1e9, "unit": "G wave-instr/s",
+            "frac": insts / seconds / VALU_PEAK}
+
+
 METRIC = "tracer cell-updates/s + % HBM peak; batched FP32 vs plain FP64 at <=1e-5 err"
 
 
@@ -266,6 +296,7 @@ def main():
                                       "tracer(s) per wave> (one advection step, two launches)",
             "achieved": b_adv / adv_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": b_adv / adv_s / HBM_PEAK, "traffic": measured_traffic("headline", rows64, g64),
+            "valu_issue": valu_issue("headline", rows64, g64, adv_s),
             "bytes_per_launch": b_adv, "launch_us": adv_s * 1e6,
             "bytes_per_cell_tracer": b_adv / (ncell * ntr)}
     # EOS (read T S, write buoy) + two internal-pressure methods (read buoy hn
@@ -421,6 +452,7 @@ def batched(torch, np, gt, inputs, dev, a, fp64_rate):
                      "achieved": bytes_step / kern / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": bytes_step / kern / HBM_PEAK,
                      "traffic": measured_traffic("batched", rows32, group),
+                     "valu_issue": valu_issue("batched", rows32, group, kern),
                      "bytes_per_launch": bytes_step},
         "vs_fp64_advection": rate / fp64_rate,
         "err_vs_fp64": {"max": emax, "rms": erms, "steps": nsteps + 2,
