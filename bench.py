@@ -329,18 +329,29 @@ def main():
         "roofline": roof,
     }
     del TS2, idpdx6, idpdy6
+    progress(f"headline done: {ms:.4f} ms/step")
     if not a.no_batched:
         out["batched_fp32"] = batched(torch, np, gt, inputs, dev, a, ncell * ntr / adv_s)
+        progress("batched section done")
     if a.shard:
         out["shard"] = shard_section(torch, np, gt, inputs, dev, a.batched_steps)
+        progress("shard section done")
     if not a.no_ext:
         out["extensions"] = extensions_section(torch, np, gt, inputs, dev, tr, ctx, st,
                                                a.steps)
+        progress("extensions done")
     if not a.no_adjoint:
         out["adjoint"] = adjoint_section(torch, np, gt, inputs, dev)
+        progress("adjoint done")
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(tr, a.cpu_seconds)
+        progress("cpu baseline done")
     print(json.dumps(out))
+
+
+def progress(msg):
+    """one line per finished section on stderr (the JSON line is on stdout)"""
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
 
 
 class _Null:
@@ -546,8 +557,43 @@ def extensions_section(torch, np, gt, inputs, dev, tr, ctx, st, steps):
             diffusion(True)
             momentum(True)
         st.synchronize()
+    # the whole macro step on the device, one stream, in integrate_3d's order
+    # (m3d.F90:446-607): start_macro, coordinates, ww_momentum_3d, T/S
+    # advection (strip kernels), salinity and temperature diffusion, linear
+    # EOS, Blumberg-Mellor and Shchepetkin-McWilliams
+    TS = torch.stack([T, S])
+    TS2 = torch.empty_like(TS)
+    hires = torch.zeros_like(T)
+    zz, zz6, ix, iy, ix6, iy6 = (torch.zeros_like(T) for _ in range(6))
+    consts = (inputs.EOS_T0, inputs.EOS_S0, inputs.EOS_DTR0, inputs.EOS_DSR0)
+    t_full = EventTimer(torch, st)
+
+    def full_step(cur, nxt):
+        producers(False)
+        gt.adv3d_batch([dict(ctx=ctx, f_in=cur, f_out=nxt, uu=uu, vv=vv, ww=ww, hu=h3["hun"],
+                             hv=h3["hvn"], ho=h3["ho"], hires=hires)], 2, tr.dt, gt.P2_PDM,
+                       gt.P2_PDM, stream=st)
+        ctx.salinity_diffusion(tr.dt, 0.5, h3["hn"], nuh, nxt[1])
+        ctx.temperature_diffusion(tr.dt, 0.5, h3["hn"], nuh, swr, shf, Aa, g1, g2, dry, nxt[1],
+                                  rad, nxt[0])
+        ctx.eos_linear(nxt[0], nxt[1], None, buoy, *consts)
+        ctx.ip_blumberg_mellor(H, h3["hn"], h3["hun"], h3["hvn"], buoy, zz, ix, iy)
+        ctx.ip_shchepetkin_mcwilliams(H, h3["hn"], h3["hun"], h3["hvn"], buoy, e2["sseo"], zz6,
+                                      ix6, iy6)
+
+    bufs = [TS, TS2]
+    with torch.cuda.stream(st):
+        for _ in range(2):
+            full_step(*bufs)
+            bufs.reverse()
+        for _ in range(steps):
+            with t_full:
+                full_step(*bufs)
+            bufs.reverse()
+        st.synchronize()
     e_s, c_s, w_s, m_s = t_eos.mean(), t_co.mean(), t_ww.mean(), t_sm.mean()
     sd_s, td_s, uv_s = t_sd.mean(), t_td.mean(), t_uv.mean()
+    full_s = t_full.mean()
     gb = lambda b, s: b / s / 1e9  # noqa: E731
     return {
         "workload": f"{d.imax}x{d.jmax}x{d.kmax} (arrays {d.NX}x{d.NY}x{d.kmax + 1})",
@@ -564,6 +610,10 @@ def extensions_section(torch, np, gt, inputs, dev, tr, ctx, st, steps):
                                "bytes_per_cell": 32, "GBps": gb(32 * n3, sd_s),
                                "frac": 32 * n3 / sd_s / HBM_PEAK},
         "uv_advect_3d_ms": uv_s * 1e3,
+        "full_step_ms": full_s * 1e3,
+        "full_step": "start_macro, general coordinates, ww_momentum_3d, P2-PDM advection of "
+                     "T and S, salinity and temperature diffusion, linear EOS, Blumberg-Mellor, "
+                     "Shchepetkin-McWilliams: one stream, FP64",
         "temperature_diffusion": {"kernel": "k_vertical_diffusion<true> (with short-wave)",
                                   "ms": td_s * 1e3, "bytes_per_cell": 40,
                                   "GBps": gb(40 * n3, td_s),

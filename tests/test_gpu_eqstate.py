@@ -12,6 +12,7 @@ from code_amd import inputs
 from code_amd import transport as gt
 from oracle import pyoracle as po
 from tests.test_gpu_parity import random_domain
+from tests.test_oracle_eqstate import GOLDEN
 from tests.util import rel_err, to_dev, to_host
 
 pytestmark = pytest.mark.gpu
@@ -27,7 +28,8 @@ def test_point_functions_bit_exact(device):
     th = rng.uniform(-2.0, 32.0, n)
     p = rng.uniform(0.0, 6000.0, n)
     p[::7] = 0.0
-    s[0], th[0], p[0] = 20.0, 20.0, 1000.0  # the reference's check value
+    g1 = GOLDEN["rho_from_theta"][0]
+    s[0], th[0], p[0] = g1["s"], g1["th"], g1["p"]  # the reference's check value
     ds, dth, dp = (to_dev(a, device) for a in (s, th, p))
     o1 = torch.empty_like(ds)
     o2 = torch.empty_like(ds)
@@ -35,12 +37,13 @@ def test_point_functions_bit_exact(device):
     got0, gotp = to_host(o1), to_host(o2)
     want = np.array([po.rho_from_theta(*a) for a in zip(s, th, p)])
     assert np.array_equal(got0, want[:, 0]) and np.array_equal(gotp, want[:, 1])
-    assert abs(gotp[0] - 1017.728868019642) <= 1e-12 * 1017.7
+    assert abs(gotp[0] - g1["densp"]) <= g1["tol_rel"] * g1["densp"]
     ctx.eosall_from_theta(ds, dth, dp, o1, o2)
     want = np.array([po.eosall_from_theta(*a) for a in zip(s, th, p)])
     assert np.array_equal(to_host(o1), want[:, 0]) and np.array_equal(to_host(o2), want[:, 1])
-    assert abs(to_host(o1)[0] - 0.7510471164699279) <= 1e-14
-    assert abs(to_host(o2)[0] + 0.2570255211349140) <= 1e-14
+    g2 = GOLDEN["eosall_from_theta"][0]
+    assert abs(to_host(o1)[0] - g2["rho_s"]) <= g2["tol_abs"]
+    assert abs(to_host(o2)[0] - g2["rho_th"]) <= g2["tol_abs"]
     for nonneg in (False, True):
         s2 = s.copy()
         s2[::5] = -rng.uniform(0.0, 2.0, len(s2[::5]))

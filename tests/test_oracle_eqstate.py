@@ -18,6 +18,9 @@
   interface means, land columns, kmax, the linear fill) against those pinned
   point functions on hand-built columns.
 """
+import json
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -25,18 +28,23 @@ from code_amd import inputs
 from oracle import pyoracle as po
 
 
+GOLDEN = json.loads((Path(__file__).parent / "golden" / "eqstate_check_values.json").read_text())
+
+
 def test_jackett_reference_check_values():
-    dens0, densp = po.rho_from_theta(20.0, 20.0, 1000.0)
-    assert abs(densp - 1017.728868019642) <= 1e-12 * 1017.7
-    assert dens0 < densp  # compression raises the density
-    rho_s, rho_th = po.eosall_from_theta(20.0, 20.0, 1000.0)
-    assert abs(rho_s - 0.7510471164699279) <= 1e-14
-    assert abs(rho_th - (-0.2570255211349140)) <= 1e-14
+    for g in GOLDEN["rho_from_theta"]:
+        dens0, densp = po.rho_from_theta(g["s"], g["th"], g["p"])
+        assert abs(densp - g["densp"]) <= g["tol_rel"] * g["densp"], g["cite"]
+        assert dens0 < densp  # compression raises the density
+    for g in GOLDEN["eosall_from_theta"]:
+        rho_s, rho_th = po.eosall_from_theta(g["s"], g["th"], g["p"])
+        assert abs(rho_s - g["rho_s"]) <= g["tol_abs"], g["cite"]
+        assert abs(rho_th - g["rho_th"]) <= g["tol_abs"], g["cite"]
 
 
 def test_unesco80_published_values():
-    assert abs(po.rho_unesco80(5.0, 35.0) - 1027.67547) <= 2e-5
-    assert abs(po.rho_unesco80(25.0, 35.0) - 1023.34306) <= 2e-5
+    for g in GOLDEN["rho_from_theta_unesco80"]:
+        assert abs(po.rho_unesco80(g["T"], g["S"]) - g["rho"]) <= g["tol_abs"], g
     T = np.linspace(3.9, 4.1, 2001)
     r = np.array([po.rho_unesco80(t, 0.0) for t in T])
     assert abs(T[r.argmax()] - 3.98) <= 0.01
