@@ -80,12 +80,13 @@ def build_fortran() -> Path | None:
     obj = fb / "getm_transport_gpu.o"
     _run([flang, "-cpp", "-O2", "-fPIC", "-c", str(src), "-o", str(obj),
           "-module-dir", str(fb)])
-    drv = PKG / "fortran" / "adv_driver.F90"
-    if drv.exists():
-        exe = fb / "adv_driver"
-        _run([flang, "-cpp", "-O2", str(drv), str(obj), "-I", str(fb),
-              "-module-dir", str(fb), "-L", str(BUILD), "-lgetm_transport",
-              f"-Wl,-rpath,{BUILD}", "-Wl,-rpath,$ORIGIN/..", "-o", str(exe)])
+    for name in ("adv_driver", "rows_driver"):
+        drv = PKG / "fortran" / f"{name}.F90"
+        if drv.exists():
+            exe = fb / name
+            _run([flang, "-cpp", "-O2", str(drv), str(obj), "-I", str(fb),
+                  "-module-dir", str(fb), "-L", str(BUILD), "-lgetm_transport",
+                  f"-Wl,-rpath,{BUILD}", "-Wl,-rpath,$ORIGIN/..", "-o", str(exe)])
     return obj
 
 
