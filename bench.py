@@ -689,47 +689,55 @@ def adjoint_section(torch, np, gt, inputs, dev, nsteps=48, ckpt=8):
 
 
 def shard_section(torch, np, gt, inputs, dev, steps):
-    """One GPU's share of the 2000x1500x50 domain split 2x4: a 1000x375x50 GVC
-    subdomain (ghost cells filled locally) with T, S and 16 passive tracers in
-    FP32, the batched step autotuned and timed."""
-    tr = inputs.config_advection(1000, 375, 50, 5, hmin=10.0, hmax=50.0, coords="gvc", umax=1.0,
-                                 modes=inputs.seeded_modes(5))
-    d = tr.dom
-    T, S = inputs.temperature_salinity(tr, 5)
+    """One GPU's share of the tidal 2000x1500x50 case split 2x4: a 1000x375x50
+    subdomain on general coordinates, an M2 tide (30 micro steps of a
+    stand-in 2-D mode per 300 s macro step), the input producers on the device
+    (start_macro, coordinates, layer transports, ww_momentum_3d) and T, S and
+    16 passive tracers in FP32 advected by the batched step (autotuned);
+    HIP events on the stream for the producers and the advection, the FP64
+    mass drift of every tracer over the timed steps."""
+    from code_amd import tidal
+    d = inputs.make_domain(1000, 375, 50, 5, hmin=10.0, hmax=50.0)
     st = torch.cuda.Stream(device=dev)
     ctx = gt.Context.from_domain(d, stream=st)
-    fin = torch.empty((18,) + d.shape3, dtype=torch.float32, device=dev)
-    fin[0] = torch.from_numpy(T).to(dev)
-    fin[1] = torch.from_numpy(S).to(dev)
-    del T, S
-    for q, b in enumerate(inputs.passive_blobs(d, 16, 5)):
-        fin[2 + q] = torch.from_numpy(b).to(dev)
-    fout = torch.empty_like(fin)
-    D = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)  # noqa: E731
-    dd = dict(ctx=ctx, uu=D(tr.uu), vv=D(tr.vv), ww=D(tr.ww), hu=D(tr.hun), hv=D(tr.hvn),
-              ho=D(tr.ho), hires=torch.zeros(d.shape3, dtype=torch.float64, device=dev))
-    bt, bg, bms, _ = gt.autotune_batch([dict(dd, f_in=fin, f_out=fout)], 18, tr.dt, gt.P2_PDM,
-                                       gt.P2_PDM, reps=3, stream=st)
-    bufs = [fin, fout]
-    timer = EventTimer(torch, st)
+    nt = 18
+    hires = torch.zeros(d.shape3, dtype=torch.float64, device=dev)
+    t_prod, t_adv = EventTimer(torch, st), EventTimer(torch, st)
     with torch.cuda.stream(st):
-        for _ in range(2):
-            gt.adv3d_batch([dict(dd, f_in=bufs[0], f_out=bufs[1])], 18, tr.dt, gt.P2_PDM,
-                           gt.P2_PDM, stream=st)
-            bufs.reverse()
+        tide = tidal.TidalShare(d, ctx, dev, M=30, dt=300.0, umax=1.0)
+        fin = tidal.initial_tracers(d, dev, nt)
+        fout = torch.empty_like(fin)
+        dd, hn = tide.macro_step()
+        bt, bg, bms, _ = gt.autotune_batch([dict(dd, ctx=ctx, f_in=fin, f_out=fout,
+                                                 hires=hires)], nt, tide.dt, gt.P2_PDM,
+                                           gt.P2_PDM, reps=3, stream=st)
+        bufs = [fin, fout]
+        gt.adv3d_batch([dict(dd, ctx=ctx, f_in=bufs[0], f_out=bufs[1], hires=hires)], nt,
+                       tide.dt, gt.P2_PDM, gt.P2_PDM, stream=st)
+        bufs.reverse()
+        m0 = [ctx.budget(hn, bufs[0][q])[0] for q in range(nt)]
         for _ in range(steps):
-            with timer:
-                gt.adv3d_batch([dict(dd, f_in=bufs[0], f_out=bufs[1])], 18, tr.dt, gt.P2_PDM,
-                               gt.P2_PDM, stream=st)
+            tide.micro_steps()  # the stand-in 2-D mode: not timed
+            with t_prod:
+                dd, hn = tide.producers()
+            with t_adv:
+                gt.adv3d_batch([dict(dd, ctx=ctx, f_in=bufs[0], f_out=bufs[1], hires=hires)],
+                               nt, tide.dt, gt.P2_PDM, gt.P2_PDM, stream=st)
             bufs.reverse()
-    t = timer.mean()
+        st.synchronize()
+        m1 = [ctx.budget(hn, bufs[0][q])[0] for q in range(nt)]
+    ta, tp = t_adv.mean(), t_prod.mean()
     ncell = d.imax * d.jmax * d.kmax
-    b = ncell * (18 * 2 * 4 + 7 * 8)
-    return {"workload": "one GPU's share of 2000x1500x50 split 2x4: 1000x375x50 GVC, T, S + 16 "
-                        "passive tracers FP32, P2-PDM HALFSPLIT",
-            "rows": rows_str(gt.strip_rows(bt)), "tracers_per_wave": bg, "ms_per_step": t * 1e3,
-            "cell_updates_per_s": ncell * 18 / t, "algorithmic_GBps": b / t / 1e9,
-            "frac_of_8TBps": b / t / HBM_PEAK}
+    b = ncell * (nt * 2 * 4 + 7 * 8)
+    drift = max(abs(b1 - a0) / abs(a0) for a0, b1 in zip(m0, m1) if a0 != 0.0)
+    return {"workload": "one GPU's share of the tidal 2000x1500x50 case split 2x4: 1000x375x50 "
+                        "general coordinates, M2 tide, producers on the device each 300 s macro "
+                        "step, T, S + 16 passive tracers FP32, P2-PDM HALFSPLIT",
+            "rows": rows_str(gt.strip_rows(bt)), "tracers_per_wave": bg,
+            "ms_per_step": (ta + tp) * 1e3, "advection_ms": ta * 1e3, "producers_ms": tp * 1e3,
+            "cell_updates_per_s": ncell * nt / ta, "algorithmic_GBps": b / ta / 1e9,
+            "frac_of_8TBps": b / ta / HBM_PEAK, "mass_drift_max_rel": drift,
+            "steps": steps}
 
 
 def cpu_baseline(tr, seconds):
