@@ -61,8 +61,9 @@ def test_fused_f64_bit_exact(device, kind):
 
 
 def test_fused_f64_bit_exact_wide(device):
-    """several strips across i and j (NX = 154: strips at i = 0, 60, 120)"""
-    dom = random_domain(150, 40, 6, 7, "islands")
+    """several strips across i and j (NX = 154: strips at i = 0, 60, 120;
+    NY = 134: five to seventeen row strips)"""
+    dom = random_domain(150, 130, 6, 7, "islands")
     F = random_fields(dom, 200)
     ctx = gt.Context.from_domain(dom)
     orc = po.Oracle.from_domain(dom)
