@@ -726,6 +726,32 @@ def shard_section(torch, np, gt, inputs, dev, steps):
             bufs.reverse()
         st.synchronize()
         m1 = [ctx.budget(hn, bufs[0][q])[0] for q in range(nt)]
+        # the FP32 reverse mode over 24 steps of the last macro step's forcing
+        # (held fixed), a checkpoint every 6 steps: one untimed call that
+        # allocates the work fields, then one timed call
+        nadj, ck = 24, 6
+        f0 = bufs[0]
+        gen = torch.Generator(device=dev).manual_seed(7)
+        lamT = torch.rand(f0.shape, generator=gen, device=dev, dtype=torch.float32) - 0.5
+        lam0, ffin = torch.empty_like(f0), torch.empty_like(f0)
+        vel = [dd[n] for n in ("uu", "vv", "ww", "hu", "hv", "ho")]
+        t_rev = EventTimer(torch, st)
+        ctx.adv3d_adjoint_f32(nt, nadj, ck, tide.dt, f0, lamT, lam0, *vel, gt.P2_PDM, gt.P2_PDM,
+                              f_final=ffin)
+        with t_rev:
+            ctx.adv3d_adjoint_f32(nt, nadj, ck, tide.dt, f0, lamT, lam0, *vel, gt.P2_PDM,
+                                  gt.P2_PDM, f_final=ffin)
+        st.synchronize()
+        # Euler: each step is positively homogeneous of degree 1 in the tracer,
+        # so <lam0, f0> = <lamT, f_final>
+        lhs = sum(float(torch.dot(lam0[q].reshape(-1).double(), f0[q].reshape(-1).double()))
+                  for q in range(nt))
+        rhs = sum(float(torch.dot(lamT[q].reshape(-1).double(), ffin[q].reshape(-1).double()))
+                  for q in range(nt))
+        del lamT, lam0, ffin
+    tr_ms = t_rev.mean() * 1e3
+    n3 = (d.kmax + 1) * (d.jmax + 4) * (d.imax + 4)
+    work_gb = (5 * n3 * 8 + (6 + nadj // ck + ck - 1) * nt * n3 * 4) / 1e9
     ta, tp = t_adv.mean(), t_prod.mean()
     ncell = d.imax * d.jmax * d.kmax
     b = ncell * (nt * 2 * 4 + 7 * 8)
@@ -737,7 +763,11 @@ def shard_section(torch, np, gt, inputs, dev, steps):
             "ms_per_step": (ta + tp) * 1e3, "advection_ms": ta * 1e3, "producers_ms": tp * 1e3,
             "cell_updates_per_s": ncell * nt / ta, "algorithmic_GBps": b / ta / 1e9,
             "frac_of_8TBps": b / ta / HBM_PEAK, "mass_drift_max_rel": drift,
-            "steps": steps}
+            "steps": steps,
+            "adjoint_f32": {"steps": nadj, "ckpt_every": ck, "tracers": nt, "ms": tr_ms,
+                            "ms_per_step": tr_ms / nadj, "work_GB": work_gb,
+                            "euler_identity_rel": abs(lhs - rhs) / abs(rhs),
+                            "forcing": "the last macro step's, held fixed"}}
 
 
 def cpu_baseline(tr, seconds):

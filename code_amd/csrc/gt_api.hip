@@ -16,6 +16,7 @@
 
 #include "../../include/getm_transport.h"
 #include "gt_adjoint.h"
+#include "gt_adjoint_f32.h"
 #include "gt_batched.h"
 #include "gt_kernels.h"
 
@@ -77,6 +78,8 @@ struct gt_ctx {
   double* tri = nullptr;   // batched tridiagonal work (grown on demand)
   long tri_n = 0;
   double* mwork = nullptr; // uv_advect_3d work: fadv uuadv vvadv wwadv huadv hvadv
+  void* w32 = nullptr;     // FP32 per-sub-step and reverse-mode work (grown on demand)
+  size_t w32_bytes = 0;
 };
 
 extern "C" const char* gt_last_error(void) { return g_err; }
@@ -227,7 +230,7 @@ extern "C" int gt_destroy(gt_ctx* c) {
   if (!c) return GT_OK;
   void* ps[] = {c->dmask, c->dint, c->fi, c->hi, c->adv, c->tmp, c->scr7, c->zz,
                 c->P, c->slice3, c->bpart, c->bout, c->nwarn, c->ws, c->dlev, c->gga,
-                c->dbad, c->vdw, c->tri, c->mwork};
+                c->dbad, c->vdw, c->tri, c->mwork, c->w32};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   for (double* p : c->adj)
@@ -1099,6 +1102,72 @@ extern "C" int gt_set_tuning(gt_ctx* c, int fused, int tile, int group) {
   return GT_OK;
 }
 
+// ---- the FP32 step one sub-step at a time ---------------------------------------
+static int ensure_w32(gt_ctx* c, size_t bytes) {
+  if (c->w32_bytes >= bytes) return GT_OK;
+  if (c->w32) {
+    HIPCHK(hipDeviceSynchronize());  // the old fields may still be in use on any stream
+    HIPCHK(hipFree(c->w32));
+    c->w32 = nullptr;
+    c->w32_bytes = 0;
+  }
+  HIPCHK(hipMalloc(&c->w32, bytes));
+  c->w32_bytes = bytes;
+  return GT_OK;
+}
+
+static int copy_f32(hipStream_t st, float* dst, const float* src, size_t n) {
+  if (dst == src || n == 0) return GT_OK;
+  HIPCHK(hipMemcpyAsync(dst, src, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+  return GT_OK;
+}
+
+// the five FP32 sub-steps of one HALFSPLIT step (advection_3d.F90:439-547)
+// for ntr tracers: in -> s[0] (u/2) -> s[1] (v/2) -> s[2] (w) -> s[3] (v/2)
+// -> out (u/2).  D[0..5]: the layer heights before/after each sub-step.
+// Consecutive buffers must differ (s may repeat two buffers).
+static int substeps_f32(gt_ctx* c, hipStream_t st, const GridMasks* g, const double* const D[6],
+                        double dt, int ntr, const float* in, float* const s[4], float* out,
+                        const double* uu, const double* vv, const double* ww, const double* hu,
+                        const double* hv, int hs, int vs, double AH, int kshift) {
+  const long ts = c->N3;
+  const Dims d = c->d;
+  LCHK(gt::launch_sub_h_f32(st, d, 0, ntr, ts, in, s[0], D[0], D[1], uu, hu, g->flags, dt, c->dx,
+                            c->dy, c->ard1, hs, AH));
+  LCHK(gt::launch_sub_h_f32(st, d, 1, ntr, ts, s[0], s[1], D[1], D[2], vv, hv, g->flags, dt,
+                            c->dx, c->dy, c->ard1, hs, AH));
+  LCHK(gt::launch_sub_w_f32(st, d, ntr, ts, s[1], s[2], D[2], D[3], ww, g->flags, dt, vs, kshift));
+  LCHK(gt::launch_sub_h_f32(st, d, 1, ntr, ts, s[2], s[3], D[3], D[4], vv, hv, g->flags, dt,
+                            c->dx, c->dy, c->ard1, hs, AH));
+  if (out)
+    LCHK(gt::launch_sub_h_f32(st, d, 0, ntr, ts, s[3], out, D[4], D[5], uu, hu, g->flags, dt,
+                              c->dx, c->dy, c->ard1, hs, AH));
+  return GT_OK;
+}
+
+// gt_adv3d_batch_f32 with the per-sub-step kernels (tuning fused = 0), one
+// domain: work = D1..D5 + two tracer sets
+static int batch_substeps_f32(const gt_batch_domain& bd, const GridMasks* g, int ntr, double dt,
+                              int hs, int vs, double AH, int kshift, hipStream_t st) {
+  gt_ctx* c = bd.ctx;
+  const long N3 = c->N3;
+  const size_t set = (size_t)ntr * N3;
+  TRY(ensure_w32(c, sizeof(double) * 5 * (size_t)N3 + sizeof(float) * 2 * set));
+  double* Dh[5];
+  for (int n = 0; n < 5; n++) Dh[n] = (double*)c->w32 + n * N3;
+  float* A = (float*)((double*)c->w32 + 5 * N3);
+  float* B = A + set;
+  LCHK(gt::launch_heights(st, c->d, g->flags, bd.ho, bd.uu, bd.vv, bd.ww, Dh, dt, c->dx, c->dy,
+                          c->ard1, hs, kshift));
+  const double* D[6] = {bd.ho, Dh[0], Dh[1], Dh[2], Dh[3], Dh[4]};
+  float* const s[4] = {A, B, A, B};
+  TRY(substeps_f32(c, st, g, D, dt, ntr, (const float*)bd.f_in, s, (float*)bd.f_out, bd.uu, bd.vv,
+                   bd.ww, bd.hu, bd.hv, hs, vs, AH, kshift));
+  if (bd.hires)
+    HIPCHK(hipMemcpyAsync(bd.hires, Dh[4], sizeof(double) * N3, hipMemcpyDeviceToDevice, st));
+  return GT_OK;
+}
+
 template <typename TF>
 static int batch_impl(int ndom, const gt_batch_domain* doms, int ntracers, double dt,
                       int hscheme, int vscheme, double AH, int tag, void* stream) {
@@ -1106,7 +1175,9 @@ static int batch_impl(int ndom, const gt_batch_domain* doms, int ntracers, doubl
     return set_err(GT_EARG, "gt_adv3d_batch: bad argument");
   if (!split_scheme(hscheme) || !split_scheme(vscheme))
     return set_err(GT_EINVAL_SCHEME, "gt_adv3d_batch: schemes must be UPSTREAM..P2_PDM");
-  if (!gt::fused_supported(hscheme, vscheme))
+  // FP32 with fused = 0 on the first context: the per-sub-step kernels
+  const bool sub = sizeof(TF) == 4 && doms[0].ctx && !doms[0].ctx->fused;
+  if (!sub && !gt::fused_supported(hscheme, vscheme))
     return set_err(GT_ENOTIMPL, "gt_adv3d_batch: hscheme and vscheme must be equal");
   const GridMasks* gsel = nullptr;
   for (int i = 0; i < ndom; i++) {
@@ -1123,6 +1194,12 @@ static int batch_impl(int ndom, const gt_batch_domain* doms, int ntracers, doubl
   }
   gt_ctx* c0 = doms[0].ctx;
   hipStream_t st = stream ? (hipStream_t)stream : c0->st;
+  if (sub) {
+    for (int i = 0; i < ndom; i++)
+      TRY(batch_substeps_f32(doms[i], grid_for(doms[i].ctx, tag), ntracers, dt, hscheme, vscheme,
+                             AH, tag == GT_W_TAG ? 1 : 0, st));
+    return GT_OK;
+  }
   // FP64: two tracers per wavefront (T and S) when there are two or more,
   // unless the tuning asks for one
   const int group = sizeof(TF) == 4 ? c0->group : (ntracers >= 2 && c0->group != 1 ? 2 : 1);
@@ -1375,6 +1452,106 @@ extern "C" int gt_adv3d_adjoint(gt_ctx* c, int nsteps, int ckpt_every, double dt
     }
   }
   return copy_d2d(c, lam0, lam[0], N3);
+}
+
+// transpose of one FP32 HALFSPLIT step from s0: the states before the last
+// four sub-steps are recomputed into S[0..3]; lam[0] (adjoint of the step's
+// output) is replaced by the adjoint of s0, lam[1] is work
+static int reverse_step_f32(gt_ctx* c, const GridMasks* g, const double* const D[6], double dt,
+                            int ntr, const float* s0, float* const S[4], float** lam,
+                            const double* uu, const double* vv, const double* ww,
+                            const double* hu, const double* hv, int hs, int vs, double AH,
+                            int kshift) {
+  hipStream_t st = c->st;
+  const Dims d = c->d;
+  const long ts = c->N3;
+  TRY(substeps_f32(c, st, g, D, dt, ntr, s0, S, nullptr, uu, vv, ww, hu, hv, hs, vs, AH, kshift));
+  float* a = lam[0];
+  float* b = lam[1];
+  LCHK(gt::launch_sub_h_f32_adj(st, d, 0, ntr, ts, S[3], a, b, D[4], D[5], uu, hu, g->flags, dt,
+                                c->dx, c->dy, c->ard1, hs, AH));
+  LCHK(gt::launch_sub_h_f32_adj(st, d, 1, ntr, ts, S[2], b, a, D[3], D[4], vv, hv, g->flags, dt,
+                                c->dx, c->dy, c->ard1, hs, AH));
+  LCHK(gt::launch_sub_w_f32_adj(st, d, ntr, ts, S[1], a, b, D[2], D[3], ww, g->flags, dt, vs,
+                                kshift));
+  LCHK(gt::launch_sub_h_f32_adj(st, d, 1, ntr, ts, S[0], b, a, D[1], D[2], vv, hv, g->flags, dt,
+                                c->dx, c->dy, c->ard1, hs, AH));
+  LCHK(gt::launch_sub_h_f32_adj(st, d, 0, ntr, ts, s0, a, b, D[0], D[1], uu, hu, g->flags, dt,
+                                c->dx, c->dy, c->ard1, hs, AH));
+  lam[0] = b;
+  lam[1] = a;
+  return GT_OK;
+}
+
+extern "C" int gt_adv3d_adjoint_f32(gt_ctx* c, int ntracers, int nsteps, int ckpt_every,
+                                    double dt, const float* f0, float* f_final,
+                                    const float* lam_final, float* lam0, const double* uu,
+                                    const double* vv, const double* ww, const double* hu,
+                                    const double* hv, const double* ho, int hscheme,
+                                    int vscheme, double AH, int tag) {
+  if (!c || ntracers < 1 || !f0 || !lam_final || !lam0 || nsteps < 1 || ckpt_every < 1)
+    return set_err(GT_EARG, "gt_adv3d_adjoint_f32: bad argument");
+  if (!uu || !vv || !ww || !hu || !hv || !ho)
+    return set_err(GT_EARG, "gt_adv3d_adjoint_f32: NULL velocity or layer height");
+  const GridMasks* g = grid_for(c, tag);
+  if (!g) return set_err(GT_EINVAL_TAG, "gt_adv3d_adjoint_f32: tag is invalid");
+  if (!split_scheme(hscheme) || !split_scheme(vscheme))
+    return set_err(GT_EINVAL_SCHEME, "gt_adv3d_adjoint_f32: schemes must be UPSTREAM..P2_PDM");
+  if (c->kmax < 2) return set_err(GT_EARG, "gt_adv3d_adjoint_f32: kmax must be >= 2");
+  const int kshift = tag == GT_W_TAG ? 1 : 0;
+  const long N3 = c->N3;
+  const size_t set = (size_t)ntracers * N3;
+  const int nck = (nsteps + ckpt_every - 1) / ckpt_every;
+  const int seg = ckpt_every;
+  // D1..D5, then tracer sets: S[4] lam[2] ck[nck] seg-1 states of a segment
+  const size_t nsets = 4 + 2 + (size_t)nck + (size_t)(seg - 1);
+  TRY(ensure_w32(c, sizeof(double) * 5 * (size_t)N3 + sizeof(float) * nsets * set));
+  double* Dh[5];
+  for (int n = 0; n < 5; n++) Dh[n] = (double*)c->w32 + n * N3;
+  float* base = (float*)((double*)c->w32 + 5 * N3);
+  float* const S[4] = {base, base + set, base + 2 * set, base + 3 * set};
+  float* lam[2] = {base + 4 * set, base + 5 * set};
+  float* ck = base + 6 * set;
+  float* sg = ck + (size_t)nck * set;
+  LCHK(gt::launch_heights(c->st, c->d, g->flags, ho, uu, vv, ww, Dh, dt, c->dx, c->dy, c->ard1,
+                          hscheme, kshift));
+  const double* D[6] = {ho, Dh[0], Dh[1], Dh[2], Dh[3], Dh[4]};
+  // the strip kernels where they apply, else the per-sub-step kernels (the
+  // same states bit for bit)
+  const bool plain = c->fused && gt::fused_supported(hscheme, vscheme) && strip_fits(c);
+  auto step = [&](const float* in, float* out) -> int {
+    if (plain) {
+      gt_batch_domain bd{c, in, out, uu, vv, ww, hu, hv, ho, nullptr};
+      return batch_impl<float>(1, &bd, ntracers, dt, hscheme, vscheme, AH, tag, c->st);
+    }
+    float* const s[4] = {S[0], S[1], S[0], S[1]};
+    return substeps_f32(c, c->st, g, D, dt, ntracers, in, s, out, uu, vv, ww, hu, hv, hscheme,
+                        vscheme, AH, kshift);
+  };
+  // forward sweep (lam[0], lam[1] as the two states), a checkpoint every
+  // ckpt_every steps
+  float* a = lam[0];
+  float* b = lam[1];
+  TRY(copy_f32(c->st, a, f0, set));
+  for (int t = 0; t < nsteps; t++) {
+    if (t % ckpt_every == 0) TRY(copy_f32(c->st, ck + (size_t)(t / ckpt_every) * set, a, set));
+    TRY(step(a, b));
+    float* x = a; a = b; b = x;
+  }
+  if (f_final) TRY(copy_f32(c->st, f_final, a, set));
+  // reverse sweep, segment by segment from the last checkpoint
+  TRY(copy_f32(c->st, lam[0], lam_final, set));
+  for (int s = nck - 1; s >= 0; s--) {
+    const int t0 = s * ckpt_every, t1 = t0 + ckpt_every < nsteps ? t0 + ckpt_every : nsteps;
+    auto state = [&](int t) -> float* {
+      return t == t0 ? ck + (size_t)s * set : sg + (size_t)(t - t0 - 1) * set;
+    };
+    for (int t = t0; t < t1 - 1; t++) TRY(step(state(t), state(t + 1)));
+    for (int t = t1 - 1; t >= t0; t--)
+      TRY(reverse_step_f32(c, g, D, dt, ntracers, state(t), S, lam, uu, vv, ww, hu, hv, hscheme,
+                           vscheme, AH, kshift));
+  }
+  return copy_f32(c->st, lam0, lam[0], set);
 }
 
 extern "C" int gt_ip_blumberg_mellor_adjoint(gt_ctx* c, const double* hun, const double* hvn,

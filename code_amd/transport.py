@@ -40,7 +40,7 @@ EXPORTS = [
     "gt_update_3d_halo", "gt_update_2d_halo", "gt_wait_halo",
     "gt_exchange_tiles", "gt_budget", "gt_malloc", "gt_free",
     "gt_memcpy_h2d", "gt_memcpy_d2h", "gt_adv3d_batch_f32", "gt_adv3d_batch_f64",
-    "gt_set_tuning", "gt_budget_f32", "gt_adv3d_adjoint",
+    "gt_set_tuning", "gt_budget_f32", "gt_adv3d_adjoint", "gt_adv3d_adjoint_f32",
     "gt_ip_blumberg_mellor_adjoint", "gt_eos_linear_adjoint", "gt_autotune_batch",
     "gt_do_eqstate", "gt_rho_from_theta_unesco80", "gt_rho_from_theta", "gt_eosall_from_theta",
     "gt_start_macro", "gt_set_coordinates", "gt_coordinates", "gt_ww_momentum_3d",
@@ -139,6 +139,8 @@ def load_library():
         "gt_budget_f32": (i, [vp, vp, vp, C.POINTER(d), C.POINTER(d)]),
         "gt_adv3d_adjoint": (i, [vp, i, i, d, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i, i,
                                  d, i]),
+        "gt_adv3d_adjoint_f32": (i, [vp, i, i, i, d, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                     i, i, d, i]),
         "gt_ip_blumberg_mellor_adjoint": (i, [vp, vp, vp, vp, vp, vp, vp]),
         "gt_eos_linear_adjoint": (i, [vp, vp, vp, vp, d, d]),
         "gt_autotune_batch": (i, [i, C.POINTER(BatchDomain), i, d, i, i, d, i, i, i, vp,
@@ -414,6 +416,22 @@ class Context:
                                         _ptr(lam_final), _ptr(lam0), _ptr(uu), _ptr(vv),
                                         _ptr(ww), _ptr(hu), _ptr(hv), _ptr(ho), hscheme,
                                         vscheme, AH, tag))
+
+    def adv3d_adjoint_f32(self, ntracers, nsteps, ckpt_every, dt, f0, lam_final, lam0, uu, vv,
+                          ww, hu, hv, ho, hscheme, vscheme, AH=-1.0, tag=H_TAG, f_final=None):
+        """gt_adv3d_adjoint_f32: f0, lam_final, lam0 (and f_final) are FP32
+        tensors of shape (ntracers, kmax+1, NY, NX)"""
+        import torch
+        n3 = (self.kmax + 1) * self.NY * self.NX
+        for t in (f0, lam_final, lam0, f_final):
+            if t is not None and (t.dtype != torch.float32 or t.numel() != ntracers * n3
+                                  or not t.is_contiguous()):
+                raise ValueError("adv3d_adjoint_f32: tracer sets must be contiguous FP32 "
+                                 "tensors of ntracers x (kmax+1) x NY x NX")
+        check(self.lib.gt_adv3d_adjoint_f32(self.h, ntracers, nsteps, ckpt_every, dt, _ptr(f0),
+                                            _ptr(f_final), _ptr(lam_final), _ptr(lam0), _ptr(uu),
+                                            _ptr(vv), _ptr(ww), _ptr(hu), _ptr(hv), _ptr(ho),
+                                            hscheme, vscheme, AH, tag))
 
     def ip_blumberg_mellor_adjoint(self, hun, hvn, zz, idpdx_bar, idpdy_bar, buoy_bar):
         check(self.lib.gt_ip_blumberg_mellor_adjoint(self.h, _ptr(hun), _ptr(hvn), _ptr(zz),

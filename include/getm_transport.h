@@ -327,7 +327,10 @@ int gt_adv3d_batch_f64(int ndom, const gt_batch_domain *doms, int ntracers,
                        double dt, int hscheme, int vscheme, double AH, int tag,
                        void *hip_stream);
 /* kernel choice: the two HALFSPLIT strip kernels (fused=1) or the
- * per-sub-step kernels (0); rows per strip: tile 0, 1, 2 give 16, 32, 8 rows
+ * per-sub-step kernels (0; gt_adv3d_batch_f32 then runs the FP32 sub-steps
+ * one kernel each, bit-identical to the strip kernels, and accepts
+ * hscheme != vscheme; gt_adv3d_batch_f64 always runs the strip kernels);
+ * rows per strip: tile 0, 1, 2 give 16, 32, 8 rows
  * to both kernels, tile 3..11 gives {16, 32, 8}[(tile-3)%3] rows to the first
  * (u/2 v/2) and {16, 32, 8}[(tile-3)/3] to the second (w v/2 u/2); tracers
  * per wavefront (group: FP32 1, 3, 6 or 9; FP64 1, or 2 for any other value
@@ -356,6 +359,22 @@ int gt_adv3d_adjoint(gt_ctx *ctx, int nsteps, int ckpt_every, double dt,
                      const double *ww, const double *hu, const double *hv,
                      const double *ho, int hscheme, int vscheme, double AH,
                      int tag);
+/* the same for ntracers FP32 tracers sharing the velocities, in the
+ * arithmetic of gt_adv3d_batch_f32 (tracer t of f0, f_final, lam_final and
+ * lam0 at + t*(kmax+1)*NY*NX; layer heights FP64; adv_ver_iterations = 1).
+ * The forward sweep runs the strip kernels (fused = 1 and hscheme == vscheme)
+ * or the per-sub-step kernels, which give the same states bit for bit; the
+ * reverse sweep recomputes each step's sub-step states from the checkpoint
+ * and applies the transpose of each FP32 sub-step, the limiter through the
+ * derivative of the branch term the FP32 forward selected.  Work memory:
+ * 5 FP64 fields + (6 + ceil(nsteps/ckpt_every) + ckpt_every - 1) tracer
+ * sets of ntracers FP32 fields. */
+int gt_adv3d_adjoint_f32(gt_ctx *ctx, int ntracers, int nsteps, int ckpt_every,
+                         double dt, const float *f0, float *f_final,
+                         const float *lam_final, float *lam0, const double *uu,
+                         const double *vv, const double *ww, const double *hu,
+                         const double *hv, const double *ho, int hscheme,
+                         int vscheme, double AH, int tag);
 /* transpose of ip_blumberg_mellor (ip_blumberg_mellor.F90:99-149) with
  * respect to buoy, zz/hun/hvn from the forward call: buoy_bar is overwritten
  * (kmax <= 64) */

@@ -1,0 +1,225 @@
+"""The FP32 reverse mode (gt_adv3d_adjoint_f32) and the FP32 per-sub-step
+kernels it recomputes the states with.
+
+* The per-sub-step FP32 forward (tuning fused = 0) equals the strip kernels
+  bit for bit: every scheme, with and without horizontal diffusion, H and W
+  tags, three tracers, hires included.  Mixed schemes (hscheme != vscheme,
+  per-sub-step only) agree with the oracle's FP64 step to FP32 rounding
+  (bound 1e-5 of the field's range).
+* Linear schemes (UPSTREAM, P2): the forward is a linear map M of the
+  tracer, and <M x, y> = <x, M^T y> over several steps to FP32 rounding
+  (bound 1e-5 relative); the FP32 adjoint also equals the FP64 reverse mode
+  (pinned by central differences in test_gpu_adjoint.py) within 1e-5 of
+  max |lam|.
+* Limited schemes: against the FP64 reverse mode on the same problem (a
+  Gaussian blob in a rotating basin, 12 steps, checkpoint every 4).  Where
+  the FP32 and FP64 trajectories sit on different sides of a limiter kink
+  the two gradients differ by O(1) at that face, so the bound is on the
+  relative L2 norm of the difference (1e-5; measured 2e-7) and on the share
+  of cells that differ by more than 1e-4 of max |lam| (<= 0.1%; measured 0).
+* Every scheme: each sub-step is positively homogeneous of degree 1 in the
+  tracer (the limiter depends on ratios of differences only, the update is
+  linear), so on the branch the forward took the Jacobian J satisfies
+  J f = M(f) (Euler), and <lam0, f0> = <lam_final, M(f0)> with lam0 =
+  J^T lam_final.  A wrong branch derivative, sign or factor breaks it (bound
+  1e-5 relative).
+* Checkpoint spacing and the forward kernel choice change nothing (bit for
+  bit); several tracers in one call equal one call per tracer.
+"""
+import numpy as np
+import pytest
+import torch
+
+from code_amd import inputs
+from code_amd import transport as gt
+from oracle import pyoracle as po
+from tests.test_gpu_parity import random_domain, random_fields
+from tests.util import to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+SCHEMES = [gt.UPSTREAM, gt.P2, gt.SUPERBEE, gt.MUSCL, gt.P2_PDM]
+VEL = ("uu", "vv", "ww", "hu", "hv", "ho")
+
+
+def tracer_set(dom, n, seed, device):
+    """n FP32 tracers (the first from the seed's uniform field), level 0 kept"""
+    sh = dom.shape3
+    f = np.stack([inputs.uniform(sh, seed + 31 * t) for t in range(n)])
+    return torch.from_numpy(f.astype(np.float32)).to(device).contiguous()
+
+
+def problem(device, seed=8, kind="islands", size=(21, 17, 5)):
+    dom = random_domain(*size, seed, kind)
+    F = random_fields(dom, 400 + seed, plateaus=False)
+    ctx = gt.Context.from_domain(dom)
+    vel = [to_dev(F[n], device) for n in VEL]
+    return dom, F, ctx, vel
+
+
+def batch_step(ctx, f_in, vel, ntr, dt, hs, vs, AH, tag, fused):
+    ctx.set_tuning(fused=fused, tile=0, group=3)
+    out = torch.empty_like(f_in)
+    hires = torch.empty_like(vel[5])
+    gt.adv3d_batch([dict(ctx=ctx, f_in=f_in, f_out=out, uu=vel[0], vv=vel[1], ww=vel[2],
+                         hu=vel[3], hv=vel[4], ho=vel[5], hires=hires)], ntr, dt, hs, vs, AH, tag)
+    torch.cuda.synchronize()
+    ctx.set_tuning(fused=1, tile=0, group=3)
+    return out, hires
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_substeps_equal_strip_kernels(device, scheme):
+    for kind, tag, AH in (("islands", gt.H_TAG, -1.0), ("open", gt.W_TAG, -1.0),
+                          ("islands", gt.H_TAG, 250.0), ("walls", gt.U_TAG, 0.0)):
+        dom, F, ctx, vel = problem(device, seed=5, kind=kind, size=(70, 23, 5))
+        f = tracer_set(dom, 3, 11, device)
+        a, ha = batch_step(ctx, f, vel, 3, F["dt"], scheme, scheme, AH, tag, fused=1)
+        b, hb = batch_step(ctx, f, vel, 3, F["dt"], scheme, scheme, AH, tag, fused=0)
+        case = (scheme, kind, tag, AH)
+        assert torch.equal(ha, hb), case
+        assert torch.equal(a, b), (case, float((a - b).abs().max()))
+
+
+def test_substeps_mixed_schemes_vs_oracle(device):
+    dom, F, ctx, vel = problem(device, seed=6, size=(33, 29, 6))
+    orc = po.Oracle.from_domain(dom)
+    f = tracer_set(dom, 2, 21, device)
+    for hs, vs in ((gt.SUPERBEE, gt.P2_PDM), (gt.P2_PDM, gt.UPSTREAM), (gt.MUSCL, gt.P2)):
+        out, _ = batch_step(ctx, f, vel, 2, F["dt"], hs, vs, -1.0, gt.H_TAG, fused=0)
+        got = to_host(out)
+        for t in range(2):
+            want = to_host(f[t]).astype(np.float64)
+            orc.do_advection_3d(F["dt"], want, *(F[n] for n in ("uu", "vv", "ww", "hu", "hv",
+                                                                  "ho", "ho")),
+                                gt.HALFSPLIT, hs, vs, -1.0, gt.H_TAG)
+            rng = float(want.max() - want.min())
+            err = float(np.abs(got[t] - want).max()) / rng
+            assert err <= 1e-5, (hs, vs, t, err)
+
+
+def adjoint32(ctx, ntr, nsteps, ck, dt, f0, lamT, vel, hs, vs, AH=-1.0, tag=gt.H_TAG):
+    lam0 = torch.empty_like(f0)
+    ff = torch.empty_like(f0)
+    ctx.adv3d_adjoint_f32(ntr, nsteps, ck, dt, f0, lamT, lam0, *vel, hs, vs, AH, tag, f_final=ff)
+    torch.cuda.synchronize()
+    return lam0, ff
+
+
+def adjoint64(ctx, nsteps, ck, dt, f0, lamT, vel, hs, vs, AH=-1.0, tag=gt.H_TAG):
+    lam0 = torch.empty_like(f0)
+    ff = torch.empty_like(f0)
+    ctx.adv3d_adjoint(nsteps, ck, dt, f0, lamT, lam0, *vel, hs, vs, AH, tag, f_final=ff)
+    torch.cuda.synchronize()
+    return lam0, ff
+
+
+def dot(a, b):
+    return float(torch.sum(a.double() * b.double()))
+
+
+@pytest.mark.parametrize("scheme", [gt.UPSTREAM, gt.P2])
+def test_dot_product_linear_schemes(device, scheme):
+    dom, F, ctx, vel = problem(device, seed=9, size=(26, 21, 6))
+    x = tracer_set(dom, 2, 41, device) - 0.5
+    y = tracer_set(dom, 2, 43, device) - 0.5
+    for AH in (-1.0, 300.0):
+        lam0, Mx = adjoint32(ctx, 2, 5, 2, F["dt"], x, y, vel, scheme, scheme, AH)
+        lhs, rhs = dot(Mx, y), dot(x, lam0)
+        assert abs(lhs - rhs) <= 1e-5 * abs(rhs), (scheme, AH, lhs, rhs)
+        # the FP64 reverse mode of the same map, tracer by tracer
+        for t in range(2):
+            l64, _ = adjoint64(ctx, 5, 2, F["dt"], x[t].double().contiguous(),
+                               y[t].double().contiguous(), vel, scheme, scheme, AH)
+            e = float((lam0[t].double() - l64).abs().max() / l64.abs().max())
+            assert e <= 1e-5, (scheme, AH, t, e)
+
+
+@pytest.mark.parametrize("scheme", [gt.SUPERBEE, gt.MUSCL, gt.P2_PDM])
+def test_limited_schemes_vs_fp64_reverse_mode(device, scheme):
+    tr = inputs.blob_case(seed=4, imax=40, jmax=36, kmax=6)
+    d = tr.dom
+    ctx = gt.Context.from_domain(d)
+    vel = [to_dev(a, device) for a in (tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho)]
+    c0 = tr.tracers["c"]
+    f0 = torch.from_numpy(c0.astype(np.float32)[None]).to(device).contiguous()
+    rng = np.random.default_rng(scheme)
+    lamT = torch.from_numpy(rng.standard_normal((1,) + d.shape3).astype(np.float32)).to(device)
+    lam32, ff32 = adjoint32(ctx, 1, 12, 4, tr.dt, f0, lamT, vel, scheme, scheme)
+    lam64, ff64 = adjoint64(ctx, 12, 4, tr.dt, f0[0].double().contiguous(),
+                            lamT[0].double().contiguous(), vel, scheme, scheme)
+    diff = (lam32[0].double() - lam64).abs()
+    scale = float(lam64.abs().max())
+    l2 = float(diff.norm() / lam64.norm())
+    frac = float((diff > 1e-4 * scale).double().mean())
+    print(f"scheme {scheme}: rel L2 {l2:.2e}, max {float(diff.max()) / scale:.2e}, "
+          f"cells > 1e-4 of max: {frac:.2e}; forward err "
+          f"{float((ff32[0].double() - ff64).abs().max()):.2e}")
+    assert l2 <= 1e-5 and frac <= 1e-3, (scheme, l2, frac)
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_euler_identity_every_scheme(device, scheme):
+    """the rotating basin (volume-consistent transports): a Gaussian blob and
+    a random tracer with plateaus (limiter branches with deltaf*deltafu <= 0),
+    8 steps, without and with horizontal diffusion"""
+    tr = inputs.blob_case(seed=6, imax=34, jmax=30, kmax=6)
+    d = tr.dom
+    ctx = gt.Context.from_domain(d)
+    vel = [to_dev(a, device) for a in (tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho)]
+    rnd = inputs.uniform(d.shape3, 61)
+    rnd[inputs.uniform(d.shape3, 62) < 0.1] = 0.5
+    f0 = torch.from_numpy(np.stack([tr.tracers["c"], rnd]).astype(np.float32)).to(device)
+    lamT = tracer_set(d, 2, 63, device) - 0.5
+    for AH in (-1.0, 200.0):
+        lam0, ff = adjoint32(ctx, 2, 8, 3, tr.dt, f0, lamT, vel, scheme, scheme, AH)
+        for t in range(2):
+            lhs, rhs = dot(lam0[t], f0[t]), dot(lamT[t], ff[t])
+            assert abs(lhs - rhs) <= 1e-5 * abs(rhs), (scheme, AH, t, lhs, rhs)
+
+
+def bits(a, b):
+    """a and b hold the same FP32 bit patterns"""
+    return torch.equal(a.contiguous().view(torch.int32), b.contiguous().view(torch.int32))
+
+
+def test_checkpoints_kernels_and_tracers_bit_identical(device):
+    tr = inputs.blob_case(seed=5, imax=40, jmax=27, kmax=5)
+    d = tr.dom
+    ctx = gt.Context.from_domain(d)
+    vel = [to_dev(a, device) for a in (tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho)]
+    f0 = tracer_set(d, 3, 51, device)
+    f0[0] = torch.from_numpy(tr.tracers["c"].astype(np.float32)).to(device)
+    lamT = tracer_set(d, 3, 53, device) - 0.5
+    dt = tr.dt
+    ref, ffr = adjoint32(ctx, 3, 7, 3, dt, f0, lamT, vel, gt.P2_PDM, gt.P2_PDM)
+    assert bool(torch.isfinite(ref).all())
+    for ck in (1, 7):
+        lam, ff = adjoint32(ctx, 3, 7, ck, dt, f0, lamT, vel, gt.P2_PDM, gt.P2_PDM)
+        assert bits(lam, ref) and bits(ff, ffr), ck
+    ctx.set_tuning(fused=0)
+    lam, ff = adjoint32(ctx, 3, 7, 3, dt, f0, lamT, vel, gt.P2_PDM, gt.P2_PDM)
+    ctx.set_tuning(fused=1)
+    assert bits(lam, ref) and bits(ff, ffr), "per-sub-step forward"
+    for t in range(3):
+        lam, ff = adjoint32(ctx, 1, 7, 3, dt, f0[t:t + 1].contiguous(),
+                            lamT[t:t + 1].contiguous(), vel, gt.P2_PDM, gt.P2_PDM)
+        assert bits(lam[0], ref[t]) and bits(ff[0], ffr[t]), t
+    # the forward sweep's final state is seven strip steps
+    x = f0.clone()
+    for _ in range(7):
+        x, _ = batch_step(ctx, x, vel, 3, dt, gt.P2_PDM, gt.P2_PDM, -1.0, gt.H_TAG, fused=1)
+    assert bits(x, ffr)
+
+
+def test_adjoint_f32_errors(device):
+    dom, F, ctx, vel = problem(device, seed=3, size=(12, 10, 3))
+    f0 = tracer_set(dom, 1, 1, device)
+    lam = torch.empty_like(f0)
+    with pytest.raises(gt.GtError, match="tag"):
+        ctx.adv3d_adjoint_f32(1, 2, 1, F["dt"], f0, f0, lam, *vel, gt.P2, gt.P2, -1.0, 99)
+    with pytest.raises(gt.GtError, match="scheme"):
+        ctx.adv3d_adjoint_f32(1, 2, 1, F["dt"], f0, f0, lam, *vel, gt.J7, gt.P2)
+    with pytest.raises(gt.GtError, match="bad argument"):
+        ctx.adv3d_adjoint_f32(1, 0, 1, F["dt"], f0, f0, lam, *vel, gt.P2, gt.P2)
+    with pytest.raises(ValueError):
+        ctx.adv3d_adjoint_f32(2, 2, 1, F["dt"], f0, f0, lam, *vel, gt.P2, gt.P2)
