@@ -133,12 +133,12 @@ __global__ __launch_bounds__(256) void k_sub_h_f32(
 }
 
 // ---- the vertical sub-step (FP32) ------------------------------------------------
-// the flags of the w faces at level K, as the strip kernels set them
-// (adv_split_w.F90:249-300): the flux at K+1/2 exists below kmax, the one at
-// K-1/2 above level 1 (or at level 1 with kshift); level K is updated up to
-// kmax-kshift; the limiter applies above level 1-kshift going up and below
-// kmax-1 going down
-__device__ __forceinline__ int lev_or(int l, int K, int kmax) { return l >= 0 && l <= kmax ? l : K; }
+// at level K, as the strip kernels set them (adv_split_w.F90:249-300): the
+// flux at K+1/2 exists below kmax, the one at K-1/2 above level 1 (or at
+// level 1 with kshift); level K is updated up to kmax-kshift; the limiter
+// applies above level 1-kshift going up and below kmax-1 going down
+// tracers per thread of the forward kernel (the face constants are shared)
+constexpr int kWGroup = 18;
 
 template <int SCH>
 __global__ __launch_bounds__(256) void k_sub_w_f32(Dims d, int ntr, long ts,
@@ -149,41 +149,52 @@ __global__ __launch_bounds__(256) void k_sub_w_f32(Dims d, int ntr, long ts,
                                                    const double* __restrict__ ww,
                                                    const uint8_t* __restrict__ flags, double dt,
                                                    int kshift) {
+  // one thread per column and group of kWGroup tracers (blockIdx.z),
+  // marching up; the face constants of a level serve the whole group
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  if (i >= d.NX || j >= d.NY) return;
+  const int t0 = blockIdx.z * kWGroup;
+  if (i >= d.NX || j >= d.NY || t0 >= ntr) return;
+  const int t1 = min(t0 + kWGroup, ntr);
   const int kmax = d.kmax;
   const long NXY = d.NXY, c2 = i + (long)d.NX * j;
   const bool wcol = (flags[c2] & MB_WCOL) && i <= d.NX - 2 && j <= d.NY - 2;
-  for (int t = 0; t < ntr; t++) fout[t * ts + c2] = fin[t * ts + c2];  // level 0
+  if (!wcol) {
+    for (int t = t0; t < t1; t++)
+      for (int K = 0; K <= kmax; K++) fout[t * ts + c2 + NXY * K] = fin[t * ts + c2 + NXY * K];
+    return;
+  }
+  for (int t = t0; t < t1; t++) fout[t * ts + c2] = fin[t * ts + c2];  // level 0
+  double dm = D2[c2], dk = D2[c2 + NXY], dp = D2[c2 + 2 * NXY];  // D2 at K-1, K, K+1
+  double wm = ww[c2], wkk = ww[c2 + NXY];                         // ww at K-1, K
   for (int K = 1; K <= kmax; K++) {
     const long cK = c2 + NXY * K;
-    if (!wcol) {
-      for (int t = 0; t < ntr; t++) fout[t * ts + cK] = fin[t * ts + cK];
-      continue;
-    }
     const bool wk_on = K <= kmax - 1, wp_on = K > 1 || kshift, kdo = K <= kmax - kshift;
     const bool limp_up = SCH != UPSTREAM && K - 1 > 1 - kshift;
     const bool limp_dn = SCH != UPSTREAM && K - 1 < kmax - 1;
     const bool limk_up = SCH != UPSTREAM && K > 1 - kshift;
     const bool limk_dn = SCH != UPSTREAM && K < kmax - 1;
-    const double D2K = D2[cK];
-    const double D2p = D2[c2 + NXY * lev_or(K + 1, K, kmax)];
     WFace<false> wfk, wfp;
-    wface_setup<SCH>(wfk, ww[cK], dt, D2K, D2p);
-    wface_setup<SCH>(wfp, ww[cK - NXY], dt, D2[cK - NXY], D2K);
-    const CellK k3 = cell_factors(D2K, D3[cK], dt);
-    const long l0 = c2 + NXY * lev_or(K - 2, K, kmax), l1 = cK - NXY;
-    const long l3 = c2 + NXY * lev_or(K + 1, K, kmax), l4 = c2 + NXY * lev_or(K + 2, K, kmax);
-    for (int t = 0; t < ntr; t++) {
+    wface_setup<SCH>(wfk, wkk, dt, dk, K + 1 <= kmax ? dp : dk);
+    wface_setup<SCH>(wfp, wm, dt, dm, dk);
+    const CellK k3 = cell_factors(dk, D3[cK], dt);
+    // the operands of the strip kernels: level -1 is zero, levels past kmax
+    // are replaced by level K
+    const long l0 = cK - 2 * NXY, l1 = cK - NXY;
+    const long l3 = K + 1 <= kmax ? cK + NXY : cK, l4 = K + 2 <= kmax ? cK + 2 * NXY : cK;
+    for (int t = t0; t < t1; t++) {
       const float* f = fin + t * ts;
       const float fKm2 = K > 1 ? f[l0] : 0.0f, fKm1 = f[l1], fK = f[cK];
       const float fKp1 = f[l3], fKp2 = f[l4];
       const float p = wflux_fast<SCH>(wfp, limp_up, limp_dn, fKm2, fKm1, fK, fKp1);
-      const float qf = wflux_fast<SCH>(wfk, limk_up, limk_dn, fKm1, fK, fKp1, fKp2);
-      const float wprev = wp_on ? p : 0.0f, wk = wk_on ? qf : 0.0f;
+      const float q = wflux_fast<SCH>(wfk, limk_up, limk_dn, fKm1, fK, fKp1, fKp2);
+      const float wprev = wp_on ? p : 0.0f, wk = wk_on ? q : 0.0f;
       fout[t * ts + cK] = kdo ? cell_update_fast(k3, fK, wk, wprev) : fK;
     }
+    dm = dk; dk = dp;
+    dp = K + 2 <= kmax ? D2[c2 + NXY * (K + 2)] : 0.0;
+    wm = wkk;
+    wkk = K + 1 <= kmax ? ww[c2 + NXY * (K + 1)] : 0.0;
   }
 }
 
@@ -350,8 +361,13 @@ __global__ __launch_bounds__(256) void k_sub_h_f32_adj(
 }
 
 // ---- transpose of the vertical sub-step ----------------------------------------
-// one thread per column: lam_out starts as the diagonal part, then every face
-// K' adds its flux adjoint times the flux's partials at levels K'-1..K'+2
+// one thread per column and tracer (blockIdx.z).  Marching over the faces
+// K' = 0..kmax-1, the window holds levels K'-1..K'+2 (the flux at K'+1/2
+// reads them): each level enters with its diagonal part (the adjoint through
+// its own value) and mu = -b*lam (the adjoint of the flux through its upper
+// face, into its own update), collects the face terms, and is stored when it
+// leaves.  A face is the wk of level K' and the wprev of level K'+1 (K' >= 1,
+// or K' = 0 with kshift).
 template <int SCH>
 __global__ __launch_bounds__(256) void k_sub_w_f32_adj(Dims d, int ntr, long ts,
                                                        const float* __restrict__ f,
@@ -364,68 +380,70 @@ __global__ __launch_bounds__(256) void k_sub_w_f32_adj(Dims d, int ntr, long ts,
                                                        double dt, int kshift) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  if (i >= d.NX || j >= d.NY) return;
+  const int t = blockIdx.z;
+  if (i >= d.NX || j >= d.NY || t >= ntr) return;
   const int kmax = d.kmax;
   const long NXY = d.NXY, c2 = i + (long)d.NX * j;
+  const float* ft = f + t * ts;
+  const float* lt = lam_in + t * ts;
+  float* lo = lam_out + t * ts;
   const bool wcol = (flags[c2] & MB_WCOL) && i <= d.NX - 2 && j <= d.NY - 2;
   if (!wcol) {
-    for (int t = 0; t < ntr; t++)
-      for (int K = 0; K <= kmax; K++) lam_out[t * ts + c2 + NXY * K] = lam_in[t * ts + c2 + NXY * K];
+    for (int K = 0; K <= kmax; K++) lo[c2 + NXY * K] = lt[c2 + NXY * K];
     return;
   }
-  auto kdo = [&](int K) { return K >= 1 && K <= kmax - kshift; };
-  for (int t = 0; t < ntr; t++) {
-    const float* ft = f + t * ts;
-    const float* lt = lam_in + t * ts;
-    float* lo = lam_out + t * ts;
-    for (int K = 0; K <= kmax; K++) {
-      const long cK = c2 + NXY * K;
-      const float l = lt[cK];
-      if (kdo(K)) {
-        const CellK ck = cell_factors(D2[cK], D3[cK], dt);
-        lo[cK] = __builtin_fmaf(l, ck.am1, l);
-      } else {
-        lo[cK] = l;
-      }
+  struct Lev {
+    float f, acc, mu;
+  };
+  auto enter = [&](int L) -> Lev {
+    if (L > kmax) return {0.0f, 0.0f, 0.0f};
+    const long cL = c2 + NXY * L;
+    const float l = lt[cL];
+    Lev v{ft[cL], l, 0.0f};
+    if (L >= 1 && L <= kmax - kshift) {
+      const CellK ck = cell_factors(D2[cL], D3[cL], dt);
+      v.acc = __builtin_fmaf(l, ck.am1, l);
+      v.mu = -ck.b * l;
     }
-    // mu(K) = -b(K) lam(K) where level K is updated
-    auto mu = [&](int K) -> float {
-      if (!kdo(K)) return 0.0f;
-      const long cK = c2 + NXY * K;
-      return -cell_factors(D2[cK], D3[cK], dt).b * lt[cK];
-    };
-    // face Kf (Kf+1/2): the wk of level Kf (Kf <= kmax-1) and the wprev of
-    // level Kf+1 (Kf >= 1, or Kf = 0 with kshift)
-    const int kf0 = kshift ? 0 : 1;
-    float mu_lo = mu(kf0);
-    for (int Kf = kf0; Kf <= kmax - 1; Kf++) {
-      const float mu_hi = mu(Kf + 1);
-      const float Fbar = (Kf >= 1 ? mu_lo : 0.0f) - mu_hi;
-      mu_lo = mu_hi;
-      if (Fbar == 0.0f) continue;
-      const long cf = c2 + NXY * Kf;
+    return v;
+  };
+  Lev w0{0.0f, 0.0f, 0.0f}, w1 = enter(0), w2 = enter(1), w3 = enter(2);
+  double d0 = D2[c2], d1 = D2[c2 + NXY];  // D2 at K', K'+1
+  for (int Kf = 0; Kf <= kmax - 1; Kf++) {
+    // level 0 is never updated (mu = 0), so at K' = 0 this is -mu(1) with
+    // kshift and 0 without
+    const float Fbar = (Kf >= 1 || kshift) ? w1.mu - w2.mu : 0.0f;
+    if (Fbar != 0.0f) {
       WFace<false> wf;
-      wface_setup<SCH>(wf, ww[cf], dt, D2[cf], D2[cf + NXY]);
+      wface_setup<SCH>(wf, ww[c2 + NXY * Kf], dt, d0, d1);
       const bool lim_up = SCH != UPSTREAM && Kf > 1 - kshift;
       const bool lim_dn = SCH != UPSTREAM && Kf < kmax - 1;
       const bool up = wf.w > 0.0f;
-      const float fkm1 = Kf >= 1 ? ft[cf - NXY] : 0.0f, fk = ft[cf], fk1 = ft[cf + NXY];
-      const float fk2 = Kf + 2 <= kmax ? ft[cf + 2 * NXY] : fk;
+      const float fkm1 = Kf >= 1 ? w0.f : 0.0f, fk = w1.f, fk1 = w2.f;
+      const float fk2 = Kf + 2 <= kmax ? w3.f : fk;
       const float fu = up ? fk : fk1, fuu = up ? fkm1 : fk2, fd = up ? fk1 : fk;
       const bool lim = up ? lim_up : lim_dn;
       const RecGrad32 r = rec_fast_grad<SCH>(fuu, fu, fd, lim ? wf.g : 0.0f, wf.A, wf.al, wf.E);
       const float a = Fbar * wf.w;
+      // (a slot past either end of the column is never stored)
       if (up) {
-        if (Kf >= 1) lo[cf - NXY] = __builtin_fmaf(a, r.duu, lo[cf - NXY]);
-        lo[cf] = __builtin_fmaf(a, r.du, lo[cf]);
-        lo[cf + NXY] = __builtin_fmaf(a, r.dd, lo[cf + NXY]);
+        w0.acc = __builtin_fmaf(a, r.duu, w0.acc);
+        w1.acc = __builtin_fmaf(a, r.du, w1.acc);
+        w2.acc = __builtin_fmaf(a, r.dd, w2.acc);
       } else {
-        lo[cf] = __builtin_fmaf(a, r.dd, lo[cf]);
-        lo[cf + NXY] = __builtin_fmaf(a, r.du, lo[cf + NXY]);
-        if (Kf + 2 <= kmax) lo[cf + 2 * NXY] = __builtin_fmaf(a, r.duu, lo[cf + 2 * NXY]);
+        w1.acc = __builtin_fmaf(a, r.dd, w1.acc);
+        w2.acc = __builtin_fmaf(a, r.du, w2.acc);
+        w3.acc = __builtin_fmaf(a, r.duu, w3.acc);
       }
     }
+    if (Kf >= 1) lo[c2 + NXY * (Kf - 1)] = w0.acc;
+    w0 = w1; w1 = w2; w2 = w3;
+    w3 = enter(Kf + 3);
+    d0 = d1;
+    d1 = Kf + 2 <= kmax ? D2[c2 + NXY * (Kf + 2)] : 0.0;
   }
+  lo[c2 + NXY * (kmax - 1)] = w0.acc;
+  lo[c2 + NXY * kmax] = w1.acc;
 }
 
 // ---- launchers --------------------------------------------------------------------
@@ -503,11 +521,12 @@ static void launch_w_one(hipStream_t st, Dims d, int ntr, long ts, const float* 
                          const float* b, float* out, const double* D2, const double* D3,
                          const double* ww, const uint8_t* flags, double dt, int kshift) {
   if constexpr (ADJ) {
-    hipLaunchKernelGGL((k_sub_w_f32_adj<SCH>), grid3(d, 1), dim3(64, 4), 0, st, d, ntr, ts, a, b,
+    hipLaunchKernelGGL((k_sub_w_f32_adj<SCH>), grid3(d, ntr), dim3(64, 4), 0, st, d, ntr, ts, a, b,
                        out, D2, D3, ww, flags, dt, kshift);
   } else {
     (void)b;
-    hipLaunchKernelGGL((k_sub_w_f32<SCH>), grid3(d, 1), dim3(64, 4), 0, st, d, ntr, ts, a, out,
+    hipLaunchKernelGGL((k_sub_w_f32<SCH>), grid3(d, (ntr + kWGroup - 1) / kWGroup), dim3(64, 4),
+                       0, st, d, ntr, ts, a, out,
                        D2, D3, ww, flags, dt, kshift);
   }
 }
