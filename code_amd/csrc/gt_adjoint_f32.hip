@@ -90,45 +90,115 @@ __device__ __forceinline__ FaceC<false> face_at(const Sweep& s, int X, long x2, 
   return fc;
 }
 
-// levels 0..kmax, one thread per cell; level 0 and cells the sub-step leaves
-// alone are copied
+// the u half sub-step: a wavefront holds 64 consecutive cells of a row, lane L
+// forms the flux of face x from the cells x-1..x+2 (cross-lane moves) and
+// cell x takes the flux of face x-1 from lane L-1, as the strip kernels do:
+// lanes 2..61 are complete, 60 cells per wavefront.  Level 0 and cells the
+// sub-step leaves alone are copied.
+constexpr int kUFwdOut = 60;
+
 template <int SCH, bool DIFF>
-__global__ __launch_bounds__(256) void k_sub_h_f32(
-    Dims d, int dir, int ntr, long ts, const float* __restrict__ fin, float* __restrict__ fout,
-    const double* __restrict__ Dio, const double* __restrict__ Dn, const double* __restrict__ vel,
-    const double* __restrict__ Dv, const uint8_t* __restrict__ flags, double dt,
-    double dlen_flux, double dlen_face, double ard1, double AH) {
+__global__ __launch_bounds__(256) void k_sub_u_f32(
+    Dims d, int ntr, long ts, const float* __restrict__ fin, float* __restrict__ fout,
+    const double* __restrict__ Dio, const double* __restrict__ Dn, const double* __restrict__ uu,
+    const double* __restrict__ hu, const uint8_t* __restrict__ flags, double dt, double dx,
+    double dy, double ard1, double AH) {
   constexpr int isub = SCH == UPSTREAM ? 0 : 1;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x;  // blockDim (64, 4): one row per wavefront
+  const int x = blockIdx.x * kUFwdOut - 2 + lane;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   const int k = blockIdx.z;
-  if (i >= d.NX || j >= d.NY || k > d.kmax) return;
-  const long c2 = i + (long)d.NX * j, o = d.NXY * (long)k, c = o + c2;
-  const Sweep s = sweep(d, dir, i, j);
-  const bool upd = k >= 1 && s.pos >= 1 + isub && s.pos <= s.n - 2 - isub && (flags[c2] & s.mupd);
-  if (!upd) {
-    for (int t = 0; t < ntr; t++) fout[t * ts + c] = fin[t * ts + c];
+  if (j >= d.NY) return;  // the whole wavefront
+  const bool xin = x >= 0 && x < d.NX;
+  const long c2 = (long)min(max(x, 0), d.NX - 1) + (long)d.NX * j;
+  const long o = d.NXY * (long)k, c = o + c2;
+  const bool out = xin && lane >= 2 && lane <= 61;
+  if (k == 0) {
+    if (out)
+      for (int t = 0; t < ntr; t++) fout[t * ts + c] = fin[t * ts + c];
     return;
   }
-  const FaceK fk = face_k(0.5 * dt, dlen_flux, AH);
-  const FaceC<false> fcc = face_at<SCH>(s, s.pos, c2, o, flags, vel, Dv, fk);
-  const FaceC<false> fcm = face_at<SCH>(s, s.pos - 1, c2 - s.st, o, flags, vel, Dv, fk);
-  const CellK ck = cell_factors(Dio[c], Dn[c], dt * (0.5 * dlen_face * ard1));
-  // cells pos-2..pos+2 (clamped into the array: UPSTREAM reaches past the
-  // ends with operands it does not use)
-  long q[5];
-#pragma unroll
-  for (int m = 0; m < 5; m++) {
-    int p = s.pos - 2 + m;
-    p = p < 0 ? 0 : (p > s.n - 1 ? s.n - 1 : p);
-    q[m] = c + (long)(p - s.pos) * s.st;
-  }
+  const Sweep s = sweep(d, 0, x, j);
+  FaceC<false> fc{};
+  if (xin && x >= isub && x <= d.NX - 2 - isub)
+    fc = face_at<SCH>(s, x, c2, o, flags, uu, hu, face_k(0.5 * dt, dx, AH));
+  const bool upd = xin && x >= 1 + isub && x <= d.NX - 2 - isub && (flags[c2] & MB_UUPD);
+  CellK ck{0.0f, 0.0f};
+  if (upd) ck = cell_factors(Dio[c], Dn[c], dt * (0.5 * dy * ard1));
   for (int t = 0; t < ntr; t++) {
-    const float* f = fin + t * ts;
-    const float fm2 = f[q[0]], fm1 = f[q[1]], f0 = f[q[2]], fp1 = f[q[3]], fp2 = f[q[4]];
-    const float Fc = face_flux<float, SCH, DIFF>(fcc, fm1, f0, fp1, fp2, AH, dlen_flux);
-    const float Fm = face_flux<float, SCH, DIFF>(fcm, fm2, fm1, f0, fp1, AH, dlen_flux);
-    fout[t * ts + c] = cell_update_fast(ck, f0, Fc, Fm);
+    const float fx = fin[t * ts + c];
+    const float fm1 = from_lower_lane(fx), fp1 = from_upper_lane(fx);
+    const float fp2 = from_upper_lane(fp1);
+    const float Fc = face_flux<float, SCH, DIFF>(fc, fm1, fx, fp1, fp2, AH, dx);
+    const float Fm = from_lower_lane(Fc);
+    const float u = cell_update_fast(ck, fx, Fc, Fm);
+    if (out) fout[t * ts + c] = upd ? u : fx;
+  }
+}
+
+// the v half sub-step: a wavefront holds 64 columns and marches the faces
+// Y = y0-1 .. y1-1 for a group of G tracers, with the rows Y-1..Y+2 and the
+// flux of face Y-1 in registers; row Y is updated after face Y
+constexpr int kVRows = 16;
+constexpr int kVGroup = 6;
+
+template <int SCH, bool DIFF>
+__global__ __launch_bounds__(64) void k_sub_v_f32(
+    Dims d, int ntr, long ts, const float* __restrict__ fin, float* __restrict__ fout,
+    const double* __restrict__ Dio, const double* __restrict__ Dn, const double* __restrict__ vv,
+    const double* __restrict__ hv, const uint8_t* __restrict__ flags, double dt, double dx,
+    double dy, double ard1, double AH, int ngroups) {
+  constexpr int isub = SCH == UPSTREAM ? 0 : 1;
+  constexpr int G = kVGroup;
+  const int x = blockIdx.x * 64 + threadIdx.x;
+  const int y0 = blockIdx.y * kVRows;
+  const int k = blockIdx.z / ngroups, t0 = (blockIdx.z % ngroups) * G;
+  if (x >= d.NX) return;  // no cross-lane moves here
+  const int NX = d.NX, NY = d.NY;
+  const long o = d.NXY * (long)k;
+  const int y1 = min(y0 + kVRows, NY);
+  long tb[G];
+#pragma unroll
+  for (int g = 0; g < G; g++) tb[g] = (long)min(t0 + g, ntr - 1) * ts;
+  if (k == 0) {
+    for (int y = y0; y < y1; y++)
+#pragma unroll
+      for (int g = 0; g < G; g++)
+        if (t0 + g < ntr) fout[tb[g] + x + (long)NX * y] = fin[tb[g] + x + (long)NX * y];
+    return;
+  }
+  const Sweep s{(long)NX, 0, NY, MB_VFLUX, MB_VUPD};
+  const FaceK fk = face_k(0.5 * dt, dy, AH);
+  const double dtc = dt * (0.5 * dx * ard1);
+  auto row = [&](int y) -> long { return o + x + (long)NX * (y >= 0 && y < NY ? y : 0); };
+  float w[G][4], Fp[G];
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    Fp[g] = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 4; q++) w[g][q] = fin[tb[g] + row(y0 - 2 + q)];
+  }
+  for (int Y = y0 - 1; Y < y1; Y++) {
+    FaceC<false> fc{};
+    if (Y >= isub && Y <= NY - 2 - isub) fc = face_at<SCH>(s, Y, x + (long)NX * Y, o, flags, vv, hv, fk);
+    const bool st = Y >= y0;
+    const long cY = o + x + (long)NX * (Y >= 0 ? Y : 0);
+    const bool upd = st && Y >= 1 + isub && Y <= NY - 2 - isub && (flags[cY - o] & MB_VUPD);
+    CellK ck{0.0f, 0.0f};
+    if (upd) ck = cell_factors(Dio[cY], Dn[cY], dtc);
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const float F = face_flux<float, SCH, DIFF>(fc, w[g][0], w[g][1], w[g][2], w[g][3], AH, dy);
+      if (st && t0 + g < ntr) {
+        const float u = cell_update_fast(ck, w[g][1], F, Fp[g]);
+        fout[tb[g] + cY] = upd ? u : w[g][1];
+      }
+      Fp[g] = F;
+      w[g][0] = w[g][1];
+      w[g][1] = w[g][2];
+      w[g][2] = w[g][3];
+      w[g][3] = fin[tb[g] + row(Y + 3)];
+    }
   }
 }
 
@@ -273,101 +343,155 @@ __device__ __forceinline__ void face_flux_grad32(const FaceC<false>& fc, float f
   }
 }
 
-// ---- transpose of a horizontal sub-step ----------------------------------------
-// f: the state before the sub-step; lam_in: adjoint of its output; lam_out:
-// adjoint of its input.  Cell pos sits in the stencils of faces pos-2..pos+1.
+// ---- transpose of the u sub-step: one face per lane, cross-lane sums --------
+// A wavefront holds 64 consecutive cells of a row; lane L owns cell x and the
+// face x (between x and x+1).  Face x reads the cells x-1..x+2 and its flux
+// enters the updates of cells x and x+1; cell x collects the faces x-2..x+1
+// (lanes L-2..L+1).  So lanes 3..60 are complete: 58 cells per wavefront.
+constexpr int kUOut = 58;
+
 template <int SCH, bool DIFF>
-__global__ __launch_bounds__(256) void k_sub_h_f32_adj(
-    Dims d, int dir, int ntr, long ts, const float* __restrict__ f,
-    const float* __restrict__ lam_in, float* __restrict__ lam_out,
-    const double* __restrict__ Dio, const double* __restrict__ Dn, const double* __restrict__ vel,
-    const double* __restrict__ Dv, const uint8_t* __restrict__ flags, double dt,
-    double dlen_flux, double dlen_face, double ard1, double AH) {
+__global__ __launch_bounds__(256) void k_sub_u_f32_adj(
+    Dims d, int ntr, long ts, const float* __restrict__ f, const float* __restrict__ lam_in,
+    float* __restrict__ lam_out, const double* __restrict__ Dio, const double* __restrict__ Dn,
+    const double* __restrict__ uu, const double* __restrict__ hu,
+    const uint8_t* __restrict__ flags, double dt, double dx, double dy, double ard1, double AH) {
   constexpr int isub = SCH == UPSTREAM ? 0 : 1;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x;  // blockDim (64, 4): one row per wavefront
+  const int x = blockIdx.x * kUOut - 3 + lane;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   const int k = blockIdx.z;
-  if (i >= d.NX || j >= d.NY || k > d.kmax) return;
-  const long c2 = i + (long)d.NX * j, o = d.NXY * (long)k, c = o + c2;
+  if (j >= d.NY) return;  // the whole wavefront
+  const bool xin = x >= 0 && x < d.NX;
+  const long c2 = (long)min(max(x, 0), d.NX - 1) + (long)d.NX * j;
+  const long o = d.NXY * (long)k, c = o + c2;
+  const bool out = xin && lane >= 3 && lane <= 60;
   if (k == 0) {
-    for (int t = 0; t < ntr; t++) lam_out[t * ts + c] = lam_in[t * ts + c];
+    if (out)
+      for (int t = 0; t < ntr; t++) lam_out[t * ts + c] = lam_in[t * ts + c];
     return;
   }
-  const Sweep s = sweep(d, dir, i, j);
-  const FaceK fk = face_k(0.5 * dt, dlen_flux, AH);
-  const double dtc = dt * (0.5 * dlen_face * ard1);
-  // cells pos-2..pos+2: updated?, the factors of their update
-  bool updm[5];
-  float bm[5];
-  float am1c = 0.0f;
-  long q[7];  // element offsets of cells pos-3..pos+3 (clamped), q[m+1] = pos-2+m
-#pragma unroll
-  for (int m = 0; m < 7; m++) {
-    int p = s.pos - 3 + m;
-    p = p < 0 ? 0 : (p > s.n - 1 ? s.n - 1 : p);
-    q[m] = c + (long)(p - s.pos) * s.st;
-  }
-#pragma unroll
-  for (int m = 0; m < 5; m++) {
-    const int p = s.pos - 2 + m;
-    const long x2 = c2 + (long)(p - s.pos) * s.st;
-    updm[m] = p >= 1 + isub && p <= s.n - 2 - isub && (flags[x2] & s.mupd);
-    bm[m] = 0.0f;
-    if (updm[m]) {
-      const CellK ck = cell_factors(Dio[o + x2], Dn[o + x2], dtc);
-      bm[m] = ck.b;
-      if (m == 2) am1c = ck.am1;
-    }
-  }
-  // faces X = pos-2..pos+1 (index m: X = pos-2+m); a face outside 0..n-2 or
-  // inactive contributes nothing
-  FaceC<false> fc[4];
-  bool fon[4];
-#pragma unroll
-  for (int m = 0; m < 4; m++) {
-    const int X = s.pos - 2 + m;
-    fon[m] = X >= isub && X <= s.n - 2 - isub && (updm[m] || updm[m + 1]);
-    if (fon[m]) {
-      fc[m] = face_at<SCH>(s, X, c2 + (long)(X - s.pos) * s.st, o, flags, vel, Dv, fk);
-    } else {
-      fc[m] = FaceC<false>{};
-    }
-  }
+  const Sweep s = sweep(d, 0, x, j);
+  FaceC<false> fc{};
+  if (xin && x >= isub && x <= d.NX - 2 - isub)
+    fc = face_at<SCH>(s, x, c2, o, flags, uu, hu, face_k(0.5 * dt, dx, AH));
+  const bool upd = xin && x >= 1 + isub && x <= d.NX - 2 - isub && (flags[c2] & MB_UUPD);
+  CellK ck{0.0f, 0.0f};
+  if (upd) ck = cell_factors(Dio[c], Dn[c], dt * (0.5 * dy * ard1));
   for (int t = 0; t < ntr; t++) {
-    const float* ft = f + t * ts;
-    const float* lt = lam_in + t * ts;
-    const float lc = lt[c];
-    float acc = updm[2] ? __builtin_fmaf(lc, am1c, lc) : lc;
-    float fv[7];
+    const float fx = f[t * ts + c], lx = lam_in[t * ts + c];
+    const float fm1 = from_lower_lane(fx), fp1 = from_upper_lane(fx);
+    const float fp2 = from_upper_lane(fp1);
+    const float mu = upd ? -ck.b * lx : 0.0f;  // adjoint of the flux through face x into cell x
+    const float Fbar = mu - from_upper_lane(mu);
+    float g[4];
+    face_flux_grad32<SCH, DIFF>(fc, fm1, fx, fp1, fp2, g);
+    float acc = upd ? __builtin_fmaf(lx, ck.am1, lx) : lx;
+    acc = __builtin_fmaf(Fbar, g[1], acc);                      // face x
+    acc += from_upper_lane(Fbar * g[0]);                         // face x+1
+    acc += from_lower_lane(Fbar * g[2]);                         // face x-1
+    acc += from_lower_lane(from_lower_lane(Fbar * g[3]));        // face x-2
+    if (out) lam_out[t * ts + c] = acc;
+  }
+}
+
+// ---- transpose of the v sub-step: a row march -----------------------------------
+// A wavefront holds 64 columns and marches the faces Y = y0-2 .. y0+R (the
+// faces of the output rows y0..y0+R-1) for a group of G tracers.  Before face
+// Y the window holds rows Y-1..Y+2: each row enters with its diagonal part
+// and mu = -b*lam, collects the terms of the faces whose stencil holds it, and
+// is stored when it leaves (row Y-1 after face Y).
+template <int SCH, bool DIFF>
+__global__ __launch_bounds__(64) void k_sub_v_f32_adj(
+    Dims d, int ntr, long ts, const float* __restrict__ f, const float* __restrict__ lam_in,
+    float* __restrict__ lam_out, const double* __restrict__ Dio, const double* __restrict__ Dn,
+    const double* __restrict__ vv, const double* __restrict__ hv,
+    const uint8_t* __restrict__ flags, double dt, double dx, double dy, double ard1, double AH,
+    int ngroups) {
+  constexpr int isub = SCH == UPSTREAM ? 0 : 1;
+  constexpr int G = kVGroup;
+  const int x = blockIdx.x * 64 + threadIdx.x;
+  const int y0 = blockIdx.y * kVRows;
+  const int k = blockIdx.z / ngroups, t0 = (blockIdx.z % ngroups) * G;
+  if (x >= d.NX) return;  // no cross-lane moves here
+  const int NX = d.NX, NY = d.NY;
+  const long o = d.NXY * (long)k;
+  const int y1 = min(y0 + kVRows, NY);  // output rows y0..y1-1
+  long tb[G];
 #pragma unroll
-    for (int m = 0; m < 7; m++) fv[m] = ft[q[m]];
-    // mu(cell) = -b * lam: the adjoint of the flux through the cell's right
-    // face (its left face enters with the opposite sign)
-    float mu[5];
+  for (int g = 0; g < G; g++) tb[g] = (long)min(t0 + g, ntr - 1) * ts;
+  if (k == 0) {
+    for (int y = y0; y < y1; y++)
 #pragma unroll
-    for (int m = 0; m < 5; m++) mu[m] = updm[m] ? -bm[m] * lt[q[m + 1]] : 0.0f;
+      for (int g = 0; g < G; g++)
+        if (t0 + g < ntr) lam_out[tb[g] + x + (long)NX * y] = lam_in[tb[g] + x + (long)NX * y];
+    return;
+  }
+  const Sweep s{(long)NX, 0, NY, MB_VFLUX, MB_VUPD};
+  const FaceK fk = face_k(0.5 * dt, dy, AH);
+  const double dtc = dt * (0.5 * dx * ard1);
+  struct Row {
+    float f, acc, mu;
+  };
+  Row w[G][4];
+  // row y enters the window slot q
+  auto enter = [&](int y, int q) {
+    const bool yin = y >= 0 && y < NY;
+    const long c = o + x + (long)NX * (yin ? y : 0);
+    const bool upd = yin && y >= 1 + isub && y <= NY - 2 - isub && (flags[c - o] & MB_VUPD);
+    CellK ck{0.0f, 0.0f};
+    if (upd) ck = cell_factors(Dio[c], Dn[c], dtc);
 #pragma unroll
-    for (int m = 0; m < 4; m++) {
-      if (!fon[m]) continue;
-      const float Fbar = mu[m] - mu[m + 1];
-      float g[4];
-      // face X = pos-2+m reads cells X-1..X+2 = fv[m..m+3]
-      face_flux_grad32<SCH, DIFF>(fc[m], fv[m], fv[m + 1], fv[m + 2], fv[m + 3], g);
-      // cell pos is stencil slot pos - X + 1 = 3 - m
-      acc = __builtin_fmaf(Fbar, g[3 - m], acc);
+    for (int g = 0; g < G; g++) {
+      const float l = lam_in[tb[g] + c];
+      w[g][q].f = f[tb[g] + c];
+      w[g][q].acc = upd ? __builtin_fmaf(l, ck.am1, l) : l;
+      w[g][q].mu = upd ? -ck.b * l : 0.0f;
     }
-    lam_out[t * ts + c] = acc;
+  };
+  enter(y0 - 3, 0);
+  enter(y0 - 2, 1);
+  enter(y0 - 1, 2);
+  enter(y0, 3);
+  for (int Y = y0 - 2; Y <= y1; Y++) {
+    FaceC<false> fc{};
+    if (Y >= isub && Y <= NY - 2 - isub) {
+      const long y2 = x + (long)NX * Y;
+      fc = face_at<SCH>(s, Y, y2, o, flags, vv, hv, fk);
+    }
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const float Fbar = w[g][1].mu - w[g][2].mu;
+      float gr[4];
+      face_flux_grad32<SCH, DIFF>(fc, w[g][0].f, w[g][1].f, w[g][2].f, w[g][3].f, gr);
+#pragma unroll
+      for (int q = 0; q < 4; q++) w[g][q].acc = __builtin_fmaf(Fbar, gr[q], w[g][q].acc);
+    }
+    const int yd = Y - 1;  // complete
+    if (yd >= y0 && yd < y1) {
+#pragma unroll
+      for (int g = 0; g < G; g++)
+        if (t0 + g < ntr) lam_out[tb[g] + o + x + (long)NX * yd] = w[g][0].acc;
+    }
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      w[g][0] = w[g][1];
+      w[g][1] = w[g][2];
+      w[g][2] = w[g][3];
+    }
+    enter(Y + 3, 3);
   }
 }
 
 // ---- transpose of the vertical sub-step ----------------------------------------
-// one thread per column and tracer (blockIdx.z).  Marching over the faces
-// K' = 0..kmax-1, the window holds levels K'-1..K'+2 (the flux at K'+1/2
-// reads them): each level enters with its diagonal part (the adjoint through
-// its own value) and mu = -b*lam (the adjoint of the flux through its upper
-// face, into its own update), collects the face terms, and is stored when it
-// leaves.  A face is the wk of level K' and the wprev of level K'+1 (K' >= 1,
-// or K' = 0 with kshift).
+// one thread per column and group of kVGroup tracers (blockIdx.z).  Marching
+// over the faces K' = 0..kmax-1, the window holds levels K'-1..K'+2 (the
+// flux at K'+1/2 reads them): each level enters with its diagonal part (the
+// adjoint through its own value) and mu = -b*lam (the adjoint of the flux
+// through its upper face, into its own update), collects the face terms, and
+// is stored when it leaves.  A face is the wk of level K' and the wprev of
+// level K'+1 (K' >= 1, or K' = 0 with kshift).  The face constants and the
+// cell factors serve the whole group.
 template <int SCH>
 __global__ __launch_bounds__(256) void k_sub_w_f32_adj(Dims d, int ntr, long ts,
                                                        const float* __restrict__ f,
@@ -378,72 +502,96 @@ __global__ __launch_bounds__(256) void k_sub_w_f32_adj(Dims d, int ntr, long ts,
                                                        const double* __restrict__ ww,
                                                        const uint8_t* __restrict__ flags,
                                                        double dt, int kshift) {
+  constexpr int G = kVGroup;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int t = blockIdx.z;
-  if (i >= d.NX || j >= d.NY || t >= ntr) return;
+  const int t0 = blockIdx.z * G;
+  if (i >= d.NX || j >= d.NY || t0 >= ntr) return;
   const int kmax = d.kmax;
   const long NXY = d.NXY, c2 = i + (long)d.NX * j;
-  const float* ft = f + t * ts;
-  const float* lt = lam_in + t * ts;
-  float* lo = lam_out + t * ts;
+  long tb[G];
+#pragma unroll
+  for (int g = 0; g < G; g++) tb[g] = (long)min(t0 + g, ntr - 1) * ts;
   const bool wcol = (flags[c2] & MB_WCOL) && i <= d.NX - 2 && j <= d.NY - 2;
   if (!wcol) {
-    for (int K = 0; K <= kmax; K++) lo[c2 + NXY * K] = lt[c2 + NXY * K];
+    for (int K = 0; K <= kmax; K++)
+#pragma unroll
+      for (int g = 0; g < G; g++)
+        if (t0 + g < ntr) lam_out[tb[g] + c2 + NXY * K] = lam_in[tb[g] + c2 + NXY * K];
     return;
   }
   struct Lev {
     float f, acc, mu;
   };
-  auto enter = [&](int L) -> Lev {
-    if (L > kmax) return {0.0f, 0.0f, 0.0f};
-    const long cL = c2 + NXY * L;
-    const float l = lt[cL];
-    Lev v{ft[cL], l, 0.0f};
-    if (L >= 1 && L <= kmax - kshift) {
-      const CellK ck = cell_factors(D2[cL], D3[cL], dt);
-      v.acc = __builtin_fmaf(l, ck.am1, l);
-      v.mu = -ck.b * l;
+  Lev w[G][4];
+  auto enter = [&](int L, int q) {
+    const bool in = L <= kmax;
+    const long cL = c2 + NXY * (in ? L : kmax);
+    const bool kdo = in && L >= 1 && L <= kmax - kshift;
+    CellK ck{0.0f, 0.0f};
+    if (kdo) ck = cell_factors(D2[cL], D3[cL], dt);
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const float l = lam_in[tb[g] + cL];
+      w[g][q].f = f[tb[g] + cL];
+      w[g][q].acc = kdo ? __builtin_fmaf(l, ck.am1, l) : l;
+      w[g][q].mu = kdo ? -ck.b * l : 0.0f;
     }
-    return v;
   };
-  Lev w0{0.0f, 0.0f, 0.0f}, w1 = enter(0), w2 = enter(1), w3 = enter(2);
+#pragma unroll
+  for (int g = 0; g < G; g++) w[g][0] = Lev{0.0f, 0.0f, 0.0f};
+  enter(0, 1);
+  enter(1, 2);
+  enter(2, 3);
   double d0 = D2[c2], d1 = D2[c2 + NXY];  // D2 at K', K'+1
   for (int Kf = 0; Kf <= kmax - 1; Kf++) {
-    // level 0 is never updated (mu = 0), so at K' = 0 this is -mu(1) with
+    // level 0 is never updated (mu = 0), so at K' = 0 Fbar is -mu(1) with
     // kshift and 0 without
-    const float Fbar = (Kf >= 1 || kshift) ? w1.mu - w2.mu : 0.0f;
-    if (Fbar != 0.0f) {
-      WFace<false> wf;
-      wface_setup<SCH>(wf, ww[c2 + NXY * Kf], dt, d0, d1);
-      const bool lim_up = SCH != UPSTREAM && Kf > 1 - kshift;
-      const bool lim_dn = SCH != UPSTREAM && Kf < kmax - 1;
-      const bool up = wf.w > 0.0f;
-      const float fkm1 = Kf >= 1 ? w0.f : 0.0f, fk = w1.f, fk1 = w2.f;
-      const float fk2 = Kf + 2 <= kmax ? w3.f : fk;
+    const bool used = Kf >= 1 || kshift;
+    WFace<false> wf;
+    wface_setup<SCH>(wf, ww[c2 + NXY * Kf], dt, d0, d1);
+    const bool up = wf.w > 0.0f;
+    const bool lim = up ? (SCH != UPSTREAM && Kf > 1 - kshift)
+                        : (SCH != UPSTREAM && Kf < kmax - 1);
+    const float gl = lim ? wf.g : 0.0f;
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const float Fbar = used ? w[g][1].mu - w[g][2].mu : 0.0f;
+      const float fkm1 = Kf >= 1 ? w[g][0].f : 0.0f, fk = w[g][1].f, fk1 = w[g][2].f;
+      const float fk2 = Kf + 2 <= kmax ? w[g][3].f : fk;
       const float fu = up ? fk : fk1, fuu = up ? fkm1 : fk2, fd = up ? fk1 : fk;
-      const bool lim = up ? lim_up : lim_dn;
-      const RecGrad32 r = rec_fast_grad<SCH>(fuu, fu, fd, lim ? wf.g : 0.0f, wf.A, wf.al, wf.E);
+      const RecGrad32 r = rec_fast_grad<SCH>(fuu, fu, fd, gl, wf.A, wf.al, wf.E);
       const float a = Fbar * wf.w;
       // (a slot past either end of the column is never stored)
-      if (up) {
-        w0.acc = __builtin_fmaf(a, r.duu, w0.acc);
-        w1.acc = __builtin_fmaf(a, r.du, w1.acc);
-        w2.acc = __builtin_fmaf(a, r.dd, w2.acc);
-      } else {
-        w1.acc = __builtin_fmaf(a, r.dd, w1.acc);
-        w2.acc = __builtin_fmaf(a, r.du, w2.acc);
-        w3.acc = __builtin_fmaf(a, r.duu, w3.acc);
-      }
+      const float a0 = up ? a * r.duu : 0.0f, a1 = up ? a * r.du : a * r.dd;
+      const float a2 = up ? a * r.dd : a * r.du, a3 = up ? 0.0f : a * r.duu;
+      w[g][0].acc += a0;
+      w[g][1].acc += a1;
+      w[g][2].acc += a2;
+      w[g][3].acc += a3;
     }
-    if (Kf >= 1) lo[c2 + NXY * (Kf - 1)] = w0.acc;
-    w0 = w1; w1 = w2; w2 = w3;
-    w3 = enter(Kf + 3);
+    if (Kf >= 1) {
+#pragma unroll
+      for (int g = 0; g < G; g++)
+        if (t0 + g < ntr) lam_out[tb[g] + c2 + NXY * (Kf - 1)] = w[g][0].acc;
+    }
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      w[g][0] = w[g][1];
+      w[g][1] = w[g][2];
+      w[g][2] = w[g][3];
+    }
+    enter(Kf + 3, 3);
     d0 = d1;
     d1 = Kf + 2 <= kmax ? D2[c2 + NXY * (Kf + 2)] : 0.0;
   }
-  lo[c2 + NXY * (kmax - 1)] = w0.acc;
-  lo[c2 + NXY * kmax] = w1.acc;
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    if (t0 + g < ntr) {
+      lam_out[tb[g] + c2 + NXY * (kmax - 1)] = w[g][0].acc;
+      lam_out[tb[g] + c2 + NXY * kmax] = w[g][1].acc;
+    }
+  }
 }
 
 // ---- launchers --------------------------------------------------------------------
@@ -467,12 +615,32 @@ static void launch_h_one(hipStream_t st, Dims d, int dir, int ntr, long ts, cons
                          const double* vel, const double* Dv, const uint8_t* flags, double dt,
                          double lf, double lc, double ard1, double AH) {
   if constexpr (ADJ) {
-    hipLaunchKernelGGL((k_sub_h_f32_adj<SCH, DIFF>), grid3(d, d.kmax + 1), dim3(64, 4), 0, st, d,
-                       dir, ntr, ts, a, b, out, Dio, Dn, vel, Dv, flags, dt, lf, lc, ard1, AH);
+    if (dir == 0) {
+      hipLaunchKernelGGL((k_sub_u_f32_adj<SCH, DIFF>),
+                         dim3((d.NX + kUOut - 1) / kUOut, (d.NY + 3) / 4, d.kmax + 1),
+                         dim3(64, 4), 0, st, d, ntr, ts, a, b, out, Dio, Dn, vel, Dv, flags, dt,
+                         lf, lc, ard1, AH);
+    } else {
+      const int ng = (ntr + kVGroup - 1) / kVGroup;
+      hipLaunchKernelGGL((k_sub_v_f32_adj<SCH, DIFF>),
+                         dim3((d.NX + 63) / 64, (d.NY + kVRows - 1) / kVRows, (d.kmax + 1) * ng),
+                         dim3(64), 0, st, d, ntr, ts, a, b, out, Dio, Dn, vel, Dv, flags, dt, lc,
+                         lf, ard1, AH, ng);
+    }
   } else {
     (void)b;
-    hipLaunchKernelGGL((k_sub_h_f32<SCH, DIFF>), grid3(d, d.kmax + 1), dim3(64, 4), 0, st, d,
-                       dir, ntr, ts, a, out, Dio, Dn, vel, Dv, flags, dt, lf, lc, ard1, AH);
+    if (dir == 0) {
+      hipLaunchKernelGGL((k_sub_u_f32<SCH, DIFF>),
+                         dim3((d.NX + kUFwdOut - 1) / kUFwdOut, (d.NY + 3) / 4, d.kmax + 1),
+                         dim3(64, 4), 0, st, d, ntr, ts, a, out, Dio, Dn, vel, Dv, flags, dt, lf,
+                         lc, ard1, AH);
+    } else {
+      const int ng = (ntr + kVGroup - 1) / kVGroup;
+      hipLaunchKernelGGL((k_sub_v_f32<SCH, DIFF>),
+                         dim3((d.NX + 63) / 64, (d.NY + kVRows - 1) / kVRows, (d.kmax + 1) * ng),
+                         dim3(64), 0, st, d, ntr, ts, a, out, Dio, Dn, vel, Dv, flags, dt, lc, lf,
+                         ard1, AH, ng);
+    }
   }
 }
 
@@ -521,7 +689,8 @@ static void launch_w_one(hipStream_t st, Dims d, int ntr, long ts, const float* 
                          const float* b, float* out, const double* D2, const double* D3,
                          const double* ww, const uint8_t* flags, double dt, int kshift) {
   if constexpr (ADJ) {
-    hipLaunchKernelGGL((k_sub_w_f32_adj<SCH>), grid3(d, ntr), dim3(64, 4), 0, st, d, ntr, ts, a, b,
+    hipLaunchKernelGGL((k_sub_w_f32_adj<SCH>), grid3(d, (ntr + kVGroup - 1) / kVGroup),
+                       dim3(64, 4), 0, st, d, ntr, ts, a, b,
                        out, D2, D3, ww, flags, dt, kshift);
   } else {
     (void)b;

@@ -117,18 +117,26 @@ def dot(a, b):
     return float(torch.sum(a.double() * b.double()))
 
 
+def basin(device, seed=6, imax=34, jmax=30, kmax=6):
+    """the rotating basin (volume-consistent transports, sigma layers)"""
+    tr = inputs.blob_case(seed=seed, imax=imax, jmax=jmax, kmax=kmax)
+    ctx = gt.Context.from_domain(tr.dom)
+    vel = [to_dev(a, device) for a in (tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho)]
+    return tr, ctx, vel
+
+
 @pytest.mark.parametrize("scheme", [gt.UPSTREAM, gt.P2])
 def test_dot_product_linear_schemes(device, scheme):
-    dom, F, ctx, vel = problem(device, seed=9, size=(26, 21, 6))
-    x = tracer_set(dom, 2, 41, device) - 0.5
-    y = tracer_set(dom, 2, 43, device) - 0.5
+    tr, ctx, vel = basin(device, seed=9, imax=30, jmax=26)
+    x = tracer_set(tr.dom, 2, 41, device) - 0.5
+    y = tracer_set(tr.dom, 2, 43, device) - 0.5
     for AH in (-1.0, 300.0):
-        lam0, Mx = adjoint32(ctx, 2, 5, 2, F["dt"], x, y, vel, scheme, scheme, AH)
+        lam0, Mx = adjoint32(ctx, 2, 5, 2, tr.dt, x, y, vel, scheme, scheme, AH)
         lhs, rhs = dot(Mx, y), dot(x, lam0)
         assert abs(lhs - rhs) <= 1e-5 * abs(rhs), (scheme, AH, lhs, rhs)
         # the FP64 reverse mode of the same map, tracer by tracer
         for t in range(2):
-            l64, _ = adjoint64(ctx, 5, 2, F["dt"], x[t].double().contiguous(),
+            l64, _ = adjoint64(ctx, 5, 2, tr.dt, x[t].double().contiguous(),
                                y[t].double().contiguous(), vel, scheme, scheme, AH)
             e = float((lam0[t].double() - l64).abs().max() / l64.abs().max())
             assert e <= 1e-5, (scheme, AH, t, e)
@@ -162,10 +170,8 @@ def test_euler_identity_every_scheme(device, scheme):
     """the rotating basin (volume-consistent transports): a Gaussian blob and
     a random tracer with plateaus (limiter branches with deltaf*deltafu <= 0),
     8 steps, without and with horizontal diffusion"""
-    tr = inputs.blob_case(seed=6, imax=34, jmax=30, kmax=6)
+    tr, ctx, vel = basin(device)
     d = tr.dom
-    ctx = gt.Context.from_domain(d)
-    vel = [to_dev(a, device) for a in (tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho)]
     rnd = inputs.uniform(d.shape3, 61)
     rnd[inputs.uniform(d.shape3, 62) < 0.1] = 0.5
     f0 = torch.from_numpy(np.stack([tr.tracers["c"], rnd]).astype(np.float32)).to(device)
