@@ -19,9 +19,12 @@
 //   f_new(i) = f(i) + f(i)*am1(i) - (F(i) - F(i-1))*b(i)   (cell_update_fast)
 //   F(X)     = vel(X)*rec(fuu, fu, fd) - kA(X)*(f(X+1) - f(X))
 // with am1 = (Dio-Dn)/Dn and b = dt*splitfac*dlen*ard1/Dn.  Its transpose is
-// gathered per cell (a thread owns a cell, or a column for w: no atomics).
-// The limiter's derivative is that of the branch term the FP32 forward
-// selected (the min or max of the terms times deltaf), one-sided at ties.
+// formed face by face, every face once: along i a lane owns a face and the
+// cells collect their faces' terms by cross-lane moves; along j and k a
+// thread marches rows or levels with a four-deep window in registers and
+// stores each output when no further face reaches it (no atomics).  The
+// limiter's derivative is that of the branch term the FP32 forward selected
+// (the min or max of the terms times deltaf), one-sided at ties.
 #include "gt_adjoint_f32.h"
 #include "gt_strip_impl.h"
 
