@@ -2,8 +2,13 @@
 
 GETM's 2-D mode (outside the transport path) is stood in for by a prescribed
 M2 seiche of the depth-integrated transport,
-U = umax * HU * sin(pi s) * cos(omega t), V = 0, with s the position across
-the water span (zero on the walls), advanced over the M micro steps of each
+U = umax * Href * sin(pi s) * cos(omega t), V = 0, with s the position across
+the water span (zero on the walls) and Href the smallest water depth at the
+U points: the velocity U/HU peaks at umax where the water is shallowest, and
+the divergence, hence the elevation change and the vertical velocity, comes
+from the seiche's shape alone (a transport proportional to the local depth
+would pump the bathymetry's roughness into the vertical), advanced over the
+M micro steps of each
 macro step with the elevation update of sealevel.F90:92-94 and the transport
 sum of m2d.F90:471.  Everything else runs through the library on the
 device, in integrate_3d's order (m3d.F90:446-548): start_macro, coordinates
@@ -38,7 +43,8 @@ class TidalShare:
         ii = np.arange(dom.NX, dtype=np.float64)
         s = (ii + 0.5 - (i0 - 0.5)) / float(i1 - i0 + 1)  # face i+1/2 across the water span
         shape = np.clip(np.sin(math.pi * s), 0.0, None)[None, :] * (dom.au >= 1)
-        self.Ushape = D(umax * dom.HU * shape)
+        self.Href = float(dom.HU[dom.au >= 1].min())
+        self.Ushape = D(umax * self.Href * shape)
         self.wet = D((dom.az == 1).astype(np.float64))
         sh2 = dom.az.shape
         z2 = lambda: torch.zeros(sh2, dtype=torch.float64, device=device)  # noqa: E731
