@@ -210,9 +210,6 @@ __global__ __launch_bounds__(64) void k_sub_v_f32(
 // flux at K+1/2 exists below kmax, the one at K-1/2 above level 1 (or at
 // level 1 with kshift); level K is updated up to kmax-kshift; the limiter
 // applies above level 1-kshift going up and below kmax-1 going down
-// tracers per thread of the forward kernel (the face constants are shared)
-constexpr int kWGroup = 18;
-
 template <int SCH>
 __global__ __launch_bounds__(256) void k_sub_w_f32(Dims d, int ntr, long ts,
                                                    const float* __restrict__ fin,
@@ -222,22 +219,38 @@ __global__ __launch_bounds__(256) void k_sub_w_f32(Dims d, int ntr, long ts,
                                                    const double* __restrict__ ww,
                                                    const uint8_t* __restrict__ flags, double dt,
                                                    int kshift) {
-  // one thread per column and group of kWGroup tracers (blockIdx.z),
-  // marching up; the face constants of a level serve the whole group
+  // one thread per column and group of kVGroup tracers (blockIdx.z), marching
+  // up with the levels K-2..K+2 of each tracer, K-1..K+1 of D2 and K-1, K of
+  // ww in registers (kmax >= 2); the face constants serve the whole group
+  constexpr int G = kVGroup;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int t0 = blockIdx.z * kWGroup;
+  const int t0 = blockIdx.z * G;
   if (i >= d.NX || j >= d.NY || t0 >= ntr) return;
-  const int t1 = min(t0 + kWGroup, ntr);
   const int kmax = d.kmax;
   const long NXY = d.NXY, c2 = i + (long)d.NX * j;
+  long tb[G];
+#pragma unroll
+  for (int g = 0; g < G; g++) tb[g] = (long)min(t0 + g, ntr - 1) * ts;
   const bool wcol = (flags[c2] & MB_WCOL) && i <= d.NX - 2 && j <= d.NY - 2;
   if (!wcol) {
-    for (int t = t0; t < t1; t++)
-      for (int K = 0; K <= kmax; K++) fout[t * ts + c2 + NXY * K] = fin[t * ts + c2 + NXY * K];
+    for (int K = 0; K <= kmax; K++)
+#pragma unroll
+      for (int g = 0; g < G; g++)
+        if (t0 + g < ntr) fout[tb[g] + c2 + NXY * K] = fin[tb[g] + c2 + NXY * K];
     return;
   }
-  for (int t = t0; t < t1; t++) fout[t * ts + c2] = fin[t * ts + c2];  // level 0
+  float n[G][5];  // levels K-2..K+2 (entries past kmax unused)
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    const float* f = fin + tb[g];
+    n[g][0] = 0.0f;
+    n[g][1] = f[c2];
+    n[g][2] = f[c2 + NXY];
+    n[g][3] = f[c2 + 2 * NXY];
+    n[g][4] = kmax >= 3 ? f[c2 + 3 * NXY] : 0.0f;
+    if (t0 + g < ntr) fout[tb[g] + c2] = n[g][1];  // level 0
+  }
   double dm = D2[c2], dk = D2[c2 + NXY], dp = D2[c2 + 2 * NXY];  // D2 at K-1, K, K+1
   double wm = ww[c2], wkk = ww[c2 + NXY];                         // ww at K-1, K
   for (int K = 1; K <= kmax; K++) {
@@ -251,18 +264,24 @@ __global__ __launch_bounds__(256) void k_sub_w_f32(Dims d, int ntr, long ts,
     wface_setup<SCH>(wfk, wkk, dt, dk, K + 1 <= kmax ? dp : dk);
     wface_setup<SCH>(wfp, wm, dt, dm, dk);
     const CellK k3 = cell_factors(dk, D3[cK], dt);
-    // the operands of the strip kernels: level -1 is zero, levels past kmax
-    // are replaced by level K
-    const long l0 = cK - 2 * NXY, l1 = cK - NXY;
-    const long l3 = K + 1 <= kmax ? cK + NXY : cK, l4 = K + 2 <= kmax ? cK + 2 * NXY : cK;
-    for (int t = t0; t < t1; t++) {
-      const float* f = fin + t * ts;
-      const float fKm2 = K > 1 ? f[l0] : 0.0f, fKm1 = f[l1], fK = f[cK];
-      const float fKp1 = f[l3], fKp2 = f[l4];
+    const bool nxt = K + 3 <= kmax;
+    const long cn = c2 + NXY * (nxt ? K + 3 : kmax);
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      // the operands of the strip kernels: level -1 is zero, levels past
+      // kmax are replaced by level K
+      const float fKm2 = K > 1 ? n[g][0] : 0.0f, fKm1 = n[g][1], fK = n[g][2];
+      const float fKp1 = K + 1 <= kmax ? n[g][3] : fK, fKp2 = K + 2 <= kmax ? n[g][4] : fK;
       const float p = wflux_fast<SCH>(wfp, limp_up, limp_dn, fKm2, fKm1, fK, fKp1);
       const float q = wflux_fast<SCH>(wfk, limk_up, limk_dn, fKm1, fK, fKp1, fKp2);
       const float wprev = wp_on ? p : 0.0f, wk = wk_on ? q : 0.0f;
-      fout[t * ts + cK] = kdo ? cell_update_fast(k3, fK, wk, wprev) : fK;
+      const float r = kdo ? cell_update_fast(k3, fK, wk, wprev) : fK;
+      if (t0 + g < ntr) fout[tb[g] + cK] = r;
+      n[g][0] = n[g][1];
+      n[g][1] = n[g][2];
+      n[g][2] = n[g][3];
+      n[g][3] = n[g][4];
+      n[g][4] = nxt ? fin[tb[g] + cn] : 0.0f;
     }
     dm = dk; dk = dp;
     dp = K + 2 <= kmax ? D2[c2 + NXY * (K + 2)] : 0.0;
@@ -697,7 +716,7 @@ static void launch_w_one(hipStream_t st, Dims d, int ntr, long ts, const float* 
                        out, D2, D3, ww, flags, dt, kshift);
   } else {
     (void)b;
-    hipLaunchKernelGGL((k_sub_w_f32<SCH>), grid3(d, (ntr + kWGroup - 1) / kWGroup), dim3(64, 4),
+    hipLaunchKernelGGL((k_sub_w_f32<SCH>), grid3(d, (ntr + kVGroup - 1) / kVGroup), dim3(64, 4),
                        0, st, d, ntr, ts, a, out,
                        D2, D3, ww, flags, dt, kshift);
   }
