@@ -16,7 +16,7 @@
 
 #include "../../include/getm_transport.h"
 #include "gt_adjoint.h"
-#include "gt_adjoint_f32.h"
+#include "gt_substeps.h"
 #include "gt_batched.h"
 #include "gt_kernels.h"
 
@@ -59,7 +59,6 @@ struct gt_ctx {
   double *fi = nullptr, *hi = nullptr, *adv = nullptr, *tmp = nullptr;
   double *scr7 = nullptr, *zz = nullptr, *P = nullptr, *slice3 = nullptr;
   double *bpart = nullptr, *bout = nullptr;
-  std::vector<double*> adj;  // reverse-mode work fields (allocated on first use)
   unsigned long long* nwarn = nullptr;
   int adv_ver_iterations = 1;
   int do_mirror_2d = 1;
@@ -78,7 +77,7 @@ struct gt_ctx {
   double* tri = nullptr;   // batched tridiagonal work (grown on demand)
   long tri_n = 0;
   double* mwork = nullptr; // uv_advect_3d work: fadv uuadv vvadv wwadv huadv hvadv
-  void* w32 = nullptr;     // FP32 per-sub-step and reverse-mode work (grown on demand)
+  void* w32 = nullptr;     // per-sub-step and reverse-mode work (grown on demand)
   size_t w32_bytes = 0;
 };
 
@@ -232,8 +231,6 @@ extern "C" int gt_destroy(gt_ctx* c) {
                 c->P, c->slice3, c->bpart, c->bout, c->nwarn, c->ws, c->dlev, c->gga,
                 c->dbad, c->vdw, c->tri, c->mwork, c->w32};
   for (void* p : ps)
-    if (p) (void)hipFree(p);
-  for (double* p : c->adj)
     if (p) (void)hipFree(p);
   delete c;
   return GT_OK;
@@ -1116,31 +1113,28 @@ static int ensure_w32(gt_ctx* c, size_t bytes) {
   return GT_OK;
 }
 
-static int copy_f32(hipStream_t st, float* dst, const float* src, size_t n) {
-  if (dst == src || n == 0) return GT_OK;
-  HIPCHK(hipMemcpyAsync(dst, src, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
-  return GT_OK;
-}
 
-// the five FP32 sub-steps of one HALFSPLIT step (advection_3d.F90:439-547)
-// for ntr tracers: in -> s[0] (u/2) -> s[1] (v/2) -> s[2] (w) -> s[3] (v/2)
-// -> out (u/2).  D[0..5]: the layer heights before/after each sub-step.
-// Consecutive buffers must differ (s may repeat two buffers).
-static int substeps_f32(gt_ctx* c, hipStream_t st, const GridMasks* g, const double* const D[6],
-                        double dt, int ntr, const float* in, float* const s[4], float* out,
-                        const double* uu, const double* vv, const double* ww, const double* hu,
-                        const double* hv, int hs, int vs, double AH, int kshift) {
+// the five sub-steps of one HALFSPLIT step (advection_3d.F90:439-547) for
+// ntr tracers, one kernel each: in -> s[0] (u/2) -> s[1] (v/2) -> s[2] (w)
+// -> s[3] (v/2) -> out (u/2, skipped if out is NULL).  D[0..5]: the layer
+// heights before/after each sub-step.  Consecutive buffers must differ (s
+// may repeat two buffers).
+template <typename TF>
+static int substeps(gt_ctx* c, hipStream_t st, const GridMasks* g, const double* const D[6],
+                    double dt, int ntr, const TF* in, TF* const s[4], TF* out, const double* uu,
+                    const double* vv, const double* ww, const double* hu, const double* hv, int hs,
+                    int vs, double AH, int kshift) {
   const long ts = c->N3;
   const Dims d = c->d;
-  LCHK(gt::launch_sub_h_f32(st, d, 0, ntr, ts, in, s[0], D[0], D[1], uu, hu, g->flags, dt, c->dx,
+  LCHK(gt::launch_sub_h<TF>(st, d, 0, ntr, ts, in, s[0], D[0], D[1], uu, hu, g->flags, dt, c->dx,
                             c->dy, c->ard1, hs, AH));
-  LCHK(gt::launch_sub_h_f32(st, d, 1, ntr, ts, s[0], s[1], D[1], D[2], vv, hv, g->flags, dt,
+  LCHK(gt::launch_sub_h<TF>(st, d, 1, ntr, ts, s[0], s[1], D[1], D[2], vv, hv, g->flags, dt,
                             c->dx, c->dy, c->ard1, hs, AH));
-  LCHK(gt::launch_sub_w_f32(st, d, ntr, ts, s[1], s[2], D[2], D[3], ww, g->flags, dt, vs, kshift));
-  LCHK(gt::launch_sub_h_f32(st, d, 1, ntr, ts, s[2], s[3], D[3], D[4], vv, hv, g->flags, dt,
+  LCHK(gt::launch_sub_w<TF>(st, d, ntr, ts, s[1], s[2], D[2], D[3], ww, g->flags, dt, vs, kshift));
+  LCHK(gt::launch_sub_h<TF>(st, d, 1, ntr, ts, s[2], s[3], D[3], D[4], vv, hv, g->flags, dt,
                             c->dx, c->dy, c->ard1, hs, AH));
   if (out)
-    LCHK(gt::launch_sub_h_f32(st, d, 0, ntr, ts, s[3], out, D[4], D[5], uu, hu, g->flags, dt,
+    LCHK(gt::launch_sub_h<TF>(st, d, 0, ntr, ts, s[3], out, D[4], D[5], uu, hu, g->flags, dt,
                               c->dx, c->dy, c->ard1, hs, AH));
   return GT_OK;
 }
@@ -1161,8 +1155,8 @@ static int batch_substeps_f32(const gt_batch_domain& bd, const GridMasks* g, int
                           c->ard1, hs, kshift));
   const double* D[6] = {bd.ho, Dh[0], Dh[1], Dh[2], Dh[3], Dh[4]};
   float* const s[4] = {A, B, A, B};
-  TRY(substeps_f32(c, st, g, D, dt, ntr, (const float*)bd.f_in, s, (float*)bd.f_out, bd.uu, bd.vv,
-                   bd.ww, bd.hu, bd.hv, hs, vs, AH, kshift));
+  TRY(substeps<float>(c, st, g, D, dt, ntr, (const float*)bd.f_in, s, (float*)bd.f_out, bd.uu,
+                       bd.vv, bd.ww, bd.hu, bd.hv, hs, vs, AH, kshift));
   if (bd.hires)
     HIPCHK(hipMemcpyAsync(bd.hires, Dh[4], sizeof(double) * N3, hipMemcpyDeviceToDevice, st));
   return GT_OK;
@@ -1311,86 +1305,114 @@ extern "C" int gt_autotune_batch(int ndom, const gt_batch_domain* doms, int ntra
 }
 
 // ---- reverse mode (extension) ------------------------------------------------
-// work fields: D0..D5 (layer heights through the five sub-steps), S0..S4
-// (tracer states before each sub-step), two adjoint buffers, then the
-// checkpoints and the states of one checkpoint segment
-static int adj_fields(gt_ctx* c, int n) {
-  while ((int)c->adj.size() < n) {
-    double* p = nullptr;
-    HIPCHK(hipMalloc(&p, sizeof(double) * (size_t)c->N3));
-    c->adj.push_back(p);
-  }
+// transpose of one HALFSPLIT step from s0: the states before the last four
+// sub-steps are recomputed into S[0..3]; lam[0] (adjoint of the step's
+// output) is replaced by the adjoint of s0, lam[1] is work
+template <typename TF>
+static int reverse_step(gt_ctx* c, const GridMasks* g, const double* const D[6], double dt,
+                        int ntr, const TF* s0, TF* const S[4], TF** lam, const double* uu,
+                        const double* vv, const double* ww, const double* hu, const double* hv,
+                        int hs, int vs, double AH, int kshift) {
+  hipStream_t st = c->st;
+  const Dims d = c->d;
+  const long ts = c->N3;
+  TRY(substeps<TF>(c, st, g, D, dt, ntr, s0, S, nullptr, uu, vv, ww, hu, hv, hs, vs, AH, kshift));
+  TF* a = lam[0];
+  TF* b = lam[1];
+  LCHK(gt::launch_sub_h_adj<TF>(st, d, 0, ntr, ts, S[3], a, b, D[4], D[5], uu, hu, g->flags, dt,
+                                c->dx, c->dy, c->ard1, hs, AH));
+  LCHK(gt::launch_sub_h_adj<TF>(st, d, 1, ntr, ts, S[2], b, a, D[3], D[4], vv, hv, g->flags, dt,
+                                c->dx, c->dy, c->ard1, hs, AH));
+  LCHK(gt::launch_sub_w_adj<TF>(st, d, ntr, ts, S[1], a, b, D[2], D[3], ww, g->flags, dt, vs,
+                                kshift));
+  LCHK(gt::launch_sub_h_adj<TF>(st, d, 1, ntr, ts, S[0], b, a, D[1], D[2], vv, hv, g->flags, dt,
+                                c->dx, c->dy, c->ard1, hs, AH));
+  LCHK(gt::launch_sub_h_adj<TF>(st, d, 0, ntr, ts, s0, a, b, D[0], D[1], uu, hu, g->flags, dt,
+                                c->dx, c->dy, c->ard1, hs, AH));
+  lam[0] = b;
+  lam[1] = a;
   return GT_OK;
 }
 
-// one HALFSPLIT step from s0, keeping every intermediate state and layer
-// height (the forward of advection_3d.F90:439-547 with the sub-step kernels)
-static int forward_capture(gt_ctx* c, const GridMasks* g, double dt, const double* s0,
-                           double* out, const double* uu, const double* vv, const double* ww,
-                           const double* hu, const double* hv, const double* ho, int hs, int vs,
-                           double AH, int kshift, double** D, double** S) {
-  const long N3 = c->N3, NXY = c->d.NXY;
-  const int kmax = c->kmax;
-  ENSURE(c->adv, N3);
-  double* adv = c->adv;
-  TRY(copy_d2d(c, D[0], ho, N3));
-  TRY(copy_d2d(c, D[1], D[0], N3));
-  TRY(copy_d2d(c, S[1], s0, NXY));  // level 0 is never advanced horizontally
-  TRY(step_u(c, g, 1, kmax, s0, s0, S[1], D[1], adv, uu, hu, dt, 0.5, hs, AH));
-  TRY(copy_d2d(c, D[2], D[1], N3));
-  TRY(copy_d2d(c, S[2], s0, NXY));
-  TRY(step_v(c, g, 1, kmax, S[1], S[1], S[2], D[2], adv, vv, hv, dt, 0.5, hs, AH));
-  TRY(copy_d2d(c, D[3], D[2], N3));
-  LCHK(gt::launch_split_w(c->st, c->d, S[2], S[2], S[3], D[3], adv, ww, g->az, dt, 1.0, vs,
-                          kshift));
-  TRY(copy_d2d(c, D[4], D[3], N3));
-  TRY(copy_d2d(c, S[4], S[3], NXY));
-  TRY(step_v(c, g, 1, kmax, S[3], S[3], S[4], D[4], adv, vv, hv, dt, 0.5, hs, AH));
-  TRY(copy_d2d(c, D[5], D[4], N3));
-  if (out) {
-    TRY(copy_d2d(c, out, S[4], NXY));
-    TRY(step_u(c, g, 1, kmax, S[4], S[4], out, D[5], adv, uu, hu, dt, 0.5, hs, AH));
-  } else {
-    // only D5 is wanted
-    ENSURE(c->tmp, N3);
-    TRY(step_u(c, g, 1, kmax, S[4], S[4], c->tmp, D[5], adv, uu, hu, dt, 0.5, hs, AH));
-  }
+template <typename TF>
+static int copy_set(hipStream_t st, TF* dst, const TF* src, size_t n) {
+  if (dst == src || n == 0) return GT_OK;
+  HIPCHK(hipMemcpyAsync(dst, src, sizeof(TF) * n, hipMemcpyDeviceToDevice, st));
   return GT_OK;
 }
 
-// transpose of one HALFSPLIT step: lam (adjoint of the step's output) is
-// replaced by the adjoint of its input state s0
-static int reverse_step(gt_ctx* c, const GridMasks* g, double dt, const double* s0,
-                        double** lam, const double* uu, const double* vv, const double* ww,
-                        const double* hu, const double* hv, const double* ho, int hs, int vs,
-                        double AH, int kshift, double** D, double** S) {
-  const long NXY = c->d.NXY;
-  const int kmax = c->kmax;
-  TRY(forward_capture(c, g, dt, s0, nullptr, uu, vv, ww, hu, hv, ho, hs, vs, AH, kshift, D, S));
-  double* a = lam[0];
-  double* b = lam[1];
-  auto uv = [&](int dir, const double* f, const double* Dio, const double* Dn) -> int {
-    const double* vel = dir == 0 ? uu : vv;
-    const double* Dv = dir == 0 ? hu : hv;
-    const uint8_t* mf = dir == 0 ? g->uflux : g->vflux;
-    const uint8_t* mu = dir == 0 ? g->uupd : g->vupd;
-    double lflux = dir == 0 ? c->dx : c->dy, lface = dir == 0 ? c->dy : c->dx;
-    TRY(copy_d2d(c, b, a, NXY));
-    LCHK(gt::launch_split_uv_adj(c->st, c->d, 1, kmax, dir, f, a, b, Dio, Dn, vel, Dv, mf, mu,
-                                 dt, 0.5, hs, AH, lflux, lface, c->ard1));
-    double* t = a; a = b; b = t;
-    return GT_OK;
+// Forward sweep from f0 with a checkpoint every ckpt_every steps (the strip
+// kernels where they apply, else the per-sub-step kernels: the same states
+// bit for bit), then the reverse sweep segment by segment: each segment's
+// states are recomputed from its checkpoint, then every step is transposed
+// from the last.  The layer heights of the five sub-steps are the same each
+// step (velocities and heights held fixed) and are formed once.
+template <typename TF>
+static int adjoint_impl(const char* name, gt_ctx* c, int ntr, int nsteps, int ckpt_every,
+                        double dt, const TF* f0, TF* f_final, const TF* lam_final, TF* lam0,
+                        const double* uu, const double* vv, const double* ww, const double* hu,
+                        const double* hv, const double* ho, int hscheme, int vscheme, double AH,
+                        int tag) {
+  if (!c || ntr < 1 || !f0 || !lam_final || !lam0 || nsteps < 1 || ckpt_every < 1)
+    return set_err(GT_EARG, "%s: bad argument", name);
+  if (!uu || !vv || !ww || !hu || !hv || !ho)
+    return set_err(GT_EARG, "%s: NULL velocity or layer height", name);
+  const GridMasks* g = grid_for(c, tag);
+  if (!g) return set_err(GT_EINVAL_TAG, "%s: tag is invalid", name);
+  if (!split_scheme(hscheme) || !split_scheme(vscheme))
+    return set_err(GT_EINVAL_SCHEME, "%s: schemes must be UPSTREAM..P2_PDM", name);
+  if (c->kmax < 2) return set_err(GT_EARG, "%s: kmax must be >= 2", name);
+  const int kshift = tag == GT_W_TAG ? 1 : 0;
+  const long N3 = c->N3;
+  const size_t set = (size_t)ntr * N3;
+  const int nck = (nsteps + ckpt_every - 1) / ckpt_every;
+  const int seg = ckpt_every;
+  // D1..D5, then tracer sets: S[4] lam[2] ck[nck] seg-1 states of a segment
+  const size_t nsets = 4 + 2 + (size_t)nck + (size_t)(seg - 1);
+  TRY(ensure_w32(c, sizeof(double) * 5 * (size_t)N3 + sizeof(TF) * nsets * set));
+  double* Dh[5];
+  for (int n = 0; n < 5; n++) Dh[n] = (double*)c->w32 + n * N3;
+  TF* base = (TF*)((double*)c->w32 + 5 * N3);
+  TF* const S[4] = {base, base + set, base + 2 * set, base + 3 * set};
+  TF* lam[2] = {base + 4 * set, base + 5 * set};
+  TF* ck = base + 6 * set;
+  TF* sg = ck + (size_t)nck * set;
+  LCHK(gt::launch_heights(c->st, c->d, g->flags, ho, uu, vv, ww, Dh, dt, c->dx, c->dy, c->ard1,
+                          hscheme, kshift));
+  const double* D[6] = {ho, Dh[0], Dh[1], Dh[2], Dh[3], Dh[4]};
+  const bool plain = c->fused && gt::fused_supported(hscheme, vscheme) && strip_fits(c);
+  auto step = [&](const TF* in, TF* out) -> int {
+    if (plain) {
+      gt_batch_domain bd{c, in, out, uu, vv, ww, hu, hv, ho, nullptr};
+      return batch_impl<TF>(1, &bd, ntr, dt, hscheme, vscheme, AH, tag, c->st);
+    }
+    TF* const s[4] = {S[0], S[1], S[0], S[1]};
+    return substeps<TF>(c, c->st, g, D, dt, ntr, in, s, out, uu, vv, ww, hu, hv, hscheme, vscheme,
+                        AH, kshift);
   };
-  TRY(uv(0, S[4], D[4], D[5]));
-  TRY(uv(1, S[3], D[3], D[4]));
-  LCHK(gt::launch_split_w_adj(c->st, c->d, S[2], a, b, D[2], D[3], ww, g->az, dt, 1.0, vs,
-                              kshift));
-  { double* t = a; a = b; b = t; }
-  TRY(uv(1, S[1], D[1], D[2]));
-  TRY(uv(0, S[0], D[0], D[1]));
-  lam[0] = a;
-  lam[1] = b;
-  return GT_OK;
+  // forward sweep (lam[0], lam[1] as the two states)
+  TF* a = lam[0];
+  TF* b = lam[1];
+  TRY(copy_set<TF>(c->st, a, f0, set));
+  for (int t = 0; t < nsteps; t++) {
+    if (t % ckpt_every == 0) TRY(copy_set<TF>(c->st, ck + (size_t)(t / ckpt_every) * set, a, set));
+    TRY(step(a, b));
+    TF* x = a; a = b; b = x;
+  }
+  if (f_final) TRY(copy_set<TF>(c->st, f_final, a, set));
+  // reverse sweep, segment by segment from the last checkpoint
+  TRY(copy_set<TF>(c->st, lam[0], lam_final, set));
+  for (int s = nck - 1; s >= 0; s--) {
+    const int t0 = s * ckpt_every, t1 = t0 + ckpt_every < nsteps ? t0 + ckpt_every : nsteps;
+    auto state = [&](int t) -> TF* {
+      return t == t0 ? ck + (size_t)s * set : sg + (size_t)(t - t0 - 1) * set;
+    };
+    for (int t = t0; t < t1 - 1; t++) TRY(step(state(t), state(t + 1)));
+    for (int t = t1 - 1; t >= t0; t--)
+      TRY(reverse_step<TF>(c, g, D, dt, ntr, state(t), S, lam, uu, vv, ww, hu, hv, hscheme,
+                           vscheme, AH, kshift));
+  }
+  return copy_set<TF>(c->st, lam0, lam[0], set);
 }
 
 extern "C" int gt_adv3d_adjoint(gt_ctx* c, int nsteps, int ckpt_every, double dt,
@@ -1399,88 +1421,8 @@ extern "C" int gt_adv3d_adjoint(gt_ctx* c, int nsteps, int ckpt_every, double dt
                                 const double* ww, const double* hu, const double* hv,
                                 const double* ho, int hscheme, int vscheme, double AH,
                                 int tag) {
-  if (!c || !f0 || !lam_final || !lam0 || nsteps < 1 || ckpt_every < 1)
-    return set_err(GT_EARG, "gt_adv3d_adjoint: bad argument");
-  const GridMasks* g = grid_for(c, tag);
-  if (!g) return set_err(GT_EINVAL_TAG, "gt_adv3d_adjoint: tag is invalid");
-  if (!split_scheme(hscheme) || !split_scheme(vscheme))
-    return set_err(GT_EINVAL_SCHEME, "gt_adv3d_adjoint: schemes must be UPSTREAM..P2_PDM");
-  if (c->kmax < 2) return set_err(GT_EARG, "gt_adv3d_adjoint: kmax must be >= 2");
-  const int kshift = tag == GT_W_TAG ? 1 : 0;
-  const int nck = (nsteps + ckpt_every - 1) / ckpt_every;
-  const int seg = ckpt_every;
-  // D[6] S[5] lam[2] work[1] ck[nck] seg[seg]
-  const int nf = 6 + 5 + 2 + 1 + nck + seg;
-  TRY(adj_fields(c, nf));
-  double** D = &c->adj[0];
-  double** S = &c->adj[6];
-  double* lam[2] = {c->adj[11], c->adj[12]};
-  double* work = c->adj[13];
-  double** ck = &c->adj[14];
-  double** sg = &c->adj[14 + nck];
-  const long N3 = c->N3;
-  // forward sweep, keeping a checkpoint every ckpt_every steps; the plain
-  // step (the strip kernels, bit-identical to the captured sub-steps) where
-  // it applies, since nothing but the checkpoints is kept here
-  TRY(copy_d2d(c, work, f0, N3));
-  const bool plain = c->fused && c->adv_ver_iterations == 1 &&
-                     gt::fused_supported(hscheme, vscheme) && strip_fits(c);
-  for (int t = 0; t < nsteps; t++) {
-    if (t % ckpt_every == 0) TRY(copy_d2d(c, ck[t / ckpt_every], work, N3));
-    if (plain) {
-      TRY(gt_do_advection_3d(c, dt, work, uu, vv, ww, hu, hv, ho, ho, GT_HALFSPLIT, hscheme,
-                             vscheme, AH, tag, nullptr, nullptr));
-    } else {
-      TRY(forward_capture(c, g, dt, work, sg[0], uu, vv, ww, hu, hv, ho, hscheme, vscheme, AH,
-                          kshift, D, S));
-      TRY(copy_d2d(c, work, sg[0], N3));
-    }
-  }
-  if (f_final) TRY(copy_d2d(c, f_final, work, N3));
-  // reverse sweep, segment by segment from the last checkpoint
-  TRY(copy_d2d(c, lam[0], lam_final, N3));
-  for (int s = nck - 1; s >= 0; s--) {
-    int t0 = s * ckpt_every, t1 = t0 + ckpt_every < nsteps ? t0 + ckpt_every : nsteps;
-    TRY(copy_d2d(c, sg[0], ck[s], N3));
-    for (int t = t0; t < t1 - 1; t++)
-      TRY(forward_capture(c, g, dt, sg[t - t0], sg[t - t0 + 1], uu, vv, ww, hu, hv, ho, hscheme,
-                          vscheme, AH, kshift, D, S));
-    for (int t = t1 - 1; t >= t0; t--) {
-      TRY(copy_d2d(c, S[0], sg[t - t0], N3));
-      TRY(reverse_step(c, g, dt, S[0], lam, uu, vv, ww, hu, hv, ho, hscheme, vscheme, AH, kshift,
-                       D, S));
-    }
-  }
-  return copy_d2d(c, lam0, lam[0], N3);
-}
-
-// transpose of one FP32 HALFSPLIT step from s0: the states before the last
-// four sub-steps are recomputed into S[0..3]; lam[0] (adjoint of the step's
-// output) is replaced by the adjoint of s0, lam[1] is work
-static int reverse_step_f32(gt_ctx* c, const GridMasks* g, const double* const D[6], double dt,
-                            int ntr, const float* s0, float* const S[4], float** lam,
-                            const double* uu, const double* vv, const double* ww,
-                            const double* hu, const double* hv, int hs, int vs, double AH,
-                            int kshift) {
-  hipStream_t st = c->st;
-  const Dims d = c->d;
-  const long ts = c->N3;
-  TRY(substeps_f32(c, st, g, D, dt, ntr, s0, S, nullptr, uu, vv, ww, hu, hv, hs, vs, AH, kshift));
-  float* a = lam[0];
-  float* b = lam[1];
-  LCHK(gt::launch_sub_h_f32_adj(st, d, 0, ntr, ts, S[3], a, b, D[4], D[5], uu, hu, g->flags, dt,
-                                c->dx, c->dy, c->ard1, hs, AH));
-  LCHK(gt::launch_sub_h_f32_adj(st, d, 1, ntr, ts, S[2], b, a, D[3], D[4], vv, hv, g->flags, dt,
-                                c->dx, c->dy, c->ard1, hs, AH));
-  LCHK(gt::launch_sub_w_f32_adj(st, d, ntr, ts, S[1], a, b, D[2], D[3], ww, g->flags, dt, vs,
-                                kshift));
-  LCHK(gt::launch_sub_h_f32_adj(st, d, 1, ntr, ts, S[0], b, a, D[1], D[2], vv, hv, g->flags, dt,
-                                c->dx, c->dy, c->ard1, hs, AH));
-  LCHK(gt::launch_sub_h_f32_adj(st, d, 0, ntr, ts, s0, a, b, D[0], D[1], uu, hu, g->flags, dt,
-                                c->dx, c->dy, c->ard1, hs, AH));
-  lam[0] = b;
-  lam[1] = a;
-  return GT_OK;
+  return adjoint_impl<double>("gt_adv3d_adjoint", c, 1, nsteps, ckpt_every, dt, f0, f_final,
+                              lam_final, lam0, uu, vv, ww, hu, hv, ho, hscheme, vscheme, AH, tag);
 }
 
 extern "C" int gt_adv3d_adjoint_f32(gt_ctx* c, int ntracers, int nsteps, int ckpt_every,
@@ -1489,69 +1431,9 @@ extern "C" int gt_adv3d_adjoint_f32(gt_ctx* c, int ntracers, int nsteps, int ckp
                                     const double* vv, const double* ww, const double* hu,
                                     const double* hv, const double* ho, int hscheme,
                                     int vscheme, double AH, int tag) {
-  if (!c || ntracers < 1 || !f0 || !lam_final || !lam0 || nsteps < 1 || ckpt_every < 1)
-    return set_err(GT_EARG, "gt_adv3d_adjoint_f32: bad argument");
-  if (!uu || !vv || !ww || !hu || !hv || !ho)
-    return set_err(GT_EARG, "gt_adv3d_adjoint_f32: NULL velocity or layer height");
-  const GridMasks* g = grid_for(c, tag);
-  if (!g) return set_err(GT_EINVAL_TAG, "gt_adv3d_adjoint_f32: tag is invalid");
-  if (!split_scheme(hscheme) || !split_scheme(vscheme))
-    return set_err(GT_EINVAL_SCHEME, "gt_adv3d_adjoint_f32: schemes must be UPSTREAM..P2_PDM");
-  if (c->kmax < 2) return set_err(GT_EARG, "gt_adv3d_adjoint_f32: kmax must be >= 2");
-  const int kshift = tag == GT_W_TAG ? 1 : 0;
-  const long N3 = c->N3;
-  const size_t set = (size_t)ntracers * N3;
-  const int nck = (nsteps + ckpt_every - 1) / ckpt_every;
-  const int seg = ckpt_every;
-  // D1..D5, then tracer sets: S[4] lam[2] ck[nck] seg-1 states of a segment
-  const size_t nsets = 4 + 2 + (size_t)nck + (size_t)(seg - 1);
-  TRY(ensure_w32(c, sizeof(double) * 5 * (size_t)N3 + sizeof(float) * nsets * set));
-  double* Dh[5];
-  for (int n = 0; n < 5; n++) Dh[n] = (double*)c->w32 + n * N3;
-  float* base = (float*)((double*)c->w32 + 5 * N3);
-  float* const S[4] = {base, base + set, base + 2 * set, base + 3 * set};
-  float* lam[2] = {base + 4 * set, base + 5 * set};
-  float* ck = base + 6 * set;
-  float* sg = ck + (size_t)nck * set;
-  LCHK(gt::launch_heights(c->st, c->d, g->flags, ho, uu, vv, ww, Dh, dt, c->dx, c->dy, c->ard1,
-                          hscheme, kshift));
-  const double* D[6] = {ho, Dh[0], Dh[1], Dh[2], Dh[3], Dh[4]};
-  // the strip kernels where they apply, else the per-sub-step kernels (the
-  // same states bit for bit)
-  const bool plain = c->fused && gt::fused_supported(hscheme, vscheme) && strip_fits(c);
-  auto step = [&](const float* in, float* out) -> int {
-    if (plain) {
-      gt_batch_domain bd{c, in, out, uu, vv, ww, hu, hv, ho, nullptr};
-      return batch_impl<float>(1, &bd, ntracers, dt, hscheme, vscheme, AH, tag, c->st);
-    }
-    float* const s[4] = {S[0], S[1], S[0], S[1]};
-    return substeps_f32(c, c->st, g, D, dt, ntracers, in, s, out, uu, vv, ww, hu, hv, hscheme,
-                        vscheme, AH, kshift);
-  };
-  // forward sweep (lam[0], lam[1] as the two states), a checkpoint every
-  // ckpt_every steps
-  float* a = lam[0];
-  float* b = lam[1];
-  TRY(copy_f32(c->st, a, f0, set));
-  for (int t = 0; t < nsteps; t++) {
-    if (t % ckpt_every == 0) TRY(copy_f32(c->st, ck + (size_t)(t / ckpt_every) * set, a, set));
-    TRY(step(a, b));
-    float* x = a; a = b; b = x;
-  }
-  if (f_final) TRY(copy_f32(c->st, f_final, a, set));
-  // reverse sweep, segment by segment from the last checkpoint
-  TRY(copy_f32(c->st, lam[0], lam_final, set));
-  for (int s = nck - 1; s >= 0; s--) {
-    const int t0 = s * ckpt_every, t1 = t0 + ckpt_every < nsteps ? t0 + ckpt_every : nsteps;
-    auto state = [&](int t) -> float* {
-      return t == t0 ? ck + (size_t)s * set : sg + (size_t)(t - t0 - 1) * set;
-    };
-    for (int t = t0; t < t1 - 1; t++) TRY(step(state(t), state(t + 1)));
-    for (int t = t1 - 1; t >= t0; t--)
-      TRY(reverse_step_f32(c, g, D, dt, ntracers, state(t), S, lam, uu, vv, ww, hu, hv, hscheme,
-                           vscheme, AH, kshift));
-  }
-  return copy_f32(c->st, lam0, lam[0], set);
+  return adjoint_impl<float>("gt_adv3d_adjoint_f32", c, ntracers, nsteps, ckpt_every, dt, f0,
+                             f_final, lam_final, lam0, uu, vv, ww, hu, hv, ho, hscheme, vscheme,
+                             AH, tag);
 }
 
 extern "C" int gt_ip_blumberg_mellor_adjoint(gt_ctx* c, const double* hun, const double* hvn,

@@ -227,3 +227,37 @@ def test_internal_pressure_adjoint_vs_fd(device):
         assert abs(gh[k, j, i] - fd) <= 1e-6 * max(abs(fd), 1e-9), (k, j, i, gh[k, j, i], fd)
         n += 1
     assert n >= 10
+
+
+@pytest.mark.parametrize("tag", [gt.H_TAG, gt.W_TAG])
+def test_substep_forward_and_checkpoints_bit_identical(device, tag):
+    """The reverse mode recomputes each step's states with the per-sub-step
+    kernels: their final state equals the strip kernels' (tuning fused = 0
+    runs the whole forward sweep through them) and do_advection_3d's, bit for
+    bit, and the checkpoint spacing changes nothing"""
+    tr, d, ctx, fields = setup(device, imax=40, jmax=36, kmax=6, seed=4)
+    uu, vv, ww, hu, hv, ho, hn = fields
+    f0 = to_dev(tr.tracers["c"], device)
+    rng = np.random.default_rng(2)
+    lamT = torch.from_numpy(rng.standard_normal(d.shape3)).to(device)
+
+    def run(ck, fused=1, AH=-1.0):
+        ctx.set_tuning(fused=fused)
+        lam0, ff = torch.empty_like(f0), torch.empty_like(f0)
+        ctx.adv3d_adjoint(7, ck, tr.dt, f0, lamT, lam0, uu, vv, ww, hu, hv, ho, gt.P2_PDM,
+                          gt.P2_PDM, AH, tag, f_final=ff)
+        torch.cuda.synchronize()
+        ctx.set_tuning(fused=1)
+        return lam0, ff
+
+    for AH in (-1.0, 150.0):
+        ref, ffr = run(3, AH=AH)
+        for ck, fused in ((1, 1), (7, 1), (3, 0)):
+            lam, ff = run(ck, fused, AH)
+            assert torch.equal(lam, ref) and torch.equal(ff, ffr), (ck, fused, AH)
+        x = f0.clone()
+        for _ in range(7):
+            ctx.do_advection_3d(tr.dt, x, uu, vv, ww, hu, hv, ho, ho, gt.HALFSPLIT, gt.P2_PDM,
+                                gt.P2_PDM, AH, tag)
+        torch.cuda.synchronize()
+        assert torch.equal(x, ffr), AH
