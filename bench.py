@@ -684,8 +684,8 @@ def adjoint_section(torch, np, gt, inputs, dev, nsteps=48, ckpt=8):
             "ms_per_gradient": t * 1e3,
             "gradient_cell_steps_per_s": ncell * nsteps * 2 / t,
             "note": "per tracer: 48 forward steps for the cost, then the adjoint call (48 "
-                    "forward steps to the checkpoints, 48 recomputed with the sub-step "
-                    "states, 48 reverse steps)",
+                    "forward steps to the checkpoints, 40 recomputed from them, 48 reverse "
+                    "steps of four sub-steps and five transposes)",
             "grad_norm_c": float(g_c.norm()), "grad_norm_T": float(g_T.norm())}
 
 
