@@ -352,7 +352,11 @@ int gt_autotune_batch(int ndom, const gt_batch_domain *doms, int ntracers,
  * and layer heights held fixed.  Runs the forward from f0 (result in f_final,
  * optional), keeping a checkpoint every ckpt_every steps, then sweeps back:
  * lam0 = (d f_final / d f0)^T lam_final.  The limiter enters through the
- * derivative of the branch the forward took. */
+ * derivative of the branch the forward took.  The reverse sweep recomputes
+ * each segment from its checkpoint (strip kernels), then the states before
+ * each sub-step (per-sub-step kernels, bit-identical to the strip kernels),
+ * and applies the transpose of each sub-step.  Work memory: 5 + 6 +
+ * ceil(nsteps/ckpt_every) + ckpt_every - 1 FP64 fields. */
 int gt_adv3d_adjoint(gt_ctx *ctx, int nsteps, int ckpt_every, double dt,
                      const double *f0, double *f_final, const double *lam_final,
                      double *lam0, const double *uu, const double *vv,
