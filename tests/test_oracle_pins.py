@@ -323,6 +323,71 @@ def test_halfsplit_is_the_five_substeps_and_conserves():
     assert abs(m1 - m0) < 1e-13 * m0
 
 
+@pytest.mark.parametrize("axis", ["x", "y"])
+@pytest.mark.parametrize("scheme", [U, SB, PDM])
+def test_halfsplit_open_domain_mirror_symmetry(scheme, axis):
+    """Water up to and into the halo: the reference's horizontal flux and
+    update ranges ([isub, NX-2-isub] for faces, [1+isub, NX-2-isub] for
+    cells, adv_split_u.F90:205-286, adv_split_v.F90:63-136) are symmetric
+    under a reflection, so a HALFSPLIT step commutes with reflecting the
+    basin, the tracer and the transports in x (y); a bound that is off by
+    one row updates one edge and not the other.  ww = 0: the vertical
+    sub-step's column range stops at imax+1 on one side only
+    (adv_split_w.F90:85-305), by the reference's design."""
+    imax, jmax, kmax = 22, 17, 4
+    NX, NY = imax + 4, jmax + 4
+    az = np.ones((NY, NX), np.int32)
+    isl = inputs.uniform((NY, NX), 91) < 0.1
+    isl[:3, :] = isl[-3:, :] = False
+    isl[:, :3] = isl[:, -3:] = False
+    az[isl] = 0
+    au, av, ax = inputs.masks_from_az(az)
+    H = inputs.uniform((NY, NX), 92, 10.0, 40.0)
+    HU, HV = H.copy(), H.copy()
+    HU[:, :-1] = 0.5 * (H[:, :-1] + H[:, 1:])
+    HV[:-1, :] = 0.5 * (H[:-1, :] + H[1:, :])
+    dom = inputs.Domain(imax, jmax, kmax, 500.0, 400.0, H, HU, HV, az, au, av, ax)
+    sh = dom.shape3
+    dt = 30.0
+    f = rnd(sh, 93)
+    ho = rnd(sh, 94, 0.5, 2.0)
+    hu = rnd(sh, 95, 0.5, 2.0)
+    hv = rnd(sh, 96, 0.5, 2.0)
+    uu = rnd(sh, 97, -1, 1) * hu * 0.8 * dom.dx / dt
+    vv = rnd(sh, 98, -1, 1) * hv * 0.8 * dom.dy / dt
+    ww = np.zeros(sh)
+    ax_ = -1 if axis == "x" else -2
+    n = NX if axis == "x" else NY
+
+    def m(a):
+        return np.flip(a, axis=ax_).copy()
+
+    def mface(a, sign):  # face i -> face n-2-i
+        b = np.zeros_like(a)
+        idx = [slice(None)] * a.ndim
+        idx[ax_] = slice(0, n - 1)
+        b[tuple(idx)] = sign * np.take(a, np.arange(n - 2, -1, -1), axis=ax_)
+        return b
+
+    def step(o, f_, uu_, vv_, hu_, hv_, ho_):
+        g, hires = f_.copy(), np.zeros(sh)
+        rc, _ = o.do_advection_3d(dt, g, uu_, vv_, ww, hu_, hv_, ho_, ho_, po.HALFSPLIT, scheme,
+                                  scheme, -1.0, po.H_TAG, 1, hires, None)
+        assert rc == 0
+        return g, hires
+
+    g, hires = step(po.Oracle.from_domain(dom), f, uu, vv, hu, hv, ho)
+    if axis == "x":
+        args = (mface(uu, -1.0), m(vv), mface(hu, 1.0), m(hv))
+    else:
+        args = (m(uu), mface(vv, -1.0), m(hu), mface(hv, 1.0))
+    gm, hm = step(po.Oracle(imax, jmax, kmax, dom.dx, dom.dy, m(az)), m(f), *args[:2], *args[2:],
+                  m(ho))
+    assert np.max(np.abs(m(gm) - g)) < 1e-13 and np.max(np.abs(m(hm) - hires)) < 1e-13
+    # the step reached the halo rows and columns
+    assert np.max(np.abs(g - f)[1:, :2, :]) > 0 and np.max(np.abs(g - f)[1:, :, :2]) > 0
+
+
 def test_error_codes():
     dom = closed_basin()
     orc = po.Oracle.from_domain(dom)
@@ -408,6 +473,80 @@ def test_ip_linear_buoyancy_hydrostatic_integral(method):
         sel[:, :2] = sel[:, -2:] = False
         assert np.max(np.abs(ix[k][sel] - wx)) < 1e-12 * abs(wx) + 1e-20, (method, k)
         assert np.max(np.abs(iy[k][sel] - wy)) < 1e-12 * abs(wy) + 1e-20, (method, k)
+
+
+@pytest.mark.parametrize("axis", ["x", "y"])
+@pytest.mark.parametrize("method", ["bm", "sm"])
+def test_ip_mirror_symmetry_with_coastlines(method, axis):
+    """Reflecting the basin, its islands and every field in x (y) reflects
+    the pressure gradients: idpdx (idpdy) changes sign at the mirrored U (V)
+    point NX-2-i (NY-2-j), the other gradient is carried over.  Islands and
+    land sentinels in the buoyancy make this pin the coastline stencils'
+    indices and signs, which no closed form covers (Shchepetkin-McWilliams
+    reads land values there: ~200 of its gradients change with the
+    sentinel); both runs do the same operations on negated or swapped
+    operands, so the bound is 1e-13 of the largest gradient."""
+    imax, jmax, kmax = 26, 18, 6
+    dom0 = inputs.make_domain(imax, jmax, kmax, 7, hmin=10.0, hmax=40.0, walls=2)
+    NY, NX = dom0.NY, dom0.NX
+    az = dom0.az.copy()
+    isl = inputs.uniform((NY, NX), 71) < 0.12
+    isl[:, :4] = isl[:, -4:] = False
+    isl[:4, :] = isl[-4:, :] = False
+    az[isl & (az == 1)] = 0
+    au, av, ax = inputs.masks_from_az(az)
+    H = dom0.H.copy()
+    H[az == 0] = -10.0
+    HU, HV = H.copy(), H.copy()
+    HU[:, :-1] = 0.5 * (H[:, :-1] + H[:, 1:])
+    HV[:-1, :] = 0.5 * (H[:-1, :] + H[1:, :])
+    dom = inputs.Domain(imax, jmax, kmax, dom0.dx, dom0.dy, H, HU, HV, az, au, av, ax)
+    sse = inputs.uniform((NY, NX), 72, -0.2, 0.2) if method == "sm" else np.zeros((NY, NX))
+    hn = inputs.gvc_layers(dom, np.where(az > 0, H, 10.0) + sse)
+    hun = inputs.gvc_layers(dom, np.where(au > 0, HU, 10.0))
+    hvn = inputs.gvc_layers(dom, np.where(av > 0, HV, 10.0))
+    buoy = inputs.uniform(dom.shape3, 73, -0.02, 0.01)
+    buoy[:, az == 0] = 3.7   # the land sentinel's buoyancy
+    buoy[0] = 3.7
+
+    def run(o, H_, hn_, hun_, hvn_, b_, s_):
+        zz_, ix_, iy_ = (np.zeros(dom.shape3) for _ in range(3))
+        if method == "bm":
+            o.ip_blumberg_mellor(H_, hn_, hun_, hvn_, b_, zz_, ix_, iy_)
+        else:
+            o.ip_shchepetkin_mcwilliams(H_, hn_, hun_, hvn_, b_, s_, zz_, ix_, iy_)
+        return ix_, iy_
+
+    ix, iy = run(po.Oracle.from_domain(dom), H, hn, hun, hvn, buoy, sse)
+    ax_ = -1 if axis == "x" else -2
+    n = NX if axis == "x" else NY
+
+    def m(a):  # a cell field
+        return np.flip(a, axis=ax_).copy()
+
+    def mface(a):  # a face field of the mirrored direction: face i -> face n-2-i
+        b = np.zeros_like(a)
+        src = np.take(a, np.arange(n - 2, -1, -1), axis=ax_)
+        idx = [slice(None)] * a.ndim
+        idx[ax_] = slice(0, n - 1)
+        b[tuple(idx)] = src
+        return b
+    hun_m, hvn_m = (mface(hun), m(hvn)) if axis == "x" else (m(hun), mface(hvn))
+    ixm, iym = run(po.Oracle(imax, jmax, kmax, dom.dx, dom.dy, m(az)), m(H), m(hn), hun_m,
+                   hvn_m, m(buoy), m(sse))
+    scale = max(np.abs(ix).max(), np.abs(iy).max())
+    assert scale > 0
+    # faces 2..n-4 of the mirrored direction are computed in both runs (the
+    # loops end at imax+HALO-2)
+    flip, keep = (ix, iy) if axis == "x" else (iy, ix)
+    flipm, keepm = (ixm, iym) if axis == "x" else (iym, ixm)
+    sel = [slice(1, None), slice(None), slice(None)]
+    selm = list(sel)
+    sel[ax_] = slice(2, n - 3)
+    selm[ax_] = slice(n - 4, 1, -1)
+    e_flip = np.abs(flipm[tuple(selm)] + flip[tuple(sel)]).max()
+    e_keep = np.abs(m(keepm)[1:] - keep[1:]).max()
+    assert e_flip <= 1e-13 * scale and e_keep <= 1e-13 * scale, (method, axis, e_flip, e_keep)
 
 
 # ------------------------------------------------------------ halo & budget --
