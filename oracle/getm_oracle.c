@@ -10,6 +10,12 @@
  * splits the outer j loops as the reference's !$OMP DO directives do; no
  * result depends on the thread count (there are no reductions except in
  * or_budget, which is serial and error-free).
+ *
+ * Every function is pinned by tests/test_oracle_*.py (closed forms,
+ * published values, invariants, reflection symmetry, the reference's own
+ * check values).  Parity unpinned beyond this restatement's reading of the
+ * source: a loop bound of adv_split_u/v that would be wrong by the same
+ * amount on both edges (reflection symmetry catches any other bound error).
  */
 #include "getm_oracle.h"
 
