@@ -292,6 +292,53 @@ def test_upstream_2dh_bounds_and_textbook():
     assert np.max(np.abs(adv - div)[wet]) < 1e-12
 
 
+def test_nosplit_upstream_2dh_discards_the_clip():
+    """3-D NOSPLIT with UPSTREAM_2DH rebuilds the tracer from the advective
+    term, f = (ho f - dt adv)/hi (advection_3d.F90:338-350,416-423), so the
+    clip of adv_upstream_2dh does not survive: on water cells the result is
+    the textbook upwind flux form, including cells where the clip would
+    have changed the value (ww = 0, vertical UPSTREAM is then the identity).
+    The upwind flux form is a convex combination of a cell and its upwind
+    neighbours while the outflow stays below the cell's volume, so the
+    transports here reach Courant numbers above 1 for the clip to act."""
+    dom = closed_basin(imax=24, jmax=20, kmax=3, seed=9)
+    orc = po.Oracle.from_domain(dom)
+    sh = dom.shape3
+    dt = 30.0
+    f = rnd(sh, 51)
+    ho = rnd(sh, 52, 1.0, 2.0)
+    hu = rnd(sh, 53, 1.0, 2.0)
+    hv = rnd(sh, 54, 1.0, 2.0)
+    uu = rnd(sh, 55, -1, 1) * 1.5 * dom.dx / dt
+    vv = rnd(sh, 56, -1, 1) * 1.5 * dom.dy / dt
+    uu[:, dom.au < 1] = 0.0
+    vv[:, dom.av < 1] = 0.0
+    ww = np.zeros(sh)
+    g, hires = f.copy(), np.zeros(sh)
+    rc, _ = orc.do_advection_3d(dt, g, uu, vv, ww, hu, hv, ho, ho, po.NOSPLIT, po.UPSTREAM_2DH,
+                                U, -1.0, po.H_TAG, 1, hires, None)
+    assert rc == 0
+    wet = dom.az == 1
+    clipped = 0
+    for k in range(1, dom.kmax + 1):
+        fk, U_, V_ = f[k], uu[k], vv[k]
+        uf = np.where(U_ > 0, U_ * fk, U_ * np.roll(fk, -1, axis=1))
+        vf = np.where(V_ > 0, V_ * fk, V_ * np.roll(fk, -1, axis=0))
+        div = ((uf - np.roll(uf, 1, axis=1)) * dom.dy + (vf - np.roll(vf, 1, axis=0)) * dom.dx) \
+            / (dom.dx * dom.dy)
+        dU = ((U_ - np.roll(U_, 1, axis=1)) * dom.dy + (V_ - np.roll(V_, 1, axis=0)) * dom.dx) \
+            / (dom.dx * dom.dy)
+        hi = ho[k] - dt * dU
+        want = (ho[k] * fk - dt * div) / hi
+        assert np.max((np.abs(g[k] - want) / np.maximum(1.0, np.abs(want)))[wet]) < 1e-12, k
+        # the 2-D routine alone clips: its result differs where the flux
+        # form leaves the neighbours' range
+        fi, Di, adv = fk.copy(), ho[k].copy(), np.zeros_like(fk)
+        orc.adv_upstream_2dh(dt, fk, fi, Di, adv, U_, V_, ho[k], hu[k], hv[k], -1.0)
+        clipped += int((np.abs(fi - want) > 1e-9 * np.maximum(1.0, np.abs(want)))[wet].sum())
+    assert clipped > 0
+
+
 # ---------------------------------------------------------- composition --
 def test_halfsplit_is_the_five_substeps_and_conserves():
     tr = inputs.blob_case(imax=40, jmax=36, kmax=6)
