@@ -73,7 +73,10 @@ constexpr bool kExact = sizeof(TF) == 8;
 // tracers per thread of the row and column marches
 template <typename TF>
 constexpr int kGroup = sizeof(TF) == 8 ? 1 : 6;
-constexpr int kVRows = 16;  // output rows per wavefront of the v marches
+// output rows per wavefront of the v marches (FP64 carries one tracer, so
+// shorter marches for more wavefronts)
+template <typename TF>
+constexpr int kVRows = sizeof(TF) == 8 ? 8 : 16;
 constexpr int kUOutF = 60;  // outputs per wavefront of the u forward
 constexpr int kUOutA = 58;  // ... and of the u transpose
 
@@ -389,12 +392,12 @@ __global__ __launch_bounds__(64) void k_sub_v(Dims d, int ntr, long ts, const TF
   constexpr int isub = SCH == UPSTREAM ? 0 : 1;
   constexpr int G = kGroup<TF>;
   const int x = blockIdx.x * 64 + threadIdx.x;
-  const int y0 = blockIdx.y * kVRows;
+  const int y0 = blockIdx.y * kVRows<TF>;
   const int k = blockIdx.z / ngroups, t0 = (blockIdx.z % ngroups) * G;
   if (x >= d.NX) return;  // no cross-lane moves here
   const int NX = d.NX, NY = d.NY;
   const long o = d.NXY * (long)k;
-  const int y1 = min(y0 + kVRows, NY);
+  const int y1 = min(y0 + kVRows<TF>, NY);
   long tb[G];
 #pragma unroll
   for (int g = 0; g < G; g++) tb[g] = (long)min(t0 + g, ntr - 1) * ts;
@@ -593,12 +596,12 @@ __global__ __launch_bounds__(64) void k_sub_v_adj(
   constexpr int isub = SCH == UPSTREAM ? 0 : 1;
   constexpr int G = kGroup<TF>;
   const int x = blockIdx.x * 64 + threadIdx.x;
-  const int y0 = blockIdx.y * kVRows;
+  const int y0 = blockIdx.y * kVRows<TF>;
   const int k = blockIdx.z / ngroups, t0 = (blockIdx.z % ngroups) * G;
   if (x >= d.NX) return;  // no cross-lane moves here
   const int NX = d.NX, NY = d.NY;
   const long o = d.NXY * (long)k;
-  const int y1 = min(y0 + kVRows, NY);
+  const int y1 = min(y0 + kVRows<TF>, NY);
   long tb[G];
 #pragma unroll
   for (int g = 0; g < G; g++) tb[g] = (long)min(t0 + g, ntr - 1) * ts;
@@ -782,7 +785,7 @@ static void launch_h_one(hipStream_t st, Dims d, int dir, int ntr, long ts, cons
                          const double* vel, const double* Dv, const uint8_t* flags, double dt,
                          double dx, double dy, double ard1, double AH) {
   const int ng = (ntr + kGroup<TF> - 1) / kGroup<TF>;
-  const dim3 gv((d.NX + 63) / 64, (d.NY + kVRows - 1) / kVRows, (d.kmax + 1) * ng);
+  const dim3 gv((d.NX + 63) / 64, (d.NY + kVRows<TF> - 1) / kVRows<TF>, (d.kmax + 1) * ng);
   if constexpr (ADJ) {
     if (dir == 0)
       hipLaunchKernelGGL((k_sub_u_adj<TF, SCH, DIFF>),
