@@ -1,6 +1,6 @@
 """FP64 HIP path against the oracle, through the C-ABI.
 
-Tolerance: the north-star bound for the plain FP64 path is 1e-12 relative per
+Tolerance: the bound for the plain FP64 path is 1e-12 relative per
 cell; the kernels keep the reference's operation order and are built without
 FMA contraction, so they are expected (and reported) to agree bit for bit.
 Each test compares full arrays, halos included.
