@@ -1341,22 +1341,32 @@ static int copy_set(hipStream_t st, TF* dst, const TF* src, size_t n) {
   return GT_OK;
 }
 
+// the inputs of each step: one set for every step (velocities and layer
+// heights held fixed) or one set per step
+struct StepInputs {
+  const double* const* f[6];  // uu vv ww hu hv ho, indexed by step (or [0] for all)
+  bool per_step;
+  const double* get(int n, int t) const { return f[n][per_step ? t : 0]; }
+};
+
 // Forward sweep from f0 with a checkpoint every ckpt_every steps (the strip
 // kernels where they apply, else the per-sub-step kernels: the same states
 // bit for bit), then the reverse sweep segment by segment: each segment's
 // states are recomputed from its checkpoint, then every step is transposed
-// from the last.  The layer heights of the five sub-steps are the same each
-// step (velocities and heights held fixed) and are formed once.
+// from the last.  The layer heights of the five sub-steps depend on the
+// step's inputs only: formed once when they are held fixed, per step when
+// they change.
 template <typename TF>
 static int adjoint_impl(const char* name, gt_ctx* c, int ntr, int nsteps, int ckpt_every,
                         double dt, const TF* f0, TF* f_final, const TF* lam_final, TF* lam0,
-                        const double* uu, const double* vv, const double* ww, const double* hu,
-                        const double* hv, const double* ho, int hscheme, int vscheme, double AH,
-                        int tag) {
+                        const StepInputs& in, int hscheme, int vscheme, double AH, int tag) {
   if (!c || ntr < 1 || !f0 || !lam_final || !lam0 || nsteps < 1 || ckpt_every < 1)
     return set_err(GT_EARG, "%s: bad argument", name);
-  if (!uu || !vv || !ww || !hu || !hv || !ho)
-    return set_err(GT_EARG, "%s: NULL velocity or layer height", name);
+  for (int n = 0; n < 6; n++) {
+    if (!in.f[n]) return set_err(GT_EARG, "%s: NULL velocity or layer height", name);
+    for (int t = 0; t < (in.per_step ? nsteps : 1); t++)
+      if (!in.f[n][t]) return set_err(GT_EARG, "%s: NULL velocity or layer height", name);
+  }
   const GridMasks* g = grid_for(c, tag);
   if (!g) return set_err(GT_EINVAL_TAG, "%s: tag is invalid", name);
   if (!split_scheme(hscheme) || !split_scheme(vscheme))
@@ -1377,18 +1387,28 @@ static int adjoint_impl(const char* name, gt_ctx* c, int ntr, int nsteps, int ck
   TF* lam[2] = {base + 4 * set, base + 5 * set};
   TF* ck = base + 6 * set;
   TF* sg = ck + (size_t)nck * set;
-  LCHK(gt::launch_heights(c->st, c->d, g->flags, ho, uu, vv, ww, Dh, dt, c->dx, c->dy, c->ard1,
-                          hscheme, kshift));
-  const double* D[6] = {ho, Dh[0], Dh[1], Dh[2], Dh[3], Dh[4]};
+  const double* D[6] = {nullptr, Dh[0], Dh[1], Dh[2], Dh[3], Dh[4]};
+  int d_step = -1;  // the step whose layer heights Dh holds
+  auto heights = [&](int t) -> int {
+    const int key = in.per_step ? t : 0;
+    if (d_step == key) return GT_OK;
+    LCHK(gt::launch_heights(c->st, c->d, g->flags, in.get(5, t), in.get(0, t), in.get(1, t),
+                            in.get(2, t), Dh, dt, c->dx, c->dy, c->ard1, hscheme, kshift));
+    D[0] = in.get(5, t);
+    d_step = key;
+    return GT_OK;
+  };
   const bool plain = c->fused && gt::fused_supported(hscheme, vscheme) && strip_fits(c);
-  auto step = [&](const TF* in, TF* out) -> int {
+  auto step = [&](int t, const TF* src, TF* dst) -> int {
     if (plain) {
-      gt_batch_domain bd{c, in, out, uu, vv, ww, hu, hv, ho, nullptr};
+      gt_batch_domain bd{c, src, dst, in.get(0, t), in.get(1, t), in.get(2, t), in.get(3, t),
+                         in.get(4, t), in.get(5, t), nullptr};
       return batch_impl<TF>(1, &bd, ntr, dt, hscheme, vscheme, AH, tag, c->st);
     }
+    TRY(heights(t));
     TF* const s[4] = {S[0], S[1], S[0], S[1]};
-    return substeps<TF>(c, c->st, g, D, dt, ntr, in, s, out, uu, vv, ww, hu, hv, hscheme, vscheme,
-                        AH, kshift);
+    return substeps<TF>(c, c->st, g, D, dt, ntr, src, s, dst, in.get(0, t), in.get(1, t),
+                        in.get(2, t), in.get(3, t), in.get(4, t), hscheme, vscheme, AH, kshift);
   };
   // forward sweep (lam[0], lam[1] as the two states)
   TF* a = lam[0];
@@ -1396,7 +1416,7 @@ static int adjoint_impl(const char* name, gt_ctx* c, int ntr, int nsteps, int ck
   TRY(copy_set<TF>(c->st, a, f0, set));
   for (int t = 0; t < nsteps; t++) {
     if (t % ckpt_every == 0) TRY(copy_set<TF>(c->st, ck + (size_t)(t / ckpt_every) * set, a, set));
-    TRY(step(a, b));
+    TRY(step(t, a, b));
     TF* x = a; a = b; b = x;
   }
   if (f_final) TRY(copy_set<TF>(c->st, f_final, a, set));
@@ -1407,13 +1427,28 @@ static int adjoint_impl(const char* name, gt_ctx* c, int ntr, int nsteps, int ck
     auto state = [&](int t) -> TF* {
       return t == t0 ? ck + (size_t)s * set : sg + (size_t)(t - t0 - 1) * set;
     };
-    for (int t = t0; t < t1 - 1; t++) TRY(step(state(t), state(t + 1)));
-    for (int t = t1 - 1; t >= t0; t--)
-      TRY(reverse_step<TF>(c, g, D, dt, ntr, state(t), S, lam, uu, vv, ww, hu, hv, hscheme,
-                           vscheme, AH, kshift));
+    for (int t = t0; t < t1 - 1; t++) TRY(step(t, state(t), state(t + 1)));
+    for (int t = t1 - 1; t >= t0; t--) {
+      TRY(heights(t));
+      TRY(reverse_step<TF>(c, g, D, dt, ntr, state(t), S, lam, in.get(0, t), in.get(1, t),
+                           in.get(2, t), in.get(3, t), in.get(4, t), hscheme, vscheme, AH,
+                           kshift));
+    }
   }
   return copy_set<TF>(c->st, lam0, lam[0], set);
 }
+
+// one set of inputs for every step
+struct FixedInputs {
+  const double* p[6];
+  StepInputs in;
+  FixedInputs(const double* uu, const double* vv, const double* ww, const double* hu,
+              const double* hv, const double* ho)
+      : p{uu, vv, ww, hu, hv, ho} {
+    for (int n = 0; n < 6; n++) in.f[n] = &p[n];
+    in.per_step = false;
+  }
+};
 
 extern "C" int gt_adv3d_adjoint(gt_ctx* c, int nsteps, int ckpt_every, double dt,
                                 const double* f0, double* f_final, const double* lam_final,
@@ -1421,8 +1456,9 @@ extern "C" int gt_adv3d_adjoint(gt_ctx* c, int nsteps, int ckpt_every, double dt
                                 const double* ww, const double* hu, const double* hv,
                                 const double* ho, int hscheme, int vscheme, double AH,
                                 int tag) {
+  const FixedInputs fi(uu, vv, ww, hu, hv, ho);
   return adjoint_impl<double>("gt_adv3d_adjoint", c, 1, nsteps, ckpt_every, dt, f0, f_final,
-                              lam_final, lam0, uu, vv, ww, hu, hv, ho, hscheme, vscheme, AH, tag);
+                              lam_final, lam0, fi.in, hscheme, vscheme, AH, tag);
 }
 
 extern "C" int gt_adv3d_adjoint_f32(gt_ctx* c, int ntracers, int nsteps, int ckpt_every,
@@ -1431,9 +1467,33 @@ extern "C" int gt_adv3d_adjoint_f32(gt_ctx* c, int ntracers, int nsteps, int ckp
                                     const double* vv, const double* ww, const double* hu,
                                     const double* hv, const double* ho, int hscheme,
                                     int vscheme, double AH, int tag) {
+  const FixedInputs fi(uu, vv, ww, hu, hv, ho);
   return adjoint_impl<float>("gt_adv3d_adjoint_f32", c, ntracers, nsteps, ckpt_every, dt, f0,
-                             f_final, lam_final, lam0, uu, vv, ww, hu, hv, ho, hscheme, vscheme,
-                             AH, tag);
+                             f_final, lam_final, lam0, fi.in, hscheme, vscheme, AH, tag);
+}
+
+extern "C" int gt_adv3d_adjoint_seq(gt_ctx* c, int nsteps, int ckpt_every, double dt,
+                                    const double* f0, double* f_final, const double* lam_final,
+                                    double* lam0, const double* const* uu,
+                                    const double* const* vv, const double* const* ww,
+                                    const double* const* hu, const double* const* hv,
+                                    const double* const* ho, int hscheme, int vscheme,
+                                    double AH, int tag) {
+  const StepInputs in{{uu, vv, ww, hu, hv, ho}, true};
+  return adjoint_impl<double>("gt_adv3d_adjoint_seq", c, 1, nsteps, ckpt_every, dt, f0,
+                              f_final, lam_final, lam0, in, hscheme, vscheme, AH, tag);
+}
+
+extern "C" int gt_adv3d_adjoint_seq_f32(gt_ctx* c, int ntracers, int nsteps, int ckpt_every,
+                                        double dt, const float* f0, float* f_final,
+                                        const float* lam_final, float* lam0,
+                                        const double* const* uu, const double* const* vv,
+                                        const double* const* ww, const double* const* hu,
+                                        const double* const* hv, const double* const* ho,
+                                        int hscheme, int vscheme, double AH, int tag) {
+  const StepInputs in{{uu, vv, ww, hu, hv, ho}, true};
+  return adjoint_impl<float>("gt_adv3d_adjoint_seq_f32", c, ntracers, nsteps, ckpt_every, dt,
+                             f0, f_final, lam_final, lam0, in, hscheme, vscheme, AH, tag);
 }
 
 extern "C" int gt_ip_blumberg_mellor_adjoint(gt_ctx* c, const double* hun, const double* hvn,

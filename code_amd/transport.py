@@ -41,6 +41,7 @@ EXPORTS = [
     "gt_exchange_tiles", "gt_budget", "gt_malloc", "gt_free",
     "gt_memcpy_h2d", "gt_memcpy_d2h", "gt_adv3d_batch_f32", "gt_adv3d_batch_f64",
     "gt_set_tuning", "gt_budget_f32", "gt_adv3d_adjoint", "gt_adv3d_adjoint_f32",
+    "gt_adv3d_adjoint_seq", "gt_adv3d_adjoint_seq_f32",
     "gt_ip_blumberg_mellor_adjoint", "gt_eos_linear_adjoint", "gt_autotune_batch",
     "gt_do_eqstate", "gt_rho_from_theta_unesco80", "gt_rho_from_theta", "gt_eosall_from_theta",
     "gt_start_macro", "gt_set_coordinates", "gt_coordinates", "gt_ww_momentum_3d",
@@ -141,6 +142,10 @@ def load_library():
                                  d, i]),
         "gt_adv3d_adjoint_f32": (i, [vp, i, i, i, d, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                      i, i, d, i]),
+        "gt_adv3d_adjoint_seq": (i, [vp, i, i, d, vp, vp, vp, vp] + [C.POINTER(vp)] * 6 +
+                                 [i, i, d, i]),
+        "gt_adv3d_adjoint_seq_f32": (i, [vp, i, i, i, d, vp, vp, vp, vp] + [C.POINTER(vp)] * 6 +
+                                     [i, i, d, i]),
         "gt_ip_blumberg_mellor_adjoint": (i, [vp, vp, vp, vp, vp, vp, vp]),
         "gt_eos_linear_adjoint": (i, [vp, vp, vp, vp, d, d]),
         "gt_autotune_batch": (i, [i, C.POINTER(BatchDomain), i, d, i, i, d, i, i, i, vp,
@@ -432,6 +437,30 @@ class Context:
                                             _ptr(f_final), _ptr(lam_final), _ptr(lam0), _ptr(uu),
                                             _ptr(vv), _ptr(ww), _ptr(hu), _ptr(hv), _ptr(ho),
                                             hscheme, vscheme, AH, tag))
+
+    def adv3d_adjoint_seq(self, ntracers, ckpt_every, dt, f0, lam_final, lam0, steps, hscheme,
+                          vscheme, AH=-1.0, tag=H_TAG, f_final=None):
+        """gt_adv3d_adjoint_seq (FP64 tensors, ntracers = 1) or _seq_f32 (FP32,
+        shape (ntracers, kmax+1, NY, NX)); steps: one (uu, vv, ww, hu, hv, ho)
+        tuple of device tensors per step"""
+        import torch
+        n = len(steps)
+        arrs = []
+        for q in range(6):
+            a = (C.c_void_p * n)(*[steps[t][q].data_ptr() for t in range(n)])
+            arrs.append(a)
+        ptrs = [C.cast(a, C.POINTER(C.c_void_p)) for a in arrs]
+        self._seq_keep = arrs  # alive until the call returns
+        if f0.dtype == torch.float32:
+            check(self.lib.gt_adv3d_adjoint_seq_f32(self.h, ntracers, n, ckpt_every, dt, _ptr(f0),
+                                                    _ptr(f_final), _ptr(lam_final), _ptr(lam0),
+                                                    *ptrs, hscheme, vscheme, AH, tag))
+        else:
+            if ntracers != 1:
+                raise ValueError("adv3d_adjoint_seq: FP64 takes one tracer")
+            check(self.lib.gt_adv3d_adjoint_seq(self.h, n, ckpt_every, dt, _ptr(f0), _ptr(f_final),
+                                                _ptr(lam_final), _ptr(lam0), *ptrs, hscheme,
+                                                vscheme, AH, tag))
 
     def ip_blumberg_mellor_adjoint(self, hun, hvn, zz, idpdx_bar, idpdy_bar, buoy_bar):
         check(self.lib.gt_ip_blumberg_mellor_adjoint(self.h, _ptr(hun), _ptr(hvn), _ptr(zz),

@@ -379,6 +379,28 @@ int gt_adv3d_adjoint_f32(gt_ctx *ctx, int ntracers, int nsteps, int ckpt_every,
                          const double *vv, const double *ww, const double *hu,
                          const double *hv, const double *ho, int hscheme,
                          int vscheme, double AH, int tag);
+/* the two above with inputs that change from step to step (a tide, the
+ * producers of each macro step): uu[t], vv[t], ww[t], hu[t], hv[t], ho[t]
+ * (host arrays of nsteps device pointers) are the inputs of step t; the layer
+ * heights of the sub-steps are formed per step.  With the same pointers for
+ * every step the result equals gt_adv3d_adjoint / _f32 bit for bit. */
+int gt_adv3d_adjoint_seq(gt_ctx *ctx, int nsteps, int ckpt_every, double dt,
+                         const double *f0, double *f_final,
+                         const double *lam_final, double *lam0,
+                         const double *const *uu, const double *const *vv,
+                         const double *const *ww, const double *const *hu,
+                         const double *const *hv, const double *const *ho,
+                         int hscheme, int vscheme, double AH, int tag);
+int gt_adv3d_adjoint_seq_f32(gt_ctx *ctx, int ntracers, int nsteps,
+                             int ckpt_every, double dt, const float *f0,
+                             float *f_final, const float *lam_final,
+                             float *lam0, const double *const *uu,
+                             const double *const *vv,
+                             const double *const *ww,
+                             const double *const *hu,
+                             const double *const *hv,
+                             const double *const *ho, int hscheme,
+                             int vscheme, double AH, int tag);
 /* transpose of ip_blumberg_mellor (ip_blumberg_mellor.F90:99-149) with
  * respect to buoy, zz/hun/hvn from the forward call: buoy_bar is overwritten
  * (kmax <= 64) */
