@@ -1507,6 +1507,23 @@ extern "C" int gt_ip_blumberg_mellor_adjoint(gt_ctx* c, const double* hun, const
   return GT_OK;
 }
 
+extern "C" int gt_ip_shchepetkin_mcwilliams_adjoint(gt_ctx* c, const double* hn,
+                                                    const double* hun, const double* hvn,
+                                                    const double* buoy, const double* sseo,
+                                                    const double* zz, const double* idpdx_bar,
+                                                    const double* idpdy_bar, double* buoy_bar) {
+  if (!c || !hn || !hun || !hvn || !buoy || !sseo || !zz || !idpdx_bar || !idpdy_bar ||
+      !buoy_bar)
+    return set_err(GT_EARG, "gt_ip_shchepetkin_mcwilliams_adjoint: NULL argument");
+  if (c->kmax < 2 || c->kmax > 64)
+    return set_err(GT_EARG, "gt_ip_shchepetkin_mcwilliams_adjoint: kmax must be 2..64");
+  ENSURE(c->fi, c->N3);
+  ENSURE(c->hi, c->N3);
+  LCHK(gt::launch_sm_adj(c->st, c->d, c->az, c->au, c->av, buoy, zz, hn, hun, hvn, sseo,
+                         idpdx_bar, idpdy_bar, c->fi, c->hi, buoy_bar, c->dx, c->dy));
+  return GT_OK;
+}
+
 extern "C" int gt_eos_linear_adjoint(gt_ctx* c, const double* buoy_bar, double* T_bar,
                                      double* S_bar, double dtr0, double dsr0) {
   if (!c || !buoy_bar) return set_err(GT_EARG, "gt_eos_linear_adjoint: NULL argument");

@@ -41,7 +41,7 @@ EXPORTS = [
     "gt_exchange_tiles", "gt_budget", "gt_malloc", "gt_free",
     "gt_memcpy_h2d", "gt_memcpy_d2h", "gt_adv3d_batch_f32", "gt_adv3d_batch_f64",
     "gt_set_tuning", "gt_budget_f32", "gt_adv3d_adjoint", "gt_adv3d_adjoint_f32",
-    "gt_adv3d_adjoint_seq", "gt_adv3d_adjoint_seq_f32",
+    "gt_adv3d_adjoint_seq", "gt_adv3d_adjoint_seq_f32", "gt_ip_shchepetkin_mcwilliams_adjoint",
     "gt_ip_blumberg_mellor_adjoint", "gt_eos_linear_adjoint", "gt_autotune_batch",
     "gt_do_eqstate", "gt_rho_from_theta_unesco80", "gt_rho_from_theta", "gt_eosall_from_theta",
     "gt_start_macro", "gt_set_coordinates", "gt_coordinates", "gt_ww_momentum_3d",
@@ -147,6 +147,7 @@ def load_library():
         "gt_adv3d_adjoint_seq_f32": (i, [vp, i, i, i, d, vp, vp, vp, vp] + [C.POINTER(vp)] * 6 +
                                      [i, i, d, i]),
         "gt_ip_blumberg_mellor_adjoint": (i, [vp, vp, vp, vp, vp, vp, vp]),
+        "gt_ip_shchepetkin_mcwilliams_adjoint": (i, [vp] * 10),
         "gt_eos_linear_adjoint": (i, [vp, vp, vp, vp, d, d]),
         "gt_autotune_batch": (i, [i, C.POINTER(BatchDomain), i, d, i, i, d, i, i, i, vp,
                                   C.POINTER(i), C.POINTER(i), C.POINTER(C.c_float),
@@ -466,6 +467,12 @@ class Context:
         check(self.lib.gt_ip_blumberg_mellor_adjoint(self.h, _ptr(hun), _ptr(hvn), _ptr(zz),
                                                      _ptr(idpdx_bar), _ptr(idpdy_bar),
                                                      _ptr(buoy_bar)))
+
+    def ip_shchepetkin_mcwilliams_adjoint(self, hn, hun, hvn, buoy, sseo, zz, idpdx_bar,
+                                          idpdy_bar, buoy_bar):
+        check(self.lib.gt_ip_shchepetkin_mcwilliams_adjoint(
+            self.h, _ptr(hn), _ptr(hun), _ptr(hvn), _ptr(buoy), _ptr(sseo), _ptr(zz),
+            _ptr(idpdx_bar), _ptr(idpdy_bar), _ptr(buoy_bar)))
 
     def eos_linear_adjoint(self, buoy_bar, T_bar, S_bar, dtr0, dsr0):
         check(self.lib.gt_eos_linear_adjoint(self.h, _ptr(buoy_bar), _ptr(T_bar), _ptr(S_bar),

@@ -408,6 +408,18 @@ int gt_ip_blumberg_mellor_adjoint(gt_ctx *ctx, const double *hun,
                                   const double *hvn, const double *zz,
                                   const double *idpdx_bar,
                                   const double *idpdy_bar, double *buoy_bar);
+/* transpose of ip_shchepetkin_mcwilliams (ip_shchepetkin_mcwilliams.F90:
+ * 90-286) with respect to buoy at the forward point: hn, hun, hvn, sseo and
+ * the buoyancy of the forward call, zz as that call wrote it; the harmonic
+ * means enter through their derivatives (zero where the forward's eps test
+ * failed).  buoy_bar is overwritten (kmax <= 64). */
+int gt_ip_shchepetkin_mcwilliams_adjoint(gt_ctx *ctx, const double *hn,
+                                         const double *hun, const double *hvn,
+                                         const double *buoy, const double *sseo,
+                                         const double *zz,
+                                         const double *idpdx_bar,
+                                         const double *idpdy_bar,
+                                         double *buoy_bar);
 /* transpose of the linear EOS + buoyancy (eqstate.F90:127-149,252-261):
  * T_bar and S_bar (either may be NULL) are overwritten on levels 1..kmax */
 int gt_eos_linear_adjoint(gt_ctx *ctx, const double *buoy_bar, double *T_bar,

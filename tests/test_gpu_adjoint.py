@@ -261,3 +261,99 @@ def test_substep_forward_and_checkpoints_bit_identical(device, tag):
                                 gt.P2_PDM, AH, tag)
         torch.cuda.synchronize()
         assert torch.equal(x, ffr), AH
+
+
+def smcw_case(device):
+    """islands, land sentinels in the buoyancy, a sloping surface, a smooth
+    stratified buoyancy in the water"""
+    imax, jmax, kmax = 30, 24, 7
+    dom0 = inputs.make_domain(imax, jmax, kmax, 7, hmin=10.0, hmax=40.0, walls=2)
+    NY, NX = dom0.NY, dom0.NX
+    az = dom0.az.copy()
+    isl = inputs.uniform((NY, NX), 71) < 0.1
+    isl[:4, :] = isl[-4:, :] = False
+    isl[:, :4] = isl[:, -4:] = False
+    az[isl & (az == 1)] = 0
+    au, av, ax = inputs.masks_from_az(az)
+    H = dom0.H.copy()
+    H[az == 0] = -10.0
+    HU, HV = H.copy(), H.copy()
+    HU[:, :-1] = 0.5 * (H[:, :-1] + H[:, 1:])
+    HV[:-1, :] = 0.5 * (H[:-1, :] + H[1:, :])
+    dom = inputs.Domain(imax, jmax, kmax, dom0.dx, dom0.dy, H, HU, HV, az, au, av, ax)
+    sse = inputs.uniform((NY, NX), 72, -0.2, 0.2)
+    hn = inputs.gvc_layers(dom, np.where(az > 0, H, 10.0) + sse)
+    hun = inputs.gvc_layers(dom, np.where(au > 0, HU, 10.0))
+    hvn = inputs.gvc_layers(dom, np.where(av > 0, HV, 10.0))
+    kk, jj, ii = np.meshgrid(np.arange(kmax + 1), np.arange(NY), np.arange(NX), indexing="ij")
+    # horizontal differences stay >= ~1e-6 (the amplitude 1.5 + cos never
+    # vanishes), so a 1e-9 perturbation does not move a harmonic mean across
+    # its eps test (a kink the central difference would straddle)
+    b = -0.01 + 0.004 * np.sin(ii / 6.0 + 0.3) * (1.5 + np.cos(jj / 8.0)) - 0.003 * kk / kmax
+    b[:, az == 0] = 3.7
+    b[0] = 3.7
+    ctx = gt.Context.from_domain(dom)
+    dv = {n: to_dev(a, device) for n, a in (("H", H), ("hn", hn), ("hun", hun), ("hvn", hvn),
+                                             ("b", b), ("sse", sse))}
+    return dom, ctx, dv
+
+
+def test_smcw_adjoint_vs_central_differences(device):
+    """Shchepetkin-McWilliams transpose (nonlinear through its harmonic
+    means): the dot-product identity against central differences of the
+    forward for a random perturbation of every cell (1e-6 relative), and
+    single cells at coastlines, water and land (the stencils read the land
+    sentinel), to 1e-6 of the largest adjoint entry"""
+    dom, ctx, v = smcw_case(device)
+    sh = dom.shape3
+    zz = torch.zeros(sh, dtype=torch.float64, device=device)
+
+    def forward(b):
+        ix, iy = (torch.zeros(sh, dtype=torch.float64, device=device) for _ in range(2))
+        ctx.ip_shchepetkin_mcwilliams(v["H"], v["hn"], v["hun"], v["hvn"], b, v["sse"], zz, ix,
+                                      iy)
+        torch.cuda.synchronize()
+        return ix, iy
+
+    forward(v["b"])
+    rng = np.random.default_rng(17)
+    wx = torch.from_numpy(rng.standard_normal(sh)).to(device)
+    wy = torch.from_numpy(rng.standard_normal(sh)).to(device)
+    bbar = torch.zeros(sh, dtype=torch.float64, device=device)
+    ctx.ip_shchepetkin_mcwilliams_adjoint(v["hn"], v["hun"], v["hvn"], v["b"], v["sse"], zz, wx,
+                                          wy, bbar)
+    torch.cuda.synchronize()
+    eps = 1e-9
+
+    def fd(delta):
+        xp, yp = forward(v["b"] + eps * delta)
+        xm, ym = forward(v["b"] - eps * delta)
+        return float(torch.sum(wx * (xp - xm) + wy * (yp - ym))) / (2 * eps)
+
+    delta = torch.from_numpy(rng.standard_normal(sh)).to(device)
+    delta[0] = 0.0
+    lhs, rhs = fd(delta), float(torch.sum(delta * bbar))
+    assert abs(lhs - rhs) <= 1e-6 * abs(rhs), (lhs, rhs)
+    az = dom.az
+    land_next = np.zeros_like(az, bool)
+    for q in ((0, 1), (0, -1), (1, 0), (-1, 0)):
+        land_next |= np.roll(az == 0, q, axis=(0, 1))
+    inner = np.zeros_like(az, bool)
+    inner[3:-3, 3:-3] = True
+    coast_w = np.argwhere(inner & (az == 1) & land_next)
+    coast_l = np.argwhere(inner & (az == 0) & np.roll(az == 1, 1, axis=1))
+    picks = [tuple(p) for p in coast_w[rng.choice(len(coast_w), 12, replace=False)]] + \
+            [tuple(p) for p in coast_l[rng.choice(len(coast_l), min(8, len(coast_l)), replace=False)]]
+    scale = float(bbar.abs().max())
+    nonzero_land = 0
+    for (j, i) in picks:
+        for k in (1, dom.kmax // 2, dom.kmax):
+            e = torch.zeros(sh, dtype=torch.float64, device=device)
+            e[k, j, i] = 1.0
+            got = float(bbar[k, j, i])
+            want = fd(e)
+            assert abs(got - want) <= 1e-6 * scale, (k, j, i, az[j, i], got, want)
+            nonzero_land += int(az[j, i] == 0 and got != 0.0)
+    # the coastline stencils read land values (through the harmonic means,
+    # whose derivative in the large land-side difference is small, ~(1e-4/3.7)^2)
+    assert nonzero_land > 0
