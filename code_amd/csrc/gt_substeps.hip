@@ -445,8 +445,8 @@ __global__ __launch_bounds__(64) void k_sub_v(Dims d, int ntr, long ts, const TF
 
 // ---- the vertical sub-step (adv_ver_iterations = 1) ------------------------------
 // One thread per column and group of tracers (blockIdx.z), marching up with
-// the levels K-2..K+2 of each tracer, K-1..K+1 of D2 and K-1, K of ww in
-// registers (kmax >= 2).  At level K, as the strip kernels set them
+// the levels K-1..K+2 of each tracer, K..K+1 of D2, ww at K and the flux at
+// K-1/2 in registers (kmax >= 2).  At level K, as the strip kernels set them
 // (adv_split_w.F90:249-300): the flux at K+1/2 exists below kmax, the one at
 // K-1/2 above level 1 (or at level 1 with kshift); level K is updated up to
 // kmax-kshift; the limiter applies above level 1-kshift going up and below
@@ -478,52 +478,61 @@ __global__ __launch_bounds__(256) void k_sub_w(Dims d, int ntr, long ts, const T
         if (t0 + g < ntr) fout[tb[g] + c2 + NXY * K] = fin[tb[g] + c2 + NXY * K];
     return;
   }
-  TF n[G][5];  // levels K-2..K+2 (entries past kmax unused)
+  TF n[G][5];  // n[1..4]: levels K-1..K+2 (entries past kmax unused)
 #pragma unroll
   for (int g = 0; g < G; g++) {
     const TF* f = fin + tb[g];
-    n[g][0] = TF(0);
     n[g][1] = f[c2];
     n[g][2] = f[c2 + NXY];
     n[g][3] = f[c2 + 2 * NXY];
     n[g][4] = kmax >= 3 ? f[c2 + 3 * NXY] : TF(0);
     if (t0 + g < ntr) fout[tb[g] + c2] = n[g][1];  // level 0
   }
-  double dm = D2[c2], dk = D2[c2 + NXY], dp = D2[c2 + 2 * NXY];  // D2 at K-1, K, K+1
-  double wm = ww[c2], wkk = ww[c2 + NXY];                         // ww at K-1, K
+  double dk = D2[c2 + NXY], dp = D2[c2 + 2 * NXY];  // D2 at K, K+1
+  double wkk = ww[c2 + NXY];                         // ww at K
+  // The flux at K-1/2 that level K reads is the flux at K+1/2 of level K-1,
+  // so each face is evaluated once and carried up (pc).  The operands agree
+  // except the one past kmax that the strip kernels substitute, which is the
+  // upwind-upwind level of a downward face with the limiter off: not read.
+  TF pc[G];
+  {
+    WFace<EX> wf0;
+    wface_setup<SCH>(wf0, ww[c2], dt, D2[c2], dk);
+    const bool lim_up = SCH != UPSTREAM && 0 > 1 - kshift;
+    const bool lim_dn = SCH != UPSTREAM && 0 < kmax - 1;
+#pragma unroll
+    for (int g = 0; g < G; g++)
+      pc[g] = w_flux<TF, SCH>(wf0, 0, kshift, kmax, lim_up, lim_dn, TF(0), n[g][1], n[g][2],
+                              2 <= kmax ? n[g][3] : n[g][2]);
+  }
   for (int K = 1; K <= kmax; K++) {
     const long cK = c2 + NXY * K;
     const bool wk_on = K <= kmax - 1, wp_on = K > 1 || kshift, kdo = K <= kmax - kshift;
-    const bool limp_up = SCH != UPSTREAM && K - 1 > 1 - kshift;
-    const bool limp_dn = SCH != UPSTREAM && K - 1 < kmax - 1;
     const bool limk_up = SCH != UPSTREAM && K > 1 - kshift;
     const bool limk_dn = SCH != UPSTREAM && K < kmax - 1;
-    WFace<EX> wfk, wfp;
+    WFace<EX> wfk;
     wface_setup<SCH>(wfk, wkk, dt, dk, K + 1 <= kmax ? dp : dk);
-    wface_setup<SCH>(wfp, wm, dt, dm, dk);
     const CellOp<EX> cw = CellOp<EX>::make(dk, D3[cK], dt);
     const bool nxt = K + 3 <= kmax;
     const long cn = c2 + NXY * (nxt ? K + 3 : kmax);
 #pragma unroll
     for (int g = 0; g < G; g++) {
-      // the operands of the strip kernels: level -1 is zero, levels past
-      // kmax are replaced by level K
-      const TF fKm2 = K > 1 ? n[g][0] : TF(0), fKm1 = n[g][1], fK = n[g][2];
+      // the operands of the strip kernels: levels past kmax are replaced by
+      // level K
+      const TF fKm1 = n[g][1], fK = n[g][2];
       const TF fKp1 = K + 1 <= kmax ? n[g][3] : fK, fKp2 = K + 2 <= kmax ? n[g][4] : fK;
-      const TF p = w_flux<TF, SCH>(wfp, K - 1, kshift, kmax, limp_up, limp_dn, fKm2, fKm1, fK, fKp1);
       const TF q = w_flux<TF, SCH>(wfk, K, kshift, kmax, limk_up, limk_dn, fKm1, fK, fKp1, fKp2);
-      const TF wprev = wp_on ? p : TF(0), wk = wk_on ? q : TF(0);
+      const TF wprev = wp_on ? pc[g] : TF(0), wk = wk_on ? q : TF(0);
       const TF r = kdo ? cw.w_step(fK, wk, wprev, dt) : fK;
       if (t0 + g < ntr) fout[tb[g] + cK] = r;
-      n[g][0] = n[g][1];
+      pc[g] = q;
       n[g][1] = n[g][2];
       n[g][2] = n[g][3];
       n[g][3] = n[g][4];
       n[g][4] = nxt ? fin[tb[g] + cn] : TF(0);
     }
-    dm = dk; dk = dp;
+    dk = dp;
     dp = K + 2 <= kmax ? D2[c2 + NXY * (K + 2)] : 0.0;
-    wm = wkk;
     wkk = K + 1 <= kmax ? ww[c2 + NXY * (K + 1)] : 0.0;
   }
 }
