@@ -427,10 +427,18 @@ __global__ void k_eos_linear(long n, long off, const double* __restrict__ T,
   buoy[c] = x * (r - rho_0);
 }
 
+// levels whose loads a column march issues together: the per-column
+// recurrences below stay sequential, but a block's operands arrive in one
+// memory round trip instead of one per level
+// (4 for Blumberg-Mellor, whose eight operands per level would otherwise
+// need more registers than a wavefront beside the advection's can take)
+constexpr int kColBlock = 8;
+constexpr int kBmBlock = 4;
+
 // ---------------------------------------------------------------------------
 // a9: ip_blumberg_mellor, src/3d/ip_blumberg_mellor.F90:67-150.
 // zz: interface heights over [imin..imax+1]x[jmin..jmax+1] where az>=1.
-__global__ void k_bm_zz(Dims d, const int32_t* __restrict__ az, const double* __restrict__ H,
+__global__ __launch_bounds__(256) void k_bm_zz(Dims d, const int32_t* __restrict__ az, const double* __restrict__ H,
                         const double* __restrict__ hn, double* __restrict__ zz) {
   int ii = blockIdx.x * blockDim.x + threadIdx.x;
   int jj = blockIdx.y * blockDim.y + threadIdx.y;
@@ -440,17 +448,26 @@ __global__ void k_bm_zz(Dims d, const int32_t* __restrict__ az, const double* __
   if (ii < 2 || ii > d.NX - 2 || jj < 2 || jj > d.NY - 2 || az[c0] < 1) return;
   double z = -H[c0] + hn[c0 + d.NXY];
   zz[c0 + d.NXY] = z;
-#pragma unroll 4
-  for (int k = 2; k <= d.kmax; k++) {
-    z = z + hn[c0 + d.NXY * k];
-    zz[c0 + d.NXY * k] = z;
+  for (int kb = 2; kb <= d.kmax; kb += kColBlock) {
+    double q[kColBlock];
+#pragma unroll
+    for (int u = 0; u < kColBlock; u++) q[u] = hn[c0 + d.NXY * min(kb + u, d.kmax)];
+    // levels past kmax leave z alone (a select, not a branch, so the loads
+    // stay ahead of their uses)
+#pragma unroll
+    for (int u = 0; u < kColBlock; u++) {
+      const bool on = kb + u <= d.kmax;
+      const double zn = z + q[u];
+      z = on ? zn : z;
+      if (on) zz[c0 + d.NXY * (kb + u)] = z;
+    }
   }
 }
 
 // idpdx and idpdy (:99-149) in one downward loop per column (the column's
 // own buoy and zz are loaded once for both directions); each recurrence is
 // the reference's, in its order.
-__global__ void k_bm_idpd(Dims d, const int32_t* __restrict__ au, const int32_t* __restrict__ av,
+__global__ __launch_bounds__(256) void k_bm_idpd(Dims d, const int32_t* __restrict__ au, const int32_t* __restrict__ av,
                           const double* __restrict__ buoy, const double* __restrict__ zz,
                           const double* __restrict__ hun, const double* __restrict__ hvn,
                           double* __restrict__ idpdx, double* __restrict__ idpdy, double dxm1,
@@ -475,33 +492,53 @@ __global__ void k_bm_idpd(Dims d, const int32_t* __restrict__ au, const int32_t*
   double prgry = grdly * 0.5 * hky;
   if (ux) AT(idpdx, c0, kmax) = hkx * prgrx;
   if (vy) AT(idpdy, c0, kmax) = hky * prgry;
-#pragma unroll 4
-  for (int k = kmax - 1; k >= 1; k--) {
-    b0 = AT(buoy, c0, k);
-    be = AT(buoy, e, k);
-    bn = AT(buoy, n, k);
-    const double z0 = AT(zz, c0, k), ze = AT(zz, e, k), zn = AT(zz, n, k);
-    {
-      double grdu = grdlx;
-      grdlx = (be - b0) * dxm1;
-      double buoyu = buoylx;
-      buoylx = 0.5 * (be + b0);
-      double dxz = (ze - z0) * dxm1;
-      double hkk = AT(hun, c0, k);
-      prgrx = prgrx + 0.5 * (grdu + grdlx) * 0.5 * (hkk + hkx) - dxz * (buoyu - buoylx);
-      if (ux) AT(idpdx, c0, k) = hkk * prgrx;
-      hkx = hkk;
+  for (int kb = kmax - 1; kb >= 1; kb -= kBmBlock) {
+    // the operands of kBmBlock levels are loaded first (one memory round
+    // trip per block instead of per level); levels below 1 repeat level 1
+    double qb0[kBmBlock], qbe[kBmBlock], qbn[kBmBlock], qz0[kBmBlock], qze[kBmBlock],
+        qzn[kBmBlock], qhu[kBmBlock], qhv[kBmBlock];
+#pragma unroll
+    for (int u = 0; u < kBmBlock; u++) {
+      const int k = max(kb - u, 1);
+      qb0[u] = AT(buoy, c0, k);
+      qbe[u] = AT(buoy, e, k);
+      qbn[u] = AT(buoy, n, k);
+      qz0[u] = AT(zz, c0, k);
+      qze[u] = AT(zz, e, k);
+      qzn[u] = AT(zz, n, k);
+      qhu[u] = AT(hun, c0, k);
+      qhv[u] = AT(hvn, c0, k);
     }
-    {
-      double grdu = grdly;
-      grdly = (bn - b0) * dym1;
-      double buoyu = buoyly;
-      buoyly = 0.5 * (bn + b0);
-      double dyz = (zn - z0) * dym1;
-      double hkk = AT(hvn, c0, k);
-      prgry = prgry + 0.5 * (grdu + grdly) * 0.5 * (hkk + hky) - dyz * (buoyu - buoyly);
-      if (vy) AT(idpdy, c0, k) = hkk * prgry;
-      hky = hkk;
+#pragma unroll
+    for (int u = 0; u < kBmBlock; u++) {
+      const int k = kb - u;
+      if (k < 1) break;
+      b0 = qb0[u];
+      be = qbe[u];
+      bn = qbn[u];
+      const double z0 = qz0[u], ze = qze[u], zn = qzn[u];
+      {
+        double grdu = grdlx;
+        grdlx = (be - b0) * dxm1;
+        double buoyu = buoylx;
+        buoylx = 0.5 * (be + b0);
+        double dxz = (ze - z0) * dxm1;
+        double hkk = qhu[u];
+        prgrx = prgrx + 0.5 * (grdu + grdlx) * 0.5 * (hkk + hkx) - dxz * (buoyu - buoylx);
+        if (ux) AT(idpdx, c0, k) = hkk * prgrx;
+        hkx = hkk;
+      }
+      {
+        double grdu = grdly;
+        grdly = (bn - b0) * dym1;
+        double buoyu = buoyly;
+        buoyly = 0.5 * (bn + b0);
+        double dyz = (zn - z0) * dym1;
+        double hkk = qhv[u];
+        prgry = prgry + 0.5 * (grdu + grdly) * 0.5 * (hkk + hky) - dyz * (buoyu - buoyly);
+        if (vy) AT(idpdy, c0, k) = hkk * prgry;
+        hky = hkk;
+      }
     }
   }
 #undef AT
@@ -523,7 +560,7 @@ __device__ __forceinline__ double sm_hmR(double a, double b) {
   double cff = 2.0 * a * b;
   return cff > SM_EPS ? cff / (a + b) : 0.0;
 }
-__global__ void k_sm_column(Dims d, const int32_t* __restrict__ az, const double* __restrict__ H,
+__global__ __launch_bounds__(256) void k_sm_column(Dims d, const int32_t* __restrict__ az, const double* __restrict__ H,
                             const double* __restrict__ hn, const double* __restrict__ buoy,
                             double* __restrict__ zz, double* __restrict__ P) {
   int ii = blockIdx.x * blockDim.x + threadIdx.x;
@@ -541,13 +578,20 @@ __global__ void k_sm_column(Dims d, const int32_t* __restrict__ az, const double
     AT(zz, 1) = z;
     double hprev = AT(hn, 1);
     zm = z;
-#pragma unroll 4
-    for (int k = 2; k <= kmax; k++) {
-      const double hk = AT(hn, k);
-      zm = z;
-      z = z + 0.5 * (hprev + hk);
-      AT(zz, k) = z;
-      hprev = hk;
+    for (int kb = 2; kb <= kmax; kb += kColBlock) {
+      double q[kColBlock];
+#pragma unroll
+      for (int u = 0; u < kColBlock; u++) q[u] = AT(hn, min(kb + u, kmax));
+#pragma unroll
+      for (int u = 0; u < kColBlock; u++) {
+        const bool on = kb + u <= kmax;  // a select, as in k_bm_zz
+        const double hk = q[u];
+        const double zn = z + 0.5 * (hprev + hk);
+        zm = on ? z : zm;
+        z = on ? zn : z;
+        hprev = on ? hk : hprev;
+        if (on) AT(zz, kb + u) = z;
+      }
     }
     zt = z;
   }
@@ -565,31 +609,44 @@ __global__ void k_sm_column(Dims d, const int32_t* __restrict__ az, const double
   AT(P, kmax) = p;
   double rk = b1 - b0, zk = z1 - z0;             // dR, dZ at kmax-1 (= at kmax)
   double hR1 = sm_hmR(rk, rk), hZ1 = hmZ(zk, zk);  // harmonic means at kmax
-#pragma unroll 4
-  for (int k = kmax - 1; k >= 1; k--) {
-    double bm = 0.0, zmv = 0.0, rkm, zkm;
-    if (k >= 2) {
-      bm = AT(buoy, k - 1);
-      zmv = AT(zz, k - 1);
-      rkm = b0 - bm;
-      zkm = z0 - zmv;
-    } else {
-      rkm = b1 - b0;  // dR(0) = dR(1)
-      zkm = z1 - z0;
+  for (int kb = kmax - 1; kb >= 1; kb -= kColBlock) {
+    // buoy and zz at k-1 for the block's levels, loaded first (level 1
+    // stands in below level 1, unused)
+    double qb[kColBlock], qz[kColBlock];
+#pragma unroll
+    for (int u = 0; u < kColBlock; u++) {
+      const int l = max(kb - u - 1, 1);
+      qb[u] = AT(buoy, l);
+      qz[u] = AT(zz, l);
     }
-    double hR = sm_hmR(rk, rkm), hZ = hmZ(zk, zkm);
-    p = p + 0.5 * ((b1 + b0) * (z1 - z0) -
-                   SM_ONEFIFTH * ((hR1 - hR) * (z1 - z0 - SM_C12 * (hZ1 + hZ)) -
-                                  (hZ1 - hZ) * (b1 - b0 - SM_C12 * (hR1 + hR))));
-    AT(P, k) = p;
-    hR1 = hR;
-    hZ1 = hZ;
-    rk = rkm;
-    zk = zkm;
-    b1 = b0;
-    b0 = bm;
-    z1 = z0;
-    z0 = zmv;
+#pragma unroll
+    for (int u = 0; u < kColBlock; u++) {
+      const int k = kb - u;
+      if (k < 1) break;
+      double bm = 0.0, zmv = 0.0, rkm, zkm;
+      if (k >= 2) {
+        bm = qb[u];
+        zmv = qz[u];
+        rkm = b0 - bm;
+        zkm = z0 - zmv;
+      } else {
+        rkm = b1 - b0;  // dR(0) = dR(1)
+        zkm = z1 - z0;
+      }
+      double hR = sm_hmR(rk, rkm), hZ = hmZ(zk, zkm);
+      p = p + 0.5 * ((b1 + b0) * (z1 - z0) -
+                     SM_ONEFIFTH * ((hR1 - hR) * (z1 - z0 - SM_C12 * (hZ1 + hZ)) -
+                                    (hZ1 - hZ) * (b1 - b0 - SM_C12 * (hR1 + hR))));
+      AT(P, k) = p;
+      hR1 = hR;
+      hZ1 = hZ;
+      rk = rkm;
+      zk = zkm;
+      b1 = b0;
+      b0 = bm;
+      z1 = z0;
+      z0 = zmv;
+    }
   }
 #undef AT
 }
