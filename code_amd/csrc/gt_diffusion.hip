@@ -60,10 +60,22 @@ __global__ void __launch_bounds__(kBx* kBy) k_vertical_diffusion(DiffArgs a) {
     L(a.rad, kmax) = swr;
     double zz = 0.0;
     double rad0 = swr;
-    for (int k = kmax - 1; k >= 0; k--) {
-      zz = zz + L(a.hn, k + 1);
-      rad0 = swr * (Ac * exp(-zz / g1) + (1 - Ac) * exp(-zz / g2));
-      L(a.rad, k) = rad0;
+    // kChunk levels of hn per memory round trip; the levels below 0 of the
+    // last chunk are selected away (a select, not a branch, so the loads stay
+    // ahead of their uses)
+    for (int kb = kmax - 1; kb >= 0; kb -= kChunk) {
+      double q[kChunk];
+#pragma unroll
+      for (int v = 0; v < kChunk; v++) q[v] = L(a.hn, max(kb - v, 0) + 1);
+#pragma unroll
+      for (int v = 0; v < kChunk; v++) {
+        const bool on = kb - v >= 0;
+        const double zn = zz + q[v];
+        zz = on ? zn : zz;
+        const double r = swr * (Ac * exp(-zz / g1) + (1 - Ac) * exp(-zz / g2));
+        rad0 = on ? r : rad0;
+        if (on) L(a.rad, kb - v) = rad0;
+      }
     }
     if (azc != 1) return;
     // the primitive sea-ice model, :490-493
@@ -74,9 +86,21 @@ __global__ void __launch_bounds__(kBx* kBy) k_vertical_diffusion(DiffArgs a) {
       const double swr_refl = rad0 * a.refl_frac;
       L(a.rad1d, 0) = rad0 - swr_refl;
       double z2 = 0.0;
-      for (int k = 1; k <= kmax; k++) {
-        z2 = z2 + L(a.hn, k);
-        L(a.rad1d, k) = L(a.rad, k) - swr_refl * (Ac * exp(-z2 / g1) + (1 - Ac) * exp(-z2 / g2));
+      for (int kb = 1; kb <= kmax; kb += kChunk) {  // loads ahead, as above
+        double qh[kChunk], qr[kChunk];
+#pragma unroll
+        for (int v = 0; v < kChunk; v++) {
+          qh[v] = L(a.hn, min(kb + v, kmax));
+          qr[v] = L(a.rad, min(kb + v, kmax));
+        }
+#pragma unroll
+        for (int v = 0; v < kChunk; v++) {
+          const bool on = kb + v <= kmax;
+          const double zn = z2 + qh[v];
+          z2 = on ? zn : z2;
+          const double r = qr[v] - swr_refl * (Ac * exp(-z2 / g1) + (1 - Ac) * exp(-z2 / g2));
+          if (on) L(a.rad1d, kb + v) = r;
+        }
       }
       radsrc = a.rad1d;
     }
@@ -93,25 +117,41 @@ __global__ void __launch_bounds__(kBx* kBy) k_vertical_diffusion(DiffArgs a) {
     double ruv[KM + 1], quv[KM + 1];
     double fkp = 0.0, fk = 0.0, hk = 0.0, auxo_k = 0.0, auxn_k = 0.0, rk = 0.0;
     double ru = 0.0, qu = 0.0;
+    const double h_top = L(a.hn, kmax), f_top = L(a.f, kmax);
+    const double r_top = TEMP ? L(radsrc, kmax) : 0.0;
+    // level k-1 of rows k = kc..kc-kChunk+1, loaded together (one memory
+    // round trip per kChunk rows); rows past kmax read level kmax, unused
+    double ph[kChunk], pf[kChunk], pn[kChunk], pr[kChunk];
 #pragma unroll
     for (int k = KM; k >= 1; k--) {
+      const int u = (KM - k) % kChunk;
+      if (u == 0) {
+#pragma unroll
+        for (int v = 0; v < kChunk; v++) {
+          const int l = min(max(k - v - 1, 0), kmax);
+          ph[v] = L(a.hn, l);
+          pf[v] = L(a.f, l);
+          pn[v] = L(a.nuh, l);
+          if (TEMP) pr[v] = L(radsrc, l);
+        }
+      }
       // a guard, not continue/break: the loop must unroll fully so ruv/quv
       // stay in registers
       if (k <= kmax) {
       if (k == kmax) {
-        hk = L(a.hn, k);
-        fk = L(a.f, k);
-        if (TEMP) rk = L(radsrc, k) * rc;
+        hk = h_top;
+        fk = f_top;
+        if (TEMP) rk = r_top * rc;
       }
       double hkm = 0.0, fkm = 0.0, auxo_m = 0.0, auxn_m = 0.0, rkm = 0.0;
       if (k >= 2) {
-        hkm = L(a.hn, k - 1);
-        fkm = L(a.f, k - 1);
-        const double nk = L(a.nuh, k - 1);
+        hkm = ph[u];
+        fkm = pf[u];
+        const double nk = pn[u];
         auxo_m = 2.0 * (1 - cnpar) * dt * (nk + avm) / (hk + hkm);
         auxn_m = 2.0 * cnpar * dt * (nk + avm) / (hk + hkm);
       }
-      if (TEMP) rkm = L(radsrc, k - 1) * rc;
+      if (TEMP) rkm = pr[u] * rc;
       if (k == kmax) {
         const double a1 = -auxn_m, a2 = hk + auxn_m;
         double a4 = fk * (hk - auxo_m) + fkm * auxo_m;
