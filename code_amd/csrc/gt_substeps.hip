@@ -445,8 +445,8 @@ __global__ __launch_bounds__(64) void k_sub_v(Dims d, int ntr, long ts, const TF
 
 // ---- the vertical sub-step (adv_ver_iterations = 1) ------------------------------
 // One thread per column and group of tracers (blockIdx.z), marching up with
-// the levels K-1..K+2 of each tracer, K..K+1 of D2, ww at K and the flux at
-// K-1/2 in registers (kmax >= 2).  At level K, as the strip kernels set them
+// the levels K-1..K+2 of each tracer, K..K+3 of D2, D3 and ww and the flux
+// at K-1/2 in registers (kmax >= 2).  At level K, as the strip kernels set them
 // (adv_split_w.F90:249-300): the flux at K+1/2 exists below kmax, the one at
 // K-1/2 above level 1 (or at level 1 with kshift); level K is updated up to
 // kmax-kshift; the limiter applies above level 1-kshift going up and below
@@ -488,8 +488,16 @@ __global__ __launch_bounds__(256) void k_sub_w(Dims d, int ntr, long ts, const T
     n[g][4] = kmax >= 3 ? f[c2 + 3 * NXY] : TF(0);
     if (t0 + g < ntr) fout[tb[g] + c2] = n[g][1];  // level 0
   }
-  double dk = D2[c2 + NXY], dp = D2[c2 + 2 * NXY];  // D2 at K, K+1
-  double wkk = ww[c2 + NXY];                         // ww at K
+  // D2, D3 and ww of levels K..K+3 in registers, each loaded three levels
+  // ahead of its use (levels past kmax read level kmax, never used)
+  auto lv = [&](int L) -> long { return c2 + NXY * (long)min(L, kmax); };
+  double d2r[4], d3r[4], wr[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    d2r[q] = D2[lv(1 + q)];
+    d3r[q] = D3[lv(1 + q)];
+    wr[q] = ww[lv(1 + q)];
+  }
   // The flux at K-1/2 that level K reads is the flux at K+1/2 of level K-1,
   // so each face is evaluated once and carried up (pc).  The operands agree
   // except the one past kmax that the strip kernels substitute, which is the
@@ -497,7 +505,7 @@ __global__ __launch_bounds__(256) void k_sub_w(Dims d, int ntr, long ts, const T
   TF pc[G];
   {
     WFace<EX> wf0;
-    wface_setup<SCH>(wf0, ww[c2], dt, D2[c2], dk);
+    wface_setup<SCH>(wf0, ww[c2], dt, D2[c2], d2r[0]);
     const bool lim_up = SCH != UPSTREAM && 0 > 1 - kshift;
     const bool lim_dn = SCH != UPSTREAM && 0 < kmax - 1;
 #pragma unroll
@@ -510,9 +518,10 @@ __global__ __launch_bounds__(256) void k_sub_w(Dims d, int ntr, long ts, const T
     const bool wk_on = K <= kmax - 1, wp_on = K > 1 || kshift, kdo = K <= kmax - kshift;
     const bool limk_up = SCH != UPSTREAM && K > 1 - kshift;
     const bool limk_dn = SCH != UPSTREAM && K < kmax - 1;
+    const double dk = d2r[0], dp = d2r[1];  // D2 at K, K+1
     WFace<EX> wfk;
-    wface_setup<SCH>(wfk, wkk, dt, dk, K + 1 <= kmax ? dp : dk);
-    const CellOp<EX> cw = CellOp<EX>::make(dk, D3[cK], dt);
+    wface_setup<SCH>(wfk, wr[0], dt, dk, K + 1 <= kmax ? dp : dk);
+    const CellOp<EX> cw = CellOp<EX>::make(dk, d3r[0], dt);
     const bool nxt = K + 3 <= kmax;
     const long cn = c2 + NXY * (nxt ? K + 3 : kmax);
 #pragma unroll
@@ -531,9 +540,15 @@ __global__ __launch_bounds__(256) void k_sub_w(Dims d, int ntr, long ts, const T
       n[g][3] = n[g][4];
       n[g][4] = nxt ? fin[tb[g] + cn] : TF(0);
     }
-    dk = dp;
-    dp = K + 2 <= kmax ? D2[c2 + NXY * (K + 2)] : 0.0;
-    wkk = K + 1 <= kmax ? ww[c2 + NXY * (K + 1)] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      d2r[q] = d2r[q + 1];
+      d3r[q] = d3r[q + 1];
+      wr[q] = wr[q + 1];
+    }
+    d2r[3] = D2[lv(K + 4)];
+    d3r[3] = D3[lv(K + 4)];
+    wr[3] = ww[lv(K + 4)];
   }
 }
 
@@ -713,32 +728,62 @@ __global__ __launch_bounds__(256) void k_sub_w_adj(Dims d, int ntr, long ts,
     TF f, acc, mu;
   };
   Lev w[G][4];
-  auto enter = [&](int L, int q) {
-    const bool in = L <= kmax;
-    const long cL = c2 + NXY * (in ? L : kmax);
-    const bool kdo = in && L >= 1 && L <= kmax - kshift;
-    CellOp<EX> cw;
-    if (kdo) cw = CellOp<EX>::make(D2[cL], D3[cL], dt);
+  // a level's loads (read two faces before it enters the window; levels past
+  // kmax read level kmax, whose slot is never stored)
+  struct Raw {
+    TF l[G], f[G];
+    double d2, d3;
+  };
+  auto lv = [&](int L) -> long { return c2 + NXY * (long)min(L, kmax); };
+  auto fetch = [&](int L, Raw& r) {
+    const long cL = lv(L);
+    r.d2 = D2[cL];
+    r.d3 = D3[cL];
 #pragma unroll
     for (int g = 0; g < G; g++) {
-      const TF l = lam_in[tb[g] + cL];
-      w[g][q].f = f[tb[g] + cL];
+      r.l[g] = lam_in[tb[g] + cL];
+      r.f[g] = f[tb[g] + cL];
+    }
+  };
+  auto enter = [&](int L, int q, const Raw& r) {
+    const bool kdo = L <= kmax && L >= 1 && L <= kmax - kshift;
+    CellOp<EX> cw;
+    if (kdo) cw = CellOp<EX>::make(r.d2, r.d3, dt);
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const TF l = r.l[g];
+      w[g][q].f = r.f[g];
       w[g][q].acc = kdo ? cw.diag(l) : l;
       w[g][q].mu = kdo ? cw.mu(l, dt, 1.0) : TF(0);
     }
   };
 #pragma unroll
   for (int g = 0; g < G; g++) w[g][0] = Lev{TF(0), TF(0), TF(0)};
-  enter(0, 1);
-  enter(1, 2);
-  enter(2, 3);
-  double d0 = D2[c2], d1 = D2[c2 + NXY];  // D2 at K', K'+1
+  {
+    Raw r0, r1, r2;
+    fetch(0, r0);
+    fetch(1, r1);
+    fetch(2, r2);
+    enter(0, 1, r0);
+    enter(1, 2, r1);
+    enter(2, 3, r2);
+  }
+  Raw rq0, rq1;  // levels K'+3, K'+4
+  fetch(3, rq0);
+  fetch(4, rq1);
+  // D2 at K', K'+1, K'+2 and ww at K', K'+1, K'+2, loaded two faces ahead
+  double dq[3], wq[3];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    dq[q] = D2[lv(q)];
+    wq[q] = ww[lv(q)];
+  }
   for (int Kf = 0; Kf <= kmax - 1; Kf++) {
     // level 0 is never updated (mu = 0), so at K' = 0 Fbar is -mu(1) with
     // kshift and 0 without
     const bool used = Kf >= 1 || kshift;
     WFace<EX> wf;
-    wface_setup<SCH>(wf, ww[c2 + NXY * Kf], dt, d0, d1);
+    wface_setup<SCH>(wf, wq[0], dt, dq[0], dq[1]);
     const bool lim_up = SCH != UPSTREAM && Kf > 1 - kshift;
     const bool lim_dn = SCH != UPSTREAM && Kf < kmax - 1;
 #pragma unroll
@@ -763,9 +808,16 @@ __global__ __launch_bounds__(256) void k_sub_w_adj(Dims d, int ntr, long ts,
       w[g][1] = w[g][2];
       w[g][2] = w[g][3];
     }
-    enter(Kf + 3, 3);
-    d0 = d1;
-    d1 = Kf + 2 <= kmax ? D2[c2 + NXY * (Kf + 2)] : 0.0;
+    enter(Kf + 3, 3, rq0);
+    rq0 = rq1;
+    fetch(Kf + 5, rq1);
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      dq[q] = dq[q + 1];
+      wq[q] = wq[q + 1];
+    }
+    dq[2] = D2[lv(Kf + 3)];
+    wq[2] = ww[lv(Kf + 3)];
   }
 #pragma unroll
   for (int g = 0; g < G; g++) {
