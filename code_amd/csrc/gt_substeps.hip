@@ -113,6 +113,42 @@ __device__ __forceinline__ FaceC<EXACT> no_face() {
   return fc;
 }
 
+// the loads of row Y of a v march: face Y's transport, layer height and
+// flux-mask bytes (rows Y-1..Y+1) and cell Y's heights before and after,
+// read one row ahead of their use (rows outside 0..NY-1 read a clamped row;
+// the face and the update are switched off there by Y itself)
+struct VRaw {
+  double vel, Dv, Dio, Dn;
+  uint8_t fm, f0, fp;
+};
+__device__ __forceinline__ VRaw vraw(const Dims& d, int x, long o, int Y,
+                                     const uint8_t* flags, const double* vel,
+                                     const double* Dv, const double* Dio, const double* Dn) {
+  const int NX = d.NX, Yc = min(max(Y, 0), d.NY - 1);
+  const long x2 = x + (long)NX * Yc;
+  VRaw r;
+  r.vel = vel[o + x2];
+  r.Dv = Dv[o + x2];
+  r.Dio = Dio[o + x2];
+  r.Dn = Dn[o + x2];
+  r.f0 = flags[x2];
+  r.fm = flags[x + (long)NX * max(Yc - 1, 0)];
+  r.fp = flags[x + (long)NX * min(Yc + 1, d.NY - 1)];
+  return r;
+}
+// face Y of the v sweep from its loads, as face_at forms it (0 <= Y <= NY-2)
+template <int SCH, bool EXACT>
+__device__ __forceinline__ FaceC<EXACT> vface(const Dims& d, int Y, const VRaw& r,
+                                              const FaceK& k) {
+  constexpr int isub = SCH == UPSTREAM ? 0 : 1;
+  const bool active = Y >= isub && Y <= d.NY - 2 - isub && (r.f0 & MB_VFLUX);
+  const bool lim_l = SCH != UPSTREAM && Y >= 1 && (r.fm & MB_VFLUX);
+  const bool lim_r = SCH != UPSTREAM && (r.fp & MB_VFLUX);
+  FaceC<EXACT> fc;
+  face_setup<SCH>(fc, active, lim_l, lim_r, r.vel, r.Dv, k);
+  return fc;
+}
+
 // the update of one cell: f_new = (Dio f - dt advn)/Dn (EXACT, the
 // reference's expression) or f + f am1 - (Fc - Fm) b (FP32), and its
 // linearisation: diag (d f_new / d f) and mu (d f_new / d Fc, = -d/d Fm)
@@ -408,7 +444,6 @@ __global__ __launch_bounds__(64) void k_sub_v(Dims d, int ntr, long ts, const TF
         if (t0 + g < ntr) fout[tb[g] + x + (long)NX * y] = fin[tb[g] + x + (long)NX * y];
     return;
   }
-  const Sweep s = sweep(d, 1);
   const FaceK fk = face_k(0.5 * dt, dy, AH);
   const double dtc = dt * (0.5 * dx * ard1);
   auto row = [&](int y) -> long { return o + x + (long)NX * (y >= 0 && y < NY ? y : 0); };
@@ -419,14 +454,17 @@ __global__ __launch_bounds__(64) void k_sub_v(Dims d, int ntr, long ts, const TF
 #pragma unroll
     for (int q = 0; q < 4; q++) w[g][q] = fin[tb[g] + row(y0 - 2 + q)];
   }
+  VRaw rn = vraw(d, x, o, y0 - 1, flags, vv, hv, Dio, Dn);
   for (int Y = y0 - 1; Y < y1; Y++) {
+    const VRaw r = rn;
+    rn = vraw(d, x, o, Y + 1, flags, vv, hv, Dio, Dn);
     FaceC<EX> fc = no_face<EX>();
-    if (Y >= 0 && Y <= NY - 2) fc = face_at<SCH, EX>(s, Y, x + (long)NX * Y, o, flags, vv, hv, fk);
+    if (Y >= 0 && Y <= NY - 2) fc = vface<SCH, EX>(d, Y, r, fk);
     const bool st = Y >= y0;
     const long cY = o + x + (long)NX * (Y >= 0 ? Y : 0);
-    const bool upd = st && Y >= 1 + isub && Y <= NY - 2 - isub && (flags[cY - o] & MB_VUPD);
+    const bool upd = st && Y >= 1 + isub && Y <= NY - 2 - isub && (r.f0 & MB_VUPD);
     CellOp<EX> cu;
-    if (upd) cu = CellOp<EX>::make(Dio[cY], Dn[cY], dtc);
+    if (upd) cu = CellOp<EX>::make(r.Dio, r.Dn, dtc);
 #pragma unroll
     for (int g = 0; g < G; g++) {
       const TF F = face_flux<TF, SCH, DIFF>(fc, w[g][0], w[g][1], w[g][2], w[g][3], AH, dy);
@@ -636,35 +674,59 @@ __global__ __launch_bounds__(64) void k_sub_v_adj(
         if (t0 + g < ntr) lam_out[tb[g] + x + (long)NX * y] = lam_in[tb[g] + x + (long)NX * y];
     return;
   }
-  const Sweep s = sweep(d, 1);
   const FaceK fk = face_k(0.5 * dt, dy, AH);
   const double fac = 0.5 * dx * ard1;
   struct Row {
     TF f, acc, mu;
   };
   Row w[G][4];
-  // row y enters the window slot q
-  auto enter = [&](int y, int q) {
+  // the loads of a row entering the window, read one face ahead
+  struct ERaw {
+    TF l[G], f[G];
+    double Dio, Dn;
+    uint8_t fl;
+  };
+  auto fetch = [&](int y) -> ERaw {
     const bool yin = y >= 0 && y < NY;
     const long cy = o + x + (long)NX * (yin ? y : 0);
-    const bool upd = yin && y >= 1 + isub && y <= NY - 2 - isub && (flags[cy - o] & MB_VUPD);
-    CellOp<EX> cu;
-    if (upd) cu = CellOp<EX>::make(Dio[cy], Dn[cy], dt * fac);
+    ERaw r;
+    r.Dio = Dio[cy];
+    r.Dn = Dn[cy];
+    r.fl = flags[cy - o];
 #pragma unroll
     for (int g = 0; g < G; g++) {
-      const TF l = lam_in[tb[g] + cy];
-      w[g][q].f = f[tb[g] + cy];
+      r.l[g] = lam_in[tb[g] + cy];
+      r.f[g] = f[tb[g] + cy];
+    }
+    return r;
+  };
+  // row y enters the window slot q
+  auto enter = [&](int y, int q, const ERaw& r) {
+    const bool yin = y >= 0 && y < NY;
+    const bool upd = yin && y >= 1 + isub && y <= NY - 2 - isub && (r.fl & MB_VUPD);
+    CellOp<EX> cu;
+    if (upd) cu = CellOp<EX>::make(r.Dio, r.Dn, dt * fac);
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const TF l = r.l[g];
+      w[g][q].f = r.f[g];
       w[g][q].acc = upd ? cu.diag(l) : l;
       w[g][q].mu = upd ? cu.mu(l, dt, fac) : TF(0);
     }
   };
-  enter(y0 - 3, 0);
-  enter(y0 - 2, 1);
-  enter(y0 - 1, 2);
-  enter(y0, 3);
+  enter(y0 - 3, 0, fetch(y0 - 3));
+  enter(y0 - 2, 1, fetch(y0 - 2));
+  enter(y0 - 1, 2, fetch(y0 - 1));
+  enter(y0, 3, fetch(y0));
+  ERaw en = fetch(y0 + 1);
+  VRaw rn = vraw(d, x, o, y0 - 2, flags, vv, hv, Dio, Dn);
   for (int Y = y0 - 2; Y <= y1; Y++) {
+    const VRaw r = rn;
+    rn = vraw(d, x, o, Y + 1, flags, vv, hv, Dio, Dn);
+    const ERaw e = en;
+    en = fetch(Y + 4);
     FaceC<EX> fc = no_face<EX>();
-    if (Y >= 0 && Y <= NY - 2) fc = face_at<SCH, EX>(s, Y, x + (long)NX * Y, o, flags, vv, hv, fk);
+    if (Y >= 0 && Y <= NY - 2) fc = vface<SCH, EX>(d, Y, r, fk);
 #pragma unroll
     for (int g = 0; g < G; g++) {
       const TF Fbar = w[g][1].mu - w[g][2].mu;
@@ -685,7 +747,7 @@ __global__ __launch_bounds__(64) void k_sub_v_adj(
       w[g][1] = w[g][2];
       w[g][2] = w[g][3];
     }
-    enter(Y + 3, 3);
+    enter(Y + 3, 3, e);
   }
 }
 
