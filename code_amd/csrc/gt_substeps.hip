@@ -77,6 +77,7 @@ constexpr int kGroup = sizeof(TF) == 8 ? 1 : 6;
 // shorter marches for more wavefronts)
 template <typename TF>
 constexpr int kVRows = sizeof(TF) == 8 ? 8 : 16;
+constexpr long kWaveTarget = 8192;  // wavefronts a column march aims for (8 per SIMD)
 constexpr int kUOutF = 60;  // outputs per wavefront of the u forward
 constexpr int kUOutA = 58;  // ... and of the u transpose
 
@@ -496,35 +497,40 @@ __global__ __launch_bounds__(256) void k_sub_w(Dims d, int ntr, long ts, const T
                                                const double* __restrict__ D3,
                                                const double* __restrict__ ww,
                                                const uint8_t* __restrict__ flags, double dt,
-                                               int kshift) {
+                                               int kshift, int ngroups, int kc) {
   constexpr bool EX = kExact<TF>;
   constexpr int G = kGroup<TF>;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int t0 = blockIdx.z * G;
-  if (i >= d.NX || j >= d.NY || t0 >= ntr) return;
+  const int t0 = (blockIdx.z % ngroups) * G;
   const int kmax = d.kmax;
+  // this thread's levels K0..K1 (chunk blockIdx.z / ngroups of kc levels;
+  // the first chunk also carries level 0)
+  const int K0 = 1 + (int)(blockIdx.z / ngroups) * kc, K1 = min(K0 + kc - 1, kmax);
+  if (i >= d.NX || j >= d.NY || t0 >= ntr || K0 > kmax) return;
   const long NXY = d.NXY, c2 = i + (long)d.NX * j;
   long tb[G];
 #pragma unroll
   for (int g = 0; g < G; g++) tb[g] = (long)min(t0 + g, ntr - 1) * ts;
   const bool wcol = (flags[c2] & MB_WCOL) && i <= d.NX - 2 && j <= d.NY - 2;
   if (!wcol) {
-    for (int K = 0; K <= kmax; K++)
+    for (int K = K0 == 1 ? 0 : K0; K <= K1; K++)
 #pragma unroll
       for (int g = 0; g < G; g++)
         if (t0 + g < ntr) fout[tb[g] + c2 + NXY * K] = fin[tb[g] + c2 + NXY * K];
     return;
   }
   TF n[G][5];  // n[1..4]: levels K-1..K+2 (entries past kmax unused)
+  TF fKm2[G];  // level K0-2 (zero below level 0), for the flux at K0-1/2
 #pragma unroll
   for (int g = 0; g < G; g++) {
     const TF* f = fin + tb[g];
-    n[g][1] = f[c2];
-    n[g][2] = f[c2 + NXY];
-    n[g][3] = f[c2 + 2 * NXY];
-    n[g][4] = kmax >= 3 ? f[c2 + 3 * NXY] : TF(0);
-    if (t0 + g < ntr) fout[tb[g] + c2] = n[g][1];  // level 0
+    fKm2[g] = K0 >= 2 ? f[c2 + NXY * (K0 - 2)] : TF(0);
+    n[g][1] = f[c2 + NXY * (K0 - 1)];
+    n[g][2] = f[c2 + NXY * K0];
+    n[g][3] = K0 + 1 <= kmax ? f[c2 + NXY * (K0 + 1)] : TF(0);
+    n[g][4] = K0 + 2 <= kmax ? f[c2 + NXY * (K0 + 2)] : TF(0);
+    if (K0 == 1 && t0 + g < ntr) fout[tb[g] + c2] = n[g][1];  // level 0
   }
   // D2, D3 and ww of levels K..K+3 in registers, each loaded three levels
   // ahead of its use (levels past kmax read level kmax, never used)
@@ -532,9 +538,9 @@ __global__ __launch_bounds__(256) void k_sub_w(Dims d, int ntr, long ts, const T
   double d2r[4], d3r[4], wr[4];
 #pragma unroll
   for (int q = 0; q < 4; q++) {
-    d2r[q] = D2[lv(1 + q)];
-    d3r[q] = D3[lv(1 + q)];
-    wr[q] = ww[lv(1 + q)];
+    d2r[q] = D2[lv(K0 + q)];
+    d3r[q] = D3[lv(K0 + q)];
+    wr[q] = ww[lv(K0 + q)];
   }
   // The flux at K-1/2 that level K reads is the flux at K+1/2 of level K-1,
   // so each face is evaluated once and carried up (pc).  The operands agree
@@ -542,16 +548,17 @@ __global__ __launch_bounds__(256) void k_sub_w(Dims d, int ntr, long ts, const T
   // upwind-upwind level of a downward face with the limiter off: not read.
   TF pc[G];
   {
+    const int F = K0 - 1;  // the face below level K0
     WFace<EX> wf0;
-    wface_setup<SCH>(wf0, ww[c2], dt, D2[c2], d2r[0]);
-    const bool lim_up = SCH != UPSTREAM && 0 > 1 - kshift;
-    const bool lim_dn = SCH != UPSTREAM && 0 < kmax - 1;
+    wface_setup<SCH>(wf0, ww[c2 + NXY * F], dt, D2[c2 + NXY * F], d2r[0]);
+    const bool lim_up = SCH != UPSTREAM && F > 1 - kshift;
+    const bool lim_dn = SCH != UPSTREAM && F < kmax - 1;
 #pragma unroll
     for (int g = 0; g < G; g++)
-      pc[g] = w_flux<TF, SCH>(wf0, 0, kshift, kmax, lim_up, lim_dn, TF(0), n[g][1], n[g][2],
-                              2 <= kmax ? n[g][3] : n[g][2]);
+      pc[g] = w_flux<TF, SCH>(wf0, F, kshift, kmax, lim_up, lim_dn, fKm2[g], n[g][1], n[g][2],
+                              K0 + 1 <= kmax ? n[g][3] : n[g][2]);
   }
-  for (int K = 1; K <= kmax; K++) {
+  for (int K = K0; K <= K1; K++) {
     const long cK = c2 + NXY * K;
     const bool wk_on = K <= kmax - 1, wp_on = K > 1 || kshift, kdo = K <= kmax - kshift;
     const bool limk_up = SCH != UPSTREAM && K > 1 - kshift;
@@ -766,21 +773,26 @@ __global__ __launch_bounds__(256) void k_sub_w_adj(Dims d, int ntr, long ts,
                                                    const double* __restrict__ D3,
                                                    const double* __restrict__ ww,
                                                    const uint8_t* __restrict__ flags, double dt,
-                                                   int kshift) {
+                                                   int kshift, int ngroups, int kc) {
   constexpr bool EX = kExact<TF>;
   constexpr int G = kGroup<TF>;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int t0 = blockIdx.z * G;
-  if (i >= d.NX || j >= d.NY || t0 >= ntr) return;
+  const int t0 = (blockIdx.z % ngroups) * G;
   const int kmax = d.kmax;
+  // this thread stores levels L0..L1 (chunk blockIdx.z / ngroups of kc
+  // levels); it marches the faces L0-2..L1+1 that reach them, in the order
+  // of the whole column, so every stored sum is formed in the same order
+  const int L0 = (int)(blockIdx.z / ngroups) * kc, L1 = min(L0 + kc - 1, kmax);
+  if (i >= d.NX || j >= d.NY || t0 >= ntr || L0 > kmax) return;
+  const int Fs = max(L0 - 2, 0), Fe = min(L1 + 1, kmax - 1);
   const long NXY = d.NXY, c2 = i + (long)d.NX * j;
   long tb[G];
 #pragma unroll
   for (int g = 0; g < G; g++) tb[g] = (long)min(t0 + g, ntr - 1) * ts;
   const bool wcol = (flags[c2] & MB_WCOL) && i <= d.NX - 2 && j <= d.NY - 2;
   if (!wcol) {
-    for (int K = 0; K <= kmax; K++)
+    for (int K = L0; K <= L1; K++)
 #pragma unroll
       for (int g = 0; g < G; g++)
         if (t0 + g < ntr) lam_out[tb[g] + c2 + NXY * K] = lam_in[tb[g] + c2 + NXY * K];
@@ -819,28 +831,34 @@ __global__ __launch_bounds__(256) void k_sub_w_adj(Dims d, int ntr, long ts,
       w[g][q].mu = kdo ? cw.mu(l, dt, 1.0) : TF(0);
     }
   };
-#pragma unroll
-  for (int g = 0; g < G; g++) w[g][0] = Lev{TF(0), TF(0), TF(0)};
+  // the window at face Fs holds levels Fs-1..Fs+2 (level -1 is zero)
   {
-    Raw r0, r1, r2;
-    fetch(0, r0);
-    fetch(1, r1);
-    fetch(2, r2);
-    enter(0, 1, r0);
-    enter(1, 2, r1);
-    enter(2, 3, r2);
+    Raw r0, r1, r2, r3;
+    fetch(max(Fs - 1, 0), r0);
+    fetch(Fs, r1);
+    fetch(Fs + 1, r2);
+    fetch(Fs + 2, r3);
+    if (Fs >= 1) {
+      enter(Fs - 1, 0, r0);
+    } else {
+#pragma unroll
+      for (int g = 0; g < G; g++) w[g][0] = Lev{TF(0), TF(0), TF(0)};
+    }
+    enter(Fs, 1, r1);
+    enter(Fs + 1, 2, r2);
+    enter(Fs + 2, 3, r3);
   }
   Raw rq0, rq1;  // levels K'+3, K'+4
-  fetch(3, rq0);
-  fetch(4, rq1);
+  fetch(Fs + 3, rq0);
+  fetch(Fs + 4, rq1);
   // D2 at K', K'+1, K'+2 and ww at K', K'+1, K'+2, loaded two faces ahead
   double dq[3], wq[3];
 #pragma unroll
   for (int q = 0; q < 3; q++) {
-    dq[q] = D2[lv(q)];
-    wq[q] = ww[lv(q)];
+    dq[q] = D2[lv(Fs + q)];
+    wq[q] = ww[lv(Fs + q)];
   }
-  for (int Kf = 0; Kf <= kmax - 1; Kf++) {
+  for (int Kf = Fs; Kf <= Fe; Kf++) {
     // level 0 is never updated (mu = 0), so at K' = 0 Fbar is -mu(1) with
     // kshift and 0 without
     const bool used = Kf >= 1 || kshift;
@@ -859,7 +877,7 @@ __global__ __launch_bounds__(256) void k_sub_w_adj(Dims d, int ntr, long ts,
 #pragma unroll
       for (int q = 0; q < 4; q++) w[g][q].acc += Fbar * gr[q];
     }
-    if (Kf >= 1) {
+    if (Kf - 1 >= L0 && Kf - 1 <= L1) {
 #pragma unroll
       for (int g = 0; g < G; g++)
         if (t0 + g < ntr) lam_out[tb[g] + c2 + NXY * (Kf - 1)] = w[g][0].acc;
@@ -881,11 +899,14 @@ __global__ __launch_bounds__(256) void k_sub_w_adj(Dims d, int ntr, long ts,
     dq[2] = D2[lv(Kf + 3)];
     wq[2] = ww[lv(Kf + 3)];
   }
+  // after the last face, levels kmax-1 and kmax are complete
+  if (Fe == kmax - 1) {
 #pragma unroll
-  for (int g = 0; g < G; g++) {
-    if (t0 + g < ntr) {
-      lam_out[tb[g] + c2 + NXY * (kmax - 1)] = w[g][0].acc;
-      lam_out[tb[g] + c2 + NXY * kmax] = w[g][1].acc;
+    for (int g = 0; g < G; g++) {
+      if (t0 + g < ntr) {
+        if (kmax - 1 >= L0 && kmax - 1 <= L1) lam_out[tb[g] + c2 + NXY * (kmax - 1)] = w[g][0].acc;
+        if (L1 == kmax) lam_out[tb[g] + c2 + NXY * kmax] = w[g][1].acc;
+      }
     }
   }
 }
@@ -957,14 +978,23 @@ template <typename TF, bool ADJ, int SCH>
 static void launch_w_one(hipStream_t st, Dims d, int ntr, long ts, const TF* a, const TF* b,
                          TF* out, const double* D2, const double* D3, const double* ww,
                          const uint8_t* flags, double dt, int kshift) {
-  const dim3 g((d.NX + 63) / 64, (d.NY + 3) / 4, (ntr + kGroup<TF> - 1) / kGroup<TF>);
+  // A column march has one wavefront per 64 columns and tracer group; where
+  // that leaves the chip short of wavefronts (one FP64 tracer on a small
+  // grid) the columns are cut into up to four chunks of at least 8 levels.
+  const int ng = (ntr + kGroup<TF> - 1) / kGroup<TF>;
+  const long waves = (long)((d.NX + 63) / 64) * d.NY * ng;
+  int nch = 1;
+  while (nch < 4 && waves * nch < kWaveTarget && d.kmax / (nch + 1) >= 8) nch++;
+  const int nlev = ADJ ? d.kmax + 1 : d.kmax;  // levels split: 0..kmax, resp. 1..kmax
+  const int kc = (nlev + nch - 1) / nch;
+  const dim3 g((d.NX + 63) / 64, (d.NY + 3) / 4, ng * ((nlev + kc - 1) / kc));
   if constexpr (ADJ) {
     hipLaunchKernelGGL((k_sub_w_adj<TF, SCH>), g, dim3(64, 4), 0, st, d, ntr, ts, a, b, out, D2,
-                       D3, ww, flags, dt, kshift);
+                       D3, ww, flags, dt, kshift, ng, kc);
   } else {
     (void)b;
     hipLaunchKernelGGL((k_sub_w<TF, SCH>), g, dim3(64, 4), 0, st, d, ntr, ts, a, out, D2, D3, ww,
-                       flags, dt, kshift);
+                       flags, dt, kshift, ng, kc);
   }
 }
 

@@ -229,13 +229,16 @@ def test_internal_pressure_adjoint_vs_fd(device):
     assert n >= 10
 
 
+@pytest.mark.parametrize("kmax", [6, 24])
 @pytest.mark.parametrize("tag", [gt.H_TAG, gt.W_TAG])
-def test_substep_forward_and_checkpoints_bit_identical(device, tag):
+def test_substep_forward_and_checkpoints_bit_identical(device, tag, kmax):
     """The reverse mode recomputes each step's states with the per-sub-step
     kernels: their final state equals the strip kernels' (tuning fused = 0
     runs the whole forward sweep through them) and do_advection_3d's, bit for
-    bit, and the checkpoint spacing changes nothing"""
-    tr, d, ctx, fields = setup(device, imax=40, jmax=36, kmax=6, seed=4)
+    bit, and the checkpoint spacing changes nothing.  At kmax = 24 on this
+    small grid the vertical sub-step and its transpose run in three chunks of
+    levels per column."""
+    tr, d, ctx, fields = setup(device, imax=40, jmax=36, kmax=kmax, seed=4)
     uu, vv, ww, hu, hv, ho, hn = fields
     f0 = to_dev(tr.tracers["c"], device)
     rng = np.random.default_rng(2)
@@ -357,3 +360,37 @@ def test_smcw_adjoint_vs_central_differences(device):
     # the coastline stencils read land values (through the harmonic means,
     # whose derivative in the large land-side difference is small, ~(1e-4/3.7)^2)
     assert nonzero_land > 0
+
+
+@pytest.mark.parametrize("scheme", [gt.UPSTREAM, gt.P2])
+def test_column_chunks_dot_product(device, scheme):
+    """kmax = 24 on a small grid: the vertical sub-step and its transpose run
+    in three chunks of levels per column (each chunk's transpose marches the
+    faces that reach its levels).  For the linear schemes the dot-product
+    identity <M x, y> = <x, M^T y> over several steps holds to 1e-12 (FP64,
+    relative) and, FP32, to 1e-6 of |M x| |y| (the random fields make the sum
+    cancel to ~1e-4 of that scale, below which FP32 rounding is not
+    resolved; a term lost at a chunk boundary is O(1)), with and without
+    horizontal diffusion, for both tags."""
+    tr, d, ctx, fields = setup(device, imax=30, jmax=28, kmax=24, seed=8)
+    uu, vv, ww, hu, hv, ho, hn = fields
+    rng = np.random.default_rng(scheme)
+    x = torch.from_numpy(rng.standard_normal(d.shape3)).to(device)
+    y = torch.from_numpy(rng.standard_normal(d.shape3)).to(device)
+    for tag in (gt.H_TAG, gt.W_TAG):
+        for AH in (-1.0, 200.0):
+            lam0, Mx = torch.empty_like(x), torch.empty_like(x)
+            ctx.adv3d_adjoint(5, 2, tr.dt, x, y, lam0, uu, vv, ww, hu, hv, ho, scheme, scheme,
+                              AH, tag, f_final=Mx)
+            x32 = x.float()[None].contiguous()
+            y32 = y.float()[None].contiguous()
+            l32, M32 = torch.empty_like(x32), torch.empty_like(x32)
+            ctx.adv3d_adjoint_f32(1, 5, 2, tr.dt, x32, y32, l32, uu, vv, ww, hu, hv, ho, scheme,
+                                  scheme, AH, tag, f_final=M32)
+            torch.cuda.synchronize()
+            lhs, rhs = float(torch.sum(Mx * y)), float(torch.sum(x * lam0))
+            assert abs(lhs - rhs) <= 1e-12 * abs(rhs), (tag, AH, "FP64", lhs, rhs)
+            lhs = float(torch.sum(M32.double() * y32.double()))
+            rhs = float(torch.sum(x32.double() * l32.double()))
+            scale = float(M32.double().norm() * y32.double().norm())
+            assert abs(lhs - rhs) <= 1e-6 * scale, (tag, AH, "FP32", lhs, rhs, scale)
