@@ -632,6 +632,7 @@ def adjoint_section(torch, np, gt, inputs, dev, nsteps=48, ckpt=8):
     d = tr.dom
     st = torch.cuda.Stream(device=dev)
     ctx = gt.Context.from_domain(d, stream=st)
+    ctx.set_adjoint_mode(1)  # each segment's sub-step states kept while it is recomputed
     D = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)  # noqa: E731
     uu, vv, ww, hu, hv, ho, hn = (D(x) for x in (tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho,
                                                  tr.hn))
@@ -684,8 +685,9 @@ def adjoint_section(torch, np, gt, inputs, dev, nsteps=48, ckpt=8):
             "ms_per_gradient": t * 1e3,
             "gradient_cell_steps_per_s": ncell * nsteps * 2 / t,
             "note": "per tracer: 48 forward steps for the cost, then the adjoint call (48 "
-                    "forward steps to the checkpoints, 40 recomputed from them, 48 reverse "
-                    "steps of four sub-steps and five transposes)",
+                    "forward steps to the checkpoints, 48 recomputed from them through the "
+                    "sub-step kernels keeping each segment's sub-step states, 48 reverse "
+                    "steps of five transposes)",
             "grad_norm_c": float(g_c.norm()), "grad_norm_T": float(g_T.norm())}
 
 
@@ -729,9 +731,11 @@ def shard_section(torch, np, gt, inputs, dev, steps):
         m1 = [ctx.budget(hn, bufs[0][q])[0] for q in range(nt)]
         # the FP32 reverse mode over the next 24 macro steps of the tide (the
         # producers' outputs of every step kept on the device), a checkpoint
-        # every 6 steps: one untimed call that allocates the work fields, then
-        # one timed call
+        # every 6 steps, each segment's layer heights and sub-step states kept
+        # while it is recomputed: one untimed call that allocates the work
+        # fields, then one timed call
         nadj, ck = 24, 6
+        ctx.set_adjoint_mode(1)
         seq = []
         for _ in range(nadj):
             dd_t, _ = tide.macro_step()
@@ -757,7 +761,9 @@ def shard_section(torch, np, gt, inputs, dev, steps):
         del lamT, lam0, ffin
     tr_ms = t_rev.mean() * 1e3
     n3 = (d.kmax + 1) * (d.jmax + 4) * (d.imax + 4)
-    work_gb = (5 * n3 * 8 + (6 + nadj // ck + ck - 1) * nt * n3 * 4) / 1e9
+    # heights D1..D5 per step of a segment; tracer sets: 4 sub-step states per
+    # step of a segment, lam x2, the checkpoints, the states inside a segment
+    work_gb = (5 * n3 * 8 * ck + (4 * ck + 2 + nadj // ck + ck - 1) * nt * n3 * 4) / 1e9
     ta, tp = t_adv.mean(), t_prod.mean()
     ncell = d.imax * d.jmax * d.kmax
     b = ncell * (nt * 2 * 4 + 7 * 8)

@@ -63,6 +63,7 @@ struct gt_ctx {
   int adv_ver_iterations = 1;
   int do_mirror_2d = 1;
   int fused = 1;        // use the HALFSPLIT strip kernels where they apply
+  int adj_keep = -1;    // reverse mode keeps each segment's sub-step states (-1: if they fit)
   int tile = 0;         // rows per strip of the two strip kernels (gt::strip_rows)
   int group = 3;        // tracers per wavefront of the FP32 batched kernels
   void* ws = nullptr;   // work fields of the strip kernels (grown on demand)
@@ -1099,6 +1100,14 @@ extern "C" int gt_set_tuning(gt_ctx* c, int fused, int tile, int group) {
   return GT_OK;
 }
 
+extern "C" int gt_set_adjoint_mode(gt_ctx* c, int keep_states) {
+  if (!c) return set_err(GT_EARG, "gt_set_adjoint_mode: NULL context");
+  if (keep_states < -1 || keep_states > 1)
+    return set_err(GT_EARG, "gt_set_adjoint_mode: keep_states must be -1, 0 or 1");
+  c->adj_keep = keep_states;
+  return GT_OK;
+}
+
 // ---- the FP32 step one sub-step at a time ---------------------------------------
 static int ensure_w32(gt_ctx* c, size_t bytes) {
   if (c->w32_bytes >= bytes) return GT_OK;
@@ -1306,17 +1315,20 @@ extern "C" int gt_autotune_batch(int ndom, const gt_batch_domain* doms, int ntra
 
 // ---- reverse mode (extension) ------------------------------------------------
 // transpose of one HALFSPLIT step from s0: the states before the last four
-// sub-steps are recomputed into S[0..3]; lam[0] (adjoint of the step's
-// output) is replaced by the adjoint of s0, lam[1] is work
+// sub-steps are in S[0..3] (have_states) or are recomputed there; lam[0]
+// (adjoint of the step's output) is replaced by the adjoint of s0, lam[1] is
+// work
 template <typename TF>
 static int reverse_step(gt_ctx* c, const GridMasks* g, const double* const D[6], double dt,
                         int ntr, const TF* s0, TF* const S[4], TF** lam, const double* uu,
                         const double* vv, const double* ww, const double* hu, const double* hv,
-                        int hs, int vs, double AH, int kshift) {
+                        int hs, int vs, double AH, int kshift, bool have_states) {
   hipStream_t st = c->st;
   const Dims d = c->d;
   const long ts = c->N3;
-  TRY(substeps<TF>(c, st, g, D, dt, ntr, s0, S, nullptr, uu, vv, ww, hu, hv, hs, vs, AH, kshift));
+  if (!have_states)
+    TRY(substeps<TF>(c, st, g, D, dt, ntr, s0, S, nullptr, uu, vv, ww, hu, hv, hs, vs, AH,
+                     kshift));
   TF* a = lam[0];
   TF* b = lam[1];
   LCHK(gt::launch_sub_h_adj<TF>(st, d, 0, ntr, ts, S[3], a, b, D[4], D[5], uu, hu, g->flags, dt,
@@ -1377,46 +1389,73 @@ static int adjoint_impl(const char* name, gt_ctx* c, int ntr, int nsteps, int ck
   const size_t set = (size_t)ntr * N3;
   const int nck = (nsteps + ckpt_every - 1) / ckpt_every;
   const int seg = ckpt_every;
-  // D1..D5, then tracer sets: S[4] lam[2] ck[nck] seg-1 states of a segment
-  const size_t nsets = 4 + 2 + (size_t)nck + (size_t)(seg - 1);
-  TRY(ensure_w32(c, sizeof(double) * 5 * (size_t)N3 + sizeof(TF) * nsets * set));
-  double* Dh[5];
-  for (int n = 0; n < 5; n++) Dh[n] = (double*)c->w32 + n * N3;
-  TF* base = (TF*)((double*)c->w32 + 5 * N3);
-  TF* const S[4] = {base, base + set, base + 2 * set, base + 3 * set};
-  TF* lam[2] = {base + 4 * set, base + 5 * set};
-  TF* ck = base + 6 * set;
+  // Work memory: the layer heights D1..D5 (one slot, or one per step of a
+  // segment when the states are kept and the inputs change), then tracer
+  // sets: the sub-step states (4, or 4 per step of a segment when kept),
+  // lam[2], the checkpoints and the seg-1 states inside a segment.  Keeping
+  // the states lets the reverse steps skip the four sub-steps that would
+  // recover them; it is taken when asked for, or (default) when it needs at
+  // most half of the free device memory.
+  const size_t rest = 2 + (size_t)nck + (size_t)(seg - 1);
+  auto bytes_for = [&](bool k) -> size_t {
+    const size_t hs = k && in.per_step ? (size_t)seg : 1;
+    return sizeof(double) * 5 * (size_t)N3 * hs + sizeof(TF) * ((k ? 4 * (size_t)seg : 4) + rest) * set;
+  };
+  bool keep = c->adj_keep == 1;
+  if (c->adj_keep < 0) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess) keep = bytes_for(true) <= (fr + c->w32_bytes) / 2;
+  }
+  TRY(ensure_w32(c, bytes_for(keep)));
+  const int hslots = keep && in.per_step ? seg : 1;
+  double* hbase = (double*)c->w32;
+  TF* base = (TF*)(hbase + 5 * (size_t)N3 * hslots);
+  const int nS = keep ? 4 * seg : 4;
+  TF* lam[2] = {base + (size_t)nS * set, base + (size_t)(nS + 1) * set};
+  TF* ck = base + (size_t)(nS + 2) * set;
   TF* sg = ck + (size_t)nck * set;
-  const double* D[6] = {nullptr, Dh[0], Dh[1], Dh[2], Dh[3], Dh[4]};
-  int d_step = -1;  // the step whose layer heights Dh holds
-  auto heights = [&](int t) -> int {
+  auto sub_states = [&](int n, TF* S[4]) {  // the 4 sub-step states of set group n
+    for (int q = 0; q < 4; q++) S[q] = base + (size_t)(4 * n + q) * set;
+  };
+  std::vector<int> slot_key(hslots, -1);  // the step whose heights a slot holds
+  auto heights = [&](int t, int slot, const double* D[6]) -> int {
+    double* Hs = hbase + 5 * (size_t)N3 * slot;
+    double* Dh[5];
+    for (int n = 0; n < 5; n++) Dh[n] = Hs + n * N3;
     const int key = in.per_step ? t : 0;
-    if (d_step == key) return GT_OK;
-    LCHK(gt::launch_heights(c->st, c->d, g->flags, in.get(5, t), in.get(0, t), in.get(1, t),
-                            in.get(2, t), Dh, dt, c->dx, c->dy, c->ard1, hscheme, kshift));
+    if (slot_key[slot] != key) {
+      LCHK(gt::launch_heights(c->st, c->d, g->flags, in.get(5, t), in.get(0, t), in.get(1, t),
+                              in.get(2, t), Dh, dt, c->dx, c->dy, c->ard1, hscheme, kshift));
+      slot_key[slot] = key;
+    }
     D[0] = in.get(5, t);
-    d_step = key;
+    for (int n = 0; n < 5; n++) D[n + 1] = Dh[n];
     return GT_OK;
   };
   const bool plain = c->fused && gt::fused_supported(hscheme, vscheme) && strip_fits(c);
-  auto step = [&](int t, const TF* src, TF* dst) -> int {
-    if (plain) {
+  // one step from src to dst; S: where the sub-step kernels leave the states
+  // (the strip kernels where they apply and the states are not wanted)
+  auto step = [&](int t, int slot, const TF* src, TF* dst, TF* const S[4], bool want) -> int {
+    if (plain && !want) {
       gt_batch_domain bd{c, src, dst, in.get(0, t), in.get(1, t), in.get(2, t), in.get(3, t),
                          in.get(4, t), in.get(5, t), nullptr};
       return batch_impl<TF>(1, &bd, ntr, dt, hscheme, vscheme, AH, tag, c->st);
     }
-    TRY(heights(t));
-    TF* const s[4] = {S[0], S[1], S[0], S[1]};
+    const double* D[6];
+    TRY(heights(t, slot, D));
+    TF* const s[4] = {S[0], S[1], want ? S[2] : S[0], want ? S[3] : S[1]};
     return substeps<TF>(c, c->st, g, D, dt, ntr, src, s, dst, in.get(0, t), in.get(1, t),
                         in.get(2, t), in.get(3, t), in.get(4, t), hscheme, vscheme, AH, kshift);
   };
+  TF* S0[4];
+  sub_states(0, S0);
   // forward sweep (lam[0], lam[1] as the two states)
   TF* a = lam[0];
   TF* b = lam[1];
   TRY(copy_set<TF>(c->st, a, f0, set));
   for (int t = 0; t < nsteps; t++) {
     if (t % ckpt_every == 0) TRY(copy_set<TF>(c->st, ck + (size_t)(t / ckpt_every) * set, a, set));
-    TRY(step(t, a, b));
+    TRY(step(t, 0, a, b, S0, false));
     TF* x = a; a = b; b = x;
   }
   if (f_final) TRY(copy_set<TF>(c->st, f_final, a, set));
@@ -1427,12 +1466,26 @@ static int adjoint_impl(const char* name, gt_ctx* c, int ntr, int nsteps, int ck
     auto state = [&](int t) -> TF* {
       return t == t0 ? ck + (size_t)s * set : sg + (size_t)(t - t0 - 1) * set;
     };
-    for (int t = t0; t < t1 - 1; t++) TRY(step(t, state(t), state(t + 1)));
+    auto slot = [&](int t) -> int { return hslots > 1 ? t - t0 : 0; };
+    if (keep) {
+      // recompute the segment through the sub-step kernels, keeping every
+      // step's four states (the last step's output is not needed)
+      for (int t = t0; t < t1; t++) {
+        TF* St[4];
+        sub_states(t - t0, St);
+        TRY(step(t, slot(t), state(t), t < t1 - 1 ? state(t + 1) : nullptr, St, true));
+      }
+    } else {
+      for (int t = t0; t < t1 - 1; t++) TRY(step(t, 0, state(t), state(t + 1), S0, false));
+    }
     for (int t = t1 - 1; t >= t0; t--) {
-      TRY(heights(t));
-      TRY(reverse_step<TF>(c, g, D, dt, ntr, state(t), S, lam, in.get(0, t), in.get(1, t),
+      const double* D[6];
+      TRY(heights(t, slot(t), D));
+      TF* St[4];
+      sub_states(keep ? t - t0 : 0, St);
+      TRY(reverse_step<TF>(c, g, D, dt, ntr, state(t), St, lam, in.get(0, t), in.get(1, t),
                            in.get(2, t), in.get(3, t), in.get(4, t), hscheme, vscheme, AH,
-                           kshift));
+                           kshift, keep));
     }
   }
   return copy_set<TF>(c->st, lam0, lam[0], set);

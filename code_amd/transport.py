@@ -40,7 +40,8 @@ EXPORTS = [
     "gt_update_3d_halo", "gt_update_2d_halo", "gt_wait_halo",
     "gt_exchange_tiles", "gt_budget", "gt_malloc", "gt_free",
     "gt_memcpy_h2d", "gt_memcpy_d2h", "gt_adv3d_batch_f32", "gt_adv3d_batch_f64",
-    "gt_set_tuning", "gt_budget_f32", "gt_adv3d_adjoint", "gt_adv3d_adjoint_f32",
+    "gt_set_tuning", "gt_set_adjoint_mode", "gt_budget_f32", "gt_adv3d_adjoint",
+    "gt_adv3d_adjoint_f32",
     "gt_adv3d_adjoint_seq", "gt_adv3d_adjoint_seq_f32", "gt_ip_shchepetkin_mcwilliams_adjoint",
     "gt_ip_blumberg_mellor_adjoint", "gt_eos_linear_adjoint", "gt_autotune_batch",
     "gt_do_eqstate", "gt_rho_from_theta_unesco80", "gt_rho_from_theta", "gt_eosall_from_theta",
@@ -137,6 +138,7 @@ def load_library():
         "gt_adv3d_batch_f32": (i, [i, C.POINTER(BatchDomain), i, d, i, i, d, i, vp]),
         "gt_adv3d_batch_f64": (i, [i, C.POINTER(BatchDomain), i, d, i, i, d, i, vp]),
         "gt_set_tuning": (i, [vp, i, i, i]),
+        "gt_set_adjoint_mode": (i, [vp, i]),
         "gt_budget_f32": (i, [vp, vp, vp, C.POINTER(d), C.POINTER(d)]),
         "gt_adv3d_adjoint": (i, [vp, i, i, d, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i, i,
                                  d, i]),
@@ -480,6 +482,11 @@ class Context:
 
     def set_tuning(self, fused=1, tile=0, group=3):
         check(self.lib.gt_set_tuning(self.h, int(fused), int(tile), int(group)))
+
+    def set_adjoint_mode(self, keep_states=-1):
+        """gt_set_adjoint_mode: 1 keeps each segment's sub-step states, 0
+        recomputes them per reverse step, -1 decides by free memory"""
+        check(self.lib.gt_set_adjoint_mode(self.h, int(keep_states)))
 
 
 def exchange_tiles(px, py, ni, nj, nk, tiles, stream=None):

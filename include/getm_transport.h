@@ -336,6 +336,15 @@ int gt_adv3d_batch_f64(int ndom, const gt_batch_domain *doms, int ntracers,
  * per wavefront (group: FP32 1, 3, 6 or 9; FP64 1, or 2 for any other value
  * when there are two or more tracers) */
 int gt_set_tuning(gt_ctx *ctx, int fused, int tile, int group);
+/* reverse mode (gt_adv3d_adjoint*): keep_states 1 keeps the four sub-step
+ * states of every step of a checkpoint segment while the segment is
+ * recomputed, so the reverse steps do not recompute them (work memory
+ * 4*ckpt_every tracer sets instead of 4, and, with per-step inputs,
+ * ckpt_every sets of layer heights instead of one); 0 recomputes them per
+ * reverse step; -1 (default) keeps them when that needs at most half of the
+ * free device memory.  Results are the same bit for bit.  No counterpart in
+ * the reference (its reverse mode is this build's extension). */
+int gt_set_adjoint_mode(gt_ctx *ctx, int keep_states);
 /* autotuner: times gt_adv3d_batch_f32 (f32=1) or _f64 over the nine strip
  * height pairs (tile 3..11) x tracers per wavefront {1, 3, 6, 9} (FP64:
  * {1, 2}), reps timed launches each, keeps the fastest on every context.

@@ -244,20 +244,25 @@ def test_substep_forward_and_checkpoints_bit_identical(device, tag, kmax):
     rng = np.random.default_rng(2)
     lamT = torch.from_numpy(rng.standard_normal(d.shape3)).to(device)
 
-    def run(ck, fused=1, AH=-1.0):
+    def run(ck, fused=1, AH=-1.0, keep=0):
         ctx.set_tuning(fused=fused)
+        ctx.set_adjoint_mode(keep)
         lam0, ff = torch.empty_like(f0), torch.empty_like(f0)
         ctx.adv3d_adjoint(7, ck, tr.dt, f0, lamT, lam0, uu, vv, ww, hu, hv, ho, gt.P2_PDM,
                           gt.P2_PDM, AH, tag, f_final=ff)
         torch.cuda.synchronize()
         ctx.set_tuning(fused=1)
+        ctx.set_adjoint_mode(-1)
         return lam0, ff
 
+    # keep: the reverse steps read the sub-step states kept while each
+    # segment was recomputed (1) or recompute them (0)
     for AH in (-1.0, 150.0):
         ref, ffr = run(3, AH=AH)
-        for ck, fused in ((1, 1), (7, 1), (3, 0)):
-            lam, ff = run(ck, fused, AH)
-            assert torch.equal(lam, ref) and torch.equal(ff, ffr), (ck, fused, AH)
+        for ck, fused, keep in ((1, 1, 0), (7, 1, 0), (3, 0, 0), (3, 1, 1), (3, 0, 1), (7, 1, 1),
+                                (1, 1, 1)):
+            lam, ff = run(ck, fused, AH, keep)
+            assert torch.equal(lam, ref) and torch.equal(ff, ffr), (ck, fused, keep, AH)
         x = f0.clone()
         for _ in range(7):
             ctx.do_advection_3d(tr.dt, x, uu, vv, ww, hu, hv, ho, ho, gt.HALFSPLIT, gt.P2_PDM,

@@ -62,22 +62,58 @@ def test_same_inputs_equal_fixed_entry_points(device):
                            .astype(np.float32)).to(device)
     l32 = torch.from_numpy(rng.standard_normal((2,) + d.shape3).astype(np.float32)).to(device)
     n = 7
-    for ck in (1, 3):
+    # keep: the per-step forms keep every step's layer heights and sub-step
+    # states of a segment (1) or recompute them per reverse step (0)
+    for ck, keep in ((1, 0), (3, 0), (3, 1), (7, 1)):
+        ctx.set_adjoint_mode(-1)
         a, fa = torch.empty_like(c64), torch.empty_like(c64)
         ctx.adv3d_adjoint(n, ck, tr.dt, c64, l64, a, *vel, gt.P2_PDM, gt.P2_PDM, f_final=fa)
+        ctx.set_adjoint_mode(keep)
         b, fb = torch.empty_like(c64), torch.empty_like(c64)
         ctx.adv3d_adjoint_seq(1, ck, tr.dt, c64, l64, b, [tuple(vel)] * n, gt.P2_PDM, gt.P2_PDM,
                               f_final=fb)
         torch.cuda.synchronize()
-        assert bits(a, b) and bits(fa, fb), ("FP64", ck)
+        assert bits(a, b) and bits(fa, fb), ("FP64", ck, keep)
+        ctx.set_adjoint_mode(-1)
         a, fa = torch.empty_like(f32), torch.empty_like(f32)
         ctx.adv3d_adjoint_f32(2, n, ck, tr.dt, f32, l32, a, *vel, gt.P2_PDM, gt.P2_PDM,
                               f_final=fa)
+        ctx.set_adjoint_mode(keep)
         b, fb = torch.empty_like(f32), torch.empty_like(f32)
         ctx.adv3d_adjoint_seq(2, ck, tr.dt, f32, l32, b, [tuple(vel)] * n, gt.P2_PDM, gt.P2_PDM,
                               f_final=fb)
         torch.cuda.synchronize()
-        assert bits(a, b) and bits(fa, fb), ("FP32", ck)
+        assert bits(a, b) and bits(fa, fb), ("FP32", ck, keep)
+    ctx.set_adjoint_mode(-1)
+
+
+def test_keep_states_with_changing_inputs(device):
+    """over steps whose inputs change, keeping each step's layer heights and
+    sub-step states gives the recomputing mode's result bit for bit (FP64
+    and FP32, limited scheme, checkpoint spacings 3 and 8)"""
+    tr, ctx, vel = basin(device)
+    d = tr.dom
+    steps = varying(vel, 8)
+    rng = np.random.default_rng(11)
+    f64 = to_dev(tr.tracers["c"], device)
+    l64 = torch.from_numpy(rng.standard_normal(d.shape3)).to(device)
+    f32 = f64.float()[None].contiguous()
+    l32 = l64.float()[None].contiguous()
+    for ck in (3, 8):
+        out = {}
+        for keep in (0, 1):
+            ctx.set_adjoint_mode(keep)
+            a, fa = torch.empty_like(f64), torch.empty_like(f64)
+            ctx.adv3d_adjoint_seq(1, ck, tr.dt, f64, l64, a, steps, gt.MUSCL, gt.MUSCL,
+                                  f_final=fa)
+            b, fb = torch.empty_like(f32), torch.empty_like(f32)
+            ctx.adv3d_adjoint_seq(1, ck, tr.dt, f32, l32, b, steps, gt.MUSCL, gt.MUSCL,
+                                  f_final=fb)
+            torch.cuda.synchronize()
+            out[keep] = (a, fa, b, fb)
+        for x, y in zip(out[0], out[1]):
+            assert bits(x, y), ck
+    ctx.set_adjoint_mode(-1)
 
 
 @pytest.mark.parametrize("scheme", [gt.UPSTREAM, gt.P2])
