@@ -362,10 +362,12 @@ int gt_autotune_batch(int ndom, const gt_batch_domain *doms, int ntracers,
  * optional), keeping a checkpoint every ckpt_every steps, then sweeps back:
  * lam0 = (d f_final / d f0)^T lam_final.  The limiter enters through the
  * derivative of the branch the forward took.  The reverse sweep recomputes
- * each segment from its checkpoint (strip kernels), then the states before
- * each sub-step (per-sub-step kernels, bit-identical to the strip kernels),
- * and applies the transpose of each sub-step.  Work memory: 5 + 6 +
- * ceil(nsteps/ckpt_every) + ckpt_every - 1 FP64 fields. */
+ * each segment from its checkpoint and applies the transpose of each
+ * sub-step; the states before the sub-steps come from the per-sub-step
+ * kernels (bit-identical to the strip kernels), kept for the whole segment
+ * or recomputed per reverse step (gt_set_adjoint_mode).  Work memory in FP64
+ * fields: 5 + 2 + ceil(nsteps/ckpt_every) + ckpt_every - 1, plus 4 (states
+ * recomputed) or 4*ckpt_every (states kept). */
 int gt_adv3d_adjoint(gt_ctx *ctx, int nsteps, int ckpt_every, double dt,
                      const double *f0, double *f_final, const double *lam_final,
                      double *lam0, const double *uu, const double *vv,
@@ -378,9 +380,10 @@ int gt_adv3d_adjoint(gt_ctx *ctx, int nsteps, int ckpt_every, double dt,
  * The forward sweep runs the strip kernels (fused = 1 and hscheme == vscheme)
  * or the per-sub-step kernels, which give the same states bit for bit; the
  * reverse sweep recomputes each step's sub-step states from the checkpoint
- * and applies the transpose of each FP32 sub-step, the limiter through the
- * derivative of the branch term the FP32 forward selected.  Work memory:
- * 5 FP64 fields + (6 + ceil(nsteps/ckpt_every) + ckpt_every - 1) tracer
+ * (kept per segment or per reverse step, gt_set_adjoint_mode) and applies
+ * the transpose of each FP32 sub-step, the limiter through the derivative of
+ * the branch term the FP32 forward selected.  Work memory: 5 FP64 fields +
+ * (2 + ceil(nsteps/ckpt_every) + ckpt_every - 1 + 4 or 4*ckpt_every) tracer
  * sets of ntracers FP32 fields. */
 int gt_adv3d_adjoint_f32(gt_ctx *ctx, int ntracers, int nsteps, int ckpt_every,
                          double dt, const float *f0, float *f_final,
@@ -391,8 +394,10 @@ int gt_adv3d_adjoint_f32(gt_ctx *ctx, int ntracers, int nsteps, int ckpt_every,
 /* the two above with inputs that change from step to step (a tide, the
  * producers of each macro step): uu[t], vv[t], ww[t], hu[t], hv[t], ho[t]
  * (host arrays of nsteps device pointers) are the inputs of step t; the layer
- * heights of the sub-steps are formed per step.  With the same pointers for
- * every step the result equals gt_adv3d_adjoint / _f32 bit for bit. */
+ * heights of the sub-steps are formed per step (with the states kept, once
+ * per step of a segment: ckpt_every sets of 5 FP64 fields).  With the same
+ * pointers for every step the result equals gt_adv3d_adjoint / _f32 bit for
+ * bit. */
 int gt_adv3d_adjoint_seq(gt_ctx *ctx, int nsteps, int ckpt_every, double dt,
                          const double *f0, double *f_final,
                          const double *lam_final, double *lam0,
