@@ -685,9 +685,9 @@ def adjoint_section(torch, np, gt, inputs, dev, nsteps=48, ckpt=8):
             "ms_per_gradient": t * 1e3,
             "gradient_cell_steps_per_s": ncell * nsteps * 2 / t,
             "note": "per tracer: 48 forward steps for the cost, then the adjoint call (48 "
-                    "forward steps to the checkpoints, 48 recomputed from them through the "
-                    "sub-step kernels keeping each segment's sub-step states, 48 reverse "
-                    "steps of five transposes)",
+                    "forward steps to the checkpoints, the last 8 through the sub-step "
+                    "kernels keeping their sub-step states, the other 40 recomputed so from "
+                    "the checkpoints, 48 reverse steps of five transposes)",
             "grad_norm_c": float(g_c.norm()), "grad_norm_T": float(g_T.norm())}
 
 

@@ -1453,8 +1453,23 @@ static int adjoint_impl(const char* name, gt_ctx* c, int ntr, int nsteps, int ck
   TF* a = lam[0];
   TF* b = lam[1];
   TRY(copy_set<TF>(c->st, a, f0, set));
+  // With the states kept, the last segment keeps them already here (its
+  // states inside the segment go where its recompute would put them), so
+  // the reverse sweep need not recompute it.
+  const int tl = (nck - 1) * ckpt_every;  // the last segment's first step
   for (int t = 0; t < nsteps; t++) {
     if (t % ckpt_every == 0) TRY(copy_set<TF>(c->st, ck + (size_t)(t / ckpt_every) * set, a, set));
+    if (keep && t >= tl) {
+      TF* St[4];
+      sub_states(t - tl, St);
+      const TF* src = t == tl ? a : sg + (size_t)(t - tl - 1) * set;
+      TF* dst = t < nsteps - 1 ? sg + (size_t)(t - tl) * set : b;
+      TRY(step(t, hslots > 1 ? t - tl : 0, src, dst, St, true));
+      if (t == nsteps - 1) {
+        TF* x = a; a = b; b = x;
+      }
+      continue;
+    }
     TRY(step(t, 0, a, b, S0, false));
     TF* x = a; a = b; b = x;
   }
@@ -1467,7 +1482,7 @@ static int adjoint_impl(const char* name, gt_ctx* c, int ntr, int nsteps, int ck
       return t == t0 ? ck + (size_t)s * set : sg + (size_t)(t - t0 - 1) * set;
     };
     auto slot = [&](int t) -> int { return hslots > 1 ? t - t0 : 0; };
-    if (keep) {
+    if (keep && s < nck - 1) {
       // recompute the segment through the sub-step kernels, keeping every
       // step's four states (the last step's output is not needed)
       for (int t = t0; t < t1; t++) {
@@ -1475,7 +1490,7 @@ static int adjoint_impl(const char* name, gt_ctx* c, int ntr, int nsteps, int ck
         sub_states(t - t0, St);
         TRY(step(t, slot(t), state(t), t < t1 - 1 ? state(t + 1) : nullptr, St, true));
       }
-    } else {
+    } else if (!keep) {
       for (int t = t0; t < t1 - 1; t++) TRY(step(t, 0, state(t), state(t + 1), S0, false));
     }
     for (int t = t1 - 1; t >= t0; t--) {
