@@ -483,7 +483,7 @@ __global__ __launch_bounds__(64) void k_sub_v(Dims d, int ntr, long ts, const TF
 }
 
 // ---- the vertical sub-step (adv_ver_iterations = 1) ------------------------------
-// One thread per column and group of tracers (blockIdx.z), marching up with
+// One thread per column, group of tracers and chunk of levels, marching up with
 // the levels K-1..K+2 of each tracer, K..K+3 of D2, D3 and ww and the flux
 // at K-1/2 in registers (kmax >= 2).  At level K, as the strip kernels set them
 // (adv_split_w.F90:249-300): the flux at K+1/2 exists below kmax, the one at
@@ -497,16 +497,20 @@ __global__ __launch_bounds__(256) void k_sub_w(Dims d, int ntr, long ts, const T
                                                const double* __restrict__ D3,
                                                const double* __restrict__ ww,
                                                const uint8_t* __restrict__ flags, double dt,
-                                               int kshift, int ngroups, int kc) {
+                                               int kshift, int ngroups, int nchunks, int kc) {
   constexpr bool EX = kExact<TF>;
   constexpr int G = kGroup<TF>;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // blockIdx.x = column block * nz + z, z = chunk * ngroups + group: the
+  // groups and chunks of one column block run together, so the FP64 fields
+  // they all read meet in the cache
+  const int nz = ngroups * nchunks, bz = (int)(blockIdx.x % nz);
+  const int i = (int)(blockIdx.x / nz) * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int t0 = (blockIdx.z % ngroups) * G;
+  const int t0 = (bz % ngroups) * G;
   const int kmax = d.kmax;
-  // this thread's levels K0..K1 (chunk blockIdx.z / ngroups of kc levels;
-  // the first chunk also carries level 0)
-  const int K0 = 1 + (int)(blockIdx.z / ngroups) * kc, K1 = min(K0 + kc - 1, kmax);
+  // this thread's levels K0..K1 (chunk bz / ngroups of kc levels; the first
+  // chunk also carries level 0)
+  const int K0 = 1 + (bz / ngroups) * kc, K1 = min(K0 + kc - 1, kmax);
   if (i >= d.NX || j >= d.NY || t0 >= ntr || K0 > kmax) return;
   const long NXY = d.NXY, c2 = i + (long)d.NX * j;
   long tb[G];
@@ -773,17 +777,21 @@ __global__ __launch_bounds__(256) void k_sub_w_adj(Dims d, int ntr, long ts,
                                                    const double* __restrict__ D3,
                                                    const double* __restrict__ ww,
                                                    const uint8_t* __restrict__ flags, double dt,
-                                                   int kshift, int ngroups, int kc) {
+                                                   int kshift, int ngroups, int nchunks, int kc) {
   constexpr bool EX = kExact<TF>;
   constexpr int G = kGroup<TF>;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // blockIdx.x = column block * nz + z, z = chunk * ngroups + group: the
+  // groups and chunks of one column block run together, so the FP64 fields
+  // they all read meet in the cache
+  const int nz = ngroups * nchunks, bz = (int)(blockIdx.x % nz);
+  const int i = (int)(blockIdx.x / nz) * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int t0 = (blockIdx.z % ngroups) * G;
+  const int t0 = (bz % ngroups) * G;
   const int kmax = d.kmax;
-  // this thread stores levels L0..L1 (chunk blockIdx.z / ngroups of kc
-  // levels); it marches the faces L0-2..L1+1 that reach them, in the order
-  // of the whole column, so every stored sum is formed in the same order
-  const int L0 = (int)(blockIdx.z / ngroups) * kc, L1 = min(L0 + kc - 1, kmax);
+  // this thread stores levels L0..L1 (chunk bz / ngroups of kc levels); it
+  // marches the faces L0-2..L1+1 that reach them, in the order of the whole
+  // column, so every stored sum is formed in the same order
+  const int L0 = (bz / ngroups) * kc, L1 = min(L0 + kc - 1, kmax);
   if (i >= d.NX || j >= d.NY || t0 >= ntr || L0 > kmax) return;
   const int Fs = max(L0 - 2, 0), Fe = min(L1 + 1, kmax - 1);
   const long NXY = d.NXY, c2 = i + (long)d.NX * j;
@@ -987,14 +995,15 @@ static void launch_w_one(hipStream_t st, Dims d, int ntr, long ts, const TF* a, 
   while (nch < 4 && waves * nch < kWaveTarget && d.kmax / (nch + 1) >= 8) nch++;
   const int nlev = ADJ ? d.kmax + 1 : d.kmax;  // levels split: 0..kmax, resp. 1..kmax
   const int kc = (nlev + nch - 1) / nch;
-  const dim3 g((d.NX + 63) / 64, (d.NY + 3) / 4, ng * ((nlev + kc - 1) / kc));
+  const int nchunks = (nlev + kc - 1) / kc;
+  const dim3 g((unsigned)(((d.NX + 63) / 64) * ng * nchunks), (d.NY + 3) / 4, 1);
   if constexpr (ADJ) {
     hipLaunchKernelGGL((k_sub_w_adj<TF, SCH>), g, dim3(64, 4), 0, st, d, ntr, ts, a, b, out, D2,
-                       D3, ww, flags, dt, kshift, ng, kc);
+                       D3, ww, flags, dt, kshift, ng, nchunks, kc);
   } else {
     (void)b;
     hipLaunchKernelGGL((k_sub_w<TF, SCH>), g, dim3(64, 4), 0, st, d, ntr, ts, a, out, D2, D3, ww,
-                       flags, dt, kshift, ng, kc);
+                       flags, dt, kshift, ng, nchunks, kc);
   }
 }
 
