@@ -1388,7 +1388,7 @@ static int adjoint_impl(const char* name, gt_ctx* c, int ntr, int nsteps, int ck
   const long N3 = c->N3;
   const size_t set = (size_t)ntr * N3;
   const int nck = (nsteps + ckpt_every - 1) / ckpt_every;
-  const int seg = ckpt_every;
+  const int seg = ckpt_every < nsteps ? ckpt_every : nsteps;  // steps in a segment, at most
   // Work memory: the layer heights D1..D5 (one slot, or one per step of a
   // segment when the states are kept and the inputs change), then tracer
   // sets: the sub-step states (4, or 4 per step of a segment when kept),

@@ -339,8 +339,8 @@ int gt_set_tuning(gt_ctx *ctx, int fused, int tile, int group);
 /* reverse mode (gt_adv3d_adjoint*): keep_states 1 keeps the four sub-step
  * states of every step of a checkpoint segment while the segment is
  * recomputed, so the reverse steps do not recompute them (work memory
- * 4*ckpt_every tracer sets instead of 4, and, with per-step inputs,
- * ckpt_every sets of layer heights instead of one); 0 recomputes them per
+ * 4*c tracer sets instead of 4, and, with per-step inputs, c sets of layer
+ * heights instead of one, c = min(ckpt_every, nsteps)); 0 recomputes them per
  * reverse step; -1 (default) keeps them when that needs at most half of the
  * free device memory.  Results are the same bit for bit.  No counterpart in
  * the reference (its reverse mode is this build's extension). */
@@ -366,8 +366,8 @@ int gt_autotune_batch(int ndom, const gt_batch_domain *doms, int ntracers,
  * sub-step; the states before the sub-steps come from the per-sub-step
  * kernels (bit-identical to the strip kernels), kept for the whole segment
  * or recomputed per reverse step (gt_set_adjoint_mode).  Work memory in FP64
- * fields: 5 + 2 + ceil(nsteps/ckpt_every) + ckpt_every - 1, plus 4 (states
- * recomputed) or 4*ckpt_every (states kept). */
+ * fields, with c = min(ckpt_every, nsteps): 5 + 2 + ceil(nsteps/ckpt_every)
+ * + c - 1, plus 4 (states recomputed) or 4*c (states kept). */
 int gt_adv3d_adjoint(gt_ctx *ctx, int nsteps, int ckpt_every, double dt,
                      const double *f0, double *f_final, const double *lam_final,
                      double *lam0, const double *uu, const double *vv,
@@ -382,9 +382,9 @@ int gt_adv3d_adjoint(gt_ctx *ctx, int nsteps, int ckpt_every, double dt,
  * reverse sweep recomputes each step's sub-step states from the checkpoint
  * (kept per segment or per reverse step, gt_set_adjoint_mode) and applies
  * the transpose of each FP32 sub-step, the limiter through the derivative of
- * the branch term the FP32 forward selected.  Work memory: 5 FP64 fields +
- * (2 + ceil(nsteps/ckpt_every) + ckpt_every - 1 + 4 or 4*ckpt_every) tracer
- * sets of ntracers FP32 fields. */
+ * the branch term the FP32 forward selected.  Work memory, with c =
+ * min(ckpt_every, nsteps): 5 FP64 fields + (2 + ceil(nsteps/ckpt_every) +
+ * c - 1 + 4 or 4*c) tracer sets of ntracers FP32 fields. */
 int gt_adv3d_adjoint_f32(gt_ctx *ctx, int ntracers, int nsteps, int ckpt_every,
                          double dt, const float *f0, float *f_final,
                          const float *lam_final, float *lam0, const double *uu,
@@ -395,7 +395,7 @@ int gt_adv3d_adjoint_f32(gt_ctx *ctx, int ntracers, int nsteps, int ckpt_every,
  * producers of each macro step): uu[t], vv[t], ww[t], hu[t], hv[t], ho[t]
  * (host arrays of nsteps device pointers) are the inputs of step t; the layer
  * heights of the sub-steps are formed per step (with the states kept, once
- * per step of a segment: ckpt_every sets of 5 FP64 fields).  With the same
+ * per step of a segment: min(ckpt_every, nsteps) sets of 5 FP64 fields).  With the same
  * pointers for every step the result equals gt_adv3d_adjoint / _f32 bit for
  * bit. */
 int gt_adv3d_adjoint_seq(gt_ctx *ctx, int nsteps, int ckpt_every, double dt,

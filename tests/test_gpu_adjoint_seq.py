@@ -63,8 +63,9 @@ def test_same_inputs_equal_fixed_entry_points(device):
     l32 = torch.from_numpy(rng.standard_normal((2,) + d.shape3).astype(np.float32)).to(device)
     n = 7
     # keep: the per-step forms keep every step's layer heights and sub-step
-    # states of a segment (1) or recompute them per reverse step (0)
-    for ck, keep in ((1, 0), (3, 0), (3, 1), (7, 1)):
+    # states of a segment (1) or recompute them per reverse step (0); a
+    # checkpoint spacing beyond nsteps is one segment of nsteps steps
+    for ck, keep in ((1, 0), (3, 0), (3, 1), (7, 1), (20, 1), (20, 0)):
         ctx.set_adjoint_mode(-1)
         a, fa = torch.empty_like(c64), torch.empty_like(c64)
         ctx.adv3d_adjoint(n, ck, tr.dt, c64, l64, a, *vel, gt.P2_PDM, gt.P2_PDM, f_final=fa)
