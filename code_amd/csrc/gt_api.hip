@@ -1406,7 +1406,14 @@ static int adjoint_impl(const char* name, gt_ctx* c, int ntr, int nsteps, int ck
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) == hipSuccess) keep = bytes_for(true) <= (fr + c->w32_bytes) / 2;
   }
-  TRY(ensure_w32(c, bytes_for(keep)));
+  int got = ensure_w32(c, bytes_for(keep));
+  if (got != GT_OK && keep && c->adj_keep < 0) {
+    // the free memory was there but not in one piece: recompute instead
+    (void)hipGetLastError();
+    keep = false;
+    got = ensure_w32(c, bytes_for(false));
+  }
+  if (got != GT_OK) return got;
   const int hslots = keep && in.per_step ? seg : 1;
   double* hbase = (double*)c->w32;
   TF* base = (TF*)(hbase + 5 * (size_t)N3 * hslots);
