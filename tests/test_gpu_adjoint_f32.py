@@ -229,3 +229,6 @@ def test_adjoint_f32_errors(device):
         ctx.adv3d_adjoint_f32(1, 0, 1, F["dt"], f0, f0, lam, *vel, gt.P2, gt.P2)
     with pytest.raises(ValueError):
         ctx.adv3d_adjoint_f32(2, 2, 1, F["dt"], f0, f0, lam, *vel, gt.P2, gt.P2)
+    with pytest.raises(gt.GtError, match="keep_states"):
+        ctx.set_adjoint_mode(2)
+    ctx.set_adjoint_mode(-1)
