@@ -670,6 +670,12 @@ def adjoint_section(torch, np, gt, inputs, dev, nsteps=48, ckpt=8):
         ctx.adv3d_adjoint(nsteps, ckpt, tr.dt, T0, Tbar, g_T, uu, vv, ww, hu, hv, ho,
                           gt.SUPERBEE, gt.SUPERBEE)
 
+    # strip heights for one FP64 tracer on this grid (untimed; the context's
+    # strip steps, forward and reverse, use them)
+    hires = torch.empty_like(c0)
+    bt, bg, _, _ = gt.autotune_batch([dict(ctx=ctx, f_in=c0, f_out=cF, uu=uu, vv=vv, ww=ww,
+                                           hu=hu, hv=hv, ho=ho, hires=hires)], 1, tr.dt,
+                                     gt.SUPERBEE, gt.SUPERBEE, reps=3, stream=st)
     with torch.cuda.stream(st):
         gradient()
         st.synchronize()
@@ -682,7 +688,7 @@ def adjoint_section(torch, np, gt, inputs, dev, nsteps=48, ckpt=8):
     ncell = d.imax * d.jmax * d.kmax
     return {"workload": f"{d.imax}x{d.jmax}x{d.kmax} sigma, Superbee HALFSPLIT, {nsteps} steps, "
                         f"checkpoint every {ckpt}; dJ/dc0 and dJ/dT0 (J: box c^2 + BM term)",
-            "ms_per_gradient": t * 1e3,
+            "ms_per_gradient": t * 1e3, "rows": rows_str(gt.strip_rows(bt)),
             "gradient_cell_steps_per_s": ncell * nsteps * 2 / t,
             "note": "per tracer: 48 forward steps for the cost, then the adjoint call (48 "
                     "forward steps to the checkpoints, the last 8 through the sub-step "
