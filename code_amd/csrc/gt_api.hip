@@ -157,6 +157,13 @@ extern "C" int gt_create(gt_ctx** out, int imin, int imax, int jmin, int jmax, i
   c->d.kmax = kmax;
   c->d.NXY = (long)c->d.NX * c->d.NY;
   c->N3 = c->d.NXY * (kmax + 1);
+  // default strip height: 16 rows, or 8 where 16-row strips would give the
+  // strip kernels fewer than 4096 wavefronts (one tracer group), so a small
+  // grid still fills the chip (gt_set_tuning / gt_autotune_batch override)
+  {
+    const long items16 = (long)((c->d.NX + 59) / 60) * ((c->d.NY + 15) / 16) * (kmax + 1);
+    c->tile = items16 < 4096 ? 2 : 0;
+  }
   c->dx = dx; c->dy = dy;
   c->ard1 = 1.0 / (dx * dy);  // domain.F90:596
   const long NX = c->d.NX, NY = c->d.NY, NXY = c->d.NXY;

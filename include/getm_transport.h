@@ -330,7 +330,8 @@ int gt_adv3d_batch_f64(int ndom, const gt_batch_domain *doms, int ntracers,
  * per-sub-step kernels (0; gt_adv3d_batch_f32 then runs the FP32 sub-steps
  * one kernel each, bit-identical to the strip kernels, and accepts
  * hscheme != vscheme; gt_adv3d_batch_f64 always runs the strip kernels);
- * rows per strip: tile 0, 1, 2 give 16, 32, 8 rows
+ * rows per strip (default: 16, or 8 on grids where 16-row strips give fewer
+ * than 4096 strip-level items): tile 0, 1, 2 give 16, 32, 8 rows
  * to both kernels, tile 3..11 gives {16, 32, 8}[(tile-3)%3] rows to the first
  * (u/2 v/2) and {16, 32, 8}[(tile-3)/3] to the second (w v/2 u/2); tracers
  * per wavefront (group: FP32 1, 3, 6 or 9; FP64 1, or 2 for any other value
