@@ -13,9 +13,13 @@
  *
  * Every function is pinned by tests/test_oracle_*.py (closed forms,
  * published values, invariants, reflection symmetry, the reference's own
- * check values).  Parity unpinned beyond this restatement's reading of the
- * source: a loop bound of adv_split_u/v that would be wrong by the same
- * amount on both edges (reflection symmetry catches any other bound error).
+ * check values): the Shchepetkin-McWilliams brackets against exact
+ * cubic-Hermite integrals, the W-point column, the sub-cycling restart,
+ * and the NOSPLIT/HVSPLIT/FULLSPLIT and 2-D driver compositions against
+ * textbook operators (tests/test_oracle_pins_branches.py).  What no check
+ * fixes beyond this restatement's reading of the source: a halo-row loop
+ * bound of adv_split_u/v wrong by the same amount on both edges
+ * (reflection symmetry catches every asymmetric bound error).
  */
 #include "getm_oracle.h"
 
