@@ -80,6 +80,10 @@ struct gt_ctx {
   double* mwork = nullptr; // uv_advect_3d work: fadv uuadv vvadv wwadv huadv hvadv
   void* w32 = nullptr;     // per-sub-step and reverse-mode work (grown on demand)
   size_t w32_bytes = 0;
+  // SUBSTR_INI_PRESS (internal_pressure.F90:132-141,235-258): the first
+  // gradients (idpdx0 | idpdy0) and the SAVE'd first-call flag
+  int sip = 0, sip_first = 1;
+  double* idp0 = nullptr;
 };
 
 extern "C" const char* gt_last_error(void) { return g_err; }
@@ -237,7 +241,7 @@ extern "C" int gt_destroy(gt_ctx* c) {
   if (!c) return GT_OK;
   void* ps[] = {c->dmask, c->dint, c->fi, c->hi, c->adv, c->tmp, c->scr7, c->zz,
                 c->P, c->slice3, c->bpart, c->bout, c->nwarn, c->ws, c->dlev, c->gga,
-                c->dbad, c->vdw, c->tri, c->mwork, c->w32};
+                c->dbad, c->vdw, c->tri, c->mwork, c->w32, c->idp0};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   delete c;
@@ -1022,15 +1026,34 @@ extern "C" int gt_do_internal_pressure(gt_ctx* c, int ip_method, const double* H
   if (!c) return set_err(GT_EARG, "gt_do_internal_pressure: NULL context");
   switch (ip_method) {
     case GT_IP_BLUMBERG_MELLOR:
-      return gt_ip_blumberg_mellor(c, H, hn, hun, hvn, buoy, c->zz, idpdx, idpdy);
+      TRY(gt_ip_blumberg_mellor(c, H, hn, hun, hvn, buoy, c->zz, idpdx, idpdy));
+      break;
     case GT_IP_SHCHEPETKIN_MCWILLIAMS:
-      return gt_ip_shchepetkin_mcwilliams(c, H, hn, hun, hvn, buoy, sseo, c->zz, idpdx, idpdy);
+      TRY(gt_ip_shchepetkin_mcwilliams(c, H, hn, hun, hvn, buoy, sseo, c->zz, idpdx, idpdy));
+      break;
     case 2: case 3: case 4: case 5: case 7:
       return set_err(GT_ENOTIMPL, "do_internal_pressure: ip_method %d is not part of this build",
                      ip_method);
     default:
       return set_err(GT_EINVAL_SCHEME, "Not valid ip_method specified");
   }
+  if (c->sip) {
+    LCHK(gt::launch_substr_ini_press(c->st, c->d, c->sip_first, idpdx, idpdy, c->idp0,
+                                     c->idp0 + c->N3));
+    c->sip_first = 0;
+  }
+  return GT_OK;
+}
+
+extern "C" int gt_set_substr_ini_press(gt_ctx* c, int on) {
+  if (!c) return set_err(GT_EARG, "gt_set_substr_ini_press: NULL context");
+  if (on && !c->idp0) {
+    HIPCHK(hipMalloc(&c->idp0, sizeof(double) * 2 * (size_t)c->N3));
+    HIPCHK(hipMemsetAsync(c->idp0, 0, sizeof(double) * 2 * (size_t)c->N3, c->st));
+  }
+  c->sip = on != 0;
+  c->sip_first = 1;
+  return GT_OK;
 }
 
 // ---- ghost cells ---------------------------------------------------------------
@@ -1069,28 +1092,18 @@ extern "C" int gt_exchange_tiles(int px, int py, const int* ni, const int* nj, i
       if (nj[t] != nj[ty * px] || ni[t] != ni[tx])
         return set_err(GT_EARG, "gt_exchange_tiles: tiles of one row/column differ in size");
     }
-  std::vector<gt::TileDesc> td(nt);
-  std::vector<long> off(nt + 1, 0);
+  if (nt > gt::kMaxTiles)
+    return set_err(GT_EARG, "gt_exchange_tiles: at most %d tiles (got %d)", gt::kMaxTiles, nt);
+  // the tile table travels in the kernel argument: nothing is allocated and
+  // the call returns once the exchange is enqueued
+  gt::TileSet ts{};
+  ts.ring_off[0] = 0;
   for (int t = 0; t < nt; t++) {
-    td[t] = {tiles[t], ni[t], nj[t]};
+    ts.t[t] = {tiles[t], ni[t], nj[t]};
     long NX = ni[t] + 2 * GT_HALO;
-    off[t + 1] = off[t] + 2 * GT_HALO * NX + 2 * GT_HALO * (long)nj[t];
+    ts.ring_off[t + 1] = ts.ring_off[t] + 2 * GT_HALO * NX + 2 * GT_HALO * (long)nj[t];
   }
-  hipStream_t st = (hipStream_t)stream;
-  void* dbuf = nullptr;
-  size_t bytes = sizeof(gt::TileDesc) * nt + sizeof(long) * (nt + 1);
-  HIPCHK(hipMalloc(&dbuf, bytes));
-  gt::TileDesc* dtd = (gt::TileDesc*)dbuf;
-  long* doff = (long*)((char*)dbuf + sizeof(gt::TileDesc) * nt);
-  hipError_t e = hipMemcpyAsync(dtd, td.data(), sizeof(gt::TileDesc) * nt,
-                                hipMemcpyHostToDevice, st);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(doff, off.data(), sizeof(long) * (nt + 1), hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = gt::launch_tile_exchange(st, nt, px, dtd, doff, off[nt], nk);
-  hipError_t e2 = hipStreamSynchronize(st);
-  (void)hipFree(dbuf);
-  if (e != hipSuccess) return set_err(GT_EHIP, "gt_exchange_tiles: %s", hipGetErrorString(e));
-  if (e2 != hipSuccess) return set_err(GT_EHIP, "gt_exchange_tiles: %s", hipGetErrorString(e2));
+  LCHK(gt::launch_tile_exchange((hipStream_t)stream, nt, px, ts, nk));
   return GT_OK;
 }
 
@@ -1104,6 +1117,30 @@ extern "C" int gt_set_tuning(gt_ctx* c, int fused, int tile, int group) {
   c->fused = fused != 0;
   c->tile = tile;
   c->group = group;
+  return GT_OK;
+}
+
+// sizes every work field of the transport step up front, so later calls
+// neither allocate nor synchronise the device to grow one
+extern "C" int gt_reserve(gt_ctx* c, int ntracers_f32, int ntracers_f64) {
+  if (!c || ntracers_f32 < 0 || ntracers_f64 < 0)
+    return set_err(GT_EARG, "gt_reserve: bad argument");
+  size_t ws = strip_ws_bytes(c, sizeof(double), 1, true);
+  const size_t w32 = strip_ws_bytes(c, sizeof(float), ntracers_f32, false);
+  const size_t w64 = strip_ws_bytes(c, sizeof(double), ntracers_f64, false);
+  if (w32 > ws) ws = w32;
+  if (w64 > ws) ws = w64;
+  TRY(ensure_ws(c, ws));
+  const long N3 = c->N3;
+  ENSURE(c->fi, N3);
+  ENSURE(c->hi, N3);
+  ENSURE(c->adv, N3);
+  ENSURE(c->tmp, N3);
+  ENSURE(c->P, N3);
+  ENSURE(c->slice3, 3 * c->d.NXY);
+  ENSURE(c->bpart, 4 * 1024);
+  ENSURE(c->bout, 2);
+  if (c->adv_ver_iterations > 1) ENSURE(c->scr7, 7 * N3);
   return GT_OK;
 }
 
@@ -1201,6 +1238,12 @@ static int batch_impl(int ndom, const gt_batch_domain* doms, int ntracers, doubl
                                   "stay below 4 GiB in FP64 (split the domain)", i, c->N3);
     gsel = grid_for(c, tag);
     if (!gsel) return set_err(GT_EINVAL_TAG, "gt_adv3d_batch: tag is invalid");
+    // the strip kernels keep their work fields in the context: one context
+    // twice in a launch would have two domains write the same work fields
+    for (int q = 0; q < i; q++)
+      if (doms[q].ctx == c)
+        return set_err(GT_EARG, "gt_adv3d_batch: domains %d and %d share one context (give "
+                                "each domain its own)", q, i);
   }
   gt_ctx* c0 = doms[0].ctx;
   hipStream_t st = stream ? (hipStream_t)stream : c0->st;

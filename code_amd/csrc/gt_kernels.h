@@ -10,6 +10,14 @@ struct TileDesc {
   int ni, nj;
 };
 
+// the tiles of one exchange, passed by value as a kernel argument (no
+// device-side table, so the exchange allocates nothing)
+constexpr int kMaxTiles = 64;
+struct TileSet {
+  TileDesc t[kMaxTiles];
+  long ring_off[kMaxTiles + 1];  // start of each tile's ghost ring in the flat index
+};
+
 // every launcher enqueues on `st` and returns the launch status
 hipError_t launch_split_u(hipStream_t st, Dims d, int nslice, const double* f,
                           const double* fi_src, double* fi_dst, double* Di, double* adv,
@@ -103,8 +111,11 @@ hipError_t launch_ip_smcw(hipStream_t st, Dims d, const int32_t* az, const int32
                           double* idpdy, double dx, double dy);
 hipError_t launch_mirror_3d(hipStream_t st, Dims d, double* f1, const double* f2);
 hipError_t launch_mirror_2d(hipStream_t st, Dims d, double* f1, const double* f2);
-hipError_t launch_tile_exchange(hipStream_t st, int ntiles, int px, const TileDesc* dtiles,
-                                const long* dring_off, long ring_total, int nk);
+hipError_t launch_tile_exchange(hipStream_t st, int ntiles, int px, const TileSet& tiles, int nk);
+// SUBSTR_INI_PRESS (internal_pressure.F90:235-258) over the interior, all levels:
+// first call keeps the gradients in x0/y0 and zeroes them, later calls subtract
+hipError_t launch_substr_ini_press(hipStream_t st, Dims d, int first, double* idpdx,
+                                   double* idpdy, double* x0, double* y0);
 // partial must hold 4*nblocks doubles; out two doubles (mass, volume)
 template <typename TF>
 hipError_t launch_budget(hipStream_t st, Dims d, const int32_t* az, const double* h,

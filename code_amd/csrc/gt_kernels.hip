@@ -823,16 +823,16 @@ __global__ void k_mirror_2d(Dims d, double* f1, const double* f2, int aliased) {
 // in a neighbour tile's interior takes that value.  One launch covers all
 // tiles: blockIdx.z = k, and a flat thread index walks the ghost ring of
 // every tile (ring offsets in ring_off[]).
-__global__ void k_tile_exchange(int ntiles, int px, const TileDesc* __restrict__ tiles,
-                                const long* __restrict__ ring_off, long ring_total, int nk) {
+__global__ void k_tile_exchange(int ntiles, int px, TileSet ts, int nk) {
+  const long ring_total = ts.ring_off[ntiles];
   long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
   int k = blockIdx.y;
   if (q >= ring_total || k >= nk) return;
   // find the tile (few tiles: linear scan)
   int t = 0;
-  while (t + 1 < ntiles && ring_off[t + 1] <= q) t++;
-  const TileDesc& T = tiles[t];
-  long r = q - ring_off[t];
+  while (t + 1 < ntiles && ts.ring_off[t + 1] <= q) t++;
+  const TileDesc& T = ts.t[t];
+  long r = q - ts.ring_off[t];
   long NX = T.ni + 2 * GT_HALO, NY = T.nj + 2 * GT_HALO;
   // ring enumeration: rows jj in [0,2) and [nj+2, nj+4) full width, then
   // columns ii in [0,2) and [ni+2, ni+4) for the interior rows
@@ -854,7 +854,7 @@ __global__ void k_tile_exchange(int ntiles, int px, const TileDesc* __restrict__
   int sx = tx + ox, sy = ty + oy;
   int py = ntiles / px;
   if (sx < 0 || sx >= px || sy < 0 || sy >= py) return;
-  const TileDesc& S = tiles[sy * px + sx];
+  const TileDesc& S = ts.t[sy * px + sx];
   long SNX = S.ni + 2 * GT_HALO, SNY = S.nj + 2 * GT_HALO;
   long si = ox < 0 ? ii + S.ni : (ox > 0 ? ii - T.ni : ii);
   long sj = oy < 0 ? jj + S.nj : (oy > 0 ? jj - T.nj : jj);
@@ -1070,11 +1070,42 @@ hipError_t launch_mirror_2d(hipStream_t st, Dims d, double* f1, const double* f2
   return hipGetLastError();
 }
 
-hipError_t launch_tile_exchange(hipStream_t st, int ntiles, int px, const TileDesc* dtiles,
-                                const long* dring_off, long ring_total, int nk) {
+hipError_t launch_tile_exchange(hipStream_t st, int ntiles, int px, const TileSet& tiles,
+                                int nk) {
+  const long ring_total = tiles.ring_off[ntiles];
   if (ring_total <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_tile_exchange, dim3((unsigned)((ring_total + 255) / 256), nk), dim3(256),
-                     0, st, ntiles, px, dtiles, dring_off, ring_total, nk);
+                     0, st, ntiles, px, tiles, nk);
+  return hipGetLastError();
+}
+
+// SUBSTR_INI_PRESS, internal_pressure.F90:235-258: i = imin..imax, j =
+// jmin..jmax, k = 0..kmax
+__global__ void k_substr_ini_press(Dims d, int first, double* __restrict__ idpdx,
+                                   double* __restrict__ idpdy, double* __restrict__ x0,
+                                   double* __restrict__ y0) {
+  const int i = GT_HALO + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = GT_HALO + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  if (i > d.NX - 1 - GT_HALO || j > d.NY - 1 - GT_HALO || k > d.kmax) return;
+  const long c = i + (long)d.NX * j + d.NXY * (long)k;
+  if (first) {
+    x0[c] = idpdx[c];
+    idpdx[c] = 0.0;
+    y0[c] = idpdy[c];
+    idpdy[c] = 0.0;
+  } else {
+    idpdx[c] = idpdx[c] - x0[c];
+    idpdy[c] = idpdy[c] - y0[c];
+  }
+}
+
+hipError_t launch_substr_ini_press(hipStream_t st, Dims d, int first, double* idpdx,
+                                   double* idpdy, double* x0, double* y0) {
+  const int ni = d.NX - 2 * GT_HALO, nj = d.NY - 2 * GT_HALO;
+  dim3 blk(64, 4);
+  dim3 grd((unsigned)((ni + 63) / 64), (unsigned)((nj + 3) / 4), (unsigned)(d.kmax + 1));
+  hipLaunchKernelGGL(k_substr_ini_press, grd, blk, 0, st, d, first, idpdx, idpdy, x0, y0);
   return hipGetLastError();
 }
 

@@ -48,7 +48,7 @@ EXPORTS = [
     "gt_start_macro", "gt_set_coordinates", "gt_coordinates", "gt_ww_momentum_3d",
     "gt_salinity_diffusion", "gt_temperature_diffusion", "gt_tridiagonal_batch",
     "gt_uv_advect_3d", "gt_restart_write_host", "gt_restart_read_host", "gt_write_restart",
-    "gt_read_restart",
+    "gt_read_restart", "gt_set_substr_ini_press", "gt_reserve",
 ]
 
 
@@ -126,6 +126,8 @@ def load_library():
         "gt_ip_blumberg_mellor": (i, [vp, vp, vp, vp, vp, vp, vp, vp, vp]),
         "gt_ip_shchepetkin_mcwilliams": (i, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
         "gt_do_internal_pressure": (i, [vp, i, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "gt_set_substr_ini_press": (i, [vp, i]),
+        "gt_reserve": (i, [vp, i, i]),
         "gt_update_3d_halo": (i, [vp, vp, vp, i, i]),
         "gt_update_2d_halo": (i, [vp, vp, vp, i, i]),
         "gt_wait_halo": (i, [vp, i]),
@@ -164,9 +166,34 @@ def load_library():
 
 
 def _ptr(t):
-    """device pointer of a tensor (or None)"""
+    """device pointer of a tensor (or None); callers validate first"""
     if t is None:
         return None
+    return C.c_void_p(t.data_ptr())
+
+
+def _dev(t, name, dtype=None, shape=None, numel=None, optional=False):
+    """validated device pointer: a contiguous CUDA tensor of ``dtype`` (FP64
+    by default) and ``shape`` (or ``numel`` elements).  Raises ValueError
+    instead of handing the library an address it would read out of bounds."""
+    import torch
+    if t is None:
+        if optional:
+            return None
+        raise ValueError(f"{name}: a device tensor is required")
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name}: expected a torch tensor, got {type(t).__name__}")
+    if t.device.type != "cuda":
+        raise ValueError(f"{name}: must be a GPU tensor (got {t.device})")
+    want = torch.float64 if dtype is None else dtype
+    if t.dtype != want:
+        raise ValueError(f"{name}: dtype must be {want} (got {t.dtype})")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: must be contiguous")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: shape must be {tuple(shape)} (got {tuple(t.shape)})")
+    if numel is not None and t.numel() != numel:
+        raise ValueError(f"{name}: must hold {numel} elements (got {t.numel()})")
     return C.c_void_p(t.data_ptr())
 
 
@@ -206,7 +233,9 @@ class Context:
     """One domain on the device (gt_create).
 
     ``az``, ``au``, ``av``, ``ax`` are (NY, NX) int32 numpy arrays; ``au``,
-    ``av``, ``ax`` may be None (derived from az as domain.F90 does).
+    ``av``, ``ax`` may be None (derived from az as domain.F90 does).  Every
+    method checks its tensors (GPU, dtype, contiguous, shape) before the
+    library sees them and raises ValueError otherwise.
     """
 
     def __init__(self, imax, jmax, kmax, dx, dy, az, au=None, av=None, ax=None,
@@ -222,9 +251,12 @@ class Context:
             if a is None:
                 return None
             a = np.ascontiguousarray(a, dtype=np.int32)
-            assert a.shape == (self.NY, self.NX), (a.shape, (self.NY, self.NX))
+            if a.shape != (self.NY, self.NX):
+                raise ValueError(f"mask shape {a.shape} != {(self.NY, self.NX)}")
             return a
         self._masks = [arr(m) for m in (az, au, av, ax)]
+        if self._masks[0] is None:
+            raise ValueError("az is required")
         ptrs = [None if m is None else m.ctypes.data_as(C.c_void_p) for m in self._masks]
         h = C.c_void_p()
         check(lib.gt_create(C.byref(h), imin, imax, jmin, jmax, kmax, self.dx, self.dy,
@@ -237,6 +269,25 @@ class Context:
     def from_domain(cls, dom, **kw):
         return cls(dom.imax, dom.jmax, dom.kmax, dom.dx, dom.dy, dom.az, dom.au, dom.av,
                    dom.ax, imin=dom.imin, jmin=dom.jmin, **kw)
+
+    @property
+    def shape3(self):
+        return (self.kmax + 1, self.NY, self.NX)
+
+    @property
+    def shape2(self):
+        return (self.NY, self.NX)
+
+    def _f3(self, t, name, optional=False, dtype=None):
+        return _dev(t, name, dtype=dtype, shape=self.shape3, optional=optional)
+
+    def _f2(self, t, name, optional=False):
+        return _dev(t, name, shape=self.shape2, optional=optional)
+
+    def _set(self, t, name, ntracers, dtype, optional=False):
+        """a tracer set: ntracers fields back to back"""
+        return _dev(t, name, dtype=dtype, numel=ntracers * int(np.prod(self.shape3)),
+                    optional=optional)
 
     def close(self):
         if getattr(self, "h", None) is not None:
@@ -264,44 +315,58 @@ class Context:
     def warnings(self):
         return int(self.lib.gt_warnings(self.h))
 
+    def reserve(self, ntracers_f32=0, ntracers_f64=0):
+        """gt_reserve: size every work field of the transport step now"""
+        check(self.lib.gt_reserve(self.h, int(ntracers_f32), int(ntracers_f64)))
+
     # ---- advection --------------------------------------------------------
     def adv_interfacial_reconstruction(self, scheme, cfl, fuu, fu, fd, out):
-        check(self.lib.gt_adv_interfacial_reconstruction(
-            self.h, scheme, cfl.numel(), _ptr(cfl), _ptr(fuu), _ptr(fu), _ptr(fd), _ptr(out)))
+        n = cfl.numel() if cfl is not None else 0
+        ps = [_dev(t, nm, numel=n) for t, nm in ((cfl, "cfl"), (fuu, "fuu"), (fu, "fu"),
+                                                  (fd, "fd"), (out, "out"))]
+        check(self.lib.gt_adv_interfacial_reconstruction(self.h, scheme, n, *ps))
 
     def adv_split_u(self, dt, f, fi, Di, adv, U, DU, splitfac, scheme, AH, tag=H_TAG):
-        check(self.lib.gt_adv_split_u(self.h, dt, _ptr(f), _ptr(fi), _ptr(Di), _ptr(adv),
-                                      _ptr(U), _ptr(DU), splitfac, scheme, AH, tag))
+        ps = [self._f2(t, nm) for t, nm in ((f, "f"), (fi, "fi"), (Di, "Di"), (adv, "adv"),
+                                            (U, "U"), (DU, "DU"))]
+        check(self.lib.gt_adv_split_u(self.h, dt, *ps, splitfac, scheme, AH, tag))
 
     def adv_split_v(self, dt, f, fi, Di, adv, V, DV, splitfac, scheme, AH, tag=H_TAG):
-        check(self.lib.gt_adv_split_v(self.h, dt, _ptr(f), _ptr(fi), _ptr(Di), _ptr(adv),
-                                      _ptr(V), _ptr(DV), splitfac, scheme, AH, tag))
+        ps = [self._f2(t, nm) for t, nm in ((f, "f"), (fi, "fi"), (Di, "Di"), (adv, "adv"),
+                                            (V, "V"), (DV, "DV"))]
+        check(self.lib.gt_adv_split_v(self.h, dt, *ps, splitfac, scheme, AH, tag))
 
     def adv_split_w(self, dt, f, fi, hi, adv, ww, splitfac, scheme, tag, itersmax):
-        check(self.lib.gt_adv_split_w(self.h, dt, _ptr(f), _ptr(fi), _ptr(hi), _ptr(adv),
-                                      _ptr(ww), splitfac, scheme, tag, itersmax))
+        ps = [self._f3(t, nm) for t, nm in ((f, "f"), (fi, "fi"), (hi, "hi"), (adv, "adv"),
+                                            (ww, "ww"))]
+        check(self.lib.gt_adv_split_w(self.h, dt, *ps, splitfac, scheme, tag, itersmax))
 
     def adv_upstream_2dh(self, dt, f, fi, Di, adv, U, V, Dn, DU, DV, AH, tag=H_TAG):
-        check(self.lib.gt_adv_upstream_2dh(self.h, dt, _ptr(f), _ptr(fi), _ptr(Di),
-                                           _ptr(adv), _ptr(U), _ptr(V), _ptr(Dn), _ptr(DU),
-                                           _ptr(DV), AH, tag))
+        ps = [self._f2(t, nm) for t, nm in ((f, "f"), (fi, "fi"), (Di, "Di"), (adv, "adv"),
+                                            (U, "U"), (V, "V"), (Dn, "Dn"))]
+        ps += [self._f2(t, nm, optional=True) for t, nm in ((DU, "DU"), (DV, "DV"))]
+        check(self.lib.gt_adv_upstream_2dh(self.h, dt, *ps, AH, tag))
 
     def do_advection(self, dt, f, U, V, DU, DV, Do, Dn, split, scheme, AH, tag,
                      Dires=None, advres=None):
-        check(self.lib.gt_do_advection(self.h, dt, _ptr(f), _ptr(U), _ptr(V), _ptr(DU),
-                                       _ptr(DV), _ptr(Do), _ptr(Dn), split, scheme, AH, tag,
-                                       _ptr(Dires), _ptr(advres)))
+        ps = [self._f2(t, nm) for t, nm in ((f, "f"), (U, "U"), (V, "V"), (DU, "DU"),
+                                            (DV, "DV"), (Do, "Do"), (Dn, "Dn"))]
+        check(self.lib.gt_do_advection(self.h, dt, *ps, split, scheme, AH, tag,
+                                       self._f2(Dires, "Dires", True),
+                                       self._f2(advres, "advres", True)))
 
     def do_advection_3d(self, dt, f, uu, vv, ww, hu, hv, ho, hn, split, hscheme, vscheme,
                         AH, tag, hires=None, advres=None):
-        check(self.lib.gt_do_advection_3d(self.h, dt, _ptr(f), _ptr(uu), _ptr(vv), _ptr(ww),
-                                          _ptr(hu), _ptr(hv), _ptr(ho), _ptr(hn), split,
-                                          hscheme, vscheme, AH, tag, _ptr(hires),
-                                          _ptr(advres)))
+        ps = [self._f3(t, nm) for t, nm in ((f, "f"), (uu, "uu"), (vv, "vv"), (ww, "ww"),
+                                            (hu, "hu"), (hv, "hv"), (ho, "ho"), (hn, "hn"))]
+        check(self.lib.gt_do_advection_3d(self.h, dt, *ps, split, hscheme, vscheme, AH, tag,
+                                          self._f3(hires, "hires", True),
+                                          self._f3(advres, "advres", True)))
 
     # ---- internal pressure --------------------------------------------------
     def eos_linear(self, T, S, rho, buoy, T0, S0, dtr0, dsr0):
-        check(self.lib.gt_eos_linear(self.h, _ptr(T), _ptr(S), _ptr(rho), _ptr(buoy),
+        check(self.lib.gt_eos_linear(self.h, self._f3(T, "T"), self._f3(S, "S"),
+                                     self._f3(rho, "rho", True), self._f3(buoy, "buoy"),
                                      T0, S0, dtr0, dsr0))
 
     def do_eqstate(self, method, T, S, hn, rho, buoy, alpha=None, beta=None, T0=10.0,
@@ -309,16 +374,21 @@ class Context:
         """do_eqstate (eqstate.F90:94-271); the defaults are the reference's
         namelist defaults, FP32 literals (rounded here)."""
         f32 = lambda v: float(np.float32(v))
-        check(self.lib.gt_do_eqstate(self.h, method, int(nonnegsalt), _ptr(T), _ptr(S),
-                                     _ptr(hn), _ptr(rho), _ptr(buoy), _ptr(alpha), _ptr(beta),
+        check(self.lib.gt_do_eqstate(self.h, method, int(nonnegsalt), self._f3(T, "T"),
+                                     self._f3(S, "S"), self._f3(hn, "hn", True),
+                                     self._f3(rho, "rho"), self._f3(buoy, "buoy"),
+                                     self._f3(alpha, "alpha", True),
+                                     self._f3(beta, "beta", True),
                                      f32(T0), f32(S0), f32(dtr0), f32(dsr0)))
 
     # ---- input producers (m3d.F90:446-548) ----------------------------------
     def start_macro(self, M, min_depth, H, HU, HV, z, sseo, ssen, ssuo, ssun, ssvo, ssvn, Dn,
                     Dun, Dvn, Uint, Vint):
+        names = ("H", "HU", "HV", "z", "sseo", "ssen", "ssuo", "ssun", "ssvo", "ssvn", "Dn",
+                 "Dun", "Dvn", "Uint", "Vint")
+        vals = (H, HU, HV, z, sseo, ssen, ssuo, ssun, ssvo, ssvn, Dn, Dun, Dvn, Uint, Vint)
         check(self.lib.gt_start_macro(self.h, int(M), min_depth,
-                                      *(_ptr(a) for a in (H, HU, HV, z, sseo, ssen, ssuo, ssun,
-                                                          ssvo, ssvn, Dn, Dun, Dvn, Uint, Vint))))
+                                      *(self._f2(a, nm) for a, nm in zip(vals, names))))
 
     def set_coordinates(self, vert_cord, ddu=0.0, ddl=0.0, d_gamma=20.0, gamma_surf=True,
                         min_depth=0.5):
@@ -327,23 +397,31 @@ class Context:
 
     def coordinates(self, dt, cord_relax, maxdepth, H, HU, HV, sseo, ssen, ssuo, ssun, ssvo,
                     ssvn, ho, hn, huo, hun, hvo, hvn, zc=None):
-        check(self.lib.gt_coordinates(self.h, dt, cord_relax, maxdepth,
-                                      *(_ptr(a) for a in (H, HU, HV, sseo, ssen, ssuo, ssun,
-                                                          ssvo, ssvn, ho, hn, huo, hun, hvo, hvn,
-                                                          zc))))
+        p2 = [self._f2(a, nm, optional=nm != "H") for a, nm in
+              zip((H, HU, HV, sseo, ssen, ssuo, ssun, ssvo, ssvn),
+                  ("H", "HU", "HV", "sseo", "ssen", "ssuo", "ssun", "ssvo", "ssvn"))]
+        p3 = [self._f3(a, nm, optional=nm != "hn") for a, nm in
+              zip((ho, hn, huo, hun, hvo, hvn, zc), ("ho", "hn", "huo", "hun", "hvo", "hvn",
+                                                     "zc"))]
+        check(self.lib.gt_coordinates(self.h, dt, cord_relax, maxdepth, *p2, *p3))
 
     def ww_momentum_3d(self, dt, uu, vv, ho, hn, ww):
-        check(self.lib.gt_ww_momentum_3d(self.h, dt, _ptr(uu), _ptr(vv), _ptr(ho), _ptr(hn),
-                                         _ptr(ww)))
+        ps = [self._f3(t, nm) for t, nm in ((uu, "uu"), (vv, "vv"), (ho, "ho"), (hn, "hn"),
+                                            (ww, "ww"))]
+        check(self.lib.gt_ww_momentum_3d(self.h, dt, *ps))
 
     # ---- restart fields (the NetCDF hotstart file) ----------------------------
+    def _restart_ptr(self, t):
+        shape = self.shape3 if t.ndim == 3 else self.shape2
+        return _dev(t, "restart field", shape=shape).value
+
     def write_restart(self, path, fields, loop=0, with_halos=False):
-        arr, keep = _restart_fields(fields, lambda t: t.data_ptr())
+        arr, keep = _restart_fields(fields, self._restart_ptr)
         check(self.lib.gt_write_restart(self.h, str(path).encode(), int(with_halos), int(loop),
                                         len(fields), arr))
 
     def read_restart(self, path, fields, with_halos=False):
-        arr, keep = _restart_fields(fields, lambda t: t.data_ptr())
+        arr, keep = _restart_fields(fields, self._restart_ptr)
         loop = C.c_int()
         check(self.lib.gt_read_restart(self.h, str(path).encode(), int(with_halos),
                                        C.byref(loop), len(fields), arr))
@@ -351,58 +429,89 @@ class Context:
 
     def uv_advect_3d(self, dt, uu, vv, ww, ho, hn, hun, hvn, split, hscheme, vscheme, uuEx,
                      vvEx):
-        check(self.lib.gt_uv_advect_3d(self.h, dt, *(_ptr(a) for a in (uu, vv, ww, ho, hn, hun,
-                                                                       hvn)),
-                                       split, hscheme, vscheme, _ptr(uuEx), _ptr(vvEx)))
+        ps = [self._f3(t, nm) for t, nm in ((uu, "uu"), (vv, "vv"), (ww, "ww"), (ho, "ho"),
+                                            (hn, "hn"), (hun, "hun"), (hvn, "hvn"))]
+        check(self.lib.gt_uv_advect_3d(self.h, dt, *ps, split, hscheme, vscheme,
+                                       self._f3(uuEx, "uuEx"), self._f3(vvEx, "vvEx")))
 
     # ---- vertical diffusion (temperature.F90:463-551, salinity.F90:346-393) ----
     def salinity_diffusion(self, dt, cnpar, hn, nuh, S):
-        check(self.lib.gt_salinity_diffusion(self.h, dt, cnpar, _ptr(hn), _ptr(nuh), _ptr(S)))
+        check(self.lib.gt_salinity_diffusion(self.h, dt, cnpar, self._f3(hn, "hn"),
+                                             self._f3(nuh, "nuh"), self._f3(S, "S")))
 
     def temperature_diffusion(self, dt, cnpar, hn, nuh, swr, shf, A, g1, g2, dry_z, S, rad, T,
                               swr_bot_refl_frac=-1.0, swr_min_bot_frac=0.01):
         f32 = lambda v: float(np.float32(v))
+        p2 = [self._f2(a, nm) for a, nm in ((swr, "swr"), (shf, "shf"), (A, "A"), (g1, "g1"),
+                                            (g2, "g2"), (dry_z, "dry_z"))]
         check(self.lib.gt_temperature_diffusion(
-            self.h, dt, cnpar, *(_ptr(x) for x in (hn, nuh, swr, shf, A, g1, g2, dry_z, S)),
-            swr_bot_refl_frac, f32(swr_min_bot_frac), _ptr(rad), _ptr(T)))
+            self.h, dt, cnpar, self._f3(hn, "hn"), self._f3(nuh, "nuh"), *p2,
+            self._f3(S, "S"), swr_bot_refl_frac, f32(swr_min_bot_frac), self._f3(rad, "rad"),
+            self._f3(T, "T")))
 
     def tridiagonal_batch(self, fi, lt, au, bu, cu, du, value):
         """systems along the last axis: arrays of shape (lt+1, n)"""
-        check(self.lib.gt_tridiagonal_batch(self.h, au.shape[-1], fi, lt, _ptr(au), _ptr(bu),
-                                            _ptr(cu), _ptr(du), _ptr(value)))
+        n = au.shape[-1] if au is not None else 0
+        ps = [_dev(t, nm, shape=(lt + 1, n)) for t, nm in ((au, "au"), (bu, "bu"), (cu, "cu"),
+                                                           (du, "du"), (value, "value"))]
+        check(self.lib.gt_tridiagonal_batch(self.h, n, fi, lt, *ps))
 
     def rho_from_theta_unesco80(self, T, S, rho, nonnegsalt=False):
-        check(self.lib.gt_rho_from_theta_unesco80(self.h, T.numel(), _ptr(T), _ptr(S),
-                                                  _ptr(rho), int(nonnegsalt)))
+        n = T.numel() if T is not None else 0
+        check(self.lib.gt_rho_from_theta_unesco80(self.h, n, _dev(T, "T", numel=n),
+                                                  _dev(S, "S", numel=n),
+                                                  _dev(rho, "rho", numel=n), int(nonnegsalt)))
 
     def rho_from_theta(self, s, th, p, dens0, densp=None):
-        check(self.lib.gt_rho_from_theta(self.h, s.numel(), _ptr(s), _ptr(th), _ptr(p),
-                                         _ptr(dens0), _ptr(densp)))
+        n = s.numel() if s is not None else 0
+        check(self.lib.gt_rho_from_theta(self.h, n, _dev(s, "s", numel=n),
+                                         _dev(th, "th", numel=n),
+                                         _dev(p, "p", numel=n, optional=True),
+                                         _dev(dens0, "dens0", numel=n),
+                                         _dev(densp, "densp", numel=n, optional=True)))
 
     def eosall_from_theta(self, s, th, p, rho_s, rho_th):
-        check(self.lib.gt_eosall_from_theta(self.h, s.numel(), _ptr(s), _ptr(th), _ptr(p),
-                                            _ptr(rho_s), _ptr(rho_th)))
+        n = s.numel() if s is not None else 0
+        check(self.lib.gt_eosall_from_theta(self.h, n, _dev(s, "s", numel=n),
+                                            _dev(th, "th", numel=n),
+                                            _dev(p, "p", numel=n, optional=True),
+                                            _dev(rho_s, "rho_s", numel=n),
+                                            _dev(rho_th, "rho_th", numel=n)))
 
     def ip_blumberg_mellor(self, H, hn, hun, hvn, buoy, zz, idpdx, idpdy):
-        check(self.lib.gt_ip_blumberg_mellor(self.h, _ptr(H), _ptr(hn), _ptr(hun), _ptr(hvn),
-                                             _ptr(buoy), _ptr(zz), _ptr(idpdx), _ptr(idpdy)))
+        ps = [self._f3(t, nm) for t, nm in ((hn, "hn"), (hun, "hun"), (hvn, "hvn"),
+                                            (buoy, "buoy"), (zz, "zz"), (idpdx, "idpdx"),
+                                            (idpdy, "idpdy"))]
+        check(self.lib.gt_ip_blumberg_mellor(self.h, self._f2(H, "H"), *ps))
 
     def ip_shchepetkin_mcwilliams(self, H, hn, hun, hvn, buoy, sseo, zz, idpdx, idpdy):
+        ps = [self._f3(t, nm) for t, nm in ((hn, "hn"), (hun, "hun"), (hvn, "hvn"),
+                                            (buoy, "buoy"))]
         check(self.lib.gt_ip_shchepetkin_mcwilliams(
-            self.h, _ptr(H), _ptr(hn), _ptr(hun), _ptr(hvn), _ptr(buoy), _ptr(sseo), _ptr(zz),
-            _ptr(idpdx), _ptr(idpdy)))
+            self.h, self._f2(H, "H"), *ps, self._f2(sseo, "sseo"), self._f3(zz, "zz"),
+            self._f3(idpdx, "idpdx"), self._f3(idpdy, "idpdy")))
 
     def do_internal_pressure(self, ip_method, H, hn, hun, hvn, buoy, sseo, idpdx, idpdy):
+        ps = [self._f3(t, nm) for t, nm in ((hn, "hn"), (hun, "hun"), (hvn, "hvn"),
+                                            (buoy, "buoy"))]
         check(self.lib.gt_do_internal_pressure(
-            self.h, ip_method, _ptr(H), _ptr(hn), _ptr(hun), _ptr(hvn), _ptr(buoy), _ptr(sseo),
-            _ptr(idpdx), _ptr(idpdy)))
+            self.h, ip_method, self._f2(H, "H"), *ps,
+            self._f2(sseo, "sseo", optional=ip_method != SHCHEPETKIN_MCWILLIAMS),
+            self._f3(idpdx, "idpdx"), self._f3(idpdy, "idpdy")))
+
+    def set_substr_ini_press(self, on=True):
+        """gt_set_substr_ini_press: the SUBSTR_INI_PRESS build of
+        do_internal_pressure (internal_pressure.F90:235-258)"""
+        check(self.lib.gt_set_substr_ini_press(self.h, int(on)))
 
     # ---- ghost cells -----------------------------------------------------------
     def update_3d_halo(self, f1, f2, tag=H_TAG, mirror=-1):
-        check(self.lib.gt_update_3d_halo(self.h, _ptr(f1), _ptr(f2), tag, mirror))
+        check(self.lib.gt_update_3d_halo(self.h, self._f3(f1, "f1"), self._f3(f2, "f2"), tag,
+                                         mirror))
 
     def update_2d_halo(self, f1, f2, tag=H_TAG, mirror=-1):
-        check(self.lib.gt_update_2d_halo(self.h, _ptr(f1), _ptr(f2), tag, mirror))
+        check(self.lib.gt_update_2d_halo(self.h, self._f2(f1, "f1"), self._f2(f2, "f2"), tag,
+                                         mirror))
 
     def wait_halo(self, tag=H_TAG):
         check(self.lib.gt_wait_halo(self.h, tag))
@@ -412,17 +521,25 @@ class Context:
         """(mass, volume); f may be an FP64 or FP32 field"""
         import torch
         m, v = C.c_double(), C.c_double()
-        fn = self.lib.gt_budget_f32 if (f is not None and f.dtype == torch.float32) \
-            else self.lib.gt_budget
-        check(fn(self.h, _ptr(h), _ptr(f), C.byref(m), C.byref(v)))
+        f32 = f is not None and f.dtype == torch.float32
+        fn = self.lib.gt_budget_f32 if f32 else self.lib.gt_budget
+        check(fn(self.h, self._f3(h, "h"),
+                 self._f3(f, "f", True, dtype=torch.float32 if f32 else None),
+                 C.byref(m), C.byref(v)))
         return m.value, v.value
 
     # ---- reverse mode -------------------------------------------------------------
+    def _vel(self, uu, vv, ww, hu, hv, ho):
+        return [self._f3(t, nm) for t, nm in ((uu, "uu"), (vv, "vv"), (ww, "ww"), (hu, "hu"),
+                                              (hv, "hv"), (ho, "ho"))]
+
     def adv3d_adjoint(self, nsteps, ckpt_every, dt, f0, lam_final, lam0, uu, vv, ww, hu, hv, ho,
                       hscheme, vscheme, AH=-1.0, tag=H_TAG, f_final=None):
-        check(self.lib.gt_adv3d_adjoint(self.h, nsteps, ckpt_every, dt, _ptr(f0), _ptr(f_final),
-                                        _ptr(lam_final), _ptr(lam0), _ptr(uu), _ptr(vv),
-                                        _ptr(ww), _ptr(hu), _ptr(hv), _ptr(ho), hscheme,
+        check(self.lib.gt_adv3d_adjoint(self.h, nsteps, ckpt_every, dt, self._f3(f0, "f0"),
+                                        self._f3(f_final, "f_final", True),
+                                        self._f3(lam_final, "lam_final"),
+                                        self._f3(lam0, "lam0"),
+                                        *self._vel(uu, vv, ww, hu, hv, ho), hscheme,
                                         vscheme, AH, tag))
 
     def adv3d_adjoint_f32(self, ntracers, nsteps, ckpt_every, dt, f0, lam_final, lam0, uu, vv,
@@ -430,15 +547,11 @@ class Context:
         """gt_adv3d_adjoint_f32: f0, lam_final, lam0 (and f_final) are FP32
         tensors of shape (ntracers, kmax+1, NY, NX)"""
         import torch
-        n3 = (self.kmax + 1) * self.NY * self.NX
-        for t in (f0, lam_final, lam0, f_final):
-            if t is not None and (t.dtype != torch.float32 or t.numel() != ntracers * n3
-                                  or not t.is_contiguous()):
-                raise ValueError("adv3d_adjoint_f32: tracer sets must be contiguous FP32 "
-                                 "tensors of ntracers x (kmax+1) x NY x NX")
-        check(self.lib.gt_adv3d_adjoint_f32(self.h, ntracers, nsteps, ckpt_every, dt, _ptr(f0),
-                                            _ptr(f_final), _ptr(lam_final), _ptr(lam0), _ptr(uu),
-                                            _ptr(vv), _ptr(ww), _ptr(hu), _ptr(hv), _ptr(ho),
+        st = lambda t, nm, o=False: self._set(t, nm, ntracers, torch.float32, o)
+        check(self.lib.gt_adv3d_adjoint_f32(self.h, ntracers, nsteps, ckpt_every, dt,
+                                            st(f0, "f0"), st(f_final, "f_final", True),
+                                            st(lam_final, "lam_final"), st(lam0, "lam0"),
+                                            *self._vel(uu, vv, ww, hu, hv, ho),
                                             hscheme, vscheme, AH, tag))
 
     def adv3d_adjoint_seq(self, ntracers, ckpt_every, dt, f0, lam_final, lam0, steps, hscheme,
@@ -448,37 +561,51 @@ class Context:
         tuple of device tensors per step"""
         import torch
         n = len(steps)
+        for t in range(n):
+            if len(steps[t]) != 6:
+                raise ValueError("adv3d_adjoint_seq: each step needs (uu, vv, ww, hu, hv, ho)")
+            self._vel(*steps[t])
         arrs = []
         for q in range(6):
             a = (C.c_void_p * n)(*[steps[t][q].data_ptr() for t in range(n)])
             arrs.append(a)
         ptrs = [C.cast(a, C.POINTER(C.c_void_p)) for a in arrs]
         self._seq_keep = arrs  # alive until the call returns
-        if f0.dtype == torch.float32:
-            check(self.lib.gt_adv3d_adjoint_seq_f32(self.h, ntracers, n, ckpt_every, dt, _ptr(f0),
-                                                    _ptr(f_final), _ptr(lam_final), _ptr(lam0),
+        f32 = f0 is not None and f0.dtype == torch.float32
+        dt_ = torch.float32 if f32 else torch.float64
+        st = lambda t, nm, o=False: self._set(t, nm, ntracers, dt_, o)
+        if f32:
+            check(self.lib.gt_adv3d_adjoint_seq_f32(self.h, ntracers, n, ckpt_every, dt,
+                                                    st(f0, "f0"), st(f_final, "f_final", True),
+                                                    st(lam_final, "lam_final"), st(lam0, "lam0"),
                                                     *ptrs, hscheme, vscheme, AH, tag))
         else:
             if ntracers != 1:
                 raise ValueError("adv3d_adjoint_seq: FP64 takes one tracer")
-            check(self.lib.gt_adv3d_adjoint_seq(self.h, n, ckpt_every, dt, _ptr(f0), _ptr(f_final),
-                                                _ptr(lam_final), _ptr(lam0), *ptrs, hscheme,
-                                                vscheme, AH, tag))
+            check(self.lib.gt_adv3d_adjoint_seq(self.h, n, ckpt_every, dt, st(f0, "f0"),
+                                                st(f_final, "f_final", True),
+                                                st(lam_final, "lam_final"), st(lam0, "lam0"),
+                                                *ptrs, hscheme, vscheme, AH, tag))
 
     def ip_blumberg_mellor_adjoint(self, hun, hvn, zz, idpdx_bar, idpdy_bar, buoy_bar):
-        check(self.lib.gt_ip_blumberg_mellor_adjoint(self.h, _ptr(hun), _ptr(hvn), _ptr(zz),
-                                                     _ptr(idpdx_bar), _ptr(idpdy_bar),
-                                                     _ptr(buoy_bar)))
+        ps = [self._f3(t, nm) for t, nm in ((hun, "hun"), (hvn, "hvn"), (zz, "zz"),
+                                            (idpdx_bar, "idpdx_bar"), (idpdy_bar, "idpdy_bar"),
+                                            (buoy_bar, "buoy_bar"))]
+        check(self.lib.gt_ip_blumberg_mellor_adjoint(self.h, *ps))
 
     def ip_shchepetkin_mcwilliams_adjoint(self, hn, hun, hvn, buoy, sseo, zz, idpdx_bar,
                                           idpdy_bar, buoy_bar):
-        check(self.lib.gt_ip_shchepetkin_mcwilliams_adjoint(
-            self.h, _ptr(hn), _ptr(hun), _ptr(hvn), _ptr(buoy), _ptr(sseo), _ptr(zz),
-            _ptr(idpdx_bar), _ptr(idpdy_bar), _ptr(buoy_bar)))
+        ps = [self._f3(t, nm) for t, nm in ((hn, "hn"), (hun, "hun"), (hvn, "hvn"),
+                                            (buoy, "buoy"))]
+        qs = [self._f3(t, nm) for t, nm in ((zz, "zz"), (idpdx_bar, "idpdx_bar"),
+                                            (idpdy_bar, "idpdy_bar"), (buoy_bar, "buoy_bar"))]
+        check(self.lib.gt_ip_shchepetkin_mcwilliams_adjoint(self.h, *ps, self._f2(sseo, "sseo"),
+                                                            *qs))
 
     def eos_linear_adjoint(self, buoy_bar, T_bar, S_bar, dtr0, dsr0):
-        check(self.lib.gt_eos_linear_adjoint(self.h, _ptr(buoy_bar), _ptr(T_bar), _ptr(S_bar),
-                                             dtr0, dsr0))
+        check(self.lib.gt_eos_linear_adjoint(self.h, self._f3(buoy_bar, "buoy_bar"),
+                                             self._f3(T_bar, "T_bar", True),
+                                             self._f3(S_bar, "S_bar", True), dtr0, dsr0))
 
     def set_tuning(self, fused=1, tile=0, group=3):
         check(self.lib.gt_set_tuning(self.h, int(fused), int(tile), int(group)))
@@ -490,9 +617,17 @@ class Context:
 
 
 def exchange_tiles(px, py, ni, nj, nk, tiles, stream=None):
-    """gt_exchange_tiles on a list of device tensors"""
+    """gt_exchange_tiles on a list of device tensors: tile t = ty*px + tx has
+    shape (>= nk, nj[t]+4, ni[t]+4)"""
     lib = load_library()
     n = px * py
+    if len(tiles) != n or len(ni) != n or len(nj) != n:
+        raise ValueError("exchange_tiles: need px*py tiles and sizes")
+    for q, t in enumerate(tiles):
+        _dev(t, f"tile {q}")
+        if t.ndim != 3 or t.shape[1:] != (nj[q] + 4, ni[q] + 4) or t.shape[0] < nk:
+            raise ValueError(f"exchange_tiles: tile {q} has shape {tuple(t.shape)}, needs "
+                             f"(>= {nk}, {nj[q] + 4}, {ni[q] + 4})")
     ni_a = (C.c_int * n)(*ni)
     nj_a = (C.c_int * n)(*nj)
     tp = (C.c_void_p * n)(*[t.data_ptr() for t in tiles])
@@ -501,13 +636,34 @@ def exchange_tiles(px, py, ni, nj, nk, tiles, stream=None):
                                 C.c_void_p(handle) if handle else None))
 
 
-def _batch_array(domains):
+def _batch_array(domains, ntracers):
+    """gt_batch_domain[] from dicts, each tensor checked against its
+    context: one tracer dtype for the whole batch, f_in/f_out of shape
+    (ntracers, kmax+1, NY, NX), FP64 velocities and heights, one context per
+    domain"""
+    import torch
+    if not domains:
+        raise ValueError("adv3d_batch: no domains")
+    dtype = domains[0]["f_in"].dtype
+    if dtype not in (torch.float32, torch.float64):
+        raise ValueError(f"adv3d_batch: tracers must be FP32 or FP64 (got {dtype})")
+    seen = set()
     n = len(domains)
     arr = (BatchDomain * n)()
     for q, d in enumerate(domains):
-        arr[q] = BatchDomain(d["ctx"].h.value, d["f_in"].data_ptr(), d["f_out"].data_ptr(),
-                             *(d[k].data_ptr() for k in ("uu", "vv", "ww", "hu", "hv", "ho")),
-                             d["hires"].data_ptr() if d.get("hires") is not None else None)
+        ctx = d["ctx"]
+        if id(ctx) in seen:
+            raise ValueError(f"adv3d_batch: domain {q} repeats a context (one per domain)")
+        seen.add(id(ctx))
+        shp = (ntracers,) + ctx.shape3
+        for key in ("f_in", "f_out"):
+            _dev(d[key], f"domain {q} {key}", dtype=dtype, numel=int(np.prod(shp)))
+        if d["f_in"].data_ptr() == d["f_out"].data_ptr():
+            raise ValueError(f"adv3d_batch: domain {q} f_in and f_out must differ")
+        vel = [ctx._f3(d[k], f"domain {q} {k}") for k in ("uu", "vv", "ww", "hu", "hv", "ho")]
+        hires = ctx._f3(d.get("hires"), f"domain {q} hires", True)
+        arr[q] = BatchDomain(ctx.h.value, d["f_in"].data_ptr(), d["f_out"].data_ptr(),
+                             *(v.value for v in vel), hires.value if hires is not None else None)
     return arr
 
 
@@ -539,7 +695,7 @@ def autotune_batch(domains, ntracers, dt, hscheme, vscheme, AH=-1.0, tag=H_TAG, 
     is a strip-height index (strip_rows), group the tracers per wavefront"""
     import torch
     lib = load_library()
-    arr = _batch_array(domains)
+    arr = _batch_array(domains, ntracers)
     f32 = domains[0]["f_in"].dtype == torch.float32
     bt, bg, bm = C.c_int(), C.c_int(), C.c_float()
     table = (C.c_float * 36)()
@@ -558,7 +714,7 @@ def adv3d_batch(domains, ntracers, dt, hscheme, vscheme, AH=-1.0, tag=H_TAG, str
     import torch
     lib = load_library()
     n = len(domains)
-    arr = _batch_array(domains)
+    arr = _batch_array(domains, ntracers)
     f32 = domains[0]["f_in"].dtype == torch.float32
     fn = lib.gt_adv3d_batch_f32 if f32 else lib.gt_adv3d_batch_f64
     handle = getattr(stream, "cuda_stream", stream)

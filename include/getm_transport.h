@@ -282,6 +282,13 @@ int gt_do_internal_pressure(gt_ctx *ctx, int ip_method, const double *H,
                             const double *hn, const double *hun,
                             const double *hvn, const double *buoy,
                             const double *sseo, double *idpdx, double *idpdy);
+/* the SUBSTR_INI_PRESS build of do_internal_pressure (internal_pressure.F90:
+ * 132-141,235-258), as a context switch instead of a compile option: with
+ * on = 1 the next gt_do_internal_pressure keeps its gradients (interior,
+ * all levels) and returns zero there; later calls subtract those first
+ * gradients.  Switching on allocates the two stored fields and restarts the
+ * first-call state; 0 switches it off. */
+int gt_set_substr_ini_press(gt_ctx *ctx, int on);
 
 /* ---- ghost cells --------------------------------------------------------- */
 /* update_3d_halo, src/futils/halo_zones.F90:141-187, serial semantics:
@@ -298,8 +305,9 @@ int gt_wait_halo(gt_ctx *ctx, int tag);
  * corners, levels 0..nk-1) between the px*py tiles of one decomposed domain
  * held in one device memory.  tiles[t] (device pointer, t = ty*px+tx) has
  * interior ni[t] x nj[t]; tiles in one row share nj, in one column share ni.
- * Edges without a neighbour are left unchanged (MPI_PROC_NULL).  Synchronous
- * on the given stream (NULL = null stream). */
+ * Edges without a neighbour are left unchanged (MPI_PROC_NULL).  At most 64
+ * tiles.  Enqueued on the given stream (NULL = null stream); allocates
+ * nothing and returns without waiting. */
 int gt_exchange_tiles(int px, int py, const int *ni, const int *nj, int nk,
                       double *const *tiles, void *hip_stream);
 
@@ -312,7 +320,10 @@ int gt_exchange_tiles(int px, int py, const int *ni, const int *nj, int nk,
  * (optional) receives the end of the layer-height trajectory.
  * _f32: tracer and flux arithmetic in FP32, the update (Dio*f-dt*advn)/Di in
  * FP64.  _f64: the reference arithmetic, bit-identical to do_advection_3d.
- * Vertical sub-cycling is not applied (adv_ver_iterations = 1). */
+ * Vertical sub-cycling is not applied (adv_ver_iterations = 1).  Each
+ * domain needs its own context (the kernels' work fields live in it; a
+ * context given twice is rejected), and those work fields are shared with
+ * the context's own stream: order the two streams when both are used. */
 typedef struct {
   gt_ctx *ctx;
   const void *f_in;
@@ -346,6 +357,15 @@ int gt_set_tuning(gt_ctx *ctx, int fused, int tile, int group);
  * free device memory.  Results are the same bit for bit.  No counterpart in
  * the reference (its reverse mode is this build's extension). */
 int gt_set_adjoint_mode(gt_ctx *ctx, int keep_states);
+/* sizes the context's work fields for the transport step up front (the
+ * HALFSPLIT step with advres, batched steps of up to ntracers_f32 FP32 or
+ * ntracers_f64 FP64 tracers, NOSPLIT/FULLSPLIT/HVSPLIT buffers, SMcW's P,
+ * the budget partials, and the sub-cycling columns if adv_ver_iterations >
+ * 1), so later calls of those entry points neither allocate nor synchronise
+ * the device to grow a field.  Without it they allocate on first use.  The
+ * work memory of the transport step replaces the reference's automatic
+ * arrays (do_advection_3d fi/hi/adv, SMcW dR/dZ/P). */
+int gt_reserve(gt_ctx *ctx, int ntracers_f32, int ntracers_f64);
 /* autotuner: times gt_adv3d_batch_f32 (f32=1) or _f64 over the nine strip
  * height pairs (tile 3..11) x tracers per wavefront {1, 3, 6, 9} (FP64:
  * {1, 2}), reps timed launches each, keeps the fastest on every context.

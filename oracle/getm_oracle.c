@@ -1237,6 +1237,29 @@ void or_ip_shchepetkin_mcwilliams(const or_dom *d, const int32_t *az,
 #undef BU
 }
 
+/* ---- a11: the SUBSTR_INI_PRESS part of do_internal_pressure,
+ * internal_pressure.F90:235-258.  *first is the routine's SAVE'd flag. */
+void or_substr_ini_press(const or_dom *d, int *first, double *idpdx,
+                         double *idpdy, double *idpdx0, double *idpdy0)
+{
+    DOMVARS(d);
+    for (int k = 0; k <= kmax; k++)
+        for (int j = jl; j <= jh; j++)
+            for (int i = il; i <= ih; i++) {
+                long c = IJK(i, j, k);
+                if (*first) {
+                    idpdx0[c] = idpdx[c];
+                    idpdx[c] = 0.0;
+                    idpdy0[c] = idpdy[c];
+                    idpdy[c] = 0.0;
+                } else {
+                    idpdx[c] = idpdx[c] - idpdx0[c];
+                    idpdy[c] = idpdy[c] - idpdy0[c];
+                }
+            }
+    *first = 0;
+}
+
 /* ---- a13: halo_zones.F90:81-187, serial ------------------------------- */
 void or_update_3d_halo(const or_dom *d, double *f1, const double *f2,
                        int mirror)

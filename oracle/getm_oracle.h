@@ -137,6 +137,11 @@ void or_ip_shchepetkin_mcwilliams(const or_dom *d, const int32_t *az,
                                   const double *buoy, const double *sseo,
                                   double *zz, double *idpdx, double *idpdy);
 
+/* a11: SUBSTR_INI_PRESS, internal_pressure.F90:235-258 (first: the SAVE'd
+ * first-call flag, cleared on return) */
+void or_substr_ini_press(const or_dom *d, int *first, double *idpdx,
+                         double *idpdy, double *idpdx0, double *idpdy0);
+
 /* a13: halo_zones.F90:81-187 serial semantics.  do_mirror_state carries the
  * SAVE'd flag of update_2d_halo (pass mirror = -1 for "absent"). */
 void or_update_3d_halo(const or_dom *d, double *f1, const double *f2,

@@ -70,6 +70,7 @@ def lib():
             "or_ip_blumberg_mellor": (None, [D, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
             "or_ip_shchepetkin_mcwilliams": (None, [D, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                                     vp, vp]),
+            "or_substr_ini_press": (None, [D, C.POINTER(i), vp, vp, vp, vp]),
             "or_update_3d_halo": (None, [D, vp, vp, i]),
             "or_update_2d_halo": (None, [D, vp, vp, i, C.POINTER(C.c_int)]),
             "or_exchange_tiles": (None, [i, i, vp, vp, i, vp]),
@@ -283,6 +284,25 @@ class Oracle:
         lib().or_ip_shchepetkin_mcwilliams(C.byref(self.dom), P(self.az), P(self.au),
                                            P(self.av), P(H), P(hn), P(hun), P(hvn), P(buoy),
                                            P(sseo), P(zz), P(idpdx), P(idpdy))
+
+    def do_internal_pressure(self, ip_method, H, hn, hun, hvn, buoy, sseo, zz, idpdx, idpdy,
+                             substr=None):
+        """do_internal_pressure (internal_pressure.F90:181-274) for methods 1
+        and 6; substr: None, or a dict holding the SUBSTR_INI_PRESS state
+        (keys first, idpdx0, idpdy0; created on the first call)"""
+        if ip_method == 1:
+            self.ip_blumberg_mellor(H, hn, hun, hvn, buoy, zz, idpdx, idpdy)
+        elif ip_method == 6:
+            self.ip_shchepetkin_mcwilliams(H, hn, hun, hvn, buoy, sseo, zz, idpdx, idpdy)
+        else:
+            raise ValueError(f"ip_method {ip_method} is not restated")
+        if substr is not None:
+            if "first" not in substr:
+                substr.update(first=1, idpdx0=np.zeros_like(idpdx), idpdy0=np.zeros_like(idpdy))
+            fl = C.c_int(substr["first"])
+            lib().or_substr_ini_press(C.byref(self.dom), C.byref(fl), P(idpdx), P(idpdy),
+                                      P(substr["idpdx0"]), P(substr["idpdy0"]))
+            substr["first"] = fl.value
 
     def update_3d_halo(self, f1, f2, mirror=-1):
         lib().or_update_3d_halo(C.byref(self.dom), P(f1), P(f2), mirror)

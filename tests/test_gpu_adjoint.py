@@ -114,54 +114,102 @@ def test_multistep_checkpointed_gradient_vs_fd(device):
     assert ok >= 40
 
 
+def _fd_cells(state, lam, gh, cells, eps, what):
+    """central FD of <lam, state(x)> at each cell against the adjoint gh.
+    The states are differenced before the sum with lam (differencing two
+    values of J would lose ~1e-16 |J| / eps).  A cell whose one-sided slopes
+    disagree by more than the bound sits within eps of a limiter kink and is
+    counted apart.  Bound per cell: |ad - fd| <= 1e-6 |fd| + floor, where
+    floor = 8 ulp(max|lam| max|state|) sqrt(n) / eps bounds the rounding of
+    the difference quotient itself (n cells in the sum, rounding errors of
+    the two perturbed states independent from cell to cell)."""
+    import torch
+    x0 = state.base
+    s0 = state(x0)
+    n = int((lam != 0).sum())
+    floor = 8 * 2.2e-16 * float(lam.abs().max()) * float(s0.abs().max()) * n ** 0.5 / eps
+    scale = float(np.abs(gh).max())
+    rows = []
+    kinks = 0
+    for (k, j, i) in cells:
+        e = torch.zeros_like(x0)
+        e[k, j, i] = eps
+        sp, sm = state(x0 + e), state(x0 - e)
+        fwd = float(torch.sum(lam * (sp - s0))) / eps
+        bwd = float(torch.sum(lam * (s0 - sm))) / eps
+        fd = float(torch.sum(lam * (sp - sm))) / (2 * eps)
+        ad = float(gh[k, j, i])
+        bound = 1e-6 * abs(fd) + floor
+        if abs(fwd - bwd) > 2 * bound:
+            kinks += 1
+            continue
+        rows.append((abs(ad - fd) / bound, abs(ad - fd), abs(fd), (k, j, i), ad, fd))
+    rows.sort(reverse=True)
+    print(f"{what}: {len(rows)} cells compared, {kinks} within eps of a kink; floor "
+          f"{floor:.2e} ({floor / scale:.1e} of max|grad| {scale:.3e})")
+    for r in rows[:5]:
+        print(f"  err/bound {r[0]:.3f}  |ad-fd| {r[1]:.2e}  |fd| {r[2]:.2e}  cell {r[3]}")
+    bad = [r for r in rows if r[0] > 1.0]
+    assert not bad, (what, bad[:5])
+    return len(rows)
+
+
 def test_adjoint_case_full_size_vs_fd(device):
     """The reverse-mode workload at full size (256x256x30 sigma, Superbee
-    HALFSPLIT, 48 steps, checkpoint every 8): dJ/dc0 for J = sum over a 32x32
-    box (all k) of c^2 at the final step, against central FD (eps 1e-6) of
-    J's linearisation at 24 seeded wet cells of the box's domain of
-    dependence, to 1e-6 of the largest gradient entry (kinks as in
-    test_multistep_checkpointed_gradient_vs_fd)."""
+    HALFSPLIT, 48 steps, checkpoint every 8).  dJ/dc0 for J = sum over a
+    32x32 box (all k) of c^2 at the final step, and dJ/dT0 for J = sum over
+    a box of w * idpdx, Blumberg-Mellor of the linear-EOS buoyancy of the
+    advected T: each against central FD (eps 1e-6) at 64 seeded wet cells of
+    the box's domain of dependence, per cell (see _fd_cells)."""
     tr = inputs.adjoint_case()
     d = tr.dom
     ctx = gt.Context.from_domain(d)
     fields = tuple(to_dev(a, device) for a in (tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho, tr.hn))
+    uu, vv, ww, hu, hv, ho, hn = fields
     nsteps, ck = 48, 8
+    rng = np.random.default_rng(23)
+
+    def cells(j0, i0, n=64):
+        out = []
+        while len(out) < n:
+            j = int(rng.integers(j0 - 2, j0 + 34))
+            i = int(rng.integers(i0 - 2, i0 + 34))
+            k = int(rng.integers(1, d.kmax + 1))
+            if d.az[j, i] == 1:
+                out.append((k, j, i))
+        return out
+
+    def advect(x):
+        return AdvectSteps.apply(x, ctx, fields, nsteps, ck, gt.SUPERBEE, gt.SUPERBEE, tr.dt)
+
+    # ---- dJ/dc0
     c0 = to_dev(tr.tracers["c"], device)
     j0, i0 = d.NY // 2 - 16, d.NX // 2 + 24
-    box = (j0, j0 + 32, i0, i0 + 32)
     c0r = c0.clone().requires_grad_(True)
-    out = AdvectSteps.apply(c0r, ctx, fields, nsteps, ck, gt.SUPERBEE, gt.SUPERBEE, tr.dt)
-    (g,) = torch.autograd.grad(box_cost(out, box), c0r)
+    out = advect(c0r)
+    (g,) = torch.autograd.grad(box_cost(out, (j0, j0 + 32, i0, i0 + 32)), c0r)
     lam = torch.zeros_like(c0)
     lam[1:, j0:j0 + 32, i0:i0 + 32] = 2.0 * out.detach()[1:, j0:j0 + 32, i0:i0 + 32]
-    # J sums ~3e4 cells, so the states are differenced before the sum with
-    # lam (differencing two values of J would lose ~1e-16*|J|/eps)
-    state = lambda c: AdvectSteps.apply(  # noqa: E731
-        c, ctx, fields, nsteps, ck, gt.SUPERBEE, gt.SUPERBEE, tr.dt)
-    gh = g.cpu().numpy()
-    rng = np.random.default_rng(23)
-    s0 = state(c0)
-    scale = float(np.abs(gh).max())
-    assert scale > 0
-    eps = 1e-6
-    ok = kinks = 0
-    for _ in range(24):
-        j = int(rng.integers(j0 - 1, j0 + 33))
-        i = int(rng.integers(i0 - 1, i0 + 33))
-        k = int(rng.integers(1, d.kmax + 1))
-        e = torch.zeros_like(c0)
-        e[k, j, i] = eps
-        sp, sm = state(c0 + e), state(c0 - e)
-        fwd = float(torch.sum(lam * (sp - s0))) / eps
-        bwd = float(torch.sum(lam * (s0 - sm))) / eps
-        if abs(fwd - bwd) > 1e-6 * scale:
-            kinks += 1
-            continue
-        fd = float(torch.sum(lam * (sp - sm))) / (2 * eps)
-        assert abs(gh[k, j, i] - fd) <= 1e-6 * scale, (k, j, i, gh[k, j, i], fd)
-        ok += 1
-    print(f"full-size adjoint vs FD: {ok} cells agree, {kinks} within eps of a kink")
-    assert ok >= 16
+    advect.base = c0
+    ok = _fd_cells(advect, lam, g.cpu().numpy(), cells(j0, i0), 1e-6, "dJ/dc0")
+    assert ok >= 56
+
+    # ---- dJ/dT0 through the advection, the linear EOS and Blumberg-Mellor
+    T0 = to_dev(tr.tracers["T"], device)
+    S0 = to_dev(tr.tracers["S"], device)
+    H = to_dev(d.H, device)
+    jb, ib = d.NY // 2 - 16, d.NX // 2 - 16
+    w = torch.zeros_like(T0)
+    w[1:, jb:jb + 32, ib:ib + 32] = to_dev(inputs.uniform((d.kmax, 32, 32), 17, -1, 1), device)
+
+    def pressure(T):
+        ix, _ = PressureFromT.apply(advect(T), S0, ctx, H, hn, hu, hv)
+        return ix
+    T0r = T0.clone().requires_grad_(True)
+    (gT,) = torch.autograd.grad(torch.sum(w * pressure(T0r)), T0r)
+    pressure.base = T0
+    ok = _fd_cells(pressure, w, gT.cpu().numpy(), cells(jb, ib), 1e-6, "dJ/dT0")
+    assert ok >= 56
 
 
 def test_dot_product_smooth_superbee(device):

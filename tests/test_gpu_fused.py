@@ -121,12 +121,19 @@ def normwise(a, b):
     return float(d.max() / scale), float(np.sqrt(np.mean(d ** 2)) / scale)
 
 
-@pytest.mark.parametrize("tile,group", [(0, 1), (1, 3), (2, 6)])
-def test_batch_f32_multidomain(device, tile, group):
+@pytest.mark.parametrize("rows,group,npassive", [
+    ((16, 16), 1, 4), ((32, 32), 3, 4), ((8, 8), 6, 4),
+    ((8, 16), 9, 7),      # the bench's tuning: 9 tracers per wavefront, 8/16 rows
+    ((8, 32), 9, 9),      # 11 tracers: a full group of 9 and a partial one of 2
+    ((16, 8), 3, 5),      # 7 tracers: groups of 3, 3 and 1
+    ((32, 16), 6, 6)])    # 8 tracers: groups of 6 and 2
+def test_batch_f32_multidomain(device, rows, group, npassive):
     """T, S and passive tracers on uneven subdomains in one launch, FP32
-    against the FP64 oracle over several P2-PDM steps."""
+    against the FP64 oracle over several P2-PDM steps, for every strip-height
+    pair and tracer group the tuner can pick, full and partial last groups."""
+    tile = gt.tile_for_rows(*rows)
     sizes = [(24, 20), (71, 33), (30, 52)]
-    subs = inputs.multitracer_case(seed=9, kmax=8, sizes=sizes, npassive=4)
+    subs = inputs.multitracer_case(seed=9, kmax=8, sizes=sizes, npassive=npassive)
     names = list(subs[0].tracers)
     nt = len(names)
     doms, refs, oracles, tins, touts, ctxs = [], [], [], [], [], []
@@ -169,3 +176,62 @@ def test_batch_f32_multidomain(device, tile, group):
             m1 = ctxs[s].budget(to_dev(tr.hn, device), tins[s][q])[0]
             mass0 = m0[s][q][0]
             assert abs(m1 - mass0) <= 1e-6 * abs(mass0), (s, n, mass0, m1)
+
+
+@pytest.mark.timeout(600)
+def test_batch_f32_multitracer_workload_full_size(device):
+    """the batched workload itself: T, S and 16 passive tracers on the eight
+    subdomains 64x64 ... 512x384 with 30 levels, at the tuning the bench
+    uses (9 tracers per wavefront, 8/16 rows) and at the default, two P2-PDM
+    steps against the FP64 oracle: max and RMS error per field normalised
+    by the field's largest value <= 1e-5, and each field's FP64 mass budget
+    conserved to 1e-6 (the layer heights change: hires != ho)"""
+    from oracle import pyoracle as po_
+    subs = inputs.multitracer_case(seed=3, kmax=30)
+    names = list(subs[0].tracers)
+    nt = len(names)
+    assert nt == 18 and len(subs) == 8
+    po_.set_threads(16)
+    refs = [np.stack([tr.tracers[n] for n in names]) for tr in subs]
+    nsteps = 2
+    for s, tr in enumerate(subs):
+        orc = po_.Oracle.from_domain(tr.dom)
+        host = [tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho, tr.hn]
+        for _ in range(nsteps):
+            for q in range(nt):
+                orc.do_advection_3d(tr.dt, refs[s][q], *host, gt.HALFSPLIT, gt.P2_PDM,
+                                    gt.P2_PDM, -1.0, gt.H_TAG)
+    for rows, group in (((8, 16), 9), ((16, 16), 3)):
+        ctxs, doms, tins, touts = [], [], [], []
+        for tr in subs:
+            ctx = gt.Context.from_domain(tr.dom)
+            ctxs.append(ctx)
+            tin = to_dev(np.stack([tr.tracers[n] for n in names]).astype(np.float32), device)
+            tins.append(tin)
+            touts.append(torch.empty_like(tin))
+            host = [tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho]
+            doms.append(dict(ctx=ctx, **dict(zip(NAMES[:6], [to_dev(a, device)
+                                                             for a in host]))))
+        ctxs[0].set_tuning(fused=1, tile=gt.tile_for_rows(*rows), group=group)
+        m0 = [[ctxs[s].budget(to_dev(subs[s].ho, device), tins[s][q])[0] for q in range(nt)]
+              for s in range(len(subs))]
+        for _ in range(nsteps):
+            for s in range(len(subs)):
+                doms[s]["f_in"], doms[s]["f_out"] = tins[s], touts[s]
+                doms[s]["hires"] = torch.empty_like(doms[s]["ho"])
+            gt.adv3d_batch(doms, nt, subs[0].dt, gt.P2_PDM, gt.P2_PDM)
+            tins, touts = touts, tins
+        worst = (0.0, None)
+        for s, tr in enumerate(subs):
+            wet = tr.dom.az == 1
+            got = to_host(tins[s]).astype(np.float64)
+            hires = doms[s]["hires"]
+            for q, n in enumerate(names):
+                emax, erms = normwise(got[q][1:, wet], refs[s][q][1:, wet])
+                print(f"rows {rows} group {group} domain {s} {n}: max {emax:.2e} "
+                      f"rms {erms:.2e}")
+                assert emax <= 1e-5 and erms <= 1e-6, (rows, group, s, n, emax, erms)
+                worst = max(worst, (emax, (s, n)))
+                m1 = ctxs[s].budget(hires, tins[s][q])[0]
+                assert abs(m1 - m0[s][q]) <= 1e-6 * abs(m0[s][q]), (s, n, m0[s][q], m1)
+        print(f"rows {rows} group {group}: worst field {worst}")

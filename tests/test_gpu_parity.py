@@ -370,3 +370,162 @@ def test_config2_full_size_properties(device):
     ctx.do_advection_3d(tr.dt, c, *dn, gt.HALFSPLIT, gt.P2_PDM, gt.P2_PDM, -1.0, gt.H_TAG)
     cc = to_host(c)
     assert np.abs(cc[1:, wet] - 2.5).max() <= 1e-12
+
+
+@pytest.mark.parametrize("substr", [False, True])
+def test_do_internal_pressure_dispatch(device, substr):
+    """gt_do_internal_pressure (internal_pressure.F90:181-274): methods 1 and 6
+    through the dispatch with the context's own zz, with and without
+    SUBSTR_INI_PRESS (:235-258), over three calls whose buoyancy changes;
+    bit-exact against the oracle (outputs start from the same values, since
+    the routines leave cells outside their loops alone).  Methods outside
+    this build give GT_ENOTIMPL, unknown ones 'Not valid ip_method
+    specified'."""
+    import torch
+    tr, d, T, S, sseo = ip_fields(imax=70, jmax=40, kmax=8)
+    consts = (inputs.EOS_T0, inputs.EOS_S0, inputs.EOS_DTR0, inputs.EOS_DSR0)
+    dH, dhn, dhun, dhvn = (to_dev(a, device) for a in (d.H, tr.hn, tr.hun, tr.hvn))
+    dsse = to_dev(sseo, device)
+    for method in (gt.BLUMBERG_MELLOR, gt.SHCHEPETKIN_MCWILLIAMS):
+        ctx = gt.Context.from_domain(d)
+        orc = po.Oracle.from_domain(d)
+        if substr:
+            ctx.set_substr_ini_press(True)
+        state = {} if substr else None
+        zz = np.zeros(d.shape3)
+        for q in range(3):
+            Tq = np.where(T > -9000, T + 0.7 * q * inputs.uniform(T.shape, 500 + q, -1, 1), T)
+            rho, buoy = np.zeros(d.shape3), np.zeros(d.shape3)
+            orc.eos_linear(Tq, S, rho, buoy, *consts)
+            ix, iy = np.full(d.shape3, 7.0), np.full(d.shape3, 7.0)
+            orc.do_internal_pressure(method, d.H, tr.hn, tr.hun, tr.hvn, buoy, sseo, zz, ix, iy,
+                                     substr=state)
+            tx = torch.full(d.shape3, 7.0, dtype=torch.float64, device=device)
+            ty = torch.full_like(tx, 7.0)
+            ctx.do_internal_pressure(method, dH, dhn, dhun, dhvn, to_dev(buoy, device),
+                                     dsse, tx, ty)
+            assert np.array_equal(to_host(tx), ix), (method, substr, q)
+            assert np.array_equal(to_host(ty), iy), (method, substr, q)
+            if substr and q == 0:
+                assert np.all(ix[:, 2:-2, 2:-2] == 0.0)
+            if q > 0:
+                assert np.abs(ix[1:, 2:-2, 2:-2]).max() > 0
+    ctx = gt.Context.from_domain(d)
+    z = torch.zeros(d.shape3, dtype=torch.float64, device=device)
+    for m, code in ((2, 8), (3, 8), (4, 8), (5, 8), (7, 8), (0, 3), (8, 3), (-1, 3)):
+        with pytest.raises(gt.GtError) as e:
+            ctx.do_internal_pressure(m, dH, dhn, dhun, dhvn, z, dsse, z, z)
+        assert e.value.code == code, (m, e.value.code)
+
+
+def test_context_derives_uvx_masks_from_az(device):
+    """gt_create with au/av/ax NULL derives them as domain.F90:294-346 does;
+    the step then equals the oracle's (masks given) bit for bit, on a basin
+    with islands and open-boundary cells (az = 2)"""
+    dom = random_domain(41, 23, 6, 13, "islands")
+    az = dom.az.copy()
+    az[5:9, 2] = 2
+    au, av, ax = inputs.masks_from_az(az)
+    d = inputs.Domain(dom.imax, dom.jmax, dom.kmax, dom.dx, dom.dy, dom.H, dom.HU, dom.HV, az,
+                      au, av, ax)
+    F = random_fields(d, 700)
+    ctx = gt.Context(d.imax, d.jmax, d.kmax, d.dx, d.dy, az)       # masks derived in C
+    orc = po.Oracle.from_domain(d)
+    host = [F[k] for k in ("uu", "vv", "ww", "hu", "hv", "ho", "hn")]
+    dv = [to_dev(a, device) for a in host]
+    for split, hs, vs in ((gt.HALFSPLIT, gt.P2_PDM, gt.P2_PDM), (gt.NOSPLIT, gt.SUPERBEE,
+                                                                   gt.MUSCL)):
+        for tag in (gt.H_TAG, gt.U_TAG, gt.V_TAG):
+            f = F["f"].copy()
+            tf = to_dev(f, device)
+            orc.do_advection_3d(F["dt"], f, *host, split, hs, vs, -1.0, tag)
+            ctx.do_advection_3d(F["dt"], tf, *dv, split, hs, vs, -1.0, tag)
+            assert np.array_equal(to_host(tf), f), (split, tag)
+
+
+def test_batch_rejects_a_shared_context(device):
+    """two batch entries on one context would share its work fields: the
+    C-ABI refuses with GT_EARG (the Python binding refuses earlier)"""
+    import ctypes as C
+    import torch
+    dom = random_domain(20, 12, 4, 3, "walls")
+    c1, c2 = gt.Context.from_domain(dom), gt.Context.from_domain(dom)
+    F = random_fields(dom, 40)
+    dv = {k: to_dev(F[k], device) for k in ("uu", "vv", "ww", "hu", "hv", "ho")}
+    mk = lambda c: dict(ctx=c, f_in=torch.zeros((2,) + dom.shape3, dtype=torch.float32,
+                                                device=device),
+                        f_out=torch.zeros((2,) + dom.shape3, dtype=torch.float32,
+                                          device=device), **dv)
+    doms = [mk(c1), mk(c2)]
+    arr = gt._batch_array(doms, 2)
+    arr[1].ctx = c1.h.value
+    lib = gt.load_library()
+    rc = lib.gt_adv3d_batch_f32(2, arr, 2, 30.0, gt.P2_PDM, gt.P2_PDM, -1.0, gt.H_TAG, None)
+    assert rc == 7, rc
+    assert b"share one context" in lib.gt_last_error()
+    with pytest.raises(ValueError):
+        gt.adv3d_batch([mk(c1), mk(c1)], 2, 30.0, gt.P2_PDM, gt.P2_PDM)
+
+
+def test_reserve_sizes_the_work_fields(device):
+    """after gt_reserve the step runs as before (bit-exact with the oracle)"""
+    dom = random_domain(33, 21, 6, 5, "open")
+    F = random_fields(dom, 900)
+    ctx = gt.Context.from_domain(dom)
+    ctx.set_adv_ver_iterations(3)
+    ctx.reserve(18, 2)
+    orc = po.Oracle.from_domain(dom)
+    host = [F[k] for k in ("uu", "vv", "ww", "hu", "hv", "ho", "hn")]
+    dv = [to_dev(a, device) for a in host]
+    for split in (gt.HALFSPLIT, gt.NOSPLIT, gt.FULLSPLIT, gt.HVSPLIT):
+        f = F["f"].copy()
+        tf = to_dev(f, device)
+        orc.do_advection_3d(F["dt"], f, *host, split, gt.SUPERBEE, gt.SUPERBEE, -1.0, gt.H_TAG,
+                            adv_ver_iterations=3)
+        ctx.do_advection_3d(F["dt"], tf, *dv, split, gt.SUPERBEE, gt.SUPERBEE, -1.0, gt.H_TAG)
+        assert np.array_equal(to_host(tf), f), split
+
+
+@pytest.mark.timeout(600)
+def test_config2_full_size_20_steps_against_oracle(device):
+    """The T/S workload at full size (400x400x40 GVC): 20 steps of linear EOS,
+    Blumberg-Mellor, Shchepetkin-McWilliams and P2-PDM HALFSPLIT advection of
+    T and S, on the GPU and in the oracle: T, S and the four pressure
+    gradients of the last step bit for bit (bound 1e-12 per cell)"""
+    import torch
+    po.set_threads(16)
+    tr = inputs.ts_case()
+    d = tr.dom
+    sseo = np.zeros((d.NY, d.NX))
+    ctx = gt.Context.from_domain(d)
+    orc = po.Oracle.from_domain(d)
+    host = [tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho, tr.hn]
+    dv = [to_dev(a, device) for a in host]
+    consts = (inputs.EOS_T0, inputs.EOS_S0, inputs.EOS_DTR0, inputs.EOS_DSR0)
+    T, S = tr.tracers["T"].copy(), tr.tracers["S"].copy()
+    tT, tS = to_dev(T, device), to_dev(S, device)
+    z3 = lambda: np.zeros(d.shape3)
+    rho, buoy = z3(), z3()
+    out = {k: z3() for k in ("zb", "bx", "by", "zs", "sx", "sy")}
+    tb = torch.zeros(d.shape3, dtype=torch.float64, device=device)
+    tout = {k: torch.zeros_like(tb) for k in out}
+    dH, dsse = to_dev(d.H, device), to_dev(sseo, device)
+    for step in range(20):
+        orc.eos_linear(T, S, rho, buoy, *consts)
+        orc.ip_blumberg_mellor(d.H, tr.hn, tr.hun, tr.hvn, buoy, out["zb"], out["bx"], out["by"])
+        orc.ip_shchepetkin_mcwilliams(d.H, tr.hn, tr.hun, tr.hvn, buoy, sseo, out["zs"],
+                                      out["sx"], out["sy"])
+        for f in (T, S):
+            orc.do_advection_3d(tr.dt, f, *host, gt.HALFSPLIT, gt.P2_PDM, gt.P2_PDM, -1.0,
+                                gt.H_TAG)
+        ctx.eos_linear(tT, tS, None, tb, *consts)
+        ctx.ip_blumberg_mellor(dH, dv[6], dv[3], dv[4], tb, tout["zb"], tout["bx"], tout["by"])
+        ctx.ip_shchepetkin_mcwilliams(dH, dv[6], dv[3], dv[4], tb, dsse, tout["zs"], tout["sx"],
+                                      tout["sy"])
+        for f in (tT, tS):
+            ctx.do_advection_3d(tr.dt, f, *dv, gt.HALFSPLIT, gt.P2_PDM, gt.P2_PDM, -1.0,
+                                gt.H_TAG)
+    for nm, got, ref in [("T", tT, T), ("S", tS, S)] + [(k, tout[k], out[k]) for k in out]:
+        h = to_host(got)
+        assert rel_err(h, ref) <= TOL, (nm, rel_err(h, ref))
+        assert np.array_equal(h, ref), (nm, "not bit-exact")

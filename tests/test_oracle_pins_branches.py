@@ -767,3 +767,30 @@ def test_hvsplit_uniform_flow_textbook(hs):
         hnew = D - dt * ((wv if k < K else 0.0) - (wv if k > 1 else 0.0))
         want = (D * fh[k] - dt * (Fu - Fd)) / hnew
         assert np.max(np.abs(g[k] - want)[sel]) <= 1e-13, k
+
+
+# ========================================================= SUBSTR_INI_PRESS
+@pytest.mark.parametrize("method", [1, 6])
+def test_substr_ini_press_keeps_and_subtracts_the_first_gradient(method):
+    """internal_pressure.F90:235-258: the first call returns zero on the
+    interior and keeps its gradient; every later call returns the plain
+    gradient minus that first one; halo cells are the plain gradient"""
+    dom, H, hn, hun, hvn, b, sse = smcw_case(seed=11)
+    orc = po.Oracle.from_domain(dom)
+    sh = dom.shape3
+    state = {}
+    plain = []
+    for q in range(3):
+        bq = b * (1.0 + 0.1 * q) + 1e-3 * q * inputs.uniform(sh, 90 + q, -1, 1)
+        zz, ix, iy = (np.zeros(sh) for _ in range(3))
+        orc.do_internal_pressure(method, H, hn, hun, hvn, bq, sse, zz, ix, iy)
+        plain.append((ix.copy(), iy.copy()))
+        zz2, jx, jy = (np.zeros(sh) for _ in range(3))
+        orc.do_internal_pressure(method, H, hn, hun, hvn, bq, sse, zz2, jx, jy, substr=state)
+        inner = np.zeros(sh, bool)
+        inner[:, 2:-2, 2:-2] = True
+        for got, ref, first in ((jx, ix, plain[0][0]), (jy, iy, plain[0][1])):
+            assert np.array_equal(got[~inner], ref[~inner])
+            want = np.zeros(sh) if q == 0 else ref - first
+            assert np.array_equal(got[inner], want[inner]), (method, q)
+    assert np.max(np.abs(plain[1][0] - plain[0][0])) > 0
