@@ -852,22 +852,22 @@ static int restart_host_copies(gt_ctx* c, int n, const gt_restart_field* f,
   return GT_OK;
 }
 
-extern "C" int gt_write_restart(gt_ctx* c, const char* path, int with_halos, int loop, int n,
-                                const gt_restart_field* f) {
+extern "C" int gt_write_restart(gt_ctx* c, const char* path, int with_halos,
+                                const gt_restart_time* t, int n, const gt_restart_field* f) {
   std::vector<std::vector<double>> host;
   std::vector<gt_restart_field> hf;
   TRY(restart_host_copies(c, n, f, host, hf, true));
   return gt_restart_write_host(path, c->imin, c->imax, c->jmin, c->jmax, c->kmax, with_halos,
-                               loop, n, hf.data());
+                               t, n, hf.data());
 }
 
-extern "C" int gt_read_restart(gt_ctx* c, const char* path, int with_halos, int* loop, int n,
-                               const gt_restart_field* f) {
+extern "C" int gt_read_restart(gt_ctx* c, const char* path, int with_halos, gt_restart_time* t,
+                               int n, const gt_restart_field* f) {
   std::vector<std::vector<double>> host;
   std::vector<gt_restart_field> hf;
   // start from the device arrays, so the cells the file does not hold stay
   TRY(restart_host_copies(c, n, f, host, hf, true));
-  TRY(gt_restart_read_host(path, c->imin, c->imax, c->jmin, c->jmax, c->kmax, with_halos, loop,
+  TRY(gt_restart_read_host(path, c->imin, c->imax, c->jmin, c->jmax, c->kmax, with_halos, t,
                            n, hf.data()));
   for (int v = 0; v < n; v++)
     HIPCHK(hipMemcpy(f[v].data, host[v].data(), sizeof(double) * host[v].size(),

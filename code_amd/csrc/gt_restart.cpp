@@ -2,13 +2,19 @@
 // (src/ncdf/create_restart_ncdf.F90, write_restart_ncdf.F90,
 // read_restart_ncdf.F90) in the NetCDF classic format, written and read
 // natively (no NetCDF library): dimensions xax, yax, zax (= kmax+1), the
-// integer scalar "loop", one FP64 variable per field over (yax, xax) or
-// (zax, yax, xax) in the file's C order (the Fortran (x, y[, z]) order of the
-// reference), the global attribute title.  Interior cells imin..imax,
-// jmin..jmax, or the whole array with the halos (the reference's
-// _WRITE_HOT_HALOS_ / __READ_HOT_HALOS_ builds).  Version 1 of the format
-// (32-bit offsets) when the file fits in 2 GiB, else version 2 (64-bit
-// offsets), as NF90_64BIT_OFFSET would write it; both are read.
+// scalars of create_restart_ncdf.F90:94-104 (loop, julianday, secondsofday
+// as int, timestep as double), one FP64 variable per field over (yax, xax)
+// or (zax, yax, xax) in the file's C order (the Fortran (x, y[, z]) order of
+// the reference), the global attribute title.  Writing: interior cells
+// imin..imax, jmin..jmax, or the whole array with the halos (the reference's
+// _WRITE_HOT_HALOS_ build).  Reading (read_restart_ncdf.F90:71-118): the four
+// scalars are required, as open_restart_ncdf.F90:48-62 requires them; the
+// interior cells are read from an interior-sized file or, with halos (the
+// __READ_HOT_HALOS_ build), also from a halo-inclusive one, whose halo
+// values are skipped: the caller fills halos (update_2d/3d_halo), as the
+// reference does.  Version 1 of the format (32-bit offsets) when the file
+// fits in 2 GiB, else version 2 (64-bit offsets), as NF90_64BIT_OFFSET would
+// write it; both are read.
 //
 // Host code only: the device entry points (gt_api.hip) copy through host
 // buffers.
@@ -118,8 +124,10 @@ const char* kTitle = "GETM NetCDF hotstart file";
 }  // namespace
 
 extern "C" int gt_restart_write_host(const char* path, int imin, int imax, int jmin, int jmax,
-                                     int kmax, int with_halos, int loop, int n,
+                                     int kmax, int with_halos, const gt_restart_time* tm, int n,
                                      const gt_restart_field* f) {
+  const gt_restart_time t0 = {0, 0, 0, 0.0};
+  const gt_restart_time& T = tm ? *tm : t0;
   if (!path || n < 0 || (n > 0 && !f) || imax < imin || jmax < jmin || kmax < 0)
     return fail(GT_EARG, "gt_restart_write: bad argument%s%s");
   const Shape s = shape(imin, imax, jmin, jmax, kmax, with_halos);
@@ -128,7 +136,7 @@ extern "C" int gt_restart_write_host(const char* path, int imin, int imax, int j
       return fail(GT_EARG, "gt_restart_write: field %s: bad name, data or ndim%s",
                   f[v].name ? f[v].name : "?");
   const uint64_t plane = (uint64_t)s.xl * s.yl * 8;
-  uint64_t data = 4;  // loop
+  uint64_t data = 4 + 4 + 4 + 8;  // loop, julianday, secondsofday, timestep
   for (int v = 0; v < n; v++) data += f[v].ndim == 3 ? plane * s.NK : plane;
   // the header; its size depends only on the format version (4- or 8-byte
   // offsets), so it is built once to measure and once with the offsets
@@ -152,7 +160,7 @@ extern "C" int gt_restart_write_host(const char* path, int imin, int imax, int j
     h.b.insert(h.b.end(), kTitle, kTitle + std::strlen(kTitle));
     while (h.b.size() % 4) h.b.push_back(0);
     h.u32(NC_VARIABLE);
-    h.u32((uint32_t)n + 1);
+    h.u32((uint32_t)n + 4);
     uint64_t off = hdr;
     auto var = [&](const std::string& nm, int ndim, uint32_t type, uint64_t bytes) {
       h.name(nm);
@@ -174,6 +182,9 @@ extern "C" int gt_restart_write_host(const char* path, int imin, int imax, int j
       off += bytes;
     };
     var("loop", 0, NC_INT, 4);
+    var("julianday", 0, NC_INT, 4);
+    var("secondsofday", 0, NC_INT, 4);
+    var("timestep", 0, NC_DOUBLE, 8);
     for (int v = 0; v < n; v++)
       var(f[v].name, f[v].ndim, NC_DOUBLE, f[v].ndim == 3 ? plane * s.NK : plane);
     return h;
@@ -183,9 +194,12 @@ extern "C" int gt_restart_write_host(const char* path, int imin, int imax, int j
   FILE* fp = std::fopen(path, "wb");
   if (!fp) return fail(GT_EARG, "gt_restart_write: cannot create %s%s", path);
   bool ok = std::fwrite(h.b.data(), 1, h.b.size(), fp) == h.b.size();
-  uint8_t lb[4] = {(uint8_t)((uint32_t)loop >> 24), (uint8_t)((uint32_t)loop >> 16),
-                   (uint8_t)((uint32_t)loop >> 8), (uint8_t)loop};
-  ok = ok && std::fwrite(lb, 1, 4, fp) == 4;
+  uint8_t sc[20];
+  const int32_t ints[3] = {T.loop, T.julianday, T.secondsofday};
+  for (int q = 0; q < 3; q++)
+    for (int b = 0; b < 4; b++) sc[4 * q + b] = (uint8_t)((uint32_t)ints[q] >> (24 - 8 * b));
+  put_be_doubles(sc + 12, &T.timestep, 1);
+  ok = ok && std::fwrite(sc, 1, 20, fp) == 20;
   std::vector<uint8_t> row((size_t)s.xl * 8);
   for (int v = 0; v < n && ok; v++) {
     const long nk = f[v].ndim == 3 ? s.NK : 1;
@@ -202,10 +216,12 @@ extern "C" int gt_restart_write_host(const char* path, int imin, int imax, int j
 }
 
 extern "C" int gt_restart_read_host(const char* path, int imin, int imax, int jmin, int jmax,
-                                    int kmax, int with_halos, int* loop, int n,
+                                    int kmax, int with_halos, gt_restart_time* tm, int n,
                                     const gt_restart_field* f) {
   if (!path || n < 0 || (n > 0 && !f)) return fail(GT_EARG, "gt_restart_read: bad argument%s%s");
-  const Shape s = shape(imin, imax, jmin, jmax, kmax, with_halos);
+  // the cells filled: always the interior (read_restart_ncdf.F90:83-101)
+  const Shape s = shape(imin, imax, jmin, jmax, kmax, 0);
+  const Shape sh = shape(imin, imax, jmin, jmax, kmax, 1);
   FILE* fp = std::fopen(path, "rb");
   if (!fp) return fail(GT_EARG, "gt_restart_read: cannot open %s%s", path);
   // the header: read a generous prefix, grow if it does not parse
@@ -275,10 +291,17 @@ extern "C" int gt_restart_read_host(const char* path, int imin, int imax, int jm
       if (d.first == nm) return (long)d.second;
     return -1;
   };
-  if (dimlen("xax") != s.xl || dimlen("yax") != s.yl || dimlen("zax") != s.NK) {
+  // an interior-sized file, or with halos also a halo-inclusive one whose
+  // first HALO rows and columns are skipped
+  long fxl = s.xl, fx0 = 0;
+  if (dimlen("zax") != s.NK) fxl = -1;
+  if (with_halos && dimlen("xax") == sh.xl && dimlen("yax") == sh.yl) fx0 = 2, fxl = sh.xl;
+  else if (dimlen("xax") != s.xl || dimlen("yax") != s.yl) fxl = -1;
+  if (fxl < 0) {
     std::fclose(fp);
     return fail(GT_EARG, "gt_restart_read: %s: xax/yax/zax do not match the domain%s", path);
   }
+  const long fyl = fx0 ? sh.yl : s.yl;
   int rc = GT_OK;
   auto dimid = [&](const char* nm) -> uint32_t {
     for (size_t q = 0; q < dims.size(); q++)
@@ -286,16 +309,30 @@ extern "C" int gt_restart_read_host(const char* path, int imin, int imax, int jm
     return 0xFFFFFFFFu;
   };
   const uint32_t dx = dimid("xax"), dy = dimid("yax"), dz = dimid("zax");
-  if (loop) {
-    auto it = vars.find("loop");
-    uint8_t lb[4];
-    if (it == vars.end() || it->second.type != NC_INT ||
-        std::fseek(fp, (long)it->second.begin, SEEK_SET) != 0 || std::fread(lb, 1, 4, fp) != 4)
-      rc = fail(GT_EARG, "gt_restart_read: %s: no integer 'loop'%s", path);
-    else
-      *loop = (int32_t)((uint32_t)lb[0] << 24 | (uint32_t)lb[1] << 16 | (uint32_t)lb[2] << 8 |
-                        lb[3]);
+  // the scalars open_restart_ncdf.F90:48-62 requires and read_restart_ncdf
+  // reads first (:71-81)
+  gt_restart_time got = {0, 0, 0, 0.0};
+  const char* sname[4] = {"loop", "julianday", "secondsofday", "timestep"};
+  for (int q = 0; q < 4 && rc == GT_OK; q++) {
+    auto it = vars.find(sname[q]);
+    const uint32_t type = q < 3 ? NC_INT : NC_DOUBLE;
+    uint8_t b[8];
+    const size_t sz = q < 3 ? 4 : 8;
+    if (it == vars.end() || it->second.type != type || !it->second.dims.empty() ||
+        std::fseek(fp, (long)it->second.begin, SEEK_SET) != 0 || std::fread(b, 1, sz, fp) != sz) {
+      rc = fail(GT_EARG, "gt_restart_read: %s: no scalar '%s' of the reference's type", path,
+                sname[q]);
+      break;
+    }
+    if (q < 3) {
+      const int32_t v = (int32_t)((uint32_t)b[0] << 24 | (uint32_t)b[1] << 16 |
+                                  (uint32_t)b[2] << 8 | b[3]);
+      (q == 0 ? got.loop : q == 1 ? got.julianday : got.secondsofday) = v;
+    } else {
+      get_be_doubles(&got.timestep, b, 1);
+    }
   }
+  if (rc == GT_OK && tm) *tm = got;
   std::vector<uint8_t> row((size_t)s.xl * 8);
   for (int v = 0; v < n && rc == GT_OK; v++) {
     auto it = vars.find(f[v].name ? f[v].name : "");
@@ -306,14 +343,12 @@ extern "C" int gt_restart_read_host(const char* path, int imin, int imax, int jm
                 path, f[v].name ? f[v].name : "?");
       break;
     }
-    if (std::fseek(fp, (long)it->second.begin, SEEK_SET) != 0) {
-      rc = fail(GT_EARG, "gt_restart_read: %s: seek failed%s", path);
-      break;
-    }
     const long nk = f[v].ndim == 3 ? s.NK : 1;
     for (long k = 0; k < nk && rc == GT_OK; k++)
       for (long j = 0; j < s.yl; j++) {
-        if (std::fread(row.data(), 1, row.size(), fp) != row.size()) {
+        const long at = (long)it->second.begin + 8 * ((k * fyl + fx0 + j) * fxl + fx0);
+        if (std::fseek(fp, at, SEEK_SET) != 0 ||
+            std::fread(row.data(), 1, row.size(), fp) != row.size()) {
           rc = fail(GT_EARG, "gt_restart_read: %s: short read of %s", path, f[v].name);
           break;
         }

@@ -65,6 +65,16 @@ class RestartField(C.Structure):
     _fields_ = [("name", C.c_char_p), ("ndim", C.c_int), ("data", C.c_void_p)]
 
 
+class RestartTime(C.Structure):
+    """gt_restart_time: the scalars of create_restart_ncdf.F90:94-104"""
+    _fields_ = [("loop", C.c_int), ("julianday", C.c_int), ("secondsofday", C.c_int),
+                ("timestep", C.c_double)]
+
+    def as_dict(self):
+        return dict(loop=self.loop, julianday=self.julianday, secondsofday=self.secondsofday,
+                    timestep=self.timestep)
+
+
 class GtError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{GT_ERRORS.get(code, code)}: {msg}")
@@ -114,10 +124,12 @@ def load_library():
         "gt_ww_momentum_3d": (i, [vp, d, vp, vp, vp, vp, vp]),
         "gt_salinity_diffusion": (i, [vp, d, d, vp, vp, vp]),
         "gt_uv_advect_3d": (i, [vp, d] + [vp] * 7 + [i, i, i, vp, vp]),
-        "gt_restart_write_host": (i, [C.c_char_p, i, i, i, i, i, i, i, i, vp]),
-        "gt_restart_read_host": (i, [C.c_char_p, i, i, i, i, i, i, vp, i, vp]),
-        "gt_write_restart": (i, [vp, C.c_char_p, i, i, i, vp]),
-        "gt_read_restart": (i, [vp, C.c_char_p, i, vp, i, vp]),
+        "gt_restart_write_host": (i, [C.c_char_p, i, i, i, i, i, i, C.POINTER(RestartTime), i,
+                                      vp]),
+        "gt_restart_read_host": (i, [C.c_char_p, i, i, i, i, i, i, C.POINTER(RestartTime), i,
+                                     vp]),
+        "gt_write_restart": (i, [vp, C.c_char_p, i, C.POINTER(RestartTime), i, vp]),
+        "gt_read_restart": (i, [vp, C.c_char_p, i, C.POINTER(RestartTime), i, vp]),
         "gt_temperature_diffusion": (i, [vp, d, d] + [vp] * 9 + [d, d, vp, vp]),
         "gt_tridiagonal_batch": (i, [vp, C.c_long, i, i, vp, vp, vp, vp, vp]),
         "gt_rho_from_theta_unesco80": (i, [vp, C.c_long, vp, vp, vp, i]),
@@ -213,20 +225,27 @@ def _restart_fields(fields, ptr):
     return arr, keep
 
 
-def restart_write_host(path, imin, imax, jmin, jmax, kmax, fields, loop=0, with_halos=False):
+def _restart_time(loop, julianday, secondsofday, timestep):
+    return RestartTime(int(loop), int(julianday), int(secondsofday), float(timestep))
+
+
+def restart_write_host(path, imin, imax, jmin, jmax, kmax, fields, loop=0, with_halos=False,
+                       julianday=0, secondsofday=0, timestep=0.0):
     """gt_restart_write_host on C-contiguous float64 numpy arrays"""
     arr, keep = _restart_fields(fields, lambda a: a.ctypes.data)
+    t = _restart_time(loop, julianday, secondsofday, timestep)
     check(load_library().gt_restart_write_host(str(path).encode(), imin, imax, jmin, jmax, kmax,
-                                               int(with_halos), int(loop), len(fields), arr))
+                                               int(with_halos), C.byref(t), len(fields), arr))
 
 
 def restart_read_host(path, imin, imax, jmin, jmax, kmax, fields, with_halos=False):
-    """gt_restart_read_host into numpy arrays; returns loop"""
+    """gt_restart_read_host into numpy arrays; returns the scalars as a dict
+    (loop, julianday, secondsofday, timestep)"""
     arr, keep = _restart_fields(fields, lambda a: a.ctypes.data)
-    loop = C.c_int()
+    t = RestartTime()
     check(load_library().gt_restart_read_host(str(path).encode(), imin, imax, jmin, jmax, kmax,
-                                              int(with_halos), C.byref(loop), len(fields), arr))
-    return loop.value
+                                              int(with_halos), C.byref(t), len(fields), arr))
+    return t.as_dict()
 
 
 class Context:
@@ -415,17 +434,21 @@ class Context:
         shape = self.shape3 if t.ndim == 3 else self.shape2
         return _dev(t, "restart field", shape=shape).value
 
-    def write_restart(self, path, fields, loop=0, with_halos=False):
+    def write_restart(self, path, fields, loop=0, with_halos=False, julianday=0,
+                      secondsofday=0, timestep=0.0):
         arr, keep = _restart_fields(fields, self._restart_ptr)
-        check(self.lib.gt_write_restart(self.h, str(path).encode(), int(with_halos), int(loop),
+        t = _restart_time(loop, julianday, secondsofday, timestep)
+        check(self.lib.gt_write_restart(self.h, str(path).encode(), int(with_halos), C.byref(t),
                                         len(fields), arr))
 
     def read_restart(self, path, fields, with_halos=False):
+        """returns the scalars as a dict (loop, julianday, secondsofday,
+        timestep)"""
         arr, keep = _restart_fields(fields, self._restart_ptr)
-        loop = C.c_int()
-        check(self.lib.gt_read_restart(self.h, str(path).encode(), int(with_halos),
-                                       C.byref(loop), len(fields), arr))
-        return loop.value
+        t = RestartTime()
+        check(self.lib.gt_read_restart(self.h, str(path).encode(), int(with_halos), C.byref(t),
+                                       len(fields), arr))
+        return t.as_dict()
 
     def uv_advect_3d(self, dt, uu, vv, ww, ho, hn, hun, hvn, split, hscheme, vscheme, uuEx,
                      vvEx):

@@ -203,31 +203,44 @@ int gt_tridiagonal_batch(gt_ctx *ctx, long n, int fi, int lt,
 
 /* ---- restart fields: the NetCDF hotstart file ---------------------------
  * create_restart_ncdf.F90 / write_restart_ncdf.F90 / read_restart_ncdf.F90
- * for the fields a caller names (e.g. hn, T, S, uu, vv, ww, ssen, sseo):
+ * for the fields a caller names (e.g. z, hn, T, S, uu, vv, ww, ssen, sseo):
  * NetCDF classic format written and read natively, dimensions xax, yax,
- * zax = kmax+1, the integer "loop", FP64 variables over (x, y) or
- * (x, y, z) in the reference's Fortran order, the title attribute.
+ * zax = kmax+1, the scalars loop, julianday, secondsofday (int) and timestep
+ * (double) of create_restart_ncdf.F90:94-104, FP64 variables over (x, y) or
+ * (x, y, z) in the reference's Fortran order, the title attribute.  Writing:
  * with_halos = 0 stores imin..imax x jmin..jmax (the default build), 1 the
- * whole array (_WRITE_HOT_HALOS_).  64-bit offsets when the file exceeds
- * 2 GiB.  Reading checks the dimensions and fills the same cells (halos of
- * the arrays are kept when with_halos = 0); loop may be NULL. */
+ * whole array (_WRITE_HOT_HALOS_); time may be NULL (zeros).  64-bit
+ * offsets when the file exceeds 2 GiB.  Reading requires the four scalars
+ * (open_restart_ncdf.F90:48-62) and returns them in time (may be NULL); it
+ * fills the interior cells only, from an interior-sized file or, with
+ * with_halos = 1 (__READ_HOT_HALOS_, read_restart_ncdf.F90:83-101), also
+ * from a halo-inclusive one; halo cells of the arrays are left for the
+ * caller's update_2d/3d_halo.  The reference reader also needs 'z'. */
 typedef struct {
   const char *name; /* NetCDF variable name, as the reference's */
   int ndim;         /* 2: E2DFIELD, 3: I3DFIELD */
   double *data;     /* the whole array, halos included */
 } gt_restart_field;
+typedef struct {
+  int loop, julianday, secondsofday;
+  double timestep;
+} gt_restart_time;
 /* data are host arrays */
 int gt_restart_write_host(const char *path, int imin, int imax, int jmin,
-                          int jmax, int kmax, int with_halos, int loop, int n,
+                          int jmax, int kmax, int with_halos,
+                          const gt_restart_time *time, int n,
                           const gt_restart_field *fields);
 int gt_restart_read_host(const char *path, int imin, int imax, int jmin,
-                         int jmax, int kmax, int with_halos, int *loop, int n,
+                         int jmax, int kmax, int with_halos,
+                         gt_restart_time *time, int n,
                          const gt_restart_field *fields);
 /* data are device arrays of the context's domain (synchronises) */
-int gt_write_restart(gt_ctx *ctx, const char *path, int with_halos, int loop,
-                     int n, const gt_restart_field *fields);
-int gt_read_restart(gt_ctx *ctx, const char *path, int with_halos, int *loop,
-                    int n, const gt_restart_field *fields);
+int gt_write_restart(gt_ctx *ctx, const char *path, int with_halos,
+                     const gt_restart_time *time, int n,
+                     const gt_restart_field *fields);
+int gt_read_restart(gt_ctx *ctx, const char *path, int with_halos,
+                    gt_restart_time *time, int n,
+                    const gt_restart_field *fields);
 
 /* ---- internal pressure ------------------------------------------------- */
 /* do_eqstate with eqstate_method=1 (linear), src/3d/eqstate.F90:127-149 and

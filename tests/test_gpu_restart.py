@@ -32,14 +32,25 @@ def test_restart_round_trip_on_device(device, tmp_path):
     finally:
         nc.close()
     back = {n: torch.full_like(t, 9.0) for n, t in dev.items()}
-    assert ctx.read_restart(p, back) == 4321
+    assert ctx.read_restart(p, back)["loop"] == 4321
     for n, a in host.items():
         got = to_host(back[n])
         assert np.array_equal(got[..., 2:-2, 2:-2], a[..., 2:-2, 2:-2]), n
         assert got[..., 0, 0].tolist() == np.full(got[..., 0, 0].shape, 9.0).tolist(), n
-    # the whole arrays with their halos (the _WRITE_HOT_HALOS_ build)
-    ctx.write_restart(tmp_path / "halo.nc", dev, loop=1, with_halos=True)
+    # the whole arrays with their halos (the _WRITE_HOT_HALOS_ build); the
+    # __READ_HOT_HALOS_ read takes the interior and leaves the halos
+    ctx.write_restart(tmp_path / "halo.nc", dev, loop=1, with_halos=True, julianday=7,
+                      secondsofday=8, timestep=9.0)
+    nc = netcdf_file(str(tmp_path / "halo.nc"), "r", mmap=False)
+    try:
+        for n, a in host.items():
+            assert np.array_equal(nc.variables[n][:], a), n
+    finally:
+        nc.close()
     back = {n: torch.zeros_like(t) for n, t in dev.items()}
-    ctx.read_restart(tmp_path / "halo.nc", back, with_halos=True)
+    t = ctx.read_restart(tmp_path / "halo.nc", back, with_halos=True)
+    assert t == dict(loop=1, julianday=7, secondsofday=8, timestep=9.0)
     for n, a in host.items():
-        assert np.array_equal(to_host(back[n]), a), n
+        got = to_host(back[n])
+        assert np.array_equal(got[..., 2:-2, 2:-2], a[..., 2:-2, 2:-2]), n
+        assert np.all(got[..., :2, :] == 0.0), n
