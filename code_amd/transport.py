@@ -473,10 +473,15 @@ class Context:
             self._f3(T, "T")))
 
     def tridiagonal_batch(self, fi, lt, au, bu, cu, du, value):
-        """systems along the last axis: arrays of shape (lt+1, n)"""
-        n = au.shape[-1] if au is not None else 0
-        ps = [_dev(t, nm, shape=(lt + 1, n)) for t, nm in ((au, "au"), (bu, "bu"), (cu, "cu"),
-                                                           (du, "du"), (value, "value"))]
+        """systems along the last axis: arrays of shape (rows, n) with at
+        least lt+1 rows (the library checks 0 <= fi < lt)"""
+        n = au.shape[-1] if au is not None and au.dim() == 2 else 0
+        rows = au.shape[0] if n else 0
+        if n and lt >= rows:
+            raise ValueError(f"tridiagonal_batch: lt = {lt} needs at least {lt + 1} rows "
+                             f"(got {rows})")
+        ps = [_dev(t, nm, shape=(rows, n)) for t, nm in ((au, "au"), (bu, "bu"), (cu, "cu"),
+                                                         (du, "du"), (value, "value"))]
         check(self.lib.gt_tridiagonal_batch(self.h, n, fi, lt, *ps))
 
     def rho_from_theta_unesco80(self, T, S, rho, nonnegsalt=False):
