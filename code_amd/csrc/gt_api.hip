@@ -59,6 +59,8 @@ struct gt_ctx {
   double *fi = nullptr, *hi = nullptr, *adv = nullptr, *tmp = nullptr;
   double *scr7 = nullptr, *zz = nullptr, *P = nullptr, *slice3 = nullptr;
   double *bpart = nullptr, *bout = nullptr;
+  double* bbatch = nullptr;  // partials of gt_budget_batch_f32 (grown on demand)
+  long bbatch_n = 0;
   unsigned long long* nwarn = nullptr;
   int adv_ver_iterations = 1;
   int do_mirror_2d = 1;
@@ -241,7 +243,7 @@ extern "C" int gt_destroy(gt_ctx* c) {
   if (!c) return GT_OK;
   void* ps[] = {c->dmask, c->dint, c->fi, c->hi, c->adv, c->tmp, c->scr7, c->zz,
                 c->P, c->slice3, c->bpart, c->bout, c->nwarn, c->ws, c->dlev, c->gga,
-                c->dbad, c->vdw, c->tri, c->mwork, c->w32, c->idp0};
+                c->dbad, c->vdw, c->tri, c->mwork, c->w32, c->idp0, c->bbatch};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   delete c;
@@ -1122,6 +1124,8 @@ extern "C" int gt_set_tuning(gt_ctx* c, int fused, int tile, int group) {
 
 // sizes every work field of the transport step up front, so later calls
 // neither allocate nor synchronise the device to grow one
+static int ensure_bbatch(gt_ctx* c, int ntr);
+
 extern "C" int gt_reserve(gt_ctx* c, int ntracers_f32, int ntracers_f64) {
   if (!c || ntracers_f32 < 0 || ntracers_f64 < 0)
     return set_err(GT_EARG, "gt_reserve: bad argument");
@@ -1140,6 +1144,7 @@ extern "C" int gt_reserve(gt_ctx* c, int ntracers_f32, int ntracers_f64) {
   ENSURE(c->slice3, 3 * c->d.NXY);
   ENSURE(c->bpart, 4 * 1024);
   ENSURE(c->bout, 2);
+  if (ntracers_f32 > 0) TRY(ensure_bbatch(c, ntracers_f32));
   if (c->adv_ver_iterations > 1) ENSURE(c->scr7, 7 * N3);
   return GT_OK;
 }
@@ -1657,10 +1662,14 @@ extern "C" int gt_eos_linear_adjoint(gt_ctx* c, const double* buoy_bar, double* 
 }
 
 // ---- budgets -----------------------------------------------------------------------
+// blocks of the partial sums (the batched form uses the same number, so each
+// of its outputs equals gt_budget_f32's bit for bit)
+constexpr int kBudgetBlocks = 1024;
+
 template <typename TF>
 static int budget_impl(gt_ctx* c, const double* h, const TF* f, double* mass, double* volume) {
   if (!c || !h) return set_err(GT_EARG, "gt_budget: NULL argument");
-  const int nblocks = 1024;
+  const int nblocks = kBudgetBlocks;
   ENSURE(c->bpart, 4 * nblocks);
   ENSURE(c->bout, 2);
   LCHK(gt::launch_budget<TF>(c->st, c->d, c->az, h, f, c->bpart, nblocks, c->dx * c->dy,
@@ -1681,6 +1690,38 @@ extern "C" int gt_budget(gt_ctx* c, const double* h, const double* f, double* ma
 extern "C" int gt_budget_f32(gt_ctx* c, const double* h, const float* f, double* mass,
                              double* volume) {
   return budget_impl<float>(c, h, f, mass, volume);
+}
+
+static long budget_batch_partials(int ntr) { return 2L * (ntr + 1) * kBudgetBlocks; }
+
+static int ensure_bbatch(gt_ctx* c, int ntr) {
+  const long n = budget_batch_partials(ntr);
+  if (c->bbatch_n >= n) return GT_OK;
+  if (c->bbatch) {
+    HIPCHK(hipDeviceSynchronize());  // the old partials may still be in use on any stream
+    HIPCHK(hipFree(c->bbatch));
+    c->bbatch = nullptr;
+    c->bbatch_n = 0;
+  }
+  HIPCHK(hipMalloc(&c->bbatch, sizeof(double) * (size_t)n));
+  c->bbatch_n = n;
+  return GT_OK;
+}
+
+static int budget_batch_check(const char* name, gt_ctx* c, int ntr, const double* h,
+                              const float* f, const double* out) {
+  if (!c || !h || !f || !out) return set_err(GT_EARG, "%s: NULL argument", name);
+  if (ntr < 1) return set_err(GT_EARG, "%s: ntracers must be >= 1", name);
+  return GT_OK;
+}
+
+extern "C" int gt_budget_batch_f32(gt_ctx* c, int ntracers, const double* h, const float* f,
+                                   double* out) {
+  TRY(budget_batch_check("gt_budget_batch_f32", c, ntracers, h, f, out));
+  TRY(ensure_bbatch(c, ntracers));
+  LCHK(gt::launch_budget_batch<float>(c->st, c->d, c->az, ntracers, c->N3, h, f, c->bbatch,
+                                      kBudgetBlocks, c->dx * c->dy, out));
+  return GT_OK;
 }
 
 // ---- memory helpers -----------------------------------------------------------------

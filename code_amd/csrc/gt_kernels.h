@@ -120,5 +120,13 @@ hipError_t launch_substr_ini_press(hipStream_t st, Dims d, int first, double* id
 template <typename TF>
 hipError_t launch_budget(hipStream_t st, Dims d, const int32_t* az, const double* h,
                          const TF* f, double* partial, int nblocks, double area, double* out);
+// ntr tracers of one h (tracer t at f + t*tstride) in one pass: partial
+// must hold 2*(ntr+1)*nblocks doubles, out ntr+1 (the masses, then the
+// volume); each output equals launch_budget's with the same nblocks bit for
+// bit
+template <typename TF>
+hipError_t launch_budget_batch(hipStream_t st, Dims d, const int32_t* az, int ntr, long tstride,
+                               const double* h, const TF* f, double* partial, int nblocks,
+                               double area, double* out);
 
 }  // namespace gt

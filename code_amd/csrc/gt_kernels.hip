@@ -950,6 +950,113 @@ __global__ void k_budget_final(int nblocks, const double* __restrict__ partial, 
   out[1] = p + pe;
 }
 
+// The budgets of ntr tracers sharing one h in one pass: blockIdx.y is a
+// group of up to kBudgetGroup tracers, so h is read once per group.  Each
+// thread visits the same cells in the same order as k_budget_partial and the
+// block tree is the same, so every tracer's sums (and the volume, formed by
+// group 0) are bit for bit those of k_budget_partial.  Partial q (tracer q,
+// or the volume at q = ntr) of block b: partial[2*(q*nblocks + b) + {0,1}].
+constexpr int kBudgetGroup = 6;
+
+template <typename TF>
+__global__ __launch_bounds__(256) void k_budget_batch_partial(
+    Dims d, const int32_t* __restrict__ az, int ntr, long tstride,
+    const double* __restrict__ h, const TF* __restrict__ f, double* partial) {
+  constexpr int G = kBudgetGroup;
+  const long nint = (long)(d.NX - 4) * (d.NY - 4) * d.kmax;
+  const int t0 = blockIdx.y * G;
+  const int ng = min(G, ntr - t0);
+  const bool vol = blockIdx.y == 0;
+  double mh[G], ml[G], vh = 0, vl = 0;
+#pragma unroll
+  for (int g = 0; g < G; g++) mh[g] = ml[g] = 0;
+  const TF* fg = f + (long)t0 * tstride;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < nint;
+       q += (long)gridDim.x * blockDim.x) {
+    const long ni = d.NX - 4, nj = d.NY - 4;
+    const long i = q % ni, r = q / ni;
+    const long j = r % nj, k = 1 + r / nj;
+    const long c2 = (i + 2) + (long)d.NX * (j + 2);
+    if (az[c2] != 1) continue;
+    const long c = c2 + d.NXY * k;
+    const double hv = h[c];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      if (g < ng) {
+        const double fv = (double)fg[(long)g * tstride + c];
+        const double p = hv * fv;
+        const double pe = fma(hv, fv, -p);
+        dd_add(mh[g], ml[g], p);
+        dd_add(mh[g], ml[g], pe);
+      }
+    }
+    if (vol) dd_add(vh, vl, hv);
+  }
+  // one (hi, lo) pair per output and thread, reduced by the tree of
+  // k_budget_partial
+  __shared__ double sm[2 * (G + 1)][256];
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    sm[2 * g][t] = mh[g];
+    sm[2 * g + 1][t] = ml[g];
+  }
+  sm[2 * G][t] = vh;
+  sm[2 * G + 1][t] = vl;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) {
+#pragma unroll
+      for (int g = 0; g <= G; g++) {
+        double a = sm[2 * g][t], b = sm[2 * g + 1][t];
+        dd_add_dd(a, b, sm[2 * g][t + s], sm[2 * g + 1][t + s]);
+        sm[2 * g][t] = a;
+        sm[2 * g + 1][t] = b;
+      }
+    }
+    __syncthreads();
+  }
+  const long nb = gridDim.x;
+  if (t < ng) {
+    double* p = partial + 2 * ((long)(t0 + t) * nb + blockIdx.x);
+    p[0] = sm[2 * t][0];
+    p[1] = sm[2 * t + 1][0];
+  } else if (vol && t == G) {
+    double* p = partial + 2 * ((long)ntr * nb + blockIdx.x);
+    p[0] = sm[2 * G][0];
+    p[1] = sm[2 * G + 1][0];
+  }
+}
+
+// one block per output q: the partials of all blocks in k_budget_final's
+// order and rounding; out[q] = sum * area
+__global__ __launch_bounds__(256) void k_budget_batch_final(int nblocks,
+                                                            const double* __restrict__ partial,
+                                                            double area, double* out) {
+  __shared__ double sm[2][256];
+  const int t = threadIdx.x;
+  const double* pq = partial + 2 * (long)blockIdx.x * nblocks;
+  double sh = 0, sl = 0;
+  for (int b = t; b < nblocks; b += blockDim.x) dd_add_dd(sh, sl, pq[2 * b], pq[2 * b + 1]);
+  sm[0][t] = sh;
+  sm[1][t] = sl;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      double a = sm[0][t], b = sm[1][t];
+      dd_add_dd(a, b, sm[0][t + s], sm[1][t + s]);
+      sm[0][t] = a;
+      sm[1][t] = b;
+    }
+    __syncthreads();
+  }
+  if (t != 0) return;
+  const double p = sm[0][0] * area;
+  double pe = fma(sm[0][0], area, -p);
+  pe += sm[1][0] * area;
+  out[blockIdx.x] = p + pe;
+}
+
 }  // namespace gt
 
 // ---------------------------------------------------------------------------
@@ -1119,6 +1226,20 @@ hipError_t launch_budget(hipStream_t st, Dims d, const int32_t* az, const double
   return hipGetLastError();
 }
 
+template <typename TF>
+hipError_t launch_budget_batch(hipStream_t st, Dims d, const int32_t* az, int ntr, long tstride,
+                               const double* h, const TF* f, double* partial, int nblocks,
+                               double area, double* out) {
+  const unsigned ngroups = (unsigned)((ntr + kBudgetGroup - 1) / kBudgetGroup);
+  hipLaunchKernelGGL(k_budget_batch_partial<TF>, dim3(nblocks, ngroups), dim3(256), 0, st, d, az,
+                     ntr, tstride, h, f, partial);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_budget_batch_final, dim3(ntr + 1), dim3(256), 0, st, nblocks, partial,
+                     area, out);
+  return hipGetLastError();
+}
+
 }  // namespace gt
 
 namespace gt {
@@ -1126,4 +1247,10 @@ template hipError_t launch_budget<double>(hipStream_t, Dims, const int32_t*, con
                                           const double*, double*, int, double, double*);
 template hipError_t launch_budget<float>(hipStream_t, Dims, const int32_t*, const double*,
                                          const float*, double*, int, double, double*);
+template hipError_t launch_budget_batch<double>(hipStream_t, Dims, const int32_t*, int, long,
+                                                const double*, const double*, double*, int,
+                                                double, double*);
+template hipError_t launch_budget_batch<float>(hipStream_t, Dims, const int32_t*, int, long,
+                                               const double*, const float*, double*, int, double,
+                                               double*);
 }  // namespace gt

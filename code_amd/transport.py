@@ -48,7 +48,7 @@ EXPORTS = [
     "gt_start_macro", "gt_set_coordinates", "gt_coordinates", "gt_ww_momentum_3d",
     "gt_salinity_diffusion", "gt_temperature_diffusion", "gt_tridiagonal_batch",
     "gt_uv_advect_3d", "gt_restart_write_host", "gt_restart_read_host", "gt_write_restart",
-    "gt_read_restart", "gt_set_substr_ini_press", "gt_reserve",
+    "gt_read_restart", "gt_set_substr_ini_press", "gt_reserve", "gt_budget_batch_f32",
 ]
 
 
@@ -154,6 +154,7 @@ def load_library():
         "gt_set_tuning": (i, [vp, i, i, i]),
         "gt_set_adjoint_mode": (i, [vp, i]),
         "gt_budget_f32": (i, [vp, vp, vp, C.POINTER(d), C.POINTER(d)]),
+        "gt_budget_batch_f32": (i, [vp, i, vp, vp, vp]),
         "gt_adv3d_adjoint": (i, [vp, i, i, d, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i, i,
                                  d, i]),
         "gt_adv3d_adjoint_f32": (i, [vp, i, i, i, d, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
@@ -555,6 +556,19 @@ class Context:
                  self._f3(f, "f", True, dtype=torch.float32 if f32 else None),
                  C.byref(m), C.byref(v)))
         return m.value, v.value
+
+    def budget_batch(self, h, f, ntracers=None, out=None):
+        """the FP64 budgets of a set of FP32 tracers (ntracers, kmax+1, NY, NX)
+        sharing h, in one pass: a device tensor of ntracers + 1 doubles (the
+        masses, then the volume), asynchronous on the context's stream"""
+        import torch
+        nt = f.shape[0] if ntracers is None else ntracers
+        if out is None:
+            out = torch.empty(nt + 1, dtype=torch.float64, device=h.device)
+        check(self.lib.gt_budget_batch_f32(self.h, nt, self._f3(h, "h"),
+                                           self._set(f, "f", nt, torch.float32),
+                                           _dev(out, "out", numel=nt + 1)))
+        return out
 
     # ---- reverse mode -------------------------------------------------------------
     def _vel(self, uu, vv, ww, hu, hv, ho):

@@ -482,6 +482,14 @@ int gt_budget(gt_ctx *ctx, const double *h, const double *f, double *mass,
 /* the same with an FP32 tracer (each h*f product formed exactly in FP64) */
 int gt_budget_f32(gt_ctx *ctx, const double *h, const float *f, double *mass,
                   double *volume);
+/* the budgets of ntracers FP32 tracers sharing h (tracer t at
+ * f + t*(kmax+1)*NY*NX, as gt_adv3d_batch_f32 lays them out) in one pass
+ * that reads h once per six tracers: out (device memory, ntracers + 1
+ * doubles) receives the masses, then the volume; each equals gt_budget_f32's
+ * bit for bit.  Asynchronous on the context's stream; the partial sums live
+ * in the context (grown on first use, or sized by gt_reserve). */
+int gt_budget_batch_f32(gt_ctx *ctx, int ntracers, const double *h,
+                        const float *f, double *out);
 
 /* ---- device memory helpers for callers without a HIP binding ---------- */
 int gt_malloc(void **ptr, size_t bytes);

@@ -346,6 +346,31 @@ def test_budget(case):
     assert abs(gv - v) <= 1e-15 * abs(v)
 
 
+@pytest.mark.parametrize("nt", [1, 6, 7, 20])
+def test_budget_batch_f32_equals_per_tracer(device, nt):
+    """gt_budget_batch_f32 (one pass, groups of six tracers, the last one
+    partial for 7 and 20) equals gt_budget_f32 per tracer bit for bit, and
+    the oracle's double-double sum of the FP32 values to 1e-15"""
+    import torch
+    dom = random_domain(37, 29, 6, 11, "islands")
+    ctx = gt.Context.from_domain(dom)
+    orc = po.Oracle.from_domain(dom)
+    rng = np.random.default_rng(nt)
+    h = rng.uniform(0.5, 3.0, dom.shape3)
+    f = rng.uniform(-2.0, 5.0, (nt,) + dom.shape3).astype(np.float32)
+    th, tf = to_dev(h, device), torch.from_numpy(f).to(device)
+    out = ctx.budget_batch(th, tf).cpu().numpy()
+    for q in range(nt):
+        m, v = ctx.budget(th, tf[q])
+        assert out[q] == m, (q, out[q], m)
+        assert out[nt] == v
+        mo, vo = orc.budget(h, f[q].astype(np.float64))
+        assert abs(out[q] - mo) <= 1e-15 * abs(mo)
+        assert abs(out[nt] - vo) <= 1e-15 * abs(vo)
+    with pytest.raises(ValueError):
+        ctx.budget_batch(th, tf, ntracers=nt + 1)
+
+
 def test_config2_full_size_properties(device):
     """The T/S case at full size (400x400x40 GVC, P2-PDM): one step conserves tracer
     mass to round-off, keeps a constant field constant, and ends the volume
