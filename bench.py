@@ -122,6 +122,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-batched", action="store_true")
     p.add_argument("--no-adjoint", action="store_true")
+    p.add_argument("--no-tidal", action="store_true",
+                   help="skip the 2000x1500x50 tidal section (forward and reverse mode)")
     p.add_argument("--no-ext", action="store_true",
                    help="skip the full-EOS and input-producer timings")
     p.add_argument("--serial", action="store_true",
@@ -337,6 +339,9 @@ def main():
     if a.shard:
         out["shard"] = shard_section(torch, np, gt, inputs, dev, a.batched_steps)
         progress("shard section done")
+    if not a.no_tidal:
+        out["tidal_2000x1500x50"] = tidal_full_section(torch, np, gt, inputs, dev, 5)
+        progress("tidal 2000x1500x50 section done")
     if not a.no_ext:
         out["extensions"] = extensions_section(torch, np, gt, inputs, dev, tr, ctx, st,
                                                a.steps)
@@ -787,6 +792,118 @@ def shard_section(torch, np, gt, inputs, dev, steps):
                             "euler_identity_rel": abs(lhs - rhs) / abs(rhs),
                             "forcing": "24 consecutive macro steps of the M2 tide (the "
                                        "producers' outputs of each step)"}}
+
+
+def tidal_full_section(torch, np, gt, inputs, dev, steps):
+    """The tidal 2000x1500x50 case on one GPU at its stated size: general
+    coordinates, an M2 tide (30 micro steps of a stand-in 2-D mode per 300 s
+    macro step), the input producers on the device, T, S and 16 passive
+    tracers in FP32 advected by the batched step (autotuned) with the FP64
+    mass and volume budgets of all 18 tracers in one pass per step
+    (gt_budget_batch_f32), timed with HIP events on the stream; then the
+    checkpointed FP32 reverse mode of the 18 tracers over 24 macro steps
+    (code_amd/tidal.py TidalAdjointWindow: the tide's state and the tracers
+    every 6 steps, a segment's inputs regenerated by the producers in the
+    reverse sweep, the library's checkpoint every 2 steps inside a segment, 6
+    tracers per call), timed as a whole, with its device memory."""
+    import gc
+
+    from code_amd import tidal
+    gc.collect()
+    torch.cuda.empty_cache()
+    d = inputs.make_domain(2000, 1500, 50, 5, hmin=10.0, hmax=50.0)
+    st = torch.cuda.Stream(device=dev)
+    ctx = gt.Context.from_domain(d, stream=st)
+    nt = 18
+    hires = torch.zeros(d.shape3, dtype=torch.float64, device=dev)
+    t_prod, t_adv, t_bud = (EventTimer(torch, st) for _ in range(3))
+    mass = []
+    with torch.cuda.stream(st):
+        tide = tidal.TidalShare(d, ctx, dev, M=30, dt=300.0, umax=1.0)
+        tide.macro_step()  # the coordinates' first call
+        fin = tidal.initial_tracers(d, dev, nt)
+        snap0 = tide.snapshot()
+        f_init = fin.clone()
+        fout = torch.empty_like(fin)
+        dd, hn = tide.macro_step()
+        bt, bg, _, _ = gt.autotune_batch([dict(dd, ctx=ctx, f_in=fin, f_out=fout,
+                                               hires=hires)], nt, tide.dt, gt.P2_PDM,
+                                         gt.P2_PDM, reps=2, stream=st)
+        tide.restore(snap0)
+        bufs = [fin, fout]
+        for q in range(steps + 1):
+            tide.micro_steps()  # the stand-in 2-D mode: not timed
+            timed = q > 0
+            with (t_prod if timed else _Null()):
+                dd, hn = tide.producers()
+            with (t_adv if timed else _Null()):
+                gt.adv3d_batch([dict(dd, ctx=ctx, f_in=bufs[0], f_out=bufs[1], hires=hires)],
+                               nt, tide.dt, gt.P2_PDM, gt.P2_PDM, stream=st)
+            with (t_bud if timed else _Null()):
+                mass.append(ctx.budget_batch(hn, bufs[1]))
+            bufs.reverse()
+        st.synchronize()
+        m = torch.stack(mass).cpu().numpy()
+        drift = float(np.max(np.abs(m[-1, :nt] - m[0, :nt]) / np.abs(m[0, :nt])))
+        del bufs, fin, fout, mass, dd, hn
+        # reverse mode over 24 macro steps from f_init
+        tide.restore(snap0)
+        del snap0
+        gc.collect()
+        torch.cuda.empty_cache()
+        nadj, seg, inner, batch = 24, 6, 2, 6
+        win = tidal.TidalAdjointWindow(tide, ctx, nsteps=nadj, seg=seg, inner_ckpt=inner,
+                                       batch=batch, stream=st)
+        f_fin = torch.empty_like(f_init)
+        st.synchronize()
+        t0 = time.perf_counter()
+        win.forward(f_init, f_fin, hires=hires)
+        st.synchronize()
+        t1 = time.perf_counter()
+        gen = torch.Generator(device=dev).manual_seed(7)
+        lam = torch.rand(f_init.shape, generator=gen, device=dev, dtype=torch.float32) - 0.5
+        rhs = [float(torch.dot(lam[q].reshape(-1).double(), f_fin[q].reshape(-1).double()))
+               for q in range(nt)]
+        del f_fin
+        torch.cuda.empty_cache()
+        st.synchronize()
+        t2 = time.perf_counter()
+        win.reverse(lam)
+        st.synchronize()
+        t3 = time.perf_counter()
+        free, total = torch.cuda.mem_get_info(dev)
+        used_gb = (total - free) / 1e9
+        lhs = [float(torch.dot(lam[q].reshape(-1).double(), f_init[q].reshape(-1).double()))
+               for q in range(nt)]
+        euler = max(abs(a - b) / abs(b) for a, b in zip(lhs, rhs))
+        win.release()
+        del lam, f_init, win
+    ta, tp, tb = t_adv.mean(), t_prod.mean(), t_bud.mean()
+    ncell = d.imax * d.jmax * d.kmax
+    n3 = (d.kmax + 1) * (d.jmax + 4) * (d.imax + 4)
+    b = ncell * (nt * 2 * 4 + 7 * 8)
+    set_gb = nt * n3 * 4 / 1e9
+    out = {"workload": "tidal 2000x1500x50 general coordinates on one GPU, M2 tide, producers "
+                       "on the device each 300 s macro step, T, S + 16 passive tracers FP32, "
+                       "P2-PDM HALFSPLIT, FP64 budgets of all tracers each step",
+           "rows": rows_str(gt.strip_rows(bt)), "tracers_per_wave": bg, "steps": steps,
+           "ms_per_step": (ta + tp + tb) * 1e3, "advection_ms": ta * 1e3,
+           "producers_ms": tp * 1e3, "budgets_ms": tb * 1e3,
+           "cell_updates_per_s": ncell * nt / ta, "algorithmic_GBps": b / ta / 1e9,
+           "frac_of_8TBps": b / ta / HBM_PEAK, "mass_drift_max_rel": drift,
+           "tracer_set_GB": set_gb,
+           "adjoint_f32": {"steps": nadj, "tide_and_tracer_checkpoint_every": seg,
+                           "inner_ckpt_every": inner, "tracers_per_call": batch,
+                           "tracers": nt, "forward_ms": (t1 - t0) * 1e3,
+                           "reverse_ms": (t3 - t2) * 1e3,
+                           "ms_per_step": (t3 - t2 + t1 - t0) * 1e3 / nadj,
+                           "device_GB_in_use_after_reverse": used_gb,
+                           "euler_identity_rel": euler}}
+    ctx.close()
+    del ctx, tide, hires
+    gc.collect()
+    torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(tr, seconds):

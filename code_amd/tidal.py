@@ -77,6 +77,23 @@ class TidalShare:
         self.micro_steps()
         return self.producers()
 
+    def snapshot(self):
+        """the state a macro step starts from: the time, the elevation, the
+        2-D fields of start_macro and the layer heights the coordinates relax
+        from (hn, hun, hvn; ho, huo, hvo and zc are rewritten from them)"""
+        return (self.t, self.z.clone(), {n: v.clone() for n, v in self.e.items()},
+                {n: self.h[n].clone() for n in ("hn", "hun", "hvn")})
+
+    def restore(self, snap):
+        """back to a snapshot: the macro steps that follow repeat bit for bit"""
+        t, z, e, h = snap
+        self.t = t
+        self.z.copy_(z)
+        for n, v in e.items():
+            self.e[n].copy_(v)
+        for n, v in h.items():
+            self.h[n].copy_(v)
+
     def producers(self):
         """start_macro, coordinates, layer transports, ww_momentum_3d"""
         c, e, h = self.ctx, self.e, self.h
@@ -125,3 +142,82 @@ def initial_tracers(dom, device, nt=18, seed=5):
     f[nt - 1] = 1.0
     f[:, 0] = 0.0
     return f
+
+
+STEP_INPUTS = ("uu", "vv", "ww", "hu", "hv", "ho")
+
+
+class TidalAdjointWindow:
+    """The FP32 reverse mode of the tracers over nsteps macro steps of the
+    tide, with memory bounded for the full 2000x1500x50 grid on one GPU.
+
+    The per-step inputs (six FP64 fields, 7.4 GB a step at that size) are
+    never all kept: the forward sweep keeps, every `seg` steps, the tide's
+    state (snapshot: 2-D fields and three layer-height fields) and the tracer
+    set; the reverse sweep restores each snapshot, regenerates the inputs of
+    that segment's steps through the producers (bit for bit, the producers
+    are deterministic) and hands them with the segment's tracer checkpoint to
+    gt_adv3d_adjoint_seq_f32, `batch` tracers at a time (the tracers are
+    independent given the inputs, so batching changes no bit), the library
+    itself keeping a checkpoint every `inner_ckpt` steps inside the segment.
+    lam_final -> lam0 equals one gt_adv3d_adjoint_seq_f32 call over all
+    nsteps steps with every step's inputs kept."""
+
+    def __init__(self, tide, ctx, nsteps=24, seg=6, inner_ckpt=2, batch=6,
+                 scheme=None, stream=None):
+        from . import transport as gt
+        self.tide, self.ctx = tide, ctx
+        self.nsteps, self.seg, self.inner_ckpt, self.batch = nsteps, seg, inner_ckpt, batch
+        self.scheme = gt.P2_PDM if scheme is None else scheme
+        self.stream = stream
+        self.ckpts = []
+
+    def forward(self, f0, f_out, hires=None):
+        """nsteps batched FP32 steps from f0 (left untouched); the result in
+        f_out; keeps the checkpoints (f0 itself is the first tracer set)"""
+        from . import transport as gt
+        nt = f0.shape[0]
+        self.ckpts = []
+        cur, nxt = f0, f_out
+        spare = None
+        for t in range(self.nsteps):
+            if t % self.seg == 0:
+                self.ckpts.append((self.tide.snapshot(), f0 if t == 0 else cur.clone()))
+            dd, _ = self.tide.macro_step()
+            gt.adv3d_batch([dict(dd, ctx=self.ctx, f_in=cur, f_out=nxt, hires=hires)], nt,
+                           self.tide.dt, self.scheme, self.scheme, stream=self.stream)
+            if t == 0:
+                # f0 stays the caller's: the next steps ping-pong between
+                # f_out and a work set
+                spare = f0.new_empty(f0.shape)
+                cur, nxt = nxt, spare
+            else:
+                cur, nxt = nxt, cur
+        if cur is not f_out:
+            f_out.copy_(cur)
+        del spare
+        return f_out
+
+    def reverse(self, lam):
+        """lam (the adjoint of the final tracer set) is replaced by the
+        adjoint of f0, segment by segment from the last"""
+        nt = lam.shape[0]
+        for s in range(len(self.ckpts) - 1, -1, -1):
+            snap, fck = self.ckpts[s]
+            self.tide.restore(snap)
+            t0 = s * self.seg
+            t1 = min(self.nsteps, t0 + self.seg)
+            seq = []
+            for _ in range(t0, t1):
+                dd, _ = self.tide.macro_step()
+                seq.append(tuple(dd[n].clone() for n in STEP_INPUTS))
+            for b0 in range(0, nt, self.batch):
+                b1 = min(nt, b0 + self.batch)
+                self.ctx.adv3d_adjoint_seq(b1 - b0, self.inner_ckpt, self.tide.dt, fck[b0:b1],
+                                           lam[b0:b1], lam[b0:b1], seq, self.scheme,
+                                           self.scheme)
+            del seq
+        return lam
+
+    def release(self):
+        self.ckpts = []

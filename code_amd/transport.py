@@ -475,10 +475,11 @@ class Context:
 
     def tridiagonal_batch(self, fi, lt, au, bu, cu, du, value):
         """systems along the last axis: arrays of shape (rows, n) with at
-        least lt+1 rows (the library checks 0 <= fi < lt)"""
+        least lt+1 rows; fi, lt outside 0 <= fi < lt go to the library, which
+        rejects them before touching memory"""
         n = au.shape[-1] if au is not None and au.dim() == 2 else 0
         rows = au.shape[0] if n else 0
-        if n and lt >= rows:
+        if n and 0 <= fi < lt and lt >= rows:
             raise ValueError(f"tridiagonal_batch: lt = {lt} needs at least {lt + 1} rows "
                              f"(got {rows})")
         ps = [_dev(t, nm, shape=(rows, n)) for t, nm in ((au, "au"), (bu, "bu"), (cu, "cu"),
