@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/getm_transport.h"
@@ -61,6 +62,7 @@ struct gt_ctx {
   double *bpart = nullptr, *bout = nullptr;
   double* bbatch = nullptr;  // partials of gt_budget_batch_f32 (grown on demand)
   long bbatch_n = 0;
+  double* noalloc = nullptr;  // this device's copy of the static scratch array
   unsigned long long* nwarn = nullptr;
   int adv_ver_iterations = 1;
   int do_mirror_2d = 1;
@@ -148,6 +150,8 @@ static void masks_from_az(long NX, long NY, const int32_t* az, int32_t* au, int3
         ax[i + NX * j] = 1;
 }
 
+static int noalloc_scratch(double** p);
+
 extern "C" int gt_create(gt_ctx** out, int imin, int imax, int jmin, int jmax, int kmax,
                          double dx, double dy, const int32_t* az, const int32_t* au,
                          const int32_t* av, const int32_t* ax) {
@@ -228,6 +232,12 @@ extern "C" int gt_create(gt_ctx** out, int imin, int imax, int jmin, int jmax, i
   if (e != hipSuccess) {
     gt_destroy(c);
     return set_err(GT_EHIP, "gt_create: %s", hipGetErrorString(e));
+  }
+  // the device address of the allocation-free entry points' scratch array,
+  // looked up once per device with that device current
+  if (noalloc_scratch(&c->noalloc) != GT_OK) {
+    gt_destroy(c);
+    return GT_EHIP;  // the message names the failed lookup
   }
   const uint8_t* dm = c->dmask;
   c->az = c->dint; c->au = c->dint + NXY; c->av = c->dint + 2 * NXY; c->ax = c->dint + 3 * NXY;
@@ -1720,6 +1730,54 @@ extern "C" int gt_budget_batch_f32(gt_ctx* c, int ntracers, const double* h, con
   TRY(budget_batch_check("gt_budget_batch_f32", c, ntracers, h, f, out));
   TRY(ensure_bbatch(c, ntracers));
   LCHK(gt::launch_budget_batch<float>(c->st, c->d, c->az, ntracers, c->N3, h, f, c->bbatch,
+                                      kBudgetBlocks, c->dx * c->dy, out));
+  return GT_OK;
+}
+
+// ---- allocation-free budgets --------------------------------------------------------
+// gt_budget_batch_f32_noalloc keeps its partial sums in this array instead of
+// the context: every GPU the library runs on holds one copy until the process
+// ends.  In host code the array's name is a host-side stand-in, not a device
+// address: host code uses only the address hipGetSymbolAddress returns for
+// the current device, looked up once per device (noalloc_scratch, called by
+// gt_create with the context's device current) and kept in the context.
+constexpr int kNoallocMaxTracers = 32;
+constexpr long kNoallocScratch = 2L * (kNoallocMaxTracers + 1) * kBudgetBlocks;
+static __device__ __align__(256) double scratch[kNoallocScratch];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[kMaxDevices] = {};
+
+static int noalloc_scratch(double** p) {
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return set_err(GT_EHIP, "gt_create: device %d is beyond the %d the library tracks", dev,
+                   kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* q = nullptr;
+    HIPCHK(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)));
+    g_scratch_dev[dev] = static_cast<double*>(q);
+  }
+  *p = g_scratch_dev[dev];
+  return GT_OK;
+}
+
+extern "C" int gt_budget_batch_f32_noalloc(gt_ctx* c, int ntracers, const double* h,
+                                           const float* f, double* out) {
+  TRY(budget_batch_check("gt_budget_batch_f32_noalloc", c, ntracers, h, f, out));
+  if (ntracers > kNoallocMaxTracers)
+    return set_err(GT_EARG,
+                   "gt_budget_batch_f32_noalloc: %d tracers exceed the %d its scratch holds "
+                   "(gt_budget_batch_f32 takes any number)",
+                   ntracers, kNoallocMaxTracers);
+  if (!c->noalloc)
+    return set_err(GT_EHIP, "gt_budget_batch_f32_noalloc: no scratch address for this context");
+  // every partial the final pass reads is written by the first pass of the
+  // same call, so nothing is cleared
+  LCHK(gt::launch_budget_batch<float>(c->st, c->d, c->az, ntracers, c->N3, h, f, c->noalloc,
                                       kBudgetBlocks, c->dx * c->dy, out));
   return GT_OK;
 }

@@ -49,6 +49,7 @@ EXPORTS = [
     "gt_salinity_diffusion", "gt_temperature_diffusion", "gt_tridiagonal_batch",
     "gt_uv_advect_3d", "gt_restart_write_host", "gt_restart_read_host", "gt_write_restart",
     "gt_read_restart", "gt_set_substr_ini_press", "gt_reserve", "gt_budget_batch_f32",
+    "gt_budget_batch_f32_noalloc",
 ]
 
 
@@ -155,6 +156,7 @@ def load_library():
         "gt_set_adjoint_mode": (i, [vp, i]),
         "gt_budget_f32": (i, [vp, vp, vp, C.POINTER(d), C.POINTER(d)]),
         "gt_budget_batch_f32": (i, [vp, i, vp, vp, vp]),
+        "gt_budget_batch_f32_noalloc": (i, [vp, i, vp, vp, vp]),
         "gt_adv3d_adjoint": (i, [vp, i, i, d, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i, i,
                                  d, i]),
         "gt_adv3d_adjoint_f32": (i, [vp, i, i, i, d, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,

@@ -490,6 +490,17 @@ int gt_budget_f32(gt_ctx *ctx, const double *h, const float *f, double *mass,
  * in the context (grown on first use, or sized by gt_reserve). */
 int gt_budget_batch_f32(gt_ctx *ctx, int ntracers, const double *h,
                         const float *f, double *out);
+/* gt_budget_batch_f32 that allocates nothing, for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation).  No counterpart in the reference.  The partial
+ * sums go to a static device array of 2*(32+1)*1024 doubles (528 KiB) that
+ * each GPU holds until the process ends, not to the context: at most 32
+ * tracers (more return GT_EARG).  Same kernels, same results bit for bit;
+ * asynchronous on the context's stream, no synchronisation.  All calls on one
+ * GPU share its array: calls that overlap on one GPU (two streams, two
+ * threads) race. */
+int gt_budget_batch_f32_noalloc(gt_ctx *ctx, int ntracers, const double *h,
+                                const float *f, double *out);
 
 /* ---- device memory helpers for callers without a HIP binding ---------- */
 int gt_malloc(void **ptr, size_t bytes);
