@@ -80,13 +80,29 @@ def build_fortran() -> Path | None:
     obj = fb / "getm_transport_gpu.o"
     _run([flang, "-cpp", "-O2", "-fPIC", "-c", str(src), "-o", str(obj),
           "-module-dir", str(fb)])
+    link = ["-L", str(BUILD), "-lgetm_transport", f"-Wl,-rpath,{BUILD}",
+            "-Wl,-rpath,$ORIGIN/.."]
     for name in ("adv_driver", "rows_driver"):
         drv = PKG / "fortran" / f"{name}.F90"
         if drv.exists():
             exe = fb / name
             _run([flang, "-cpp", "-O2", str(drv), str(obj), "-I", str(fb),
-                  "-module-dir", str(fb), "-L", str(BUILD), "-lgetm_transport",
-                  f"-Wl,-rpath,{BUILD}", "-Wl,-rpath,$ORIGIN/..", "-o", str(exe)])
+                  "-module-dir", str(fb), *link, "-o", str(exe)])
+    # the external routines with the reference's argument lists, built
+    # against the test's stand-ins of the host model's modules
+    ext = [PKG / "fortran" / n for n in ("ext_host_modules.F90", "getm_external.F90",
+                                         "ext_driver.F90")]
+    if all(p.exists() for p in ext):
+        eb = fb / "ext"
+        eb.mkdir(parents=True, exist_ok=True)
+        objs = []
+        for src in ext[:2]:
+            o = eb / (src.stem + ".o")
+            _run([flang, "-cpp", "-O2", "-c", str(src), "-I", str(fb), "-I", str(eb),
+                  "-module-dir", str(eb), "-o", str(o)])
+            objs.append(str(o))
+        _run([flang, "-cpp", "-O2", str(ext[2]), *objs, str(obj), "-I", str(fb), "-I", str(eb),
+              "-module-dir", str(eb), *link, "-o", str(fb / "ext_driver")])
     return obj
 
 

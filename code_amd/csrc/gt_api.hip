@@ -481,12 +481,9 @@ extern "C" int gt_adv_interfacial_reconstruction(gt_ctx* c, int scheme, long n,
   return GT_OK;
 }
 
-extern "C" int gt_adv_split_u(gt_ctx* c, double dt, const double* f, double* fi, double* Di,
-                              double* adv, const double* U, const double* DU, double splitfac,
-                              int scheme, double AH, int tag) {
-  if (!c) return set_err(GT_EARG, "gt_adv_split_u: NULL context");
-  const GridMasks* g = grid_for(c, tag);
-  if (!g) return set_err(GT_EINVAL_TAG, "adv_split_u: tag is invalid");
+static int split_u_impl(gt_ctx* c, const GridMasks* g, double dt, const double* f, double* fi,
+                        double* Di, double* adv, const double* U, const double* DU,
+                        double splitfac, int scheme, double AH) {
   if (!split_scheme(scheme)) return set_err(GT_EINVAL_SCHEME, "adv_split_u: scheme is invalid");
   if (f == fi) {
     ENSURE(c->slice3, 3 * c->d.NXY);
@@ -496,12 +493,18 @@ extern "C" int gt_adv_split_u(gt_ctx* c, double dt, const double* f, double* fi,
   return step_u(c, g, 0, 1, f, fi, fi, Di, adv, U, DU, dt, splitfac, scheme, AH);
 }
 
-extern "C" int gt_adv_split_v(gt_ctx* c, double dt, const double* f, double* fi, double* Di,
-                              double* adv, const double* V, const double* DV, double splitfac,
+extern "C" int gt_adv_split_u(gt_ctx* c, double dt, const double* f, double* fi, double* Di,
+                              double* adv, const double* U, const double* DU, double splitfac,
                               int scheme, double AH, int tag) {
-  if (!c) return set_err(GT_EARG, "gt_adv_split_v: NULL context");
+  if (!c) return set_err(GT_EARG, "gt_adv_split_u: NULL context");
   const GridMasks* g = grid_for(c, tag);
-  if (!g) return set_err(GT_EINVAL_TAG, "adv_split_v: tag is invalid");
+  if (!g) return set_err(GT_EINVAL_TAG, "adv_split_u: tag is invalid");
+  return split_u_impl(c, g, dt, f, fi, Di, adv, U, DU, splitfac, scheme, AH);
+}
+
+static int split_v_impl(gt_ctx* c, const GridMasks* g, double dt, const double* f, double* fi,
+                        double* Di, double* adv, const double* V, const double* DV,
+                        double splitfac, int scheme, double AH) {
   if (!split_scheme(scheme)) return set_err(GT_EINVAL_SCHEME, "adv_split_v: scheme is invalid");
   if (f == fi) {
     ENSURE(c->slice3, 3 * c->d.NXY);
@@ -511,22 +514,48 @@ extern "C" int gt_adv_split_v(gt_ctx* c, double dt, const double* f, double* fi,
   return step_v(c, g, 0, 1, f, fi, fi, Di, adv, V, DV, dt, splitfac, scheme, AH);
 }
 
-extern "C" int gt_adv_split_w(gt_ctx* c, double dt, const double* f, double* fi, double* hi,
-                              double* adv, const double* ww, double splitfac, int scheme,
-                              int tag, int itersmax) {
-  if (!c) return set_err(GT_EARG, "gt_adv_split_w: NULL context");
-  if (!grid_for(c, tag)) return set_err(GT_EINVAL_TAG, "adv_split_w: tag is invalid");
+extern "C" int gt_adv_split_v(gt_ctx* c, double dt, const double* f, double* fi, double* Di,
+                              double* adv, const double* V, const double* DV, double splitfac,
+                              int scheme, double AH, int tag) {
+  if (!c) return set_err(GT_EARG, "gt_adv_split_v: NULL context");
+  const GridMasks* g = grid_for(c, tag);
+  if (!g) return set_err(GT_EINVAL_TAG, "adv_split_v: tag is invalid");
+  return split_v_impl(c, g, dt, f, fi, Di, adv, V, DV, splitfac, scheme, AH);
+}
+
+static int split_w_impl(gt_ctx* c, const GridMasks* g, double dt, const double* f, double* fi,
+                        double* hi, double* adv, const double* ww, double splitfac, int scheme,
+                        int tag, int itersmax) {
   if (scheme != GT_NOADV && !split_scheme(scheme))
     return set_err(GT_EINVAL_SCHEME, "adv_split_w: scheme is invalid");
   if (scheme == GT_NOADV) return GT_OK;
-  // adv_split_w receives adv_grid%az; for the 3-D tags that is the H grid
-  const GridMasks* g = grid_for(c, tag);
   if (f == fi) {
     ENSURE(c->tmp, c->N3);
     TRY(step_w(c, g, f, f, c->tmp, hi, adv, ww, dt, splitfac, scheme, tag, itersmax));
     return copy_d2d(c, fi, c->tmp, c->N3);
   }
   return step_w(c, g, f, fi, fi, hi, adv, ww, dt, splitfac, scheme, tag, itersmax);
+}
+
+extern "C" int gt_adv_split_w(gt_ctx* c, double dt, const double* f, double* fi, double* hi,
+                              double* adv, const double* ww, double splitfac, int scheme,
+                              int tag, int itersmax) {
+  if (!c) return set_err(GT_EARG, "gt_adv_split_w: NULL context");
+  // adv_split_w receives adv_grid%az; for the 3-D tags that is the H grid
+  const GridMasks* g = grid_for(c, tag);
+  if (!g) return set_err(GT_EINVAL_TAG, "adv_split_w: tag is invalid");
+  return split_w_impl(c, g, dt, f, fi, hi, adv, ww, splitfac, scheme, tag, itersmax);
+}
+
+static int upstream_2dh_impl(gt_ctx* c, const GridMasks* g, double dt, const double* f,
+                             double* fi, double* Di, double* adv, const double* U,
+                             const double* V, const double* Dn, double AH) {
+  if (f == fi) {
+    ENSURE(c->slice3, 3 * c->d.NXY);
+    TRY(step_up2dh(c, g, 0, 1, f, f, c->slice3, Di, adv, U, V, Dn, dt, AH));
+    return copy_d2d(c, fi, c->slice3, c->d.NXY);
+  }
+  return step_up2dh(c, g, 0, 1, f, fi, fi, Di, adv, U, V, Dn, dt, AH);
 }
 
 extern "C" int gt_adv_upstream_2dh(gt_ctx* c, double dt, const double* f, double* fi,
@@ -537,12 +566,54 @@ extern "C" int gt_adv_upstream_2dh(gt_ctx* c, double dt, const double* f, double
   if (!c) return set_err(GT_EARG, "gt_adv_upstream_2dh: NULL context");
   const GridMasks* g = grid_for(c, tag);
   if (!g) return set_err(GT_EINVAL_TAG, "adv_upstream_2dh: tag is invalid");
-  if (f == fi) {
-    ENSURE(c->slice3, 3 * c->d.NXY);
-    TRY(step_up2dh(c, g, 0, 1, f, f, c->slice3, Di, adv, U, V, Dn, dt, AH));
-    return copy_d2d(c, fi, c->slice3, c->d.NXY);
-  }
-  return step_up2dh(c, g, 0, 1, f, fi, fi, Di, adv, U, V, Dn, dt, AH);
+  return upstream_2dh_impl(c, g, dt, f, fi, Di, adv, U, V, Dn, AH);
+}
+
+// The same four routines with the masks the reference's explicit interfaces
+// take as arguments (advection.F90:91-194, advection_3d.F90:55-68): the
+// caller's t_adv_grid masks as bytes (1 = .true.) over E2DFIELD, and az
+// (adv_grid%az) as int32, all device memory.
+extern "C" int gt_adv_split_u_m(gt_ctx* c, double dt, const double* f, double* fi, double* Di,
+                                double* adv, const double* U, const double* DU,
+                                double splitfac, int scheme, double AH,
+                                const uint8_t* mask_flux, const uint8_t* mask_update) {
+  if (!c || !mask_flux || !mask_update) return set_err(GT_EARG, "gt_adv_split_u_m: NULL argument");
+  GridMasks g{};
+  g.uflux = mask_flux;
+  g.uupd = mask_update;
+  return split_u_impl(c, &g, dt, f, fi, Di, adv, U, DU, splitfac, scheme, AH);
+}
+
+extern "C" int gt_adv_split_v_m(gt_ctx* c, double dt, const double* f, double* fi, double* Di,
+                                double* adv, const double* V, const double* DV,
+                                double splitfac, int scheme, double AH,
+                                const uint8_t* mask_flux, const uint8_t* mask_update) {
+  if (!c || !mask_flux || !mask_update) return set_err(GT_EARG, "gt_adv_split_v_m: NULL argument");
+  GridMasks g{};
+  g.vflux = mask_flux;
+  g.vupd = mask_update;
+  return split_v_impl(c, &g, dt, f, fi, Di, adv, V, DV, splitfac, scheme, AH);
+}
+
+extern "C" int gt_adv_split_w_m(gt_ctx* c, double dt, const double* f, double* fi, double* hi,
+                                double* adv, const double* ww, double splitfac, int scheme,
+                                int tag, const int32_t* az, int itersmax) {
+  if (!c || !az) return set_err(GT_EARG, "gt_adv_split_w_m: NULL argument");
+  if (!grid_for(c, tag)) return set_err(GT_EINVAL_TAG, "adv_split_w: tag is invalid");
+  GridMasks g{};
+  g.az = az;
+  return split_w_impl(c, &g, dt, f, fi, hi, adv, ww, splitfac, scheme, tag, itersmax);
+}
+
+extern "C" int gt_adv_upstream_2dh_m(gt_ctx* c, double dt, const double* f, double* fi,
+                                     double* Di, double* adv, const double* U, const double* V,
+                                     const double* Dn, const double* DU, const double* DV,
+                                     double AH, const int32_t* az) {
+  (void)DU; (void)DV;
+  if (!c || !az) return set_err(GT_EARG, "gt_adv_upstream_2dh_m: NULL argument");
+  GridMasks g{};
+  g.az = az;
+  return upstream_2dh_impl(c, &g, dt, f, fi, Di, adv, U, V, Dn, AH);
 }
 
 extern "C" int gt_do_advection(gt_ctx* c, double dt, double* f, const double* U,

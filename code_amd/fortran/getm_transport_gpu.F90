@@ -35,6 +35,8 @@
    public do_eqstate_gpu, start_macro_gpu, set_coordinates_gpu, coordinates_gpu
    public ww_momentum_3d_gpu, salinity_diffusion_gpu, temperature_diffusion_gpu
    public uv_advect_3d_gpu
+   public adv_split_u_gpu, adv_split_v_gpu, adv_split_w_gpu, adv_upstream_2dh_gpu
+   public ip_blumberg_mellor_gpu, ip_shchepetkin_mcwilliams_gpu
 
    integer, public, parameter :: NOSPLIT=0, FULLSPLIT=1, HALFSPLIT=2, HVSPLIT=3
    integer, public, parameter :: NOADV=0, UPSTREAM=1, UPSTREAM_2DH=2, P2=3
@@ -46,6 +48,9 @@
    integer(c_size_t), save :: n3 = 0, n2 = 0
    ! device mirrors: f uu vv ww hu hv ho hn hires advres
    type(c_ptr), save :: dbuf(10) = c_null_ptr
+   ! device masks of the per-direction routines: two byte masks and one
+   ! integer mask (adv_grid%az), E2DFIELD each
+   type(c_ptr), save :: dmask(3) = c_null_ptr
 
    interface
       function c_gt_create(ctx, imin, imax, jmin, jmax, kmax, dx, dy, az, au, av, ax) &
@@ -74,6 +79,49 @@
          type(c_ptr), value :: ctx, f, uu, vv, ww, hu, hv, ho, hn, hires, advres
          real(c_double), value :: dt, AH
          integer(c_int), value :: split, hscheme, vscheme, tag
+         integer(c_int) :: rc
+      end function
+      function c_gt_adv_split_u_m(ctx, dt, f, fi, Di, adv, U, DU, splitfac, scheme, AH, &
+               mflux, mupd) bind(C, name='gt_adv_split_u_m') result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx, f, fi, Di, adv, U, DU, mflux, mupd
+         real(c_double), value :: dt, splitfac, AH
+         integer(c_int), value :: scheme
+         integer(c_int) :: rc
+      end function
+      function c_gt_adv_split_v_m(ctx, dt, f, fi, Di, adv, V, DV, splitfac, scheme, AH, &
+               mflux, mupd) bind(C, name='gt_adv_split_v_m') result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx, f, fi, Di, adv, V, DV, mflux, mupd
+         real(c_double), value :: dt, splitfac, AH
+         integer(c_int), value :: scheme
+         integer(c_int) :: rc
+      end function
+      function c_gt_adv_split_w_m(ctx, dt, f, fi, hi, adv, ww, splitfac, scheme, tag, az, &
+               itersmax) bind(C, name='gt_adv_split_w_m') result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx, f, fi, hi, adv, ww, az
+         real(c_double), value :: dt, splitfac
+         integer(c_int), value :: scheme, tag, itersmax
+         integer(c_int) :: rc
+      end function
+      function c_gt_adv_upstream_2dh_m(ctx, dt, f, fi, Di, adv, U, V, Dn, DU, DV, AH, az) &
+               bind(C, name='gt_adv_upstream_2dh_m') result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx, f, fi, Di, adv, U, V, Dn, DU, DV, az
+         real(c_double), value :: dt, AH
+         integer(c_int) :: rc
+      end function
+      function c_gt_ip_blumberg_mellor(ctx, H, hn, hun, hvn, buoy, zz, idpdx, idpdy) &
+               bind(C, name='gt_ip_blumberg_mellor') result(rc)
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx, H, hn, hun, hvn, buoy, zz, idpdx, idpdy
+         integer(c_int) :: rc
+      end function
+      function c_gt_ip_shchepetkin_mcwilliams(ctx, H, hn, hun, hvn, buoy, sseo, zz, idpdx, &
+               idpdy) bind(C, name='gt_ip_shchepetkin_mcwilliams') result(rc)
+         import :: c_ptr, c_int
+         type(c_ptr), value :: ctx, H, hn, hun, hvn, buoy, sseo, zz, idpdx, idpdy
          integer(c_int) :: rc
       end function
       function c_gt_do_internal_pressure(ctx, method, H, hn, hun, hvn, buoy, sseo, &
@@ -226,6 +274,9 @@
    do q = 1, size(dbuf)
       call check(c_gt_malloc(dbuf(q), 8_c_size_t * n3), 'gt_init')
    end do
+   do q = 1, size(dmask)
+      call check(c_gt_malloc(dmask(q), 4_c_size_t * n2), 'gt_init')
+   end do
    end subroutine gt_init
 
    subroutine gt_finalize()
@@ -233,6 +284,10 @@
    do q = 1, size(dbuf)
       if (c_associated(dbuf(q))) call check(c_gt_free(dbuf(q)), 'gt_finalize')
       dbuf(q) = c_null_ptr
+   end do
+   do q = 1, size(dmask)
+      if (c_associated(dmask(q))) call check(c_gt_free(dmask(q)), 'gt_finalize')
+      dmask(q) = c_null_ptr
    end do
    if (c_associated(ctx)) call check(c_gt_destroy(ctx), 'gt_finalize')
    ctx = c_null_ptr
@@ -406,6 +461,191 @@
    call check(c_gt_temperature_diffusion(ctx, dt, cnpar, hn, nuh, swr, shf, A, g1, g2, &
               dry_z, S, swr_bot_refl_frac, swr_min_bot_frac, rad, T), 'do_temperature()')
    end subroutine temperature_diffusion_gpu
+
+   ! ---- the per-direction routines with the reference's argument lists ----
+   ! (advection.F90:91-194, advection_3d.F90:55-68; Cartesian build).  The
+   ! slices and masks are copied to the device, the routine runs there, and
+   ! the in/out arrays come back.
+
+   subroutine slice_to_device(dst, src)
+   type(c_ptr), intent(in) :: dst
+   real(c_double), intent(in), target :: src(*)
+   call check(c_gt_memcpy_h2d(dst, c_loc(src), 8_c_size_t * n2), 'slice_to_device')
+   end subroutine slice_to_device
+
+   subroutine slice_to_host(dst, src)
+   real(c_double), intent(inout), target :: dst(*)
+   type(c_ptr), intent(in) :: src
+   call check(c_gt_memcpy_d2h(c_loc(dst), src, 8_c_size_t * n2), 'slice_to_host')
+   end subroutine slice_to_host
+
+   ! a logical mask as E2DFIELD bytes: the entries the caller's array (or
+   ! pointer view) holds, .false. elsewhere (never read there)
+   subroutine mask_to_device(dst, m, i0, j0)
+   type(c_ptr), intent(in) :: dst
+   logical, intent(in) :: m(:,:)
+   integer, intent(in) :: i0, j0   ! the caller's lower bounds of m
+   integer(c_int8_t), allocatable, target :: b(:,:)
+   integer :: i, j
+   allocate(b(il-HALO:ih+HALO, jl-HALO:jh+HALO))
+   b = 0_c_int8_t
+   do j = max(j0, jl-HALO), min(j0 + size(m, 2) - 1, jh+HALO)
+      do i = max(i0, il-HALO), min(i0 + size(m, 1) - 1, ih+HALO)
+         if (m(i-i0+1, j-j0+1)) b(i, j) = 1_c_int8_t
+      end do
+   end do
+   call check(c_gt_memcpy_h2d(dst, c_loc(b), n2), 'mask_to_device')
+   end subroutine mask_to_device
+
+   subroutine az_to_device(dst, az)
+   type(c_ptr), intent(in) :: dst
+   integer, intent(in), target :: az(il-HALO:ih+HALO, jl-HALO:jh+HALO)
+   integer(c_int), allocatable, target :: b(:,:)
+   allocate(b(il-HALO:ih+HALO, jl-HALO:jh+HALO))
+   b = int(az, c_int)
+   call check(c_gt_memcpy_h2d(dst, c_loc(b), 4_c_size_t * n2), 'az_to_device')
+   end subroutine az_to_device
+
+   ! adv_split_u (adv_split_u.F90:8-295); mask_flux as the reference's
+   ! pointer dummy (its lower bounds kept), mask_update E2DFIELD
+   subroutine adv_split_u_gpu(dt, f, fi, Di, adv, U, DU, splitfac, scheme, AH, mask_flux, &
+                              mask_update)
+   real(c_double), intent(in) :: dt, splitfac, AH
+   real(c_double), dimension(il-HALO:ih+HALO, jl-HALO:jh+HALO), intent(in) :: f, U, DU
+   real(c_double), dimension(il-HALO:ih+HALO, jl-HALO:jh+HALO), intent(inout) :: fi, Di, adv
+   integer, intent(in) :: scheme
+   logical, dimension(:,:), pointer, intent(in) :: mask_flux
+   logical, dimension(il-HALO:ih+HALO, jl-HALO:jh+HALO), intent(in) :: mask_update
+   call slice_to_device(dbuf(1), f)
+   call slice_to_device(dbuf(2), fi)
+   call slice_to_device(dbuf(3), Di)
+   call slice_to_device(dbuf(4), adv)
+   call slice_to_device(dbuf(5), U)
+   call slice_to_device(dbuf(6), DU)
+   call mask_to_device(dmask(1), mask_flux, lbound(mask_flux, 1), lbound(mask_flux, 2))
+   call mask_to_device(dmask(2), mask_update, il-HALO, jl-HALO)
+   call check(c_gt_adv_split_u_m(ctx, dt, dbuf(1), dbuf(2), dbuf(3), dbuf(4), dbuf(5), &
+              dbuf(6), splitfac, int(scheme, c_int), AH, dmask(1), dmask(2)), 'adv_split_u')
+   call slice_to_host(fi, dbuf(2))
+   call slice_to_host(Di, dbuf(3))
+   call slice_to_host(adv, dbuf(4))
+   end subroutine adv_split_u_gpu
+
+   ! adv_split_v (adv_split_v.F90:8-145); mask_flux has NY-1 rows
+   subroutine adv_split_v_gpu(dt, f, fi, Di, adv, V, DV, splitfac, scheme, AH, mask_flux, &
+                              mask_update)
+   real(c_double), intent(in) :: dt, splitfac, AH
+   real(c_double), dimension(il-HALO:ih+HALO, jl-HALO:jh+HALO), intent(in) :: f, V, DV
+   real(c_double), dimension(il-HALO:ih+HALO, jl-HALO:jh+HALO), intent(inout) :: fi, Di, adv
+   integer, intent(in) :: scheme
+   logical, dimension(il-HALO:ih+HALO, jl-HALO:jh+HALO-1), intent(in) :: mask_flux
+   logical, dimension(il-HALO:ih+HALO, jl-HALO:jh+HALO), intent(in) :: mask_update
+   call slice_to_device(dbuf(1), f)
+   call slice_to_device(dbuf(2), fi)
+   call slice_to_device(dbuf(3), Di)
+   call slice_to_device(dbuf(4), adv)
+   call slice_to_device(dbuf(5), V)
+   call slice_to_device(dbuf(6), DV)
+   call mask_to_device(dmask(1), mask_flux, il-HALO, jl-HALO)
+   call mask_to_device(dmask(2), mask_update, il-HALO, jl-HALO)
+   call check(c_gt_adv_split_v_m(ctx, dt, dbuf(1), dbuf(2), dbuf(3), dbuf(4), dbuf(5), &
+              dbuf(6), splitfac, int(scheme, c_int), AH, dmask(1), dmask(2)), 'adv_split_v')
+   call slice_to_host(fi, dbuf(2))
+   call slice_to_host(Di, dbuf(3))
+   call slice_to_host(adv, dbuf(4))
+   end subroutine adv_split_v_gpu
+
+   ! adv_split_w (adv_split_w.F90:7-360), I3DFIELD arrays
+   subroutine adv_split_w_gpu(dt, f, fi, hi, adv, ww, splitfac, scheme, tag, az, itersmax)
+   real(c_double), intent(in) :: dt, splitfac
+   real(c_double), dimension(il-HALO:ih+HALO, jl-HALO:jh+HALO, 0:km), intent(in) :: f, ww
+   real(c_double), dimension(il-HALO:ih+HALO, jl-HALO:jh+HALO, 0:km), intent(inout) :: &
+        fi, hi, adv
+   integer, intent(in) :: scheme, tag, itersmax
+   integer, dimension(il-HALO:ih+HALO, jl-HALO:jh+HALO), intent(in) :: az
+   call gt_to_device(dbuf(1), f)
+   call gt_to_device(dbuf(2), fi)
+   call gt_to_device(dbuf(3), hi)
+   call gt_to_device(dbuf(4), adv)
+   call gt_to_device(dbuf(5), ww)
+   call az_to_device(dmask(3), az)
+   call check(c_gt_adv_split_w_m(ctx, dt, dbuf(1), dbuf(2), dbuf(3), dbuf(4), dbuf(5), &
+              splitfac, int(scheme, c_int), int(tag, c_int), dmask(3), &
+              int(itersmax, c_int)), 'adv_split_w')
+   call gt_to_host(fi, dbuf(2))
+   call gt_to_host(hi, dbuf(3))
+   call gt_to_host(adv, dbuf(4))
+   end subroutine adv_split_w_gpu
+
+   ! adv_upstream_2dh (adv_upstream_2dh.F90:8-169)
+   subroutine adv_upstream_2dh_gpu(dt, f, fi, Di, adv, U, V, Dn, DU, DV, AH, az)
+   real(c_double), intent(in) :: dt, AH
+   real(c_double), dimension(il-HALO:ih+HALO, jl-HALO:jh+HALO), intent(in) :: &
+        f, U, V, Dn, DU, DV
+   real(c_double), dimension(il-HALO:ih+HALO, jl-HALO:jh+HALO), intent(inout) :: fi, Di, adv
+   integer, dimension(il-HALO:ih+HALO, jl-HALO:jh+HALO), intent(in) :: az
+   call slice_to_device(dbuf(1), f)
+   call slice_to_device(dbuf(2), fi)
+   call slice_to_device(dbuf(3), Di)
+   call slice_to_device(dbuf(4), adv)
+   call slice_to_device(dbuf(5), U)
+   call slice_to_device(dbuf(6), V)
+   call slice_to_device(dbuf(7), Dn)
+   call slice_to_device(dbuf(8), DU)
+   call slice_to_device(dbuf(9), DV)
+   call az_to_device(dmask(3), az)
+   call check(c_gt_adv_upstream_2dh_m(ctx, dt, dbuf(1), dbuf(2), dbuf(3), dbuf(4), dbuf(5), &
+              dbuf(6), dbuf(7), dbuf(8), dbuf(9), AH, dmask(3)), 'adv_upstream_2dh')
+   call slice_to_host(fi, dbuf(2))
+   call slice_to_host(Di, dbuf(3))
+   call slice_to_host(adv, dbuf(4))
+   end subroutine adv_upstream_2dh_gpu
+
+   ! ip_blumberg_mellor (ip_blumberg_mellor.F90:8-152) on the module
+   ! variables it reads, passed as host arrays: zz, idpdx, idpdy come back
+   subroutine ip_blumberg_mellor_gpu(H, hn, hun, hvn, buoy, zz, idpdx, idpdy)
+   real(c_double), dimension(il-HALO:ih+HALO, jl-HALO:jh+HALO), intent(in) :: H
+   real(c_double), dimension(il-HALO:ih+HALO, jl-HALO:jh+HALO, 0:km), intent(in) :: &
+        hn, hun, hvn, buoy
+   real(c_double), dimension(il-HALO:ih+HALO, jl-HALO:jh+HALO, 0:km), intent(inout) :: &
+        zz, idpdx, idpdy
+   call slice_to_device(dbuf(1), H)
+   call gt_to_device(dbuf(2), hn)
+   call gt_to_device(dbuf(3), hun)
+   call gt_to_device(dbuf(4), hvn)
+   call gt_to_device(dbuf(5), buoy)
+   call gt_to_device(dbuf(6), zz)
+   call gt_to_device(dbuf(7), idpdx)
+   call gt_to_device(dbuf(8), idpdy)
+   call check(c_gt_ip_blumberg_mellor(ctx, dbuf(1), dbuf(2), dbuf(3), dbuf(4), dbuf(5), &
+              dbuf(6), dbuf(7), dbuf(8)), 'ip_blumberg_mellor()')
+   call gt_to_host(zz, dbuf(6))
+   call gt_to_host(idpdx, dbuf(7))
+   call gt_to_host(idpdy, dbuf(8))
+   end subroutine ip_blumberg_mellor_gpu
+
+   ! ip_shchepetkin_mcwilliams (ip_shchepetkin_mcwilliams.F90:8-291)
+   subroutine ip_shchepetkin_mcwilliams_gpu(H, hn, hun, hvn, buoy, sseo, zz, idpdx, idpdy)
+   real(c_double), dimension(il-HALO:ih+HALO, jl-HALO:jh+HALO), intent(in) :: H, sseo
+   real(c_double), dimension(il-HALO:ih+HALO, jl-HALO:jh+HALO, 0:km), intent(in) :: &
+        hn, hun, hvn, buoy
+   real(c_double), dimension(il-HALO:ih+HALO, jl-HALO:jh+HALO, 0:km), intent(inout) :: &
+        zz, idpdx, idpdy
+   call slice_to_device(dbuf(1), H)
+   call gt_to_device(dbuf(2), hn)
+   call gt_to_device(dbuf(3), hun)
+   call gt_to_device(dbuf(4), hvn)
+   call gt_to_device(dbuf(5), buoy)
+   call slice_to_device(dbuf(9), sseo)
+   call gt_to_device(dbuf(6), zz)
+   call gt_to_device(dbuf(7), idpdx)
+   call gt_to_device(dbuf(8), idpdy)
+   call check(c_gt_ip_shchepetkin_mcwilliams(ctx, dbuf(1), dbuf(2), dbuf(3), dbuf(4), &
+              dbuf(5), dbuf(9), dbuf(6), dbuf(7), dbuf(8)), 'ip_shchepetkin_mcwilliams()')
+   call gt_to_host(zz, dbuf(6))
+   call gt_to_host(idpdx, dbuf(7))
+   call gt_to_host(idpdy, dbuf(8))
+   end subroutine ip_shchepetkin_mcwilliams_gpu
 
    ! update_3d_halo (halo_zones.F90:141-187) on host arrays, serial semantics
    subroutine update_3d_halo(f1, f2, mask, imin, jmin, imax, jmax, kmax, tag, mirror)

@@ -49,7 +49,8 @@ EXPORTS = [
     "gt_salinity_diffusion", "gt_temperature_diffusion", "gt_tridiagonal_batch",
     "gt_uv_advect_3d", "gt_restart_write_host", "gt_restart_read_host", "gt_write_restart",
     "gt_read_restart", "gt_set_substr_ini_press", "gt_reserve", "gt_budget_batch_f32",
-    "gt_budget_batch_f32_noalloc",
+    "gt_budget_batch_f32_noalloc", "gt_adv_split_u_m", "gt_adv_split_v_m", "gt_adv_split_w_m",
+    "gt_adv_upstream_2dh_m",
 ]
 
 
@@ -114,6 +115,10 @@ def load_library():
         "gt_adv_split_v": (i, [vp, d, vp, vp, vp, vp, vp, vp, d, i, d, i]),
         "gt_adv_split_w": (i, [vp, d, vp, vp, vp, vp, vp, d, i, i, i]),
         "gt_adv_upstream_2dh": (i, [vp, d, vp, vp, vp, vp, vp, vp, vp, vp, vp, d, i]),
+        "gt_adv_split_u_m": (i, [vp, d, vp, vp, vp, vp, vp, vp, d, i, d, vp, vp]),
+        "gt_adv_split_v_m": (i, [vp, d, vp, vp, vp, vp, vp, vp, d, i, d, vp, vp]),
+        "gt_adv_split_w_m": (i, [vp, d, vp, vp, vp, vp, vp, d, i, i, vp, i]),
+        "gt_adv_upstream_2dh_m": (i, [vp, d, vp, vp, vp, vp, vp, vp, vp, vp, vp, d, vp]),
         "gt_do_advection": (i, [vp, d, vp, vp, vp, vp, vp, vp, vp, i, i, d, i, vp, vp]),
         "gt_do_advection_3d": (i, [vp, d, vp, vp, vp, vp, vp, vp, vp, vp, i, i, i, d, i,
                                    vp, vp]),

@@ -105,6 +105,28 @@ int gt_adv_upstream_2dh(gt_ctx *ctx, double dt, const double *f, double *fi,
                         double *Di, double *adv, const double *U,
                         const double *V, const double *Dn, const double *DU,
                         const double *DV, double AH, int tag);
+/* The four routines above with the masks the reference's explicit
+ * interfaces take as arguments instead of a tag (advection.F90:91-194,
+ * advection_3d.F90:55-68): the caller's t_adv_grid masks as device bytes
+ * (1 = .true.) over E2DFIELD, as seen through the reference's pointer
+ * views, and adv_grid%az as device int32.  adv_split_v reads rows 0..NY-2
+ * of mask_flux only (the reference declares it with NY-1 rows). */
+int gt_adv_split_u_m(gt_ctx *ctx, double dt, const double *f, double *fi,
+                     double *Di, double *adv, const double *U, const double *DU,
+                     double splitfac, int scheme, double AH,
+                     const uint8_t *mask_flux, const uint8_t *mask_update);
+int gt_adv_split_v_m(gt_ctx *ctx, double dt, const double *f, double *fi,
+                     double *Di, double *adv, const double *V, const double *DV,
+                     double splitfac, int scheme, double AH,
+                     const uint8_t *mask_flux, const uint8_t *mask_update);
+int gt_adv_split_w_m(gt_ctx *ctx, double dt, const double *f, double *fi,
+                     double *hi, double *adv, const double *ww,
+                     double splitfac, int scheme, int tag, const int32_t *az,
+                     int itersmax);
+int gt_adv_upstream_2dh_m(gt_ctx *ctx, double dt, const double *f, double *fi,
+                          double *Di, double *adv, const double *U,
+                          const double *V, const double *Dn, const double *DU,
+                          const double *DV, double AH, const int32_t *az);
 /* do_advection, src/2d/advection.F90:362-702 (one slice).  Dires and advres
  * are the optional outputs (NULL = absent). */
 int gt_do_advection(gt_ctx *ctx, double dt, double *f, const double *U,

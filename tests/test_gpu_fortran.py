@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from code_amd import build, inputs
+from code_amd import transport as gt
 from oracle import pyoracle as po
 from tests.util import rel_err
 
@@ -80,3 +81,80 @@ def test_fortran_rows_driver(tmp_path, device):
     for q, (name, want) in enumerate((("rho", rho), ("buoy", buoy), ("alpha", alpha),
                                       ("beta", beta), ("S", S))):
         assert np.array_equal(out[q], want), (name, rel_err(out[q], want))
+
+
+@pytest.mark.parametrize("scheme", [1, 4, 6])
+def test_fortran_external_routines(tmp_path, device, scheme):
+    """The external routines with the reference's own argument lists
+    (code_amd/fortran/getm_external.F90), called from Fortran as
+    do_advection and do_internal_pressure call them: adv_split_u with the H
+    grid's and the U grid's (shifted, NX-1 columns) flux masks, adv_split_v
+    with the NY-1-row view of the flux mask, adv_split_w, adv_upstream_2dh
+    and the argument-less ip_blumberg_mellor() / ip_shchepetkin_mcwilliams()
+    on the host model's module variables: bit for bit the oracle's routines
+    with the masks of init_advection."""
+    exe = build.BUILD / "fortran" / "ext_driver"
+    if not exe.exists():
+        build.build_fortran()
+    tr = inputs.ts_case(seed=5, imax=29, jmax=23, kmax=6)
+    d = tr.dom
+    k = 3
+    dt, AH, splitfac = tr.dt, 2.0, 0.5
+    orc = po.Oracle.from_domain(d)
+    f = tr.tracers["T"].copy()
+    rho, buoy = np.zeros(d.shape3), np.zeros(d.shape3)
+    orc.eos_linear(f, tr.tracers["S"], rho, buoy, inputs.EOS_T0, inputs.EOS_S0,
+                   inputs.EOS_DTR0, inputs.EOS_DSR0)
+    sseo = inputs.uniform(d.H.shape, 9, -0.3, 0.3)
+    hdr = np.array([d.imin, d.imax, d.jmin, d.jmax, d.kmax, scheme, k, 1], np.int32)
+    rhdr = np.array([d.dx, d.dy, dt, AH, splitfac])
+    with open(tmp_path / "in.bin", "wb") as fh:
+        fh.write(hdr.tobytes())
+        fh.write(rhdr.tobytes())
+        for m in (d.az, d.au, d.av, d.ax):
+            fh.write(np.ascontiguousarray(m, np.int32).tobytes())
+        for a in (d.H, sseo, f, tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho, tr.hn, buoy):
+            fh.write(np.ascontiguousarray(a, np.float64).tobytes())
+    r = subprocess.run([str(exe), str(tmp_path / "in.bin"), str(tmp_path / "out.bin")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(tmp_path / "out.bin", np.float64)
+    n2, n3 = int(np.prod(d.H.shape)), int(np.prod(d.shape3))
+    sizes = [3 * n2] * 4 + [3 * n3] * 3
+    assert raw.size == sum(sizes)
+    parts = np.split(raw, np.cumsum(sizes)[:-1])
+    two = [p.reshape(3, *d.H.shape) for p in parts[:4]]
+    three = [p.reshape(3, *d.shape3) for p in parts[4:]]
+
+    def slice_case(call, *args, tag=gt.H_TAG):
+        fi, Di, adv = f[k].copy(), tr.ho[k].copy(), np.zeros(d.H.shape)
+        call(dt, np.ascontiguousarray(f[k]), fi, Di, adv, *args, tag=tag)
+        return fi, Di, adv
+
+    want = [
+        slice_case(lambda *a, tag: orc.adv_split_u(*a[:5], np.ascontiguousarray(tr.uu[k]),
+                                                    np.ascontiguousarray(tr.hun[k]), splitfac,
+                                                    scheme, AH, tag)),
+        slice_case(lambda *a, tag: orc.adv_split_v(*a[:5], np.ascontiguousarray(tr.vv[k]),
+                                                    np.ascontiguousarray(tr.hvn[k]), splitfac,
+                                                    scheme, AH, tag)),
+        slice_case(lambda *a, tag: orc.adv_split_u(*a[:5], np.ascontiguousarray(tr.uu[k]),
+                                                    np.ascontiguousarray(tr.hun[k]), splitfac,
+                                                    scheme, AH, tag), tag=gt.U_TAG),
+        slice_case(lambda *a, tag: orc.adv_upstream_2dh(
+            *a[:5], *(np.ascontiguousarray(x[k]) for x in (tr.uu, tr.vv, tr.hn, tr.hun, tr.hvn)),
+            AH, tag)),
+    ]
+    for q, (w, got) in enumerate(zip(want, two)):
+        for name, a, b in zip(("fi", "Di", "adv"), w, got):
+            assert np.array_equal(a, b), (q, name, rel_err(b, a))
+    fi3, hi3, adv3 = f.copy(), tr.ho.copy(), np.zeros(d.shape3)
+    orc.adv_split_w(dt, f, fi3, hi3, adv3, tr.ww, 1.0, scheme, gt.H_TAG, 1)
+    for name, a, b in zip(("fi", "hi", "adv"), (fi3, hi3, adv3), three[0]):
+        assert np.array_equal(a, b), ("w", name, rel_err(b, a))
+    for q, fn in enumerate((orc.ip_blumberg_mellor, orc.ip_shchepetkin_mcwilliams)):
+        zz, ix, iy = (np.zeros(d.shape3) for _ in range(3))
+        args = (d.H, tr.hn, tr.hun, tr.hvn, buoy) + ((sseo,) if q else ())
+        fn(*args, zz, ix, iy)
+        for name, a, b in zip(("zz", "idpdx", "idpdy"), (zz, ix, iy), three[1 + q]):
+            assert np.array_equal(a, b), ("ip", q, name, rel_err(b, a))
