@@ -864,6 +864,7 @@ def tidal_full_section(torch, np, gt, inputs, dev, steps):
         lam = torch.rand(f_init.shape, generator=gen, device=dev, dtype=torch.float32) - 0.5
         rhs = [float(torch.dot(lam[q].reshape(-1).double(), f_fin[q].reshape(-1).double()))
                for q in range(nt)]
+        scale = [float(lam[q].double().norm() * f_fin[q].double().norm()) for q in range(nt)]
         del f_fin
         torch.cuda.empty_cache()
         st.synchronize()
@@ -876,6 +877,7 @@ def tidal_full_section(torch, np, gt, inputs, dev, steps):
         lhs = [float(torch.dot(lam[q].reshape(-1).double(), f_init[q].reshape(-1).double()))
                for q in range(nt)]
         euler = max(abs(a - b) / abs(b) for a, b in zip(lhs, rhs))
+        euler_cs = max(abs(a - b) / c for a, b, c in zip(lhs, rhs, scale))
         win.release()
         del lam, f_init, win
     ta, tp, tb = t_adv.mean(), t_prod.mean(), t_bud.mean()
@@ -898,7 +900,11 @@ def tidal_full_section(torch, np, gt, inputs, dev, steps):
                            "reverse_ms": (t3 - t2) * 1e3,
                            "ms_per_step": (t3 - t2 + t1 - t0) * 1e3 / nadj,
                            "device_GB_in_use_after_reverse": used_gb,
-                           "euler_identity_rel": euler}}
+                           "euler_identity_rel": euler,
+                           "euler_identity_rel_to_norms": euler_cs,
+                           "euler_note": "max over tracers of |<lam0,f0> - <lamT,fT>| over "
+                                         "|<lamT,fT>| (the random zero-mean lamT makes that "
+                                         "dot product small) and over |lamT| |fT|"}}
     ctx.close()
     del ctx, tide, hires
     gc.collect()
