@@ -23,6 +23,10 @@
 
 using gt::Dims;
 
+// blocks of the budgets' partial sums (the batched form uses the same number,
+// so each of its outputs equals gt_budget_f32's bit for bit)
+constexpr int kBudgetBlocks = 1024;
+
 static thread_local char g_err[512] = "";
 
 static int set_err(int code, const char* fmt, ...) {
@@ -450,6 +454,7 @@ static void fill_domain(gt_ctx* c, const GridMasks* g, gt::FusedDomain<TF>* D, c
   D->flags = g->flags;
   D->hires = hires; D->advres = advres;
   D->dx = c->dx; D->dy = c->dy; D->ard1 = c->ard1;
+  D->bpart = nullptr;
 }
 
 // the strip kernels address one field of one tracer with 32-bit byte offsets
@@ -460,6 +465,15 @@ static bool strip_fits(const gt_ctx* c) {
 // bytes of work fields a domain needs for fill_domain
 static size_t strip_ws_bytes(const gt_ctx* c, size_t tf_bytes, int ntracers, bool adv) {
   return sizeof(double) * (size_t)c->N3 * (adv ? 2 : 1) + tf_bytes * (size_t)c->N3 * ntracers;
+}
+
+// strip-levels of the second strip kernel at its smallest strip height (8
+// rows): the most budget partials a tuning can ask for
+static long budget_strip_levels(const gt_ctx* c) {
+  return (long)((c->d.NX + gt::kStripOut - 1) / gt::kStripOut) * ((c->d.NY + 7) / 8) * c->kmax;
+}
+static size_t budget_ws_bytes(const gt_ctx* c, int ntracers) {
+  return sizeof(double) * 2 * (size_t)(ntracers + 1) * (size_t)budget_strip_levels(c);
 }
 
 // ---- advection entry points --------------------------------------------------
@@ -1298,6 +1312,11 @@ static int batch_substeps_f32(const gt_batch_domain& bd, const GridMasks* g, int
                        bd.vv, bd.ww, bd.hu, bd.hv, hs, vs, AH, kshift));
   if (bd.hires)
     HIPCHK(hipMemcpyAsync(bd.hires, Dh[4], sizeof(double) * N3, hipMemcpyDeviceToDevice, st));
+  if (bd.budget) {
+    TRY(ensure_bbatch(c, ntr));
+    LCHK(gt::launch_budget_batch<float>(st, c->d, c->az, ntr, N3, Dh[4], (const float*)bd.f_out,
+                                        c->bbatch, kBudgetBlocks, c->dx * c->dy, bd.budget));
+  }
   return GT_OK;
 }
 
@@ -1324,6 +1343,9 @@ static int batch_impl(int ndom, const gt_batch_domain* doms, int ntracers, doubl
                                   "stay below 4 GiB in FP64 (split the domain)", i, c->N3);
     gsel = grid_for(c, tag);
     if (!gsel) return set_err(GT_EINVAL_TAG, "gt_adv3d_batch: tag is invalid");
+    if (doms[i].budget && gsel != &c->H)
+      return set_err(GT_EARG, "gt_adv3d_batch: budgets are formed on the T points (tags H, D, "
+                              "W); domain %d asks for one on a U or V grid", i);
     // the strip kernels keep their work fields in the context: one context
     // twice in a launch would have two domains write the same work fields
     for (int q = 0; q < i; q++)
@@ -1344,7 +1366,8 @@ static int batch_impl(int ndom, const gt_batch_domain* doms, int ntracers, doubl
   const int group = sizeof(TF) == 4 ? c0->group : (ntracers >= 2 && c0->group != 1 ? 2 : 1);
   for (int i = 0; i < ndom; i++) {
     gt_ctx* c = doms[i].ctx;
-    TRY(ensure_ws(c, strip_ws_bytes(c, sizeof(TF), ntracers, false)));
+    TRY(ensure_ws(c, strip_ws_bytes(c, sizeof(TF), ntracers, false) +
+                         (doms[i].budget ? budget_ws_bytes(c, ntracers) : 0)));
   }
   for (int b = 0; b < ndom; b += gt::kMaxFusedDomains) {
     gt::FusedArgs<TF> A{};
@@ -1355,6 +1378,9 @@ static int batch_impl(int ndom, const gt_batch_domain* doms, int ntracers, doubl
       fill_domain<TF>(c, grid_for(c, tag), &A.doms[i], (const TF*)bd.f_in, (TF*)bd.f_out,
                       c->N3, ntracers, bd.uu, bd.vv, bd.ww, bd.hu, bd.hv, bd.ho, bd.hires,
                       nullptr);
+      // the budget partials follow the work fields
+      if (bd.budget)
+        A.doms[i].bpart = (double*)((char*)c->ws + strip_ws_bytes(c, sizeof(TF), ntracers, false));
     }
     A.ndom = nd;
     A.ntracers = ntracers;
@@ -1367,6 +1393,15 @@ static int batch_impl(int ndom, const gt_batch_domain* doms, int ntracers, doubl
     } else {
       LCHK(gt::launch_fused_f64(st, A, group, false));
     }
+    // the domains' budgets from their strip-level partials, one launch
+    gt::BudgetPairsSet S{};
+    for (int i = 0; i < nd; i++) {
+      const gt::FusedDomain<TF>& D = A.doms[i];
+      if (!D.bpart) continue;
+      S.d[S.n++] = {D.bpart, doms[b + i].budget, (long)D.nsx * D.nsy[1] * D.d.kmax,
+                    D.dx * D.dy, ntracers + 1};
+    }
+    LCHK(gt::launch_budget_pairs_final(st, S));
   }
   return GT_OK;
 }
@@ -1743,9 +1778,6 @@ extern "C" int gt_eos_linear_adjoint(gt_ctx* c, const double* buoy_bar, double* 
 }
 
 // ---- budgets -----------------------------------------------------------------------
-// blocks of the partial sums (the batched form uses the same number, so each
-// of its outputs equals gt_budget_f32's bit for bit)
-constexpr int kBudgetBlocks = 1024;
 
 template <typename TF>
 static int budget_impl(gt_ctx* c, const double* h, const TF* f, double* mass, double* volume) {

@@ -35,6 +35,10 @@ struct FusedDomain {
   const uint8_t* flags;  // MB_* bits of the mask set of the grid
   double* hires;      // optional: end of the layer-height trajectory
   double* advres;     // optional (FP64, one tracer): accumulated advective term
+  // optional: the budget partials of the second kernel, one (hi, lo) pair
+  // per output q (tracer q, the volume at q = ntracers) and strip-level sl:
+  // bpart[2*(q*nsl + sl) + {0,1}], nsl = nsx*nsy[1]*kmax
+  double* bpart;
   double dx, dy, ard1;
 };
 

@@ -51,6 +51,22 @@ __device__ __forceinline__ double from_upper_lane(double v) {
   return __builtin_bit_cast(double, (unsigned long long)lo | ((unsigned long long)hi << 32));
 }
 
+// double-double accumulation (Knuth's two-sum of the running sum and the
+// addend, the error carried in sl): the budgets' compensated sums
+__device__ __forceinline__ void dd_add(double& sh, double& sl, double a) {
+  double s = sh + a;
+  double bv = s - sh;
+  double e = (sh - (s - bv)) + (a - bv);
+  e += sl;
+  double r = s + e;
+  sl = e - (r - s);
+  sh = r;
+}
+__device__ __forceinline__ void dd_add_dd(double& sh, double& sl, double ah, double al) {
+  dd_add(sh, sl, ah);
+  dd_add(sh, sl, al);
+}
+
 template <typename T>
 __device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
 template <typename T>

@@ -128,5 +128,20 @@ template <typename TF>
 hipError_t launch_budget_batch(hipStream_t st, Dims d, const int32_t* az, int ntr, long tstride,
                                const double* h, const TF* f, double* partial, int nblocks,
                                double area, double* out);
+// the final pass alone, for up to 8 domains in one launch: for domain i,
+// out[q] = area * (sum of the npairs (hi, lo) pairs at partial + 2*q*npairs),
+// q < nout, in k_budget_batch_final's order
+struct BudgetPairs {
+  const double* partial;
+  double* out;
+  long npairs;
+  double area;
+  int nout;
+};
+struct BudgetPairsSet {
+  BudgetPairs d[8];
+  int n;
+};
+hipError_t launch_budget_pairs_final(hipStream_t st, const BudgetPairsSet& S);
 
 }  // namespace gt

@@ -864,20 +864,8 @@ __global__ void k_tile_exchange(int ntiles, int px, TileSet ts, int nk) {
 // ---------------------------------------------------------------------------
 // a15: budgets.  Error-free accumulation: each thread keeps a double-double
 // (hi, lo) sum of h*f (exact products via fma) and of h; waves and blocks
-// combine double-doubles; one partial per block, then one block finishes.
-__device__ __forceinline__ void dd_add(double& sh, double& sl, double a) {
-  double s = sh + a;
-  double bv = s - sh;
-  double e = (sh - (s - bv)) + (a - bv);
-  e += sl;
-  double r = s + e;
-  sl = e - (r - s);
-  sh = r;
-}
-__device__ __forceinline__ void dd_add_dd(double& sh, double& sl, double ah, double al) {
-  dd_add(sh, sl, ah);
-  dd_add(sh, sl, al);
-}
+// combine double-doubles (dd_add, gt_device.h); one partial per block, then
+// one block finishes.
 
 template <typename TF>
 __global__ void k_budget_partial(Dims d, const int32_t* __restrict__ az,
@@ -1028,16 +1016,14 @@ __global__ __launch_bounds__(256) void k_budget_batch_partial(
   }
 }
 
-// one block per output q: the partials of all blocks in k_budget_final's
-// order and rounding; out[q] = sum * area
-__global__ __launch_bounds__(256) void k_budget_batch_final(int nblocks,
-                                                            const double* __restrict__ partial,
-                                                            double area, double* out) {
+// one block: the npairs (hi, lo) pairs at pq in k_budget_final's order and
+// rounding; *outq = sum * area
+__device__ __forceinline__ void budget_pairs_block(long npairs, const double* __restrict__ pq,
+                                                   double area, double* outq) {
   __shared__ double sm[2][256];
   const int t = threadIdx.x;
-  const double* pq = partial + 2 * (long)blockIdx.x * nblocks;
   double sh = 0, sl = 0;
-  for (int b = t; b < nblocks; b += blockDim.x) dd_add_dd(sh, sl, pq[2 * b], pq[2 * b + 1]);
+  for (long b = t; b < npairs; b += blockDim.x) dd_add_dd(sh, sl, pq[2 * b], pq[2 * b + 1]);
   sm[0][t] = sh;
   sm[1][t] = sl;
   __syncthreads();
@@ -1054,7 +1040,29 @@ __global__ __launch_bounds__(256) void k_budget_batch_final(int nblocks,
   const double p = sm[0][0] * area;
   double pe = fma(sm[0][0], area, -p);
   pe += sm[1][0] * area;
-  out[blockIdx.x] = p + pe;
+  *outq = p + pe;
+}
+
+// one block per output q: out[q] from the partials of all blocks
+__global__ __launch_bounds__(256) void k_budget_batch_final(int nblocks,
+                                                            const double* __restrict__ partial,
+                                                            double area, double* out) {
+  budget_pairs_block(nblocks, partial + 2 * (long)blockIdx.x * nblocks, area,
+                     out + blockIdx.x);
+}
+
+// the budgets of several domains in one launch: nout blocks per domain, one
+// per output
+__global__ __launch_bounds__(256) void k_budget_pairs_final(BudgetPairsSet S) {
+  int i = 0;
+  long b0 = 0;
+  while (i + 1 < S.n && blockIdx.x >= b0 + S.d[i].nout) {
+    b0 += S.d[i].nout;
+    i++;
+  }
+  const BudgetPairs& D = S.d[i];
+  const long q = blockIdx.x - b0;
+  budget_pairs_block(D.npairs, D.partial + 2 * q * D.npairs, D.area, D.out + q);
 }
 
 }  // namespace gt
@@ -1237,6 +1245,14 @@ hipError_t launch_budget_batch(hipStream_t st, Dims d, const int32_t* az, int nt
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_budget_batch_final, dim3(ntr + 1), dim3(256), 0, st, nblocks, partial,
                      area, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_budget_pairs_final(hipStream_t st, const BudgetPairsSet& S) {
+  long nblocks = 0;
+  for (int i = 0; i < S.n; i++) nblocks += S.d[i].nout;
+  if (nblocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_budget_pairs_final, dim3((unsigned)nblocks), dim3(256), 0, st, S);
   return hipGetLastError();
 }
 

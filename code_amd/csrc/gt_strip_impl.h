@@ -675,6 +675,16 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
   double d3w[4] = {1, 1, 1, 1};
   double a3w[ADV ? 4 : 1][G];
   TF f3w[G][4];
+  // budget (when a.bpart is set): the lane's sums over the rows it writes of
+  // hires*f per tracer and of hires, interior water cells only, kept in LDS
+  // (one slot per lane, output and wavefront) so they cost no registers
+  const bool bud = a.bpart != nullptr;
+  __shared__ double bsh[4][G + 1][64];
+  const int wv = threadIdx.x >> 6;
+  if (bud) {
+#pragma unroll
+    for (int q = 0; q <= G; q++) bsh[wv][q][lane] = 0.0;
+  }
   TF fvp[G];
 #pragma unroll
   for (int g = 0; g < G; g++) {
@@ -737,6 +747,9 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
     }
     const bool st_ok = vrow && (fow[w1] & MB_XOUT) && yv < NY;
     const uint32_t cv = (uint32_t)(x + NX * yv);
+    // interior water cell of the mass and volume budgets
+    const bool bcell = bud && st_ok && (flw[w1] & MB_WCOL) && x >= 2 && x <= NX - 3 &&
+                       yv >= 2 && yv <= NY - 3;
 
 #pragma unroll
     for (int g = 0; g < G; g++) {
@@ -796,6 +809,7 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
             F5 = updu ? u : F4;
           }
           if (st_ok && g < ntr) sto(fo_g[g], cv * (uint32_t)sizeof(TF), F5);
+          if (bcell) bsh[wv][g][lane] = __builtin_fma(D5, (double)F5, bsh[wv][g][lane]);
           if constexpr (ADV) {
             if (st_ok && g == 0 && a.advres) {
               const double A4 = a3w[w1][0] + advv;
@@ -807,6 +821,7 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
       }
     }
     if (st_ok && it.grp == 0 && hires) sto(hires, cv * 8u, D5);
+    if (bcell && it.grp == 0) bsh[wv][G][lane] += D5;
     load_row(y + 2, buf);
   };
 
@@ -818,6 +833,29 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
     step(Slot<1>{}, rb, s + 1);
     step(Slot<2>{}, ra, s + 2);
     step(Slot<3>{}, rb, s + 3);
+  }
+
+  // the budget partials of this strip-level: the lanes' sums added over the
+  // wavefront in a six-level FP64 tree (a partial is then within R + 6
+  // roundings of exact for terms of one sign, R the strip's rows), written
+  // by lane 0 as a (value, 0) pair; the final pass adds the pairs as
+  // double-doubles
+  if (bud) {
+    const long nsl = (long)a.nsx * a.nsy[1] * kmax;
+    const long sl = ((long)(K - 1) * a.nsy[1] + it.sy) * a.nsx + it.sx;
+    auto wave_sum = [&](double v, int q) {
+      double h = v;
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1) h += __shfl_xor(h, m);
+      if (lane == 0) {
+        a.bpart[2 * ((long)q * nsl + sl)] = h;
+        a.bpart[2 * ((long)q * nsl + sl) + 1] = 0.0;
+      }
+    };
+#pragma unroll
+    for (int g = 0; g < G; g++)
+      if (g < ntr) wave_sum(bsh[wv][g][lane], t0 + g);
+    if (it.grp == 0) wave_sum(bsh[wv][G][lane], A.ntracers);
   }
 
   // level 0 passes through (the step updates levels 1..kmax)

@@ -59,7 +59,7 @@ class BatchDomain(C.Structure):
     _fields_ = [("ctx", C.c_void_p), ("f_in", C.c_void_p), ("f_out", C.c_void_p),
                 ("uu", C.c_void_p), ("vv", C.c_void_p), ("ww", C.c_void_p),
                 ("hu", C.c_void_p), ("hv", C.c_void_p), ("ho", C.c_void_p),
-                ("hires", C.c_void_p)]
+                ("hires", C.c_void_p), ("budget", C.c_void_p)]
 
 
 class RestartField(C.Structure):
@@ -690,7 +690,8 @@ def _batch_array(domains, ntracers):
     """gt_batch_domain[] from dicts, each tensor checked against its
     context: one tracer dtype for the whole batch, f_in/f_out of shape
     (ntracers, kmax+1, NY, NX), FP64 velocities and heights, one context per
-    domain"""
+    domain; optional hires and budget (ntracers + 1 FP64 values: the
+    masses of the result, then the volume)"""
     import torch
     if not domains:
         raise ValueError("adv3d_batch: no domains")
@@ -712,8 +713,11 @@ def _batch_array(domains, ntracers):
             raise ValueError(f"adv3d_batch: domain {q} f_in and f_out must differ")
         vel = [ctx._f3(d[k], f"domain {q} {k}") for k in ("uu", "vv", "ww", "hu", "hv", "ho")]
         hires = ctx._f3(d.get("hires"), f"domain {q} hires", True)
+        budget = _dev(d.get("budget"), f"domain {q} budget", numel=ntracers + 1,
+                      optional=True)
         arr[q] = BatchDomain(ctx.h.value, d["f_in"].data_ptr(), d["f_out"].data_ptr(),
-                             *(v.value for v in vel), hires.value if hires is not None else None)
+                             *(v.value for v in vel), hires.value if hires is not None else None,
+                             budget.value if budget is not None else None)
     return arr
 
 
@@ -760,7 +764,8 @@ def autotune_batch(domains, ntracers, dt, hscheme, vscheme, AH=-1.0, tag=H_TAG, 
 def adv3d_batch(domains, ntracers, dt, hscheme, vscheme, AH=-1.0, tag=H_TAG, stream=None):
     """gt_adv3d_batch_f32/_f64.  domains: list of dicts with keys ctx, f_in,
     f_out (tensors of shape (ntracers, kmax+1, NY, NX)), uu, vv, ww, hu, hv,
-    ho and optional hires.  The dtype of f_in picks the FP32 or FP64 kernel."""
+    ho and optional hires and budget (ntracers + 1 FP64 values, formed in
+    the step).  The dtype of f_in picks the FP32 or FP64 kernel."""
     import torch
     lib = load_library()
     n = len(domains)

@@ -365,6 +365,15 @@ typedef struct {
   void *f_out;
   const double *uu, *vv, *ww, *hu, *hv, *ho;
   double *hires;
+  /* optional (device, ntracers + 1 doubles, NULL = none): the FP64 budgets
+   * of the result formed inside the step, tracer t's mass
+   * sum(hires*f_out*dx*dy) over the interior water cells (az == 1) of levels
+   * 1..kmax, then the volume sum(hires*dx*dy); each lane sums its column's
+   * products over its strip's rows in FP64, a wavefront adds its lanes in
+   * an FP64 tree and the strip-level partials are added as double-doubles,
+   * so for terms of one sign the result is within (rows + 6) roundings of
+   * the error-free gt_budget_batch_f32.  Tags H, D, W (the T points) only. */
+  double *budget;
 } gt_batch_domain;
 int gt_adv3d_batch_f32(int ndom, const gt_batch_domain *doms, int ntracers,
                        double dt, int hscheme, int vscheme, double AH, int tag,

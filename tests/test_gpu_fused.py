@@ -235,3 +235,73 @@ def test_batch_f32_multitracer_workload_full_size(device):
                 m1 = ctxs[s].budget(hires, tins[s][q])[0]
                 assert abs(m1 - m0[s][q]) <= 1e-6 * abs(m0[s][q]), (s, n, m0[s][q], m1)
         print(f"rows {rows} group {group}: worst field {worst}")
+
+
+@pytest.mark.parametrize("rows,group,npassive,fused", [
+    ((8, 16), 9, 7, 1),   # the bench's tuning
+    ((16, 32), 3, 5, 1),  # 7 tracers: groups of 3, 3 and 1
+    ((8, 8), 6, 4, 1),
+    ((16, 16), 3, 3, 0)])  # the per-sub-step kernels: the separate one-pass budget
+def test_batch_budgets_formed_in_the_step(device, rows, group, npassive, fused):
+    """gt_batch_domain.budget: the FP64 masses of the step's result and its
+    volume, formed inside the second strip kernel (lane sums over the strip's
+    rows, double-double over lanes and strip-levels), on three uneven
+    domains in one launch.  Against the error-free gt_budget_batch_f32 of
+    the same result and hires: within 1e-14 relative per tracer and for the
+    volume (a lane's FP64 sum of at most 32 positive products and the
+    six-level FP64 tree over the wavefront are within 38 roundings of exact,
+    4.2e-15; the double-double additions after them lose nothing; all
+    tracers here are positive).  The per-sub-step path forms them with gt_budget_batch_f32
+    itself: equal bit for bit.  A U-grid tag with a budget is rejected."""
+    sizes = [(24, 20), (71, 33), (30, 52)]
+    subs = inputs.multitracer_case(seed=11, kmax=7, sizes=sizes, npassive=npassive)
+    names = list(subs[0].tracers)
+    nt = len(names)
+    doms, outs, buds, ctxs = [], [], [], []
+    for tr in subs:
+        ctx = gt.Context.from_domain(tr.dom)
+        ctxs.append(ctx)
+        tin = to_dev(np.stack([tr.tracers[n] for n in names]).astype(np.float32), device)
+        host = [tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho]
+        dd = dict(zip(NAMES[:6], [to_dev(a, device) for a in host]))
+        out = torch.empty_like(tin)
+        bud = torch.full((nt + 1,), float("nan"), dtype=torch.float64, device=device)
+        doms.append(dict(ctx=ctx, f_in=tin, f_out=out, hires=torch.empty_like(dd["ho"]),
+                         budget=bud, **dd))
+        outs.append(out)
+        buds.append(bud)
+    ctxs[0].set_tuning(fused=fused, tile=gt.tile_for_rows(*rows), group=group)
+    gt.adv3d_batch(doms, nt, subs[0].dt, gt.P2_PDM, gt.P2_PDM)
+    torch.cuda.synchronize()
+    for s in range(len(subs)):
+        exact = ctxs[s].budget_batch(doms[s]["hires"], outs[s]).cpu().numpy()
+        got = buds[s].cpu().numpy()
+        if not fused:
+            assert np.array_equal(got, exact), s
+            continue
+        err = np.abs(got - exact) / np.abs(exact)
+        assert err.max() <= 1e-14, (s, err.max())
+        assert np.all(np.isfinite(got))
+    with pytest.raises(gt.GtError):
+        gt.adv3d_batch(doms, nt, subs[0].dt, gt.P2_PDM, gt.P2_PDM, tag=gt.U_TAG)
+
+
+def test_batch_budgets_f64(device):
+    """the FP64 batched step (T and S) with its budgets formed in the step,
+    against gt_budget of each result (error-free): within 1e-14 relative"""
+    tr = inputs.ts_case(seed=4, imax=125, jmax=38, kmax=9)
+    d = tr.dom
+    ctx = gt.Context.from_domain(d)
+    host = [tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho]
+    dv = dict(zip(NAMES[:6], [to_dev(a, device) for a in host]))
+    fin = to_dev(np.stack([tr.tracers["T"], tr.tracers["S"]]), device)
+    fout = torch.empty_like(fin)
+    hires = torch.empty_like(dv["ho"])
+    bud = torch.zeros(3, dtype=torch.float64, device=device)
+    gt.adv3d_batch([dict(ctx=ctx, f_in=fin, f_out=fout, hires=hires, budget=bud, **dv)], 2,
+                   tr.dt, gt.P2_PDM, gt.P2_PDM)
+    got = bud.cpu().numpy()
+    for q in range(2):
+        m, v = ctx.budget(hires, fout[q])
+        assert abs(got[q] - m) <= 1e-14 * abs(m), (q, got[q], m)
+        assert abs(got[2] - v) <= 1e-14 * abs(v)
