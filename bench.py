@@ -128,6 +128,9 @@ def parse():
                    help="skip the full-EOS and input-producer timings")
     p.add_argument("--serial", action="store_true",
                    help="run EOS + pressure and the advection on one stream")
+    p.add_argument("--separate-ip", action="store_true",
+                   help="run the EOS and each pressure method as separate launches in the "
+                        "step (default: the fused two-launch form)")
     p.add_argument("--shard", action="store_true",
                    help="also time one GPU's share (1000x375x50, 18 FP32 tracers) of the "
                         "2000x1500x50 case")
@@ -220,6 +223,7 @@ def main():
     bufs = [TS, TS2]
     t_adv = EventTimer(torch, st)
     t_ip = EventTimer(torch, st)
+    t_ip_other = EventTimer(torch, st)
 
     tuned = None
     rows64, g64 = parse_rows(a.rows64), 2
@@ -234,12 +238,23 @@ def main():
 
     ev = {}
 
-    def ip_chain(c, s, cur):
+    def ip_separate(c, s, cur):
+        # EOS, then each method from the stored buoyancy (five launches)
         c.eos_linear(cur[0], cur[1], None, buoy, *consts)
         ev["eos"] = torch.cuda.Event()
         ev["eos"].record(s)
         c.ip_blumberg_mellor(H, hn, hun, hvn, buoy, zz, idpdx, idpdy)
         c.ip_shchepetkin_mcwilliams(H, hn, hun, hvn, buoy, sseo, zz6, idpdx6, idpdy6)
+
+    def ip_fused(c, s, cur):
+        # both methods with the EOS inside (two launches); T and S are read
+        # until the last one ends
+        c.eos_internal_pressure(3, cur[0], cur[1], *consts, H, hn, hun, hvn, sseo, zz, idpdx,
+                                idpdy, zz6, idpdx6, idpdy6)
+        ev["eos"] = torch.cuda.Event()
+        ev["eos"].record(s)
+
+    ip_chain = ip_separate if a.separate_ip else ip_fused
 
     def advect(cur, nxt):
         gt.adv3d_batch([dict(ctx=ctx, f_in=cur, f_out=nxt, uu=uu, vv=vv, ww=ww, hu=hun,
@@ -283,6 +298,8 @@ def main():
                 advect(cur, nxt)
             with t_ip:
                 ip_chain(ctx, st, cur)
+            with t_ip_other:
+                (ip_fused if a.separate_ip else ip_separate)(ctx, st, cur)
             bufs.reverse()
         st.synchronize()
     ms = (t1 - t0) / a.steps * 1e3
@@ -301,9 +318,12 @@ def main():
             "valu_issue": valu_issue("headline", rows64, g64, adv_s),
             "bytes_per_launch": b_adv, "launch_us": adv_s * 1e6,
             "bytes_per_cell_tracer": b_adv / (ncell * ntr)}
-    # EOS (read T S, write buoy) + two internal-pressure methods (read buoy hn
-    # hun hvn, write idpdx idpdy): 24 + 2*48 B/cell
-    b_ip = ncell * (24 + 2 * 48)
+    # EOS + both internal-pressure methods, algorithmic bytes: T S hn hun hvn
+    # read, zz idpdx idpdy of each method written, 88 B/cell (the separate
+    # chain also writes and reads back the buoyancy, 16 B/cell more)
+    ip_o = t_ip_other.mean()
+    ip_fused_s, ip_sep_s = (ip_o, ip_s) if a.separate_ip else (ip_s, ip_o)
+    b_ip = ncell * 88
     out = {
         "metric": METRIC,
         "value": value,
@@ -324,10 +344,13 @@ def main():
                    "wet_cell_fraction": float((d.az[2:-2, 2:-2] == 1).mean())},
         "fp64_advection_cell_updates_per_s": ncell * ntr / adv_s,
         "components_ms": {"advection_T_S": adv_s * 1e3, "eos_and_pressure": ip_s * 1e3,
+                          "eos_and_pressure_fused": ip_fused_s * 1e3,
+                          "eos_and_pressure_separate": ip_sep_s * 1e3,
+                          "eos_and_pressure_in_step": "separate" if a.separate_ip else "fused",
                           "timing": "each component alone on one stream, after the timed "
                                     "steps"},
         "streams": 1 if a.serial else 2,
-        "eos_and_pressure_GBps": b_ip / ip_s / 1e9,
+        "eos_and_pressure_GBps": b_ip / ip_fused_s / 1e9,
         "tuning": tuned,
         "roofline": roof,
     }

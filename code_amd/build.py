@@ -25,7 +25,7 @@ ORACLE_LIB = "libgetm_oracle.so"
 HIP_SOURCES = ["gt_kernels.hip", "gt_api.hip", "gt_batched_f64.hip", "gt_batched_f32_g1.hip",
                "gt_batched_f32_g3.hip", "gt_batched_f32_g6.hip", "gt_batched_f32_g9.hip",
                "gt_adjoint.hip", "gt_substeps.hip", "gt_eqstate.hip", "gt_inputs.hip",
-               "gt_diffusion.hip", "gt_momentum.hip", "gt_restart.cpp"]
+               "gt_diffusion.hip", "gt_momentum.hip", "gt_restart.cpp", "gt_ip_fused.hip"]
 
 # -ffp-contract=off: no fused multiply-add, so FP64 results round exactly as
 # the CPU build's.  Packed FP32 instructions are disabled on purpose.

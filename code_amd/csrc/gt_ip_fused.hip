@@ -1,0 +1,391 @@
+// gt_ip_fused.hip -- the linear equation of state fused into the internal
+// pressure gradients of Blumberg-Mellor (src/3d/ip_blumberg_mellor.F90) and
+// Shchepetkin-McWilliams (src/3d/ip_shchepetkin_mcwilliams.F90), either or
+// both, in two launches:
+//   k_ip_columns  one thread per column: the level-0 zeros, the interface
+//                 heights of Blumberg-Mellor and the layer centres of
+//                 Shchepetkin-McWilliams, both from one read of hn;
+//   k_ip_rows     one wavefront per row of a 60-column strip, marching down
+//                 the levels: the buoyancy formed from T and S where it is
+//                 read (never stored), Blumberg-Mellor's two downward
+//                 recurrences, the pressure integral P of
+//                 Shchepetkin-McWilliams and the Jacobians of both faces.
+// Every expression is the per-method kernels' (gt_kernels.hip), in their
+// order, so the gradients are bit for bit those of gt_eos_linear followed by
+// gt_ip_blumberg_mellor / gt_ip_shchepetkin_mcwilliams.
+#include <hip/hip_runtime.h>
+
+#include "gt_ip.h"
+#include "gt_kernels.h"
+
+namespace gt {
+
+template <bool BM, bool SM>
+__global__ __launch_bounds__(256) void k_ip_columns(Dims d, const int32_t* __restrict__ az,
+                                                    const double* __restrict__ H,
+                                                    const double* __restrict__ hn,
+                                                    double* __restrict__ zzb,
+                                                    double* __restrict__ zzs,
+                                                    double* __restrict__ ixb,
+                                                    double* __restrict__ iyb,
+                                                    double* __restrict__ ixs,
+                                                    double* __restrict__ iys) {
+  const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  const int jj = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ii >= d.NX || jj >= d.NY) return;
+  const long NXY = d.NXY;
+  const int kmax = d.kmax;
+  const long c0 = ii + (long)d.NX * jj;
+  if constexpr (BM) {
+    zzb[c0] = 0.0;
+    ixb[c0] = 0.0;
+    iyb[c0] = 0.0;
+  }
+  if constexpr (SM) {
+    zzs[c0] = 0.0;
+    ixs[c0] = 0.0;
+    iys[c0] = 0.0;
+  }
+  if (az[c0] < 1) return;
+  // Blumberg-Mellor's interfaces over [imin..imax+1]x[jmin..jmax+1]
+  // (ip_blumberg_mellor.F90:83-92), Shchepetkin-McWilliams' centres over
+  // the whole halo range (ip_shchepetkin_mcwilliams.F90:90-99)
+  const bool bm = BM && ii >= 2 && ii <= d.NX - 2 && jj >= 2 && jj <= d.NY - 2;
+  if (!bm && !SM) return;
+  const double h1 = hn[c0 + NXY];
+  double zb = -H[c0] + h1;
+  double zs = -H[c0] + 0.5 * h1;
+  if (bm) zzb[c0 + NXY] = zb;
+  if constexpr (SM) zzs[c0 + NXY] = zs;
+  double hprev = h1;
+  constexpr int kB = 8;  // levels whose heights are loaded together
+  for (int kb = 2; kb <= kmax; kb += kB) {
+    double q[kB];
+#pragma unroll
+    for (int u = 0; u < kB; u++) q[u] = hn[c0 + NXY * min(kb + u, kmax)];
+#pragma unroll
+    for (int u = 0; u < kB; u++) {
+      const bool on = kb + u <= kmax;
+      const double hk = q[u];
+      const double zbn = zb + hk;
+      const double zsn = zs + 0.5 * (hprev + hk);
+      zb = on ? zbn : zb;
+      zs = on ? zsn : zs;
+      hprev = on ? hk : hprev;
+      if (on) {
+        if (bm) zzb[c0 + NXY * (kb + u)] = zb;
+        if constexpr (SM) zzs[c0 + NXY * (kb + u)] = zs;
+      }
+    }
+  }
+}
+
+// the downward recurrence of Shchepetkin-McWilliams' pressure integral P of
+// one column (ip_shchepetkin_mcwilliams.F90:108-146): the state at level k
+// holds b1 = b(k+1), b0 = b(k), z1 = z(k+1), z0 = z(k), the harmonic means of
+// the differences above, and p = P(k+1) until step() forms P(k)
+struct PCol {
+  double p, hR1, hZ1, rk, zk, b1, b0, z1, z0;
+  // P(kmax) from the top two levels
+  __device__ __forceinline__ void top(double bt, double bb, double zt, double zb, double ht) {
+    b1 = bt; b0 = bb; z1 = zt; z0 = zb;
+    const double cff1 = 1.0 / (z1 - z0);
+    const double cff2 = 0.5 * (b1 - b0) * 0.5 * ht * cff1;
+    p = (b1 + cff2) * 0.5 * ht;
+    rk = b1 - b0;
+    zk = z1 - z0;
+    hR1 = sm_hmR(rk, rk);
+    hZ1 = sm_hmZ(zk, zk);
+  }
+  // P(k) for k < kmax from b(k-1), z(k-1) (unused at k = 1, where the
+  // differences below are those above: dR(0) = dR(1)), then the window moves
+  // down one level
+  __device__ __forceinline__ void step(int k, double bm, double zm) {
+    double rkm, zkm;
+    if (k >= 2) {
+      rkm = b0 - bm;
+      zkm = z0 - zm;
+    } else {
+      rkm = b1 - b0;
+      zkm = z1 - z0;
+      bm = 0.0;
+      zm = 0.0;
+    }
+    const double hR = sm_hmR(rk, rkm), hZ = sm_hmZ(zk, zkm);
+    p = p + 0.5 * ((b1 + b0) * (z1 - z0) -
+                   SM_ONEFIFTH * ((hR1 - hR) * (z1 - z0 - SM_C12 * (hZ1 + hZ)) -
+                                  (hZ1 - hZ) * (b1 - b0 - SM_C12 * (hR1 + hR))));
+    hR1 = hR;
+    hZ1 = hZ;
+    rk = rkm;
+    zk = zkm;
+    b1 = b0;
+    b0 = bm;
+    z1 = z0;
+    z0 = zm;
+  }
+};
+
+// the face term of Shchepetkin-McWilliams (ip_shchepetkin_mcwilliams.F90:
+// 188-209 for x, 262-284 for y): b, z at the two cells of the face, the
+// harmonic means either side, P either side, the surface elevation either
+// side and b(kmax) either side
+__device__ __forceinline__ double sm_face(double b0, double b1, double z0, double z1,
+                                          double dR0, double dR1, double dZ0, double dZ1,
+                                          double p0, double p1, double h, double dm1, double s0,
+                                          double s1, double k0, double k1) {
+  double AJ = p1 - p0;
+  const double FC = 0.5 * ((b1 + b0) * (z1 - z0) -
+                           SM_ONEFIFTH * ((dR1 - dR0) * (z1 - z0 - SM_C12 * (dZ1 + dZ0)) -
+                                          (dZ1 - dZ0) * (b1 - b0 - SM_C12 * (dR1 + dR0))));
+  AJ = AJ + FC;
+  return AJ * dm1 * h - h * dm1 * (s1 - s0) * 0.5 * (k1 + k0);
+}
+
+// levels by which k_ip_rows loads ahead of its arithmetic
+constexpr int kIpAhead = 1;
+
+// the operands of one level k of the march, loaded one level ahead of their
+// use (raw T and S: the buoyancy is formed where it is used, so the loads
+// stay in flight across the previous level's arithmetic).  Row y is the
+// wavefront's own row.
+struct IpLevel {
+  double t0, s0, z0;  // T S zz(centres) at (y, k-1): the P step (SM); T S at (y, k) (BM only)
+  double tp, sp, zp;  // T S zz(centres) at (y+1, k)
+  double tm, sm, zm;  // T S zz(centres) at (y-1, k) (SM)
+  double zb0, zbp;    // zz(interfaces) at (y, k), (y+1, k) (BM)
+  double hu, hv;      // hun hvn at (y, k)
+};
+
+// One workgroup of W+1 wavefronts (W without Shchepetkin-McWilliams) per
+// strip of 60 interior columns x W rows, marching down the levels;
+// wavefront w takes row y = j0 + w and lane l
+// column x = 60*sx + 1 + l (lanes 1..60 of wavefronts 0..W-1 write; the
+// last wavefront only carries row j0+W for the one below it).  The
+// i-neighbours x-1, x+1, x+2 are adjacent lanes (DPP).  Each wavefront
+// carries the P recurrence of its row and forms the two harmonic means of
+// its row across y (rows y-1|y|y+1); the y face of row y takes P and those
+// means of row y+1 from the wavefront above through LDS (one barrier per
+// level, two LDS slots by level parity).  A harmonic mean across a face is
+// formed once and shared by the two faces that use it, from the same
+// differences, so every gradient rounds as the per-face formula.  Rows past
+// NY-1 and columns past NX-1 are clamped (read, never written).
+template <bool BM, bool SM, int W>
+__global__ __launch_bounds__(64 * (W + 1)) void k_ip_rows(
+    Dims d, int nsx, int nby, const int32_t* __restrict__ au, const int32_t* __restrict__ av,
+    const double* __restrict__ T, const double* __restrict__ S, double T0, double S0,
+    double dtr0, double dsr0, const double* __restrict__ hn, const double* __restrict__ hun,
+    const double* __restrict__ hvn, const double* __restrict__ zzb,
+    const double* __restrict__ zzs, const double* __restrict__ sseo, double* __restrict__ ixb,
+    double* __restrict__ iyb, double* __restrict__ ixs, double* __restrict__ iys, double dxm1,
+    double dym1) {
+  // workgroups are dealt round-robin to the 8 XCDs: workgroup b takes the
+  // work of (b%8)*per + b/8, so each XCD marches a contiguous run of strips
+  // (rows fastest) whose shared rows meet in its L2; the test is uniform
+  // over the workgroup, so every wavefront that starts reaches each barrier
+  const long nblk = (long)nsx * nby;
+  const long per = (nblk + 7) / 8;
+  const long blk = (long)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (blk >= nblk) return;
+  const int by = (int)(blk % nby), sx = (int)(blk / nby);
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const bool carrier = SM && wave == W;  // row j0+W: P and means only
+  const int NX = d.NX, NY = d.NY, kmax = d.kmax;
+  const long NXY = d.NXY;
+  const int x = 60 * sx + 1 + lane;  // >= 1
+  const int xc = x < NX ? x : NX - 1;
+  const int y = 2 + W * by + wave;
+  // 32-bit cell offsets within a level (a level holds < 2^31 cells)
+  auto cell = [&](int yy) -> unsigned { return (unsigned)(xc + NX * (yy < NY ? yy : NY - 1)); };
+  const unsigned cm = cell(y - 1), c0 = cell(y), cp = cell(y + 1);
+  const bool out = !carrier && lane >= 1 && lane <= 60 && x <= NX - 3 && y <= NY - 3;
+  const bool ux = out && au[c0] >= 1, vy = out && av[c0] >= 1;
+  auto buoy = [&](double t, double s) { return eos_linear_buoy(t, s, T0, S0, dtr0, dsr0); };
+  auto load = [&](int k, IpLevel& L) {
+    const long ok = NXY * (long)k, okm = NXY * (long)(k >= 2 ? k - 1 : 1);
+    const long o0 = SM ? okm : ok;
+    L.t0 = (T + o0)[c0];
+    L.s0 = (S + o0)[c0];
+    L.tp = (T + ok)[cp];
+    L.sp = (S + ok)[cp];
+    if constexpr (SM) {
+      L.z0 = (zzs + o0)[c0];
+      L.zp = (zzs + ok)[cp];
+      L.tm = (T + ok)[cm];
+      L.sm = (S + ok)[cm];
+      L.zm = (zzs + ok)[cm];
+    }
+    if constexpr (BM) {
+      L.zb0 = (zzb + ok)[c0];
+      L.zbp = (zzb + ok)[cp];
+    }
+    L.hu = (hun + ok)[c0];
+    L.hv = (hvn + ok)[c0];
+  };
+
+  // ---- level kmax
+  const long okx = NXY * (long)kmax;
+  const double kb0 = buoy((T + okx)[c0], (S + okx)[c0]);
+  const double kbp = buoy((T + okx)[cp], (S + okx)[cp]);
+  // Blumberg-Mellor (ip_blumberg_mellor.F90:99-149), as k_bm_idpd
+  double grdlx = 0, buoylx = 0, hkx = 0, prgrx = 0, grdly = 0, buoyly = 0, hky = 0, prgry = 0;
+  if constexpr (BM) {
+    const double be = from_upper_lane(kb0);
+    grdlx = (be - kb0) * dxm1;
+    buoylx = 0.5 * (be + kb0);
+    hkx = (hun + okx)[c0];
+    prgrx = grdlx * 0.5 * hkx;
+    grdly = (kbp - kb0) * dym1;
+    buoyly = 0.5 * (kbp + kb0);
+    hky = (hvn + okx)[c0];
+    prgry = grdly * 0.5 * hky;
+    if (ux) (ixb + okx)[c0] = hkx * prgrx;
+    if (vy) (iyb + okx)[c0] = hky * prgry;
+  }
+  // Shchepetkin-McWilliams: P of the row, the face constants
+  __shared__ double sh[SM ? 2 : 1][3][W + 1][64];
+  PCol ps{};
+  double ss0 = 0, ssp = 0, sU = 0, kU = 0;
+  // the faces of level k; ps holds b(k), z(k) in b1/z1 and P(k) in p
+  auto faces = [&](int k, const IpLevel& L) {
+    const long ok = NXY * (long)k;
+    const int slot = k & 1;
+    const double b0 = ps.b1, z0 = ps.z1, p0 = ps.p;
+    const double bm = buoy(L.tm, L.sm), bp = buoy(L.tp, L.sp);
+    // y: the means of rows y-1|y|y+1, shared with the wavefront below
+    const double hZ = sm_hmx(z0 - L.zm, L.zp - z0), hR = sm_hmx(b0 - bm, bp - b0);
+    sh[slot][0][wave][lane] = p0;
+    sh[slot][1][wave][lane] = hZ;
+    sh[slot][2][wave][lane] = hR;
+    if (!carrier) {
+      // x: the means of columns x-1|x|x+1 on this lane, x|x+1|x+2 on the next
+      const double bL = from_lower_lane(b0), zL = from_lower_lane(z0);
+      const double bU = from_upper_lane(b0), zU = from_upper_lane(z0), pU = from_upper_lane(p0);
+      const double dZ = sm_hmx(z0 - zL, zU - z0), dR = sm_hmx(b0 - bL, bU - b0);
+      const double dZ1 = from_upper_lane(dZ), dR1 = from_upper_lane(dR);
+      if (ux)
+        (ixs + ok)[c0] = sm_face(b0, bU, z0, zU, dR, dR1, dZ, dZ1, p0, pU, L.hu, dxm1, ss0, sU,
+                                 kb0, kU);
+    }
+    __syncthreads();
+    if (!carrier) {
+      const double pP = sh[slot][0][wave + 1][lane];
+      const double hZ1 = sh[slot][1][wave + 1][lane];
+      const double hR1 = sh[slot][2][wave + 1][lane];
+      if (vy)
+        (iys + ok)[c0] = sm_face(b0, bp, z0, L.zp, hR, hR1, hZ, hZ1, p0, pP, L.hv, dym1, ss0, ssp,
+                                 kb0, kbp);
+    }
+    return bp;
+  };
+  // Blumberg-Mellor at level k < kmax from b(y,k), b(y+1,k)
+  auto bm_level = [&](int k, double b0, double bn, const IpLevel& L) {
+    const long ok = NXY * (long)k;
+    const double be = from_upper_lane(b0);
+    const double z0 = L.zb0, ze = from_upper_lane(L.zb0), zn = L.zbp;
+    {
+      const double grdu = grdlx;
+      grdlx = (be - b0) * dxm1;
+      const double buoyu = buoylx;
+      buoylx = 0.5 * (be + b0);
+      const double dxz = (ze - z0) * dxm1;
+      const double hkk = L.hu;
+      prgrx = prgrx + 0.5 * (grdu + grdlx) * 0.5 * (hkk + hkx) - dxz * (buoyu - buoylx);
+      if (ux) (ixb + ok)[c0] = hkk * prgrx;
+      hkx = hkk;
+    }
+    {
+      const double grdu = grdly;
+      grdly = (bn - b0) * dym1;
+      const double buoyu = buoyly;
+      buoyly = 0.5 * (bn + b0);
+      const double dyz = (zn - z0) * dym1;
+      const double hkk = L.hv;
+      prgry = prgry + 0.5 * (grdu + grdly) * 0.5 * (hkk + hky) - dyz * (buoyu - buoyly);
+      if (vy) (iyb + ok)[c0] = hkk * prgry;
+      hky = hkk;
+    }
+  };
+  if constexpr (SM) {
+    IpLevel top;
+    load(kmax, top);
+    ps.top(kb0, buoy(top.t0, top.s0), (zzs + okx)[c0], top.z0, (hn + okx)[c0]);
+    ss0 = sseo[c0];
+    ssp = sseo[cp];
+    sU = from_upper_lane(ss0);
+    kU = from_upper_lane(kb0);
+    faces(kmax, top);
+  }
+
+  // ---- levels kmax-1 .. 1: two operand buffers, each loaded one level ahead
+  auto level = [&](int k, const IpLevel& L) {
+    double b0, bn;
+    if constexpr (SM) {
+      ps.step(k, buoy(L.t0, L.s0), L.z0);
+      b0 = ps.b1;
+      bn = faces(k, L);
+    } else {
+      b0 = buoy(L.t0, L.s0);
+      bn = buoy(L.tp, L.sp);
+    }
+    if constexpr (BM) {
+      if (!carrier) bm_level(k, b0, bn, L);
+    }
+  };
+  // a ring of NB operand buffers: level k's operands are loaded NB-1 levels
+  // ahead (levels below 1 re-read level 1, harmlessly)
+  constexpr int NB = kIpAhead + 1;
+  auto lv = [](int k) { return k >= 1 ? k : 1; };
+  IpLevel ring[NB];
+#pragma unroll
+  for (int q = 0; q < NB - 1; q++) load(lv(kmax - 1 - q), ring[q]);
+  for (int k0 = kmax - 1; k0 >= 1; k0 -= NB) {
+#pragma unroll
+    for (int q = 0; q < NB; q++) {
+      const int k = k0 - q;
+      if (k < 1) break;
+      load(lv(k - (NB - 1)), ring[(q + NB - 1) % NB]);
+      level(k, ring[q]);
+    }
+  }
+}
+
+constexpr int kIpRowsW = 3;  // output rows per workgroup of k_ip_rows (W+1 wavefronts)
+
+hipError_t launch_eos_ip_fused(hipStream_t st, Dims d, bool bm, bool sm, const int32_t* az,
+                               const int32_t* au, const int32_t* av, const double* T,
+                               const double* S, double T0, double S0, double dtr0, double dsr0,
+                               const double* H, const double* hn, const double* hun,
+                               const double* hvn, const double* sseo, double* zzb, double* ixb,
+                               double* iyb, double* zzs, double* ixs, double* iys, double dx,
+                               double dy) {
+  const dim3 blk(64, 4, 1);
+  const dim3 g1((d.NX + 63) / 64, (d.NY + 3) / 4, 1);
+  constexpr int W = kIpRowsW;
+  const int nsx = (d.NX - 4 + 59) / 60, nby = (d.NY - 4 + W - 1) / W;
+  const long nblk = (long)nsx * nby;
+  const dim3 g2((unsigned)(((nblk + 7) / 8) * 8));
+  const double dxm1 = 1.0 / dx, dym1 = 1.0 / dy;
+#define GT_IPF(B, M)                                                                          \
+  do {                                                                                        \
+    hipLaunchKernelGGL((k_ip_columns<B, M>), g1, blk, 0, st, d, az, H, hn, zzb, zzs, ixb, iyb, \
+                       ixs, iys);                                                             \
+    hipError_t e_ = hipGetLastError();                                                        \
+    if (e_ != hipSuccess) return e_;                                                          \
+    if (d.NX > 4 && d.NY > 4)                                                                 \
+      hipLaunchKernelGGL((k_ip_rows<B, M, W>), g2, dim3(64 * (W + (M ? 1 : 0))), 0, st, d,    \
+                         nsx, nby, au,                                                        \
+                         av,                                                                  \
+                         T, S, T0, S0, dtr0, dsr0, hn, hun, hvn, zzb, zzs, sseo, ixb, iyb,    \
+                         ixs, iys, dxm1, dym1);                                               \
+    return hipGetLastError();                                                                 \
+  } while (0)
+  if (bm && sm) GT_IPF(true, true);
+  if (bm) GT_IPF(true, false);
+  if (sm) GT_IPF(false, true);
+#undef GT_IPF
+  return hipSuccess;
+}
+
+}  // namespace gt

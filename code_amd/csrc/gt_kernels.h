@@ -104,6 +104,16 @@ hipError_t launch_ip_bm(hipStream_t st, Dims d, const int32_t* az, const int32_t
                         const int32_t* av, const double* H, const double* hn,
                         const double* hun, const double* hvn, const double* buoy, double* zz,
                         double* idpdx, double* idpdy, double dx, double dy);
+// the linear EOS fused into Blumberg-Mellor (bm) and/or Shchepetkin-McWilliams
+// (sm): two launches, gradients bit for bit those of launch_eos_linear +
+// launch_ip_bm / launch_ip_smcw (gt_ip_fused.hip)
+hipError_t launch_eos_ip_fused(hipStream_t st, Dims d, bool bm, bool sm, const int32_t* az,
+                               const int32_t* au, const int32_t* av, const double* T,
+                               const double* S, double T0, double S0, double dtr0, double dsr0,
+                               const double* H, const double* hn, const double* hun,
+                               const double* hvn, const double* sseo, double* zzb, double* ixb,
+                               double* iyb, double* zzs, double* ixs, double* iys, double dx,
+                               double dy);
 hipError_t launch_ip_smcw(hipStream_t st, Dims d, const int32_t* az, const int32_t* au,
                           const int32_t* av, const double* H, const double* hn,
                           const double* hun, const double* hvn, const double* buoy,

@@ -5,6 +5,7 @@
 // reference updates fi in place while it reads neighbours of f; when f and
 // fi alias, the host passes a second buffer as fi_dst and swaps afterwards,
 // and every cell the reference does not touch is copied through.
+#include "gt_ip.h"
 #include "gt_kernels.h"
 
 namespace gt {
@@ -420,11 +421,8 @@ __global__ void k_eos_linear(long n, long off, const double* __restrict__ T,
   long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (q >= n) return;
   long c = off + q;
-  const double g = (double)9.81f, rho_0 = (double)1025.f;
-  const double x = -g / rho_0;
-  double r = rho_0 + dtr0 * (T[c] - T0) + dsr0 * (S[c] - S0);
-  if (rho) rho[c] = r;
-  buoy[c] = x * (r - rho_0);
+  if (rho) rho[c] = (double)1025.f + dtr0 * (T[c] - T0) + dsr0 * (S[c] - S0);
+  buoy[c] = eos_linear_buoy(T[c], S[c], T0, S0, dtr0, dsr0);
 }
 
 // levels whose loads a column march issues together: the per-column
@@ -546,20 +544,13 @@ __global__ __launch_bounds__(256) void k_bm_idpd(Dims d, const int32_t* __restri
 
 // ---------------------------------------------------------------------------
 // a10: ip_shchepetkin_mcwilliams, src/3d/ip_shchepetkin_mcwilliams.F90:71-288
-// The reference's eps, OneFifth and 1./12. are single-precision literals.
-#define SM_EPS ((double)1.e-10f)
-#define SM_ONEFIFTH ((double)0.2f)
-#define SM_C12 ((double)(1.0f / 12.0f))
+// (constants and harmonic means in gt_ip.h)
 
 // One thread per column: the layer centres zz over the full halo range where
 // az>=1 (:90-99), then, on [imin..imax+1]x[jmin..jmax+1], the vertical
 // harmonic means and the pressure integral P (:108-146), reading back the
 // column's zz just written (one launch instead of two, zz and hn stay in
 // cache between the two loops).
-__device__ __forceinline__ double sm_hmR(double a, double b) {
-  double cff = 2.0 * a * b;
-  return cff > SM_EPS ? cff / (a + b) : 0.0;
-}
 __global__ __launch_bounds__(256) void k_sm_column(Dims d, const int32_t* __restrict__ az, const double* __restrict__ H,
                             const double* __restrict__ hn, const double* __restrict__ buoy,
                             double* __restrict__ zz, double* __restrict__ P) {
@@ -599,7 +590,7 @@ __global__ __launch_bounds__(256) void k_sm_column(Dims d, const int32_t* __rest
   // vertical differences dR(k) = b(k+1)-b(k), dZ likewise, for k in
   // 1..kmax-1, extended by dR(kmax) = dR(kmax-1) and dR(0) = dR(1) (:108-121);
   // the window b(k+1), b(k), zz(k+1), zz(k) moves down in registers
-  auto hmZ = [&](double a, double b) -> double { return 2.0 * a * b / (a + b); };
+  auto hmZ = [&](double a, double b) -> double { return sm_hmZ(a, b); };
   double b1 = AT(buoy, kmax), b0 = AT(buoy, kmax - 1);
   double z1 = zt, z0 = zm;
   const double ht = AT(hn, kmax);
@@ -651,15 +642,6 @@ __global__ __launch_bounds__(256) void k_sm_column(Dims d, const int32_t* __rest
 #undef AT
 }
 
-// horizontal harmonic mean with the eps test (:171-188)
-__device__ __forceinline__ double sm_hmx(double a, double b) {
-  double cff = 2.0 * a * b;
-  if (cff > SM_EPS) {
-    double cff1 = 1.0 / (a + b);
-    return cff * cff1;
-  }
-  return 0.0;
-}
 
 // idpdx (:153-211) and idpdy (:214-287).  One wavefront per (strip of 60
 // columns x 16 rows, level): lane l owns column x = 60*sx - 1 + l, the

@@ -310,6 +310,23 @@ int gt_ip_shchepetkin_mcwilliams(gt_ctx *ctx, const double *H,
                                  const double *hvn, const double *buoy,
                                  const double *sseo, double *zz,
                                  double *idpdx, double *idpdy);
+/* do_eqstate with the linear EOS (eqstate.F90:127-149,252-261) followed by
+ * ip_blumberg_mellor (methods & 1) and/or ip_shchepetkin_mcwilliams
+ * (methods & 2), fused: the buoyancy is formed from T and S where the
+ * pressure gradients read it and is not stored (a caller that needs buoy
+ * calls gt_eos_linear), the two methods' heights come from one read of hn,
+ * and every gradient is bit for bit that of gt_eos_linear followed by
+ * gt_ip_blumberg_mellor / gt_ip_shchepetkin_mcwilliams on the same zz
+ * arrays.  The outputs of a method not asked for may be NULL; sseo is read
+ * by Shchepetkin-McWilliams only.  No counterpart in the reference as one
+ * routine (it calls do_eqstate, then do_internal_pressure, m3d.F90:532). */
+int gt_eos_internal_pressure(gt_ctx *ctx, int methods, const double *T,
+                             const double *S, double T0, double S0,
+                             double dtr0, double dsr0, const double *H,
+                             const double *hn, const double *hun,
+                             const double *hvn, const double *sseo,
+                             double *zz_bm, double *idpdx_bm, double *idpdy_bm,
+                             double *zz_sm, double *idpdx_sm, double *idpdy_sm);
 /* do_internal_pressure, src/3d/internal_pressure.F90:181-274, with the
  * context's own zz (zero at creation, as init_internal_pressure leaves it).
  * sseo is only read by method 6. */

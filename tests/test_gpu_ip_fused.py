@@ -1,0 +1,136 @@
+"""The linear EOS fused into the internal pressure gradients
+(gt_eos_internal_pressure, gt_ip_fused.hip) against gt_eos_linear followed by
+gt_ip_blumberg_mellor / gt_ip_shchepetkin_mcwilliams and against the oracle
+(eqstate.F90:127-149,252-261, ip_blumberg_mellor.F90:71-152,
+ip_shchepetkin_mcwilliams.F90:71-288): bit for bit, every method alone and
+both together, on open water, on islands with sentinel-filled outputs (the
+cells a method does not write keep what they held) and at full size."""
+import numpy as np
+import pytest
+import torch
+
+from code_amd import inputs
+from code_amd import transport as gt
+from oracle import pyoracle as po
+from tests.util import to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+CONSTS = (inputs.EOS_T0, inputs.EOS_S0, inputs.EOS_DTR0, inputs.EOS_DSR0)
+NAMES = ("zb", "bx", "by", "zs", "sx", "sy")
+
+
+def with_islands(dom, seed):
+    """the domain with land cells scattered through its interior (masks
+    derived from az as the reference's do)"""
+    rng = np.random.default_rng(seed)
+    az = dom.az.copy()
+    land = rng.random(az.shape) < 0.12
+    az[land] = 0
+    au, av, ax = inputs.masks_from_az(az)
+    return inputs.Domain(dom.imax, dom.jmax, dom.kmax, dom.dx, dom.dy, dom.H, dom.HU, dom.HV,
+                         az, au, av, ax)
+
+
+def run_case(device, tr, dom, methods, sentinel, sseo):
+    """fused GPU, per-method GPU and oracle outputs for one case"""
+    T, S = tr.tracers["T"], tr.tracers["S"]
+    bm, sm = bool(methods & 1), bool(methods & 2)
+    start = {nm: (np.full(dom.shape3, sentinel) if sentinel is not None else np.zeros(dom.shape3))
+             for nm in NAMES}
+    # the oracle: eos_linear, then the methods asked for
+    ref = {nm: a.copy() for nm, a in start.items()}
+    orc = po.Oracle.from_domain(dom)
+    rho, buoy = np.zeros(dom.shape3), np.zeros(dom.shape3)
+    orc.eos_linear(T, S, rho, buoy, *CONSTS)
+    if bm:
+        orc.ip_blumberg_mellor(dom.H, tr.hn, tr.hun, tr.hvn, buoy, ref["zb"], ref["bx"],
+                               ref["by"])
+    if sm:
+        orc.ip_shchepetkin_mcwilliams(dom.H, tr.hn, tr.hun, tr.hvn, buoy, sseo, ref["zs"],
+                                      ref["sx"], ref["sy"])
+    ctx = gt.Context.from_domain(dom)
+    dv = {nm: to_dev(a, device) for nm, a in (("T", T), ("S", S), ("H", dom.H), ("hn", tr.hn),
+                                              ("hun", tr.hun), ("hvn", tr.hvn), ("sseo", sseo))}
+    # per-method kernels
+    per = {nm: to_dev(a, device) for nm, a in start.items()}
+    tb = torch.zeros(dom.shape3, dtype=torch.float64, device=device)
+    ctx.eos_linear(dv["T"], dv["S"], None, tb, *CONSTS)
+    if bm:
+        ctx.ip_blumberg_mellor(dv["H"], dv["hn"], dv["hun"], dv["hvn"], tb, per["zb"], per["bx"],
+                               per["by"])
+    if sm:
+        ctx.ip_shchepetkin_mcwilliams(dv["H"], dv["hn"], dv["hun"], dv["hvn"], tb, dv["sseo"],
+                                      per["zs"], per["sx"], per["sy"])
+    # fused
+    fus = {nm: to_dev(a, device) for nm, a in start.items()}
+    ctx.eos_internal_pressure(methods, dv["T"], dv["S"], *CONSTS, dv["H"], dv["hn"], dv["hun"],
+                              dv["hvn"], dv["sseo"] if sm else None,
+                              *(fus[nm] if on else None for nm, on in zip(
+                                  NAMES, (bm, bm, bm, sm, sm, sm))))
+    torch.cuda.synchronize()
+    ctx.close()
+    return ref, {k: to_host(v) for k, v in per.items()}, {k: to_host(v) for k, v in fus.items()}
+
+
+def check(ref, per, fus, methods):
+    used = (["zb", "bx", "by"] if methods & 1 else []) + (["zs", "sx", "sy"] if methods & 2
+                                                          else [])
+    for nm in NAMES:
+        assert np.array_equal(per[nm], ref[nm], equal_nan=True), (nm, "per-method vs oracle")
+        assert np.array_equal(fus[nm], ref[nm], equal_nan=True), (nm, "fused vs oracle")
+    for nm in used:
+        assert np.abs(ref[nm]).max() > 0, nm
+
+
+def sigma_case(imax, jmax, kmax, seed):
+    """T and S on sigma layers (any kmax >= 2)"""
+    tr = inputs.config_advection(imax, jmax, kmax, seed, coords="sigma")
+    tr.tracers["T"], tr.tracers["S"] = inputs.temperature_salinity(tr, seed)
+    return tr
+
+
+@pytest.mark.parametrize("methods", [1, 2, 3])
+@pytest.mark.parametrize("size", [(30, 26, 9), (130, 70, 7), (67, 41, 3), (65, 64, 2)])
+def test_fused_open_water(device, size, methods):
+    """strips of 60 columns and workgroups of 3 rows, whole and cut short;
+    kmax = 2 (the least the methods take) on sigma layers"""
+    if size[2] == 2:
+        tr = sigma_case(*size, seed=6)
+    else:
+        tr = inputs.ts_case(seed=5, imax=size[0], jmax=size[1], kmax=size[2])
+    sseo = inputs.uniform((tr.dom.NY, tr.dom.NX), 45, -0.2, 0.2)
+    check(*run_case(device, tr, tr.dom, methods, None, sseo), methods)
+
+
+@pytest.mark.parametrize("methods", [1, 2, 3])
+def test_fused_islands_sentinels(device, methods):
+    """land cells inside the domain, every output pre-filled with the same
+    sentinel on both sides: the fused kernels write exactly the cells the
+    per-method kernels write"""
+    tr = inputs.ts_case(seed=7, imax=90, jmax=61, kmax=11)
+    dom = with_islands(tr.dom, 8)
+    sseo = inputs.uniform((dom.NY, dom.NX), 46, -0.2, 0.2)
+    check(*run_case(device, tr, dom, methods, 3.25, sseo), methods)
+
+
+@pytest.mark.timeout(300)
+def test_fused_full_size(device):
+    """400 x 400 x 40 GVC, both methods, against the per-method kernels and
+    the oracle"""
+    po.set_threads(16)
+    tr = inputs.ts_case()
+    sseo = inputs.uniform((tr.dom.NY, tr.dom.NX), 47, -0.2, 0.2)
+    check(*run_case(device, tr, tr.dom, 3, None, sseo), 3)
+
+
+def test_fused_rejects(device):
+    tr = inputs.ts_case(seed=5, imax=20, jmax=18, kmax=4)
+    ctx = gt.Context.from_domain(tr.dom)
+    z = torch.zeros(tr.dom.shape3, dtype=torch.float64, device=device)
+    H = to_dev(tr.dom.H, device)
+    with pytest.raises(gt.GtError):
+        ctx.eos_internal_pressure(0, z, z, *CONSTS, H, z, z, z)
+    with pytest.raises(ValueError):  # SMcW without sseo
+        ctx.eos_internal_pressure(2, z, z, *CONSTS, H, z, z, z, None, None, None, None, z, z, z)
+    ctx.close()
