@@ -253,6 +253,17 @@ extern "C" int gt_create(gt_ctx** out, int imin, int imax, int jmin, int jmax, i
   return GT_OK;
 }
 
+extern "C" int gt_get_masks(gt_ctx* c, int32_t* az, int32_t* au, int32_t* av, int32_t* ax) {
+  if (!c) return set_err(GT_EARG, "gt_get_masks: NULL context");
+  int32_t* dst[4] = {az, au, av, ax};
+  const size_t bytes = sizeof(int32_t) * (size_t)c->d.NXY;
+  for (int q = 0; q < 4; q++)
+    if (dst[q])
+      HIPCHK(hipMemcpyAsync(dst[q], c->dint + q * c->d.NXY, bytes, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return GT_OK;
+}
+
 extern "C" int gt_destroy(gt_ctx* c) {
   if (!c) return GT_OK;
   void* ps[] = {c->dmask, c->dint, c->fi, c->hi, c->adv, c->tmp, c->scr7, c->zz,

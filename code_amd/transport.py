@@ -50,7 +50,7 @@ EXPORTS = [
     "gt_uv_advect_3d", "gt_restart_write_host", "gt_restart_read_host", "gt_write_restart",
     "gt_read_restart", "gt_set_substr_ini_press", "gt_reserve", "gt_budget_batch_f32",
     "gt_budget_batch_f32_noalloc", "gt_adv_split_u_m", "gt_adv_split_v_m", "gt_adv_split_w_m",
-    "gt_adv_upstream_2dh_m", "gt_eos_internal_pressure",
+    "gt_adv_upstream_2dh_m", "gt_eos_internal_pressure", "gt_get_masks",
 ]
 
 
@@ -106,6 +106,7 @@ def load_library():
         "gt_version": (i, []),
         "gt_create": (i, [C.POINTER(vp), i, i, i, i, i, d, d, vp, vp, vp, vp]),
         "gt_destroy": (i, [vp]),
+        "gt_get_masks": (i, [vp, vp, vp, vp, vp]),
         "gt_set_stream": (i, [vp, vp]),
         "gt_synchronize": (i, [vp]),
         "gt_set_adv_ver_iterations": (i, [vp, i]),
@@ -292,6 +293,12 @@ class Context:
         self.h = h
         if stream is not None:
             self.set_stream(stream)
+
+    def masks(self):
+        """(az, au, av, ax) as the context holds them (gt_get_masks)"""
+        out = [np.zeros((self.NY, self.NX), np.int32) for _ in range(4)]
+        check(self.lib.gt_get_masks(self.h, *(m.ctypes.data_as(C.c_void_p) for m in out)))
+        return tuple(out)
 
     @classmethod
     def from_domain(cls, dom, **kw):

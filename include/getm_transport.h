@@ -70,6 +70,12 @@ int gt_create(gt_ctx **ctx, int imin, int imax, int jmin, int jmax, int kmax,
               double dx, double dy, const int32_t *az, const int32_t *au,
               const int32_t *av, const int32_t *ax);
 int gt_destroy(gt_ctx *ctx);
+/* The context's masks, copied to host arrays of E2DFIELD size (any may be
+ * NULL): az as given, au/av/ax as given or as derived from az at creation
+ * (domain.F90:294-346, the module variables au, av, ax of domain.F90 that
+ * the host model's other routines read).  Synchronous. */
+int gt_get_masks(gt_ctx *ctx, int32_t *az, int32_t *au, int32_t *av,
+                 int32_t *ax);
 int gt_set_stream(gt_ctx *ctx, void *hip_stream);
 int gt_synchronize(gt_ctx *ctx);
 /* module variable adv_ver_iterations, advection_3d.F90:41 (default 1) */
@@ -319,7 +325,8 @@ int gt_ip_shchepetkin_mcwilliams(gt_ctx *ctx, const double *H,
  * gt_ip_blumberg_mellor / gt_ip_shchepetkin_mcwilliams on the same zz
  * arrays.  The outputs of a method not asked for may be NULL; sseo is read
  * by Shchepetkin-McWilliams only.  No counterpart in the reference as one
- * routine (it calls do_eqstate, then do_internal_pressure, m3d.F90:532). */
+ * routine (do_eqstate at m3d.F90:607 feeds the next step's
+ * do_internal_pressure at m3d.F90:532). */
 int gt_eos_internal_pressure(gt_ctx *ctx, int methods, const double *T,
                              const double *S, double T0, double S0,
                              double dtr0, double dsr0, const double *H,

@@ -511,31 +511,33 @@ def test_reserve_sizes_the_work_fields(device):
         assert np.array_equal(to_host(tf), f), split
 
 
-@pytest.mark.timeout(600)
-def test_config2_full_size_20_steps_against_oracle(device):
-    """The T/S workload at full size (400x400x40 GVC): 20 steps of linear EOS,
-    Blumberg-Mellor, Shchepetkin-McWilliams and P2-PDM HALFSPLIT advection of
-    T and S, on the GPU and in the oracle: T, S and the four pressure
-    gradients of the last step bit for bit (bound 1e-12 per cell)"""
+@pytest.mark.timeout(900)
+def test_config2_full_size_200_steps_against_oracle(device):
+    """The T/S workload at full size (400x400x40 GVC) for the config's 200
+    steps, on the path the benchmark times: the fused EOS + Blumberg-Mellor +
+    Shchepetkin-McWilliams launch pair and the batched FP64 P2-PDM HALFSPLIT
+    advection of T and S in one launch pair (default strip tuning), against
+    the oracle's eos_linear, ip_* and do_advection_3d per tracer: T, S and
+    the four pressure gradients of the last step bit for bit"""
     import torch
     po.set_threads(16)
     tr = inputs.ts_case()
     d = tr.dom
-    sseo = np.zeros((d.NY, d.NX))
+    sseo = inputs.uniform((d.NY, d.NX), 48, -0.1, 0.1)
     ctx = gt.Context.from_domain(d)
     orc = po.Oracle.from_domain(d)
     host = [tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho, tr.hn]
-    dv = [to_dev(a, device) for a in host]
+    uu, vv, ww, hun, hvn, ho, hn = (to_dev(a, device) for a in host)
     consts = (inputs.EOS_T0, inputs.EOS_S0, inputs.EOS_DTR0, inputs.EOS_DSR0)
     T, S = tr.tracers["T"].copy(), tr.tracers["S"].copy()
-    tT, tS = to_dev(T, device), to_dev(S, device)
-    z3 = lambda: np.zeros(d.shape3)
+    TS = to_dev(np.stack([T, S]), device)
+    TS2 = torch.empty_like(TS)
+    z3 = lambda: np.zeros(d.shape3)  # noqa: E731
     rho, buoy = z3(), z3()
     out = {k: z3() for k in ("zb", "bx", "by", "zs", "sx", "sy")}
-    tb = torch.zeros(d.shape3, dtype=torch.float64, device=device)
-    tout = {k: torch.zeros_like(tb) for k in out}
+    tout = {k: torch.zeros(d.shape3, dtype=torch.float64, device=device) for k in out}
     dH, dsse = to_dev(d.H, device), to_dev(sseo, device)
-    for step in range(20):
+    for step in range(200):
         orc.eos_linear(T, S, rho, buoy, *consts)
         orc.ip_blumberg_mellor(d.H, tr.hn, tr.hun, tr.hvn, buoy, out["zb"], out["bx"], out["by"])
         orc.ip_shchepetkin_mcwilliams(d.H, tr.hn, tr.hun, tr.hvn, buoy, sseo, out["zs"],
@@ -543,14 +545,17 @@ def test_config2_full_size_20_steps_against_oracle(device):
         for f in (T, S):
             orc.do_advection_3d(tr.dt, f, *host, gt.HALFSPLIT, gt.P2_PDM, gt.P2_PDM, -1.0,
                                 gt.H_TAG)
-        ctx.eos_linear(tT, tS, None, tb, *consts)
-        ctx.ip_blumberg_mellor(dH, dv[6], dv[3], dv[4], tb, tout["zb"], tout["bx"], tout["by"])
-        ctx.ip_shchepetkin_mcwilliams(dH, dv[6], dv[3], dv[4], tb, dsse, tout["zs"], tout["sx"],
-                                      tout["sy"])
-        for f in (tT, tS):
-            ctx.do_advection_3d(tr.dt, f, *dv, gt.HALFSPLIT, gt.P2_PDM, gt.P2_PDM, -1.0,
-                                gt.H_TAG)
-    for nm, got, ref in [("T", tT, T), ("S", tS, S)] + [(k, tout[k], out[k]) for k in out]:
-        h = to_host(got)
-        assert rel_err(h, ref) <= TOL, (nm, rel_err(h, ref))
-        assert np.array_equal(h, ref), (nm, "not bit-exact")
+        ctx.eos_internal_pressure(3, TS[0], TS[1], *consts, dH, hn, hun, hvn, dsse,
+                                  *(tout[k] for k in ("zb", "bx", "by", "zs", "sx", "sy")))
+        gt.adv3d_batch([dict(ctx=ctx, f_in=TS, f_out=TS2, uu=uu, vv=vv, ww=ww, hu=hun, hv=hvn,
+                             ho=ho)], 2, tr.dt, gt.P2_PDM, gt.P2_PDM)
+        TS, TS2 = TS2, TS
+        if step % 50 == 49:
+            print(f"step {step + 1}", flush=True)
+    torch.cuda.synchronize()
+    got = {"T": TS[0], "S": TS[1], **tout}
+    want = {"T": T, "S": S, **out}
+    for nm in want:
+        h = to_host(got[nm])
+        assert rel_err(h, want[nm]) <= TOL, (nm, rel_err(h, want[nm]))
+        assert np.array_equal(h, want[nm]), (nm, "not bit-exact")
