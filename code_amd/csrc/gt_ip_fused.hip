@@ -145,31 +145,35 @@ __device__ __forceinline__ double sm_face(double b0, double b1, double z0, doubl
 // levels by which k_ip_rows loads ahead of its arithmetic
 constexpr int kIpAhead = 1;
 
-// the operands of one level k of the march, loaded one level ahead of their
-// use (raw T and S: the buoyancy is formed where it is used, so the loads
-// stay in flight across the previous level's arithmetic).  Row y is the
-// wavefront's own row.
+// the operands of one level k of the march that come from memory, loaded
+// kIpAhead levels ahead of their use (raw T and S: the buoyancy is formed
+// where it is used, so the loads stay in flight across the arithmetic of the
+// levels before).  Row y is the wavefront's own row; the rows around it come
+// from the other wavefronts of the workgroup through LDS.
 struct IpLevel {
-  double t0, s0, z0;  // T S zz(centres) at (y, k-1): the P step (SM); T S at (y, k) (BM only)
-  double tp, sp, zp;  // T S zz(centres) at (y+1, k)
-  double tm, sm, zm;  // T S zz(centres) at (y-1, k) (SM)
-  double zb0, zbp;    // zz(interfaces) at (y, k), (y+1, k) (BM)
+  double t0, s0, z0;  // T S zz(centres) at (y, k-1) (SM: the P step); T S at (y, k) (BM only)
+  double te, se, ze;  // T S zz(centres) at (y-1, k) (first wavefront) or (y+1, k) (carrier);
+                      // BM only: T S zz(interfaces) at (y+1, k) (last wavefront)
+  double zb0;         // zz(interfaces) at (y, k) (BM)
   double hu, hv;      // hun hvn at (y, k)
 };
 
 // One workgroup of W+1 wavefronts (W without Shchepetkin-McWilliams) per
 // strip of 60 interior columns x W rows, marching down the levels;
-// wavefront w takes row y = j0 + w and lane l
-// column x = 60*sx + 1 + l (lanes 1..60 of wavefronts 0..W-1 write; the
-// last wavefront only carries row j0+W for the one below it).  The
-// i-neighbours x-1, x+1, x+2 are adjacent lanes (DPP).  Each wavefront
-// carries the P recurrence of its row and forms the two harmonic means of
-// its row across y (rows y-1|y|y+1); the y face of row y takes P and those
-// means of row y+1 from the wavefront above through LDS (one barrier per
-// level, two LDS slots by level parity).  A harmonic mean across a face is
-// formed once and shared by the two faces that use it, from the same
-// differences, so every gradient rounds as the per-face formula.  Rows past
-// NY-1 and columns past NX-1 are clamped (read, never written).
+// wavefront w takes row y = j0 + w and lane l column x = 60*sx + 1 + l
+// (lanes 1..60 of wavefronts 0..W-1 write; the last wavefront only carries
+// row j0+W for the one below it).  The i-neighbours x-1, x+1, x+2 are
+// adjacent lanes (DPP); the j-neighbours are the other wavefronts' rows,
+// exchanged through LDS at each level, so each wavefront loads its own row
+// only (the first also row j0-1, the last row j0+W+1, for the harmonic means
+// across y at the workgroup's edges).  Per level: each wavefront takes the P
+// step of its row and publishes b, z (and Blumberg-Mellor's interface
+// height); after a barrier it forms the two harmonic means across y of its
+// row and publishes them with P; after a second barrier it forms its x face
+// (means shared with the next lane), its y face (means of row y+1 from the
+// wavefront above) and Blumberg-Mellor's recurrences.  Every mean is formed
+// once and rounds as the per-face formula.  Rows past NY-1 and columns past
+// NX-1 are clamped (read, never written).
 template <bool BM, bool SM, int W>
 __global__ __launch_bounds__(64 * (W + 1)) void k_ip_rows(
     Dims d, int nsx, int nby, const int32_t* __restrict__ au, const int32_t* __restrict__ av,
@@ -183,6 +187,7 @@ __global__ __launch_bounds__(64 * (W + 1)) void k_ip_rows(
   // work of (b%8)*per + b/8, so each XCD marches a contiguous run of strips
   // (rows fastest) whose shared rows meet in its L2; the test is uniform
   // over the workgroup, so every wavefront that starts reaches each barrier
+  constexpr int NW = W + (SM ? 1 : 0);
   const long nblk = (long)nsx * nby;
   const long per = (nblk + 7) / 8;
   const long blk = (long)(blockIdx.x % 8) * per + blockIdx.x / 8;
@@ -191,6 +196,7 @@ __global__ __launch_bounds__(64 * (W + 1)) void k_ip_rows(
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   const bool carrier = SM && wave == W;  // row j0+W: P and means only
+  const bool first = wave == 0, last = wave == NW - 1;
   const int NX = d.NX, NY = d.NY, kmax = d.kmax;
   const long NXY = d.NXY;
   const int x = 60 * sx + 1 + lane;  // >= 1
@@ -198,7 +204,10 @@ __global__ __launch_bounds__(64 * (W + 1)) void k_ip_rows(
   const int y = 2 + W * by + wave;
   // 32-bit cell offsets within a level (a level holds < 2^31 cells)
   auto cell = [&](int yy) -> unsigned { return (unsigned)(xc + NX * (yy < NY ? yy : NY - 1)); };
-  const unsigned cm = cell(y - 1), c0 = cell(y), cp = cell(y + 1);
+  const unsigned c0 = cell(y), cp = cell(y + 1);
+  // the extra row of the edge wavefronts (SM): y-1 for the first, y+1 for
+  // the carrier (a workgroup of one row has both: the first is not the carrier)
+  const unsigned ce = carrier ? cp : cell(y - 1);
   const bool out = !carrier && lane >= 1 && lane <= 60 && x <= NX - 3 && y <= NY - 3;
   const bool ux = out && au[c0] >= 1, vy = out && av[c0] >= 1;
   auto buoy = [&](double t, double s) { return eos_linear_buoy(t, s, T0, S0, dtr0, dsr0); };
@@ -207,130 +216,166 @@ __global__ __launch_bounds__(64 * (W + 1)) void k_ip_rows(
     const long o0 = SM ? okm : ok;
     L.t0 = (T + o0)[c0];
     L.s0 = (S + o0)[c0];
-    L.tp = (T + ok)[cp];
-    L.sp = (S + ok)[cp];
     if constexpr (SM) {
       L.z0 = (zzs + o0)[c0];
-      L.zp = (zzs + ok)[cp];
-      L.tm = (T + ok)[cm];
-      L.sm = (S + ok)[cm];
-      L.zm = (zzs + ok)[cm];
+      if (first || carrier) {
+        L.te = (T + ok)[ce];
+        L.se = (S + ok)[ce];
+        L.ze = (zzs + ok)[ce];
+      }
+    } else if (last) {
+      // Blumberg-Mellor alone: the last wavefront reads row y+1 itself
+      L.te = (T + ok)[cp];
+      L.se = (S + ok)[cp];
+      L.ze = (zzb + ok)[cp];
     }
-    if constexpr (BM) {
-      L.zb0 = (zzb + ok)[c0];
-      L.zbp = (zzb + ok)[cp];
+    if constexpr (BM) L.zb0 = (zzb + ok)[c0];
+    if (!carrier) {
+      L.hu = (hun + ok)[c0];
+      L.hv = (hvn + ok)[c0];
     }
-    L.hu = (hun + ok)[c0];
-    L.hv = (hvn + ok)[c0];
   };
+  // LDS: per wavefront and lane, b, z and zz(interfaces) of its row (slot A),
+  // P and the two means across y (slot B); two of each by level parity
+  __shared__ double shA[2][3][NW][64];
+  __shared__ double shB[SM ? 2 : 1][3][NW][64];
 
   // ---- level kmax
   const long okx = NXY * (long)kmax;
   const double kb0 = buoy((T + okx)[c0], (S + okx)[c0]);
-  const double kbp = buoy((T + okx)[cp], (S + okx)[cp]);
+  // Shchepetkin-McWilliams: P of the row, the face constants
+  PCol ps{};
+  double ss0 = 0, ssp = 0, sU = 0, kU = 0, kbp = 0;
   // Blumberg-Mellor (ip_blumberg_mellor.F90:99-149), as k_bm_idpd
   double grdlx = 0, buoylx = 0, hkx = 0, prgrx = 0, grdly = 0, buoyly = 0, hky = 0, prgry = 0;
-  if constexpr (BM) {
-    const double be = from_upper_lane(kb0);
-    grdlx = (be - kb0) * dxm1;
-    buoylx = 0.5 * (be + kb0);
-    hkx = (hun + okx)[c0];
-    prgrx = grdlx * 0.5 * hkx;
-    grdly = (kbp - kb0) * dym1;
-    buoyly = 0.5 * (kbp + kb0);
-    hky = (hvn + okx)[c0];
-    prgry = grdly * 0.5 * hky;
-    if (ux) (ixb + okx)[c0] = hkx * prgrx;
-    if (vy) (iyb + okx)[c0] = hky * prgry;
-  }
-  // Shchepetkin-McWilliams: P of the row, the face constants
-  __shared__ double sh[SM ? 2 : 1][3][W + 1][64];
-  PCol ps{};
-  double ss0 = 0, ssp = 0, sU = 0, kU = 0;
-  // the faces of level k; ps holds b(k), z(k) in b1/z1 and P(k) in p
-  auto faces = [&](int k, const IpLevel& L) {
-    const long ok = NXY * (long)k;
-    const int slot = k & 1;
-    const double b0 = ps.b1, z0 = ps.z1, p0 = ps.p;
-    const double bm = buoy(L.tm, L.sm), bp = buoy(L.tp, L.sp);
-    // y: the means of rows y-1|y|y+1, shared with the wavefront below
-    const double hZ = sm_hmx(z0 - L.zm, L.zp - z0), hR = sm_hmx(b0 - bm, bp - b0);
-    sh[slot][0][wave][lane] = p0;
-    sh[slot][1][wave][lane] = hZ;
-    sh[slot][2][wave][lane] = hR;
-    if (!carrier) {
-      // x: the means of columns x-1|x|x+1 on this lane, x|x+1|x+2 on the next
-      const double bL = from_lower_lane(b0), zL = from_lower_lane(z0);
-      const double bU = from_upper_lane(b0), zU = from_upper_lane(z0), pU = from_upper_lane(p0);
-      const double dZ = sm_hmx(z0 - zL, zU - z0), dR = sm_hmx(b0 - bL, bU - b0);
-      const double dZ1 = from_upper_lane(dZ), dR1 = from_upper_lane(dR);
-      if (ux)
-        (ixs + ok)[c0] = sm_face(b0, bU, z0, zU, dR, dR1, dZ, dZ1, p0, pU, L.hu, dxm1, ss0, sU,
-                                 kb0, kU);
-    }
-    __syncthreads();
-    if (!carrier) {
-      const double pP = sh[slot][0][wave + 1][lane];
-      const double hZ1 = sh[slot][1][wave + 1][lane];
-      const double hR1 = sh[slot][2][wave + 1][lane];
-      if (vy)
-        (iys + ok)[c0] = sm_face(b0, bp, z0, L.zp, hR, hR1, hZ, hZ1, p0, pP, L.hv, dym1, ss0, ssp,
-                                 kb0, kbp);
-    }
-    return bp;
-  };
-  // Blumberg-Mellor at level k < kmax from b(y,k), b(y+1,k)
-  auto bm_level = [&](int k, double b0, double bn, const IpLevel& L) {
-    const long ok = NXY * (long)k;
-    const double be = from_upper_lane(b0);
-    const double z0 = L.zb0, ze = from_upper_lane(L.zb0), zn = L.zbp;
-    {
-      const double grdu = grdlx;
-      grdlx = (be - b0) * dxm1;
-      const double buoyu = buoylx;
-      buoylx = 0.5 * (be + b0);
-      const double dxz = (ze - z0) * dxm1;
-      const double hkk = L.hu;
-      prgrx = prgrx + 0.5 * (grdu + grdlx) * 0.5 * (hkk + hkx) - dxz * (buoyu - buoylx);
-      if (ux) (ixb + ok)[c0] = hkk * prgrx;
-      hkx = hkk;
-    }
-    {
-      const double grdu = grdly;
-      grdly = (bn - b0) * dym1;
-      const double buoyu = buoyly;
-      buoyly = 0.5 * (bn + b0);
-      const double dyz = (zn - z0) * dym1;
-      const double hkk = L.hv;
-      prgry = prgry + 0.5 * (grdu + grdly) * 0.5 * (hkk + hky) - dyz * (buoyu - buoyly);
-      if (vy) (iyb + ok)[c0] = hkk * prgry;
-      hky = hkk;
-    }
-  };
-  if constexpr (SM) {
-    IpLevel top;
-    load(kmax, top);
-    ps.top(kb0, buoy(top.t0, top.s0), (zzs + okx)[c0], top.z0, (hn + okx)[c0]);
-    ss0 = sseo[c0];
-    ssp = sseo[cp];
-    sU = from_upper_lane(ss0);
-    kU = from_upper_lane(kb0);
-    faces(kmax, top);
-  }
 
-  // ---- levels kmax-1 .. 1: two operand buffers, each loaded one level ahead
-  auto level = [&](int k, const IpLevel& L) {
-    double b0, bn;
+  // the exchange and the faces of level k, from the row's b(k), z(k) (SM:
+  // the P state, P(k) in ps.p), zz(interfaces)(k) and hun, hvn (L)
+  auto level_faces = [&](int k, int slot, double bk, double zk, const IpLevel& L, bool top) {
+    const long ok = NXY * (long)k;
+    shA[slot][0][wave][lane] = bk;
+    if constexpr (SM) shA[slot][1][wave][lane] = zk;
+    if constexpr (BM) shA[slot][2][wave][lane] = L.zb0;
+    __syncthreads();
+    double bn = 0, zn = 0, zbn = 0;  // row y+1 at k
+    if (!SM && last) {
+      bn = buoy(L.te, L.se);
+      zbn = L.ze;
+    } else if (!carrier) {
+      bn = shA[slot][0][wave + 1][lane];
+      if constexpr (SM) zn = shA[slot][1][wave + 1][lane];
+      if constexpr (BM) zbn = shA[slot][2][wave + 1][lane];
+    }
     if constexpr (SM) {
-      ps.step(k, buoy(L.t0, L.s0), L.z0);
-      b0 = ps.b1;
-      bn = faces(k, L);
-    } else {
-      b0 = buoy(L.t0, L.s0);
-      bn = buoy(L.tp, L.sp);
+      // the means of rows y-1|y|y+1, shared with the wavefront below
+      double bm, zm, bp, zp;
+      if (first) {
+        bm = buoy(L.te, L.se);
+        zm = L.ze;
+      } else {
+        bm = shA[slot][0][wave - 1][lane];
+        zm = shA[slot][1][wave - 1][lane];
+      }
+      if (carrier) {
+        bp = buoy(L.te, L.se);
+        zp = L.ze;
+      } else {
+        bp = bn;
+        zp = zn;
+      }
+      const double hZ = sm_hmx(zk - zm, zp - zk), hR = sm_hmx(bk - bm, bp - bk);
+      shB[slot][0][wave][lane] = ps.p;
+      shB[slot][1][wave][lane] = hZ;
+      shB[slot][2][wave][lane] = hR;
+      if (!carrier) {
+        // x: the means of columns x-1|x|x+1 on this lane, x|x+1|x+2 on the next
+        const double p0 = ps.p;
+        const double bL = from_lower_lane(bk), zL = from_lower_lane(zk);
+        const double bU = from_upper_lane(bk), zU = from_upper_lane(zk);
+        const double pU = from_upper_lane(p0);
+        const double dZ = sm_hmx(zk - zL, zU - zk), dR = sm_hmx(bk - bL, bU - bk);
+        const double dZ1 = from_upper_lane(dZ), dR1 = from_upper_lane(dR);
+        if (ux)
+          (ixs + ok)[c0] = sm_face(bk, bU, zk, zU, dR, dR1, dZ, dZ1, p0, pU, L.hu, dxm1, ss0, sU,
+                                   kb0, kU);
+      }
+      __syncthreads();
+      if (!carrier) {
+        const double pP = shB[slot][0][wave + 1][lane];
+        const double hZ1 = shB[slot][1][wave + 1][lane];
+        const double hR1 = shB[slot][2][wave + 1][lane];
+        if (vy)
+          (iys + ok)[c0] = sm_face(bk, bn, zk, zn, hR, hR1, hZ, hZ1, ps.p, pP, L.hv, dym1, ss0,
+                                   ssp, kb0, kbp);
+      }
     }
     if constexpr (BM) {
-      if (!carrier) bm_level(k, b0, bn, L);
+      if (!carrier) {
+        const double b0 = bk, be = from_upper_lane(bk);
+        const double z0 = L.zb0, ze = from_upper_lane(L.zb0);
+        if (top) {
+          // level kmax starts the recurrences
+          grdlx = (be - b0) * dxm1;
+          buoylx = 0.5 * (be + b0);
+          hkx = L.hu;
+          prgrx = grdlx * 0.5 * hkx;
+          grdly = (bn - b0) * dym1;
+          buoyly = 0.5 * (bn + b0);
+          hky = L.hv;
+          prgry = grdly * 0.5 * hky;
+          if (ux) (ixb + ok)[c0] = hkx * prgrx;
+          if (vy) (iyb + ok)[c0] = hky * prgry;
+        } else {
+          {
+            const double grdu = grdlx;
+            grdlx = (be - b0) * dxm1;
+            const double buoyu = buoylx;
+            buoylx = 0.5 * (be + b0);
+            const double dxz = (ze - z0) * dxm1;
+            const double hkk = L.hu;
+            prgrx = prgrx + 0.5 * (grdu + grdlx) * 0.5 * (hkk + hkx) - dxz * (buoyu - buoylx);
+            if (ux) (ixb + ok)[c0] = hkk * prgrx;
+            hkx = hkk;
+          }
+          {
+            const double grdu = grdly;
+            grdly = (bn - b0) * dym1;
+            const double buoyu = buoyly;
+            buoyly = 0.5 * (bn + b0);
+            const double dyz = (zbn - z0) * dym1;
+            const double hkk = L.hv;
+            prgry = prgry + 0.5 * (grdu + grdly) * 0.5 * (hkk + hky) - dyz * (buoyu - buoyly);
+            if (vy) (iyb + ok)[c0] = hkk * prgry;
+            hky = hkk;
+          }
+        }
+      }
+    }
+  };
+
+  {
+    IpLevel top;
+    load(kmax, top);
+    if constexpr (SM) {
+      ps.top(kb0, buoy(top.t0, top.s0), (zzs + okx)[c0], top.z0, (hn + okx)[c0]);
+      kbp = buoy((T + okx)[cp], (S + okx)[cp]);
+      ss0 = sseo[c0];
+      ssp = sseo[cp];
+      sU = from_upper_lane(ss0);
+      kU = from_upper_lane(kb0);
+      level_faces(kmax, kmax & 1, ps.b1, ps.z1, top, true);
+    } else {
+      level_faces(kmax, kmax & 1, kb0, 0.0, top, true);
+    }
+  }
+
+  // ---- levels kmax-1 .. 1
+  auto level = [&](int k, const IpLevel& L) {
+    if constexpr (SM) {
+      ps.step(k, buoy(L.t0, L.s0), L.z0);
+      level_faces(k, k & 1, ps.b1, ps.z1, L, false);
+    } else {
+      level_faces(k, k & 1, buoy(L.t0, L.s0), 0.0, L, false);
     }
   };
   // a ring of NB operand buffers: level k's operands are loaded NB-1 levels
