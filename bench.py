@@ -143,7 +143,7 @@ def parse():
     p.add_argument("--copy-mb", type=int, default=0,
                    help="also run a plain device copy of this many MiB (counter calibration)")
     p.add_argument("--group", type=int, default=-1,
-                   help="FP32 tracers per wavefront 1, 3 or 6 (-1: autotuned)")
+                   help="FP32 tracers per wavefront 1, 3, 6 or 9 (-1: autotuned)")
     return p.parse_args()
 
 
