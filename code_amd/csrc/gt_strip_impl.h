@@ -102,37 +102,45 @@ __device__ __forceinline__ void face_setup(FaceC<true>& fc, bool active, bool li
 }
 
 // adv_interfacial_reconstruction (src/2d/advection.F90:806-845), branch-free:
-// every operation of reconstruct() in its order, the result selected by
-// deltaf*deltafu > 0, and P2-PDM's 2/(1-cfl) taken from the face (E) instead
-// of formed per tracer; the selected value is identical
+// every operation of reconstruct() in its order, P2-PDM's 2/(1-cfl) taken
+// from the face (E) instead of formed per tracer.  Returns the limited value
+// and sets take where the reference returns it (deltaf*deltafu > 0 for the
+// TVD schemes, always for P2, never for upstream) -- fu otherwise -- so the
+// caller folds this choice into its own select of the face's value
 template <int SCH>
 __device__ __forceinline__ double reconstruct_bf(double cfl, double E, double fuu, double fu,
-                                                 double fd) {
+                                                 double fd, bool& take) {
   const double deltaf = fd - fu;
   const double deltafu = fu - fuu;
   if constexpr (SCH == UPSTREAM) {
     (void)E; (void)cfl;
+    take = false;
     return fu;
   } else if constexpr (SCH == P2) {
     (void)E;
+    take = true;
     return reconstruct_p2<double>(cfl, fu, deltafu, deltaf);
   } else {
+    // The reference's min/max are the hardware's: where the result is
+    // selected (deltaf*deltafu > 0) the ratio is positive and, with
+    // cfl >= 0, no operand is NaN and no two are zeros of opposite sign, so
+    // v_min_f64 / v_max_f64 pick the operand the comparison form picks
     const double ratio = deltafu / deltaf;
     double limiter;
     if constexpr (SCH == P2_PDM) {
       const double one6th = 1.0 / 6.0;
       const double x = one6th * (1.0 - 2.0 * cfl);
       limiter = (0.5 + x) + (0.5 - x) * ratio;
-      limiter = tmin(tmin(2.0 * ratio / (cfl + 1.e-10), limiter), E);
+      limiter = __builtin_fmin(__builtin_fmin(2.0 * ratio / (cfl + 1.e-10), limiter), E);
     } else if constexpr (SCH == SUPERBEE) {
       (void)E;
-      limiter = tmax(tmin(2.0 * ratio, 1.0), tmin(ratio, 2.0));
+      limiter = __builtin_fmax(__builtin_fmin(2.0 * ratio, 1.0), __builtin_fmin(ratio, 2.0));
     } else {  // MUSCL
       (void)E;
-      limiter = tmin(tmin(2.0 * ratio, 0.5 * (1.0 + ratio)), 2.0);
+      limiter = __builtin_fmin(__builtin_fmin(2.0 * ratio, 0.5 * (1.0 + ratio)), 2.0);
     }
-    const double r = fu + 0.5 * limiter * (1.0 - cfl) * deltaf;
-    return deltaf * deltafu > 0.0 ? r : fu;
+    take = deltaf * deltafu > 0.0;
+    return fu + 0.5 * limiter * (1.0 - cfl) * deltaf;
   }
 }
 
@@ -202,8 +210,9 @@ __device__ __forceinline__ TF face_flux(const FaceC<true>& fc, TF fm1, TF f0, TF
   const bool left = fc.code <= F_LEFT_LIM;
   const TF fuu = left ? fm1 : fp2, fd = left ? fp1 : f0;
   const TF fu0 = left ? f0 : fp1;
-  const TF rr = reconstruct_bf<SCH>(fc.cfl, fc.E, fuu, fu0, fd);
-  const TF fu = (fc.code == F_LEFT_LIM || fc.code == F_RIGHT_LIM) ? rr : fu0;
+  bool take;
+  const TF rr = reconstruct_bf<SCH>(fc.cfl, fc.E, fuu, fu0, fd, take);
+  const TF fu = ((fc.code == F_LEFT_LIM || fc.code == F_RIGHT_LIM) && take) ? rr : fu0;
   TF fl = fc.vel * fu;
   if constexpr (DIFF) fl = fl - AH * fc.Dv * (fp1 - f0) / dlen;
   return fc.code == F_NONE ? TF(0) : fl;
@@ -298,8 +307,9 @@ __device__ __forceinline__ TF wflux(const WFace<EXACT>& wf, int k, int kshift, i
     if (lim) { fuu = fk2; fd = fk; }
   }
   if constexpr (EXACT) {
-    const TF rr = reconstruct_bf<SCH>(wf.cfl * 1.0, wf.E, fuu, fu, fd);
-    fu = lim ? rr : fu;
+    bool take;
+    const TF rr = reconstruct_bf<SCH>(wf.cfl * 1.0, wf.E, fuu, fu, fd, take);
+    fu = (lim && take) ? rr : fu;
   } else {
     fu = rec_fast<SCH>(fuu, fu, fd, lim ? wf.g : 0.0f, wf.A, wf.al, wf.E);
   }

@@ -436,8 +436,9 @@ int gt_set_adjoint_mode(gt_ctx *ctx, int keep_states);
 int gt_reserve(gt_ctx *ctx, int ntracers_f32, int ntracers_f64);
 /* autotuner: times gt_adv3d_batch_f32 (f32=1) or _f64 over the nine strip
  * height pairs (tile 3..11) x tracers per wavefront {1, 3, 6, 9} (FP64:
- * {1, 2}), reps timed launches each, keeps the fastest on every context.
- * table (optional, 36 floats) receives the mean ms of each pair in sweep
+ * {1, 2}), reps timed launches each, in two passes over the candidates,
+ * and keeps the fastest on every context.  table (optional, 36 floats)
+ * receives each pair's ms (the lower of its two passes' means) in sweep
  * order (tile outer).  f_out is overwritten, f_in is not touched. */
 int gt_autotune_batch(int ndom, const gt_batch_domain *doms, int ntracers,
                       double dt, int hscheme, int vscheme, double AH, int tag,
