@@ -339,11 +339,12 @@ struct StripItem {
 
 // Blocks of 4 wavefronts, one work item per wavefront.  Consecutive blocks
 // are dealt round-robin to the 8 XCDs, so block b is given the work of
-// block (b%8)*per + b/8: each XCD takes a contiguous run of items.  Items are
-// numbered tracer group fastest, then level, then strip, so the groups of one
-// strip-level (which read the same layer heights, transports and masks) and
-// then the neighbouring levels (which read the same tracer levels in the w
-// sub-step) run side by side and meet in that XCD's L2.
+// block (b%8)*per + b/8: each XCD takes a contiguous run of items.  Items of
+// the w kernel are numbered tracer group fastest, then level, then strip, so
+// the groups of one strip-level (which read the same layer heights,
+// transports and masks) and then the neighbouring levels (which read the same
+// tracer levels in the w sub-step) run side by side and meet in that XCD's
+// L2.
 template <int P, typename TF>
 __device__ __forceinline__ bool strip_item(const FusedArgs<TF>& A, StripItem& it) {
   const long nblk = (A.nitems[P] + 3) / 4;
@@ -358,10 +359,20 @@ __device__ __forceinline__ bool strip_item(const FusedArgs<TF>& A, StripItem& it
   it.dom = di;
   it.grp = (int)(loc % A.ngroups);
   loc /= A.ngroups;
-  it.lev = 1 + (int)(loc % a.d.kmax);
-  loc /= a.d.kmax;
-  it.sx = (int)(loc % a.nsx);
-  it.sy = (int)(loc / a.nsx);
+  if constexpr (P == 0) {
+    // u/v kernel: no coupling across levels; the strips of one column of
+    // strips run side by side instead, so the four ghost rows a strip reads
+    // below and above are the rows its neighbours read at the same time
+    it.sy = (int)(loc % a.nsy[0]);
+    loc /= a.nsy[0];
+    it.sx = (int)(loc % a.nsx);
+    it.lev = 1 + (int)(loc / a.nsx);
+  } else {
+    it.lev = 1 + (int)(loc % a.d.kmax);
+    loc /= a.d.kmax;
+    it.sx = (int)(loc % a.nsx);
+    it.sy = (int)(loc / a.nsx);
+  }
   return true;
 }
 
