@@ -7,7 +7,8 @@
 !   adv_split_u/v/w    src/2d/adv_split_u.F90, adv_split_v.F90, src/3d/adv_split_w.F90
 !   adv_upstream_2dh   src/2d/adv_upstream_2dh.F90
 !   update_3d_halo     src/futils/halo_zones.F90:141-187
-! plus do_internal_pressure_gpu / do_eqstate_linear_gpu / do_eqstate_gpu,
+! plus do_internal_pressure_gpu / do_eqstate_linear_gpu / do_eqstate_gpu /
+! eos_internal_pressure_gpu (the linear EOS fused into the pressure methods),
 ! which take as arguments the module variables the argument-less reference
 ! routines (internal_pressure.F90, eqstate.F90) read, and the device-resident
 ! forms of the routines either side of the step:
@@ -31,6 +32,7 @@
    public gt_init, gt_finalize
    public do_advection_3d, do_advection_3d_dev
    public do_internal_pressure_gpu, do_eqstate_linear_gpu, update_3d_halo
+   public eos_internal_pressure_gpu
    public gt_malloc_f, gt_free_f, gt_to_device, gt_to_host
    public do_eqstate_gpu, start_macro_gpu, set_coordinates_gpu, coordinates_gpu
    public ww_momentum_3d_gpu, salinity_diffusion_gpu, temperature_diffusion_gpu
@@ -135,6 +137,16 @@
                bind(C, name='gt_eos_linear') result(rc)
          import :: c_ptr, c_int, c_double
          type(c_ptr), value :: ctx, T, S, rho, buoy
+         real(c_double), value :: T0, S0, dtr0, dsr0
+         integer(c_int) :: rc
+      end function
+      function c_gt_eos_internal_pressure(ctx, methods, T, S, T0, S0, dtr0, dsr0, H, hn, &
+               hun, hvn, sseo, zz_bm, idpdx_bm, idpdy_bm, zz_sm, idpdx_sm, idpdy_sm) &
+               bind(C, name='gt_eos_internal_pressure') result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx, T, S, H, hn, hun, hvn, sseo
+         type(c_ptr), value :: zz_bm, idpdx_bm, idpdy_bm, zz_sm, idpdx_sm, idpdy_sm
+         integer(c_int), value :: methods
          real(c_double), value :: T0, S0, dtr0, dsr0
          integer(c_int) :: rc
       end function
@@ -379,6 +391,23 @@
               idpdx, idpdy), 'do_internal_pressure()')
    call check(c_gt_synchronize(ctx), 'do_internal_pressure()')
    end subroutine do_internal_pressure_gpu
+
+   ! the linear do_eqstate (eqstate.F90:127-149,252-261) fused into
+   ! ip_blumberg_mellor (methods 1), ip_shchepetkin_mcwilliams (2) or both (3),
+   ! device-resident: the gradients of do_eqstate_linear_gpu followed by the
+   ! methods, bit for bit, without storing buoy; the outputs of a method not
+   ! asked for (and sseo without Shchepetkin-McWilliams) may be c_null_ptr
+   subroutine eos_internal_pressure_gpu(methods, T, S, T0, S0, dtr0, dsr0, H, hn, hun, &
+                                        hvn, sseo, zz_bm, idpdx_bm, idpdy_bm, zz_sm, &
+                                        idpdx_sm, idpdy_sm)
+   integer, intent(in) :: methods
+   type(c_ptr), intent(in) :: T, S, H, hn, hun, hvn, sseo
+   type(c_ptr), intent(in) :: zz_bm, idpdx_bm, idpdy_bm, zz_sm, idpdx_sm, idpdy_sm
+   real(c_double), intent(in) :: T0, S0, dtr0, dsr0
+   call check(c_gt_eos_internal_pressure(ctx, methods, T, S, T0, S0, dtr0, dsr0, H, hn, &
+              hun, hvn, sseo, zz_bm, idpdx_bm, idpdy_bm, zz_sm, idpdx_sm, idpdy_sm), &
+              'eos_internal_pressure()')
+   end subroutine eos_internal_pressure_gpu
 
    ! do_eqstate (eqstate.F90:94-271), eqstate_method 1, 2 or 3, device-resident;
    ! alpha/beta may be c_null_ptr (the _OLD_BVF_ build)
