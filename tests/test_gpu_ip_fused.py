@@ -134,3 +134,26 @@ def test_fused_rejects(device):
     with pytest.raises(ValueError):  # SMcW without sseo
         ctx.eos_internal_pressure(2, z, z, *CONSTS, H, z, z, z, None, None, None, None, z, z, z)
     ctx.close()
+
+
+@pytest.mark.parametrize("size,kind", [((1, 1, 2), "open"), ((2, 3, 3), "islands"),
+                                       ((61, 4, 2), "open"), ((3, 61, 4), "islands"),
+                                       ((121, 7, 5), "islands")])
+def test_fused_small_and_ragged(device, size, kind):
+    """grids of one interior cell, of one strip plus a column, a strip of
+    one row group and the like, with land and open edges, random fields"""
+    from types import SimpleNamespace
+    from tests.test_gpu_parity import random_domain
+    dom = random_domain(*size, 50 + size[0], kind)
+    sh = dom.shape3
+    tr = SimpleNamespace(tracers={"T": inputs.uniform(sh, 61, 5.0, 20.0),
+                                  "S": inputs.uniform(sh, 62, 30.0, 35.0)},
+                         hn=inputs.uniform(sh, 63, 0.5, 2.0),
+                         hun=inputs.uniform(sh, 64, 0.5, 2.0),
+                         hvn=inputs.uniform(sh, 65, 0.5, 2.0))
+    sseo = inputs.uniform((dom.NY, dom.NX), 66, -0.2, 0.2)
+    for methods in (1, 2, 3):
+        ref, per, fus = run_case(device, tr, dom, methods, 1.5, sseo)
+        for nm in NAMES:
+            assert np.array_equal(fus[nm], ref[nm], equal_nan=True), (methods, nm)
+            assert np.array_equal(per[nm], ref[nm], equal_nan=True), (methods, nm)
