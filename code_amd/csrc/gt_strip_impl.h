@@ -412,11 +412,21 @@ __device__ __forceinline__ void sto(T* base, uint32_t off, T v) {
 
 // The row march is unrolled four times: the four-row windows rotate by
 // renaming (logical row q of step s lives in slot (s+q)&3), and the inputs of
-// a row are loaded one step ahead into one of two row buffers.  Loads read
+// a row are loaded one step ahead into one of two row buffers (FP64) or at
+// the start of its step (FP32, below).  Loads read
 // valid addresses everywhere (cell 0 outside the array, the group's last
 // tracer past its end, level K for a level outside 0..kmax); only the flags
 // are masked, since every other value of a cell outside the array, or of a
 // missing level, reaches discarded results only.
+
+// FP64 (the exact flavour) loads the inputs of a row one step ahead into the
+// other of two row buffers; FP32 loads them where the step starts: without
+// the second buffer's registers the FP32 kernels run one more wavefront per
+// SIMD (uv 129 -> 101, wvu 229 -> 159 VGPRs at 9 tracers), which hides more
+// (batched step 2.80 -> 2.74 ms), while FP64 loses more than it gains
+// (advection 0.46 -> 0.49 ms with 140 instead of 182 VGPRs in wvu)
+template <typename TF>
+constexpr bool kRowAhead = std::is_same<TF, double>::value;
 
 // ---- kernel 1: u/2 then v/2 at level L ------------------------------------------------
 template <typename TF, int SCH, int G, bool EXACT, bool ADV, bool DIFF>
@@ -500,6 +510,7 @@ __global__ __launch_bounds__(256) void k_strip_uv(FusedArgs<TF> A) {
     constexpr int R = decltype(slot)::value;
     constexpr int w0 = R & 3, w1 = (R + 1) & 3, w2 = (R + 2) & 3, w3 = (R + 3) & 3;
     const int y = y0 - 2 + s;
+    if constexpr (!kRowAhead<TF>) load_row(y, buf);
     const int fl = (y >= 0 && y < NY) ? (buf.fl & cb.xin) : 0;  // raw masks of the cell
     const int fo = (fl | MB_XOUT) & cb.xm;                      // ... with the column bounds
 
@@ -574,17 +585,27 @@ __global__ __launch_bounds__(256) void k_strip_uv(FusedArgs<TF> A) {
       }
     }
     if (st_ok && it.grp == 0) sto(d2, cv * 8u, D2);
-    load_row(y + 2, buf);
+    if constexpr (kRowAhead<TF>) load_row(y + 2, buf);
   };
 
-  Row ra, rb;
-  load_row(y0 - 2, ra);
-  load_row(y0 - 1, rb);
-  for (int s = 0; s < nsteps; s += 4) {
-    step(Slot<0>{}, ra, s);
-    step(Slot<1>{}, rb, s + 1);
-    step(Slot<2>{}, ra, s + 2);
-    step(Slot<3>{}, rb, s + 3);
+  if constexpr (kRowAhead<TF>) {
+    Row ra, rb;
+    load_row(y0 - 2, ra);
+    load_row(y0 - 1, rb);
+    for (int s = 0; s < nsteps; s += 4) {
+      step(Slot<0>{}, ra, s);
+      step(Slot<1>{}, rb, s + 1);
+      step(Slot<2>{}, ra, s + 2);
+      step(Slot<3>{}, rb, s + 3);
+    }
+  } else {
+    Row r;
+    for (int s = 0; s < nsteps; s += 4) {
+      step(Slot<0>{}, r, s);
+      step(Slot<1>{}, r, s + 1);
+      step(Slot<2>{}, r, s + 2);
+      step(Slot<3>{}, r, s + 3);
+    }
   }
 
   // level 0 of the work fields (the second kernel reads levels K-2..K+2 of
@@ -724,6 +745,7 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
     constexpr int R = decltype(slot)::value;
     constexpr int w0 = R & 3, w1 = (R + 1) & 3, w2 = (R + 2) & 3, w3 = (R + 3) & 3;
     const int y = y0 - 2 + s;
+    if constexpr (!kRowAhead<TF>) load_row(y, buf);
     const int fl = (y >= 0 && y < NY) ? (buf.fl & cb.xin) : 0;  // raw masks of the cell
     const int fo = (fl | MB_XOUT) & cb.xm;                      // ... with the column bounds
 
@@ -843,17 +865,27 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
     }
     if (st_ok && it.grp == 0 && hires) sto(hires, cv * 8u, D5);
     if (bcell && it.grp == 0) bsh[wv][G][lane] += D5;
-    load_row(y + 2, buf);
+    if constexpr (kRowAhead<TF>) load_row(y + 2, buf);
   };
 
-  Row ra, rb;
-  load_row(y0 - 2, ra);
-  load_row(y0 - 1, rb);
-  for (int s = 0; s < nsteps; s += 4) {
-    step(Slot<0>{}, ra, s);
-    step(Slot<1>{}, rb, s + 1);
-    step(Slot<2>{}, ra, s + 2);
-    step(Slot<3>{}, rb, s + 3);
+  if constexpr (kRowAhead<TF>) {
+    Row ra, rb;
+    load_row(y0 - 2, ra);
+    load_row(y0 - 1, rb);
+    for (int s = 0; s < nsteps; s += 4) {
+      step(Slot<0>{}, ra, s);
+      step(Slot<1>{}, rb, s + 1);
+      step(Slot<2>{}, ra, s + 2);
+      step(Slot<3>{}, rb, s + 3);
+    }
+  } else {
+    Row r;
+    for (int s = 0; s < nsteps; s += 4) {
+      step(Slot<0>{}, r, s);
+      step(Slot<1>{}, r, s + 1);
+      step(Slot<2>{}, r, s + 2);
+      step(Slot<3>{}, r, s + 3);
+    }
   }
 
   // the budget partials of this strip-level: the lanes' sums added over the
