@@ -1256,9 +1256,12 @@ static int ensure_bbatch(gt_ctx* c, int ntr);
 extern "C" int gt_reserve(gt_ctx* c, int ntracers_f32, int ntracers_f64) {
   if (!c || ntracers_f32 < 0 || ntracers_f64 < 0)
     return set_err(GT_EARG, "gt_reserve: bad argument");
+  // the batched step's work fields with room for the budgets formed in it
   size_t ws = strip_ws_bytes(c, sizeof(double), 1, true);
-  const size_t w32 = strip_ws_bytes(c, sizeof(float), ntracers_f32, false);
-  const size_t w64 = strip_ws_bytes(c, sizeof(double), ntracers_f64, false);
+  const size_t w32 = strip_ws_bytes(c, sizeof(float), ntracers_f32, false) +
+                     (ntracers_f32 > 0 ? budget_ws_bytes(c, ntracers_f32) : 0);
+  const size_t w64 = strip_ws_bytes(c, sizeof(double), ntracers_f64, false) +
+                     (ntracers_f64 > 0 ? budget_ws_bytes(c, ntracers_f64) : 0);
   if (w32 > ws) ws = w32;
   if (w64 > ws) ws = w64;
   TRY(ensure_ws(c, ws));

@@ -427,8 +427,9 @@ int gt_set_tuning(gt_ctx *ctx, int fused, int tile, int group);
 int gt_set_adjoint_mode(gt_ctx *ctx, int keep_states);
 /* sizes the context's work fields for the transport step up front (the
  * HALFSPLIT step with advres, batched steps of up to ntracers_f32 FP32 or
- * ntracers_f64 FP64 tracers, NOSPLIT/FULLSPLIT/HVSPLIT buffers, SMcW's P,
- * the budget partials, and the sub-cycling columns if adv_ver_iterations >
+ * ntracers_f64 FP64 tracers with their budgets formed in the step,
+ * NOSPLIT/FULLSPLIT/HVSPLIT buffers, SMcW's P, the budget partials, and the
+ * sub-cycling columns if adv_ver_iterations >
  * 1), so later calls of those entry points neither allocate nor synchronise
  * the device to grow a field.  Without it they allocate on first use.  The
  * work memory of the transport step replaces the reference's automatic

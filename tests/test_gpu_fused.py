@@ -305,3 +305,41 @@ def test_batch_budgets_f64(device):
         m, v = ctx.budget(hires, fout[q])
         assert abs(got[q] - m) <= 1e-14 * abs(m), (q, got[q], m)
         assert abs(got[2] - v) <= 1e-14 * abs(v)
+
+
+def test_reserve_covers_batch_budgets(device):
+    """after gt_reserve for the tracer counts, the batched steps (FP32 at
+    every tracer group, FP64) with their budgets formed in the step allocate
+    no device memory: the free memory the device reports is unchanged"""
+    tr = inputs.ts_case(seed=4, imax=125, jmax=38, kmax=9)
+    d = tr.dom
+    ctx = gt.Context.from_domain(d)
+    nt32 = 7
+    ctx.reserve(nt32, 2)
+    host = [tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho]
+    dv = dict(zip(NAMES[:6], [to_dev(a, device) for a in host]))
+    f64 = to_dev(np.stack([tr.tracers["T"], tr.tracers["S"]]), device)
+    o64 = torch.empty_like(f64)
+    f32 = torch.stack([f64[0].float()] * nt32).contiguous()
+    o32 = torch.empty_like(f32)
+    hires = torch.empty_like(dv["ho"])
+    b64 = torch.zeros(3, dtype=torch.float64, device=device)
+    b32 = torch.zeros(nt32 + 1, dtype=torch.float64, device=device)
+
+    def steps(c):
+        for group in (1, 3, 6, 9):
+            c.set_tuning(1, 0, group)
+            gt.adv3d_batch([dict(ctx=c, f_in=f32, f_out=o32, hires=hires, budget=b32, **dv)],
+                           nt32, tr.dt, gt.P2_PDM, gt.P2_PDM)
+        gt.adv3d_batch([dict(ctx=c, f_in=f64, f_out=o64, hires=hires, budget=b64, **dv)], 2,
+                       tr.dt, gt.P2_PDM, gt.P2_PDM)
+        torch.cuda.synchronize()
+
+    # every kernel once on a context of its own first (code loaded on first
+    # use takes device memory too), then the reserved context
+    warm = gt.Context.from_domain(d)
+    steps(warm)
+    free0 = torch.cuda.mem_get_info()[0]
+    steps(ctx)
+    assert torch.cuda.mem_get_info()[0] == free0
+    assert b32.abs().min().item() > 0 and b64.abs().min().item() > 0
