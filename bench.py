@@ -596,12 +596,27 @@ def extensions_section(torch, np, gt, inputs, dev, tr, ctx, st, steps):
                              gt.HALFSPLIT, gt.P2_PDM, gt.P2_PDM, uuEx, vvEx)
 
     ww_in = D(tr.ww)
+    # Jackett et al. feeding both pressure methods: fused (rho read on land
+    # only, buoy not stored) and as do_eqstate followed by the two methods
+    sseo_j = D(inputs.uniform(sh2, 47, -0.2, 0.2))
+    pj = [torch.zeros_like(T) for _ in range(6)]
+    t_jf, t_jc = EventTimer(torch, st), EventTimer(torch, st)
+
+    def jackett_pressure(timed):
+        with (t_jf if timed else _Null()):
+            ctx.eqstate_internal_pressure(3, 3, T, S, rho, H, hn, hun, hvn, sseo_j, *pj)
+        with (t_jc if timed else _Null()):
+            ctx.do_eqstate(3, T, S, hn, rho, buoy)
+            ctx.ip_blumberg_mellor(H, hn, hun, hvn, buoy, *pj[:3])
+            ctx.ip_shchepetkin_mcwilliams(H, hn, hun, hvn, buoy, sseo_j, *pj[3:])
+
     with torch.cuda.stream(st):
         for _ in range(3):
             ctx.do_eqstate(3, T, S, hn, rho, buoy, alpha, beta)
             producers(False)  # the first coordinates call initialises
             diffusion(False)
             momentum(False)
+            jackett_pressure(False)
         st.synchronize()
         for _ in range(steps):
             with t_eos:
@@ -609,11 +624,12 @@ def extensions_section(torch, np, gt, inputs, dev, tr, ctx, st, steps):
             producers(True)
             diffusion(True)
             momentum(True)
+            jackett_pressure(True)
         st.synchronize()
     # the whole macro step on the device, one stream, in integrate_3d's order
     # (m3d.F90:446-607): start_macro, coordinates, ww_momentum_3d, T/S
     # advection (strip kernels), salinity and temperature diffusion, linear
-    # EOS, Blumberg-Mellor and Shchepetkin-McWilliams
+    # EOS fused into Blumberg-Mellor and Shchepetkin-McWilliams
     TS = torch.stack([T, S])
     TS2 = torch.empty_like(TS)
     hires = torch.zeros_like(T)
@@ -629,10 +645,8 @@ def extensions_section(torch, np, gt, inputs, dev, tr, ctx, st, steps):
         ctx.salinity_diffusion(tr.dt, 0.5, h3["hn"], nuh, nxt[1])
         ctx.temperature_diffusion(tr.dt, 0.5, h3["hn"], nuh, swr, shf, Aa, g1, g2, dry, nxt[1],
                                   rad, nxt[0])
-        ctx.eos_linear(nxt[0], nxt[1], None, buoy, *consts)
-        ctx.ip_blumberg_mellor(H, h3["hn"], h3["hun"], h3["hvn"], buoy, zz, ix, iy)
-        ctx.ip_shchepetkin_mcwilliams(H, h3["hn"], h3["hun"], h3["hvn"], buoy, e2["sseo"], zz6,
-                                      ix6, iy6)
+        ctx.eos_internal_pressure(3, nxt[0], nxt[1], *consts, H, h3["hn"], h3["hun"],
+                                  h3["hvn"], e2["sseo"], zz, ix, iy, zz6, ix6, iy6)
 
     bufs = [TS, TS2]
     with torch.cuda.stream(st):
@@ -647,6 +661,7 @@ def extensions_section(torch, np, gt, inputs, dev, tr, ctx, st, steps):
     e_s, c_s, w_s, m_s = t_eos.mean(), t_co.mean(), t_ww.mean(), t_sm.mean()
     sd_s, td_s, uv_s = t_sd.mean(), t_td.mean(), t_uv.mean()
     full_s = t_full.mean()
+    jf_s, jc_s = t_jf.mean(), t_jc.mean()
     gb = lambda b, s: b / s / 1e9  # noqa: E731
     return {
         "workload": f"{d.imax}x{d.jmax}x{d.kmax} (arrays {d.NX}x{d.NY}x{d.kmax + 1})",
@@ -665,8 +680,14 @@ def extensions_section(torch, np, gt, inputs, dev, tr, ctx, st, steps):
         "uv_advect_3d_ms": uv_s * 1e3,
         "full_step_ms": full_s * 1e3,
         "full_step": "start_macro, general coordinates, ww_momentum_3d, P2-PDM advection of "
-                     "T and S, salinity and temperature diffusion, linear EOS, Blumberg-Mellor, "
-                     "Shchepetkin-McWilliams: one stream, FP64",
+                     "T and S, salinity and temperature diffusion, linear EOS fused into "
+                     "Blumberg-Mellor and Shchepetkin-McWilliams: one stream, FP64",
+        "eqstate_jackett_pressure": {"kernel": "k_ip_columns + k_ip_rows<3,BM,SM> (fused)",
+                                     "fused_ms": jf_s * 1e3, "chain_ms": jc_s * 1e3,
+                                     "chain": "do_eqstate(3) without alpha/beta, then the two "
+                                              "methods' kernels",
+                                     "bytes_per_cell": 88, "GBps": gb(88 * n3, jf_s),
+                                     "frac": 88 * n3 / jf_s / HBM_PEAK},
         "temperature_diffusion": {"kernel": "k_vertical_diffusion<true> (with short-wave)",
                                   "ms": td_s * 1e3, "bytes_per_cell": 40,
                                   "GBps": gb(40 * n3, td_s),

@@ -1,5 +1,6 @@
-// gt_ip_fused.hip -- the linear equation of state fused into the internal
-// pressure gradients of Blumberg-Mellor (src/3d/ip_blumberg_mellor.F90) and
+// gt_ip_fused.hip -- the equation of state (linear, UNESCO 1980 or Jackett
+// et al.; src/3d/eqstate.F90:94-261) fused into the internal pressure
+// gradients of Blumberg-Mellor (src/3d/ip_blumberg_mellor.F90) and
 // Shchepetkin-McWilliams (src/3d/ip_shchepetkin_mcwilliams.F90), either or
 // both, in two launches:
 //   k_ip_columns  one thread per column: the level-0 zeros, the interface
@@ -10,11 +11,13 @@
 //                 read (never stored), Blumberg-Mellor's two downward
 //                 recurrences, the pressure integral P of
 //                 Shchepetkin-McWilliams and the Jacobians of both faces.
-// Every expression is the per-method kernels' (gt_kernels.hip), in their
-// order, so the gradients are bit for bit those of gt_eos_linear followed by
-// gt_ip_blumberg_mellor / gt_ip_shchepetkin_mcwilliams.
+// Every expression is the per-method kernels' (gt_kernels.hip,
+// gt_eqstate.hip), in their order, so the gradients are bit for bit those of
+// gt_eos_linear or gt_do_eqstate followed by gt_ip_blumberg_mellor /
+// gt_ip_shchepetkin_mcwilliams.
 #include <hip/hip_runtime.h>
 
+#include "gt_eos.h"
 #include "gt_ip.h"
 #include "gt_kernels.h"
 
@@ -179,9 +182,16 @@ struct IpLevel {
 // wavefront above) and Blumberg-Mellor's recurrences.  Every mean is formed
 // once and rounds as the per-face formula.  Rows past NY-1 and columns past
 // NX-1 are clamped (read, never written).
-template <bool BM, bool SM, int W>
+// EOS 1: the linear equation of state at every cell; 2 (UNESCO 1980) and 3
+// (Jackett et al. at p = 0): the density of water columns from T and S
+// (S < 0 taken as 0 when nonneg), and on land columns (az = 0) the rho the
+// caller holds, which do_eqstate leaves there and turns into buoyancy
+// (eqstate.F90:154-158,198-222,252-261): Shchepetkin-McWilliams' harmonic
+// means reach one column past a water face.
+template <int EOS, bool BM, bool SM, int W>
 __global__ __launch_bounds__(64 * (W + 1)) void k_ip_rows(
     Dims d, int nsx, int nby, const int32_t* __restrict__ au, const int32_t* __restrict__ av,
+    const int32_t* __restrict__ az, const double* __restrict__ rho, int nonneg,
     const double* __restrict__ T, const double* __restrict__ S, double T0, double S0,
     double dtr0, double dsr0, const double* __restrict__ hn, const double* __restrict__ hun,
     const double* __restrict__ hvn, const double* __restrict__ zzb,
@@ -215,23 +225,37 @@ __global__ __launch_bounds__(64 * (W + 1)) void k_ip_rows(
   const unsigned ce = carrier ? cp : cell(y - 1);
   const bool out = !carrier && lane >= 1 && lane <= 60 && x <= NX - 3 && y <= NY - 3;
   const bool ux = out && au[c0] >= 1, vy = out && av[c0] >= 1;
-  auto buoy = [&](double t, double s) { return eos_linear_buoy(t, s, T0, S0, dtr0, dsr0); };
+  // water columns of c0, cp and the extra row's cell (te, se below)
+  const bool w0 = EOS == 1 || az[c0] > 0, wp = EOS == 1 || az[cp] > 0;
+  const bool wte = SM ? EOS == 1 || az[ce] > 0 : wp;
+  const bool nn = nonneg != 0;
+  // the buoyancy of a cell from its T and S, or (land, full equations of
+  // state) from its rho, which the loads put in place of T
+  auto buoy = [&](double t, double s, bool wet) -> double {
+    if constexpr (EOS == 1) {
+      return eos_linear_buoy(t, s, T0, S0, dtr0, dsr0);
+    } else {
+      return buoy_of_rho(wet ? eqstate_rho<EOS>(t, s, nn) : t);
+    }
+  };
+  auto ldt = [&](long o, unsigned c, bool wet) { return wet ? (T + o)[c] : (rho + o)[c]; };
+  auto lds = [&](long o, unsigned c, bool wet) { return wet ? (S + o)[c] : 0.0; };
   auto load = [&](int k, IpLevel& L) {
     const long ok = NXY * (long)k, okm = NXY * (long)(k >= 2 ? k - 1 : 1);
     const long o0 = SM ? okm : ok;
-    L.t0 = (T + o0)[c0];
-    L.s0 = (S + o0)[c0];
+    L.t0 = ldt(o0, c0, w0);
+    L.s0 = lds(o0, c0, w0);
     if constexpr (SM) {
       L.z0 = (zzs + o0)[c0];
       if (first || carrier) {
-        L.te = (T + ok)[ce];
-        L.se = (S + ok)[ce];
+        L.te = ldt(ok, ce, wte);
+        L.se = lds(ok, ce, wte);
         L.ze = (zzs + ok)[ce];
       }
     } else if (last) {
       // Blumberg-Mellor alone: the last wavefront reads row y+1 itself
-      L.te = (T + ok)[cp];
-      L.se = (S + ok)[cp];
+      L.te = ldt(ok, cp, wte);
+      L.se = lds(ok, cp, wte);
       L.ze = (zzb + ok)[cp];
     }
     if constexpr (BM) L.zb0 = (zzb + ok)[c0];
@@ -247,7 +271,7 @@ __global__ __launch_bounds__(64 * (W + 1)) void k_ip_rows(
 
   // ---- level kmax
   const long okx = NXY * (long)kmax;
-  const double kb0 = buoy((T + okx)[c0], (S + okx)[c0]);
+  const double kb0 = buoy(ldt(okx, c0, w0), lds(okx, c0, w0), w0);
   // Shchepetkin-McWilliams: P of the row, the face constants
   PCol ps{};
   double ss0 = 0, ssp = 0, sU = 0, kU = 0, kbp = 0;
@@ -264,7 +288,7 @@ __global__ __launch_bounds__(64 * (W + 1)) void k_ip_rows(
     __syncthreads();
     double bn = 0, zn = 0, zbn = 0;  // row y+1 at k
     if (!SM && last) {
-      bn = buoy(L.te, L.se);
+      bn = buoy(L.te, L.se, wte);
       zbn = L.ze;
     } else if (!carrier) {
       bn = shA[slot][0][wave + 1][lane];
@@ -275,14 +299,14 @@ __global__ __launch_bounds__(64 * (W + 1)) void k_ip_rows(
       // the means of rows y-1|y|y+1, shared with the wavefront below
       double bm, zm, bp, zp;
       if (first) {
-        bm = buoy(L.te, L.se);
+        bm = buoy(L.te, L.se, wte);
         zm = L.ze;
       } else {
         bm = shA[slot][0][wave - 1][lane];
         zm = shA[slot][1][wave - 1][lane];
       }
       if (carrier) {
-        bp = buoy(L.te, L.se);
+        bp = buoy(L.te, L.se, wte);
         zp = L.ze;
       } else {
         bp = bn;
@@ -362,8 +386,8 @@ __global__ __launch_bounds__(64 * (W + 1)) void k_ip_rows(
     IpLevel top;
     load(kmax, top);
     if constexpr (SM) {
-      ps.top(kb0, buoy(top.t0, top.s0), (zzs + okx)[c0], top.z0, (hn + okx)[c0]);
-      kbp = buoy((T + okx)[cp], (S + okx)[cp]);
+      ps.top(kb0, buoy(top.t0, top.s0, w0), (zzs + okx)[c0], top.z0, (hn + okx)[c0]);
+      kbp = buoy(ldt(okx, cp, wp), lds(okx, cp, wp), wp);
       ss0 = sseo[c0];
       ssp = sseo[cp];
       sU = from_upper_lane(ss0);
@@ -377,10 +401,10 @@ __global__ __launch_bounds__(64 * (W + 1)) void k_ip_rows(
   // ---- levels kmax-1 .. 1
   auto level = [&](int k, const IpLevel& L) {
     if constexpr (SM) {
-      ps.step(k, buoy(L.t0, L.s0), L.z0);
+      ps.step(k, buoy(L.t0, L.s0, w0), L.z0);
       level_faces(k, k & 1, ps.b1, ps.z1, L, false);
     } else {
-      level_faces(k, k & 1, buoy(L.t0, L.s0), 0.0, L, false);
+      level_faces(k, k & 1, buoy(L.t0, L.s0, w0), 0.0, L, false);
     }
   };
   // a ring of NB operand buffers: level k's operands are loaded NB-1 levels
@@ -403,9 +427,10 @@ __global__ __launch_bounds__(64 * (W + 1)) void k_ip_rows(
 
 constexpr int kIpRowsW = 7;  // output rows per workgroup of k_ip_rows (W+1 wavefronts)
 
-hipError_t launch_eos_ip_fused(hipStream_t st, Dims d, bool bm, bool sm, const int32_t* az,
-                               const int32_t* au, const int32_t* av, const double* T,
-                               const double* S, double T0, double S0, double dtr0, double dsr0,
+hipError_t launch_eos_ip_fused(hipStream_t st, Dims d, int eos, bool nonneg, bool bm, bool sm,
+                               const int32_t* az, const int32_t* au, const int32_t* av,
+                               const double* T, const double* S, const double* rho, double T0,
+                               double S0, double dtr0, double dsr0,
                                const double* H, const double* hn, const double* hun,
                                const double* hvn, const double* sseo, double* zzb, double* ixb,
                                double* iyb, double* zzs, double* ixs, double* iys, double dx,
@@ -417,25 +442,31 @@ hipError_t launch_eos_ip_fused(hipStream_t st, Dims d, bool bm, bool sm, const i
   const long nblk = (long)nsx * nby;
   const dim3 g2((unsigned)(((nblk + 7) / 8) * 8));
   const double dxm1 = 1.0 / dx, dym1 = 1.0 / dy;
-#define GT_IPF(B, M)                                                                          \
+  const int nn = nonneg ? 1 : 0;
+#define GT_IPF(E, B, M)                                                                       \
   do {                                                                                        \
     hipLaunchKernelGGL((k_ip_columns<B, M>), g1, blk, 0, st, d, az, H, hn, zzb, zzs, ixb, iyb, \
                        ixs, iys);                                                             \
     hipError_t e_ = hipGetLastError();                                                        \
     if (e_ != hipSuccess) return e_;                                                          \
     if (d.NX > 4 && d.NY > 4)                                                                 \
-      hipLaunchKernelGGL((k_ip_rows<B, M, W>), g2, dim3(64 * (W + (M ? 1 : 0))), 0, st, d,    \
-                         nsx, nby, au,                                                        \
-                         av,                                                                  \
-                         T, S, T0, S0, dtr0, dsr0, hn, hun, hvn, zzb, zzs, sseo, ixb, iyb,    \
-                         ixs, iys, dxm1, dym1);                                               \
+      hipLaunchKernelGGL((k_ip_rows<E, B, M, W>), g2, dim3(64 * (W + (M ? 1 : 0))), 0, st, d, \
+                         nsx, nby, au, av, az, rho, nn, T, S, T0, S0, dtr0, dsr0, hn, hun,    \
+                         hvn, zzb, zzs, sseo, ixb, iyb, ixs, iys, dxm1, dym1);                \
     return hipGetLastError();                                                                 \
   } while (0)
-  if (bm && sm) GT_IPF(true, true);
-  if (bm) GT_IPF(true, false);
-  if (sm) GT_IPF(false, true);
+#define GT_IPF_EOS(E)                  \
+  do {                                 \
+    if (bm && sm) GT_IPF(E, true, true); \
+    if (bm) GT_IPF(E, true, false);    \
+    if (sm) GT_IPF(E, false, true);    \
+    return hipSuccess;                 \
+  } while (0)
+  if (eos == 2) GT_IPF_EOS(2);
+  if (eos == 3) GT_IPF_EOS(3);
+  GT_IPF_EOS(1);
+#undef GT_IPF_EOS
 #undef GT_IPF
-  return hipSuccess;
 }
 
 }  // namespace gt

@@ -107,9 +107,12 @@ hipError_t launch_ip_bm(hipStream_t st, Dims d, const int32_t* az, const int32_t
 // the linear EOS fused into Blumberg-Mellor (bm) and/or Shchepetkin-McWilliams
 // (sm): two launches, gradients bit for bit those of launch_eos_linear +
 // launch_ip_bm / launch_ip_smcw (gt_ip_fused.hip)
-hipError_t launch_eos_ip_fused(hipStream_t st, Dims d, bool bm, bool sm, const int32_t* az,
-                               const int32_t* au, const int32_t* av, const double* T,
-                               const double* S, double T0, double S0, double dtr0, double dsr0,
+// eos: do_eqstate's method (1 linear, 2 UNESCO 1980, 3 Jackett et al.); rho
+// is read on land columns under methods 2 and 3 only
+hipError_t launch_eos_ip_fused(hipStream_t st, Dims d, int eos, bool nonneg, bool bm, bool sm,
+                               const int32_t* az, const int32_t* au, const int32_t* av,
+                               const double* T, const double* S, const double* rho, double T0,
+                               double S0, double dtr0, double dsr0,
                                const double* H, const double* hn, const double* hun,
                                const double* hvn, const double* sseo, double* zzb, double* ixb,
                                double* iyb, double* zzs, double* ixs, double* iys, double dx,

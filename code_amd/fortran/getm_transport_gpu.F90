@@ -8,7 +8,8 @@
 !   adv_upstream_2dh   src/2d/adv_upstream_2dh.F90
 !   update_3d_halo     src/futils/halo_zones.F90:141-187
 ! plus do_internal_pressure_gpu / do_eqstate_linear_gpu / do_eqstate_gpu /
-! eos_internal_pressure_gpu (the linear EOS fused into the pressure methods),
+! eos_internal_pressure_gpu and eqstate_internal_pressure_gpu (the linear or
+! any EOS fused into the pressure methods),
 ! which take as arguments the module variables the argument-less reference
 ! routines (internal_pressure.F90, eqstate.F90) read, and the device-resident
 ! forms of the routines either side of the step:
@@ -32,7 +33,7 @@
    public gt_init, gt_finalize
    public do_advection_3d, do_advection_3d_dev
    public do_internal_pressure_gpu, do_eqstate_linear_gpu, update_3d_halo
-   public eos_internal_pressure_gpu
+   public eos_internal_pressure_gpu, eqstate_internal_pressure_gpu
    public gt_malloc_f, gt_free_f, gt_to_device, gt_to_host
    public do_eqstate_gpu, start_macro_gpu, set_coordinates_gpu, coordinates_gpu
    public ww_momentum_3d_gpu, salinity_diffusion_gpu, temperature_diffusion_gpu
@@ -147,6 +148,16 @@
          type(c_ptr), value :: ctx, T, S, H, hn, hun, hvn, sseo
          type(c_ptr), value :: zz_bm, idpdx_bm, idpdy_bm, zz_sm, idpdx_sm, idpdy_sm
          integer(c_int), value :: methods
+         real(c_double), value :: T0, S0, dtr0, dsr0
+         integer(c_int) :: rc
+      end function
+      function c_gt_eqstate_internal_pressure(ctx, method, nonneg, methods, T, S, rho, T0, &
+               S0, dtr0, dsr0, H, hn, hun, hvn, sseo, zz_bm, idpdx_bm, idpdy_bm, zz_sm, &
+               idpdx_sm, idpdy_sm) bind(C, name='gt_eqstate_internal_pressure') result(rc)
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx, T, S, rho, H, hn, hun, hvn, sseo
+         type(c_ptr), value :: zz_bm, idpdx_bm, idpdy_bm, zz_sm, idpdx_sm, idpdy_sm
+         integer(c_int), value :: method, nonneg, methods
          real(c_double), value :: T0, S0, dtr0, dsr0
          integer(c_int) :: rc
       end function
@@ -408,6 +419,28 @@
               hun, hvn, sseo, zz_bm, idpdx_bm, idpdy_bm, zz_sm, idpdx_sm, idpdy_sm), &
               'eos_internal_pressure()')
    end subroutine eos_internal_pressure_gpu
+
+   ! do_eqstate of eqstate_method 1, 2 or 3 (eqstate.F90:94-261) fused into the
+   ! pressure methods as eos_internal_pressure_gpu, device-resident: rho is
+   ! read on land columns (methods 2 and 3; c_null_ptr allowed for 1) and
+   ! never written; the gradients are those of do_eqstate_gpu followed by
+   ! the methods, bit for bit
+   subroutine eqstate_internal_pressure_gpu(method, methods, T, S, rho, T0, S0, dtr0, dsr0, &
+                                            H, hn, hun, hvn, sseo, zz_bm, idpdx_bm, &
+                                            idpdy_bm, zz_sm, idpdx_sm, idpdy_sm)
+   integer, intent(in) :: method, methods
+   type(c_ptr), intent(in) :: T, S, rho, H, hn, hun, hvn, sseo
+   type(c_ptr), intent(in) :: zz_bm, idpdx_bm, idpdy_bm, zz_sm, idpdx_sm, idpdy_sm
+   real(c_double), intent(in) :: T0, S0, dtr0, dsr0
+   integer(c_int) :: nonneg
+   nonneg = 0
+#ifdef NONNEGSALT
+   nonneg = 1
+#endif
+   call check(c_gt_eqstate_internal_pressure(ctx, method, nonneg, methods, T, S, rho, T0, &
+              S0, dtr0, dsr0, H, hn, hun, hvn, sseo, zz_bm, idpdx_bm, idpdy_bm, zz_sm, &
+              idpdx_sm, idpdy_sm), 'eqstate_internal_pressure()')
+   end subroutine eqstate_internal_pressure_gpu
 
    ! do_eqstate (eqstate.F90:94-271), eqstate_method 1, 2 or 3, device-resident;
    ! alpha/beta may be c_null_ptr (the _OLD_BVF_ build)

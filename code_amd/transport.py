@@ -50,7 +50,8 @@ EXPORTS = [
     "gt_uv_advect_3d", "gt_restart_write_host", "gt_restart_read_host", "gt_write_restart",
     "gt_read_restart", "gt_set_substr_ini_press", "gt_reserve", "gt_budget_batch_f32",
     "gt_budget_batch_f32_noalloc", "gt_adv_split_u_m", "gt_adv_split_v_m", "gt_adv_split_w_m",
-    "gt_adv_upstream_2dh_m", "gt_eos_internal_pressure", "gt_get_masks",
+    "gt_adv_upstream_2dh_m", "gt_eos_internal_pressure", "gt_eqstate_internal_pressure",
+    "gt_get_masks",
 ]
 
 
@@ -146,6 +147,7 @@ def load_library():
         "gt_ip_shchepetkin_mcwilliams": (i, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
         "gt_do_internal_pressure": (i, [vp, i, vp, vp, vp, vp, vp, vp, vp, vp]),
         "gt_eos_internal_pressure": (i, [vp, i, vp, vp, d, d, d, d] + [vp] * 11),
+        "gt_eqstate_internal_pressure": (i, [vp, i, i, i, vp, vp, vp, d, d, d, d] + [vp] * 11),
         "gt_set_substr_ini_press": (i, [vp, i]),
         "gt_reserve": (i, [vp, i, i]),
         "gt_update_3d_halo": (i, [vp, vp, vp, i, i]),
@@ -552,6 +554,28 @@ class Context:
         check(self.lib.gt_eos_internal_pressure(
             self.h, methods, *ps, T0, S0, dtr0, dsr0, self._f2(H, "H"), *hs,
             self._f2(sseo, "sseo", optional=not sm), *outs))
+
+    def eqstate_internal_pressure(self, eqstate_method, methods, T, S, rho, H, hn, hun, hvn,
+                                  sseo=None, zz_bm=None, idpdx_bm=None, idpdy_bm=None,
+                                  zz_sm=None, idpdx_sm=None, idpdy_sm=None, nonnegsalt=False,
+                                  T0=10.0, S0=33.75, dtr0=-0.17, dsr0=0.78):
+        """do_eqstate of any method (eqstate.F90:94-261) fused into
+        Blumberg-Mellor and/or Shchepetkin-McWilliams: the gradients of
+        do_eqstate followed by ip_blumberg_mellor / ip_shchepetkin_mcwilliams,
+        bit for bit; rho is read on land columns (methods 2 and 3) and never
+        written.  The linear constants default to the reference's namelist
+        values, FP32 literals (rounded here)."""
+        f32 = lambda v: float(np.float32(v))
+        bm, sm = bool(methods & 1), bool(methods & 2)
+        ps = [self._f3(t, nm) for t, nm in ((T, "T"), (S, "S"))]
+        hs = [self._f3(t, nm) for t, nm in ((hn, "hn"), (hun, "hun"), (hvn, "hvn"))]
+        outs = [self._f3(t, nm, optional=not on) for t, nm, on in (
+            (zz_bm, "zz_bm", bm), (idpdx_bm, "idpdx_bm", bm), (idpdy_bm, "idpdy_bm", bm),
+            (zz_sm, "zz_sm", sm), (idpdx_sm, "idpdx_sm", sm), (idpdy_sm, "idpdy_sm", sm))]
+        check(self.lib.gt_eqstate_internal_pressure(
+            self.h, eqstate_method, int(nonnegsalt), methods, *ps,
+            self._f3(rho, "rho", optional=eqstate_method == 1), f32(T0), f32(S0), f32(dtr0),
+            f32(dsr0), self._f2(H, "H"), *hs, self._f2(sseo, "sseo", optional=not sm), *outs))
 
     def do_internal_pressure(self, ip_method, H, hn, hun, hvn, buoy, sseo, idpdx, idpdy):
         ps = [self._f3(t, nm) for t, nm in ((hn, "hn"), (hun, "hun"), (hvn, "hvn"),

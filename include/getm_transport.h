@@ -334,6 +334,26 @@ int gt_eos_internal_pressure(gt_ctx *ctx, int methods, const double *T,
                              const double *hvn, const double *sseo,
                              double *zz_bm, double *idpdx_bm, double *idpdy_bm,
                              double *zz_sm, double *idpdx_sm, double *idpdy_sm);
+/* the same for any eqstate_method (1 linear, 2 UNESCO 1980, 3 Jackett et
+ * al.; eqstate.F90:94-261, nonnegsalt as in gt_do_eqstate): every gradient
+ * is bit for bit that of gt_do_eqstate followed by gt_ip_blumberg_mellor /
+ * gt_ip_shchepetkin_mcwilliams.  Methods 2 and 3 read rho on land columns
+ * (az = 0) only: do_eqstate leaves rho there and forms the buoyancy from it
+ * (eqstate.F90:154,200,252-261), and Shchepetkin-McWilliams' harmonic means
+ * reach one column past a water face; rho may be NULL for method 1, and is
+ * never written (a caller that needs rho or buoy calls gt_do_eqstate).
+ * T0, S0, dtr0, dsr0 are read by method 1 only.  gt_eos_internal_pressure
+ * is this call with eqstate_method 1. */
+int gt_eqstate_internal_pressure(gt_ctx *ctx, int eqstate_method,
+                                 int nonnegsalt, int methods, const double *T,
+                                 const double *S, const double *rho, double T0,
+                                 double S0, double dtr0, double dsr0,
+                                 const double *H, const double *hn,
+                                 const double *hun, const double *hvn,
+                                 const double *sseo, double *zz_bm,
+                                 double *idpdx_bm, double *idpdy_bm,
+                                 double *zz_sm, double *idpdx_sm,
+                                 double *idpdy_sm);
 /* do_internal_pressure, src/3d/internal_pressure.F90:181-274, with the
  * context's own zz (zero at creation, as init_internal_pressure leaves it).
  * sseo is only read by method 6. */

@@ -1,10 +1,12 @@
-"""The linear EOS fused into the internal pressure gradients
-(gt_eos_internal_pressure, gt_ip_fused.hip) against gt_eos_linear followed by
-gt_ip_blumberg_mellor / gt_ip_shchepetkin_mcwilliams and against the oracle
-(eqstate.F90:127-149,252-261, ip_blumberg_mellor.F90:71-152,
-ip_shchepetkin_mcwilliams.F90:71-288): bit for bit, every method alone and
-both together, on open water, on islands with sentinel-filled outputs (the
-cells a method does not write keep what they held) and at full size."""
+"""The equation of state fused into the internal pressure gradients
+(gt_eos_internal_pressure, gt_eqstate_internal_pressure, gt_ip_fused.hip)
+against gt_eos_linear / gt_do_eqstate followed by gt_ip_blumberg_mellor /
+gt_ip_shchepetkin_mcwilliams and against the oracle (eqstate.F90:94-261,
+ip_blumberg_mellor.F90:71-152, ip_shchepetkin_mcwilliams.F90:71-288): bit for
+bit, every method alone and both together, on open water, on islands with
+sentinel-filled outputs (the cells a method does not write keep what they
+held) and at full size; UNESCO 1980 and Jackett et al. with and without
+NONNEGSALT, the land columns' rho read as do_eqstate leaves it."""
 import numpy as np
 import pytest
 import torch
@@ -157,3 +159,112 @@ def test_fused_small_and_ragged(device, size, kind):
         for nm in NAMES:
             assert np.array_equal(fus[nm], ref[nm], equal_nan=True), (methods, nm)
             assert np.array_equal(per[nm], ref[nm], equal_nan=True), (methods, nm)
+
+
+def run_case_eqstate(device, tr, dom, eqm, nonneg, methods, sentinel, sseo, rho0):
+    """do_eqstate of method eqm fused into the pressure gradients: fused GPU,
+    do_eqstate + per-method GPU kernels and the oracle's chain, all starting
+    from the same rho (kept on land by do_eqstate)"""
+    T, S = tr.tracers["T"], tr.tracers["S"]
+    bm, sm = bool(methods & 1), bool(methods & 2)
+    start = {nm: np.full(dom.shape3, sentinel) for nm in NAMES}
+    ref = {nm: a.copy() for nm, a in start.items()}
+    orc = po.Oracle.from_domain(dom)
+    rho, buoy = rho0.copy(), np.zeros(dom.shape3)
+    orc.do_eqstate(eqm, T, S, tr.hn, rho, buoy, nonnegsalt=nonneg)
+    if bm:
+        orc.ip_blumberg_mellor(dom.H, tr.hn, tr.hun, tr.hvn, buoy, ref["zb"], ref["bx"],
+                               ref["by"])
+    if sm:
+        orc.ip_shchepetkin_mcwilliams(dom.H, tr.hn, tr.hun, tr.hvn, buoy, sseo, ref["zs"],
+                                      ref["sx"], ref["sy"])
+    ctx = gt.Context.from_domain(dom)
+    dv = {nm: to_dev(a, device) for nm, a in (("T", T), ("S", S), ("H", dom.H), ("hn", tr.hn),
+                                              ("hun", tr.hun), ("hvn", tr.hvn), ("sseo", sseo),
+                                              ("rho", rho0))}
+    per = {nm: to_dev(a, device) for nm, a in start.items()}
+    trho = to_dev(rho0, device)
+    tb = torch.zeros(dom.shape3, dtype=torch.float64, device=device)
+    ctx.do_eqstate(eqm, dv["T"], dv["S"], dv["hn"], trho, tb, nonnegsalt=nonneg)
+    if bm:
+        ctx.ip_blumberg_mellor(dv["H"], dv["hn"], dv["hun"], dv["hvn"], tb, per["zb"], per["bx"],
+                               per["by"])
+    if sm:
+        ctx.ip_shchepetkin_mcwilliams(dv["H"], dv["hn"], dv["hun"], dv["hvn"], tb, dv["sseo"],
+                                      per["zs"], per["sx"], per["sy"])
+    fus = {nm: to_dev(a, device) for nm, a in start.items()}
+    ctx.eqstate_internal_pressure(eqm, methods, dv["T"], dv["S"], dv["rho"], dv["H"], dv["hn"],
+                                  dv["hun"], dv["hvn"], dv["sseo"] if sm else None,
+                                  *(fus[nm] if on else None for nm, on in zip(
+                                      NAMES, (bm, bm, bm, sm, sm, sm))), nonnegsalt=nonneg)
+    torch.cuda.synchronize()
+    # the fused call leaves rho as it was
+    assert torch.equal(dv["rho"], to_dev(rho0, device))
+    ctx.close()
+    return ref, {k: to_host(v) for k, v in per.items()}, {k: to_host(v) for k, v in fus.items()}
+
+
+def eqstate_case(seed, nonneg, imax=90, jmax=61, kmax=11):
+    """T/S profiles on islands, the land rho unrelated to T and S, and with
+    nonneg some salinities below zero"""
+    tr = inputs.ts_case(seed=seed, imax=imax, jmax=jmax, kmax=kmax)
+    dom = with_islands(tr.dom, seed + 1)
+    if nonneg:
+        S = tr.tracers["S"].copy()
+        neg = np.random.default_rng(seed + 2).random(S.shape) < 0.03
+        S[neg] = -0.5
+        tr.tracers["S"] = S
+    rho0 = inputs.uniform(dom.shape3, seed + 3, 1020.0, 1030.0)
+    sseo = inputs.uniform((dom.NY, dom.NX), seed + 4, -0.2, 0.2)
+    return tr, dom, rho0, sseo
+
+
+@pytest.mark.parametrize("methods", [1, 2, 3])
+@pytest.mark.parametrize("eqm,nonneg", [(2, False), (2, True), (3, False), (3, True)])
+def test_fused_eqstate_islands(device, eqm, nonneg, methods):
+    """UNESCO 1980 and Jackett et al. on islands, sentinel-filled outputs:
+    bit for bit the chain of do_eqstate and the per-method kernels, and the
+    oracle's"""
+    tr, dom, rho0, sseo = eqstate_case(20 + eqm, nonneg)
+    check(*run_case_eqstate(device, tr, dom, eqm, nonneg, methods, 2.5, sseo, rho0), methods)
+
+
+def test_fused_eqstate_reads_land_rho(device):
+    """Shchepetkin-McWilliams' gradients next to land change with the rho held
+    on land and match the chain's for both; Blumberg-Mellor's do not change"""
+    tr, dom, rho0, sseo = eqstate_case(31, False)
+    rho1 = rho0.copy()
+    rho1[:, dom.az == 0] += 1.0
+    a = run_case_eqstate(device, tr, dom, 3, False, 3, 0.0, sseo, rho0)
+    b = run_case_eqstate(device, tr, dom, 3, False, 3, 0.0, sseo, rho1)
+    check(*a, 3)
+    check(*b, 3)
+    assert not np.array_equal(a[2]["sx"], b[2]["sx"])
+    assert not np.array_equal(a[2]["sy"], b[2]["sy"])
+    for nm in ("zb", "bx", "by", "zs"):
+        assert np.array_equal(a[2][nm], b[2][nm]), nm
+
+
+@pytest.mark.timeout(300)
+def test_fused_eqstate_full_size(device):
+    """400 x 400 x 40 GVC, Jackett et al. and both methods"""
+    po.set_threads(16)
+    tr = inputs.ts_case()
+    rho0 = inputs.uniform(tr.dom.shape3, 48, 1020.0, 1030.0)
+    sseo = inputs.uniform((tr.dom.NY, tr.dom.NX), 49, -0.2, 0.2)
+    check(*run_case_eqstate(device, tr, tr.dom, 3, False, 3, 0.0, sseo, rho0), 3)
+
+
+def test_fused_eqstate_rejects(device):
+    tr = inputs.ts_case(seed=5, imax=20, jmax=18, kmax=4)
+    ctx = gt.Context.from_domain(tr.dom)
+    z = torch.zeros(tr.dom.shape3, dtype=torch.float64, device=device)
+    H = to_dev(tr.dom.H, device)
+    for eqm in (0, 4):
+        with pytest.raises(gt.GtError, match="not a valid eqstate_method"):
+            ctx.eqstate_internal_pressure(eqm, 1, z, z, z, H, z, z, z, None, z, z, z)
+    with pytest.raises(ValueError):  # rho is read on land by methods 2 and 3
+        ctx.eqstate_internal_pressure(3, 1, z, z, None, H, z, z, z, None, z, z, z)
+    # method 1 does without rho
+    ctx.eqstate_internal_pressure(1, 1, z, z, None, H, z, z, z, None, z, z, z)
+    ctx.close()
