@@ -1127,28 +1127,39 @@ extern "C" int gt_ip_shchepetkin_mcwilliams(gt_ctx* c, const double* H, const do
   return GT_OK;
 }
 
-extern "C" int gt_eqstate_internal_pressure(
-    gt_ctx* c, int eqstate_method, int nonnegsalt, int methods, const double* T, const double* S,
-    const double* rho, double T0, double S0, double dtr0, double dsr0, const double* H,
-    const double* hn, const double* hun, const double* hvn, const double* sseo, double* zz_bm,
-    double* idpdx_bm, double* idpdy_bm, double* zz_sm, double* idpdx_sm, double* idpdy_sm) {
-  if (!c) return set_err(GT_EARG, "gt_eqstate_internal_pressure: NULL context");
+static int eos_ip(gt_ctx* c, const char* who, int eqstate_method, int nonnegsalt, int methods,
+                  const double* T, const double* S, const double* rho, double T0, double S0,
+                  double dtr0, double dsr0, const double* H, const double* hn,
+                  const double* hun, const double* hvn, const double* sseo, double* zz_bm,
+                  double* idpdx_bm, double* idpdy_bm, double* zz_sm, double* idpdx_sm,
+                  double* idpdy_sm) {
+  if (!c) return set_err(GT_EARG, "%s: NULL context", who);
   if (eqstate_method < 1 || eqstate_method > 3)
     return set_err(GT_EARG, "init_eqstate(): not a valid eqstate_method");
   if (methods < 1 || methods > 3)
-    return set_err(GT_EARG, "gt_eqstate_internal_pressure: methods must be 1 (Blumberg-Mellor), "
-                            "2 (Shchepetkin-McWilliams) or 3 (both)");
-  if (c->kmax < 2) return set_err(GT_EARG, "gt_eqstate_internal_pressure: kmax must be >= 2");
+    return set_err(GT_EARG, "%s: methods must be 1 (Blumberg-Mellor), 2 (Shchepetkin-McWilliams) "
+                            "or 3 (both)", who);
+  if (c->kmax < 2) return set_err(GT_EARG, "%s: kmax must be >= 2", who);
   const bool bm = (methods & 1) != 0, sm = (methods & 2) != 0;
   if (!T || !S || !H || !hn || !hun || !hvn || (eqstate_method != 1 && !rho) ||
       (bm && (!zz_bm || !idpdx_bm || !idpdy_bm)) ||
       (sm && (!zz_sm || !idpdx_sm || !idpdy_sm || !sseo)))
-    return set_err(GT_EARG, "gt_eqstate_internal_pressure: NULL argument");
+    return set_err(GT_EARG, "%s: NULL argument", who);
   LCHK(gt::launch_eos_ip_fused(c->st, c->d, eqstate_method, nonnegsalt != 0, bm, sm, c->az,
                                c->au, c->av, T, S, rho, T0, S0, dtr0, dsr0, H, hn, hun, hvn,
                                sseo, zz_bm, idpdx_bm, idpdy_bm, zz_sm, idpdx_sm, idpdy_sm, c->dx,
                                c->dy));
   return GT_OK;
+}
+
+extern "C" int gt_eqstate_internal_pressure(
+    gt_ctx* c, int eqstate_method, int nonnegsalt, int methods, const double* T, const double* S,
+    const double* rho, double T0, double S0, double dtr0, double dsr0, const double* H,
+    const double* hn, const double* hun, const double* hvn, const double* sseo, double* zz_bm,
+    double* idpdx_bm, double* idpdy_bm, double* zz_sm, double* idpdx_sm, double* idpdy_sm) {
+  return eos_ip(c, "gt_eqstate_internal_pressure", eqstate_method, nonnegsalt, methods, T, S,
+                rho, T0, S0, dtr0, dsr0, H, hn, hun, hvn, sseo, zz_bm, idpdx_bm, idpdy_bm, zz_sm,
+                idpdx_sm, idpdy_sm);
 }
 
 extern "C" int gt_eos_internal_pressure(gt_ctx* c, int methods, const double* T, const double* S,
@@ -1157,9 +1168,8 @@ extern "C" int gt_eos_internal_pressure(gt_ctx* c, int methods, const double* T,
                                         const double* hvn, const double* sseo, double* zz_bm,
                                         double* idpdx_bm, double* idpdy_bm, double* zz_sm,
                                         double* idpdx_sm, double* idpdy_sm) {
-  return gt_eqstate_internal_pressure(c, 1, 0, methods, T, S, nullptr, T0, S0, dtr0, dsr0, H, hn,
-                                      hun, hvn, sseo, zz_bm, idpdx_bm, idpdy_bm, zz_sm, idpdx_sm,
-                                      idpdy_sm);
+  return eos_ip(c, "gt_eos_internal_pressure", 1, 0, methods, T, S, nullptr, T0, S0, dtr0, dsr0,
+                H, hn, hun, hvn, sseo, zz_bm, idpdx_bm, idpdy_bm, zz_sm, idpdx_sm, idpdy_sm);
 }
 
 extern "C" int gt_do_internal_pressure(gt_ctx* c, int ip_method, const double* H,
