@@ -154,11 +154,18 @@ def test_fused_small_and_ragged(device, size, kind):
                          hun=inputs.uniform(sh, 64, 0.5, 2.0),
                          hvn=inputs.uniform(sh, 65, 0.5, 2.0))
     sseo = inputs.uniform((dom.NY, dom.NX), 66, -0.2, 0.2)
+    rho0 = inputs.uniform(sh, 67, 1020.0, 1030.0)
     for methods in (1, 2, 3):
         ref, per, fus = run_case(device, tr, dom, methods, 1.5, sseo)
         for nm in NAMES:
             assert np.array_equal(fus[nm], ref[nm], equal_nan=True), (methods, nm)
             assert np.array_equal(per[nm], ref[nm], equal_nan=True), (methods, nm)
+        for eqm in (2, 3):
+            ref, per, fus = run_case_eqstate(device, tr, dom, eqm, True, methods, 1.5, sseo,
+                                             rho0)
+            for nm in NAMES:
+                assert np.array_equal(fus[nm], ref[nm], equal_nan=True), (eqm, methods, nm)
+                assert np.array_equal(per[nm], ref[nm], equal_nan=True), (eqm, methods, nm)
 
 
 def run_case_eqstate(device, tr, dom, eqm, nonneg, methods, sentinel, sseo, rho0):
@@ -243,6 +250,16 @@ def test_fused_eqstate_reads_land_rho(device):
     assert not np.array_equal(a[2]["sy"], b[2]["sy"])
     for nm in ("zb", "bx", "by", "zs"):
         assert np.array_equal(a[2][nm], b[2][nm]), nm
+
+
+@pytest.mark.parametrize("eqm", [2, 3])
+def test_fused_eqstate_sigma_kmax2(device, eqm):
+    """kmax = 2 (the least the methods take) on sigma layers with islands"""
+    tr = sigma_case(65, 64, 2, seed=9)
+    dom = with_islands(tr.dom, 10)
+    rho0 = inputs.uniform(dom.shape3, 11, 1020.0, 1030.0)
+    sseo = inputs.uniform((dom.NY, dom.NX), 12, -0.2, 0.2)
+    check(*run_case_eqstate(device, tr, dom, eqm, False, 3, 0.5, sseo, rho0), 3)
 
 
 @pytest.mark.timeout(300)
