@@ -45,6 +45,7 @@
    integer, public, parameter :: NOADV=0, UPSTREAM=1, UPSTREAM_2DH=2, P2=3
    integer, public, parameter :: SUPERBEE=4, MUSCL=5, P2_PDM=6
    integer, public, parameter :: H_TAG=10, D_TAG=20, U_TAG=31, V_TAG=32, W_TAG=33
+   integer, public, parameter :: HU_TAG=11, HV_TAG=12, DU_TAG=21, DV_TAG=22, z_TAG=30
 
    type(c_ptr), save :: ctx = c_null_ptr
    integer, save :: il, ih, jl, jh, km

@@ -23,6 +23,7 @@ from . import build as _build
 NOADV, UPSTREAM, UPSTREAM_2DH, P2, SUPERBEE, MUSCL, P2_PDM, J7, FCT, P2_2DH = range(10)
 NOSPLIT, FULLSPLIT, HALFSPLIT, HVSPLIT = range(4)
 H_TAG, D_TAG, U_TAG, V_TAG, W_TAG = 10, 20, 31, 32, 33
+HU_TAG, HV_TAG, DU_TAG, DV_TAG, Z_TAG = 11, 12, 21, 22, 30  # halo_zones.F90:25-27
 BLUMBERG_MELLOR, SHCHEPETKIN_MCWILLIAMS = 1, 6
 
 GT_ERRORS = {1: "EINVAL_TAG", 2: "EINVAL_SPLIT", 3: "EINVAL_SCHEME",

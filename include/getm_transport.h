@@ -41,6 +41,10 @@ enum { GT_NOADV = 0, GT_UPSTREAM = 1, GT_UPSTREAM_2DH = 2, GT_P2 = 3,
 enum { GT_NOSPLIT = 0, GT_FULLSPLIT = 1, GT_HALFSPLIT = 2, GT_HVSPLIT = 3 };
 enum { GT_H_TAG = 10, GT_D_TAG = 20, GT_U_TAG = 31, GT_V_TAG = 32,
        GT_W_TAG = 33 };
+/* the remaining halo-message tags of halo_zones.F90:25-27 (the ghost-fill
+ * entry points accept any tag: one domain exchanges no messages) */
+enum { GT_HU_TAG = 11, GT_HV_TAG = 12, GT_DU_TAG = 21, GT_DV_TAG = 22,
+       GT_Z_TAG = 30 };
 /* internal_pressure.F90:94-100 */
 enum { GT_IP_BLUMBERG_MELLOR = 1, GT_IP_SHCHEPETKIN_MCWILLIAMS = 6 };
 
