@@ -34,7 +34,7 @@
    public do_advection_3d, do_advection_3d_dev
    public do_internal_pressure_gpu, do_eqstate_linear_gpu, update_3d_halo
    public eos_internal_pressure_gpu, eqstate_internal_pressure_gpu
-   public gt_malloc_f, gt_free_f, gt_to_device, gt_to_host
+   public gt_malloc_f, gt_free_f, gt_to_device, gt_to_host, slice_to_device, slice_to_host
    public do_eqstate_gpu, start_macro_gpu, set_coordinates_gpu, coordinates_gpu
    public ww_momentum_3d_gpu, salinity_diffusion_gpu, temperature_diffusion_gpu
    public uv_advect_3d_gpu
@@ -530,6 +530,8 @@
    ! slices and masks are copied to the device, the routine runs there, and
    ! the in/out arrays come back.
 
+   ! (public: E2DFIELD host arrays to and from device buffers, e.g. H and
+   ! sseo for the device-resident pressure routines)
    subroutine slice_to_device(dst, src)
    type(c_ptr), intent(in) :: dst
    real(c_double), intent(in), target :: src(*)

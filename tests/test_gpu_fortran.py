@@ -51,8 +51,10 @@ def test_fortran_driver(tmp_path, device, split, hs, vs):
 
 
 def test_fortran_rows_driver(tmp_path, device):
-    """do_eqstate_gpu (Jackett, with alpha/beta) and salinity_diffusion_gpu
-    called from Fortran on device-resident fields: bit for bit the oracle's."""
+    """do_eqstate_gpu (Jackett, with alpha/beta), eqstate_internal_pressure_gpu
+    (Jackett, both methods, on the rho do_eqstate left), eos_internal_pressure_gpu
+    (linear, both methods) and salinity_diffusion_gpu called from Fortran on
+    device-resident fields: bit for bit the oracle's."""
     exe = build.BUILD / "fortran" / "rows_driver"
     if not exe.exists():
         build.build_fortran()
@@ -70,17 +72,33 @@ def test_fortran_rows_driver(tmp_path, device):
             fh.write(np.ascontiguousarray(m, np.int32).tobytes())
         for a in (T, S, tr.hn, nuh):
             fh.write(np.ascontiguousarray(a).tobytes())
+        sseo = inputs.uniform((d.NY, d.NX), 6, -0.2, 0.2)
+        for a in (tr.hun, tr.hvn, d.H, sseo):
+            fh.write(np.ascontiguousarray(a).tobytes())
     r = subprocess.run([str(exe), str(tmp_path / "in.bin"), str(tmp_path / "out.bin")],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    out = np.fromfile(tmp_path / "out.bin", np.float64).reshape(5, *d.shape3)
+    out = np.fromfile(tmp_path / "out.bin", np.float64).reshape(17, *d.shape3)
     orc = po.Oracle.from_domain(d)
     rho, buoy, alpha, beta = (np.zeros(d.shape3) for _ in range(4))
     orc.do_eqstate(3, T, S, tr.hn, rho, buoy, alpha, beta)
+    # the pressure methods on Jackett's and on the linear EOS's buoyancy
+    ip = []
+    lb = np.zeros(d.shape3)
+    orc.eos_linear(T, S, np.zeros(d.shape3), lb, inputs.EOS_T0, inputs.EOS_S0,
+                   inputs.EOS_DTR0, inputs.EOS_DSR0)
+    for b in (buoy, lb):
+        o = [np.zeros(d.shape3) for _ in range(6)]
+        orc.ip_blumberg_mellor(d.H, tr.hn, tr.hun, tr.hvn, b, *o[:3])
+        orc.ip_shchepetkin_mcwilliams(d.H, tr.hn, tr.hun, tr.hvn, b, sseo, *o[3:])
+        ip += o
     orc.salinity_diffusion(dt, cnpar, tr.hn, nuh, S)
-    for q, (name, want) in enumerate((("rho", rho), ("buoy", buoy), ("alpha", alpha),
-                                      ("beta", beta), ("S", S))):
+    names = ["rho", "buoy", "alpha", "beta", "S"] + [
+        f"{e} {n}" for e in ("jackett", "linear")
+        for n in ("zz_bm", "idpdx_bm", "idpdy_bm", "zz_sm", "idpdx_sm", "idpdy_sm")]
+    for q, (name, want) in enumerate(zip(names, [rho, buoy, alpha, beta, S] + ip)):
         assert np.array_equal(out[q], want), (name, rel_err(out[q], want))
+    assert np.abs(ip[7]).max() > 0 and np.abs(ip[1]).max() > 0
 
 
 @pytest.mark.parametrize("scheme", [1, 4, 6])
