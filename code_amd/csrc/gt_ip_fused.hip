@@ -145,16 +145,19 @@ __device__ __forceinline__ double sm_face(double b0, double b1, double z0, doubl
   return AJ * dm1 * h - h * dm1 * (s1 - s0) * 0.5 * (k1 + k0);
 }
 
-// levels by which k_ip_rows loads ahead of its arithmetic.  0: each level's
-// operands are loaded where the level starts; without the ring's registers
-// the kernel fits 4 wavefronts per SIMD, so all workgroups of the headline
-// grid are resident at once, and the other wavefronts hide the latency
-// (0.18 ms for both methods at 400x400x40, against 0.23 ms loading one
-// level ahead at 3 wavefronts per SIMD)
-constexpr int kIpAhead = 0;
+// levels by which k_ip_rows loads ahead of its arithmetic.  The kernel must
+// stay at 4 wavefronts per SIMD (<= 128 VGPRs), so that all workgroups of the
+// headline grid are resident at once: the linear EOS fits one level of
+// lookahead (128 VGPRs with hun and hvn loaded where a level starts and the
+// neighbours' face constants read from LDS; 175 against 181 us for both
+// methods at 400x400x40), the full equations of state do not (132 VGPRs, 3
+// wavefronts per SIMD, two rounds of workgroups) and load each level where
+// it starts, the other wavefronts hiding the latency
+template <int EOS>
+constexpr int kIpAhead = EOS == 1 ? 1 : 0;
 
 // the operands of one level k of the march that come from memory, loaded
-// kIpAhead levels ahead of their use (raw T and S, the buoyancy formed where
+// kIpAhead<EOS> levels ahead of their use (raw T and S, the buoyancy formed where
 // it is used, so with a lookahead the loads stay in flight across the
 // arithmetic of the levels before).  Row y is the wavefront's own row; the
 // rows around it come from the other wavefronts of the workgroup through LDS.
@@ -163,7 +166,11 @@ struct IpLevel {
   double te, se, ze;  // T S zz(centres) at (y-1, k) (first wavefront) or (y+1, k) (carrier);
                       // BM only: T S zz(interfaces) at (y+1, k) (last wavefront)
   double zb0;         // zz(interfaces) at (y, k) (BM)
-  double hu, hv;      // hun hvn at (y, k)
+};
+// hun and hvn at (y, k): loaded where level k starts, before the next
+// level's operands, so they hold no registers across a level
+struct IpH {
+  double hu, hv;
 };
 
 // One workgroup of W+1 wavefronts (W without Shchepetkin-McWilliams) per
@@ -259,28 +266,36 @@ __global__ __launch_bounds__(64 * (W + 1)) void k_ip_rows(
       L.ze = (zzb + ok)[cp];
     }
     if constexpr (BM) L.zb0 = (zzb + ok)[c0];
+  };
+  auto load_h = [&](int k, IpH& H) {
+    const long ok = NXY * (long)k;
     if (!carrier) {
-      L.hu = (hun + ok)[c0];
-      L.hv = (hvn + ok)[c0];
+      H.hu = (hun + ok)[c0];
+      H.hv = (hvn + ok)[c0];
     }
   };
   // LDS: per wavefront and lane, b, z and zz(interfaces) of its row (slot A),
   // P and the two means across y (slot B); two of each by level parity
   __shared__ double shA[2][3][NW][64];
   __shared__ double shB[SM ? 2 : 1][3][NW][64];
+  // Shchepetkin-McWilliams' face constants of each row: buoyancy at kmax and
+  // sseo, read by the lane below (x face) and the wavefront below (y face)
+  __shared__ double shK[2][NW][64];
+  const int lu = lane < 63 ? lane + 1 : 63;
 
   // ---- level kmax
   const long okx = NXY * (long)kmax;
   const double kb0 = buoy(ldt(okx, c0, w0), lds(okx, c0, w0), w0);
   // Shchepetkin-McWilliams: P of the row, the face constants
   PCol ps{};
-  double ss0 = 0, ssp = 0, sU = 0, kU = 0, kbp = 0;
+  double ss0 = 0;
   // Blumberg-Mellor (ip_blumberg_mellor.F90:99-149), as k_bm_idpd
   double grdlx = 0, buoylx = 0, hkx = 0, prgrx = 0, grdly = 0, buoyly = 0, hky = 0, prgry = 0;
 
   // the exchange and the faces of level k, from the row's b(k), z(k) (SM:
   // the P state, P(k) in ps.p), zz(interfaces)(k) and hun, hvn (L)
-  auto level_faces = [&](int k, int slot, double bk, double zk, const IpLevel& L, bool top) {
+  auto level_faces = [&](int k, int slot, double bk, double zk, const IpLevel& L, const IpH& H,
+                         bool top) {
     const long ok = NXY * (long)k;
     shA[slot][0][wave][lane] = bk;
     if constexpr (SM) shA[slot][1][wave][lane] = zk;
@@ -325,8 +340,8 @@ __global__ __launch_bounds__(64 * (W + 1)) void k_ip_rows(
         const double dZ = sm_hmx(zk - zL, zU - zk), dR = sm_hmx(bk - bL, bU - bk);
         const double dZ1 = from_upper_lane(dZ), dR1 = from_upper_lane(dR);
         if (ux)
-          (ixs + ok)[c0] = sm_face(bk, bU, zk, zU, dR, dR1, dZ, dZ1, p0, pU, L.hu, dxm1, ss0, sU,
-                                   kb0, kU);
+          (ixs + ok)[c0] = sm_face(bk, bU, zk, zU, dR, dR1, dZ, dZ1, p0, pU, H.hu, dxm1, ss0,
+                                   shK[1][wave][lu], kb0, shK[0][wave][lu]);
       }
       __syncthreads();
       if (!carrier) {
@@ -334,8 +349,8 @@ __global__ __launch_bounds__(64 * (W + 1)) void k_ip_rows(
         const double hZ1 = shB[slot][1][wave + 1][lane];
         const double hR1 = shB[slot][2][wave + 1][lane];
         if (vy)
-          (iys + ok)[c0] = sm_face(bk, bn, zk, zn, hR, hR1, hZ, hZ1, ps.p, pP, L.hv, dym1, ss0,
-                                   ssp, kb0, kbp);
+          (iys + ok)[c0] = sm_face(bk, bn, zk, zn, hR, hR1, hZ, hZ1, ps.p, pP, H.hv, dym1, ss0,
+                                   shK[1][wave + 1][lane], kb0, shK[0][wave + 1][lane]);
       }
     }
     if constexpr (BM) {
@@ -346,11 +361,11 @@ __global__ __launch_bounds__(64 * (W + 1)) void k_ip_rows(
           // level kmax starts the recurrences
           grdlx = (be - b0) * dxm1;
           buoylx = 0.5 * (be + b0);
-          hkx = L.hu;
+          hkx = H.hu;
           prgrx = grdlx * 0.5 * hkx;
           grdly = (bn - b0) * dym1;
           buoyly = 0.5 * (bn + b0);
-          hky = L.hv;
+          hky = H.hv;
           prgry = grdly * 0.5 * hky;
           if (ux) (ixb + ok)[c0] = hkx * prgrx;
           if (vy) (iyb + ok)[c0] = hky * prgry;
@@ -361,7 +376,7 @@ __global__ __launch_bounds__(64 * (W + 1)) void k_ip_rows(
             const double buoyu = buoylx;
             buoylx = 0.5 * (be + b0);
             const double dxz = (ze - z0) * dxm1;
-            const double hkk = L.hu;
+            const double hkk = H.hu;
             prgrx = prgrx + 0.5 * (grdu + grdlx) * 0.5 * (hkk + hkx) - dxz * (buoyu - buoylx);
             if (ux) (ixb + ok)[c0] = hkk * prgrx;
             hkx = hkk;
@@ -372,7 +387,7 @@ __global__ __launch_bounds__(64 * (W + 1)) void k_ip_rows(
             const double buoyu = buoyly;
             buoyly = 0.5 * (bn + b0);
             const double dyz = (zbn - z0) * dym1;
-            const double hkk = L.hv;
+            const double hkk = H.hv;
             prgry = prgry + 0.5 * (grdu + grdly) * 0.5 * (hkk + hky) - dyz * (buoyu - buoyly);
             if (vy) (iyb + ok)[c0] = hkk * prgry;
             hky = hkk;
@@ -384,32 +399,32 @@ __global__ __launch_bounds__(64 * (W + 1)) void k_ip_rows(
 
   {
     IpLevel top;
+    IpH hh;
     load(kmax, top);
+    load_h(kmax, hh);
     if constexpr (SM) {
       ps.top(kb0, buoy(top.t0, top.s0, w0), (zzs + okx)[c0], top.z0, (hn + okx)[c0]);
-      kbp = buoy(ldt(okx, cp, wp), lds(okx, cp, wp), wp);
       ss0 = sseo[c0];
-      ssp = sseo[cp];
-      sU = from_upper_lane(ss0);
-      kU = from_upper_lane(kb0);
-      level_faces(kmax, kmax & 1, ps.b1, ps.z1, top, true);
+      shK[0][wave][lane] = kb0;
+      shK[1][wave][lane] = ss0;
+      level_faces(kmax, kmax & 1, ps.b1, ps.z1, top, hh, true);
     } else {
-      level_faces(kmax, kmax & 1, kb0, 0.0, top, true);
+      level_faces(kmax, kmax & 1, kb0, 0.0, top, hh, true);
     }
   }
 
   // ---- levels kmax-1 .. 1
-  auto level = [&](int k, const IpLevel& L) {
+  auto level = [&](int k, const IpLevel& L, const IpH& H) {
     if constexpr (SM) {
       ps.step(k, buoy(L.t0, L.s0, w0), L.z0);
-      level_faces(k, k & 1, ps.b1, ps.z1, L, false);
+      level_faces(k, k & 1, ps.b1, ps.z1, L, H, false);
     } else {
-      level_faces(k, k & 1, buoy(L.t0, L.s0, w0), 0.0, L, false);
+      level_faces(k, k & 1, buoy(L.t0, L.s0, w0), 0.0, L, H, false);
     }
   };
   // a ring of NB operand buffers: level k's operands are loaded NB-1 levels
   // ahead (levels below 1 re-read level 1, harmlessly)
-  constexpr int NB = kIpAhead + 1;
+  constexpr int NB = kIpAhead<EOS> + 1;
   auto lv = [](int k) { return k >= 1 ? k : 1; };
   IpLevel ring[NB];
 #pragma unroll
@@ -419,8 +434,10 @@ __global__ __launch_bounds__(64 * (W + 1)) void k_ip_rows(
     for (int q = 0; q < NB; q++) {
       const int k = k0 - q;
       if (k < 1) break;
+      IpH hh;
+      load_h(k, hh);
       load(lv(k - (NB - 1)), ring[(q + NB - 1) % NB]);
-      level(k, ring[q]);
+      level(k, ring[q], hh);
     }
   }
 }
