@@ -30,8 +30,9 @@
 //     form of each limiter (phi(r)*deltaf as a min/max of products, the sign
 //     of deltaf choosing min or max) and reciprocals for the face constants;
 //     the layer heights stay FP64 and the cell update (Dio*f - dt*advn)/Di is
-//     f + [f*(Dio-Di)/Di - dF*(dt*c/Di)] with the two factors formed in FP64
-//     once per cell and shared by the tracers.
+//     f + [f*(Dio-Di)/Di - dF*(dt*c/Di)] with the two factors formed once per
+//     cell (Dio-Di in FP64, the rest in FP32, as the Courant numbers) and
+//     shared by the tracers.
 //
 // Which cells each sub-step updates follows the reference exactly (the isub /
 // jsub bounds of adv_split_u/v.F90 and the column range of adv_split_w.F90),
@@ -149,7 +150,9 @@ __device__ __forceinline__ void face_setup(FaceC<false>& fc, bool active, bool l
                                            double vel, double Dv, const FaceK& k) {
   const bool right = !(vel > 0.0);
   const bool lim = active && SCH != UPSTREAM && (right ? lim_r : lim_l);
-  const float c0 = (float)(__builtin_fabs(vel) * k.cscale * rcp_nr(Dv));
+  // the Courant number feeds FP32 arithmetic only: formed in FP32
+  const float c0 =
+      __builtin_fabsf((float)vel) * (float)k.cscale * __builtin_amdgcn_rcpf((float)Dv);
   const float c = lim ? c0 : 0.0f;
   fc.right = right;
   fc.vel = active ? (float)vel : 0.0f;
@@ -245,15 +248,16 @@ __device__ __forceinline__ double cell_update_exact(bool upd, double Dio, double
 }
 
 // fast: the same as fo + [fo*(Dio-Dn)/Dn - (Fc-Fm)*(dt*splitfac*dlen*ard1/Dn)]:
-// the two factors are formed once per cell in FP64 from the layer heights and
-// rounded to FP32 (CellK); the bracket is a small correction, so its FP32
-// rounding is far below the final rounding of fo + correction.
+// the two factors are formed once per cell in FP32 from the FP64 layer
+// heights (Dio-Dn taken in FP64, then rounded) and shared by the tracers
+// (CellK); the bracket is a small correction, so their few FP32 roundings
+// are far below the final rounding of fo + correction.
 struct CellK {
   float am1, b;
 };
 __device__ __forceinline__ CellK cell_factors(double Dio, double Dn, double dtc) {
-  const double r = rcp_nr(Dn);
-  return {(float)((Dio - Dn) * r), (float)(dtc * r)};
+  const float r = __builtin_amdgcn_rcpf((float)Dn);
+  return {(float)(Dio - Dn) * r, (float)dtc * r};
 }
 __device__ __forceinline__ float cell_update_fast(const CellK& k, float fo, float Fc, float Fm) {
   return fo + __builtin_fmaf(fo, k.am1, -(Fc - Fm) * k.b);
@@ -282,7 +286,8 @@ __device__ __forceinline__ void wface_setup(WFace<true>& wf, double w, double dt
 template <int SCH>
 __device__ __forceinline__ void wface_setup(WFace<false>& wf, double w, double dt, double Dlo,
                                             double Dhi) {
-  const float c = (float)(__builtin_fabs(w) * dt * (2.0 * rcp_nr(Dlo + Dhi)));
+  const float c =
+      __builtin_fabsf((float)w) * (float)(2.0 * dt) * __builtin_amdgcn_rcpf((float)(Dlo + Dhi));
   wf.w = (float)w;
   wf.g = 0.5f * (1.0f - c);
   wf.A = 2.0f * __builtin_amdgcn_rcpf(c + 1.e-10f);
