@@ -628,7 +628,9 @@ __global__ __launch_bounds__(256) void k_strip_uv(FusedArgs<TF> A) {
 }
 
 // ---- kernel 2: w, then v/2 and u/2 at level K -------------------------------------------
-template <typename TF, int SCH, int G, bool EXACT, bool ADV, bool DIFF>
+// BUD: some domain of the launch forms its budgets in the step (the
+// variant without them carries none of their code or registers)
+template <typename TF, int SCH, int G, bool EXACT, bool ADV, bool DIFF, bool BUD>
 __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
   StripItem it;
   if (!strip_item<1>(A, it)) return;
@@ -725,7 +727,7 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
   // budget (when a.bpart is set): the lane's sums over the rows it writes of
   // hires*f per tracer and of hires, interior water cells only, kept in LDS
   // (one slot per lane, output and wavefront) so they cost no registers
-  const bool bud = a.bpart != nullptr;
+  const bool bud = BUD && a.bpart != nullptr;
   __shared__ double bsh[4][G + 1][64];
   const int wv = threadIdx.x >> 6;
   if (bud) {
@@ -955,8 +957,14 @@ hipError_t launch_two(hipStream_t st, const FusedArgs<TF>& A) {
                      A);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_strip_wvu<TF, SCH, G, EXACT, ADV, DIFF>), dim3(grid(1)), dim3(256), 0,
-                     st, A);
+  bool bud = false;
+  for (int i = 0; i < A.ndom; i++) bud = bud || A.doms[i].bpart != nullptr;
+  if (bud)
+    hipLaunchKernelGGL((k_strip_wvu<TF, SCH, G, EXACT, ADV, DIFF, true>), dim3(grid(1)),
+                       dim3(256), 0, st, A);
+  else
+    hipLaunchKernelGGL((k_strip_wvu<TF, SCH, G, EXACT, ADV, DIFF, false>), dim3(grid(1)),
+                       dim3(256), 0, st, A);
   return hipGetLastError();
 }
 
