@@ -432,6 +432,11 @@ __device__ __forceinline__ void sto(T* base, uint32_t off, T v) {
 // (advection 0.46 -> 0.49 ms with 140 instead of 182 VGPRs in wvu)
 template <typename TF>
 constexpr bool kRowAhead = std::is_same<TF, double>::value;
+// the w kernel's shared fields of a row (masks, w, heights, the v and u
+// transports) with its tracers, two rows ahead (true), or where the row's
+// step starts (false: fewer registers held across the steps)
+template <typename TF>
+constexpr bool kWvuSharedAhead = false;
 
 // ---- kernel 1: u/2 then v/2 at level L ------------------------------------------------
 template <typename TF, int SCH, int G, bool EXACT, bool ADV, bool DIFF>
@@ -696,9 +701,12 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
     double wwK, wwKm1, dKm1, dK, dKp1, vv, vvm, hv, uu, hu, a2;
     TF f[G][5];
   };
-  auto load_row = [&](int y, Row& r) {
+  // the row's shared fields and its tracers, loaded apart: the tracers run
+  // ahead of the arithmetic (kRowAhead), the shared fields only as far as
+  // kWvuSharedAhead says
+  auto load_shared = [&](int y, Row& r) {
     const uint32_t c = cell(y), c2 = cell(y - 2), c3 = cell(y - 3);
-    const uint32_t b = c * (uint32_t)sizeof(TF), b8 = c * 8u, b82 = c2 * 8u;
+    const uint32_t b8 = c * 8u, b82 = c2 * 8u;
     r.fl = flags[c];
     r.wwK = ldo(wwKm1, b8 + lstep);
     r.wwKm1 = ldo(wwKm1, b8);
@@ -711,6 +719,9 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
     r.uu = ldo(uu, b82);
     r.hu = ldo(hu, b82);
     if constexpr (ADV) r.a2 = ldo(a2, b8);
+  };
+  auto load_f = [&](int y, Row& r) {
+    const uint32_t b = cell(y) * (uint32_t)sizeof(TF);
     uint32_t lb[5];
 #pragma unroll
     for (int m = 0; m < 5; m++) lb[m] = b + lvo[m];
@@ -718,6 +729,10 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
     for (int g = 0; g < G; g++)
 #pragma unroll
       for (int m = 0; m < 5; m++) r.f[g][m] = ldo(f2_g[g], lb[m]);
+  };
+  auto load_row = [&](int y, Row& r) {
+    load_shared(y, r);
+    load_f(y, r);
   };
 
   int flw[4] = {0, 0, 0, 0}, fow[4] = {0, 0, 0, 0};
@@ -752,7 +767,11 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
     constexpr int R = decltype(slot)::value;
     constexpr int w0 = R & 3, w1 = (R + 1) & 3, w2 = (R + 2) & 3, w3 = (R + 3) & 3;
     const int y = y0 - 2 + s;
-    if constexpr (!kRowAhead<TF>) load_row(y, buf);
+    if constexpr (!kRowAhead<TF>) {
+      load_row(y, buf);
+    } else if constexpr (!kWvuSharedAhead<TF>) {
+      load_shared(y, buf);
+    }
     const int fl = (y >= 0 && y < NY) ? (buf.fl & cb.xin) : 0;  // raw masks of the cell
     const int fo = (fl | MB_XOUT) & cb.xm;                      // ... with the column bounds
 
@@ -872,13 +891,24 @@ __global__ __launch_bounds__(256) void k_strip_wvu(FusedArgs<TF> A) {
     }
     if (st_ok && it.grp == 0 && hires) sto(hires, cv * 8u, D5);
     if (bcell && it.grp == 0) bsh[wv][G][lane] += D5;
-    if constexpr (kRowAhead<TF>) load_row(y + 2, buf);
+    if constexpr (kRowAhead<TF>) {
+      if constexpr (kWvuSharedAhead<TF>) {
+        load_row(y + 2, buf);
+      } else {
+        load_f(y + 2, buf);
+      }
+    }
   };
 
   if constexpr (kRowAhead<TF>) {
     Row ra, rb;
-    load_row(y0 - 2, ra);
-    load_row(y0 - 1, rb);
+    if constexpr (kWvuSharedAhead<TF>) {
+      load_row(y0 - 2, ra);
+      load_row(y0 - 1, rb);
+    } else {
+      load_f(y0 - 2, ra);
+      load_f(y0 - 1, rb);
+    }
     for (int s = 0; s < nsteps; s += 4) {
       step(Slot<0>{}, ra, s);
       step(Slot<1>{}, rb, s + 1);
