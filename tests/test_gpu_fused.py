@@ -286,6 +286,54 @@ def test_batch_budgets_formed_in_the_step(device, rows, group, npassive, fused):
         gt.adv3d_batch(doms, nt, subs[0].dt, gt.P2_PDM, gt.P2_PDM, tag=gt.U_TAG)
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_batch_budgets_some_domains(device, dtype):
+    """a batch whose first and last domains ask for their budgets and whose
+    middle one does not: the w kernel's budget variant runs, the middle
+    domain's budget slot is left alone, the budgets asked for equal the
+    error-free pass, and every domain's result equals, bit for bit, that of
+    the same batch without budgets (the budget-free variant)"""
+    sizes = [(24, 20), (71, 33), (30, 52)]
+    subs = inputs.multitracer_case(seed=12, kmax=6, sizes=sizes, npassive=1)
+    names = list(subs[0].tracers)
+    nt = len(names)
+
+    def run(with_budget):
+        doms, outs, buds, ctxs = [], [], [], []
+        for s, tr in enumerate(subs):
+            ctx = gt.Context.from_domain(tr.dom)
+            ctxs.append(ctx)
+            tin = to_dev(np.stack([tr.tracers[n] for n in names]), device).to(dtype)
+            host = [tr.uu, tr.vv, tr.ww, tr.hun, tr.hvn, tr.ho]
+            dd = dict(zip(NAMES[:6], [to_dev(a, device) for a in host]))
+            out = torch.empty_like(tin)
+            bud = torch.full((nt + 1,), 7.0, dtype=torch.float64, device=device)
+            ask = with_budget and s != 1
+            doms.append(dict(ctx=ctx, f_in=tin, f_out=out, hires=torch.empty_like(dd["ho"]),
+                             budget=bud if ask else None, **dd))
+            outs.append(out)
+            buds.append(bud)
+        gt.adv3d_batch(doms, nt, subs[0].dt, gt.P2_PDM, gt.P2_PDM)
+        torch.cuda.synchronize()
+        return doms, outs, buds, ctxs
+
+    d0, o0, _, _ = run(False)
+    d1, o1, b1, c1 = run(True)
+    for s in range(len(subs)):
+        assert torch.equal(o0[s], o1[s]), s
+        assert torch.equal(d0[s]["hires"], d1[s]["hires"]), s
+    assert torch.equal(b1[1], torch.full_like(b1[1], 7.0))
+    for s in (0, 2):
+        if dtype == torch.float32:
+            exact = c1[s].budget_batch(d1[s]["hires"], o1[s]).cpu().numpy()
+        else:
+            exact = np.array([c1[s].budget(d1[s]["hires"], o1[s][q])[0] for q in range(nt)] +
+                             [c1[s].budget(d1[s]["hires"], o1[s][0])[1]])
+        got = b1[s].cpu().numpy()
+        err = np.abs(got - exact) / np.abs(exact)
+        assert err.max() <= 1e-13, (s, err.max())
+
+
 def test_batch_budgets_f64(device):
     """the FP64 batched step (T and S) with its budgets formed in the step,
     against gt_budget of each result (error-free): within 1e-14 relative"""
