@@ -169,8 +169,12 @@ __device__ __forceinline__ void face_setup(FaceC<false>& fc, bool active, bool l
 // the min (deltaf > 0) or max (deltaf < 0) of the branch terms times deltaf.
 // P2-PDM: A = 2/(c+1e-10), al = 1/2 + (1-2c)/6, E = 2/(1-c).
 // (adv_interfacial_reconstruction, src/2d/advection.F90:785-879)
-// Both branch terms are formed and one is selected (no divergent branch);
-// fused multiply-adds are written out where they shorten the chain.
+// On the lanes that take the limited value deltaf and deltafu share a sign,
+// and the branch terms of deltaf < 0 are those of |deltaf|, |deltafu|
+// negated, each rounding alike, so the max there is minus the min over the
+// magnitudes: one min (abs taken as operand modifiers) and the sign folded
+// into g give bit for bit the min/max selected by the sign.  No divergent
+// branch; fused multiply-adds are written out where they shorten the chain.
 template <int SCH>
 __device__ __forceinline__ float rec_fast(float fuu, float fu, float fd, float g, float A,
                                           float al, float E) {
@@ -181,22 +185,20 @@ __device__ __forceinline__ float rec_fast(float fuu, float fu, float fd, float g
   } else if constexpr (SCH == UPSTREAM) {
     return fu;
   } else {
-    float lo, hi;
+    const float adf = __builtin_fabsf(df), adu = __builtin_fabsf(du);
+    float m;
     if constexpr (SCH == P2_PDM) {
-      const float t1 = A * du, t2 = __builtin_fmaf(al, df - du, du), t3 = E * df;
-      lo = __builtin_fminf(__builtin_fminf(t1, t2), t3);
-      hi = __builtin_fmaxf(__builtin_fmaxf(t1, t2), t3);
+      m = __builtin_fminf(__builtin_fminf(A * adu, __builtin_fmaf(al, adf - adu, adu)),
+                          E * adf);
     } else if constexpr (SCH == SUPERBEE) {
-      const float a2 = 2.0f * du, b2 = 2.0f * df;
-      lo = __builtin_fmaxf(__builtin_fminf(a2, df), __builtin_fminf(du, b2));
-      hi = __builtin_fminf(__builtin_fmaxf(a2, df), __builtin_fmaxf(du, b2));
+      const float a2 = 2.0f * adu, b2 = 2.0f * adf;
+      m = __builtin_fmaxf(__builtin_fminf(a2, adf), __builtin_fminf(adu, b2));
     } else {  // MUSCL
-      const float a2 = 2.0f * du, b2 = 2.0f * df, h = 0.5f * (df + du);
-      lo = __builtin_fminf(__builtin_fminf(a2, h), b2);
-      hi = __builtin_fmaxf(__builtin_fmaxf(a2, h), b2);
+      const float a2 = 2.0f * adu, b2 = 2.0f * adf, h = 0.5f * (adf + adu);
+      m = __builtin_fminf(__builtin_fminf(a2, h), b2);
     }
-    const float t = df > 0.0f ? lo : hi;
-    const float r = __builtin_fmaf(g, t, fu);
+    const float gs = df > 0.0f ? g : -g;
+    const float r = __builtin_fmaf(gs, m, fu);
     return df * du > 0.0f ? r : fu;
   }
 }
