@@ -117,6 +117,28 @@ __global__ void k_eos_points(int fn, long n, const double* __restrict__ s,
   }
 }
 
+// the same without the expansion coefficients: no level depends on another,
+// so one thread per cell of levels 1..kmax (a column per thread would leave
+// the chip with a few wavefronts per SIMD, each a serial chain of divisions)
+template <int METHOD, bool NONNEG>
+__global__ void __launch_bounds__(kEosBlock)
+    k_eqstate_cells(long NXY, long ncell, const int32_t* __restrict__ az,
+                    const double* __restrict__ T, const double* __restrict__ S,
+                    double* __restrict__ rho, double* __restrict__ buoy) {
+  const long i = blockIdx.x * (long)kEosBlock + threadIdx.x;
+  if (i >= ncell) return;
+  const long q = NXY + i;  // level 0 is not touched
+  const long c = i % NXY;
+  double r;
+  if (az[c] > 0) {
+    r = eqstate_rho<METHOD>(T[q], S[q], NONNEG);
+    rho[q] = r;
+  } else {
+    r = rho[q];
+  }
+  buoy[q] = buoy_of_rho(r);
+}
+
 template <int METHOD, bool NONNEG>
 hipError_t launch_column(hipStream_t st, Dims d, const int32_t* az, const double* T,
                          const double* S, const double* hn, double* rho, double* buoy,
@@ -125,9 +147,12 @@ hipError_t launch_column(hipStream_t st, Dims d, const int32_t* az, const double
   if (alpha && beta)
     hipLaunchKernelGGL((k_eqstate_column<METHOD, NONNEG, true>), grid, dim3(kEosBlock), 0, st,
                        d.NXY, d.kmax, az, T, S, hn, rho, buoy, alpha, beta);
-  else
-    hipLaunchKernelGGL((k_eqstate_column<METHOD, NONNEG, false>), grid, dim3(kEosBlock), 0,
-                       st, d.NXY, d.kmax, az, T, S, hn, rho, buoy, alpha, beta);
+  else {
+    const long ncell = d.NXY * (long)d.kmax;
+    hipLaunchKernelGGL((k_eqstate_cells<METHOD, NONNEG>),
+                       dim3((unsigned)((ncell + kEosBlock - 1) / kEosBlock)), dim3(kEosBlock), 0,
+                       st, d.NXY, ncell, az, T, S, rho, buoy);
+  }
   return hipGetLastError();
 }
 
